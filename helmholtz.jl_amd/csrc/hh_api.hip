@@ -1,0 +1,777 @@
+// C ABI of libhelmholtz_amd: operator handle, shifted-Laplacian multigrid
+// hierarchy, multigrid cycle and the batched flexible GMRES driver.
+//
+// Control flow follows solveLinearSystem (src/ShiftedLaplacianMultigridSolver.jl:33-102):
+// zero-RHS shortcut, lazy hierarchy setup, outer operator = the true
+// (unshifted) H, FGMRES preconditioned by one MG cycle, warning on max-iter.
+// The hierarchy is the rediscretised one of the reference's commented
+// geometric path (:51-62): shift folded into gamma, m and gamma restricted by
+// full weighting, the operator rebuilt matrix-free on every coarse mesh.
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/helmholtz_amd.h"
+#include "hh_kernels.h"
+
+using namespace hh;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct HHError : std::runtime_error {
+  int code;
+  HHError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+inline void hipck(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw HHError(HH_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+#define HIPCK(x) hipck((x), #x)
+
+template <typename T>
+T* dalloc(size_t n) {
+  void* p = nullptr;
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc(&p, n * sizeof(T));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    throw HHError(HH_ERR_NOMEM, "hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed");
+  }
+  return (T*)p;
+}
+inline void dfree(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+template <typename F>
+int guarded(F&& f) {
+  try {
+    f();
+    return HH_OK;
+  } catch (const HHError& e) {
+    g_err = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return HH_ERR_HIP;
+  }
+}
+
+inline double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+Grid make_grid(int dim, const int64_t nn[3], const double h[3], int bc) {
+  Grid g;
+  g.dim = dim;
+  g.n1 = (int)nn[0];
+  g.n2 = (int)nn[1];
+  g.n3 = dim == 3 ? (int)nn[2] : 1;
+  g.N = (int64_t)g.n1 * g.n2 * g.n3;
+  g.ih1 = 1.0 / (h[0] * h[0]);
+  g.ih2 = 1.0 / (h[1] * h[1]);
+  g.ih3 = dim == 3 ? 1.0 / (h[2] * h[2]) : 0.0;
+  g.bc = bc;
+  return g;
+}
+
+// Absorbing-layer profile from per-axis 1-D ramps (src/GetHelmholtz.jl:97-220):
+// 2-D  gamma = amp (p1 + p2 - p1 p2);  3-D  gamma = min(amp, amp (g1 + g2 + g3)).
+__global__ void k_abl(int dim, int n1, int n2, int n3, const double* p1, const double* p2, const double* p3,
+                      double amp, double* out) {
+  const int64_t N = (int64_t)n1 * n2 * n3;
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= N) return;
+  const int i = (int)(p % n1);
+  const int64_t q = p / n1;
+  const int j = (int)(q % n2);
+  const int k = (int)(q / n2);
+  if (dim == 2) {
+    const double a = p1[i], b = p2[j];
+    out[p] = (a + b - a * b) * amp;
+  } else {
+    const double g = amp * (p1[i] + p2[j] + p3[k]);
+    out[p] = g >= amp ? amp : g;
+  }
+}
+
+__global__ void k_sqrt_arr(const double* a, double* b, int n) {
+  int i = blockIdx.x * 64 + threadIdx.x;
+  if (i < n) b[i] = sqrt(a[i]);
+}
+
+}  // namespace
+
+// ============================================================== operator
+struct hh_op {
+  int dim = 2;
+  int64_t nn[3] = {1, 1, 1};
+  double h[3] = {1, 1, 1};
+  double wre = 0, wim = 0;
+  int neumann_top = 0, sommerfeld = 0, bc = 2;
+  Grid g;
+  double* m = nullptr;
+  double* gam = nullptr;
+  cx<double>* d64 = nullptr;  // diagonal of H, complex128
+  cx<float>* d32 = nullptr;   // diagonal of H, complex64
+  hipStream_t stream = nullptr;
+  void* stage_in = nullptr;   // host-pointer staging
+  void* stage_out = nullptr;
+  size_t stage_bytes = 0;
+
+  void ensure_stage(size_t bytes) {
+    if (bytes <= stage_bytes) return;
+    dfree(stage_in);
+    dfree(stage_out);
+    stage_in = stage_out = nullptr;
+    stage_bytes = 0;
+    stage_in = dalloc<char>(bytes);
+    stage_out = dalloc<char>(bytes);
+    stage_bytes = bytes;
+  }
+  ~hh_op() {
+    dfree(m);
+    dfree(gam);
+    dfree(d64);
+    dfree(d32);
+    dfree(stage_in);
+    dfree(stage_out);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+// ============================================================== solver
+namespace {
+
+struct Level {
+  Grid g;
+  double h[3];
+  double* m = nullptr;
+  double* gam = nullptr;
+  void* d = nullptr;      // diagonal of the shifted operator, preconditioner precision
+  void* xa = nullptr;     // ping-pong iterate buffers (cap * N)
+  void* xb = nullptr;
+  void* b = nullptr;      // right-hand side of this level (levels >= 1)
+  void* r = nullptr;      // residual scratch
+  void* cur = nullptr;    // buffer holding the current iterate
+};
+
+}  // namespace
+
+struct hh_solver {
+  hh_op* op = nullptr;
+  hh_mg_params p{};
+  std::vector<double> shift;
+  // hierarchy
+  bool built = false;
+  std::vector<Level> lv;
+  cx<double>* coarse_aug = nullptr;  // n x 2n augmented; right half = inverse
+  cx<double>* coarse_d = nullptr;    // coarsest diagonal in complex128
+  int64_t* piv = nullptr;
+  cx<double>* fac = nullptr;
+  cx<double>* pinv = nullptr;
+  int cap = 0;  // RHS capacity of the work buffers
+  // Krylov storage
+  cx<double>* V = nullptr;  // (inner+1) blocks
+  void* Z = nullptr;        // inner blocks in preconditioner precision
+  cx<double>* xk = nullptr; // iterate
+  cx<double>* bk = nullptr; // right-hand sides
+  cx<double>* wk = nullptr; // H z / residual
+  void* b0 = nullptr;       // fine-level V-cycle input in c64 mode
+  KrylovWork kw{};
+  int* skip_copy = nullptr;
+  int* conv_d = nullptr;
+  double* n2tmp = nullptr;
+  double* res_d = nullptr;
+  int* h_done = nullptr;   // pinned host mirrors
+  int* h_kact = nullptr;
+  double* h_res = nullptr;
+  double setup_time = 0, solve_time = 0;
+  int64_t n_prec = 0;
+
+  void free_work() {
+    for (auto& L : lv) {
+      dfree(L.xa); dfree(L.xb); dfree(L.b); dfree(L.r);
+      L.xa = L.xb = L.b = L.r = L.cur = nullptr;
+    }
+    dfree(V); dfree(Z); dfree(xk); dfree(bk); dfree(wk); dfree(b0);
+    V = nullptr; Z = nullptr; xk = bk = wk = nullptr; b0 = nullptr;
+    dfree(kw.part); dfree(kw.partn); dfree(kw.h1); dfree(kw.h2); dfree(kw.nrm); dfree(kw.H); dfree(kw.cs);
+    dfree(kw.sn); dfree(kw.gv); dfree(kw.y); dfree(kw.bnorm); dfree(kw.est); dfree(kw.done); dfree(kw.kact);
+    kw = KrylovWork{};
+    dfree(skip_copy); dfree(conv_d); dfree(n2tmp); dfree(res_d);
+    skip_copy = nullptr; conv_d = nullptr; n2tmp = nullptr; res_d = nullptr;
+    if (h_done) (void)hipHostFree(h_done);
+    if (h_kact) (void)hipHostFree(h_kact);
+    if (h_res) (void)hipHostFree(h_res);
+    h_done = nullptr; h_kact = nullptr; h_res = nullptr;
+    cap = 0;
+  }
+  void free_hier() {
+    free_work();
+    for (auto& L : lv) { dfree(L.m); dfree(L.gam); dfree(L.d); }
+    lv.clear();
+    dfree(coarse_aug); dfree(coarse_d); dfree(piv); dfree(fac); dfree(pinv);
+    coarse_aug = nullptr; coarse_d = nullptr; piv = nullptr; fac = nullptr; pinv = nullptr;
+    built = false;
+  }
+  ~hh_solver() { free_hier(); }
+};
+
+namespace {
+
+size_t csz(int prec) { return prec == HH_C64 ? sizeof(cx<float>) : sizeof(cx<double>); }
+
+void build_hierarchy(hh_solver* S) {
+  hh_op* op = S->op;
+  hipStream_t st = op->stream;
+  const int L = S->p.levels;
+  S->lv.resize(L);
+  int64_t nn[3] = {op->nn[0], op->nn[1], op->dim == 3 ? op->nn[2] : 1};
+  double h[3] = {op->h[0], op->h[1], op->h[2]};
+  for (int l = 0; l < L; ++l) {
+    Level& lv = S->lv[l];
+    lv.g = make_grid(op->dim, nn, h, op->bc);
+    memcpy(lv.h, h, sizeof(h));
+    lv.m = dalloc<double>(lv.g.N);
+    lv.gam = dalloc<double>(lv.g.N);
+    if (l == 0) {
+      HIPCK(hipMemcpyAsync(lv.m, op->m, sizeof(double) * lv.g.N, hipMemcpyDeviceToDevice, st));
+      HIPCK(hipMemcpyAsync(lv.gam, op->gam, sizeof(double) * lv.g.N, hipMemcpyDeviceToDevice, st));
+      launch_add_const(lv.gam, S->shift[0] * op->wre, lv.g.N, st);
+    } else {
+      Level& f = S->lv[l - 1];
+      launch_restrict_real(f.g, lv.g, f.m, lv.m, st);
+      launch_restrict_real(f.g, lv.g, f.gam, lv.gam, st);
+      launch_add_const(lv.gam, (S->shift[l] - S->shift[l - 1]) * op->wre, lv.g.N, st);
+    }
+    if (S->p.prec == HH_C64) {
+      lv.d = dalloc<cx<float>>(lv.g.N);
+      launch_diag<float>(lv.g, lv.m, lv.gam, op->wre, op->wim, op->neumann_top, op->sommerfeld, lv.h,
+                         (cx<float>*)lv.d, st);
+    } else {
+      lv.d = dalloc<cx<double>>(lv.g.N);
+      launch_diag<double>(lv.g, lv.m, lv.gam, op->wre, op->wim, op->neumann_top, op->sommerfeld, lv.h,
+                          (cx<double>*)lv.d, st);
+    }
+    if (l + 1 < L) {
+      for (int d = 0; d < op->dim; ++d) {
+        if ((nn[d] - 1) % 2 != 0)
+          throw HHError(HH_ERR_ARG, "level " + std::to_string(l) + " has an odd cell count on axis " +
+                                        std::to_string(d) + "; use fewer levels");
+        nn[d] = (nn[d] - 1) / 2 + 1;
+        h[d] *= 2.0;
+        if (nn[d] < 3) throw HHError(HH_ERR_ARG, "too many levels for this grid");
+      }
+    }
+  }
+  // coarsest level: dense complex128 inverse of the shifted operator
+  Level& c = S->lv[L - 1];
+  const int64_t n = c.g.N;
+  if (n > 4096) throw HHError(HH_ERR_ARG, "coarsest grid has " + std::to_string(n) + " nodes (> 4096); add levels");
+  S->coarse_d = dalloc<cx<double>>(n);
+  launch_diag<double>(c.g, c.m, c.gam, op->wre, op->wim, op->neumann_top, op->sommerfeld, c.h, S->coarse_d, st);
+  S->coarse_aug = dalloc<cx<double>>((size_t)n * 2 * n);
+  S->piv = dalloc<int64_t>(1);
+  S->fac = dalloc<cx<double>>(n);
+  S->pinv = dalloc<cx<double>>(1);
+  launch_dense_inverse(c.g, S->coarse_d, S->coarse_aug, S->piv, S->fac, S->pinv, st);
+  HIPCK(hipGetLastError());
+  HIPCK(hipStreamSynchronize(st));
+  S->built = true;
+}
+
+const cx<double>* coarse_inv(hh_solver* S) {
+  const int64_t n = S->lv.back().g.N;
+  return S->coarse_aug + n * n;
+}
+
+void ensure_work(hh_solver* S, int nrhs) {
+  if (nrhs <= S->cap) return;
+  S->free_work();
+  const int pr = S->p.prec;
+  const size_t es = csz(pr);
+  for (size_t l = 0; l < S->lv.size(); ++l) {
+    Level& L = S->lv[l];
+    const size_t nb = (size_t)L.g.N * nrhs * es;
+    L.xa = dalloc<char>(nb);
+    L.xb = dalloc<char>(nb);
+    if (l > 0) L.b = dalloc<char>(nb);
+    L.r = dalloc<char>(nb);
+  }
+  const int64_t N = S->lv[0].g.N;
+  const int K = S->p.inner;
+  const size_t blk = (size_t)N * nrhs;
+  S->V = dalloc<cx<double>>(blk * (K + 1));
+  S->Z = dalloc<char>(blk * K * es);
+  S->xk = dalloc<cx<double>>(blk);
+  S->bk = dalloc<cx<double>>(blk);
+  S->wk = dalloc<cx<double>>(blk);
+  if (pr == HH_C64) S->b0 = dalloc<cx<float>>(blk);
+  KrylovWork& w = S->kw;
+  w.nrhs = nrhs;
+  w.N = N;
+  w.kmax = K;
+  int nchunk = (int)std::max<int64_t>(1, std::min<int64_t>(2048 / nrhs + 1, (N + 4095) / 4096));
+  w.nchunk = nchunk;
+  w.part = dalloc<cx<double>>((size_t)nchunk * (K + 1) * nrhs);
+  w.partn = dalloc<double>((size_t)nchunk * nrhs);
+  w.h1 = dalloc<cx<double>>((size_t)(K + 1) * nrhs);
+  w.h2 = dalloc<cx<double>>((size_t)(K + 1) * nrhs);
+  w.nrm = dalloc<double>(nrhs);
+  w.H = dalloc<cx<double>>((size_t)nrhs * (K + 1) * K);
+  w.cs = dalloc<cx<double>>((size_t)nrhs * K);
+  w.sn = dalloc<cx<double>>((size_t)nrhs * K);
+  w.gv = dalloc<cx<double>>((size_t)nrhs * (K + 1));
+  w.y = dalloc<cx<double>>((size_t)nrhs * K);
+  w.bnorm = dalloc<double>(nrhs);
+  w.est = dalloc<double>(nrhs);
+  w.done = dalloc<int>(nrhs);
+  w.kact = dalloc<int>(nrhs);
+  S->skip_copy = dalloc<int>(nrhs);
+  S->conv_d = dalloc<int>(nrhs);
+  S->n2tmp = dalloc<double>(nrhs);
+  S->res_d = dalloc<double>(nrhs);
+  HIPCK(hipHostMalloc((void**)&S->h_done, sizeof(int) * nrhs, hipHostMallocDefault));
+  HIPCK(hipHostMalloc((void**)&S->h_kact, sizeof(int) * nrhs, hipHostMallocDefault));
+  HIPCK(hipHostMalloc((void**)&S->h_res, sizeof(double) * nrhs, hipHostMallocDefault));
+  S->cap = nrhs;
+}
+
+// ---------------------------------------------------------------- MG cycle
+template <typename T>
+struct Cycle {
+  hh_solver* S;
+  int nrhs;
+  const int* skip;
+  hipStream_t st;
+
+  void* other(Level& L) { return L.cur == L.xa ? L.xb : L.xa; }
+
+  void sweeps(Level& L, const cx<T>* b, int n) {
+    for (int s = 0; s < n; ++s) {
+      void* o = other(L);
+      launch_stencil<T, T, T, M_JACOBI>(L.g, nrhs, (const cx<T>*)L.cur, b, (cx<T>*)o, (const cx<T>*)L.d, nullptr,
+                                        mk<double>(0, 0), S->p.relax, skip, st);
+      L.cur = o;
+    }
+  }
+
+  // Solve approximately A_l x = b; zero_init: start from x = 0 (then L.cur is
+  // chosen by the caller through `first`), else continue from L.cur.
+  void run(int l, const cx<T>* b, bool zero_init, void* first) {
+    Level& L = S->lv[l];
+    const int nl = (int)S->lv.size();
+    if (l == nl - 1) {
+      if (!L.cur) L.cur = first ? first : L.xa;
+      launch_dense_apply<T>(coarse_inv(S), L.g.N, nrhs, b, (cx<T>*)L.cur, skip, st);
+      return;
+    }
+    const int pre = S->p.pre, post = S->p.post;
+    if (zero_init) {
+      L.cur = first ? first : L.xa;
+      if (pre > 0) {
+        launch_jacobi0<T>(L.g, nrhs, b, (cx<T>*)L.cur, (const cx<T>*)L.d, S->p.relax, skip, st);
+        sweeps(L, b, pre - 1);
+      } else {
+        HIPCK(hipMemsetAsync(L.cur, 0, sizeof(cx<T>) * L.g.N * nrhs, st));
+      }
+    } else {
+      sweeps(L, b, pre);
+    }
+    Level& C = S->lv[l + 1];
+    launch_stencil<T, T, T, M_RESID>(L.g, nrhs, (const cx<T>*)L.cur, b, (cx<T>*)L.r, (const cx<T>*)L.d, nullptr,
+                                     mk<double>(0, 0), 0.0, skip, st);
+    launch_restrict<T>(L.g, C.g, nrhs, (const cx<T>*)L.r, (cx<T>*)C.b, skip, st);
+    C.cur = nullptr;
+    run(l + 1, (const cx<T>*)C.b, true, nullptr);
+    if (S->p.cycle == 'W' && l + 1 < nl - 1) run(l + 1, (const cx<T>*)C.b, false, nullptr);
+    launch_prolong_add<T>(L.g, C.g, nrhs, (const cx<T>*)C.cur, (cx<T>*)L.cur, skip, st);
+    sweeps(L, b, post);
+  }
+
+  // z = M^-1 b on the fine level, result written to `out`
+  void apply(const cx<T>* b, cx<T>* out) {
+    Level& L0 = S->lv[0];
+    const int nl = (int)S->lv.size();
+    if (nl == 1) {
+      L0.cur = out;
+      launch_dense_apply<T>(coarse_inv(S), L0.g.N, nrhs, b, out, skip, st);
+      return;
+    }
+    // number of buffer flips on level 0: pre (incl. the zero-start sweep) + post
+    const int flips = S->p.pre + S->p.post;
+    // choose the ping-pong pair (out, xa) so the last write lands in out
+    void* saved_xb = L0.xb;
+    L0.xb = out;
+    void* first = (S->p.pre == 0) ? (void*)out : ((flips % 2 == 1) ? (void*)out : L0.xa);
+    run(0, b, true, first);
+    if (L0.cur != out)
+      HIPCK(hipMemcpyAsync(out, L0.cur, sizeof(cx<T>) * L0.g.N * nrhs, hipMemcpyDeviceToDevice, st));
+    L0.xb = saved_xb;
+  }
+};
+
+template <typename T>
+void precondition(hh_solver* S, int nrhs, const cx<double>* vj, void* zj, const int* skip, hipStream_t st) {
+  Cycle<T> c{S, nrhs, skip, st};
+  const int64_t blk = S->lv[0].g.N * (int64_t)nrhs;
+  if (std::is_same<T, float>::value) {
+    launch_cast<double, float>(vj, (cx<float>*)S->b0, blk, st);
+    c.apply((const cx<T>*)S->b0, (cx<T>*)zj);
+  } else {
+    c.apply((const cx<T>*)vj, (cx<T>*)zj);
+  }
+}
+
+void h_apply(hh_solver* S, int nrhs, const cx<double>* x, cx<double>* y, const int* skip, hipStream_t st) {
+  hh_op* op = S->op;
+  launch_stencil<double, double, double, M_APPLY>(op->g, nrhs, x, nullptr, y, op->d64, nullptr, mk<double>(0, 0),
+                                                  0.0, skip, st);
+}
+
+void h_resid(hh_solver* S, int nrhs, const cx<double>* x, const cx<double>* b, cx<double>* r, hipStream_t st) {
+  hh_op* op = S->op;
+  launch_stencil<double, double, double, M_RESID>(op->g, nrhs, x, b, r, op->d64, nullptr, mk<double>(0, 0), 0.0,
+                                                  nullptr, st);
+}
+
+void run_fgmres(hh_solver* S, int nrhs, hh_solve_stats* stats) {
+  hipStream_t st = S->op->stream;
+  KrylovWork w = S->kw;
+  w.nrhs = nrhs;  // capacity may exceed nrhs; kernels index with nrhs-major strides
+  const int64_t N = S->lv[0].g.N;
+  const int64_t blk = N * (int64_t)nrhs;
+  const int K = S->p.inner;
+  const double tol = S->p.tol;
+  const size_t zes = csz(S->p.prec);
+  std::vector<int> iters(nrhs, 0), conv(nrhs, 0);
+  // ||b|| per RHS
+  kr_norm2(w, S->bk, S->n2tmp, nullptr, st);
+  hipLaunchKernelGGL(k_sqrt_arr, dim3((nrhs + 63) / 64), dim3(64), 0, st, S->n2tmp, w.bnorm, nrhs);
+  HIPCK(hipMemcpyAsync(S->h_res, w.bnorm, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
+  HIPCK(hipStreamSynchronize(st));
+  std::vector<double> bn(S->h_res, S->h_res + nrhs);
+  for (int b = 0; b < nrhs; ++b) conv[b] = !(bn[b] > 0.0);
+  HIPCK(hipMemsetAsync(S->xk, 0, sizeof(cx<double>) * blk, st));
+  HIPCK(hipMemcpyAsync(S->wk, S->bk, sizeof(cx<double>) * blk, hipMemcpyDeviceToDevice, st));
+  std::vector<double> res(nrhs, 1.0);
+  int64_t nprec = 0;
+  for (int outer = 0; outer < S->p.max_outer; ++outer) {
+    HIPCK(hipMemcpyAsync(S->conv_d, conv.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, st));
+    kr_start_impl(w, S->wk, S->V, tol, S->conv_d, S->n2tmp, st);
+    HIPCK(hipMemcpyAsync(S->h_done, w.done, sizeof(int) * nrhs, hipMemcpyDeviceToHost, st));
+    HIPCK(hipStreamSynchronize(st));
+    int active = 0;
+    for (int b = 0; b < nrhs; ++b) active += S->h_done[b] == 0;
+    if (active == 0) {
+      // every RHS already meets the tolerance at the cycle start
+      break;
+    }
+    for (int j = 0; j < K; ++j) {
+      const cx<double>* vj = S->V + (int64_t)j * blk;
+      void* zj = (char*)S->Z + (size_t)j * blk * zes;
+      if (S->p.prec == HH_C64) {
+        precondition<float>(S, nrhs, vj, zj, w.done, st);
+        launch_stencil<float, double, double, M_APPLY>(S->op->g, nrhs, (const cx<float>*)zj, nullptr, S->wk,
+                                                       S->op->d64, nullptr, mk<double>(0, 0), 0.0, w.done, st);
+      } else {
+        precondition<double>(S, nrhs, vj, zj, w.done, st);
+        h_apply(S, nrhs, (const cx<double>*)zj, S->wk, w.done, st);
+      }
+      kr_arnoldi_impl(w, S->V, S->wk, j, tol, S->skip_copy, st);
+      ++nprec;
+      HIPCK(hipMemcpyAsync(S->h_done, w.done, sizeof(int) * nrhs, hipMemcpyDeviceToHost, st));
+      HIPCK(hipStreamSynchronize(st));
+      active = 0;
+      for (int b = 0; b < nrhs; ++b) active += S->h_done[b] == 0;
+      if (active == 0) break;
+    }
+    if (S->p.prec == HH_C64)
+      kr_update_x<float>(w, (const cx<float>*)S->Z, S->xk, st);
+    else
+      kr_update_x<double>(w, (const cx<double>*)S->Z, S->xk, st);
+    HIPCK(hipMemcpyAsync(S->h_kact, w.kact, sizeof(int) * nrhs, hipMemcpyDeviceToHost, st));
+    // true residual in complex128
+    h_resid(S, nrhs, S->xk, S->bk, S->wk, st);
+    kr_true_res(w, S->wk, S->res_d, S->n2tmp, st);
+    HIPCK(hipMemcpyAsync(S->h_res, S->res_d, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
+    HIPCK(hipStreamSynchronize(st));
+    HIPCK(hipGetLastError());
+    int nconv = 0;
+    for (int b = 0; b < nrhs; ++b) {
+      iters[b] += conv[b] ? 0 : S->h_kact[b];
+      res[b] = S->h_res[b];
+      if (!conv[b] && res[b] <= tol) conv[b] = 1;
+      if (!(res[b] == res[b])) conv[b] = 1;  // non-finite: stop iterating it
+      nconv += conv[b];
+    }
+    if (S->p.verbose) fprintf(stderr, "[helmholtz_amd] restart %d: %d/%d RHS converged\n", outer, nconv, nrhs);
+    if (nconv == nrhs) break;
+  }
+  S->n_prec += nprec;
+  if (stats) {
+    int mx = 0, nc = 0;
+    for (int b = 0; b < nrhs; ++b) {
+      int status;
+      if (!(bn[b] > 0.0))
+        status = HH_ZERO_RHS, res[b] = 0.0;
+      else if (!(res[b] == res[b]))
+        status = HH_NONFINITE;
+      else if (res[b] <= tol)
+        status = HH_CONVERGED;
+      else
+        status = HH_MAXITER;
+      if (stats->status) stats->status[b] = status;
+      if (stats->iters) stats->iters[b] = iters[b];
+      if (stats->true_res) stats->true_res[b] = res[b];
+      mx = std::max(mx, iters[b]);
+      nc += (status == HH_CONVERGED || status == HH_ZERO_RHS);
+    }
+    stats->max_iters = mx;
+    stats->n_converged = nc;
+    stats->n_prec = nprec;
+  }
+  bool warn = false;
+  for (int b = 0; b < nrhs; ++b) warn |= (bn[b] > 0.0 && !(res[b] <= tol));
+  if (warn) fprintf(stderr, "WARNING: MG solver reached maximum iterations without convergence\n");
+}
+
+}  // namespace
+
+// ============================================================== C ABI
+extern "C" {
+
+const char* hh_last_error(void) { return g_err.c_str(); }
+int hh_version(void) { return HH_API_VERSION; }
+
+int hh_op_create(int dim, const int64_t nnodes[3], const double h[3], const double* m, const double* gamma,
+                 double omega_re, double omega_im, int neumann_top, int sommerfeld, int bc_order, hh_op** out) {
+  return guarded([&] {
+    if (!out || !nnodes || !h || !m || !gamma) throw HHError(HH_ERR_ARG, "null argument");
+    if (dim != 2 && dim != 3) throw HHError(HH_ERR_ARG, "dim must be 2 or 3");
+    if (bc_order != 1 && bc_order != 2) throw HHError(HH_ERR_ARG, "bc_order must be 1 or 2");
+    if (!(omega_re != 0.0)) throw HHError(HH_ERR_ARG, "Re(omega) must be nonzero");
+    int64_t N = 1;
+    for (int d = 0; d < dim; ++d) {
+      if (nnodes[d] < 3 || nnodes[d] > (1 << 24)) throw HHError(HH_ERR_ARG, "each axis needs 3..2^24 nodes");
+      if (!(h[d] > 0.0)) throw HHError(HH_ERR_ARG, "h must be positive");
+      N *= nnodes[d];
+    }
+    if (N > ((int64_t)1 << 31)) throw HHError(HH_ERR_ARG, "grid too large");
+    hh_op* op = new hh_op();
+    try {
+      op->dim = dim;
+      for (int d = 0; d < 3; ++d) {
+        op->nn[d] = d < dim ? nnodes[d] : 1;
+        op->h[d] = d < dim ? h[d] : 1.0;
+      }
+      op->wre = omega_re;
+      op->wim = omega_im;
+      op->neumann_top = neumann_top;
+      op->sommerfeld = sommerfeld;
+      op->bc = bc_order;
+      op->g = make_grid(dim, op->nn, op->h, bc_order);
+      HIPCK(hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking));
+      op->m = dalloc<double>(N);
+      op->gam = dalloc<double>(N);
+      op->d64 = dalloc<cx<double>>(N);
+      op->d32 = dalloc<cx<float>>(N);
+      HIPCK(hipMemcpyAsync(op->m, m, sizeof(double) * N, hipMemcpyHostToDevice, op->stream));
+      HIPCK(hipMemcpyAsync(op->gam, gamma, sizeof(double) * N, hipMemcpyHostToDevice, op->stream));
+      launch_diag<double>(op->g, op->m, op->gam, omega_re, omega_im, neumann_top, sommerfeld, op->h, op->d64,
+                          op->stream);
+      launch_diag<float>(op->g, op->m, op->gam, omega_re, omega_im, neumann_top, sommerfeld, op->h, op->d32,
+                         op->stream);
+      HIPCK(hipGetLastError());
+      HIPCK(hipStreamSynchronize(op->stream));
+    } catch (...) {
+      delete op;
+      throw;
+    }
+    *out = op;
+  });
+}
+
+int hh_op_destroy(hh_op* op) {
+  return guarded([&] { delete op; });
+}
+
+int hh_op_apply(hh_op* op, double shift, const void* X, void* Y, int64_t nrhs, int prec, int loc) {
+  return guarded([&] {
+    if (!op || !X || !Y) throw HHError(HH_ERR_ARG, "null argument");
+    if (nrhs < 1 || nrhs > 65535) throw HHError(HH_ERR_ARG, "nrhs must be in 1..65535");
+    if (prec != HH_C128 && prec != HH_C64) throw HHError(HH_ERR_ARG, "prec must be HH_C128 or HH_C64");
+    if (loc != HH_HOST && loc != HH_DEVICE) throw HHError(HH_ERR_ARG, "loc must be HH_HOST or HH_DEVICE");
+    const size_t bytes = (size_t)op->g.N * nrhs * csz(prec);
+    const void* xin = X;
+    void* yout = Y;
+    if (loc == HH_HOST) {
+      op->ensure_stage(bytes);
+      HIPCK(hipMemcpyAsync(op->stage_in, X, bytes, hipMemcpyHostToDevice, op->stream));
+      xin = op->stage_in;
+      yout = op->stage_out;
+    }
+    // i * shift * omega^2
+    const double w2r = op->wre * op->wre - op->wim * op->wim, w2i = 2.0 * op->wre * op->wim;
+    const cx<double> sfac = mk<double>(-shift * w2i, shift * w2r);
+    const double* mm = shift != 0.0 ? op->m : nullptr;
+    if (prec == HH_C128)
+      launch_stencil<double, double, double, M_APPLY>(op->g, (int)nrhs, (const cx<double>*)xin, nullptr,
+                                                      (cx<double>*)yout, op->d64, mm, sfac, 0.0, nullptr,
+                                                      op->stream);
+    else
+      launch_stencil<float, float, float, M_APPLY>(op->g, (int)nrhs, (const cx<float>*)xin, nullptr,
+                                                   (cx<float>*)yout, op->d32, mm, sfac, 0.0, nullptr, op->stream);
+    HIPCK(hipGetLastError());
+    if (loc == HH_HOST) HIPCK(hipMemcpyAsync(Y, yout, bytes, hipMemcpyDeviceToHost, op->stream));
+    HIPCK(hipStreamSynchronize(op->stream));
+  });
+}
+
+int hh_solver_create(hh_op* op, const hh_mg_params* p, const double* shift_per_level, hh_solver** out) {
+  return guarded([&] {
+    if (!op || !p || !out || !shift_per_level) throw HHError(HH_ERR_ARG, "null argument");
+    if (p->levels < 1 || p->levels > 16) throw HHError(HH_ERR_ARG, "levels must be in 1..16");
+    if (p->inner < 1 || p->inner > 32) throw HHError(HH_ERR_ARG, "inner must be in 1..32");
+    if (p->pre < 0 || p->post < 0 || p->pre > 64 || p->post > 64) throw HHError(HH_ERR_ARG, "bad sweep counts");
+    if (p->cycle != 'V' && p->cycle != 'W') throw HHError(HH_ERR_ARG, "cycle must be 'V' or 'W'");
+    if (p->krylov != HH_FGMRES) throw HHError(HH_ERR_UNSUPPORTED, "only HH_FGMRES is implemented");
+    if (p->prec != HH_C128 && p->prec != HH_C64) throw HHError(HH_ERR_ARG, "prec must be HH_C128 or HH_C64");
+    if (p->learned != 0) throw HHError(HH_ERR_UNSUPPORTED, "learned correction not available in this entry point");
+    if (p->max_outer < 1 || !(p->tol > 0.0)) throw HHError(HH_ERR_ARG, "max_outer >= 1 and tol > 0 required");
+    hh_solver* s = new hh_solver();
+    s->op = op;
+    s->p = *p;
+    s->shift.assign(shift_per_level, shift_per_level + p->levels);
+    *out = s;
+  });
+}
+
+int hh_solver_copy(const hh_solver* s, hh_solver** out) {
+  return guarded([&] {
+    if (!s || !out) throw HHError(HH_ERR_ARG, "null argument");
+    hh_solver* c = new hh_solver();
+    c->op = s->op;
+    c->p = s->p;
+    c->shift = s->shift;
+    *out = c;
+  });
+}
+
+int hh_solver_clear(hh_solver* s) {
+  return guarded([&] {
+    if (!s) throw HHError(HH_ERR_ARG, "null argument");
+    s->free_hier();
+  });
+}
+
+int hh_solver_destroy(hh_solver* s) {
+  return guarded([&] { delete s; });
+}
+
+int hh_solver_levels(const hh_solver* s, int* levels, int64_t nnodes_of_level[][3], int max_levels) {
+  return guarded([&] {
+    if (!s || !levels) throw HHError(HH_ERR_ARG, "null argument");
+    *levels = s->built ? (int)s->lv.size() : 0;
+    if (nnodes_of_level)
+      for (int l = 0; l < *levels && l < max_levels; ++l) {
+        nnodes_of_level[l][0] = s->lv[l].g.n1;
+        nnodes_of_level[l][1] = s->lv[l].g.n2;
+        nnodes_of_level[l][2] = s->lv[l].g.n3;
+      }
+  });
+}
+
+int hh_solve(hh_solver* s, const void* B, void* X, int64_t nrhs, int do_transpose, int loc, hh_solve_stats* stats) {
+  return guarded([&] {
+    if (!s || !B || !X) throw HHError(HH_ERR_ARG, "null argument");
+    if (nrhs < 1 || nrhs > 65535) throw HHError(HH_ERR_ARG, "nrhs must be in 1..65535");
+    if (do_transpose != 0) throw HHError(HH_ERR_UNSUPPORTED, "do_transpose = 1 is not implemented");
+    if (loc != HH_HOST && loc != HH_DEVICE) throw HHError(HH_ERR_ARG, "loc must be HH_HOST or HH_DEVICE");
+    hh_op* op = s->op;
+    hipStream_t st = op->stream;
+    const int64_t N = op->g.N;
+    const size_t bytes = sizeof(cx<double>) * N * nrhs;
+    double t0 = now_s();
+    if (!s->built) build_hierarchy(s);
+    ensure_work(s, (int)nrhs);
+    double t1 = now_s();
+    s->setup_time += t1 - t0;
+    if (stats) stats->setup_time = t1 - t0;
+    HIPCK(hipMemcpyAsync(s->bk, B, bytes, loc == HH_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, st));
+    run_fgmres(s, (int)nrhs, stats);
+    HIPCK(hipMemcpyAsync(X, s->xk, bytes, loc == HH_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, st));
+    HIPCK(hipStreamSynchronize(st));
+    double t2 = now_s();
+    s->solve_time += t2 - t1;
+    if (stats) stats->solve_time = t2 - t1;
+  });
+}
+
+int hh_abl(int dim, const int64_t nnodes[3], int neumann_at_first_dim, const int64_t pad[3], double amp,
+           double* out, int loc) {
+  return guarded([&] {
+    if (!nnodes || !pad || !out) throw HHError(HH_ERR_ARG, "null argument");
+    if (dim != 2 && dim != 3) throw HHError(HH_ERR_ARG, "dim must be 2 or 3");
+    if (loc != HH_HOST && loc != HH_DEVICE) throw HHError(HH_ERR_ARG, "loc must be HH_HOST or HH_DEVICE");
+    std::vector<std::vector<double>> prof(3);
+    int64_t N = 1;
+    for (int d = 0; d < 3; ++d) {
+      const int64_t n = d < dim ? nnodes[d] : 1;
+      if (n < 1 || n > (1 << 24)) throw HHError(HH_ERR_ARG, "bad node count");
+      N *= n;
+      prof[d].assign(n, 0.0);
+      if (d >= dim) continue;
+      const int64_t pd = pad[d];
+      if (pd < 1 || pd > n) throw HHError(HH_ERR_ARG, "pad must be in 1..n");
+      std::vector<double>& P = prof[d];
+      if (dim == 2) {
+        // quadratic ramps ((pad - k)/pad)^2 at the low end, ((k + 1)/pad)^2 at the high end
+        const bool low_on = !(d == 1 && neumann_at_first_dim);
+        for (int64_t k = 0; k < pd; ++k) {
+          if (low_on) P[k] += (double)((pd - k) * (pd - k)) / (double)(pd * pd);
+          P[n - pd + k] += (double)((k + 1) * (k + 1)) / (double)(pd * pd);
+        }
+      } else {
+        // squared distance to the inner edge of the pad on a [-1,1] (x, y) or [0,1] (z) axis
+        const double lo = d == 2 ? 0.0 : -1.0, len = d == 2 ? 1.0 : 2.0;
+        auto xc = [&](int64_t k) { return lo + len * (double)k / (double)(n - 1); };
+        const bool low_on = !(d == 2 && neumann_at_first_dim);
+        for (int64_t k = 0; k < n; ++k) {
+          double v = 0.0;
+          if (low_on && k < pd) v += (xc(k) - xc(pd - 1)) * (xc(k) - xc(pd - 1));
+          if (k >= n - pd) v += (xc(k) - xc(n - pd)) * (xc(k) - xc(n - pd));
+          P[k] = v;
+        }
+        double mx = 0.0;
+        for (double v : P) mx = v > mx ? v : mx;
+        for (double& v : P) v /= (mx + 1e-5);
+      }
+    }
+    hipStream_t st = nullptr;
+    double* dp[3];
+    for (int d = 0; d < 3; ++d) {
+      dp[d] = dalloc<double>(prof[d].size());
+      HIPCK(hipMemcpy(dp[d], prof[d].data(), sizeof(double) * prof[d].size(), hipMemcpyHostToDevice));
+    }
+    double* dout = loc == HH_DEVICE ? out : dalloc<double>(N);
+    hipLaunchKernelGGL(k_abl, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, dim, (int)prof[0].size(),
+                       (int)prof[1].size(), (int)prof[2].size(), dp[0], dp[1], dp[2], amp, dout);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess && loc == HH_HOST) e = hipMemcpy(out, dout, sizeof(double) * N, hipMemcpyDeviceToHost);
+    for (int d = 0; d < 3; ++d) dfree(dp[d]);
+    if (loc == HH_HOST) dfree(dout);
+    hipck(e, "hh_abl");
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,74 @@
+// Host-side launchers for the device kernels (internal to libhelmholtz_amd).
+#pragma once
+#include "hh_common.h"
+
+namespace hh {
+
+enum { M_APPLY = 0, M_RESID = 1, M_JACOBI = 2 };
+
+// y = A x | y = b - A x | y = x + wj D^-1 (b - A x);  A = Lap + diag(d + sfac m)
+template <typename TI, typename TO, typename TD, int MODE>
+void launch_stencil(const Grid& g, int nrhs, const cx<TI>* x, const cx<TO>* b, cx<TO>* y, const cx<TD>* d,
+                    const double* m, cx<double> sfac, double wj, const int* skip, hipStream_t s);
+template <typename T>
+void launch_jacobi0(const Grid& g, int nrhs, const cx<T>* b, cx<T>* x, const cx<T>* d, double wj, const int* skip,
+                    hipStream_t s);
+template <typename T>
+void launch_restrict(const Grid& gf, const Grid& gc, int nrhs, const cx<T>* rf, cx<T>* bc, const int* skip,
+                     hipStream_t s);
+void launch_restrict_real(const Grid& gf, const Grid& gc, const double* f, double* c, hipStream_t s);
+template <typename T>
+void launch_prolong_add(const Grid& gf, const Grid& gc, int nrhs, const cx<T>* xc, cx<T>* xf, const int* skip,
+                        hipStream_t s);
+template <typename A, typename B>
+void launch_cast(const cx<A>* a, cx<B>* b, int64_t n, hipStream_t s);
+template <typename T>
+void launch_diag(const Grid& g, const double* m, const double* gam, double wre, double wim, int neumann_top,
+                 int sommerfeld, const double h[3], cx<T>* d, hipStream_t s);
+void launch_add_const(double* a, double c, int64_t n, hipStream_t s);
+void launch_dense_inverse(const Grid& g, const cx<double>* d, cx<double>* M, int64_t* piv, cx<double>* fac,
+                          cx<double>* pinv, hipStream_t s);
+template <typename T>
+void launch_dense_apply(const cx<double>* Ainv, int64_t n, int nrhs, const cx<T>* r, cx<T>* x, const int* skip,
+                        hipStream_t s);
+
+// ---- batched FGMRES (hh_krylov.hip) ----
+struct KrylovWork {
+  int nrhs;
+  int64_t N;
+  int kmax;              // restart length
+  int nchunk;            // node chunks per RHS for the reductions
+  cx<double>* part;      // [nchunk][kmax+1][nrhs] partial dots
+  double* partn;         // [nchunk][nrhs] partial squared norms
+  cx<double>* h1;        // [kmax+1][nrhs]
+  cx<double>* h2;        // [kmax+1][nrhs]
+  double* nrm;           // [nrhs] scratch norm
+  cx<double>* H;         // [nrhs][kmax+1][kmax] Hessenberg -> R
+  cx<double>* cs;        // [nrhs][kmax]
+  cx<double>* sn;        // [nrhs][kmax]
+  cx<double>* gv;        // [nrhs][kmax+1]
+  cx<double>* y;         // [nrhs][kmax]
+  double* bnorm;         // [nrhs]
+  double* est;           // [nrhs] residual estimate / ||b||
+  int* done;             // [nrhs] 1 = no more work in this cycle
+  int* kact;             // [nrhs] Krylov steps used by this RHS in this cycle
+};
+
+// ||v_b||^2 per RHS into out[b] (RHS with skip[b] != 0 untouched)
+void kr_norm2(const KrylovWork& w, const cx<double>* v, double* out, const int* skip, hipStream_t s);
+// Cycle start: beta_b = ||r_b||, V0 = r / beta, g = (beta, 0, ...), done/kact
+// flags (conv[b] != 0 marks an RHS already converged).  n2tmp: nrhs doubles.
+void kr_start_impl(const KrylovWork& w, const cx<double>* r, cx<double>* V0, double tol, const int* conv,
+                   double* n2tmp, hipStream_t s);
+// One CGS2 Arnoldi step j: orthogonalise wv against V_0..V_j, write V_{j+1},
+// fill column j of H, update Givens rotations and residual estimates.
+// skip_copy: nrhs ints of scratch.
+void kr_arnoldi_impl(const KrylovWork& w, const cx<double>* V, cx<double>* wv, int j, double tol, int* skip_copy,
+                     hipStream_t s);
+// y = R^-1 g per RHS, x += Z y (Z in TZ)
+template <typename TZ>
+void kr_update_x(const KrylovWork& w, const cx<TZ>* Z, cx<double>* x, hipStream_t s);
+// res[b] = ||r_b|| / ||b_b||
+void kr_true_res(const KrylovWork& w, const cx<double>* r, double* res, double* n2tmp, hipStream_t s);
+
+}  // namespace hh

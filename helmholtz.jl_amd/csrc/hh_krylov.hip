@@ -1,0 +1,369 @@
+// Batched flexible GMRES building blocks in complex128, one independent
+// Krylov space per right-hand side, all right-hand sides advanced by the same
+// launches ("block" Gram-Schmidt across RHS: one pass computes the projections
+// of every RHS onto every basis vector).
+//
+// Orthogonalisation is classical Gram-Schmidt applied twice (CGS2), fused into
+// three streaming passes over the basis:
+//   pass 1  h1 = V^H w
+//   pass 2  w' = w - V h1 (stored),  h2 = V^H w',  ||w'||^2
+//   pass 3  v_{j+1} = (w' - V h2) / sqrt(||w'||^2 - ||h2||^2)
+// Partial sums are per (chunk, RHS) and reduced in a fixed order, so results
+// are bit-reproducible run to run.  The small Hessenberg/Givens work runs on
+// the device (one wavefront per RHS) so a whole iteration needs no host sync.
+#include "hh_kernels.h"
+
+namespace hh {
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+
+// block-wide (256 threads = 4 waves) sum; result valid in thread 0
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) r = sh[0] + sh[1] + sh[2] + sh[3];
+  return r;
+}
+
+static __device__ __forceinline__ void chunk_range(int64_t N, int nchunk, int c, int64_t* a, int64_t* b) {
+  const int64_t L = (N + nchunk - 1) / nchunk;
+  *a = (int64_t)c * L;
+  *b = min(N, *a + L);
+}
+
+// pass 1: part[c][i][b] = sum_{p in chunk c} conj(V_i[p,b]) w[p,b], i <= j
+template <int NV>
+__global__ __launch_bounds__(256) void k_dots(KrylovWork kw, const cx<double>* __restrict__ V,
+                                              const cx<double>* __restrict__ w, int j) {
+  const int c = blockIdx.x, b = blockIdx.y;
+  if (kw.done[b]) return;
+  __shared__ double sh[4];
+  int64_t p0, p1;
+  chunk_range(kw.N, kw.nchunk, c, &p0, &p1);
+  const int64_t stride = kw.N * kw.nrhs;
+  const int64_t base = (int64_t)b * kw.N;
+  cx<double> acc[NV];
+#pragma unroll
+  for (int i = 0; i < NV; ++i) acc[i] = mk<double>(0.0, 0.0);
+  for (int64_t p = p0 + threadIdx.x; p < p1; p += 256) {
+    const cx<double> wv = w[base + p];
+#pragma unroll
+    for (int i = 0; i < NV; ++i)
+      if (i <= j) acc[i] = acc[i] + cmulc(V[i * stride + base + p], wv);
+  }
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    if (i > j) break;
+    const double re = block_sum(acc[i].re, sh);
+    const double im = block_sum(acc[i].im, sh);
+    if (threadIdx.x == 0) kw.part[((int64_t)c * (kw.kmax + 1) + i) * kw.nrhs + b] = mk<double>(re, im);
+  }
+}
+
+// reduce part over chunks into h[i][b] (one wavefront per (i, b))
+__global__ __launch_bounds__(64) void k_reduce_dots(KrylovWork kw, cx<double>* __restrict__ h, int j) {
+  const int i = blockIdx.x, b = blockIdx.y;
+  if (kw.done[b] || i > j) return;
+  double re = 0.0, im = 0.0;
+  for (int c = threadIdx.x; c < kw.nchunk; c += 64) {
+    const cx<double> v = kw.part[((int64_t)c * (kw.kmax + 1) + i) * kw.nrhs + b];
+    re += v.re;
+    im += v.im;
+  }
+  re = wave_sum(re);
+  im = wave_sum(im);
+  if (threadIdx.x == 0) h[i * kw.nrhs + b] = mk<double>(re, im);
+}
+
+// pass 2: w' = w - V h1 ; part = V^H w' ; partn = ||w'||^2
+template <int NV>
+__global__ __launch_bounds__(256) void k_update_dots(KrylovWork kw, const cx<double>* __restrict__ V,
+                                                     cx<double>* __restrict__ w, int j) {
+  const int c = blockIdx.x, b = blockIdx.y;
+  if (kw.done[b]) return;
+  __shared__ double sh[4];
+  int64_t p0, p1;
+  chunk_range(kw.N, kw.nchunk, c, &p0, &p1);
+  const int64_t stride = kw.N * kw.nrhs;
+  const int64_t base = (int64_t)b * kw.N;
+  cx<double> hc[NV];
+  cx<double> acc[NV];
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    hc[i] = i <= j ? kw.h1[i * kw.nrhs + b] : mk<double>(0.0, 0.0);
+    acc[i] = mk<double>(0.0, 0.0);
+  }
+  double nn = 0.0;
+  for (int64_t p = p0 + threadIdx.x; p < p1; p += 256) {
+    cx<double> vv[NV];
+    cx<double> wv = w[base + p];
+#pragma unroll
+    for (int i = 0; i < NV; ++i)
+      if (i <= j) {
+        vv[i] = V[i * stride + base + p];
+        wv = wv - hc[i] * vv[i];
+      }
+    w[base + p] = wv;
+    nn += cabs2(wv);
+#pragma unroll
+    for (int i = 0; i < NV; ++i)
+      if (i <= j) acc[i] = acc[i] + cmulc(vv[i], wv);
+  }
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    if (i > j) break;
+    const double re = block_sum(acc[i].re, sh);
+    const double im = block_sum(acc[i].im, sh);
+    if (threadIdx.x == 0) kw.part[((int64_t)c * (kw.kmax + 1) + i) * kw.nrhs + b] = mk<double>(re, im);
+  }
+  const double t = block_sum(nn, sh);
+  if (threadIdx.x == 0) kw.partn[(int64_t)c * kw.nrhs + b] = t;
+}
+
+// Reduce pass-2 sums, form column j of H, apply/update Givens rotations and
+// the residual estimate; one wavefront per RHS.
+__global__ __launch_bounds__(64) void k_hess_givens(KrylovWork kw, int j, double tol) {
+  const int b = blockIdx.x;
+  if (kw.done[b]) return;
+  const int lane = threadIdx.x;
+  __shared__ cx<double> col[72];
+  // h2 and ||w'||^2
+  double s2 = 0.0;
+  for (int i = 0; i <= j; ++i) {
+    double re = 0.0, im = 0.0;
+    for (int c = lane; c < kw.nchunk; c += 64) {
+      const cx<double> v = kw.part[((int64_t)c * (kw.kmax + 1) + i) * kw.nrhs + b];
+      re += v.re;
+      im += v.im;
+    }
+    re = wave_sum(re);
+    im = wave_sum(im);
+    if (lane == 0) {
+      const cx<double> h2 = mk<double>(re, im);
+      kw.h2[i * kw.nrhs + b] = h2;
+      col[i] = kw.h1[i * kw.nrhs + b] + h2;
+      s2 += cabs2(h2);
+    }
+  }
+  double nn = 0.0;
+  for (int c = lane; c < kw.nchunk; c += 64) nn += kw.partn[(int64_t)c * kw.nrhs + b];
+  nn = wave_sum(nn);
+  if (lane != 0) return;
+  const double hn2 = nn - s2;
+  const double hn = hn2 > 0.0 ? sqrt(hn2) : 0.0;
+  kw.nrm[b] = hn > 0.0 ? 1.0 / hn : 0.0;
+  col[j + 1] = mk<double>(hn, 0.0);
+  const int K = kw.kmax;
+  cx<double>* Hb = kw.H + (int64_t)b * (K + 1) * K;
+  cx<double>* csb = kw.cs + (int64_t)b * K;
+  cx<double>* snb = kw.sn + (int64_t)b * K;
+  cx<double>* gb = kw.gv + (int64_t)b * (K + 1);
+  for (int i = 0; i < j; ++i) {
+    const cx<double> a = col[i], c2 = col[i + 1];
+    const cx<double> t = csb[i] * a + snb[i] * c2;
+    col[i + 1] = cmulc(snb[i], a) * mk<double>(-1.0, 0.0) + cmulc(csb[i], c2);
+    col[i] = t;
+  }
+  const cx<double> a = col[j], bb = col[j + 1];
+  const double den = sqrt(cabs2(a) + cabs2(bb));
+  cx<double> cj, sj;
+  if (den > 0.0) {
+    cj = mk<double>(a.re / den, -a.im / den);
+    sj = mk<double>(bb.re / den, -bb.im / den);
+  } else {
+    cj = mk<double>(1.0, 0.0);
+    sj = mk<double>(0.0, 0.0);
+  }
+  csb[j] = cj;
+  snb[j] = sj;
+  col[j] = mk<double>(den, 0.0);
+  for (int i = 0; i <= j; ++i) Hb[i + (int64_t)j * (K + 1)] = col[i];
+  const cx<double> g = gb[j];
+  gb[j + 1] = cmulc(sj, g) * mk<double>(-1.0, 0.0);
+  gb[j] = cj * g;
+  const double est = sqrt(cabs2(gb[j + 1])) / kw.bnorm[b];
+  if (den > 0.0) {
+    kw.kact[b] = j + 1;
+    kw.est[b] = est;
+  }
+  if (!(den > 0.0) || est <= tol || !(est == est) || j + 1 >= K) kw.done[b] = 1;
+}
+
+// pass 3: v_{j+1} = (w' - V h2) * nrm
+template <int NV>
+__global__ __launch_bounds__(256) void k_final(KrylovWork kw, const cx<double>* __restrict__ V,
+                                               const cx<double>* __restrict__ w, cx<double>* __restrict__ vout,
+                                               int j, const int* __restrict__ skip_before) {
+  const int c = blockIdx.x, b = blockIdx.y;
+  if (skip_before[b]) return;
+  int64_t p0, p1;
+  chunk_range(kw.N, kw.nchunk, c, &p0, &p1);
+  const int64_t stride = kw.N * kw.nrhs;
+  const int64_t base = (int64_t)b * kw.N;
+  cx<double> hc[NV];
+#pragma unroll
+  for (int i = 0; i < NV; ++i) hc[i] = i <= j ? kw.h2[i * kw.nrhs + b] : mk<double>(0.0, 0.0);
+  const double s = kw.nrm[b];
+  for (int64_t p = p0 + threadIdx.x; p < p1; p += 256) {
+    cx<double> wv = w[base + p];
+#pragma unroll
+    for (int i = 0; i < NV; ++i)
+      if (i <= j) wv = wv - hc[i] * V[i * stride + base + p];
+    vout[base + p] = s * wv;
+  }
+}
+
+// ||v_b||^2 partials and reduction
+__global__ __launch_bounds__(256) void k_norm_part(KrylovWork kw, const cx<double>* __restrict__ v,
+                                                   const int* __restrict__ skip) {
+  const int c = blockIdx.x, b = blockIdx.y;
+  if (skip && skip[b]) return;
+  __shared__ double sh[4];
+  int64_t p0, p1;
+  chunk_range(kw.N, kw.nchunk, c, &p0, &p1);
+  const int64_t base = (int64_t)b * kw.N;
+  double nn = 0.0;
+  for (int64_t p = p0 + threadIdx.x; p < p1; p += 256) nn += cabs2(v[base + p]);
+  const double t = block_sum(nn, sh);
+  if (threadIdx.x == 0) kw.partn[(int64_t)c * kw.nrhs + b] = t;
+}
+
+__global__ __launch_bounds__(64) void k_norm_reduce(KrylovWork kw, double* __restrict__ out,
+                                                    const int* __restrict__ skip) {
+  const int b = blockIdx.x;
+  if (skip && skip[b]) return;
+  double nn = 0.0;
+  for (int c = threadIdx.x; c < kw.nchunk; c += 64) nn += kw.partn[(int64_t)c * kw.nrhs + b];
+  nn = wave_sum(nn);
+  if (threadIdx.x == 0) out[b] = nn;
+}
+
+// cycle start: beta = ||r||, g = [beta, 0...], flags
+__global__ __launch_bounds__(64) void k_start_flags(KrylovWork kw, const double* __restrict__ n2, double tol,
+                                                    const int* __restrict__ conv) {
+  const int b = blockIdx.x * 64 + threadIdx.x;
+  if (b >= kw.nrhs) return;
+  kw.kact[b] = 0;
+  const double bn = kw.bnorm[b];
+  const double beta = conv[b] ? 0.0 : sqrt(n2[b]);
+  int d = conv[b] || !(bn > 0.0) || !(beta == beta) || beta <= tol * bn;
+  kw.done[b] = d;
+  kw.nrm[b] = (!d && beta > 0.0) ? 1.0 / beta : 0.0;
+  kw.est[b] = bn > 0.0 ? beta / bn : 0.0;
+  cx<double>* gb = kw.gv + (int64_t)b * (kw.kmax + 1);
+  gb[0] = mk<double>(beta, 0.0);
+}
+
+__global__ __launch_bounds__(256) void k_scale(KrylovWork kw, const cx<double>* __restrict__ r,
+                                               cx<double>* __restrict__ v) {
+  const int c = blockIdx.x, b = blockIdx.y;
+  if (kw.done[b]) return;
+  int64_t p0, p1;
+  chunk_range(kw.N, kw.nchunk, c, &p0, &p1);
+  const int64_t base = (int64_t)b * kw.N;
+  const double s = kw.nrm[b];
+  for (int64_t p = p0 + threadIdx.x; p < p1; p += 256) v[base + p] = s * r[base + p];
+}
+
+// y = R^-1 g (back substitution), one thread per RHS
+__global__ __launch_bounds__(64) void k_backsolve(KrylovWork kw) {
+  const int b = blockIdx.x * 64 + threadIdx.x;
+  if (b >= kw.nrhs) return;
+  const int k = kw.kact[b];
+  const int K = kw.kmax;
+  const cx<double>* Hb = kw.H + (int64_t)b * (K + 1) * K;
+  const cx<double>* gb = kw.gv + (int64_t)b * (K + 1);
+  cx<double>* yb = kw.y + (int64_t)b * K;
+  for (int i = k - 1; i >= 0; --i) {
+    cx<double> s = gb[i];
+    for (int l = i + 1; l < k; ++l) s = s - Hb[i + (int64_t)l * (K + 1)] * yb[l];
+    yb[i] = s * cinv(Hb[i + (int64_t)i * (K + 1)]);
+  }
+}
+
+template <typename TZ>
+__global__ __launch_bounds__(256) void k_xupdate(KrylovWork kw, const cx<TZ>* __restrict__ Z,
+                                                 cx<double>* __restrict__ x) {
+  const int c = blockIdx.x, b = blockIdx.y;
+  const int k = kw.kact[b];
+  if (k == 0) return;
+  int64_t p0, p1;
+  chunk_range(kw.N, kw.nchunk, c, &p0, &p1);
+  const int64_t stride = kw.N * kw.nrhs;
+  const int64_t base = (int64_t)b * kw.N;
+  const cx<double>* yb = kw.y + (int64_t)b * kw.kmax;
+  for (int64_t p = p0 + threadIdx.x; p < p1; p += 256) {
+    cx<double> acc = x[base + p];
+    for (int i = 0; i < k; ++i) acc = acc + yb[i] * cvt<TZ, double>(Z[i * stride + base + p]);
+    x[base + p] = acc;
+  }
+}
+
+__global__ __launch_bounds__(64) void k_div_bnorm(KrylovWork kw, const double* __restrict__ n2,
+                                                  double* __restrict__ res) {
+  const int b = blockIdx.x * 64 + threadIdx.x;
+  if (b >= kw.nrhs) return;
+  const double bn = kw.bnorm[b];
+  res[b] = bn > 0.0 ? sqrt(n2[b]) / bn : 0.0;
+}
+
+// ------------------------------------------------------------------ host side
+static inline dim3 cgrid(const KrylovWork& w) { return dim3(w.nchunk, w.nrhs); }
+
+void kr_norm2(const KrylovWork& w, const cx<double>* v, double* out, const int* skip, hipStream_t s) {
+  hipLaunchKernelGGL(k_norm_part, cgrid(w), dim3(256), 0, s, w, v, skip);
+  hipLaunchKernelGGL(k_norm_reduce, dim3(w.nrhs), dim3(64), 0, s, w, out, skip);
+}
+
+// r: residual block, conv: per-RHS converged flags (device).  Uses w.nrm as scratch.
+void kr_start_impl(const KrylovWork& w, const cx<double>* r, cx<double>* V0, double tol, const int* conv,
+                   double* n2tmp, hipStream_t s) {
+  kr_norm2(w, r, n2tmp, conv, s);
+  hipLaunchKernelGGL(k_start_flags, dim3((w.nrhs + 63) / 64), dim3(64), 0, s, w, n2tmp, tol, conv);
+  hipLaunchKernelGGL(k_scale, cgrid(w), dim3(256), 0, s, w, r, V0);
+}
+
+template <int NV>
+static void arnoldi_nv(const KrylovWork& w, const cx<double>* V, cx<double>* wv, int j, double tol,
+                       int* skip_copy, hipStream_t s) {
+  const int64_t stride = w.N * w.nrhs;
+  hipLaunchKernelGGL((k_dots<NV>), cgrid(w), dim3(256), 0, s, w, V, wv, j);
+  hipLaunchKernelGGL(k_reduce_dots, dim3(j + 1, w.nrhs), dim3(64), 0, s, w, w.h1, j);
+  hipLaunchKernelGGL((k_update_dots<NV>), cgrid(w), dim3(256), 0, s, w, V, wv, j);
+  hipMemcpyAsync(skip_copy, w.done, sizeof(int) * w.nrhs, hipMemcpyDeviceToDevice, s);
+  hipLaunchKernelGGL(k_hess_givens, dim3(w.nrhs), dim3(64), 0, s, w, j, tol);
+  hipLaunchKernelGGL((k_final<NV>), cgrid(w), dim3(256), 0, s, w, V, wv, (cx<double>*)(V + (j + 1) * stride), j,
+                     skip_copy);
+}
+
+void kr_arnoldi_impl(const KrylovWork& w, const cx<double>* V, cx<double>* wv, int j, double tol, int* skip_copy,
+                     hipStream_t s) {
+  if (j < 8)
+    arnoldi_nv<8>(w, V, wv, j, tol, skip_copy, s);
+  else if (j < 16)
+    arnoldi_nv<16>(w, V, wv, j, tol, skip_copy, s);
+  else
+    arnoldi_nv<32>(w, V, wv, j, tol, skip_copy, s);
+}
+
+template <typename TZ>
+void kr_update_x(const KrylovWork& w, const cx<TZ>* Z, cx<double>* x, hipStream_t s) {
+  hipLaunchKernelGGL(k_backsolve, dim3((w.nrhs + 63) / 64), dim3(64), 0, s, w);
+  hipLaunchKernelGGL((k_xupdate<TZ>), cgrid(w), dim3(256), 0, s, w, Z, x);
+}
+template void kr_update_x<double>(const KrylovWork&, const cx<double>*, cx<double>*, hipStream_t);
+template void kr_update_x<float>(const KrylovWork&, const cx<float>*, cx<double>*, hipStream_t);
+
+void kr_true_res(const KrylovWork& w, const cx<double>* r, double* res, double* n2tmp, hipStream_t s) {
+  kr_norm2(w, r, n2tmp, nullptr, s);
+  hipLaunchKernelGGL(k_div_bnorm, dim3((w.nrhs + 63) / 64), dim3(64), 0, s, w, n2tmp, res);
+}
+
+}  // namespace hh

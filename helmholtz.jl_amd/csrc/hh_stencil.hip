@@ -1,0 +1,573 @@
+// Matrix-free Helmholtz stencil, damped Jacobi, grid transfers, casts and the
+// dense coarsest-level solve.  All HBM-bound streaming kernels.
+//
+// Operator (src/GetHelmholtz.jl:33-50, src/PlainNodalLaplacian.jl:18-46):
+//   (A x)_p = sum_d (2 x_p - x~_{p-e_d} - x~_{p+e_d}) / h_d^2 + d_p x_p
+// with mirror ghosts (BC=2) or ghost = self (BC=1) on every face, and the
+// per-node complex diagonal d_p = -w^2 m_p (1 - i g_p / Re w) + Sommerfeld
+// terms (+ i s w^2 m_p for a shifted operator).
+//
+// Launch shape: 64 x 4 threads.  threadIdx.x walks the fastest axis (one
+// coalesced 64-wide row segment per wavefront), each thread marches a run of
+// rows/planes along the slowest axis keeping the previous/current/next values
+// in registers, so x is read from HBM once; x-neighbours come from the same
+// cache lines.  blockIdx.z is the right-hand side.
+#include "hh_kernels.h"
+
+namespace hh {
+
+template <typename TD>
+__device__ __forceinline__ cx<TD> node_diag(const cx<TD>* __restrict__ d, const double* __restrict__ m,
+                                            cx<double> sfac, int64_t p) {
+  cx<TD> dv = d[p];
+  if (m) {
+    double mv = m[p];
+    dv.re += (TD)(sfac.re * mv);
+    dv.im += (TD)(sfac.im * mv);
+  }
+  return dv;
+}
+
+template <typename TI, typename TO, typename TD, int MODE>
+__device__ __forceinline__ void finish(cx<TO>* __restrict__ yo, const cx<TO>* __restrict__ bo, int64_t p,
+                                       cx<TD> xc, cx<TD> Ax, cx<TD> Dtot, TD wj) {
+  if (MODE == M_APPLY) {
+    yo[p] = cvt<TD, TO>(Ax);
+  } else if (MODE == M_RESID) {
+    cx<TD> bv = cvt<TO, TD>(bo[p]);
+    yo[p] = cvt<TD, TO>(bv - Ax);
+  } else {  // M_JACOBI
+    cx<TD> bv = cvt<TO, TD>(bo[p]);
+    cx<TD> r = bv - Ax;
+    cx<TD> up = r * cinv(Dtot);
+    yo[p] = cvt<TD, TO>(xc + wj * up);
+  }
+}
+
+template <typename TI, typename TO, typename TD, int MODE>
+__global__ __launch_bounds__(256) void k_stencil2d(Grid g, const cx<TI>* __restrict__ x,
+                                                   const cx<TO>* __restrict__ b, cx<TO>* __restrict__ y,
+                                                   const cx<TD>* __restrict__ d, const double* __restrict__ m,
+                                                   cx<double> sfac, TD wj, int rows, const int* __restrict__ skip) {
+  const int rhs = blockIdx.z;
+  if (skip && skip[rhs]) return;
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  const int j0 = (blockIdx.y * 4 + threadIdx.y) * rows;
+  const int n1 = g.n1, n2 = g.n2;
+  if (i >= n1 || j0 >= n2) return;
+  const int j1 = min(j0 + rows, n2);
+  const int64_t off = (int64_t)rhs * g.N;
+  const cx<TI>* xr = x + off;
+  const cx<TO>* bo = b ? b + off : nullptr;
+  cx<TO>* yo = y + off;
+  const int il = lo_nb(i, g.bc), ir = hi_nb(i, n1, g.bc);
+  const TD ih1 = (TD)g.ih1, ih2 = (TD)g.ih2;
+  const TD ldx = (TD)(g.ih1 * lap_diag1(i, n1, g.bc));
+  cx<TD> xm = cvt<TI, TD>(xr[(int64_t)lo_nb(j0, g.bc) * n1 + i]);
+  cx<TD> xc = cvt<TI, TD>(xr[(int64_t)j0 * n1 + i]);
+  for (int j = j0; j < j1; ++j) {
+    const int64_t row = (int64_t)j * n1;
+    const int jp = hi_nb(j, n2, g.bc);
+    cx<TD> xp = cvt<TI, TD>(xr[(int64_t)jp * n1 + i]);
+    cx<TD> xl = cvt<TI, TD>(xr[row + il]);
+    cx<TD> xrr = cvt<TI, TD>(xr[row + ir]);
+    const int64_t p = row + i;
+    cx<TD> dv = node_diag<TD>(d, m, sfac, p);
+    cx<TD> lap = ih1 * (((TD)2 * xc) - xl - xrr) + ih2 * (((TD)2 * xc) - xm - xp);
+    cx<TD> Ax = lap + dv * xc;
+    cx<TD> Dtot = dv;
+    if (MODE == M_JACOBI) {
+      Dtot.re += ldx + (TD)(g.ih2 * lap_diag1(j, n2, g.bc));
+    }
+    finish<TI, TO, TD, MODE>(yo, bo, p, xc, Ax, Dtot, wj);
+    xm = xc;
+    xc = xp;
+  }
+}
+
+template <typename TI, typename TO, typename TD, int MODE>
+__global__ __launch_bounds__(256) void k_stencil3d(Grid g, const cx<TI>* __restrict__ x,
+                                                   const cx<TO>* __restrict__ b, cx<TO>* __restrict__ y,
+                                                   const cx<TD>* __restrict__ d, const double* __restrict__ m,
+                                                   cx<double> sfac, TD wj, int rows, int ytiles,
+                                                   const int* __restrict__ skip) {
+  const int rhs = blockIdx.z;
+  if (skip && skip[rhs]) return;
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  const int yt = blockIdx.y % ytiles;
+  const int zc = blockIdx.y / ytiles;
+  const int j = yt * 4 + threadIdx.y;
+  const int k0 = zc * rows;
+  const int n1 = g.n1, n2 = g.n2, n3 = g.n3;
+  if (i >= n1 || j >= n2 || k0 >= n3) return;
+  const int k1 = min(k0 + rows, n3);
+  const int64_t plane = (int64_t)n1 * n2;
+  const int64_t off = (int64_t)rhs * g.N;
+  const cx<TI>* xr = x + off;
+  const cx<TO>* bo = b ? b + off : nullptr;
+  cx<TO>* yo = y + off;
+  const int il = lo_nb(i, g.bc), ir = hi_nb(i, n1, g.bc);
+  const int jl = lo_nb(j, g.bc), jr = hi_nb(j, n2, g.bc);
+  const TD ih1 = (TD)g.ih1, ih2 = (TD)g.ih2, ih3 = (TD)g.ih3;
+  const TD ldxy = (TD)(g.ih1 * lap_diag1(i, n1, g.bc) + g.ih2 * lap_diag1(j, n2, g.bc));
+  const int64_t col = (int64_t)j * n1 + i;
+  cx<TD> xm = cvt<TI, TD>(xr[(int64_t)lo_nb(k0, g.bc) * plane + col]);
+  cx<TD> xc = cvt<TI, TD>(xr[(int64_t)k0 * plane + col]);
+  for (int k = k0; k < k1; ++k) {
+    const int64_t pl = (int64_t)k * plane;
+    const int kp = hi_nb(k, n3, g.bc);
+    cx<TD> xp = cvt<TI, TD>(xr[(int64_t)kp * plane + col]);
+    cx<TD> x1l = cvt<TI, TD>(xr[pl + (int64_t)j * n1 + il]);
+    cx<TD> x1r = cvt<TI, TD>(xr[pl + (int64_t)j * n1 + ir]);
+    cx<TD> x2l = cvt<TI, TD>(xr[pl + (int64_t)jl * n1 + i]);
+    cx<TD> x2r = cvt<TI, TD>(xr[pl + (int64_t)jr * n1 + i]);
+    const int64_t p = pl + col;
+    cx<TD> dv = node_diag<TD>(d, m, sfac, p);
+    cx<TD> c2 = (TD)2 * xc;
+    cx<TD> lap = ih1 * (c2 - x1l - x1r) + ih2 * (c2 - x2l - x2r) + ih3 * (c2 - xm - xp);
+    cx<TD> Ax = lap + dv * xc;
+    cx<TD> Dtot = dv;
+    if (MODE == M_JACOBI) {
+      Dtot.re += ldxy + (TD)(g.ih3 * lap_diag1(k, n3, g.bc));
+    }
+    finish<TI, TO, TD, MODE>(yo, bo, p, xc, Ax, Dtot, wj);
+    xm = xc;
+    xc = xp;
+  }
+}
+
+// First Jacobi sweep from a zero guess: x = w D^-1 b (no stencil needed).
+template <typename T>
+__global__ __launch_bounds__(256) void k_jacobi0(Grid g, const cx<T>* __restrict__ b, cx<T>* __restrict__ x,
+                                                 const cx<T>* __restrict__ d, T wj, const int* __restrict__ skip) {
+  const int rhs = blockIdx.y;
+  if (skip && skip[rhs]) return;
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= g.N) return;
+  const int i = (int)(p % g.n1);
+  const int64_t q = p / g.n1;
+  const int j = (int)(q % g.n2);
+  const int k = (int)(q / g.n2);
+  double ld = g.ih1 * lap_diag1(i, g.n1, g.bc) + g.ih2 * lap_diag1(j, g.n2, g.bc);
+  if (g.dim == 3) ld += g.ih3 * lap_diag1(k, g.n3, g.bc);
+  cx<T> D = d[p];
+  D.re += (T)ld;
+  const int64_t o = (int64_t)rhs * g.N + p;
+  x[o] = wj * (b[o] * cinv(D));
+}
+
+// Full weighting with mirror ghosts, fine residual -> coarse right-hand side.
+template <typename T>
+__global__ __launch_bounds__(256) void k_restrict(Grid gf, Grid gc, const cx<T>* __restrict__ rf,
+                                                  cx<T>* __restrict__ bc, const int* __restrict__ skip) {
+  const int rhs = blockIdx.y;
+  if (skip && skip[rhs]) return;
+  const int64_t pc = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (pc >= gc.N) return;
+  const int I = (int)(pc % gc.n1);
+  const int64_t q = pc / gc.n1;
+  const int J = (int)(q % gc.n2);
+  const int K = (int)(q / gc.n2);
+  const cx<T>* r = rf + (int64_t)rhs * gf.N;
+  const T w[3] = {(T)0.25, (T)0.5, (T)0.25};
+  cx<T> acc = mk<T>(0, 0);
+  const int kk = gf.dim == 3 ? 3 : 1;
+  for (int c = 0; c < kk; ++c) {
+    int fk = 0;
+    T wk = (T)1;
+    if (gf.dim == 3) {
+      int f = 2 * K + c - 1;
+      fk = f < 0 ? 1 : (f >= gf.n3 ? gf.n3 - 2 : f);
+      wk = w[c];
+    }
+    for (int bb = 0; bb < 3; ++bb) {
+      int f2 = 2 * J + bb - 1;
+      int fj = f2 < 0 ? 1 : (f2 >= gf.n2 ? gf.n2 - 2 : f2);
+      for (int a = 0; a < 3; ++a) {
+        int f1 = 2 * I + a - 1;
+        int fi = f1 < 0 ? 1 : (f1 >= gf.n1 ? gf.n1 - 2 : f1);
+        acc = acc + (wk * w[bb] * w[a]) * r[((int64_t)fk * gf.n2 + fj) * gf.n1 + fi];
+      }
+    }
+  }
+  bc[(int64_t)rhs * gc.N + pc] = acc;
+}
+
+// Same full weighting on a real nodal coefficient field (m, gamma).
+__global__ __launch_bounds__(256) void k_restrict_real(Grid gf, Grid gc, const double* __restrict__ f,
+                                                       double* __restrict__ c) {
+  const int64_t pc = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (pc >= gc.N) return;
+  const int I = (int)(pc % gc.n1);
+  const int64_t q = pc / gc.n1;
+  const int J = (int)(q % gc.n2);
+  const int K = (int)(q / gc.n2);
+  const double w[3] = {0.25, 0.5, 0.25};
+  double acc = 0.0;
+  const int kk = gf.dim == 3 ? 3 : 1;
+  for (int cc = 0; cc < kk; ++cc) {
+    int fk = 0;
+    double wk = 1.0;
+    if (gf.dim == 3) {
+      int fz = 2 * K + cc - 1;
+      fk = fz < 0 ? 1 : (fz >= gf.n3 ? gf.n3 - 2 : fz);
+      wk = w[cc];
+    }
+    for (int bb = 0; bb < 3; ++bb) {
+      int f2 = 2 * J + bb - 1;
+      int fj = f2 < 0 ? 1 : (f2 >= gf.n2 ? gf.n2 - 2 : f2);
+      for (int a = 0; a < 3; ++a) {
+        int f1 = 2 * I + a - 1;
+        int fi = f1 < 0 ? 1 : (f1 >= gf.n1 ? gf.n1 - 2 : f1);
+        acc += wk * w[bb] * w[a] * f[((int64_t)fk * gf.n2 + fj) * gf.n1 + fi];
+      }
+    }
+  }
+  c[pc] = acc;
+}
+
+// Bi/tri-linear nodal prolongation of the coarse correction, added in place.
+template <typename T>
+__global__ __launch_bounds__(256) void k_prolong_add(Grid gf, Grid gc, const cx<T>* __restrict__ xc,
+                                                     cx<T>* __restrict__ xf, const int* __restrict__ skip) {
+  const int rhs = blockIdx.y;
+  if (skip && skip[rhs]) return;
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= gf.N) return;
+  const int i = (int)(p % gf.n1);
+  const int64_t q = p / gf.n1;
+  const int j = (int)(q % gf.n2);
+  const int k = (int)(q / gf.n2);
+  const cx<T>* c = xc + (int64_t)rhs * gc.N;
+  const int i0 = i >> 1, j0 = j >> 1, k0 = k >> 1;
+  const int i1 = (i & 1) ? i0 + 1 : i0, j1 = (j & 1) ? j0 + 1 : j0, k1 = (k & 1) ? k0 + 1 : k0;
+  const T wi = (i & 1) ? (T)0.5 : (T)1, wj = (j & 1) ? (T)0.5 : (T)1, wk = (k & 1) ? (T)0.5 : (T)1;
+  const int64_t P1 = gc.n1, P2 = (int64_t)gc.n1 * gc.n2;
+  cx<T> acc = mk<T>(0, 0);
+  const int ik[2] = {i0, i1}, jk[2] = {j0, j1}, kk[2] = {k0, k1};
+  const int ni = (i & 1) ? 2 : 1, nj = (j & 1) ? 2 : 1, nk = (k & 1) ? 2 : 1;
+  for (int c3 = 0; c3 < nk; ++c3)
+    for (int c2 = 0; c2 < nj; ++c2)
+      for (int c1 = 0; c1 < ni; ++c1) acc = acc + (wi * wj * wk) * c[kk[c3] * P2 + jk[c2] * P1 + ik[c1]];
+  const int64_t o = (int64_t)rhs * gf.N + p;
+  xf[o] = xf[o] + acc;
+}
+
+template <typename A, typename B>
+__global__ __launch_bounds__(256) void k_cast(const cx<A>* __restrict__ a, cx<B>* __restrict__ b, int64_t n) {
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p < n) b[p] = cvt<A, B>(a[p]);
+}
+
+// Nodal diagonal d = -w^2 m (1 - i g/Re w) + i Re(w) sqrt(m) sum_faces 2/h_d
+// (src/GetHelmholtz.jl:41-47, 222-247; Sommerfeld rows always use BC = 2).
+template <typename T>
+__global__ __launch_bounds__(256) void k_diag(Grid g, const double* __restrict__ m, const double* __restrict__ gam,
+                                              double wre, double wim, int neumann_top, int sommerfeld,
+                                              double h1, double h2, double h3, cx<T>* __restrict__ d) {
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= g.N) return;
+  const double mv = m[p], gv = gam[p];
+  const double w2r = wre * wre - wim * wim, w2i = 2.0 * wre * wim;
+  // -(w^2) * m * (1 - i g / wre)
+  const double ar = 1.0, ai = -gv / wre;
+  double dr = -(w2r * ar - w2i * ai) * mv;
+  double di = -(w2r * ai + w2i * ar) * mv;
+  if (sommerfeld) {
+    const int i = (int)(p % g.n1);
+    const int64_t q = p / g.n1;
+    const int j = (int)(q % g.n2);
+    const int k = (int)(q / g.n2);
+    double s = 0.0;
+    if (i == 0) s += 2.0 / h1;
+    if (i == g.n1 - 1) s += 2.0 / h1;
+    if (g.dim == 2) {
+      if (j == 0 && !neumann_top) s += 2.0 / h2;
+      if (j == g.n2 - 1) s += 2.0 / h2;
+    } else {
+      if (j == 0) s += 2.0 / h2;
+      if (j == g.n2 - 1) s += 2.0 / h2;
+      if (k == 0 && !neumann_top) s += 2.0 / h3;
+      if (k == g.n3 - 1) s += 2.0 / h3;
+    }
+    di += wre * s * sqrt(mv);
+  }
+  d[p] = mk<T>((T)dr, (T)di);
+}
+
+__global__ __launch_bounds__(256) void k_add_const(double* __restrict__ a, double c, int64_t n) {
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p < n) a[p] += c;
+}
+
+// ---- coarsest level: dense operator, Gauss-Jordan inverse, dense apply ----
+// Column-major n x 2n augmented matrix [A | I]; after elimination the right
+// half holds A^-1.
+__global__ __launch_bounds__(256) void k_dense_assemble(Grid g, const cx<double>* __restrict__ d,
+                                                        cx<double>* __restrict__ M) {
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t n = g.N;
+  if (p >= n) return;
+  const int i = (int)(p % g.n1);
+  const int64_t q = p / g.n1;
+  const int j = (int)(q % g.n2);
+  const int k = (int)(q / g.n2);
+  const int64_t P1 = g.n1, P2 = (int64_t)g.n1 * g.n2;
+  // row p of A: sum_d ih_d (2 x_p - x_lo - x_hi) + d_p x_p
+  double diag = 2.0 * (g.ih1 + g.ih2 + (g.dim == 3 ? g.ih3 : 0.0));
+  M[p + p * n] = mk<double>(diag + d[p].re, d[p].im);
+  int64_t nb[6];
+  double cf[6];
+  int cnt = 0;
+  nb[cnt] = (int64_t)k * P2 + j * P1 + lo_nb(i, g.bc); cf[cnt++] = g.ih1;
+  nb[cnt] = (int64_t)k * P2 + j * P1 + hi_nb(i, g.n1, g.bc); cf[cnt++] = g.ih1;
+  nb[cnt] = (int64_t)k * P2 + lo_nb(j, g.bc) * P1 + i; cf[cnt++] = g.ih2;
+  nb[cnt] = (int64_t)k * P2 + hi_nb(j, g.n2, g.bc) * P1 + i; cf[cnt++] = g.ih2;
+  if (g.dim == 3) {
+    nb[cnt] = (int64_t)lo_nb(k, g.bc) * P2 + j * P1 + i; cf[cnt++] = g.ih3;
+    nb[cnt] = (int64_t)hi_nb(k, g.n3, g.bc) * P2 + j * P1 + i; cf[cnt++] = g.ih3;
+  }
+  for (int c = 0; c < cnt; ++c) {
+    cx<double>& e = M[p + nb[c] * n];
+    e.re -= cf[c];
+  }
+  M[p + (n + p) * n] = mk<double>(1.0, 0.0);
+}
+
+// pivot search in column k (rows k..n-1): one block of 256 threads; stores
+// the pivot row and the reciprocal of the pivot value.
+__global__ __launch_bounds__(256) void k_gj_pivot(const cx<double>* __restrict__ M, int64_t n, int64_t k,
+                                                  int64_t* __restrict__ piv, cx<double>* __restrict__ pinv) {
+  __shared__ double sv[256];
+  __shared__ int64_t si[256];
+  double best = -1.0;
+  int64_t bi = k;
+  for (int64_t r = k + threadIdx.x; r < n; r += 256) {
+    double v = cabs2(M[r + k * n]);
+    if (v > best) { best = v; bi = r; }
+  }
+  sv[threadIdx.x] = best;
+  si[threadIdx.x] = bi;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      double o = sv[threadIdx.x + s];
+      int64_t oi = si[threadIdx.x + s];
+      if (o > sv[threadIdx.x] || (o == sv[threadIdx.x] && oi < si[threadIdx.x])) {
+        sv[threadIdx.x] = o;
+        si[threadIdx.x] = oi;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    piv[0] = si[0];
+    pinv[0] = cinv(M[si[0] + k * n]);
+  }
+}
+
+// elimination factors of column k as they stand after the row swap k <-> piv
+__global__ __launch_bounds__(256) void k_gj_fac(const cx<double>* __restrict__ M, int64_t n, int64_t k,
+                                                const int64_t* __restrict__ piv, cx<double>* __restrict__ fac) {
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (r >= n) return;
+  const int64_t pr = piv[0];
+  cx<double> f = (r == pr) ? M[k + k * n] : M[r + k * n];
+  if (r == k) f = mk<double>(0.0, 0.0);
+  fac[r] = f;
+}
+
+// swap rows k and piv and scale the new row k; one thread per column
+__global__ __launch_bounds__(256) void k_gj_swap_scale(cx<double>* __restrict__ M, int64_t n, int64_t k,
+                                                       const int64_t* __restrict__ piv,
+                                                       const cx<double>* __restrict__ pinv) {
+  const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (c >= 2 * n) return;
+  const int64_t pr = piv[0];
+  const cx<double> a = M[k + c * n];
+  const cx<double> b = M[pr + c * n];
+  if (pr != k) M[pr + c * n] = a;
+  M[k + c * n] = b * pinv[0];
+}
+
+__global__ __launch_bounds__(256) void k_gj_elim(cx<double>* __restrict__ M, int64_t n, int64_t k,
+                                                 const cx<double>* __restrict__ fac) {
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t c = (int64_t)blockIdx.y + k;  // columns k..2n-1
+  if (r >= n || c >= 2 * n || r == k) return;
+  cx<double> f = fac[r];
+  if (f.re == 0.0 && f.im == 0.0) return;
+  M[r + c * n] = M[r + c * n] - f * M[k + c * n];
+}
+
+// x_b = Ainv r_b in complex128 for every right-hand side.
+template <typename T>
+__global__ __launch_bounds__(256) void k_dense_apply(const cx<double>* __restrict__ Ainv, int64_t n,
+                                                     const cx<T>* __restrict__ r, cx<T>* __restrict__ x,
+                                                     const int* __restrict__ skip) {
+  const int rhs = blockIdx.y;
+  if (skip && skip[rhs]) return;
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  __shared__ cx<double> rs[256];
+  const cx<T>* rb = r + (int64_t)rhs * n;
+  cx<double> acc = mk<double>(0.0, 0.0);
+  for (int64_t j0 = 0; j0 < n; j0 += 256) {
+    const int64_t jj = j0 + threadIdx.x;
+    rs[threadIdx.x] = jj < n ? cvt<T, double>(rb[jj]) : mk<double>(0.0, 0.0);
+    __syncthreads();
+    const int64_t jm = min((int64_t)256, n - j0);
+    if (i < n)
+      for (int64_t t = 0; t < jm; ++t) acc = acc + Ainv[i + (j0 + t) * n] * rs[t];
+    __syncthreads();
+  }
+  if (i < n) x[(int64_t)rhs * n + i] = cvt<double, T>(acc);
+}
+
+// ---------------------------------------------------------------- launchers
+static inline dim3 stencil_grid(const Grid& g, int nrhs, int rows, int* ytiles) {
+  if (g.dim == 2) {
+    *ytiles = 1;
+    const int rowgroups = (g.n2 + rows - 1) / rows;
+    return dim3((g.n1 + 63) / 64, (rowgroups + 3) / 4, nrhs);
+  }
+  *ytiles = (g.n2 + 3) / 4;
+  const int zc = (g.n3 + rows - 1) / rows;
+  return dim3((g.n1 + 63) / 64, (*ytiles) * zc, nrhs);
+}
+
+static inline int pick_rows(const Grid& g, int nrhs) {
+  // enough independent wavefronts to fill 256 CUs, long runs otherwise
+  const int64_t cols = (int64_t)((g.n1 + 63) / 64) * (g.dim == 3 ? g.n2 : 4) * nrhs;
+  const int len = g.dim == 3 ? g.n3 : g.n2;
+  int rows = 16;
+  while (rows > 2 && cols * ((len + rows - 1) / rows) < 8192) rows >>= 1;
+  return rows;
+}
+
+template <typename TI, typename TO, typename TD, int MODE>
+void launch_stencil(const Grid& g, int nrhs, const cx<TI>* x, const cx<TO>* b, cx<TO>* y, const cx<TD>* d,
+                    const double* m, cx<double> sfac, double wj, const int* skip, hipStream_t s) {
+  int ytiles;
+  const int rows = pick_rows(g, nrhs);
+  dim3 grid = stencil_grid(g, nrhs, rows, &ytiles);
+  if (g.dim == 2)
+    hipLaunchKernelGGL((k_stencil2d<TI, TO, TD, MODE>), grid, dim3(64, 4), 0, s, g, x, b, y, d, m, sfac, (TD)wj,
+                       rows, skip);
+  else
+    hipLaunchKernelGGL((k_stencil3d<TI, TO, TD, MODE>), grid, dim3(64, 4), 0, s, g, x, b, y, d, m, sfac, (TD)wj,
+                       rows, ytiles, skip);
+}
+
+template void launch_stencil<double, double, double, M_APPLY>(const Grid&, int, const cx<double>*,
+                                                               const cx<double>*, cx<double>*,
+                                                               const cx<double>*, const double*, cx<double>,
+                                                               double, const int*, hipStream_t);
+template void launch_stencil<double, double, double, M_RESID>(const Grid&, int, const cx<double>*,
+                                                               const cx<double>*, cx<double>*,
+                                                               const cx<double>*, const double*, cx<double>,
+                                                               double, const int*, hipStream_t);
+template void launch_stencil<double, double, double, M_JACOBI>(const Grid&, int, const cx<double>*,
+                                                                const cx<double>*, cx<double>*,
+                                                                const cx<double>*, const double*, cx<double>,
+                                                                double, const int*, hipStream_t);
+template void launch_stencil<float, float, float, M_APPLY>(const Grid&, int, const cx<float>*, const cx<float>*,
+                                                            cx<float>*, const cx<float>*, const double*,
+                                                            cx<double>, double, const int*, hipStream_t);
+template void launch_stencil<float, float, float, M_RESID>(const Grid&, int, const cx<float>*, const cx<float>*,
+                                                            cx<float>*, const cx<float>*, const double*,
+                                                            cx<double>, double, const int*, hipStream_t);
+template void launch_stencil<float, float, float, M_JACOBI>(const Grid&, int, const cx<float>*, const cx<float>*,
+                                                             cx<float>*, const cx<float>*, const double*,
+                                                             cx<double>, double, const int*, hipStream_t);
+template void launch_stencil<float, double, double, M_APPLY>(const Grid&, int, const cx<float>*,
+                                                              const cx<double>*, cx<double>*,
+                                                              const cx<double>*, const double*, cx<double>,
+                                                              double, const int*, hipStream_t);
+
+template <typename T>
+void launch_jacobi0(const Grid& g, int nrhs, const cx<T>* b, cx<T>* x, const cx<T>* d, double wj, const int* skip,
+                    hipStream_t s) {
+  dim3 grid((unsigned)((g.N + 255) / 256), nrhs);
+  hipLaunchKernelGGL((k_jacobi0<T>), grid, dim3(256), 0, s, g, b, x, d, (T)wj, skip);
+}
+template void launch_jacobi0<double>(const Grid&, int, const cx<double>*, cx<double>*, const cx<double>*, double,
+                                     const int*, hipStream_t);
+template void launch_jacobi0<float>(const Grid&, int, const cx<float>*, cx<float>*, const cx<float>*, double,
+                                    const int*, hipStream_t);
+
+template <typename T>
+void launch_restrict(const Grid& gf, const Grid& gc, int nrhs, const cx<T>* rf, cx<T>* bc, const int* skip,
+                     hipStream_t s) {
+  dim3 grid((unsigned)((gc.N + 255) / 256), nrhs);
+  hipLaunchKernelGGL((k_restrict<T>), grid, dim3(256), 0, s, gf, gc, rf, bc, skip);
+}
+template void launch_restrict<double>(const Grid&, const Grid&, int, const cx<double>*, cx<double>*, const int*,
+                                      hipStream_t);
+template void launch_restrict<float>(const Grid&, const Grid&, int, const cx<float>*, cx<float>*, const int*,
+                                     hipStream_t);
+
+void launch_restrict_real(const Grid& gf, const Grid& gc, const double* f, double* c, hipStream_t s) {
+  hipLaunchKernelGGL(k_restrict_real, dim3((unsigned)((gc.N + 255) / 256)), dim3(256), 0, s, gf, gc, f, c);
+}
+
+template <typename T>
+void launch_prolong_add(const Grid& gf, const Grid& gc, int nrhs, const cx<T>* xc, cx<T>* xf, const int* skip,
+                        hipStream_t s) {
+  dim3 grid((unsigned)((gf.N + 255) / 256), nrhs);
+  hipLaunchKernelGGL((k_prolong_add<T>), grid, dim3(256), 0, s, gf, gc, xc, xf, skip);
+}
+template void launch_prolong_add<double>(const Grid&, const Grid&, int, const cx<double>*, cx<double>*,
+                                         const int*, hipStream_t);
+template void launch_prolong_add<float>(const Grid&, const Grid&, int, const cx<float>*, cx<float>*, const int*,
+                                        hipStream_t);
+
+template <typename A, typename B>
+void launch_cast(const cx<A>* a, cx<B>* b, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL((k_cast<A, B>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, b, n);
+}
+template void launch_cast<double, float>(const cx<double>*, cx<float>*, int64_t, hipStream_t);
+template void launch_cast<float, double>(const cx<float>*, cx<double>*, int64_t, hipStream_t);
+
+template <typename T>
+void launch_diag(const Grid& g, const double* m, const double* gam, double wre, double wim, int neumann_top,
+                 int sommerfeld, const double h[3], cx<T>* d, hipStream_t s) {
+  hipLaunchKernelGGL((k_diag<T>), dim3((unsigned)((g.N + 255) / 256)), dim3(256), 0, s, g, m, gam, wre, wim,
+                     neumann_top, sommerfeld, h[0], h[1], h[2], d);
+}
+template void launch_diag<double>(const Grid&, const double*, const double*, double, double, int, int,
+                                  const double*, cx<double>*, hipStream_t);
+template void launch_diag<float>(const Grid&, const double*, const double*, double, double, int, int,
+                                 const double*, cx<float>*, hipStream_t);
+
+void launch_add_const(double* a, double c, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_add_const, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, c, n);
+}
+
+void launch_dense_inverse(const Grid& g, const cx<double>* d, cx<double>* M, int64_t* piv, cx<double>* fac,
+                          cx<double>* pinv, hipStream_t s) {
+  const int64_t n = g.N;
+  hipMemsetAsync(M, 0, sizeof(cx<double>) * n * 2 * n, s);
+  hipLaunchKernelGGL(k_dense_assemble, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g, d, M);
+  for (int64_t k = 0; k < n; ++k) {
+    hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(256), 0, s, M, n, k, piv, pinv);
+    hipLaunchKernelGGL(k_gj_fac, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, M, n, k, piv, fac);
+    hipLaunchKernelGGL(k_gj_swap_scale, dim3((unsigned)((2 * n + 255) / 256)), dim3(256), 0, s, M, n, k, piv,
+                       pinv);
+    hipLaunchKernelGGL(k_gj_elim, dim3((unsigned)((n + 255) / 256), (unsigned)(2 * n - k)), dim3(256), 0, s, M, n,
+                       k, fac);
+  }
+}
+
+template <typename T>
+void launch_dense_apply(const cx<double>* Ainv, int64_t n, int nrhs, const cx<T>* r, cx<T>* x, const int* skip,
+                        hipStream_t s) {
+  dim3 grid((unsigned)((n + 255) / 256), nrhs);
+  hipLaunchKernelGGL((k_dense_apply<T>), grid, dim3(256), 0, s, Ainv, n, r, x, skip);
+}
+template void launch_dense_apply<double>(const cx<double>*, int64_t, int, const cx<double>*, cx<double>*,
+                                         const int*, hipStream_t);
+template void launch_dense_apply<float>(const cx<double>*, int64_t, int, const cx<float>*, cx<float>*,
+                                        const int*, hipStream_t);
+
+}  // namespace hh

@@ -1,0 +1,84 @@
+"""ctypes binding of libhelmholtz_amd.so (declarations in include/helmholtz_amd.h).
+
+Loading fails loudly: there is no CPU fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libhelmholtz_amd.so")
+
+HH_OK, HH_ERR_ARG, HH_ERR_HIP, HH_ERR_NOMEM, HH_ERR_UNSUPPORTED = 0, 1, 2, 3, 4
+HH_C128, HH_C64 = 0, 1
+HH_HOST, HH_DEVICE = 0, 1
+HH_FGMRES, HH_BICGSTAB = 0, 1
+HH_CONVERGED, HH_MAXITER, HH_NONFINITE, HH_ZERO_RHS = 0, 1, 2, 3
+
+# every symbol include/helmholtz_amd.h declares
+EXPORTS = (
+    "hh_last_error", "hh_version", "hh_abl", "hh_op_create", "hh_op_destroy", "hh_op_apply",
+    "hh_solver_create", "hh_solver_copy", "hh_solver_clear", "hh_solver_destroy", "hh_solve",
+    "hh_solver_levels",
+)
+
+
+class HelmholtzError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[helmholtz_amd error {code}] {msg}")
+        self.code = code
+
+
+class MGParams(C.Structure):
+    _fields_ = [
+        ("levels", C.c_int32), ("pre", C.c_int32), ("post", C.c_int32), ("relax", C.c_double),
+        ("cycle", C.c_int32), ("max_outer", C.c_int32), ("tol", C.c_double), ("inner", C.c_int32),
+        ("krylov", C.c_int32), ("prec", C.c_int32), ("learned", C.c_int32), ("verbose", C.c_int32),
+    ]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [
+        ("iters", C.POINTER(C.c_int32)), ("status", C.POINTER(C.c_int32)),
+        ("true_res", C.POINTER(C.c_double)), ("max_iters", C.c_int32), ("n_converged", C.c_int32),
+        ("setup_time", C.c_double), ("solve_time", C.c_double), ("n_prec", C.c_int64),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Load the library once; raise if it is missing (build it with build.py)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise HelmholtzError(-1, f"{LIB_PATH} not built; run helmholtz.jl_amd/build.py")
+    L = C.CDLL(LIB_PATH)
+    vp, i32, i64, dbl = C.c_void_p, C.c_int, C.c_int64, C.c_double
+    p64 = C.POINTER(C.c_int64)
+    pd = C.POINTER(C.c_double)
+    L.hh_last_error.restype = C.c_char_p
+    L.hh_last_error.argtypes = []
+    L.hh_version.restype = i32
+    L.hh_abl.argtypes = [i32, p64, i32, p64, dbl, vp, i32]
+    L.hh_op_create.argtypes = [i32, p64, pd, vp, vp, dbl, dbl, i32, i32, i32, C.POINTER(vp)]
+    L.hh_op_destroy.argtypes = [vp]
+    L.hh_op_apply.argtypes = [vp, dbl, vp, vp, i64, i32, i32]
+    L.hh_solver_create.argtypes = [vp, C.POINTER(MGParams), pd, C.POINTER(vp)]
+    L.hh_solver_copy.argtypes = [vp, C.POINTER(vp)]
+    L.hh_solver_clear.argtypes = [vp]
+    L.hh_solver_destroy.argtypes = [vp]
+    L.hh_solve.argtypes = [vp, vp, vp, i64, i32, i32, C.POINTER(SolveStats)]
+    L.hh_solver_levels.argtypes = [vp, C.POINTER(C.c_int), vp, i32]
+    for name in EXPORTS:
+        getattr(L, name).restype = getattr(L, name).restype if name in ("hh_last_error",) else C.c_int
+    _lib = L
+    return L
+
+
+def check(code: int):
+    if code != HH_OK:
+        raise HelmholtzError(code, lib().hh_last_error().decode())

@@ -1,0 +1,283 @@
+"""Operator side of the reference API, backed by libhelmholtz_amd on the GPU.
+
+Mirrors Helmholtz.jl's exports for this path: ``HelmholtzParam``,
+``getShiftedHelmholtzParam`` (src/Helmholtz.jl:13-34), ``GetHelmholtzOperator``
+(3 methods, src/GetHelmholtz.jl:14-50), ``GetHelmholtzShiftOP`` (:81-83),
+``getABL`` (:97-220), ``getMaximalFrequency`` (:75-79),
+``getAcousticPointSource``/``getTopPointSrc``/``getMidPointSrc``/``loc2cs``
+(src/getPointSource.jl:63-111) and jInv's ``getRegularMesh``.
+
+Differences from the reference, by design: the operator is never assembled.
+``GetHelmholtzOperator`` returns a matrix-free :class:`HelmholtzOperator`
+whose ``@`` runs the HIP stencil; ``H + GetHelmholtzShiftOP(...)`` gives the
+shifted operator.  Subscripts stay 1-based as in the reference; arrays are
+column-major (``order='F'``), axis 0 = x, the last axis = depth.
+
+Host vectors are numpy arrays (N,) or (N, nrhs) in Fortran order; device
+blocks are torch CUDA tensors shaped (nrhs, N) (row-major, i.e. the same bytes
+as a column-major N x nrhs matrix) or (N,).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib as L
+
+
+# ------------------------------------------------------------------ mesh
+@dataclass
+class RegularMesh:
+    """jInv.Mesh.RegularMesh: ``n`` cells per axis on ``domain`` (lo, hi pairs)."""
+    domain: tuple
+    n: np.ndarray
+    h: np.ndarray = field(init=False)
+    dim: int = field(init=False)
+
+    def __post_init__(self):
+        self.n = np.asarray(self.n, dtype=np.int64).ravel()
+        dom = np.asarray(self.domain, dtype=np.float64).ravel()
+        self.domain = tuple(dom)
+        self.dim = len(self.n)
+        self.h = np.array([(dom[2 * d + 1] - dom[2 * d]) / self.n[d] for d in range(self.dim)])
+
+    @property
+    def nodes(self) -> np.ndarray:
+        return self.n + 1
+
+
+def getRegularMesh(domain, n) -> RegularMesh:
+    return RegularMesh(tuple(np.asarray(domain, dtype=float).ravel()), np.asarray(n))
+
+
+# ------------------------------------------------------------------ parameters
+@dataclass
+class HelmholtzParam:
+    """src/Helmholtz.jl:13-20."""
+    Mesh: RegularMesh
+    gamma: np.ndarray
+    m: np.ndarray
+    omega: complex
+    NeumannOnTop: bool
+    Sommerfeld: bool
+
+
+def getShiftedHelmholtzParam(p: HelmholtzParam, s: float) -> HelmholtzParam:
+    """src/Helmholtz.jl:32-34: gamma + s * Re(omega)."""
+    return HelmholtzParam(p.Mesh, np.asarray(p.gamma, dtype=float) + s * complex(p.omega).real, p.m, p.omega,
+                          p.NeumannOnTop, p.Sommerfeld)
+
+
+def clear(p):
+    """``clear!(HelmholtzParam)`` (src/Helmholtz.jl:25-30) is effectively a no-op
+    on the record (its assignments are to locals); provided for API parity."""
+    return None
+
+
+# ------------------------------------------------------------------ helpers
+def _i64x3(v, fill=1):
+    a = (C.c_int64 * 3)(fill, fill, fill)
+    for d, x in enumerate(v):
+        a[d] = int(x)
+    return a
+
+
+def _f64x3(v, fill=1.0):
+    a = (C.c_double * 3)(fill, fill, fill)
+    for d, x in enumerate(v):
+        a[d] = float(x)
+    return a
+
+
+def _is_torch(x) -> bool:
+    return type(x).__module__.startswith("torch")
+
+
+def getABL(n, NeumannAtFirstDim: bool, ABLpad, ABLamp: float):
+    """src/GetHelmholtz.jl:97-220 computed on the GPU (``hh_abl``).  ``n`` are node
+    counts; returns a float64 array shaped ``tuple(n)`` in Fortran order."""
+    n = [int(v) for v in np.asarray(n).ravel()]
+    pad = [int(v) for v in np.asarray(ABLpad).ravel()]
+    out = np.zeros(int(np.prod(n)), dtype=np.float64)
+    L.check(L.lib().hh_abl(len(n), _i64x3(n), int(bool(NeumannAtFirstDim)), _i64x3(pad), float(ABLamp),
+                           out.ctypes.data, L.HH_HOST))
+    return out.reshape(tuple(n), order="F")
+
+
+def getMaximalFrequency(m, M: RegularMesh) -> float:
+    """src/GetHelmholtz.jl:75-79 (slowness squared in): 10 points per wavelength."""
+    return (0.1 * 2 * math.pi) / (float(np.max(M.h)) * math.sqrt(float(np.max(m))))
+
+
+@dataclass
+class ShiftOp:
+    """GetHelmholtzShiftOP(m, omega, shift) = diag(i shift omega^2 m) (src/GetHelmholtz.jl:81-83)."""
+    m: np.ndarray
+    omega: float
+    shift: float
+
+
+def GetHelmholtzShiftOP(mNodal, omega: float, shift: float) -> ShiftOp:
+    return ShiftOp(np.asarray(mNodal, dtype=float), float(omega), float(shift))
+
+
+# ------------------------------------------------------------------ operator
+class _OpHandle:
+    """Owns one ``hh_op``; shared between the shifted views of an operator."""
+
+    def __init__(self, mesh: RegularMesh, m, omega, gamma, neumann: bool, sommerfeld: bool, order: int):
+        nn = [int(v) for v in mesh.nodes]
+        N = int(np.prod(nn))
+        self.m = np.ascontiguousarray(np.asarray(m, dtype=np.float64).ravel(order="F"))
+        self.gamma = np.ascontiguousarray(np.asarray(gamma, dtype=np.float64).ravel(order="F"))
+        if self.m.size != N or self.gamma.size != N:
+            raise ValueError(f"m and gamma must have {N} nodal values")
+        w = complex(omega)
+        self.N, self.nn, self.dim = N, nn, mesh.dim
+        self.omega = w
+        h = C.c_void_p()
+        L.check(L.lib().hh_op_create(mesh.dim, _i64x3(nn), _f64x3(mesh.h), self.m.ctypes.data,
+                                     self.gamma.ctypes.data, w.real, w.imag, int(bool(neumann)),
+                                     int(bool(sommerfeld)), int(order), C.byref(h)))
+        self.ptr = h
+
+    def __del__(self):
+        p = getattr(self, "ptr", None)
+        if p:
+            try:
+                L.lib().hh_op_destroy(p)
+            except Exception:
+                pass
+            self.ptr = None
+
+
+class HelmholtzOperator:
+    """Matrix-free H (+ i s w^2 diag(m)).  ``H @ X`` runs ``hh_op_apply``."""
+
+    def __init__(self, handle: _OpHandle, shift: float = 0.0, mesh=None, param=None):
+        self._h = handle
+        self.shift = float(shift)
+        self.Mesh = mesh
+        self.param = param
+
+    @property
+    def shape(self):
+        return (self._h.N, self._h.N)
+
+    def __add__(self, other):
+        if isinstance(other, ShiftOp):
+            if not np.allclose(np.ravel(other.m, order="F"), self._h.m):
+                raise ValueError("shift operator built on a different slowness field")
+            if complex(other.omega) != complex(self._h.omega.real) or self._h.omega.imag != 0.0:
+                raise ValueError("GetHelmholtzShiftOP needs the operator's real omega")
+            return HelmholtzOperator(self._h, self.shift + other.shift, self.Mesh, self.param)
+        return NotImplemented
+
+    __radd__ = __add__
+
+    def apply(self, X, prec: str = "c128", out=None):
+        """Y = A X.  X: numpy (N,)/(N, nrhs) (Fortran order) or torch CUDA (N,)/(nrhs, N)."""
+        code = L.HH_C128 if prec == "c128" else L.HH_C64
+        if _is_torch(X):
+            import torch
+            want = torch.complex128 if code == L.HH_C128 else torch.complex64
+            if not X.is_cuda or X.dtype != want or not X.is_contiguous():
+                raise ValueError(f"device input must be a contiguous CUDA {want} tensor")
+            if X.numel() % self._h.N:
+                raise ValueError("size is not a multiple of the node count")
+            nrhs = X.numel() // self._h.N
+            Y = torch.empty_like(X) if out is None else out
+            torch.cuda.synchronize()
+            L.check(L.lib().hh_op_apply(self._h.ptr, self.shift, X.data_ptr(), Y.data_ptr(), nrhs, code,
+                                        L.HH_DEVICE))
+            return Y
+        dt = np.complex128 if code == L.HH_C128 else np.complex64
+        Xa = np.asarray(X)
+        vec = Xa.ndim == 1
+        X2 = np.asfortranarray(Xa.reshape(self._h.N, -1, order="F").astype(dt, copy=False))
+        Y = np.empty_like(X2, order="F")
+        L.check(L.lib().hh_op_apply(self._h.ptr, self.shift, X2.ctypes.data, Y.ctypes.data, X2.shape[1], code,
+                                    L.HH_HOST))
+        return Y[:, 0] if vec else Y
+
+    def __matmul__(self, X):
+        return self.apply(X)
+
+    def __mul__(self, X):
+        return self.apply(X)
+
+
+def _build(mesh, m, omega, gamma, neumann, sommerfeld, order, param=None):
+    return HelmholtzOperator(_OpHandle(mesh, m, omega, gamma, neumann, sommerfeld, order), 0.0, mesh, param)
+
+
+def GetHelmholtzOperator(*args, orderNeumannBC: int = 2):
+    """The three reference methods (src/GetHelmholtz.jl:14-16, 22-31, 33-50):
+
+    * ``GetHelmholtzOperator(Hparam[, order])`` -> H
+    * ``GetHelmholtzOperator(Msh, m, omega, gamma, NeumannAtFirstDim, Sommerfeld[, order])`` -> H
+    * ``GetHelmholtzOperator(Msh, m, omega, gamma, NeumannAtFirstDim, ABLpad, ABLamp, Sommerfeld[, order])``
+      -> (H, gamma + ABL)   (gamma may be empty: then gamma = ABL)
+    """
+    if isinstance(args[0], HelmholtzParam):
+        p = args[0]
+        order = args[1] if len(args) > 1 else orderNeumannBC
+        return _build(p.Mesh, p.m, p.omega, p.gamma, p.NeumannOnTop, p.Sommerfeld, order, p)
+    mesh, m, omega, gamma, neumann = args[:5]
+    if isinstance(args[5], (bool, np.bool_)):
+        order = args[6] if len(args) > 6 else orderNeumannBC
+        return _build(mesh, m, omega, gamma, neumann, args[5], order)
+    pad, amp, sommerfeld = args[5], args[6], args[7]
+    order = args[8] if len(args) > 8 else orderNeumannBC
+    abl = getABL(mesh.nodes, neumann, pad, float(amp))
+    if gamma is None or np.size(gamma) == 0:
+        g = abl
+    else:
+        g = np.asarray(gamma, dtype=float).reshape(abl.shape, order="F") + abl
+    return _build(mesh, m, omega, g, neumann, sommerfeld, order), g
+
+
+# ------------------------------------------------------------------ sources
+def getTopPointSrc(M: RegularMesh):
+    """src/getPointSource.jl:63-70 (1-based)."""
+    if M.dim == 3:
+        return [int(M.n[0] + 1) // 2, int(M.n[1] + 1) // 2, 1]
+    return [int(M.n[0] + 1) // 2, 1]
+
+
+def getMidPointSrc(M: RegularMesh):
+    """src/getPointSource.jl:72-79 (1-based)."""
+    return [int(v + 1) // 2 for v in M.n]
+
+
+def loc2cs(n, sub) -> int:
+    """src/getPointSource.jl:82-100: 1-based column-major linear index."""
+    n = [int(v) for v in n]
+    if len(sub) == 2:
+        return int(sub[0] + (sub[1] - 1) * n[0])
+    return int(sub[0] + (sub[1] - 1) * n[0] + (sub[2] - 1) * n[0] * n[1])
+
+
+def getAcousticPointSource(M: RegularMesh, TYPE=np.complex128, src=None):
+    """src/getPointSource.jl:105-111: q = e_src / ||h||^2, returns (q, src)."""
+    if src is None:
+        src = getTopPointSrc(M)
+    nn = [int(v) for v in M.nodes]
+    flat = np.zeros(int(np.prod(nn)), dtype=TYPE)
+    flat[loc2cs(nn, src) - 1] = 1.0 / float(np.linalg.norm(M.h)) ** 2
+    return flat.reshape(tuple(nn), order="F"), src
+
+
+def point_sources(M: RegularMesh, subs, TYPE=np.complex128, scale=None):
+    """Block of point-source right-hand sides, one column per 1-based subscript
+    (the batched form of getAcousticPointSource); N x len(subs), Fortran order."""
+    nn = [int(v) for v in M.nodes]
+    N = int(np.prod(nn))
+    B = np.zeros((N, len(subs)), dtype=TYPE, order="F")
+    val = 1.0 / float(np.linalg.norm(M.h)) ** 2 if scale is None else scale
+    for c, s in enumerate(subs):
+        B[loc2cs(nn, s) - 1, c] = val
+    return B

@@ -1,0 +1,219 @@
+"""Solver side of the reference API, backed by libhelmholtz_amd on the GPU.
+
+Mirrors ``ShiftedLaplacianMultigridSolver`` (src/ShiftedLaplacianMultigridSolver.jl:4-15),
+``getShiftedLaplacianMultigridSolver`` (:24-30), ``copySolver`` (:18-22),
+``solveLinearSystem`` (:33-102) and ``clear!`` (:105-109), plus the subset
+of Multigrid.jl's ``getMGparam`` the reference's tests use
+(test/ShiftedLaplacianTest.jl:51-64).
+
+The multigrid hierarchy is the rediscretised shifted operator on every level
+(the reference's commented geometric path, :51-62), with full-weighting
+restriction, bilinear/trilinear prolongation, damped Jacobi and an FP64
+direct coarsest solve; the outer method is flexible GMRES(inner).  The
+``ShiftedHT`` argument of ``solveLinearSystem`` is accepted for signature
+compatibility and only checked for shape: the GPU path is matrix-free from
+``helmParam``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib as L
+from .helmholtz import HelmholtzOperator, HelmholtzParam, _OpHandle, _is_torch
+
+
+@dataclass
+class MGparam:
+    """Settings of Multigrid.jl's MGparam that this path honours.
+
+    ``prec`` ("c128" | "c64") selects the preconditioner precision (an
+    extension: the reference runs the type of the matrix)."""
+    levels: int = 3
+    numCores: int = 1
+    maxOuterIter: int = 50
+    relativeTol: float = 1e-6
+    relaxType: str = "Jac"
+    relaxParam: float = 0.8
+    relaxPre: int = 2
+    relaxPost: int = 2
+    cycleType: str = "V"
+    coarseSolveType: str = "NoMUMPS"
+    extra: tuple = field(default_factory=tuple)
+    prec: str = "c128"
+    verbose: bool = False
+    _hierarchy_exists: bool = False
+
+
+def getMGparam(*args, **kw) -> MGparam:
+    """``getMGparam([T, Ti,] levels, numCores, maxIter, relativeTol, relaxType,
+    relaxParam, relaxPre, relaxPost, cycleType, coarseSolveType, ...)``."""
+    a = list(args)
+    while a and isinstance(a[0], type):
+        a.pop(0)
+    names = ["levels", "numCores", "maxOuterIter", "relativeTol", "relaxType", "relaxParam", "relaxPre",
+             "relaxPost", "cycleType", "coarseSolveType"]
+    vals = dict(zip(names, a))
+    vals.update(kw)
+    mg = MGparam(**{k: v for k, v in vals.items() if k in MGparam.__dataclass_fields__})
+    mg.extra = tuple(a[len(names):])
+    return mg
+
+
+@dataclass
+class ShiftedLaplacianMultigridSolver:
+    helmParam: HelmholtzParam
+    MG: MGparam
+    shift: np.ndarray
+    Krylov: str = "BiCGSTAB"
+    inner: int = 5
+    doClear: int = 0
+    verbose: bool = False
+    setupTime: float = 0.0
+    nPrec: int = 0
+    solveTime: float = 0.0
+    last_stats: dict = field(default_factory=dict)
+    _op: object = None
+    _op_key: object = None
+    _handle: object = None
+
+    def __del__(self):
+        _drop_handle(self)
+
+
+def _drop_handle(s):
+    h = getattr(s, "_handle", None)
+    if h:
+        try:
+            L.lib().hh_solver_destroy(h)
+        except Exception:
+            pass
+    s._handle = None
+
+
+def getShiftedLaplacianMultigridSolver(helmParam: HelmholtzParam, MG: MGparam, shift, Krylov: str = "BiCGSTAB",
+                                       inner: int = 5, verbose: bool = False) -> ShiftedLaplacianMultigridSolver:
+    """src/ShiftedLaplacianMultigridSolver.jl:24-30 (a scalar shift is broadcast to every level)."""
+    sh = np.asarray(shift, dtype=float).ravel()
+    if sh.size == 1:
+        sh = np.full(MG.levels, float(sh[0]))
+    return ShiftedLaplacianMultigridSolver(helmParam, MG, sh, Krylov, int(inner), 0, bool(verbose))
+
+
+def copySolver(s: ShiftedLaplacianMultigridSolver) -> ShiftedLaplacianMultigridSolver:
+    """src/ShiftedLaplacianMultigridSolver.jl:18-22: same settings, fresh hierarchy."""
+    mg = MGparam(**{k: getattr(s.MG, k) for k in MGparam.__dataclass_fields__})
+    mg._hierarchy_exists = False
+    return getShiftedLaplacianMultigridSolver(s.helmParam, mg, s.shift.copy(), s.Krylov, s.inner, s.verbose)
+
+
+def clear(s: ShiftedLaplacianMultigridSolver):
+    """``clear!`` (src/ShiftedLaplacianMultigridSolver.jl:105-109): drop the hierarchy."""
+    if s._handle:
+        L.check(L.lib().hh_solver_clear(s._handle))
+    s.MG._hierarchy_exists = False
+    s.doClear = 0
+
+
+def _validate(s: ShiftedLaplacianMultigridSolver):
+    mg = s.MG
+    if s.Krylov != "GMRES":
+        raise NotImplementedError(f"Krylov={s.Krylov!r}: this path implements the FGMRES outer method ('GMRES')")
+    if mg.relaxType != "Jac":
+        raise NotImplementedError(f"relaxType={mg.relaxType!r}: damped Jacobi ('Jac') only")
+    if str(mg.cycleType) not in ("V", "W"):
+        raise NotImplementedError(f"cycleType={mg.cycleType!r}: 'V' and 'W' only")
+    if callable(mg.relaxPre) or callable(mg.relaxPost):
+        raise NotImplementedError("per-level sweep counts (functions) are not supported")
+    if mg.prec not in ("c128", "c64"):
+        raise ValueError("MG.prec must be 'c128' or 'c64'")
+
+
+def _ensure_handle(s: ShiftedLaplacianMultigridSolver):
+    p = s.helmParam
+    key = (id(p), id(p.m), id(p.gamma), complex(p.omega), bool(p.NeumannOnTop), bool(p.Sommerfeld))
+    if s._op is None or s._op_key != key:
+        _drop_handle(s)
+        s._op = _OpHandle(p.Mesh, p.m, p.omega, p.gamma, p.NeumannOnTop, p.Sommerfeld, 2)
+        s._op_key = key
+    if s._handle is None:
+        mg = s.MG
+        prm = L.MGParams(levels=int(mg.levels), pre=int(mg.relaxPre), post=int(mg.relaxPost),
+                         relax=float(mg.relaxParam), cycle=ord(str(mg.cycleType)), max_outer=int(mg.maxOuterIter),
+                         tol=float(mg.relativeTol), inner=int(s.inner), krylov=L.HH_FGMRES,
+                         prec=L.HH_C64 if mg.prec == "c64" else L.HH_C128, learned=0,
+                         verbose=int(bool(s.verbose)))
+        shifts = np.ascontiguousarray(np.broadcast_to(np.asarray(s.shift, dtype=float), (int(mg.levels),)))
+        h = C.c_void_p()
+        L.check(L.lib().hh_solver_create(s._op.ptr, C.byref(prm), shifts.ctypes.data_as(C.POINTER(C.c_double)),
+                                         C.byref(h)))
+        s._handle = h
+
+
+_STATUS = {L.HH_CONVERGED: "converged", L.HH_MAXITER: "maxiter", L.HH_NONFINITE: "nonfinite",
+           L.HH_ZERO_RHS: "zero_rhs"}
+
+
+def solveLinearSystem(ShiftedHT, B, param: ShiftedLaplacianMultigridSolver, doTranspose: int = 0):
+    """src/ShiftedLaplacianMultigridSolver.jl:33-102 -> (X, param).
+
+    B: numpy (N,) or (N, nrhs) complex, or a torch CUDA complex128 tensor
+    (N,) / (nrhs, N) (then X is a CUDA tensor too and nothing crosses PCIe)."""
+    _validate(param)
+    if doTranspose != 0:
+        raise NotImplementedError("doTranspose=1 (adjoint solve) is not implemented")
+    if param.doClear == 1:
+        clear(param)
+    N = int(np.prod(param.helmParam.Mesh.nodes))
+    if ShiftedHT is not None and hasattr(ShiftedHT, "shape") and tuple(ShiftedHT.shape) != (N, N):
+        raise ValueError("ShiftedHT has the wrong shape for helmParam.Mesh")
+    dev = _is_torch(B)
+    if dev:
+        import torch
+        if not B.is_cuda or B.dtype != torch.complex128 or not B.is_contiguous():
+            raise ValueError("device B must be a contiguous CUDA complex128 tensor")
+        nrhs = B.numel() // N
+        if nrhs * N != B.numel():
+            raise ValueError("B size is not a multiple of the node count")
+        if float(torch.linalg.vector_norm(B)) == 0.0:
+            return torch.zeros_like(B), param
+        X = torch.empty_like(B)
+        bptr, xptr, loc = B.data_ptr(), X.data_ptr(), L.HH_DEVICE
+        torch.cuda.synchronize()
+    else:
+        Ba = np.asarray(B)
+        vec = Ba.ndim == 1
+        B2 = np.asfortranarray(Ba.reshape(N, -1, order="F").astype(np.complex128, copy=False))
+        nrhs = B2.shape[1]
+        if np.linalg.norm(B2) == 0.0:
+            return np.zeros(Ba.shape, dtype=Ba.dtype), param
+        X = np.empty_like(B2, order="F")
+        bptr, xptr, loc = B2.ctypes.data, X.ctypes.data, L.HH_HOST
+    _ensure_handle(param)
+    iters = np.zeros(nrhs, dtype=np.int32)
+    status = np.zeros(nrhs, dtype=np.int32)
+    res = np.zeros(nrhs, dtype=np.float64)
+    st = L.SolveStats(iters.ctypes.data_as(C.POINTER(C.c_int32)), status.ctypes.data_as(C.POINTER(C.c_int32)),
+                      res.ctypes.data_as(C.POINTER(C.c_double)), 0, 0, 0.0, 0.0, 0)
+    L.check(L.lib().hh_solve(param._handle, bptr, xptr, nrhs, 0, loc, C.byref(st)))
+    param.MG._hierarchy_exists = True
+    param.setupTime += st.setup_time
+    param.solveTime += st.solve_time
+    param.nPrec += int(st.n_prec)
+    param.last_stats = {"iters": iters, "status": [_STATUS[int(v)] for v in status], "true_res": res,
+                        "max_iters": int(st.max_iters), "setup_time": st.setup_time, "solve_time": st.solve_time}
+    if dev:
+        return X, param
+    return (X[:, 0] if vec else X), param
+
+
+def hierarchy_levels(s: ShiftedLaplacianMultigridSolver):
+    """Node counts of the solver's grid levels (empty before the first solve)."""
+    if s._handle is None:
+        return []
+    n = C.c_int()
+    arr = (C.c_int64 * (16 * 3))()
+    L.check(L.lib().hh_solver_levels(s._handle, C.byref(n), arr, 16))
+    return [tuple(arr[3 * l: 3 * l + 3]) for l in range(n.value)]

@@ -1,0 +1,66 @@
+"""C-ABI checks without a GPU: the library loads and exports every symbol the
+public header declares; errors come back as status codes with a message."""
+import os
+import re
+
+import pytest
+
+import helmholtz_amd as ha
+from helmholtz_amd import _lib
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "helmholtz_amd.h")
+
+
+def _declared():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(hh_\w+)\s*\(", text, re.M)))
+
+
+def test_header_declares_expected_entry_points():
+    assert set(_declared()) == set(_lib.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    L = ha.lib()
+    for name in _declared():
+        assert hasattr(L, name), name
+
+
+def test_version_and_error_string():
+    L = ha.lib()
+    assert L.hh_version() == 1
+    assert isinstance(L.hh_last_error(), bytes)
+
+
+def test_argument_errors_are_reported_without_touching_the_gpu():
+    L = ha.lib()
+    import ctypes as C
+    out = C.c_void_p()
+    nn = (C.c_int64 * 3)(10, 10, 1)
+    h = (C.c_double * 3)(1.0, 1.0, 1.0)
+    # dim = 4 is rejected before any device call
+    rc = L.hh_op_create(4, nn, h, None, None, 1.0, 0.0, 0, 0, 2, C.byref(out))
+    assert rc == _lib.HH_ERR_ARG
+    rc = L.hh_op_create(2, nn, h, C.c_void_p(1), C.c_void_p(1), 1.0, 0.0, 0, 0, 3, C.byref(out))
+    assert rc == _lib.HH_ERR_ARG
+    assert b"bc_order" in L.hh_last_error()
+
+
+def test_solver_rejects_unsupported_settings():
+    mg = ha.getMGparam(complex, int, 3, 1, 30, 1e-6, "Jac-GMRES", 0.75, 2, 2, "K", "NoMUMPS", 0.5, 0.0)
+    M = ha.getRegularMesh([0, 1, 0, 1], [8, 8])
+    import numpy as np
+    hp = ha.HelmholtzParam(M, np.zeros(81), np.ones(81), 1.0, False, True)
+    s = ha.getShiftedLaplacianMultigridSolver(hp, mg, 0.5, "GMRES", 5)
+    with pytest.raises(NotImplementedError):
+        ha.solveLinearSystem(None, np.ones(81, complex), s)
+    s2 = ha.getShiftedLaplacianMultigridSolver(hp, ha.getMGparam(3, 1, 30, 1e-6, "Jac", 0.8, 2, 2, "V", "NoMUMPS"),
+                                               0.5)
+    assert s2.Krylov == "BiCGSTAB" and s2.inner == 5 and list(s2.shift) == [0.5] * 3
+
+
+def test_getmgparam_both_signatures():
+    a = ha.getMGparam(complex, int, 2, 2, 30, 1e-6, "Jac", 0.75, 2, 2, "W", "NoMUMPS", 0.5, 0.0)
+    b = ha.getMGparam(5, 4, 50, 1e-5, "Jac", 0.8, 2, 2, "K", "NoMUMPS", 0.5, 0.0, "FullWeighting")
+    assert (a.levels, a.numCores, a.maxOuterIter, a.cycleType) == (2, 2, 30, "W")
+    assert (b.levels, b.relaxParam, b.extra) == (5, 0.8, (0.5, 0.0, "FullWeighting"))

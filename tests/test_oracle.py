@@ -1,0 +1,337 @@
+"""CPU pins of the oracle (no GPU).
+
+Helmholtz.jl's tests hold no assertions or golden vectors
+(test/testFictitiousSource2D.jl:57 is the only @test, commented out), so the
+oracle is pinned by: explicit KATs typed from the reference's row formulas,
+independent re-derivations (mirror-ghost loops, the reference's own
+matrix-free ``multOpNeumann!``), closed forms, invariants, the reference's
+manufactured-solution flow (second-order convergence) and committed fixtures.
+Each check is chosen so that a plausible mistake (a dropped term, a wrong
+sign, a transposed kron, an off-by-one pad) fails it.
+"""
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from oracle import helmholtz_ref as hr, slmg_ref as sr
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+# ---------------------------------------------------------------- Laplacian
+def test_dxx_mat_kat_bc2():
+    # src/PlainNodalLaplacian.jl:18-30 rows [BC,-BC], [-1,2,-1], [-BC,BC] over h^2, BC=2
+    D = hr.dxx_mat(4, 0.5, 2).toarray()
+    K = np.array([[2, -2, 0, 0], [-1, 2, -1, 0], [0, -1, 2, -1], [0, 0, -2, 2]]) / 0.25
+    np.testing.assert_array_equal(D, K)
+
+
+def test_dxx_mat_kat_bc1():
+    D = hr.dxx_mat(4, 0.5, 1).toarray()
+    K = np.array([[1, -1, 0, 0], [-1, 2, -1, 0], [0, -1, 2, -1], [0, 0, -1, 1]]) / 0.25
+    np.testing.assert_array_equal(D, K)
+
+
+@pytest.mark.parametrize("order", [1, 2])
+@pytest.mark.parametrize("cells", [[6, 5], [4, 3, 5]])
+def test_laplacian_kills_constants(order, cells):
+    M = hr.get_regular_mesh(sum(([0, 1 + d] for d in range(len(cells))), []), cells)
+    L = hr.get_nodal_laplacian_matrix(M, order)
+    assert np.abs(L @ np.ones(L.shape[0])).max() <= 1e-14 * abs(L).max()
+
+
+def _mirror_lap(X, h, mode):
+    """Independent statement of the nodal Laplacian: pad every axis by one
+    ghost (mirror x~_{-1} = x_1 for BC=2, copy x~_{-1} = x_0 for BC=1) and
+    apply the (2x - left - right)/h^2 formula per axis."""
+    pad_mode = "reflect" if mode == 2 else "edge"
+    G = np.pad(X, 1, mode=pad_mode)
+    out = np.zeros_like(X)
+    core = tuple(slice(1, -1) for _ in X.shape)
+    for d in range(X.ndim):
+        lo = list(core)
+        hi = list(core)
+        lo[d] = slice(0, -2)
+        hi[d] = slice(2, None)
+        out += (2 * G[core] - G[tuple(lo)] - G[tuple(hi)]) / h[d] ** 2
+    return out
+
+
+@pytest.mark.parametrize("order", [1, 2])
+@pytest.mark.parametrize("cells", [[7, 5], [5, 4, 6]])
+def test_kron_laplacian_equals_ghost_stencil(order, cells):
+    M = hr.get_regular_mesh(sum(([0, 1 + 0.3 * d] for d in range(len(cells))), []), cells)
+    L = hr.get_nodal_laplacian_matrix(M, order)
+    nodes = tuple(int(v) for v in M.nodes)
+    x = np.random.default_rng(0).standard_normal(int(np.prod(nodes)))
+    ref = _mirror_lap(x.reshape(nodes, order="F"), M.h, order).ravel(order="F")
+    np.testing.assert_allclose(L @ x, ref, rtol=1e-13, atol=1e-10)
+
+
+def test_bc1_equals_reference_multOpNeumann():
+    # src/PlainNodalLaplacian.jl:155-188 (Lap2DStencil with ghost = self)
+    M = hr.get_regular_mesh([0, 2, 0, 1.5], [9, 6])
+    x = np.random.default_rng(1).standard_normal(int(np.prod(M.nodes)))
+    L = hr.get_nodal_laplacian_matrix(M, 1)
+    np.testing.assert_allclose(L @ x, hr.mult_op_neumann(M, x), rtol=1e-13, atol=1e-10)
+
+
+def test_bc2_is_not_symmetric_but_bc1_is():
+    # comment at src/PlainNodalLaplacian.jl:7: with 2 the matrix is not symmetric
+    M = hr.get_regular_mesh([0, 1, 0, 1], [4, 4])
+    L2 = hr.get_nodal_laplacian_matrix(M, 2)
+    L1 = hr.get_nodal_laplacian_matrix(M, 1)
+    assert abs(L2 - L2.T).max() > 0
+    assert abs(L1 - L1.T).max() == 0
+
+
+# ---------------------------------------------------------------- ABL
+def test_abl_2d_closed_form():
+    n, pad, A = [12, 10], [3, 4], 2.0
+    g = hr.get_abl(n, False, pad, A)
+    # corner: both ramps at full strength -> A (1 + 1 - 1)
+    assert g[0, 0] == pytest.approx(A)
+    # a point only in the left x ramp at k = 1 (0-based): ((pad - 1)/pad)^2
+    assert g[1, 5] == pytest.approx(A * (2 / 3) ** 2)
+    # right x ramp at the last node: (pad/pad)^2 = 1 ; right ramp first node (1/pad)^2
+    assert g[11, 5] == pytest.approx(A)
+    assert g[9, 5] == pytest.approx(A * (1 / 3) ** 2)
+    # interior untouched
+    assert g[5, 5] == 0.0
+    # mixed: p1 = (1/3)^2 at i=9, p2 = (2/4)^2 at j=8 (high z ramp k=2 -> (3/4)^2?)
+    p1 = (1 / 3) ** 2
+    p2 = (3 / 4) ** 2  # j = 8 is the 3rd node of the last-4 ramp -> (3/4)^2
+    assert g[9, 8] == pytest.approx(A * (p1 + p2 - p1 * p2))
+
+
+def test_abl_2d_neumann_top_has_no_top_ramp():
+    g = hr.get_abl([12, 10], True, [3, 4], 1.0)
+    assert g[5, 0] == 0.0
+    assert g[0, 0] == pytest.approx(1.0)  # left ramp only
+    g2 = hr.get_abl([12, 10], False, [3, 4], 1.0)
+    assert g2[5, 0] == pytest.approx(1.0)
+
+
+def test_abl_3d_closed_form():
+    n, pad, A = [9, 8, 7], [2, 3, 2], 3.0
+    g = hr.get_abl(n, False, pad, A)
+    x = np.linspace(-1, 1, 9)
+    r0 = (x[0] - x[1]) ** 2  # left ramp at i = 0 (pad 2: reference node x[pad-1])
+    gx0 = r0 / (r0 + 1e-5)  # both x ramps peak at the same value
+    # a node in the x ramp only (y, z interior)
+    assert g[0, 4, 3] == pytest.approx(A * gx0)
+    assert g[4, 4, 3] == 0.0
+    assert g.max() <= A
+    assert g[0, 0, 0] == A  # clipped
+
+
+# ---------------------------------------------------------------- Sommerfeld and operator
+def test_sommerfeld_closed_form():
+    M = hr.get_regular_mesh([0, 2, 0, 1], [8, 4])
+    m = np.linspace(0.2, 0.5, 45)
+    w = 3.0
+    S = hr.get_sommerfeld_bc(M, m, w, True)
+    mm = m.reshape((9, 5), order="F")
+    h1, h2 = M.h
+    assert S[4, 2] == 0
+    assert S[8, 2] == pytest.approx(-1j * w * 2 / h1 * np.sqrt(mm[8, 2]))
+    assert S[8, 4] == pytest.approx(-1j * w * (2 / h1 + 2 / h2) * np.sqrt(mm[8, 4]))
+    assert S[4, 0] == 0  # Neumann top: no top term
+    S2 = hr.get_sommerfeld_bc(M, m, w, False)
+    assert S2[4, 0] == pytest.approx(-1j * w * 2 / h2 * np.sqrt(mm[4, 0]))
+
+
+def test_operator_diagonal_closed_form():
+    M = hr.get_regular_mesh([0, 1, 0, 1], [6, 6])
+    N = 49
+    m = np.full(N, 0.25)
+    g = np.full(N, 0.7)
+    w = 5.0
+    H = hr.get_helmholtz_operator7(M, m, w, g, False, False)
+    h = M.h[0]
+    # interior node: 4/h^2 - w^2 m (1 - i g / w)
+    assert H[24, 24] == pytest.approx(4 / h**2 - w**2 * 0.25 * (1 - 1j * 0.7 / w))
+    assert H[24, 23] == pytest.approx(-1 / h**2)
+
+
+def test_operator_real_without_attenuation():
+    M = hr.get_regular_mesh([0, 1, 0, 1], [8, 6])
+    m = np.random.default_rng(0).uniform(0.1, 0.4, 63)
+    H = hr.get_helmholtz_operator7(M, m, 4.0, np.zeros(63), False, False)
+    assert abs(H.imag).max() == 0.0
+
+
+def test_shifted_param_equals_shift_op():
+    M = hr.get_regular_mesh([0, 1, 0, 2], [10, 8])
+    m = np.random.default_rng(2).uniform(0.1, 0.4, 99)
+    g = np.random.default_rng(3).uniform(0.0, 2.0, 99)
+    p = hr.HelmholtzParam(M, g, m, 6.0, True, True)
+    H = hr.operator_from_param(p)
+    SH = hr.operator_from_param(hr.get_shifted_helmholtz_param(p, 0.3))
+    D = SH - (H + hr.get_helmholtz_shift_op(m, 6.0, 0.3))
+    assert abs(D).max() <= 1e-15 * abs(SH).max()
+
+
+def test_helmholtz_fun_recovers_h():
+    # getHelmholtzFun(SH^H, ShiftOP) applies SH x - i b w^2 m x = H x
+    M = hr.get_regular_mesh([0, 1, 0, 1], [8, 8])
+    m = np.random.default_rng(4).uniform(0.1, 0.4, 81)
+    H = hr.get_helmholtz_operator7(M, m, 5.0, np.full(81, 0.3), False, True)
+    S = hr.get_helmholtz_shift_op(m, 5.0, 0.5)
+    SHt = (H + S).conj().T  # the solver receives sparse(SH') (test/ShiftedLaplacianTest.jl:83)
+    x = np.random.default_rng(5).standard_normal(81) + 1j
+    # with SpMatMul's adjoint semantics the second product is (ShiftOP)^H x = -i b w^2 m x
+    f2 = hr.helmholtz_fun(SHt, S)
+    np.testing.assert_allclose(f2(x), H @ x, rtol=1e-13, atol=1e-12)
+
+
+def test_max_frequency_is_ten_ppw():
+    M = hr.get_regular_mesh([0, 1, 0, 1], [100, 100])
+    m = np.array([1 / 2.0**2, 1 / 3.0**2])
+    w = hr.get_maximal_frequency(m, M)
+    lam = 2 * np.pi * 2.0 / w  # wavelength at v_min = 2
+    assert lam / M.h[0] == pytest.approx(10.0)
+
+
+def test_point_source_and_index():
+    M = hr.get_regular_mesh([0, 1, 0, 1], [128, 128])
+    q, src = hr.get_acoustic_point_source(M)
+    assert src == [64, 1]  # div(n+1, 2), top row
+    assert hr.loc2cs([129, 129], [64, 1]) == 64
+    assert q[63] == pytest.approx(1 / np.linalg.norm(M.h) ** 2)
+    assert hr.get_mid_point_src(M) == [64, 64]
+    assert hr.loc2cs([5, 4, 3], [2, 3, 2]) == 2 + 2 * 5 + 1 * 20
+
+
+# ---------------------------------------------------------------- manufactured solution
+def _fictitious_source_errors(slowness, refinements=3):
+    """test/testFictitiousSource2D.jl:10-55 flow on [-1,1]^2 from 32x48 cells,
+    u = cos(pi x) cos(pi y) (zero normal derivative), Neumann rows, ABL gamma
+    (pad 5, amplitude 0.1) with a right-hand side consistent with the
+    operator's -w^2 m (1 - i gamma / w) mass term."""
+    w = 2 * np.pi
+    errs = []
+    n = np.array([32, 48])
+    for _ in range(refinements):
+        n = n * 2
+        M = hr.get_regular_mesh([-1, 1, -1, 1], n)
+        x = np.linspace(-1, 1, n[0] + 1)
+        y = np.linspace(-1, 1, n[1] + 1)
+        X, Y = np.meshgrid(x, y, indexing="ij")
+        u = (np.cos(np.pi * X) * np.cos(np.pi * Y)).ravel(order="F")
+        m = slowness(X, Y).ravel(order="F")
+        g = hr.get_abl(M.nodes, True, [5, 5], 0.1).ravel(order="F")
+        f = 2 * np.pi**2 * u - w**2 * m * (1 - 1j * g / w) * u
+        A = hr.get_helmholtz_operator7(M, m, w, g, True, False)
+        ut = spla.spsolve(A.tocsc(), f)
+        errs.append(np.abs(ut - u).max())
+    return np.array(errs)
+
+
+@pytest.mark.parametrize("slowness", [lambda X, Y: np.ones_like(X),
+                                      lambda X, Y: np.exp(-2.0 * (X**2 + Y**2))])
+def test_second_order_convergence(slowness):
+    e = _fictitious_source_errors(slowness)
+    ratios = e[:-1] / e[1:]
+    assert np.all(ratios > 3.6) and np.all(ratios < 4.4), (e, ratios)
+
+
+# ---------------------------------------------------------------- multigrid / FGMRES invariants
+def test_prolongation_preserves_linears():
+    P = sr.prolongation([5, 4])
+    xc = np.linspace(0, 1, 5)
+    yc = np.linspace(0, 2, 4)
+    Xc, Yc = np.meshgrid(xc, yc, indexing="ij")
+    xf = np.linspace(0, 1, 9)
+    yf = np.linspace(0, 2, 7)
+    Xf, Yf = np.meshgrid(xf, yf, indexing="ij")
+    lin_c = (3 * Xc - 2 * Yc + 1).ravel(order="F")
+    lin_f = (3 * Xf - 2 * Yf + 1).ravel(order="F")
+    np.testing.assert_allclose(P @ lin_c, lin_f, atol=1e-14)
+
+
+@pytest.mark.parametrize("nodes", [[5, 4], [3, 4, 5]])
+def test_restriction_is_weighted_transpose(nodes):
+    R = sr.restriction(nodes).toarray()
+    P = sr.prolongation(nodes).toarray()
+    np.testing.assert_allclose(R @ np.ones(R.shape[1]), 1.0, atol=1e-15)
+
+    def trap(n):
+        w = np.ones(n)
+        w[0] = w[-1] = 0.5
+        return w
+
+    Wc = np.ones(1)
+    Wf = np.ones(1)
+    for n in nodes:
+        Wc = np.kron(trap(n), Wc)
+        Wf = np.kron(trap(2 * n - 1), Wf)
+    np.testing.assert_allclose(R, (P.T * Wf[None, :]) / Wc[:, None] / 2 ** len(nodes), atol=1e-15)
+
+
+def _small_problem(cells=(64, 64), w_scale=1.0):
+    M = hr.get_regular_mesh([0, 1, 0, 1], list(cells))
+    nodes = M.nodes
+    m = np.random.default_rng(0).uniform(0.2, 0.3, int(np.prod(nodes)))
+    w = hr.get_maximal_frequency(m, M) * w_scale
+    g = hr.get_abl(nodes, False, [6, 6], w).ravel(order="F") + 0.01 * w
+    p = hr.HelmholtzParam(M, g, m, w, False, True)
+    return p, hr.operator_from_param(p)
+
+
+def test_vcycle_contracts_on_shifted_operator():
+    p, H = _small_problem()
+    hier = sr.build_hierarchy(p, 3, 0.5)
+    SH = hier[0].A
+    b = np.random.default_rng(1).standard_normal(SH.shape[0]) + 0j
+    x = np.zeros_like(b)
+    r0 = np.linalg.norm(b)
+    for _ in range(6):
+        x = x + sr.vcycle(hier, 0, b - SH @ x)
+    assert np.linalg.norm(b - SH @ x) / r0 < 0.5**6
+
+
+def test_fgmres_exact_preconditioner_one_iteration():
+    p, H = _small_problem((32, 32))
+    lu = spla.splu(H.tocsc())
+    b = np.random.default_rng(2).standard_normal(H.shape[0]) + 0j
+    x, it, hist = sr.fgmres(lambda v: H @ v, b, lu.solve, 10, 5, 1e-10)
+    assert it == 1
+    assert np.linalg.norm(H @ x - b) / np.linalg.norm(b) < 1e-10
+
+
+def test_fgmres_estimate_equals_true_residual_and_is_monotone():
+    p, H = _small_problem((32, 32))
+    hier = sr.build_hierarchy(p, 2, 0.5)
+    b = np.random.default_rng(3).standard_normal(H.shape[0]) + 1j
+    x, it, hist = sr.fgmres(lambda v: H @ v, b, lambda v: sr.vcycle(hier, 0, v), 40, 3, 1e-9)
+    assert np.all(np.diff(hist[: it + 1]) <= 1e-15)
+    true = np.linalg.norm(H @ x - b) / np.linalg.norm(b)
+    assert true <= 1e-9 and abs(true - hist[-1]) < 1e-11
+
+
+def test_fgmres_identity_preconditioner_matches_direct():
+    M = hr.get_regular_mesh([0, 1, 0, 1], [12, 12])
+    m = np.full(169, 0.1)
+    H = hr.get_helmholtz_operator7(M, m, 1.0, np.full(169, 0.5), False, True)
+    b = np.random.default_rng(4).standard_normal(169) + 0j
+    x, it, _ = sr.fgmres(lambda v: H @ v, b, lambda v: v, 30, 50, 1e-12)
+    np.testing.assert_allclose(x, spla.spsolve(H.tocsc(), b), rtol=1e-9, atol=1e-12)
+
+
+def test_golden_fixtures_unchanged():
+    """Committed operator/ABL fixtures (tests/golden/make_golden.py) still
+    match the oracle bit for bit (guards against silent oracle drift)."""
+    f = np.load(os.path.join(GOLDEN, "operator_fixtures.npz"))
+    for key in sorted({k.split("__")[0] for k in f.files}):
+        spec = f[key + "__spec"]
+        dim = int(spec[0])
+        nodes = [int(v) for v in spec[1:1 + dim]]
+        neumann, somm, order = bool(spec[4]), bool(spec[5]), int(spec[6])
+        M = hr.get_regular_mesh(f[key + "__domain"], [n - 1 for n in nodes])
+        m, g, w = f[key + "__m"], f[key + "__gamma"], complex(f[key + "__omega"])
+        H = hr.get_helmholtz_operator7(M, m, w, g, neumann, somm, order)
+        np.testing.assert_array_equal(H @ f[key + "__x"], f[key + "__y"])
