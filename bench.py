@@ -1,0 +1,200 @@
+"""Benchmark: ms per right-hand side to a 1e-6 true residual on one MI355X.
+
+Workload (the batched multi-RHS 2-D case): 1025 x 513 nodal grid, h = 10 m,
+seeded layered medium (1500-4500 m/s, 12 interfaces, elliptic 4500 m/s body,
+5% noise), one frequency at 8 points per wavelength at 1500 m/s, attenuation
+0.01 w plus a 20-cell absorbing layer of amplitude w, Neumann top, Sommerfeld
+sides, 256 point sources on depth row 2 (x = 3 + 4k).  Preconditioner: 6-level
+shifted-Laplacian W(2,2) cycle, damped Jacobi 0.8, shift 0.5 on every level;
+outer FGMRES(10) in complex128 to ||b - H x|| <= 1e-6 ||b||.
+
+A step = one solve of all 256 sources in one batched call (complex64
+preconditioner, inputs resident in HBM).  The plain method (complex128
+preconditioner, one source per call) is timed on a subset of sources and
+reported beside it; the line's ``value`` is the batched ms/RHS.
+
+Usage: python bench.py [--gpus 1] [--steps 2] [--warmup 1] [--plain-rhs 4]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "helmholtz.jl_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def c2_problem(nsrc: int):
+    import helmholtz_amd as ha
+    from helmholtz_amd import media
+    n = (1025, 513)
+    h = 10.0
+    v = media.layered_body_velocity_2d(n, seed=2)
+    m = (1.0 / v**2).ravel(order="F")
+    M = ha.getRegularMesh([0.0, h * (n[0] - 1), 0.0, h * (n[1] - 1)], [n[0] - 1, n[1] - 1])
+    w = media.omega_for_ppw(1500.0, h, 8)
+    H, gamma = ha.GetHelmholtzOperator(M, m, w, 0.01 * w * np.ones(m.size), True, [20, 20], w, True)
+    hp = ha.HelmholtzParam(M, gamma.ravel(order="F"), m, w, True, True)
+    B = ha.point_sources(M, media.surface_sources_2d(n, nsrc))
+    return M, hp, H, B, w
+
+
+def make_solver(hp, prec: str, levels=6, cycle="W", tol=1e-6):
+    import helmholtz_amd as ha
+    mg = ha.getMGparam(complex, int, levels, 1, 200, tol, "Jac", 0.8, 2, 2, cycle, "NoMUMPS", 0.5, 0.0)
+    mg.prec = prec
+    return ha.getShiftedLaplacianMultigridSolver(hp, mg, 0.5, "GMRES", 10, False)
+
+
+def roofline_jacobi(H, nrhs: int, reps: int = 20):
+    """Event-timed damped-Jacobi sweep (complex64, the batched smoother) at the
+    workload's fine grid with ``nrhs`` columns.  Algorithmic bytes per node:
+    read x, b, write x (3 * 8 B per RHS) + the complex64 diagonal (8 B)."""
+    import ctypes as C
+    from helmholtz_amd import _lib as L
+    from helmholtz_amd.device import DeviceArray
+    N = H.shape[0]
+    rng = np.random.default_rng(0)
+    col = (rng.standard_normal(N) + 1j * rng.standard_normal(N)).astype(np.complex64)
+    X = DeviceArray.from_numpy(np.asfortranarray(np.repeat(col[:, None], nrhs, axis=1)))
+    Bd = DeviceArray.from_numpy(np.asfortranarray(np.repeat(col[:, None], nrhs, axis=1)))
+    Y = X.empty_like()
+    ms = C.c_double()
+    L.check(L.lib().hh_op_time_kernel(H._h.ptr, 2, 0.5, X.ptr, Bd.ptr, Y.ptr, nrhs, L.HH_C64, reps, C.byref(ms)))
+    bytes_alg = N * (3 * 8 * nrhs + 8)
+    achieved = bytes_alg / (ms.value * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": "k_stencil2d<float,float,float,M_JACOBI>", "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "bytes_per_launch": bytes_alg, "avg_launch_ms": round(ms.value, 5), "nrhs": nrhs}
+
+
+def cpu_baseline(hp, B, iters_per_rhs: int, sample_iters: int):
+    """Oracle (NumPy/SciPy, one core) plain FP64 SL-MG FGMRES on one source for
+    ``sample_iters`` iterations; ms/RHS = time per iteration x the GPU plain
+    method's iterations per RHS."""
+    sys.path.insert(0, ROOT)
+    from oracle import helmholtz_ref as hr, slmg_ref as sr
+    M = hr.get_regular_mesh(hp.Mesh.domain, hp.Mesh.n)
+    p = hr.HelmholtzParam(M, hp.gamma, hp.m, hp.omega, True, True)
+    Hc = hr.operator_from_param(p)
+    hier = sr.build_hierarchy(p, 6, 0.5)
+    b = B[:, 0].copy()
+    t0 = time.perf_counter()
+    _, it, _ = sr.fgmres(lambda v: Hc @ v, b, lambda v: sr.vcycle(hier, 0, v, 0.8, 2, 2, "W"), 10,
+                         max(1, sample_iters // 10), 1e-30)
+    dt = time.perf_counter() - t0
+    per_it = dt / max(it, 1)
+    return {"value": round(per_it * iters_per_rhs * 1e3, 2), "unit": "ms/RHS", "cores": 1, "kind": "port",
+            "sample": f"oracle FGMRES(10)+W-cycle, 1 source, {it} iterations in {dt:.1f} s, scaled by "
+                      f"{iters_per_rhs} iterations/RHS"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nsrc", type=int, default=256)
+    ap.add_argument("--plain-rhs", type=int, default=4)
+    ap.add_argument("--cpu-iters", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the solve shards by right-hand side "
+              "with no communication, so N GPUs would each run this benchmark on nsrc/N sources.",
+              file=sys.stderr)
+        sys.exit(2)
+    import helmholtz_amd as ha
+    from helmholtz_amd.device import DeviceArray
+
+    def log(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    t = time.perf_counter()
+    M, hp, H, B, w = c2_problem(args.nsrc)
+    log(f"problem built in {time.perf_counter() - t:.1f} s: N={H.shape[0]} nsrc={args.nsrc} omega={w:.3f}")
+    Bd = DeviceArray.from_numpy(B)
+
+    # ---- batched, complex64 preconditioner
+    sb = make_solver(hp, "c64")
+    for k in range(args.warmup):
+        X, sb = ha.solveLinearSystem(None, Bd, sb)
+        log(f"warmup {k}: max iters {sb.last_stats['max_iters']}, setup {sb.last_stats['setup_time']:.2f} s")
+    times = []
+    for k in range(args.steps):
+        t0 = time.perf_counter()
+        X, sb = ha.solveLinearSystem(None, Bd, sb)
+        times.append(time.perf_counter() - t0)
+        log(f"step {k}: {times[-1]:.3f} s, max iters {sb.last_stats['max_iters']}")
+    st = sb.last_stats
+    res_b = np.asarray(st["true_res"])
+    if not np.all(res_b <= 1e-6):
+        log(f"batched solve missed the tolerance: max true residual {res_b.max():.3e}")
+    ms_step = 1e3 * float(np.mean(times))
+    batched_ms_rhs = ms_step / args.nsrc
+
+    # ---- plain FP64, one source per call
+    sp_ = make_solver(hp, "c128")
+    plain_t, plain_it, plain_res = [], [], []
+    step = max(1, args.nsrc // max(1, args.plain_rhs))
+    for c in range(args.plain_rhs + 1):
+        col = 0 if c == 0 else min(args.nsrc - 1, (c - 1) * step + step // 2)
+        bcol = DeviceArray.from_numpy(B[:, [col]])
+        t0 = time.perf_counter()
+        Xp, sp_ = ha.solveLinearSystem(None, bcol, sp_)
+        dt = time.perf_counter() - t0
+        if c == 0:
+            continue  # first call builds the hierarchy
+        plain_t.append(dt)
+        plain_it.append(int(sp_.last_stats["max_iters"]))
+        plain_res.append(float(sp_.last_stats["true_res"][0]))
+        log(f"plain rhs {c}: {dt:.3f} s, {plain_it[-1]} iters, res {plain_res[-1]:.2e}")
+    plain_ms_rhs = 1e3 * float(np.mean(plain_t))
+
+    roof = roofline_jacobi(H, args.nsrc)
+    log(f"roofline: {roof}")
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(hp, B, int(np.mean(plain_it)), args.cpu_iters)
+        log(f"cpu baseline: {cpu}")
+
+    line = {
+        "metric": "ms/RHS to 1e-6 true residual: learned+batched vs plain SL-MG, same GPU",
+        "value": round(batched_ms_rhs, 4),
+        "unit": "ms/RHS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64 preconditioner / c128 outer",
+        "data": "synthetic (seeded layered+body medium, point sources)",
+        "config": {"workload": "2-D 1025x513 Helmholtz, 256 point sources, 8 ppw, 1e-6 true residual",
+                   "nsrc": args.nsrc, "levels": 6, "cycle": "W(2,2)", "krylov": "FGMRES(10)",
+                   "learned": False, "parallelism": "rhs-batched, 1 GPU"},
+        "batched_ms_per_rhs": round(batched_ms_rhs, 4),
+        "batched_max_iters": int(st["max_iters"]),
+        "batched_max_true_res": float(res_b.max()),
+        "all_converged": bool(np.all(res_b <= 1e-6)) and bool(max(plain_res) <= 1e-6),
+        "plain_ms_per_rhs": round(plain_ms_rhs, 3),
+        "plain_iters": plain_it,
+        "plain_max_true_res": float(max(plain_res)),
+        "plain_rhs_sampled": args.plain_rhs,
+        "speedup_vs_plain": round(plain_ms_rhs / batched_ms_rhs, 2),
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -387,13 +387,18 @@ struct Cycle {
       sweeps(L, b, pre);
     }
     Level& C = S->lv[l + 1];
-    launch_stencil<T, T, T, M_RESID>(L.g, nrhs, (const cx<T>*)L.cur, b, (cx<T>*)L.r, (const cx<T>*)L.d, nullptr,
-                                     mk<double>(0, 0), 0.0, skip, st);
-    launch_restrict<T>(L.g, C.g, nrhs, (const cx<T>*)L.r, (cx<T>*)C.b, skip, st);
-    C.cur = nullptr;
-    run(l + 1, (const cx<T>*)C.b, true, nullptr);
-    if (S->p.cycle == 'W' && l + 1 < nl - 1) run(l + 1, (const cx<T>*)C.b, false, nullptr);
-    launch_prolong_add<T>(L.g, C.g, nrhs, (const cx<T>*)C.cur, (cx<T>*)L.cur, skip, st);
+    // W: two coarse-grid corrections per visit, each from this level's
+    // current residual (iterating the coarse cycle on the restricted residual
+    // instead diverges with rediscretised coarse operators near kh ~ 2).
+    const int ncorr = S->p.cycle == 'W' ? 2 : 1;
+    for (int c = 0; c < ncorr; ++c) {
+      launch_stencil<T, T, T, M_RESID>(L.g, nrhs, (const cx<T>*)L.cur, b, (cx<T>*)L.r, (const cx<T>*)L.d, nullptr,
+                                       mk<double>(0, 0), 0.0, skip, st);
+      launch_restrict<T>(L.g, C.g, nrhs, (const cx<T>*)L.r, (cx<T>*)C.b, skip, st);
+      C.cur = nullptr;
+      run(l + 1, (const cx<T>*)C.b, true, nullptr);
+      launch_prolong_add<T>(L.g, C.g, nrhs, (const cx<T>*)C.cur, (cx<T>*)L.cur, skip, st);
+    }
     sweeps(L, b, post);
   }
 
@@ -771,6 +776,71 @@ int hh_abl(int dim, const int64_t nnodes[3], int neumann_at_first_dim, const int
     for (int d = 0; d < 3; ++d) dfree(dp[d]);
     if (loc == HH_HOST) dfree(dout);
     hipck(e, "hh_abl");
+  });
+}
+
+int hh_device_alloc(int64_t bytes, void** ptr) {
+  return guarded([&] {
+    if (!ptr || bytes < 0) throw HHError(HH_ERR_ARG, "bad argument");
+    *ptr = dalloc<char>((size_t)bytes);
+  });
+}
+
+int hh_device_free(void* ptr) {
+  return guarded([&] { dfree(ptr); });
+}
+
+int hh_memcpy(void* dst, const void* src, int64_t bytes, int kind) {
+  return guarded([&] {
+    if (!dst || !src || bytes < 0) throw HHError(HH_ERR_ARG, "bad argument");
+    hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+    if (kind < 0 || kind > 2) throw HHError(HH_ERR_ARG, "kind must be 0 (H2D), 1 (D2H) or 2 (D2D)");
+    HIPCK(hipMemcpy(dst, src, (size_t)bytes, k));
+  });
+}
+
+int hh_op_time_kernel(hh_op* op, int kernel, double shift, const void* X, const void* B, void* Y, int64_t nrhs,
+                      int prec, int reps, double* avg_ms) {
+  return guarded([&] {
+    if (!op || !X || !Y || !avg_ms || reps < 1) throw HHError(HH_ERR_ARG, "bad argument");
+    if (nrhs < 1 || nrhs > 65535) throw HHError(HH_ERR_ARG, "nrhs must be in 1..65535");
+    if (kernel < 0 || kernel > 2) throw HHError(HH_ERR_ARG, "kernel must be 0 (apply), 1 (residual), 2 (Jacobi)");
+    if (kernel > 0 && !B) throw HHError(HH_ERR_ARG, "residual/Jacobi need B");
+    const double w2r = op->wre * op->wre - op->wim * op->wim, w2i = 2.0 * op->wre * op->wim;
+    const cx<double> sfac = mk<double>(-shift * w2i, shift * w2r);
+    const double* mm = shift != 0.0 ? op->m : nullptr;
+    const int n = (int)nrhs;
+    auto launch = [&]() {
+      if (prec == HH_C128) {
+        auto x = (const cx<double>*)X;
+        auto b = (const cx<double>*)B;
+        auto y = (cx<double>*)Y;
+        if (kernel == 0) launch_stencil<double, double, double, M_APPLY>(op->g, n, x, b, y, op->d64, mm, sfac, 0.8, nullptr, op->stream);
+        if (kernel == 1) launch_stencil<double, double, double, M_RESID>(op->g, n, x, b, y, op->d64, mm, sfac, 0.8, nullptr, op->stream);
+        if (kernel == 2) launch_stencil<double, double, double, M_JACOBI>(op->g, n, x, b, y, op->d64, mm, sfac, 0.8, nullptr, op->stream);
+      } else {
+        auto x = (const cx<float>*)X;
+        auto b = (const cx<float>*)B;
+        auto y = (cx<float>*)Y;
+        if (kernel == 0) launch_stencil<float, float, float, M_APPLY>(op->g, n, x, b, y, op->d32, mm, sfac, 0.8, nullptr, op->stream);
+        if (kernel == 1) launch_stencil<float, float, float, M_RESID>(op->g, n, x, b, y, op->d32, mm, sfac, 0.8, nullptr, op->stream);
+        if (kernel == 2) launch_stencil<float, float, float, M_JACOBI>(op->g, n, x, b, y, op->d32, mm, sfac, 0.8, nullptr, op->stream);
+      }
+    };
+    launch();  // warm-up
+    hipEvent_t e0, e1;
+    HIPCK(hipEventCreate(&e0));
+    HIPCK(hipEventCreate(&e1));
+    HIPCK(hipEventRecord(e0, op->stream));
+    for (int r = 0; r < reps; ++r) launch();
+    HIPCK(hipEventRecord(e1, op->stream));
+    HIPCK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIPCK(hipGetLastError());
+    *avg_ms = (double)ms / reps;
   });
 }
 

@@ -400,27 +400,47 @@ __global__ __launch_bounds__(256) void k_gj_elim(cx<double>* __restrict__ M, int
   M[r + c * n] = M[r + c * n] - f * M[k + c * n];
 }
 
-// x_b = Ainv r_b in complex128 for every right-hand side.
+// x_b = Ainv r_b in complex128 for every right-hand side.  Each block owns
+// 256 rows x DR right-hand sides, so one streamed column of Ainv serves DR
+// columns of r (the inverse is re-read nrhs/DR times instead of nrhs times).
+constexpr int DR = 8;
 template <typename T>
-__global__ __launch_bounds__(256) void k_dense_apply(const cx<double>* __restrict__ Ainv, int64_t n,
+__global__ __launch_bounds__(256) void k_dense_apply(const cx<double>* __restrict__ Ainv, int64_t n, int nrhs,
                                                      const cx<T>* __restrict__ r, cx<T>* __restrict__ x,
                                                      const int* __restrict__ skip) {
-  const int rhs = blockIdx.y;
-  if (skip && skip[rhs]) return;
+  const int rhs0 = blockIdx.y * DR;
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  __shared__ cx<double> rs[256];
-  const cx<T>* rb = r + (int64_t)rhs * n;
-  cx<double> acc = mk<double>(0.0, 0.0);
+  __shared__ cx<double> rs[DR][256];
+  bool live[DR];
+  int nlive = 0;
+#pragma unroll
+  for (int k = 0; k < DR; ++k) {
+    live[k] = rhs0 + k < nrhs && !(skip && skip[rhs0 + k]);
+    nlive += live[k];
+  }
+  if (nlive == 0) return;  // uniform per block
+  cx<double> acc[DR];
+#pragma unroll
+  for (int k = 0; k < DR; ++k) acc[k] = mk<double>(0.0, 0.0);
   for (int64_t j0 = 0; j0 < n; j0 += 256) {
     const int64_t jj = j0 + threadIdx.x;
-    rs[threadIdx.x] = jj < n ? cvt<T, double>(rb[jj]) : mk<double>(0.0, 0.0);
+#pragma unroll
+    for (int k = 0; k < DR; ++k)
+      rs[k][threadIdx.x] = (jj < n && live[k]) ? cvt<T, double>(r[(int64_t)(rhs0 + k) * n + jj]) : mk<double>(0.0, 0.0);
     __syncthreads();
     const int64_t jm = min((int64_t)256, n - j0);
     if (i < n)
-      for (int64_t t = 0; t < jm; ++t) acc = acc + Ainv[i + (j0 + t) * n] * rs[t];
+      for (int64_t t = 0; t < jm; ++t) {
+        const cx<double> a = Ainv[i + (j0 + t) * n];
+#pragma unroll
+        for (int k = 0; k < DR; ++k) acc[k] = acc[k] + a * rs[k][t];
+      }
     __syncthreads();
   }
-  if (i < n) x[(int64_t)rhs * n + i] = cvt<double, T>(acc);
+  if (i < n)
+#pragma unroll
+    for (int k = 0; k < DR; ++k)
+      if (live[k]) x[(int64_t)(rhs0 + k) * n + i] = cvt<double, T>(acc[k]);
 }
 
 // ---------------------------------------------------------------- launchers
@@ -562,8 +582,8 @@ void launch_dense_inverse(const Grid& g, const cx<double>* d, cx<double>* M, int
 template <typename T>
 void launch_dense_apply(const cx<double>* Ainv, int64_t n, int nrhs, const cx<T>* r, cx<T>* x, const int* skip,
                         hipStream_t s) {
-  dim3 grid((unsigned)((n + 255) / 256), nrhs);
-  hipLaunchKernelGGL((k_dense_apply<T>), grid, dim3(256), 0, s, Ainv, n, r, x, skip);
+  dim3 grid((unsigned)((n + 255) / 256), (unsigned)((nrhs + DR - 1) / DR));
+  hipLaunchKernelGGL((k_dense_apply<T>), grid, dim3(256), 0, s, Ainv, n, nrhs, r, x, skip);
 }
 template void launch_dense_apply<double>(const cx<double>*, int64_t, int, const cx<double>*, cx<double>*,
                                          const int*, hipStream_t);

@@ -20,7 +20,7 @@ HH_CONVERGED, HH_MAXITER, HH_NONFINITE, HH_ZERO_RHS = 0, 1, 2, 3
 EXPORTS = (
     "hh_last_error", "hh_version", "hh_abl", "hh_op_create", "hh_op_destroy", "hh_op_apply",
     "hh_solver_create", "hh_solver_copy", "hh_solver_clear", "hh_solver_destroy", "hh_solve",
-    "hh_solver_levels",
+    "hh_solver_levels", "hh_device_alloc", "hh_device_free", "hh_memcpy", "hh_op_time_kernel",
 )
 
 
@@ -73,6 +73,10 @@ def lib():
     L.hh_solver_destroy.argtypes = [vp]
     L.hh_solve.argtypes = [vp, vp, vp, i64, i32, i32, C.POINTER(SolveStats)]
     L.hh_solver_levels.argtypes = [vp, C.POINTER(C.c_int), vp, i32]
+    L.hh_device_alloc.argtypes = [i64, C.POINTER(vp)]
+    L.hh_device_free.argtypes = [vp]
+    L.hh_memcpy.argtypes = [vp, vp, i64, i32]
+    L.hh_op_time_kernel.argtypes = [vp, i32, dbl, vp, vp, vp, i64, i32, i32, pd]
     for name in EXPORTS:
         getattr(L, name).restype = getattr(L, name).restype if name in ("hh_last_error",) else C.c_int
     _lib = L

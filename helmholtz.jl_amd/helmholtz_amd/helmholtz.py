@@ -14,8 +14,7 @@ shifted operator.  Subscripts stay 1-based as in the reference; arrays are
 column-major (``order='F'``), axis 0 = x, the last axis = depth.
 
 Host vectors are numpy arrays (N,) or (N, nrhs) in Fortran order; device
-blocks are torch CUDA tensors shaped (nrhs, N) (row-major, i.e. the same bytes
-as a column-major N x nrhs matrix) or (N,).
+blocks are :class:`helmholtz_amd.device.DeviceArray` of the same shape.
 """
 from __future__ import annotations
 
@@ -92,8 +91,9 @@ def _f64x3(v, fill=1.0):
     return a
 
 
-def _is_torch(x) -> bool:
-    return type(x).__module__.startswith("torch")
+def _is_device(x) -> bool:
+    from .device import DeviceArray
+    return isinstance(x, DeviceArray)
 
 
 def getABL(n, NeumannAtFirstDim: bool, ABLpad, ABLamp: float):
@@ -179,20 +179,15 @@ class HelmholtzOperator:
     __radd__ = __add__
 
     def apply(self, X, prec: str = "c128", out=None):
-        """Y = A X.  X: numpy (N,)/(N, nrhs) (Fortran order) or torch CUDA (N,)/(nrhs, N)."""
+        """Y = A X.  X: numpy (N,)/(N, nrhs) (Fortran order) or a DeviceArray."""
         code = L.HH_C128 if prec == "c128" else L.HH_C64
-        if _is_torch(X):
-            import torch
-            want = torch.complex128 if code == L.HH_C128 else torch.complex64
-            if not X.is_cuda or X.dtype != want or not X.is_contiguous():
-                raise ValueError(f"device input must be a contiguous CUDA {want} tensor")
-            if X.numel() % self._h.N:
-                raise ValueError("size is not a multiple of the node count")
-            nrhs = X.numel() // self._h.N
-            Y = torch.empty_like(X) if out is None else out
-            torch.cuda.synchronize()
-            L.check(L.lib().hh_op_apply(self._h.ptr, self.shift, X.data_ptr(), Y.data_ptr(), nrhs, code,
-                                        L.HH_DEVICE))
+        if _is_device(X):
+            want = np.complex128 if code == L.HH_C128 else np.complex64
+            if X.dtype != want or X.shape[0] != self._h.N:
+                raise ValueError(f"device input must be an N x nrhs {np.dtype(want).name} block")
+            nrhs = 1 if len(X.shape) == 1 else X.shape[1]
+            Y = X.empty_like() if out is None else out
+            L.check(L.lib().hh_op_apply(self._h.ptr, self.shift, X.ptr, Y.ptr, nrhs, code, L.HH_DEVICE))
             return Y
         dt = np.complex128 if code == L.HH_C128 else np.complex64
         Xa = np.asarray(X)

@@ -22,7 +22,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib as L
-from .helmholtz import HelmholtzOperator, HelmholtzParam, _OpHandle, _is_torch
+from .helmholtz import HelmholtzParam, _OpHandle, _is_device
 
 
 @dataclass
@@ -159,8 +159,10 @@ _STATUS = {L.HH_CONVERGED: "converged", L.HH_MAXITER: "maxiter", L.HH_NONFINITE:
 def solveLinearSystem(ShiftedHT, B, param: ShiftedLaplacianMultigridSolver, doTranspose: int = 0):
     """src/ShiftedLaplacianMultigridSolver.jl:33-102 -> (X, param).
 
-    B: numpy (N,) or (N, nrhs) complex, or a torch CUDA complex128 tensor
-    (N,) / (nrhs, N) (then X is a CUDA tensor too and nothing crosses PCIe)."""
+    B: numpy (N,) or (N, nrhs) complex, or a complex128 DeviceArray (then X is
+    a DeviceArray too and nothing crosses PCIe).  An all-zero host B returns
+    zeros without touching the GPU (src/ShiftedLaplacianMultigridSolver.jl:40-43);
+    for device blocks the library reports zero columns per RHS."""
     _validate(param)
     if doTranspose != 0:
         raise NotImplementedError("doTranspose=1 (adjoint solve) is not implemented")
@@ -169,19 +171,13 @@ def solveLinearSystem(ShiftedHT, B, param: ShiftedLaplacianMultigridSolver, doTr
     N = int(np.prod(param.helmParam.Mesh.nodes))
     if ShiftedHT is not None and hasattr(ShiftedHT, "shape") and tuple(ShiftedHT.shape) != (N, N):
         raise ValueError("ShiftedHT has the wrong shape for helmParam.Mesh")
-    dev = _is_torch(B)
+    dev = _is_device(B)
     if dev:
-        import torch
-        if not B.is_cuda or B.dtype != torch.complex128 or not B.is_contiguous():
-            raise ValueError("device B must be a contiguous CUDA complex128 tensor")
-        nrhs = B.numel() // N
-        if nrhs * N != B.numel():
-            raise ValueError("B size is not a multiple of the node count")
-        if float(torch.linalg.vector_norm(B)) == 0.0:
-            return torch.zeros_like(B), param
-        X = torch.empty_like(B)
-        bptr, xptr, loc = B.data_ptr(), X.data_ptr(), L.HH_DEVICE
-        torch.cuda.synchronize()
+        if B.dtype != np.complex128 or B.shape[0] != N:
+            raise ValueError("device B must be an N x nrhs complex128 DeviceArray")
+        nrhs = 1 if len(B.shape) == 1 else B.shape[1]
+        X = B.empty_like()
+        bptr, xptr, loc = B.ptr, X.ptr, L.HH_DEVICE
     else:
         Ba = np.asarray(B)
         vec = Ba.ndim == 1

@@ -1,0 +1,133 @@
+# HelmholtzAMD.jl — Julia-side drop-in for Helmholtz.jl's SL-MG solver path,
+# calling libhelmholtz_amd.so (include/helmholtz_amd.h) through ccall.
+#
+# Mirrors src/ShiftedLaplacianMultigridSolver.jl:4-109 (struct, constructor,
+# copySolver, solveLinearSystem, clear!) and the operator apply of
+# src/GetHelmholtz.jl:85-95.  No Julia toolchain exists where this file was
+# written; it is checked only against the C header (argument order and types)
+# and exercised through the equivalent Python ctypes binding.
+module HelmholtzAMD
+
+using LinearAlgebra
+using SparseArrays
+
+const LIB = joinpath(@__DIR__, "..", "libhelmholtz_amd.so")
+
+const HH_C128 = Cint(0); const HH_C64 = Cint(1)
+const HH_HOST = Cint(0); const HH_DEVICE = Cint(1)
+const HH_FGMRES = Cint(0)
+
+struct HHMGParams
+    levels::Int32; pre::Int32; post::Int32; relax::Float64; cycle::Int32; max_outer::Int32
+    tol::Float64; inner::Int32; krylov::Int32; prec::Int32; learned::Int32; verbose::Int32
+end
+
+mutable struct HHSolveStats
+    iters::Ptr{Int32}; status::Ptr{Int32}; true_res::Ptr{Float64}
+    max_iters::Int32; n_converged::Int32; setup_time::Float64; solve_time::Float64; n_prec::Int64
+end
+
+check(rc) = rc == 0 || error("helmholtz_amd: " * unsafe_string(ccall((:hh_last_error, LIB), Cstring, ())))
+
+mutable struct HHOperator
+    ptr::Ptr{Cvoid}
+    function HHOperator(nnodes::Vector{Int}, h::Vector{Float64}, m::Vector{Float64}, gamma::Vector{Float64},
+                        omega::Union{Float64,ComplexF64}, neumann::Bool, sommerfeld::Bool, order::Int=2)
+        out = Ref{Ptr{Cvoid}}(C_NULL)
+        nn = Int64[nnodes; ones(Int64, 3 - length(nnodes))]
+        hh = Float64[h; ones(3 - length(h))]
+        GC.@preserve m gamma check(ccall((:hh_op_create, LIB), Cint,
+            (Cint, Ptr{Int64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Float64, Float64, Cint, Cint, Cint,
+             Ptr{Ptr{Cvoid}}),
+            length(nnodes), nn, hh, m, gamma, real(omega), imag(omega), neumann, sommerfeld, order, out))
+        op = new(out[])
+        finalizer(o -> ccall((:hh_op_destroy, LIB), Cint, (Ptr{Cvoid},), o.ptr), op)
+        return op
+    end
+end
+
+# Y = (H + i*shift*w^2*diag(m)) X  — the product of getHelmholtzFun (src/GetHelmholtz.jl:85-95)
+function apply!(Y::Array{ComplexF64}, op::HHOperator, X::Array{ComplexF64}, shift::Float64=0.0)
+    nrhs = size(X, 2)
+    GC.@preserve X Y check(ccall((:hh_op_apply, LIB), Cint,
+        (Ptr{Cvoid}, Float64, Ptr{ComplexF64}, Ptr{ComplexF64}, Int64, Cint, Cint),
+        op.ptr, shift, X, Y, nrhs, HH_C128, HH_HOST))
+    return Y
+end
+
+# jInv AbstractSolver plug-in with the fields of ShiftedLaplacianMultigridSolver
+mutable struct ShiftedLaplacianMultigridSolverAMD
+    helmParam::Any          # Helmholtz.HelmholtzParam
+    MG::Any                 # Multigrid.MGparam (levels, relaxParam, relaxPre/Post, cycleType, maxOuterIter, relativeTol)
+    shift::Vector{Float64}
+    Krylov::String
+    inner::Int64
+    doClear::Int64
+    verbose::Bool
+    setupTime::Real
+    nPrec::Int
+    solveTime::Real
+    op::Union{Nothing,HHOperator}
+    handle::Ptr{Cvoid}
+    prec::Cint              # HH_C128 (plain) or HH_C64 (mixed precision preconditioner)
+end
+
+getShiftedLaplacianMultigridSolverAMD(helmParam, MG, shift::Float64, Krylov="GMRES", inner=10, verbose=false;
+                                      prec=HH_C128) =
+    ShiftedLaplacianMultigridSolverAMD(helmParam, MG, fill(shift, MG.levels), Krylov, inner, 0, verbose,
+                                       0.0, 0, 0.0, nothing, C_NULL, prec)
+
+function ensure_handle!(s::ShiftedLaplacianMultigridSolverAMD)
+    p = s.helmParam
+    if s.op === nothing
+        s.op = HHOperator(collect(p.Mesh.n .+ 1), collect(p.Mesh.h), vec(p.m), vec(p.gamma), p.omega,
+                          p.NeumannOnTop, p.Sommerfeld)
+    end
+    if s.handle == C_NULL
+        mg = s.MG
+        prm = Ref(HHMGParams(mg.levels, mg.relaxPre, mg.relaxPost, mg.relaxParam, Int32(mg.cycleType),
+                             mg.maxOuterIter, mg.relativeTol, s.inner, HH_FGMRES, s.prec, 0, s.verbose))
+        out = Ref{Ptr{Cvoid}}(C_NULL)
+        check(ccall((:hh_solver_create, LIB), Cint, (Ptr{Cvoid}, Ref{HHMGParams}, Ptr{Float64}, Ptr{Ptr{Cvoid}}),
+                    s.op.ptr, prm, s.shift, out))
+        s.handle = out[]
+        finalizer(x -> x.handle != C_NULL && ccall((:hh_solver_destroy, LIB), Cint, (Ptr{Cvoid},), x.handle), s)
+    end
+end
+
+# solveLinearSystem(ShiftedHT, B, param, doTranspose) (src/ShiftedLaplacianMultigridSolver.jl:33-102)
+function solveLinearSystem(ShiftedHT, B, param::ShiftedLaplacianMultigridSolverAMD, doTranspose::Int64=0)
+    doTranspose == 0 || error("doTranspose = 1 is not implemented by libhelmholtz_amd")
+    if param.doClear == 1
+        clear!(param)
+    end
+    if norm(B) == 0.0
+        return zeros(eltype(B), size(B)), param
+    end
+    ensure_handle!(param)
+    Bc = convert(Array{ComplexF64}, B)
+    nrhs = size(Bc, 2)
+    X = similar(Bc)
+    iters = zeros(Int32, nrhs); status = zeros(Int32, nrhs); res = zeros(Float64, nrhs)
+    GC.@preserve Bc X iters status res begin
+        st = Ref(HHSolveStats(pointer(iters), pointer(status), pointer(res), 0, 0, 0.0, 0.0, 0))
+        check(ccall((:hh_solve, LIB), Cint,
+                    (Ptr{Cvoid}, Ptr{ComplexF64}, Ptr{ComplexF64}, Int64, Cint, Cint, Ref{HHSolveStats}),
+                    param.handle, Bc, X, nrhs, 0, HH_HOST, st))
+        param.setupTime += st[].setup_time
+        param.solveTime += st[].solve_time
+        param.nPrec += st[].n_prec
+    end
+    return (size(B, 2) == 1 ? vec(X) : X), param
+end
+
+function clear!(s::ShiftedLaplacianMultigridSolverAMD)
+    s.handle != C_NULL && check(ccall((:hh_solver_clear, LIB), Cint, (Ptr{Cvoid},), s.handle))
+    s.doClear = 0
+end
+
+copySolver(s::ShiftedLaplacianMultigridSolverAMD) =
+    ShiftedLaplacianMultigridSolverAMD(s.helmParam, s.MG, copy(s.shift), s.Krylov, s.inner, 0, s.verbose,
+                                       0.0, 0, 0.0, nothing, C_NULL, s.prec)
+
+end # module

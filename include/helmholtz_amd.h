@@ -124,6 +124,21 @@ int hh_solve(hh_solver* s, const void* B, void* X, int64_t nrhs, int do_transpos
  * and the node counts of level l. */
 int hh_solver_levels(const hh_solver* s, int* levels, int64_t nnodes_of_level[][3], int max_levels);
 
+/* Device memory for callers that keep their blocks resident in HBM (no
+ * counterpart in the reference, whose arrays live in host memory).  kind:
+ * 0 host->device, 1 device->host, 2 device->device. */
+int hh_device_alloc(int64_t bytes, void** ptr);
+int hh_device_free(void* ptr);
+int hh_memcpy(void* dst, const void* src, int64_t bytes, int kind);
+
+/* Measurement hook (no counterpart in the reference): launches one operator
+ * kernel `reps` times on the operator's stream between two HIP events and
+ * returns the average duration per launch.  kernel: 0 Y = A X, 1 Y = B - A X,
+ * 2 one damped-Jacobi sweep Y = X + 0.8 D^-1 (B - A X); A = H + i shift w^2 m.
+ * X, B, Y are device blocks of nrhs columns in precision prec. */
+int hh_op_time_kernel(hh_op* op, int kernel, double shift, const void* X, const void* B, void* Y, int64_t nrhs,
+                      int prec, int reps, double* avg_ms);
+
 #ifdef __cplusplus
 }
 #endif

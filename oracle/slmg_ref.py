@@ -120,21 +120,23 @@ def jacobi(lev: Level, x, b, w, sweeps):
     return x
 
 
-def vcycle(hier, l, b, w=0.8, pre=2, post=2, cycle="V", x0=None):
-    """One V- or W-cycle for A_l x = b from x0 (zero if None).  The W-cycle
-    treats the coarse problem with two cycles, the second starting from the
-    first's result (skipped when the coarse level is the exact coarsest solve)."""
+def vcycle(hier, l, b, w=0.8, pre=2, post=2, cycle="V"):
+    """One V- or W-cycle for A_l x = b from x = 0.
+
+    W: the coarse-grid correction is applied twice per visit, the second time
+    from the residual recomputed on this level.  With rediscretised (not
+    Galerkin) coarse operators this differs from iterating the coarse cycle
+    on the restricted residual; near kh ~ 2 the coarse two-grid cycle
+    diverges, and only the fine-residual form stays convergent."""
     lev = hier[l]
     if l == len(hier) - 1:
         return lev.lu.solve(b)
-    x = np.zeros_like(b) if x0 is None else x0
+    x = np.zeros_like(b)
     x = jacobi(lev, x, b, w, pre)
-    r = b - lev.A @ x
-    bc = lev.R @ r
-    xc = vcycle(hier, l + 1, bc, w, pre, post, cycle)
-    if cycle == "W" and l + 1 < len(hier) - 1:
-        xc = vcycle(hier, l + 1, bc, w, pre, post, cycle, xc)
-    x = x + lev.P @ xc
+    for _ in range(2 if cycle == "W" else 1):
+        r = b - lev.A @ x
+        xc = vcycle(hier, l + 1, lev.R @ r, w, pre, post, cycle)
+        x = x + lev.P @ xc
     x = jacobi(lev, x, b, w, post)
     return x
 

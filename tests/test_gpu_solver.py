@@ -84,6 +84,30 @@ def test_multi_rhs_random(cycle, levels):
     assert all(st == "converged" for st in s.last_stats["status"])
 
 
+def test_w_cycle_iterations_match_oracle():
+    """8 points per wavelength, 4-level W(2,2): same iteration count as the
+    oracle's textbook cycle (fine-residual double correction)."""
+    nodes = (257, 129)
+    M = hr.get_regular_mesh([0, 10 * 256, 0, 10 * 128], [256, 128])
+    v = media.layered_body_velocity_2d(nodes, seed=21)
+    m = (1.0 / v**2).ravel(order="F")
+    w = media.omega_for_ppw(1500.0, 10.0, 8)
+    H, g = hr.get_helmholtz_operator(M, m, w, 0.01 * w * np.ones(m.size), True, [10, 10], w, True)
+    g = g.ravel(order="F")
+    q = np.zeros(m.size, complex)
+    q[hr.loc2cs(M.nodes, [129, 2]) - 1] = 1.0 / np.linalg.norm(M.h) ** 2
+    s = _solver(M, m, g, w, True, 4, 0.5, cycle="W")
+    x, s = ha.solveLinearSystem(None, q, s)
+    p = hr.HelmholtzParam(M, g, m, w, True, True)
+    hier = sr.build_hierarchy(p, 4, 0.5)
+    xo, ito, _ = sr.fgmres(lambda v: H @ v, q, lambda v: sr.vcycle(hier, 0, v, 0.8, 2, 2, "W"), 10, 100, 1e-6)
+    it = s.last_stats["iters"][0]
+    res = np.linalg.norm(H @ x - q) / np.linalg.norm(q)
+    print(f"W-cycle 257x129 gpu iters={it} oracle={ito} res={res:.2e}")
+    assert res <= 1e-6
+    assert abs(it - ito) <= 1
+
+
 def test_batched_equals_single():
     """One call with 4 RHS gives the same iterates as 4 single-RHS calls."""
     M, m, g, w = small_case((65, 65), seed=4, neumann=False)
