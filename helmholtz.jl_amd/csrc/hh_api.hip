@@ -7,7 +7,11 @@
 // The hierarchy is the rediscretised one of the reference's commented
 // geometric path (:51-62): shift folded into gamma, m and gamma restricted by
 // full weighting, the operator rebuilt matrix-free on every coarse mesh.
+#include <algorithm>
 #include <chrono>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -18,6 +22,7 @@
 
 #include "../../include/helmholtz_amd.h"
 #include "hh_kernels.h"
+#include "hh_unet.h"
 
 using namespace hh;
 
@@ -109,6 +114,28 @@ __global__ void k_sqrt_arr(const double* a, double* b, int n) {
 
 }  // namespace
 
+void hh_set_last_error(const char* msg) { g_err = msg ? msg : ""; }
+
+namespace {
+// per-level network features: k^2 h^2 = Re(w)^2 m h_x^2 and gamma / Re(w)
+__global__ void k_level_feats(const double* __restrict__ m, const double* __restrict__ gam, int64_t N, double wre,
+                              double hx2, float* __restrict__ kh2, float* __restrict__ gw) {
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= N) return;
+  kh2[p] = (float)(wre * wre * m[p] * hx2);
+  gw[p] = (float)(gam[p] / wre);
+}
+__global__ void k_fill_f(float* __restrict__ a, float v, int64_t n) {
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p < n) a[p] = v;
+}
+__global__ void k_zero_cols(hh::cx<double>* __restrict__ x, int64_t N, const int* __restrict__ bad) {
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= N) return;
+  if (bad[blockIdx.y]) x[(int64_t)blockIdx.y * N + p] = hh::mk<double>(0.0, 0.0);
+}
+}  // namespace
+
 // ============================================================== operator
 struct hh_op {
   int dim = 2;
@@ -195,8 +222,36 @@ struct hh_solver {
   double* h_res = nullptr;
   double setup_time = 0, solve_time = 0;
   int64_t n_prec = 0;
+  // learned coarse-grid correction
+  hh_unet* net = nullptr;
+  int learn_level = -1;
+  bool use_learned = false;
+  bool fell_back = false;
+  float* feat_kh2 = nullptr;
+  float* feat_gw = nullptr;
+  float* nn_scale = nullptr;
+  float h2_learn = 1.f;
+  // launch shapes picked by the autotuner (0 = heuristic), per level
+  int rows_tuned[16] = {0};
+  int tuned_for = 0;
+  // hipGraph per Arnoldi step j (replayed from the second restart cycle on)
+  hipGraphExec_t gexec[33] = {};
+  int gkey_nrhs = 0;
+  bool gkey_learned = false;
+  void drop_graphs() {
+    for (auto& g : gexec)
+      if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+  }
+  // capture of level-l right-hand sides for training pairs
+  int cap_level = -1;
+  int64_t cap_max = 0, cap_count = 0;
+  int cap_lo = 0, cap_hi = 0;
+  bool cap_now = false;
+  cx<float>* cap_buf = nullptr;
 
   void free_work() {
+    drop_graphs();
+    tuned_for = 0;
     for (auto& L : lv) {
       dfree(L.xa); dfree(L.xb); dfree(L.b); dfree(L.r);
       L.xa = L.xb = L.b = L.r = L.cur = nullptr;
@@ -222,7 +277,15 @@ struct hh_solver {
     coarse_aug = nullptr; coarse_d = nullptr; piv = nullptr; fac = nullptr; pinv = nullptr;
     built = false;
   }
-  ~hh_solver() { free_hier(); }
+  void free_learned() {
+    dfree(feat_kh2); dfree(feat_gw); dfree(nn_scale);
+    feat_kh2 = nullptr; feat_gw = nullptr; nn_scale = nullptr;
+  }
+  ~hh_solver() {
+    free_hier();
+    free_learned();
+    dfree(cap_buf);
+  }
 };
 
 namespace {
@@ -288,6 +351,21 @@ void build_hierarchy(hh_solver* S) {
   S->built = true;
 }
 
+// network features of the learned level (after the hierarchy exists)
+void setup_learned(hh_solver* S) {
+  if (!S->net || S->feat_kh2) return;
+  if (S->learn_level < 1 || S->learn_level >= (int)S->lv.size())
+    throw HHError(HH_ERR_ARG, "learned level must be in 1..levels-1");
+  Level& L = S->lv[S->learn_level];
+  S->feat_kh2 = dalloc<float>(L.g.N);
+  S->feat_gw = dalloc<float>(L.g.N);
+  S->nn_scale = dalloc<float>(65536);
+  S->h2_learn = (float)(L.h[0] * L.h[0]);
+  hipLaunchKernelGGL(k_level_feats, dim3((unsigned)((L.g.N + 255) / 256)), dim3(256), 0, S->op->stream, L.m, L.gam,
+                     L.g.N, S->op->wre, L.h[0] * L.h[0], S->feat_kh2, S->feat_gw);
+  HIPCK(hipGetLastError());
+}
+
 const cx<double>* coarse_inv(hh_solver* S) {
   const int64_t n = S->lv.back().g.N;
   return S->coarse_aug + n * n;
@@ -345,6 +423,70 @@ void ensure_work(hh_solver* S, int nrhs) {
   S->cap = nrhs;
 }
 
+// ---------------------------------------------------------------- autotuner
+// Picks, per level, the run length each stencil thread marches (rows per
+// thread) for the damped-Jacobi sweep at this nrhs, by timing the candidates
+// with HIP events on the solver's stream.  Results are cached per process,
+// keyed by (grid, nrhs, precision).
+std::map<std::tuple<int, int, int, int, int>, int> g_tune_cache;
+std::mutex g_tune_mu;
+
+template <typename T>
+int tune_level(hh_solver* S, Level& L, int nrhs) {
+  hipStream_t st = S->op->stream;
+  const auto key = std::make_tuple(L.g.n1, L.g.n2, L.g.n3, nrhs, (int)sizeof(T));
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end()) return it->second;
+  }
+  const int64_t n = L.g.N * (int64_t)nrhs;
+  launch_zero<T>((cx<T>*)L.xa, n, st);
+  launch_zero<T>((cx<T>*)L.r, n, st);
+  hipEvent_t e0, e1;
+  HIPCK(hipEventCreate(&e0));
+  HIPCK(hipEventCreate(&e1));
+  int best = 0;
+  float best_ms = 1e30f;
+  const int cand[] = {2, 4, 8, 16, 32};
+  for (int rows : cand) {
+    const int len = L.g.dim == 3 ? L.g.n3 : L.g.n2;
+    if (rows > len) continue;
+    auto go = [&] {
+      launch_stencil<T, T, T, M_JACOBI>(L.g, nrhs, (const cx<T>*)L.xa, (const cx<T>*)L.r, (cx<T>*)L.xb,
+                                        (const cx<T>*)L.d, nullptr, mk<double>(0, 0), 0.8, nullptr, st, rows);
+    };
+    go();
+    HIPCK(hipEventRecord(e0, st));
+    for (int r = 0; r < 3; ++r) go();
+    HIPCK(hipEventRecord(e1, st));
+    HIPCK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCK(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best_ms) best_ms = ms, best = rows;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  HIPCK(hipGetLastError());
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune_cache[key] = best;
+  return best;
+}
+
+void autotune(hh_solver* S, int nrhs) {
+  if (S->tuned_for == nrhs) return;
+  for (size_t l = 0; l + 1 < S->lv.size() && l < 16; ++l) {
+    Level& L = S->lv[l];
+    // small launches are latency-bound: keep the heuristic there
+    if (L.g.N * (int64_t)nrhs < (1 << 20)) {
+      S->rows_tuned[l] = 0;
+      continue;
+    }
+    S->rows_tuned[l] = S->p.prec == HH_C64 ? tune_level<float>(S, L, nrhs) : tune_level<double>(S, L, nrhs);
+  }
+  S->tuned_for = nrhs;
+}
+
 // ---------------------------------------------------------------- MG cycle
 template <typename T>
 struct Cycle {
@@ -354,12 +496,16 @@ struct Cycle {
   hipStream_t st;
 
   void* other(Level& L) { return L.cur == L.xa ? L.xb : L.xa; }
+  int rows_of(const Level& L) const {
+    const size_t l = &L - &S->lv[0];
+    return l < 16 ? S->rows_tuned[l] : 0;
+  }
 
   void sweeps(Level& L, const cx<T>* b, int n) {
     for (int s = 0; s < n; ++s) {
       void* o = other(L);
       launch_stencil<T, T, T, M_JACOBI>(L.g, nrhs, (const cx<T>*)L.cur, b, (cx<T>*)o, (const cx<T>*)L.d, nullptr,
-                                        mk<double>(0, 0), S->p.relax, skip, st);
+                                        mk<double>(0, 0), S->p.relax, skip, st, rows_of(L));
       L.cur = o;
     }
   }
@@ -369,6 +515,27 @@ struct Cycle {
   void run(int l, const cx<T>* b, bool zero_init, void* first) {
     Level& L = S->lv[l];
     const int nl = (int)S->lv.size();
+    if (std::is_same<T, float>::value) {
+      if (S->cap_now && l == S->cap_level && zero_init) {
+        S->cap_now = false;
+        if (S->cap_count + nrhs <= S->cap_max) {
+          HIPCK(hipMemcpyAsync(S->cap_buf + S->cap_count * L.g.N, b, sizeof(cx<float>) * L.g.N * nrhs,
+                               hipMemcpyDeviceToDevice, st));
+          S->cap_count += nrhs;
+        }
+      }
+      if (S->use_learned && l == S->learn_level) {
+        L.cur = first ? first : L.xa;
+        hh_unet* u = S->net;
+        u->ensure_stage(L.g.N * nrhs);
+        unet_prep_input(nrhs, L.g.N, (const cx<float>*)b, S->feat_kh2, S->feat_gw, u->x16, S->nn_scale, skip, st);
+        u->net.stream = st;
+        unet_forward(u->net, nrhs, L.g.n1, L.g.n2, u->x16, u->y, false);
+        u->net.stream = u->stream;
+        unet_post_output(nrhs, L.g.N, u->y, S->nn_scale, S->h2_learn, (cx<float>*)L.cur, skip, st);
+        return;
+      }
+    }
     if (l == nl - 1) {
       if (!L.cur) L.cur = first ? first : L.xa;
       launch_dense_apply<T>(coarse_inv(S), L.g.N, nrhs, b, (cx<T>*)L.cur, skip, st);
@@ -381,7 +548,7 @@ struct Cycle {
         launch_jacobi0<T>(L.g, nrhs, b, (cx<T>*)L.cur, (const cx<T>*)L.d, S->p.relax, skip, st);
         sweeps(L, b, pre - 1);
       } else {
-        HIPCK(hipMemsetAsync(L.cur, 0, sizeof(cx<T>) * L.g.N * nrhs, st));
+        launch_zero<T>((cx<T>*)L.cur, L.g.N * nrhs, st);
       }
     } else {
       sweeps(L, b, pre);
@@ -393,7 +560,7 @@ struct Cycle {
     const int ncorr = S->p.cycle == 'W' ? 2 : 1;
     for (int c = 0; c < ncorr; ++c) {
       launch_stencil<T, T, T, M_RESID>(L.g, nrhs, (const cx<T>*)L.cur, b, (cx<T>*)L.r, (const cx<T>*)L.d, nullptr,
-                                       mk<double>(0, 0), 0.0, skip, st);
+                                       mk<double>(0, 0), 0.0, skip, st, rows_of(L));
       launch_restrict<T>(L.g, C.g, nrhs, (const cx<T>*)L.r, (cx<T>*)C.b, skip, st);
       C.cur = nullptr;
       run(l + 1, (const cx<T>*)C.b, true, nullptr);
@@ -467,8 +634,11 @@ void run_fgmres(hh_solver* S, int nrhs, hh_solve_stats* stats) {
   for (int b = 0; b < nrhs; ++b) conv[b] = !(bn[b] > 0.0);
   HIPCK(hipMemsetAsync(S->xk, 0, sizeof(cx<double>) * blk, st));
   HIPCK(hipMemcpyAsync(S->wk, S->bk, sizeof(cx<double>) * blk, hipMemcpyDeviceToDevice, st));
-  std::vector<double> res(nrhs, 1.0);
+  std::vector<double> res(nrhs, 1.0), prev(nrhs, 1.0);
   int64_t nprec = 0;
+  S->use_learned = S->net != nullptr && S->p.prec == HH_C64;
+  S->fell_back = false;
+  if (S->use_learned) setup_learned(S);
   for (int outer = 0; outer < S->p.max_outer; ++outer) {
     HIPCK(hipMemcpyAsync(S->conv_d, conv.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, st));
     kr_start_impl(w, S->wk, S->V, tol, S->conv_d, S->n2tmp, st);
@@ -483,15 +653,39 @@ void run_fgmres(hh_solver* S, int nrhs, hh_solve_stats* stats) {
     for (int j = 0; j < K; ++j) {
       const cx<double>* vj = S->V + (int64_t)j * blk;
       void* zj = (char*)S->Z + (size_t)j * blk * zes;
-      if (S->p.prec == HH_C64) {
-        precondition<float>(S, nrhs, vj, zj, w.done, st);
-        launch_stencil<float, double, double, M_APPLY>(S->op->g, nrhs, (const cx<float>*)zj, nullptr, S->wk,
-                                                       S->op->d64, nullptr, mk<double>(0, 0), 0.0, w.done, st);
+      S->cap_now = S->cap_buf && outer == 0 && j >= S->cap_lo && j < S->cap_hi && active == nrhs;
+      auto body = [&] {
+        if (S->p.prec == HH_C64) {
+          precondition<float>(S, nrhs, vj, zj, w.done, st);
+          launch_stencil<float, double, double, M_APPLY>(S->op->g, nrhs, (const cx<float>*)zj, nullptr, S->wk,
+                                                         S->op->d64, nullptr, mk<double>(0, 0), 0.0, w.done, st);
+        } else {
+          precondition<double>(S, nrhs, vj, zj, w.done, st);
+          h_apply(S, nrhs, (const cx<double>*)zj, S->wk, w.done, st);
+        }
+        kr_arnoldi_impl(w, S->V, S->wk, j, tol, S->skip_copy, st);
+      };
+      // hipGraph: the first restart cycle runs eagerly (it allocates the
+      // network workspace and may capture training data); later cycles replay
+      // one captured graph per Arnoldi step j.
+      if (S->p.graph && outer >= 1 && !S->cap_now) {
+        if (S->gkey_nrhs != nrhs || S->gkey_learned != S->use_learned) {
+          S->drop_graphs();
+          S->gkey_nrhs = nrhs;
+          S->gkey_learned = S->use_learned;
+        }
+        if (!S->gexec[j]) {
+          hipGraph_t g = nullptr;
+          HIPCK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+          body();
+          HIPCK(hipStreamEndCapture(st, &g));
+          HIPCK(hipGraphInstantiate(&S->gexec[j], g, nullptr, nullptr, 0));
+          (void)hipGraphDestroy(g);
+        }
+        HIPCK(hipGraphLaunch(S->gexec[j], st));
       } else {
-        precondition<double>(S, nrhs, vj, zj, w.done, st);
-        h_apply(S, nrhs, (const cx<double>*)zj, S->wk, w.done, st);
+        body();
       }
-      kr_arnoldi_impl(w, S->V, S->wk, j, tol, S->skip_copy, st);
       ++nprec;
       HIPCK(hipMemcpyAsync(S->h_done, w.done, sizeof(int) * nrhs, hipMemcpyDeviceToHost, st));
       HIPCK(hipStreamSynchronize(st));
@@ -511,12 +705,33 @@ void run_fgmres(hh_solver* S, int nrhs, hh_solve_stats* stats) {
     HIPCK(hipStreamSynchronize(st));
     HIPCK(hipGetLastError());
     int nconv = 0;
+    bool stagnated = false;
+    std::vector<int> bad(nrhs, 0);
     for (int b = 0; b < nrhs; ++b) {
       iters[b] += conv[b] ? 0 : S->h_kact[b];
       res[b] = S->h_res[b];
+      if (!conv[b]) {
+        if (!(res[b] == res[b])) bad[b] = 1, stagnated = true;
+        else if (res[b] > 0.9 * prev[b]) stagnated = true;
+      }
+    }
+    if (S->use_learned && stagnated) {
+      // the learned correction stopped helping: fall back to the plain cycle;
+      // columns it drove to non-finite values restart from zero
+      S->use_learned = false;
+      S->fell_back = true;
+      if (S->p.verbose) fprintf(stderr, "[helmholtz_amd] learned correction stagnated; plain cycle from restart %d\n", outer);
+      HIPCK(hipMemcpyAsync(S->conv_d, bad.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, st));
+      hipLaunchKernelGGL(k_zero_cols, dim3((unsigned)((N + 255) / 256), nrhs), dim3(256), 0, st, S->xk, N, S->conv_d);
+      h_resid(S, nrhs, S->xk, S->bk, S->wk, st);
+      for (int b = 0; b < nrhs; ++b)
+        if (bad[b]) res[b] = 1.0;
+    }
+    for (int b = 0; b < nrhs; ++b) {
       if (!conv[b] && res[b] <= tol) conv[b] = 1;
       if (!(res[b] == res[b])) conv[b] = 1;  // non-finite: stop iterating it
       nconv += conv[b];
+      prev[b] = res[b];
     }
     if (S->p.verbose) fprintf(stderr, "[helmholtz_amd] restart %d: %d/%d RHS converged\n", outer, nconv, nrhs);
     if (nconv == nrhs) break;
@@ -543,6 +758,8 @@ void run_fgmres(hh_solver* S, int nrhs, hh_solve_stats* stats) {
     stats->max_iters = mx;
     stats->n_converged = nc;
     stats->n_prec = nprec;
+    stats->learned_used = S->net != nullptr && S->p.prec == HH_C64;
+    stats->fell_back = S->fell_back;
   }
   bool warn = false;
   for (int b = 0; b < nrhs; ++b) warn |= (bn[b] > 0.0 && !(res[b] <= tol));
@@ -708,6 +925,7 @@ int hh_solve(hh_solver* s, const void* B, void* X, int64_t nrhs, int do_transpos
     double t0 = now_s();
     if (!s->built) build_hierarchy(s);
     ensure_work(s, (int)nrhs);
+    if (s->p.autotune) autotune(s, (int)nrhs);
     double t1 = now_s();
     s->setup_time += t1 - t0;
     if (stats) stats->setup_time = t1 - t0;
@@ -841,6 +1059,134 @@ int hh_op_time_kernel(hh_op* op, int kernel, double shift, const void* X, const 
     (void)hipEventDestroy(e1);
     HIPCK(hipGetLastError());
     *avg_ms = (double)ms / reps;
+  });
+}
+
+int hh_solver_attach_unet(hh_solver* s, hh_unet* net, int level) {
+  return guarded([&] {
+    if (!s) throw HHError(HH_ERR_ARG, "null argument");
+    if (net && s->p.prec != HH_C64) throw HHError(HH_ERR_ARG, "the learned correction needs the complex64 preconditioner");
+    if (net && (level < 1 || level >= s->p.levels)) throw HHError(HH_ERR_ARG, "level must be in 1..levels-1");
+    if (net && s->op->dim != 2) throw HHError(HH_ERR_UNSUPPORTED, "the learned correction is 2-D only");
+    if (net) {
+      int64_t n1 = s->op->nn[0], n2 = s->op->nn[1];
+      for (int l = 0; l < level; ++l) n1 = (n1 - 1) / 2 + 1, n2 = (n2 - 1) / 2 + 1;
+      if (n1 < 9 || n2 < 9 || (n1 - 1) % 8 || (n2 - 1) % 8)
+        throw HHError(HH_ERR_ARG, "the learned level's grid needs 8k + 1 nodes per axis");
+    }
+    s->free_learned();
+    s->net = net;
+    s->learn_level = net ? level : -1;
+  });
+}
+
+int hh_solver_capture_begin(hh_solver* s, int level, int64_t max_samples, int iter_lo, int iter_hi) {
+  return guarded([&] {
+    if (!s) throw HHError(HH_ERR_ARG, "null argument");
+    if (s->p.prec != HH_C64) throw HHError(HH_ERR_ARG, "capture needs the complex64 preconditioner");
+    if (level < 1 || level >= s->p.levels || max_samples < 1) throw HHError(HH_ERR_ARG, "bad level or size");
+    if (!s->built) build_hierarchy(s);
+    dfree(s->cap_buf);
+    s->cap_buf = nullptr;
+    s->cap_buf = dalloc<cx<float>>((size_t)max_samples * s->lv[level].g.N);
+    s->cap_level = level;
+    s->cap_max = max_samples;
+    s->cap_count = 0;
+    s->cap_lo = iter_lo;
+    s->cap_hi = iter_hi;
+  });
+}
+
+int hh_solver_capture_count(const hh_solver* s, int64_t* n) {
+  return guarded([&] {
+    if (!s || !n) throw HHError(HH_ERR_ARG, "null argument");
+    *n = s->cap_count;
+  });
+}
+
+int hh_dataset_add_from_solver(hh_dataset* d, hh_solver* s, int sweeps) {
+  // Targets e = A_l^-1 r for the captured level-l residuals.  A multigrid
+  // cycle started at a coarse level diverges there (kh ~ 2-4), so each block
+  // is solved to 1e-6 by FGMRES(10) on the level's shifted operator, with a
+  // W(2,2) preconditioner whose hierarchy carries a further 0.5 shift.
+  return guarded([&] {
+    if (!d || !s || sweeps < 1) throw HHError(HH_ERR_ARG, "bad argument");
+    if (!s->cap_buf || s->cap_count == 0) throw HHError(HH_ERR_ARG, "solver has no captured samples");
+    Level& L = s->lv[s->cap_level];
+    if (L.g.n1 != d->nx || L.g.n2 != d->ny || L.g.dim != 2)
+      throw HHError(HH_ERR_ARG, "dataset grid does not match the captured level");
+    hipStream_t st = s->op->stream;
+    HIPCK(hipStreamSynchronize(st));
+    const int64_t N = L.g.N;
+    std::vector<double> hm(N), hg(N);
+    HIPCK(hipMemcpy(hm.data(), L.m, sizeof(double) * N, hipMemcpyDeviceToHost));
+    HIPCK(hipMemcpy(hg.data(), L.gam, sizeof(double) * N, hipMemcpyDeviceToHost));
+    const int64_t nn[3] = {L.g.n1, L.g.n2, 1};
+    hh_op* op = nullptr;
+    int rc = hh_op_create(2, nn, L.h, hm.data(), hg.data(), s->op->wre, s->op->wim, s->op->neumann_top,
+                          s->op->sommerfeld, s->op->bc, &op);
+    if (rc != HH_OK) throw HHError(rc, g_err);
+    hh_solver* ts = nullptr;
+    hh_mg_params tp{};
+    tp.levels = (int)s->lv.size() - s->cap_level;
+    tp.pre = 2;
+    tp.post = 2;
+    tp.relax = 0.8;
+    tp.cycle = 'W';
+    tp.max_outer = std::max(4, sweeps);
+    tp.tol = 1e-6;
+    tp.inner = 10;
+    tp.krylov = HH_FGMRES;
+    tp.prec = HH_C128;
+    std::vector<double> shifts(tp.levels, 0.5);
+    cx<double>* rb = nullptr;
+    cx<double>* xb = nullptr;
+    try {
+      rc = hh_solver_create(op, &tp, shifts.data(), &ts);
+      if (rc != HH_OK) throw HHError(rc, g_err);
+      const int chunk = 256;
+      rb = dalloc<cx<double>>((size_t)N * chunk);
+      xb = dalloc<cx<double>>((size_t)N * chunk);
+      // features of this level (network inputs) once
+      if (!s->feat_kh2) {
+        s->feat_kh2 = dalloc<float>(N);
+        s->feat_gw = dalloc<float>(N);
+        hipLaunchKernelGGL(k_level_feats, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, L.m, L.gam, N,
+                           s->op->wre, L.h[0] * L.h[0], s->feat_kh2, s->feat_gw);
+      }
+      for (int64_t k0 = 0; k0 < s->cap_count && d->n < d->cap;) {
+        const int m = (int)std::min<int64_t>({(int64_t)chunk, s->cap_count - k0, d->cap - d->n});
+        launch_cast<float, double>(s->cap_buf + k0 * N, rb, N * m, st);
+        HIPCK(hipStreamSynchronize(st));
+        hh_solve_stats stt{};
+        rc = hh_solve(ts, rb, xb, m, 0, HH_DEVICE, &stt);
+        if (rc != HH_OK) throw HHError(rc, g_err);
+        HIPCK(hipMemcpyAsync(d->r + d->n * N, s->cap_buf + k0 * N, sizeof(cx<float>) * N * m,
+                             hipMemcpyDeviceToDevice, st));
+        launch_cast<double, float>(xb, d->e + d->n * N, N * m, st);
+        for (int k = 0; k < m; ++k) {
+          HIPCK(hipMemcpyAsync(d->kh2 + (d->n + k) * N, s->feat_kh2, sizeof(float) * N, hipMemcpyDeviceToDevice, st));
+          HIPCK(hipMemcpyAsync(d->gw + (d->n + k) * N, s->feat_gw, sizeof(float) * N, hipMemcpyDeviceToDevice, st));
+        }
+        hipLaunchKernelGGL(k_fill_f, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, d->h2 + d->n,
+                           (float)(L.h[0] * L.h[0]), (int64_t)m);
+        HIPCK(hipStreamSynchronize(st));
+        d->n += m;
+        k0 += m;
+      }
+    } catch (...) {
+      dfree(rb);
+      dfree(xb);
+      if (ts) hh_solver_destroy(ts);
+      hh_op_destroy(op);
+      throw;
+    }
+    dfree(rb);
+    dfree(xb);
+    hh_solver_destroy(ts);
+    hh_op_destroy(op);
+    if (s->net == nullptr) s->free_learned();
+    HIPCK(hipGetLastError());
   });
 }
 

@@ -7,9 +7,13 @@ namespace hh {
 enum { M_APPLY = 0, M_RESID = 1, M_JACOBI = 2 };
 
 // y = A x | y = b - A x | y = x + wj D^-1 (b - A x);  A = Lap + diag(d + sfac m)
+// rows_override > 0 fixes the run length each thread marches (autotuner), else a heuristic
 template <typename TI, typename TO, typename TD, int MODE>
 void launch_stencil(const Grid& g, int nrhs, const cx<TI>* x, const cx<TO>* b, cx<TO>* y, const cx<TD>* d,
-                    const double* m, cx<double> sfac, double wj, const int* skip, hipStream_t s);
+                    const double* m, cx<double> sfac, double wj, const int* skip, hipStream_t s,
+                    int rows_override = 0);
+template <typename T>
+void launch_zero(cx<T>* a, int64_t n, hipStream_t s);
 template <typename T>
 void launch_jacobi0(const Grid& g, int nrhs, const cx<T>* b, cx<T>* x, const cx<T>* d, double wj, const int* skip,
                     hipStream_t s);

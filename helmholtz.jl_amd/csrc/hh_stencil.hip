@@ -253,6 +253,12 @@ __global__ __launch_bounds__(256) void k_prolong_add(Grid gf, Grid gc, const cx<
   xf[o] = xf[o] + acc;
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void k_zero(cx<T>* __restrict__ a, int64_t n) {
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p < n) a[p] = mk<T>(0, 0);
+}
+
 template <typename A, typename B>
 __global__ __launch_bounds__(256) void k_cast(const cx<A>* __restrict__ a, cx<B>* __restrict__ b, int64_t n) {
   const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -466,9 +472,9 @@ static inline int pick_rows(const Grid& g, int nrhs) {
 
 template <typename TI, typename TO, typename TD, int MODE>
 void launch_stencil(const Grid& g, int nrhs, const cx<TI>* x, const cx<TO>* b, cx<TO>* y, const cx<TD>* d,
-                    const double* m, cx<double> sfac, double wj, const int* skip, hipStream_t s) {
+                    const double* m, cx<double> sfac, double wj, const int* skip, hipStream_t s, int rows_override) {
   int ytiles;
-  const int rows = pick_rows(g, nrhs);
+  const int rows = rows_override > 0 ? rows_override : pick_rows(g, nrhs);
   dim3 grid = stencil_grid(g, nrhs, rows, &ytiles);
   if (g.dim == 2)
     hipLaunchKernelGGL((k_stencil2d<TI, TO, TD, MODE>), grid, dim3(64, 4), 0, s, g, x, b, y, d, m, sfac, (TD)wj,
@@ -481,28 +487,28 @@ void launch_stencil(const Grid& g, int nrhs, const cx<TI>* x, const cx<TO>* b, c
 template void launch_stencil<double, double, double, M_APPLY>(const Grid&, int, const cx<double>*,
                                                                const cx<double>*, cx<double>*,
                                                                const cx<double>*, const double*, cx<double>,
-                                                               double, const int*, hipStream_t);
+                                                               double, const int*, hipStream_t, int);
 template void launch_stencil<double, double, double, M_RESID>(const Grid&, int, const cx<double>*,
                                                                const cx<double>*, cx<double>*,
                                                                const cx<double>*, const double*, cx<double>,
-                                                               double, const int*, hipStream_t);
+                                                               double, const int*, hipStream_t, int);
 template void launch_stencil<double, double, double, M_JACOBI>(const Grid&, int, const cx<double>*,
                                                                 const cx<double>*, cx<double>*,
                                                                 const cx<double>*, const double*, cx<double>,
-                                                                double, const int*, hipStream_t);
+                                                                double, const int*, hipStream_t, int);
 template void launch_stencil<float, float, float, M_APPLY>(const Grid&, int, const cx<float>*, const cx<float>*,
                                                             cx<float>*, const cx<float>*, const double*,
-                                                            cx<double>, double, const int*, hipStream_t);
+                                                            cx<double>, double, const int*, hipStream_t, int);
 template void launch_stencil<float, float, float, M_RESID>(const Grid&, int, const cx<float>*, const cx<float>*,
                                                             cx<float>*, const cx<float>*, const double*,
-                                                            cx<double>, double, const int*, hipStream_t);
+                                                            cx<double>, double, const int*, hipStream_t, int);
 template void launch_stencil<float, float, float, M_JACOBI>(const Grid&, int, const cx<float>*, const cx<float>*,
                                                              cx<float>*, const cx<float>*, const double*,
-                                                             cx<double>, double, const int*, hipStream_t);
+                                                             cx<double>, double, const int*, hipStream_t, int);
 template void launch_stencil<float, double, double, M_APPLY>(const Grid&, int, const cx<float>*,
                                                               const cx<double>*, cx<double>*,
                                                               const cx<double>*, const double*, cx<double>,
-                                                              double, const int*, hipStream_t);
+                                                              double, const int*, hipStream_t, int);
 
 template <typename T>
 void launch_jacobi0(const Grid& g, int nrhs, const cx<T>* b, cx<T>* x, const cx<T>* d, double wj, const int* skip,
@@ -540,6 +546,14 @@ template void launch_prolong_add<double>(const Grid&, const Grid&, int, const cx
                                          const int*, hipStream_t);
 template void launch_prolong_add<float>(const Grid&, const Grid&, int, const cx<float>*, cx<float>*, const int*,
                                         hipStream_t);
+
+template <typename T>
+void launch_zero(cx<T>* a, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL((k_zero<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, n);
+}
+template void launch_zero<double>(cx<double>*, int64_t, hipStream_t);
+template void launch_zero<float>(cx<float>*, int64_t, hipStream_t);
 
 template <typename A, typename B>
 void launch_cast(const cx<A>* a, cx<B>* b, int64_t n, hipStream_t s) {

@@ -1,0 +1,716 @@
+// U-Net coarse-grid correction: BF16 activations/weights with FP32
+// accumulation on MFMA (v_mfma_f32_16x16x16_bf16), hand-written forward,
+// backward and Adam.
+//
+// Layout: activations NHWC with the pixel index x-fastest (p = i + nx*j, the
+// solver's nodal order) and channels contiguous (multiples of 16).
+// 3x3 convolution = implicit GEMM: out[p][co] = sum_{tap,ci} W[tap][co][ci]
+// in[p + off(tap)][ci] (zero padding).  One wavefront computes 16 pixels x all
+// output channels (the A operand, 16 pixels x 16 input channels, is loaded once
+// and reused for every 16-channel output block).  Backward-data is the same
+// kernel on the output gradient with W transposed and taps flipped; the
+// weight gradient is a split-K MFMA over pixels with per-chunk partials summed
+// in a fixed order (deterministic).  Down-sampling is nodal full weighting
+// ([1 2 1]/4 per axis, stride 2, zero padding) and up-sampling nodal
+// bilinear interpolation; their adjoints are each other up to a factor 4.
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+#include "hh_unet.h"
+
+namespace hh {
+
+typedef short s4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+__device__ __host__ __forceinline__ bf16 f2bf(float f) {
+  unsigned u = __builtin_bit_cast(unsigned, f);
+  u = u + 0x7fffu + ((u >> 16) & 1u);
+  return (bf16)(u >> 16);
+}
+__device__ __host__ __forceinline__ float bf2f(bf16 h) {
+  unsigned u = ((unsigned)h) << 16;
+  return __builtin_bit_cast(float, u);
+}
+
+struct ConvSpec {
+  int ci, co, level;
+};
+// encoder 16/32/64, bottleneck 64, decoder with skip concatenations
+static const ConvSpec kSpec[UNET_NCONV] = {{16, 16, 0},  {16, 16, 0},  {16, 32, 1}, {32, 32, 1}, {32, 64, 2},
+                                           {64, 64, 2},  {64, 64, 3},  {64, 64, 3}, {128, 64, 2}, {64, 64, 2},
+                                           {96, 32, 1},  {32, 32, 1},  {48, 16, 0}, {16, 16, 0}};
+
+int64_t unet_param_count() {
+  int64_t n = 0;
+  for (int l = 0; l < UNET_NCONV; ++l) n += 9 * kSpec[l].co * kSpec[l].ci + kSpec[l].co;
+  return n + 2 * 16 + 2;
+}
+
+static void ck(hipError_t e, const char* w) {
+  if (e != hipSuccess) throw std::runtime_error(std::string(w) + ": " + hipGetErrorString(e));
+}
+template <typename T>
+static T* dmal(size_t n) {
+  void* p = nullptr;
+  if (hipMalloc(&p, (n ? n : 1) * sizeof(T)) != hipSuccess) {
+    (void)hipGetLastError();
+    throw std::runtime_error("hipMalloc failed in U-Net workspace");
+  }
+  return (T*)p;
+}
+
+// ------------------------------------------------------------------ kernels
+template <int NCO>
+__global__ __launch_bounds__(256) void k_conv3x3(const bf16* __restrict__ in, int cin, const bf16* __restrict__ w,
+                                                 const float* __restrict__ bias, bf16* __restrict__ out, int nx,
+                                                 int ny, int relu) {
+  const int img = blockIdx.y;
+  const int npix = nx * ny;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int tile = blockIdx.x * 64 + wv * 16;
+  if (tile >= npix) return;  // uniform per wavefront
+  constexpr int cout = NCO * 16;
+  const int m = lane & 15;
+  const int kq = (lane >> 4) * 4;
+  const int pa = tile + m;
+  const bool pa_ok = pa < npix;
+  const int ia = pa_ok ? pa % nx : 0, ja = pa_ok ? pa / nx : 0;
+  const bf16* inb = in + (int64_t)img * npix * cin;
+  f4 acc[NCO];
+#pragma unroll
+  for (int c = 0; c < NCO; ++c) acc[c] = f4{0.f, 0.f, 0.f, 0.f};
+  for (int tap = 0; tap < 9; ++tap) {
+    const int ii = ia + tap % 3 - 1, jj = ja + tap / 3 - 1;
+    const bool ok = pa_ok && ii >= 0 && ii < nx && jj >= 0 && jj < ny;
+    const bf16* src = inb + ((int64_t)(ok ? jj : 0) * nx + (ok ? ii : 0)) * cin + kq;
+    const bf16* wtap = w + (int64_t)tap * cout * cin + (int64_t)m * cin + kq;
+    for (int cb = 0; cb < cin; cb += 16) {
+      s4 a = ok ? *(const s4*)(src + cb) : s4{0, 0, 0, 0};
+#pragma unroll
+      for (int c = 0; c < NCO; ++c) {
+        const s4 bv = *(const s4*)(wtap + (int64_t)c * 16 * cin + cb);
+        acc[c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bv, acc[c], 0, 0, 0);
+      }
+    }
+  }
+  bf16* ob = out + (int64_t)img * npix * cout;
+#pragma unroll
+  for (int c = 0; c < NCO; ++c)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int p = tile + kq + i;
+      const int co = c * 16 + m;
+      if (p < npix) {
+        float v = acc[c][i] + (bias ? bias[co] : 0.f);
+        if (relu) v = v > 0.f ? v : 0.f;
+        ob[(int64_t)p * cout + co] = f2bf(v);
+      }
+    }
+}
+
+// part[chunk][tap][co][ci] = sum over the chunk's pixels of dpre[q][co] * in[q + off(tap)][ci]
+__global__ __launch_bounds__(64) void k_conv_wgrad(const bf16* __restrict__ dpre, int cout, const bf16* __restrict__ in,
+                                                   int cin, int nx, int ny, int64_t total, int64_t chunk,
+                                                   float* __restrict__ part) {
+  const int ncib = cin / 16, ncob = cout / 16;
+  int t = blockIdx.y;
+  const int cib = t % ncib;
+  t /= ncib;
+  const int cob = t % ncob;
+  const int tap = t / ncob;
+  const int di = tap % 3 - 1, dj = tap / 3 - 1;
+  const int lane = threadIdx.x;
+  const int m = lane & 15, kq = (lane >> 4) * 4;
+  const int npix = nx * ny;
+  const int64_t q0 = (int64_t)blockIdx.x * chunk;
+  const int64_t q1 = q0 + chunk < total ? q0 + chunk : total;
+  f4 acc = f4{0.f, 0.f, 0.f, 0.f};
+  for (int64_t q = q0; q < q1; q += 16) {
+    s4 a, b;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int64_t qq = q + kq + i;
+      short av = 0, bvv = 0;
+      if (qq < q1) {
+        av = (short)dpre[qq * cout + cob * 16 + m];
+        const int64_t img = qq / npix;
+        const int p = (int)(qq - img * npix);
+        const int i0 = p % nx + di, j0 = p / nx + dj;
+        if (i0 >= 0 && i0 < nx && j0 >= 0 && j0 < ny)
+          bvv = (short)in[(img * npix + (int64_t)j0 * nx + i0) * cin + cib * 16 + m];
+      }
+      a[i] = av;
+      b[i] = bvv;
+    }
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+  }
+  float* pb = part + ((int64_t)blockIdx.x * 9 + tap) * cout * cin;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) pb[(int64_t)(cob * 16 + kq + i) * cin + cib * 16 + m] = acc[i];
+}
+
+// out[k] = sum_c part[c][k] (fixed order)
+__global__ __launch_bounds__(256) void k_sum_parts(const float* __restrict__ part, int nparts, int64_t n,
+                                                   float* __restrict__ out) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (k >= n) return;
+  float s = 0.f;
+  for (int c = 0; c < nparts; ++c) s += part[(int64_t)c * n + k];
+  out[k] = s;
+}
+
+// per-chunk column sums of an NHWC bf16 tensor: part[chunk][c]
+__global__ __launch_bounds__(256) void k_colsum(const bf16* __restrict__ g, int C, int64_t total, int64_t chunk,
+                                                float* __restrict__ part) {
+  __shared__ float sh[256];
+  const int64_t q0 = (int64_t)blockIdx.x * chunk;
+  const int64_t q1 = q0 + chunk < total ? q0 + chunk : total;
+  const int c = threadIdx.x % C;
+  const int rows = 256 / C;
+  const int r0 = threadIdx.x / C;
+  float s = 0.f;
+  if (r0 < rows)
+    for (int64_t q = q0 + r0; q < q1; q += rows) s += bf2f(g[q * C + c]);
+  sh[threadIdx.x] = (r0 < rows) ? s : 0.f;
+  __syncthreads();
+  if (threadIdx.x < C) {
+    float t = 0.f;
+    for (int r = 0; r < rows; ++r) t += sh[r * C + threadIdx.x];
+    part[(int64_t)blockIdx.x * C + threadIdx.x] = t;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_relu_bwd(const bf16* __restrict__ dout, const bf16* __restrict__ out,
+                                                  bf16* __restrict__ dpre, int64_t n) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (k < n) dpre[k] = bf2f(out[k]) > 0.f ? dout[k] : (bf16)0;
+}
+
+// nodal full weighting, stride 2, zero padding; out = scale * R in
+__global__ __launch_bounds__(256) void k_down(const bf16* __restrict__ in, int C, int nx, int ny, bf16* __restrict__ out,
+                                              int ncx, int ncy, int nimg, float scale) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t nc = (int64_t)ncx * ncy;
+  if (k >= nc * nimg * C) return;
+  const int c = (int)(k % C);
+  const int64_t q = k / C;
+  const int64_t img = q / nc;
+  const int pc = (int)(q - img * nc);
+  const int I = pc % ncx, J = pc / ncx;
+  const float wgt[3] = {0.25f, 0.5f, 0.25f};
+  float s = 0.f;
+  for (int b = -1; b <= 1; ++b) {
+    const int j = 2 * J + b;
+    if (j < 0 || j >= ny) continue;
+    for (int a = -1; a <= 1; ++a) {
+      const int i = 2 * I + a;
+      if (i < 0 || i >= nx) continue;
+      s += wgt[a + 1] * wgt[b + 1] * bf2f(in[((img * ny + j) * (int64_t)nx + i) * C + c]);
+    }
+  }
+  out[k] = f2bf(scale * s);
+}
+
+// nodal bilinear prolongation; out (+)= scale * P in
+__global__ __launch_bounds__(256) void k_up(const bf16* __restrict__ in, int C, int ncx, int ncy, bf16* __restrict__ out,
+                                            int nx, int ny, int nimg, float scale, int accumulate) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t nf = (int64_t)nx * ny;
+  if (k >= nf * nimg * C) return;
+  const int c = (int)(k % C);
+  const int64_t q = k / C;
+  const int64_t img = q / nf;
+  const int p = (int)(q - img * nf);
+  const int i = p % nx, j = p / nx;
+  const int i0 = i >> 1, j0 = j >> 1;
+  const int i1 = (i & 1) ? i0 + 1 : i0, j1 = (j & 1) ? j0 + 1 : j0;
+  const float wi = (i & 1) ? 0.5f : 1.f, wj = (j & 1) ? 0.5f : 1.f;
+  auto at = [&](int a, int b) {
+    return (a < ncx && b < ncy) ? bf2f(in[((img * ncy + b) * (int64_t)ncx + a) * C + c]) : 0.f;
+  };
+  float s;
+  if ((i & 1) && (j & 1))
+    s = 0.25f * (at(i0, j0) + at(i1, j0) + at(i0, j1) + at(i1, j1));
+  else if (i & 1)
+    s = 0.5f * (at(i0, j0) + at(i1, j0));
+  else if (j & 1)
+    s = 0.5f * (at(i0, j0) + at(i0, j1));
+  else
+    s = at(i0, j0);
+  (void)wi;
+  (void)wj;
+  s *= scale;
+  if (accumulate) s += bf2f(out[k]);
+  out[k] = f2bf(s);
+}
+
+__global__ __launch_bounds__(256) void k_concat(const bf16* __restrict__ a, int Ca, const bf16* __restrict__ b, int Cb,
+                                                bf16* __restrict__ out, int64_t npix) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int C = Ca + Cb;
+  if (k >= npix * C) return;
+  const int c = (int)(k % C);
+  const int64_t p = k / C;
+  out[k] = c < Ca ? a[p * Ca + c] : b[p * Cb + (c - Ca)];
+}
+
+// da = dc[:, :Ca] ; db += dc[:, Ca:]
+__global__ __launch_bounds__(256) void k_split(const bf16* __restrict__ dc, int Ca, int Cb, bf16* __restrict__ da,
+                                               bf16* __restrict__ db, int accumulate_b, int64_t npix) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int C = Ca + Cb;
+  if (k >= npix * C) return;
+  const int c = (int)(k % C);
+  const int64_t p = k / C;
+  if (c < Ca) {
+    da[p * Ca + c] = dc[k];
+  } else {
+    const int64_t o = p * Cb + (c - Ca);
+    float v = bf2f(dc[k]);
+    if (accumulate_b) v += bf2f(db[o]);
+    db[o] = f2bf(v);
+  }
+}
+
+// 1x1 head: y[p][o] = hb[o] + sum_c hw[o][c] x[p][c], 16 -> 2, fp32 out
+__global__ __launch_bounds__(256) void k_head_fwd(const bf16* __restrict__ x, const float* __restrict__ hw,
+                                                  const float* __restrict__ hb, float* __restrict__ y, int64_t npix) {
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= npix) return;
+  float y0 = hb[0], y1 = hb[1];
+#pragma unroll
+  for (int c = 0; c < 16; ++c) {
+    const float v = bf2f(x[p * 16 + c]);
+    y0 += hw[c] * v;
+    y1 += hw[16 + c] * v;
+  }
+  y[2 * p] = y0;
+  y[2 * p + 1] = y1;
+}
+
+// dx = hw^T dy (bf16); per-block partial sums of dhw (2x16) and dhb (2)
+__global__ __launch_bounds__(256) void k_head_bwd(const bf16* __restrict__ x, const float* __restrict__ hw,
+                                                  const float* __restrict__ dy, bf16* __restrict__ dx, int64_t npix,
+                                                  int64_t chunk, float* __restrict__ part) {
+  __shared__ float sh[34][64];
+  const int64_t q0 = (int64_t)blockIdx.x * chunk;
+  const int64_t q1 = q0 + chunk < npix ? q0 + chunk : npix;
+  float acc[34];
+#pragma unroll
+  for (int k = 0; k < 34; ++k) acc[k] = 0.f;
+  for (int64_t p = q0 + threadIdx.x; p < q1; p += 256) {
+    const float d0 = dy[2 * p], d1 = dy[2 * p + 1];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+      const float v = bf2f(x[p * 16 + c]);
+      acc[c] += d0 * v;
+      acc[16 + c] += d1 * v;
+      dx[p * 16 + c] = f2bf(hw[c] * d0 + hw[16 + c] * d1);
+    }
+    acc[32] += d0;
+    acc[33] += d1;
+  }
+  // reduce 256 threads -> 1 (fixed order): wave shuffles then LDS
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 34; ++k) {
+    float v = acc[k];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    if (lane == 0) sh[k][wv] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 34) {
+    const int k = threadIdx.x;
+    part[(int64_t)blockIdx.x * 34 + k] = sh[k][0] + sh[k][1] + sh[k][2] + sh[k][3];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_mse(const float* __restrict__ y, const float* __restrict__ t,
+                                             float* __restrict__ dy, float* __restrict__ parts, int64_t n, float inv_n) {
+  __shared__ float sh[4];
+  float s = 0.f;
+  for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (int64_t)gridDim.x * 256) {
+    const float d = y[k] - t[k];
+    dy[k] = 2.f * d * inv_n;
+    s += d * d;
+  }
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) parts[blockIdx.x] = (sh[0] + sh[1] + sh[2] + sh[3]) * inv_n;
+}
+
+__global__ __launch_bounds__(256) void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
+                                              float* __restrict__ v, int64_t n, float lr, float b1, float b2, float eps,
+                                              float c1, float c2) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (k >= n) return;
+  const float gk = g[k];
+  const float mk = b1 * m[k] + (1.f - b1) * gk;
+  const float vk = b2 * v[k] + (1.f - b2) * gk * gk;
+  m[k] = mk;
+  v[k] = vk;
+  w[k] -= lr * (mk / c1) / (sqrtf(vk / c2) + eps);
+}
+
+// wb[tap][co][ci] = bf16(w[tap][co][ci]); wt[tap][ci][co] = bf16(w[8 - tap][co][ci])
+__global__ __launch_bounds__(256) void k_wcopy(const float* __restrict__ w, bf16* __restrict__ wb, bf16* __restrict__ wt,
+                                               int co, int ci) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t n = 9LL * co * ci;
+  if (k >= n) return;
+  const int c_i = (int)(k % ci);
+  const int64_t r = k / ci;
+  const int c_o = (int)(r % co);
+  const int tap = (int)(r / co);
+  const bf16 v = f2bf(w[k]);
+  wb[k] = v;
+  wt[((int64_t)(8 - tap) * ci + c_i) * co + c_o] = v;
+}
+
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+  x ^= x >> 33;
+  x *= 0xff51afd7ed558ccdULL;
+  x ^= x >> 33;
+  x *= 0xc4ceb9fe1a85ec53ULL;
+  x ^= x >> 33;
+  return x;
+}
+// normal(0, std) from a counter-based hash (Box-Muller)
+__global__ __launch_bounds__(256) void k_randn(float* __restrict__ w, int64_t n, float stdv, uint64_t seed) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (k >= n) return;
+  const uint64_t h1 = mix64(seed * 0x9e3779b97f4a7c15ULL + 2 * (uint64_t)k + 1);
+  const uint64_t h2 = mix64(h1 ^ 0x632be59bd9b4e019ULL);
+  const float u1 = ((h1 >> 40) + 1.f) / 16777217.f, u2 = (h2 >> 40) / 16777216.f;
+  w[k] = stdv * sqrtf(-2.f * logf(u1)) * cosf(6.28318530718f * u2);
+}
+
+// network input: [Re r, Im r] / rms(r), k^2 h^2, gamma / Re w, zeros
+__global__ __launch_bounds__(256) void k_rms(const cx<float>* __restrict__ r, int64_t N, float* __restrict__ scale,
+                                             const int* __restrict__ skip) {
+  __shared__ float sh[4];
+  const int img = blockIdx.x;
+  if (skip && skip[img]) return;
+  const cx<float>* rb = r + (int64_t)img * N;
+  float s = 0.f;
+  for (int64_t p = threadIdx.x; p < N; p += 256) s += cabs2(rb[p]);
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const float t = sqrtf((sh[0] + sh[1] + sh[2] + sh[3]) / (float)N);
+    scale[img] = t > 0.f ? t : 1.f;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_prep(const cx<float>* __restrict__ r, int64_t N, const float* __restrict__ kh2,
+                                              const float* __restrict__ gw, const float* __restrict__ scale,
+                                              bf16* __restrict__ x, const int* __restrict__ skip, int nimg) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (k >= N * nimg) return;
+  const int64_t img = k / N;
+  const int64_t p = k - img * N;
+  bf16* xo = x + k * UNET_CIN;
+  if (skip && skip[img]) {
+    for (int c = 0; c < UNET_CIN; ++c) xo[c] = 0;
+    return;
+  }
+  const float is = 1.f / scale[img];
+  const cx<float> v = r[k];
+  xo[0] = f2bf(v.re * is);
+  xo[1] = f2bf(v.im * is);
+  xo[2] = f2bf(kh2[p]);
+  xo[3] = f2bf(gw[p]);
+#pragma unroll
+  for (int c = 4; c < UNET_CIN; ++c) xo[c] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_post(const float* __restrict__ y, int64_t N, const float* __restrict__ scale,
+                                              float h2, cx<float>* __restrict__ e, const int* __restrict__ skip,
+                                              int nimg) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (k >= N * nimg) return;
+  const int64_t img = k / N;
+  if (skip && skip[img]) return;
+  const float s = scale[img] * h2;
+  e[k] = mk<float>(s * y[2 * k], s * y[2 * k + 1]);
+}
+
+// ------------------------------------------------------------------ host side
+static inline unsigned nblk(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+enum Act {
+  A_A0, A_S0, A_D1, A_A1, A_S1, A_D2, A_A2, A_S2, A_D3, A_A3, A_B3, A_U2, A_C2, A_A4, A_B4, A_U1, A_C1, A_A5, A_B5,
+  A_U0, A_C0, A_A6, A_B6, A_NUM
+};
+static const int kActLevel[A_NUM] = {0, 0, 1, 1, 1, 2, 2, 2, 3, 3, 3, 2, 2, 2, 2, 1, 1, 1, 1, 0, 0, 0, 0};
+static const int kActC[A_NUM] = {16, 16, 16, 32, 32, 32, 64, 64, 64, 64, 64, 64, 128, 64, 64, 64, 96, 32, 32, 32, 48, 16, 16};
+// conv layer l: input activation, output activation (-1 input = network input)
+static const int kIn[UNET_NCONV] = {-1, A_A0, A_D1, A_A1, A_D2, A_A2, A_D3, A_A3, A_C2, A_A4, A_C1, A_A5, A_C0, A_A6};
+static const int kOut[UNET_NCONV] = {A_A0, A_S0, A_A1, A_S1, A_A2, A_S2, A_A3, A_B3, A_A4, A_B4, A_A5, A_B5, A_A6, A_B6};
+
+struct Dims {
+  int nx[4], ny[4];
+  int64_t T[4];
+};
+static Dims dims_of(int nimg, int nx, int ny) {
+  Dims d;
+  d.nx[0] = nx;
+  d.ny[0] = ny;
+  for (int l = 1; l < 4; ++l) {
+    d.nx[l] = (d.nx[l - 1] + 1) / 2;
+    d.ny[l] = (d.ny[l - 1] + 1) / 2;
+  }
+  for (int l = 0; l < 4; ++l) d.T[l] = (int64_t)nimg * d.nx[l] * d.ny[l];
+  return d;
+}
+
+// buffers: [0, A_NUM) activations, [A_NUM, 2 A_NUM) gradients, then dpre, head out, dy
+static void ensure_ws(UNet& net, int nimg, int nx, int ny) {
+  if (net.ws_nimg == nimg && net.ws_nx == nx && net.ws_ny == ny) return;
+  for (void* p : net.bufs)
+    if (p) (void)hipFree(p);
+  net.bufs.assign(2 * A_NUM + 3, nullptr);
+  Dims d = dims_of(nimg, nx, ny);
+  for (int a = 0; a < A_NUM; ++a) {
+    net.bufs[a] = dmal<bf16>((size_t)d.T[kActLevel[a]] * kActC[a]);
+    net.bufs[A_NUM + a] = dmal<bf16>((size_t)d.T[kActLevel[a]] * kActC[a]);
+  }
+  net.bufs[2 * A_NUM] = dmal<bf16>((size_t)d.T[0] * 64);      // dpre scratch
+  net.bufs[2 * A_NUM + 1] = dmal<bf16>((size_t)d.T[0] * 16);  // d(head input)
+  net.bufs[2 * A_NUM + 2] = dmal<float>((size_t)d.T[0] * 2);  // unused spare
+  net.ws_nimg = nimg;
+  net.ws_nx = nx;
+  net.ws_ny = ny;
+}
+
+static bf16* act(UNet& net, int a) { return (bf16*)net.bufs[a]; }
+static bf16* grd(UNet& net, int a) { return (bf16*)net.bufs[A_NUM + a]; }
+
+static void conv_launch(const bf16* in, int cin, const bf16* w, const float* bias, bf16* out, int cout, int nimg, int nx,
+                        int ny, int relu, hipStream_t s) {
+  dim3 grid((unsigned)((nx * ny + 63) / 64), nimg);
+  switch (cout / 16) {
+    case 1: hipLaunchKernelGGL(k_conv3x3<1>, grid, dim3(256), 0, s, in, cin, w, bias, out, nx, ny, relu); break;
+    case 2: hipLaunchKernelGGL(k_conv3x3<2>, grid, dim3(256), 0, s, in, cin, w, bias, out, nx, ny, relu); break;
+    case 3: hipLaunchKernelGGL(k_conv3x3<3>, grid, dim3(256), 0, s, in, cin, w, bias, out, nx, ny, relu); break;
+    case 4: hipLaunchKernelGGL(k_conv3x3<4>, grid, dim3(256), 0, s, in, cin, w, bias, out, nx, ny, relu); break;
+    case 6: hipLaunchKernelGGL(k_conv3x3<6>, grid, dim3(256), 0, s, in, cin, w, bias, out, nx, ny, relu); break;
+    case 8: hipLaunchKernelGGL(k_conv3x3<8>, grid, dim3(256), 0, s, in, cin, w, bias, out, nx, ny, relu); break;
+    default: throw std::runtime_error("unsupported channel count");
+  }
+}
+
+void unet_sync_bf16(UNet& net) {
+  for (int l = 0; l < UNET_NCONV; ++l) {
+    const int64_t n = 9LL * kSpec[l].co * kSpec[l].ci;
+    hipLaunchKernelGGL(k_wcopy, dim3(nblk(n)), dim3(256), 0, net.stream, net.w[l], net.wb[l], net.wt[l], kSpec[l].co,
+                       kSpec[l].ci);
+  }
+}
+
+void unet_init(UNet& net, uint64_t seed, hipStream_t s) {
+  net.stream = s;
+  net.nparam = unet_param_count();
+  net.flat = dmal<float>(net.nparam);
+  net.grad = dmal<float>(net.nparam);
+  net.adam_m = dmal<float>(net.nparam);
+  net.adam_v = dmal<float>(net.nparam);
+  ck(hipMemsetAsync(net.flat, 0, sizeof(float) * net.nparam, s), "memset");
+  ck(hipMemsetAsync(net.grad, 0, sizeof(float) * net.nparam, s), "memset");
+  ck(hipMemsetAsync(net.adam_m, 0, sizeof(float) * net.nparam, s), "memset");
+  ck(hipMemsetAsync(net.adam_v, 0, sizeof(float) * net.nparam, s), "memset");
+  int64_t off = 0;
+  for (int l = 0; l < UNET_NCONV; ++l) {
+    const int64_t nw = 9LL * kSpec[l].co * kSpec[l].ci;
+    net.w[l] = net.flat + off;
+    off += nw;
+    net.b[l] = net.flat + off;
+    off += kSpec[l].co;
+    const int fan_in = 9 * (l == 0 ? UNET_FEATS : kSpec[l].ci);
+    hipLaunchKernelGGL(k_randn, dim3(nblk(nw)), dim3(256), 0, s, net.w[l], nw, sqrtf(2.f / fan_in),
+                       seed * 131 + l);
+    net.wb[l] = dmal<bf16>(nw);
+    net.wt[l] = dmal<bf16>(nw);
+  }
+  net.hw = net.flat + off;
+  off += 32;
+  net.hb = net.flat + off;
+  hipLaunchKernelGGL(k_randn, dim3(1), dim3(256), 0, s, net.hw, (int64_t)32, 0.25f, seed * 131 + 99);
+  net.adam_t = 0;
+  unet_sync_bf16(net);
+  ck(hipGetLastError(), "unet_init");
+}
+
+void unet_free(UNet& net) {
+  for (void* p : net.bufs)
+    if (p) (void)hipFree(p);
+  net.bufs.clear();
+  for (int l = 0; l < UNET_NCONV; ++l) {
+    if (net.wb[l]) (void)hipFree(net.wb[l]);
+    if (net.wt[l]) (void)hipFree(net.wt[l]);
+  }
+  for (float* p : {net.flat, net.grad, net.adam_m, net.adam_v, net.scratch})
+    if (p) (void)hipFree(p);
+  net = UNet{};
+}
+
+void unet_forward(UNet& net, int nimg, int nx, int ny, const bf16* x, float* y, bool /*keep*/) {
+  ensure_ws(net, nimg, nx, ny);
+  hipStream_t s = net.stream;
+  Dims d = dims_of(nimg, nx, ny);
+  auto conv = [&](int l) {
+    const ConvSpec& c = kSpec[l];
+    const int lv = c.level;
+    const bf16* in = kIn[l] < 0 ? x : act(net, kIn[l]);
+    conv_launch(in, c.ci, net.wb[l], net.b[l], act(net, kOut[l]), c.co, nimg, d.nx[lv], d.ny[lv], 1, s);
+  };
+  auto down = [&](int a_in, int a_out, int lf) {
+    const int C = kActC[a_in];
+    hipLaunchKernelGGL(k_down, dim3(nblk(d.T[lf + 1] * C)), dim3(256), 0, s, act(net, a_in), C, d.nx[lf], d.ny[lf],
+                       act(net, a_out), d.nx[lf + 1], d.ny[lf + 1], nimg, 1.f);
+  };
+  auto up = [&](int a_in, int a_out, int lc) {
+    const int C = kActC[a_in];
+    hipLaunchKernelGGL(k_up, dim3(nblk(d.T[lc - 1] * C)), dim3(256), 0, s, act(net, a_in), C, d.nx[lc], d.ny[lc],
+                       act(net, a_out), d.nx[lc - 1], d.ny[lc - 1], nimg, 1.f, 0);
+  };
+  auto cat = [&](int a, int b, int c, int lv) {
+    hipLaunchKernelGGL(k_concat, dim3(nblk(d.T[lv] * kActC[c])), dim3(256), 0, s, act(net, a), kActC[a], act(net, b),
+                       kActC[b], act(net, c), d.T[lv]);
+  };
+  conv(0); conv(1); down(A_S0, A_D1, 0);
+  conv(2); conv(3); down(A_S1, A_D2, 1);
+  conv(4); conv(5); down(A_S2, A_D3, 2);
+  conv(6); conv(7);
+  up(A_B3, A_U2, 3); cat(A_U2, A_S2, A_C2, 2);
+  conv(8); conv(9);
+  up(A_B4, A_U1, 2); cat(A_U1, A_S1, A_C1, 1);
+  conv(10); conv(11);
+  up(A_B5, A_U0, 1); cat(A_U0, A_S0, A_C0, 0);
+  conv(12); conv(13);
+  hipLaunchKernelGGL(k_head_fwd, dim3(nblk(d.T[0])), dim3(256), 0, s, act(net, A_B6), net.hw, net.hb, y, d.T[0]);
+}
+
+static int64_t pick_chunks(int64_t total, int64_t* chunk) {
+  int64_t nch = total / 4096;
+  if (nch < 1) nch = 1;
+  if (nch > 64) nch = 64;
+  int64_t c = (total + nch - 1) / nch;
+  c = (c + 15) / 16 * 16;
+  *chunk = c;
+  return (total + c - 1) / c;
+}
+
+static void ensure_scratch(UNet& net, int64_t n) {
+  if (net.scratch_n >= n) return;
+  if (net.scratch) (void)hipFree(net.scratch);
+  net.scratch = dmal<float>(n);
+  net.scratch_n = n;
+}
+
+void unet_backward(UNet& net, int nimg, int nx, int ny, const bf16* x, const float* dy) {
+  hipStream_t s = net.stream;
+  Dims d = dims_of(nimg, nx, ny);
+  bf16* dpre = (bf16*)net.bufs[2 * A_NUM];
+  // scratch: wgrad partials (64 chunks x largest layer) or head partials
+  ensure_scratch(net, 64LL * 9 * 128 * 64 + 64LL * 34);
+  const int64_t head_off = unet_param_count() - 34;
+  // head
+  {
+    int64_t chunk;
+    const int64_t nch = pick_chunks(d.T[0], &chunk);
+    hipLaunchKernelGGL(k_head_bwd, dim3((unsigned)nch), dim3(256), 0, s, act(net, A_B6), net.hw, dy, grd(net, A_B6),
+                       d.T[0], chunk, net.scratch);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, s, net.scratch, (int)nch, (int64_t)34, net.grad + head_off);
+  }
+  int64_t off[UNET_NCONV];
+  {
+    int64_t o = 0;
+    for (int l = 0; l < UNET_NCONV; ++l) {
+      off[l] = o;
+      o += 9LL * kSpec[l].co * kSpec[l].ci + kSpec[l].co;
+    }
+  }
+  // conv layer backward: g(out) -> dpre -> dW, db, g(in)
+  auto conv_b = [&](int l, bool need_din) {
+    const ConvSpec& c = kSpec[l];
+    const int lv = c.level;
+    const int64_t T = d.T[lv];
+    hipLaunchKernelGGL(k_relu_bwd, dim3(nblk(T * c.co)), dim3(256), 0, s, grd(net, kOut[l]), act(net, kOut[l]), dpre,
+                       T * c.co);
+    const bf16* in = kIn[l] < 0 ? x : act(net, kIn[l]);
+    int64_t chunk;
+    const int64_t nch = pick_chunks(T, &chunk);
+    const int64_t nw = 9LL * c.co * c.ci;
+    hipLaunchKernelGGL(k_conv_wgrad, dim3((unsigned)nch, 9 * (c.co / 16) * (c.ci / 16)), dim3(64), 0, s, dpre, c.co,
+                       in, c.ci, d.nx[lv], d.ny[lv], T, chunk, net.scratch);
+    hipLaunchKernelGGL(k_sum_parts, dim3(nblk(nw)), dim3(256), 0, s, net.scratch, (int)nch, nw, net.grad + off[l]);
+    hipLaunchKernelGGL(k_colsum, dim3((unsigned)nch), dim3(256), 0, s, dpre, c.co, T, chunk, net.scratch);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, s, net.scratch, (int)nch, (int64_t)c.co,
+                       net.grad + off[l] + nw);
+    if (need_din)
+      conv_launch(dpre, c.co, net.wt[l], nullptr, grd(net, kIn[l]), c.ci, nimg, d.nx[lv], d.ny[lv], 0, s);
+  };
+  auto split = [&](int ac, int au, int askip, int lv) {
+    hipLaunchKernelGGL(k_split, dim3(nblk(d.T[lv] * kActC[ac])), dim3(256), 0, s, grd(net, ac), kActC[au],
+                       kActC[askip], grd(net, au), grd(net, askip), 0, d.T[lv]);
+  };
+  // g(b) = P^T g(u) = 4 R g(u)
+  auto up_b = [&](int au, int ab, int lc) {
+    const int C = kActC[au];
+    hipLaunchKernelGGL(k_down, dim3(nblk(d.T[lc] * C)), dim3(256), 0, s, grd(net, au), C, d.nx[lc - 1], d.ny[lc - 1],
+                       grd(net, ab), d.nx[lc], d.ny[lc], nimg, 4.f);
+  };
+  // g(s) += R^T g(d) = P g(d) / 4
+  auto down_b = [&](int ad, int as, int lf) {
+    const int C = kActC[ad];
+    hipLaunchKernelGGL(k_up, dim3(nblk(d.T[lf] * C)), dim3(256), 0, s, grd(net, ad), C, d.nx[lf + 1], d.ny[lf + 1],
+                       grd(net, as), d.nx[lf], d.ny[lf], nimg, 0.25f, 1);
+  };
+  conv_b(13, true); conv_b(12, true);
+  split(A_C0, A_U0, A_S0, 0);
+  up_b(A_U0, A_B5, 1);
+  conv_b(11, true); conv_b(10, true);
+  split(A_C1, A_U1, A_S1, 1);
+  up_b(A_U1, A_B4, 2);
+  conv_b(9, true); conv_b(8, true);
+  split(A_C2, A_U2, A_S2, 2);
+  up_b(A_U2, A_B3, 3);
+  conv_b(7, true); conv_b(6, true);
+  down_b(A_D3, A_S2, 2);
+  conv_b(5, true); conv_b(4, true);
+  down_b(A_D2, A_S1, 1);
+  conv_b(3, true); conv_b(2, true);
+  down_b(A_D1, A_S0, 0);
+  conv_b(1, true); conv_b(0, false);
+}
+
+void unet_adam(UNet& net, float lr, float b1, float b2, float eps) {
+  net.adam_t += 1;
+  const float c1 = 1.f - powf(b1, (float)net.adam_t), c2 = 1.f - powf(b2, (float)net.adam_t);
+  hipLaunchKernelGGL(k_adam, dim3(nblk(net.nparam)), dim3(256), 0, net.stream, net.flat, net.grad, net.adam_m,
+                     net.adam_v, net.nparam, lr, b1, b2, eps, c1, c2);
+  unet_sync_bf16(net);
+}
+
+void unet_prep_input(int nimg, int64_t N, const cx<float>* r, const float* kh2, const float* gw, bf16* x16,
+                     float* scale, const int* skip, hipStream_t s) {
+  hipLaunchKernelGGL(k_rms, dim3(nimg), dim3(256), 0, s, r, N, scale, skip);
+  hipLaunchKernelGGL(k_prep, dim3(nblk(N * nimg)), dim3(256), 0, s, r, N, kh2, gw, scale, x16, skip, nimg);
+}
+
+void unet_post_output(int nimg, int64_t N, const float* y, const float* scale, float h2, cx<float>* e, const int* skip,
+                      hipStream_t s) {
+  hipLaunchKernelGGL(k_post, dim3(nblk(N * nimg)), dim3(256), 0, s, y, N, scale, h2, e, skip, nimg);
+}
+
+void unet_mse(int64_t n, const float* y, const float* t, float* dy, float* loss_parts, int nparts, hipStream_t s) {
+  hipLaunchKernelGGL(k_mse, dim3(nparts), dim3(256), 0, s, y, t, dy, loss_parts, n, 1.f / (float)n);
+}
+
+}  // namespace hh

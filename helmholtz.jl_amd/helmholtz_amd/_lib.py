@@ -21,6 +21,10 @@ EXPORTS = (
     "hh_last_error", "hh_version", "hh_abl", "hh_op_create", "hh_op_destroy", "hh_op_apply",
     "hh_solver_create", "hh_solver_copy", "hh_solver_clear", "hh_solver_destroy", "hh_solve",
     "hh_solver_levels", "hh_device_alloc", "hh_device_free", "hh_memcpy", "hh_op_time_kernel",
+    "hh_unet_num_params", "hh_unet_create", "hh_unet_destroy", "hh_unet_get_params", "hh_unet_set_params",
+    "hh_unet_forward", "hh_unet_loss_grad", "hh_unet_adam_step", "hh_dataset_create", "hh_dataset_destroy",
+    "hh_dataset_size", "hh_solver_capture_begin", "hh_solver_capture_count", "hh_dataset_add_from_solver",
+    "hh_unet_train", "hh_unet_eval", "hh_solver_attach_unet",
 )
 
 
@@ -35,6 +39,7 @@ class MGParams(C.Structure):
         ("levels", C.c_int32), ("pre", C.c_int32), ("post", C.c_int32), ("relax", C.c_double),
         ("cycle", C.c_int32), ("max_outer", C.c_int32), ("tol", C.c_double), ("inner", C.c_int32),
         ("krylov", C.c_int32), ("prec", C.c_int32), ("learned", C.c_int32), ("verbose", C.c_int32),
+        ("graph", C.c_int32), ("autotune", C.c_int32),
     ]
 
 
@@ -43,6 +48,7 @@ class SolveStats(C.Structure):
         ("iters", C.POINTER(C.c_int32)), ("status", C.POINTER(C.c_int32)),
         ("true_res", C.POINTER(C.c_double)), ("max_iters", C.c_int32), ("n_converged", C.c_int32),
         ("setup_time", C.c_double), ("solve_time", C.c_double), ("n_prec", C.c_int64),
+        ("learned_used", C.c_int32), ("fell_back", C.c_int32),
     ]
 
 
@@ -77,8 +83,29 @@ def lib():
     L.hh_device_free.argtypes = [vp]
     L.hh_memcpy.argtypes = [vp, vp, i64, i32]
     L.hh_op_time_kernel.argtypes = [vp, i32, dbl, vp, vp, vp, i64, i32, i32, pd]
+    f32, u64 = C.c_float, C.c_uint64
+    pf = C.POINTER(C.c_float)
+    L.hh_unet_num_params.argtypes = []
+    L.hh_unet_create.argtypes = [u64, C.POINTER(vp)]
+    L.hh_unet_destroy.argtypes = [vp]
+    L.hh_unet_get_params.argtypes = [vp, vp]
+    L.hh_unet_set_params.argtypes = [vp, vp]
+    L.hh_unet_forward.argtypes = [vp, i32, i32, i32, vp, vp]
+    L.hh_unet_loss_grad.argtypes = [vp, i32, i32, i32, vp, vp, pf, vp]
+    L.hh_unet_adam_step.argtypes = [vp, vp, f32, f32, f32, f32]
+    L.hh_dataset_create.argtypes = [i32, i32, i64, C.POINTER(vp)]
+    L.hh_dataset_destroy.argtypes = [vp]
+    L.hh_dataset_size.argtypes = [vp, C.POINTER(C.c_int64)]
+    L.hh_solver_capture_begin.argtypes = [vp, i32, i64, i32, i32]
+    L.hh_solver_capture_count.argtypes = [vp, C.POINTER(C.c_int64)]
+    L.hh_dataset_add_from_solver.argtypes = [vp, vp, i32]
+    L.hh_unet_train.argtypes = [vp, vp, i32, i32, f32, u64, pf]
+    L.hh_unet_eval.argtypes = [vp, vp, i64, i64, pf]
+    L.hh_solver_attach_unet.argtypes = [vp, vp, i32]
     for name in EXPORTS:
-        getattr(L, name).restype = getattr(L, name).restype if name in ("hh_last_error",) else C.c_int
+        if name == "hh_last_error":
+            continue
+        getattr(L, name).restype = C.c_int64 if name == "hh_unet_num_params" else C.c_int
     _lib = L
     return L
 

@@ -43,6 +43,8 @@ class MGparam:
     coarseSolveType: str = "NoMUMPS"
     extra: tuple = field(default_factory=tuple)
     prec: str = "c128"
+    graph: bool = False
+    autotune: bool = False
     verbose: bool = False
     _hierarchy_exists: bool = False
 
@@ -78,6 +80,7 @@ class ShiftedLaplacianMultigridSolver:
     _op: object = None
     _op_key: object = None
     _handle: object = None
+    _net_ref: object = None
 
     def __del__(self):
         _drop_handle(self)
@@ -144,7 +147,8 @@ def _ensure_handle(s: ShiftedLaplacianMultigridSolver):
                          relax=float(mg.relaxParam), cycle=ord(str(mg.cycleType)), max_outer=int(mg.maxOuterIter),
                          tol=float(mg.relativeTol), inner=int(s.inner), krylov=L.HH_FGMRES,
                          prec=L.HH_C64 if mg.prec == "c64" else L.HH_C128, learned=0,
-                         verbose=int(bool(s.verbose)))
+                         verbose=int(bool(s.verbose)), graph=int(bool(mg.graph)),
+                         autotune=int(bool(mg.autotune)))
         shifts = np.ascontiguousarray(np.broadcast_to(np.asarray(s.shift, dtype=float), (int(mg.levels),)))
         h = C.c_void_p()
         L.check(L.lib().hh_solver_create(s._op.ptr, C.byref(prm), shifts.ctypes.data_as(C.POINTER(C.c_double)),
@@ -192,14 +196,15 @@ def solveLinearSystem(ShiftedHT, B, param: ShiftedLaplacianMultigridSolver, doTr
     status = np.zeros(nrhs, dtype=np.int32)
     res = np.zeros(nrhs, dtype=np.float64)
     st = L.SolveStats(iters.ctypes.data_as(C.POINTER(C.c_int32)), status.ctypes.data_as(C.POINTER(C.c_int32)),
-                      res.ctypes.data_as(C.POINTER(C.c_double)), 0, 0, 0.0, 0.0, 0)
+                      res.ctypes.data_as(C.POINTER(C.c_double)), 0, 0, 0.0, 0.0, 0, 0, 0)
     L.check(L.lib().hh_solve(param._handle, bptr, xptr, nrhs, 0, loc, C.byref(st)))
     param.MG._hierarchy_exists = True
     param.setupTime += st.setup_time
     param.solveTime += st.solve_time
     param.nPrec += int(st.n_prec)
     param.last_stats = {"iters": iters, "status": [_STATUS[int(v)] for v in status], "true_res": res,
-                        "max_iters": int(st.max_iters), "setup_time": st.setup_time, "solve_time": st.solve_time}
+                        "max_iters": int(st.max_iters), "setup_time": st.setup_time, "solve_time": st.solve_time,
+                        "learned_used": bool(st.learned_used), "fell_back": bool(st.fell_back)}
     if dev:
         return X, param
     return (X[:, 0] if vec else X), param

@@ -20,11 +20,13 @@ const HH_FGMRES = Cint(0)
 struct HHMGParams
     levels::Int32; pre::Int32; post::Int32; relax::Float64; cycle::Int32; max_outer::Int32
     tol::Float64; inner::Int32; krylov::Int32; prec::Int32; learned::Int32; verbose::Int32
+    graph::Int32; autotune::Int32
 end
 
 mutable struct HHSolveStats
     iters::Ptr{Int32}; status::Ptr{Int32}; true_res::Ptr{Float64}
     max_iters::Int32; n_converged::Int32; setup_time::Float64; solve_time::Float64; n_prec::Int64
+    learned_used::Int32; fell_back::Int32
 end
 
 check(rc) = rc == 0 || error("helmholtz_amd: " * unsafe_string(ccall((:hh_last_error, LIB), Cstring, ())))
@@ -86,7 +88,7 @@ function ensure_handle!(s::ShiftedLaplacianMultigridSolverAMD)
     if s.handle == C_NULL
         mg = s.MG
         prm = Ref(HHMGParams(mg.levels, mg.relaxPre, mg.relaxPost, mg.relaxParam, Int32(mg.cycleType),
-                             mg.maxOuterIter, mg.relativeTol, s.inner, HH_FGMRES, s.prec, 0, s.verbose))
+                             mg.maxOuterIter, mg.relativeTol, s.inner, HH_FGMRES, s.prec, 0, s.verbose, 0, 0))
         out = Ref{Ptr{Cvoid}}(C_NULL)
         check(ccall((:hh_solver_create, LIB), Cint, (Ptr{Cvoid}, Ref{HHMGParams}, Ptr{Float64}, Ptr{Ptr{Cvoid}}),
                     s.op.ptr, prm, s.shift, out))
@@ -110,7 +112,7 @@ function solveLinearSystem(ShiftedHT, B, param::ShiftedLaplacianMultigridSolverA
     X = similar(Bc)
     iters = zeros(Int32, nrhs); status = zeros(Int32, nrhs); res = zeros(Float64, nrhs)
     GC.@preserve Bc X iters status res begin
-        st = Ref(HHSolveStats(pointer(iters), pointer(status), pointer(res), 0, 0, 0.0, 0.0, 0))
+        st = Ref(HHSolveStats(pointer(iters), pointer(status), pointer(res), 0, 0, 0.0, 0.0, 0, 0, 0))
         check(ccall((:hh_solve, LIB), Cint,
                     (Ptr{Cvoid}, Ptr{ComplexF64}, Ptr{ComplexF64}, Int64, Cint, Cint, Ref{HHSolveStats}),
                     param.handle, Bc, X, nrhs, 0, HH_HOST, st))

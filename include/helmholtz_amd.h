@@ -38,6 +38,8 @@ enum { HH_CONVERGED = 0, HH_MAXITER = 1, HH_NONFINITE = 2, HH_ZERO_RHS = 3 };
 
 typedef struct hh_op hh_op;
 typedef struct hh_solver hh_solver;
+typedef struct hh_unet hh_unet;
+typedef struct hh_dataset hh_dataset;
 
 /* Multigrid + Krylov settings; mirrors the fields of Multigrid.jl's MGparam
  * that the reference sets through getMGparam (test/ShiftedLaplacianTest.jl:51-64)
@@ -53,8 +55,10 @@ typedef struct hh_mg_params {
   int32_t inner;      /* solver.inner: FGMRES restart length (1..32)             */
   int32_t krylov;     /* HH_FGMRES                                               */
   int32_t prec;       /* preconditioner precision: HH_C128 or HH_C64             */
-  int32_t learned;    /* reserved: 0                                             */
+  int32_t learned;    /* reserved: 0 (attach a network with hh_solver_attach_unet) */
   int32_t verbose;    /* solver.verbose                                          */
+  int32_t graph;      /* 1: replay each FGMRES iteration as a captured hipGraph  */
+  int32_t autotune;   /* 1: time stencil launch shapes per level at first solve  */
 } hh_mg_params;
 
 /* Per-call results.  The arrays are caller-owned with nrhs entries, or NULL. */
@@ -67,6 +71,8 @@ typedef struct hh_solve_stats {
   double setup_time;   /* seconds spent building the hierarchy in this call     */
   double solve_time;   /* seconds in the Krylov solve                            */
   int64_t n_prec;      /* preconditioner applications (lock-step iterations)    */
+  int32_t learned_used; /* 1 if the solve started with the learned correction    */
+  int32_t fell_back;    /* 1 if it stagnated and switched to the plain cycle      */
 } hh_solve_stats;
 
 const char* hh_last_error(void);
@@ -138,6 +144,50 @@ int hh_memcpy(void* dst, const void* src, int64_t bytes, int kind);
  * X, B, Y are device blocks of nrhs columns in precision prec. */
 int hh_op_time_kernel(hh_op* op, int kernel, double shift, const void* X, const void* B, void* Y, int64_t nrhs,
                       int prec, int reps, double* avg_ms);
+
+/* ---- Learned coarse-grid correction (no counterpart in the reference) ----
+ * A 4-level U-Net (16/32/64/64 channels, 3x3 convolutions, ReLU, 1x1 head)
+ * in BF16 with FP32 accumulation on MFMA.  Input features per node of the
+ * learned level: Re r / rms(r), Im r / rms(r), Re(w)^2 m h_x^2, gamma / Re(w);
+ * output: the correction e = rms(r) h_x^2 (y0 + i y1) for A_l e = r.
+ * Parameters are fp32 (flat layout: per conv layer W[9][co][ci] then bias,
+ * then the head's W[2][16] and bias[2]). */
+int64_t hh_unet_num_params(void);
+int hh_unet_create(uint64_t seed, hh_unet** out);
+int hh_unet_destroy(hh_unet* u);
+int hh_unet_get_params(const hh_unet* u, float* out);
+int hh_unet_set_params(hh_unet* u, const float* in);
+/* feats: host [nimg][nx*ny][4] fp32 (rounded to BF16 on the device); y: host [nimg][nx*ny][2] */
+int hh_unet_forward(hh_unet* u, int nimg, int nx, int ny, const float* feats, float* y);
+/* mean-squared-error loss against target [nimg][nx*ny][2] and its gradient w.r.t. the flat parameters */
+int hh_unet_loss_grad(hh_unet* u, int nimg, int nx, int ny, const float* feats, const float* target, float* loss,
+                      float* grad);
+/* one Adam step with `grad` (host, flat; NULL = the last computed gradient) */
+int hh_unet_adam_step(hh_unet* u, const float* grad, float lr, float beta1, float beta2, float eps);
+
+/* Training pairs on a fixed 2-D grid of nx x ny nodes (device resident). */
+int hh_dataset_create(int nx, int ny, int64_t capacity, hh_dataset** out);
+int hh_dataset_destroy(hh_dataset* d);
+int hh_dataset_size(const hh_dataset* d, int64_t* n);
+/* Record the right-hand sides that reach grid level `level` during FGMRES
+ * iterations iter_lo..iter_hi-1 of the first restart cycle of the solver's
+ * next hh_solve calls (complex64 preconditioner). */
+int hh_solver_capture_begin(hh_solver* s, int level, int64_t max_samples, int iter_lo, int iter_hi);
+int hh_solver_capture_count(const hh_solver* s, int64_t* n);
+/* Append the captured residuals r with targets e = A_level^-1 r and the
+ * level's features.  Targets are solved to 1e-6 by FGMRES(10) on the level's
+ * shifted operator (W(2,2) preconditioner with a further 0.5 shift), at most
+ * max(4, sweeps) restarts. */
+int hh_dataset_add_from_solver(hh_dataset* d, hh_solver* s, int sweeps);
+/* Adam training on the dataset (mean-squared error on normalised targets). */
+int hh_unet_train(hh_unet* u, hh_dataset* d, int epochs, int batch, float lr, uint64_t seed, float* loss_per_epoch);
+/* mean relative error ||e_net - e|| / ||e|| over samples first..first+count-1 */
+int hh_unet_eval(hh_unet* u, hh_dataset* d, int64_t first, int64_t count, float* rel_err);
+/* Use the network as the coarse-grid correction at grid level `level` (its
+ * output replaces the cycle below that level) inside the FGMRES solve; the
+ * plain cycle takes over if a restart cycle stops reducing every residual by
+ * 10% (or produces non-finite values).  net = NULL detaches. */
+int hh_solver_attach_unet(hh_solver* s, hh_unet* net, int level);
 
 #ifdef __cplusplus
 }

@@ -159,3 +159,67 @@ def test_maxiter_status():
     x, s = ha.solveLinearSystem(None, q, s)
     assert s.last_stats["status"] == ["maxiter"]
     assert s.last_stats["iters"][0] == 2
+
+
+def _c64_solver(M, m, g, w, neumann, levels, graph=False, autotune=False, cycle="W"):
+    s = _solver(M, m, g, w, neumann, levels, 0.5, cycle=cycle, prec="c64")
+    s.MG.graph = graph
+    s.MG.autotune = autotune
+    return s
+
+
+def test_graph_replay_equals_eager():
+    """Each FGMRES iteration captured as a hipGraph from the second restart
+    cycle on and replayed (several times per solve, and again in a second
+    call) gives bit-identical iterates to eager launches."""
+    M, m, g, w = small_case((129, 129), seed=31, neumann=True)
+    rng = np.random.default_rng(8)
+    B = np.asfortranarray(rng.standard_normal((m.size, 6)) + 1j * rng.standard_normal((m.size, 6)))
+    se = _c64_solver(M, m, g, w, True, 4)
+    Xe, se = ha.solveLinearSystem(None, B, se)
+    sg = _c64_solver(M, m, g, w, True, 4, graph=True)
+    Xg, sg = ha.solveLinearSystem(None, B, sg)
+    assert se.last_stats["max_iters"] > 20  # several restart cycles replayed
+    assert np.array_equal(Xe, Xg)
+    Xg2, sg = ha.solveLinearSystem(None, B, sg)
+    assert np.array_equal(Xe, Xg2)
+
+
+def test_autotuned_launch_shapes_give_identical_results():
+    M, m, g, w = small_case((513, 257), seed=32, neumann=True)
+    rng = np.random.default_rng(9)
+    B = np.asfortranarray(rng.standard_normal((m.size, 8)) + 1j * rng.standard_normal((m.size, 8)))
+    s0 = _c64_solver(M, m, g, w, True, 5)
+    X0, s0 = ha.solveLinearSystem(None, B, s0)
+    s1 = _c64_solver(M, m, g, w, True, 5, autotune=True)
+    X1, s1 = ha.solveLinearSystem(None, B, s1)
+    assert np.array_equal(X0, X1)
+
+
+@pytest.mark.parametrize("cycle", ["V", "W"])
+def test_3d_solve_matches_oracle(cycle):
+    """7-point 3-D operator, 3-level cycle: true residual and the oracle's
+    iteration count (the 3-D configuration's parity case at reduced size)."""
+    nodes = (33, 33, 17)
+    M = hr.get_regular_mesh([0, 320, 0, 320, 0, 160], [32, 32, 16])
+    rng = np.random.default_rng(41)
+    v = 1500 + 3000 * rng.random(nodes)
+    m = (1.0 / v**2).ravel(order="F")
+    w = 2 * np.pi * 1500 / (10 * 10.0)  # 10 points per wavelength at 1500 m/s
+    H, g = hr.get_helmholtz_operator(M, m, w, 0.01 * w * np.ones(m.size), True, [4, 4, 4], w, True)
+    g = g.ravel(order="F")
+    q = np.zeros(m.size, complex)
+    q[hr.loc2cs(M.nodes, [17, 17, 2]) - 1] = 1.0 / np.linalg.norm(M.h) ** 2
+    s = _solver(M, m, g, w, True, 3, 0.5, cycle=cycle)
+    x, s = ha.solveLinearSystem(None, q, s)
+    res = np.linalg.norm(H @ x - q) / np.linalg.norm(q)
+    p = hr.HelmholtzParam(M, g, m, w, True, True)
+    hier = sr.build_hierarchy(p, 3, 0.5)
+    xo, ito, _ = sr.fgmres(lambda v_: H @ v_, q, lambda v_: sr.vcycle(hier, 0, v_, 0.8, 2, 2, cycle), 10, 100, 1e-6)
+    it = s.last_stats["iters"][0]
+    print(f"3-D {cycle}: gpu iters {it} oracle {ito} res {res:.2e}")
+    assert res <= 1e-6
+    assert abs(it - ito) <= 1
+    s64 = _solver(M, m, g, w, True, 3, 0.5, cycle=cycle, prec="c64")
+    x64, s64 = ha.solveLinearSystem(None, q, s64)
+    assert np.linalg.norm(H @ x64 - q) / np.linalg.norm(q) <= 1e-6
