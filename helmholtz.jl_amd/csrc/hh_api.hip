@@ -10,7 +10,9 @@
 #include <algorithm>
 #include <chrono>
 #include <map>
+#include <atomic>
 #include <mutex>
+#include <thread>
 #include <tuple>
 #include <cmath>
 #include <cstdio>
@@ -653,7 +655,7 @@ void run_fgmres(hh_solver* S, int nrhs, hh_solve_stats* stats) {
     for (int j = 0; j < K; ++j) {
       const cx<double>* vj = S->V + (int64_t)j * blk;
       void* zj = (char*)S->Z + (size_t)j * blk * zes;
-      S->cap_now = S->cap_buf && outer == 0 && j >= S->cap_lo && j < S->cap_hi && active == nrhs;
+      S->cap_now = S->cap_buf && nprec >= S->cap_lo && nprec < S->cap_hi && active == nrhs;
       auto body = [&] {
         if (S->p.prec == HH_C64) {
           precondition<float>(S, nrhs, vj, zj, w.done, st);
@@ -1187,6 +1189,70 @@ int hh_dataset_add_from_solver(hh_dataset* d, hh_solver* s, int sweeps) {
     hh_op_destroy(op);
     if (s->net == nullptr) s->free_learned();
     HIPCK(hipGetLastError());
+  });
+}
+
+int hh_stream_solve(int nprob, const hh_problem* probs, int nworkers, hh_problem_result* results) {
+  // Uneven stream: independent 2-D/3-D problems of different sizes and RHS
+  // counts.  Workers take problems largest-first (N * nrhs) from a shared
+  // counter; each problem gets its own operator, solver and HIP stream, so
+  // launches of different problems overlap on the device.  Results per
+  // problem are bit-identical to solving it alone.
+  return guarded([&] {
+    if (nprob < 0 || (nprob > 0 && (!probs || !results))) throw HHError(HH_ERR_ARG, "bad argument");
+    if (nworkers < 1 || nworkers > 16) throw HHError(HH_ERR_ARG, "nworkers must be in 1..16");
+    std::vector<int> order(nprob);
+    for (int i = 0; i < nprob; ++i) order[i] = i;
+    auto work = [&](int i) {
+      const hh_problem& p = probs[i];
+      int64_t N = 1;
+      for (int d = 0; d < p.dim; ++d) N *= p.nnodes[d];
+      return (double)N * (double)std::max<int64_t>(1, p.nrhs);
+    };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work(a) > work(b); });
+    std::atomic<int> next{0};
+    std::mutex err_mu;
+    std::string first_err;
+    int first_code = HH_OK;
+    auto worker = [&]() {
+      for (;;) {
+        const int k = next.fetch_add(1);
+        if (k >= nprob) return;
+        const int i = order[k];
+        const hh_problem& p = probs[i];
+        hh_problem_result& r = results[i];
+        r.code = HH_OK;
+        const double t0 = now_s();
+        hh_op* op = nullptr;
+        hh_solver* sv = nullptr;
+        int rc = hh_op_create(p.dim, p.nnodes, p.h, p.m, p.gamma, p.omega_re, p.omega_im, p.neumann_top,
+                              p.sommerfeld, p.bc_order, &op);
+        if (rc == HH_OK) rc = hh_solver_create(op, &p.params, p.shift_per_level, &sv);
+        if (rc == HH_OK && p.net) rc = hh_solver_attach_unet(sv, p.net, p.learn_level);
+        if (rc == HH_OK) {
+          hh_solve_stats st{};
+          st.iters = p.iters;
+          st.status = p.status;
+          st.true_res = p.true_res;
+          rc = hh_solve(sv, p.B, p.X, p.nrhs, 0, p.loc, &st);
+          r.max_iters = st.max_iters;
+          r.n_converged = st.n_converged;
+          r.fell_back = st.fell_back;
+        }
+        if (rc != HH_OK) {
+          r.code = rc;
+          std::lock_guard<std::mutex> lk(err_mu);
+          if (first_code == HH_OK) first_code = rc, first_err = g_err;
+        }
+        if (sv) hh_solver_destroy(sv);
+        if (op) hh_op_destroy(op);
+        r.seconds = now_s() - t0;
+      }
+    };
+    std::vector<std::thread> th;
+    for (int w = 0; w < std::min(nworkers, std::max(1, nprob)); ++w) th.emplace_back(worker);
+    for (auto& t : th) t.join();
+    if (first_code != HH_OK) throw HHError(first_code, "stream: " + first_err);
   });
 }
 

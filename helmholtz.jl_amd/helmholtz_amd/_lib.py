@@ -24,7 +24,7 @@ EXPORTS = (
     "hh_unet_num_params", "hh_unet_create", "hh_unet_destroy", "hh_unet_get_params", "hh_unet_set_params",
     "hh_unet_forward", "hh_unet_loss_grad", "hh_unet_adam_step", "hh_dataset_create", "hh_dataset_destroy",
     "hh_dataset_size", "hh_solver_capture_begin", "hh_solver_capture_count", "hh_dataset_add_from_solver",
-    "hh_unet_train", "hh_unet_eval", "hh_solver_attach_unet",
+    "hh_unet_train", "hh_unet_eval", "hh_solver_attach_unet", "hh_stream_solve",
 )
 
 

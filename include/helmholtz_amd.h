@@ -169,9 +169,10 @@ int hh_unet_adam_step(hh_unet* u, const float* grad, float lr, float beta1, floa
 int hh_dataset_create(int nx, int ny, int64_t capacity, hh_dataset** out);
 int hh_dataset_destroy(hh_dataset* d);
 int hh_dataset_size(const hh_dataset* d, int64_t* n);
-/* Record the right-hand sides that reach grid level `level` during FGMRES
- * iterations iter_lo..iter_hi-1 of the first restart cycle of the solver's
- * next hh_solve calls (complex64 preconditioner). */
+/* Record the right-hand sides that reach grid level `level` (first visit per
+ * preconditioner application) during FGMRES iterations iter_lo..iter_hi-1
+ * (counted across restarts, while every RHS is still active) of the solver's
+ * next hh_solve calls (complex64 preconditioner), up to max_samples. */
 int hh_solver_capture_begin(hh_solver* s, int level, int64_t max_samples, int iter_lo, int iter_hi);
 int hh_solver_capture_count(const hh_solver* s, int64_t* n);
 /* Append the captured residuals r with targets e = A_level^-1 r and the
@@ -188,6 +189,42 @@ int hh_unet_eval(hh_unet* u, hh_dataset* d, int64_t first, int64_t count, float*
  * plain cycle takes over if a restart cycle stops reducing every residual by
  * 10% (or produces non-finite values).  net = NULL detaches. */
 int hh_solver_attach_unet(hh_solver* s, hh_unet* net, int level);
+
+/* ---- Uneven stream of independent problems (no counterpart in the reference) ---- */
+typedef struct hh_problem {
+  int32_t dim;                 /* 2 or 3 */
+  int64_t nnodes[3];
+  double h[3];
+  const double* m;             /* host, N */
+  const double* gamma;         /* host, N (absorbing layer included) */
+  double omega_re, omega_im;
+  int32_t neumann_top, sommerfeld, bc_order;
+  hh_mg_params params;
+  const double* shift_per_level;
+  const void* B;               /* complex128 N x nrhs */
+  void* X;
+  int64_t nrhs;
+  int32_t loc;                 /* HH_HOST / HH_DEVICE */
+  int32_t* iters;              /* optional per-RHS outputs (nrhs each) */
+  int32_t* status;
+  double* true_res;
+  hh_unet* net;                /* optional learned correction */
+  int32_t learn_level;
+} hh_problem;
+
+typedef struct hh_problem_result {
+  int32_t code;                /* HH_OK or the problem's error code */
+  int32_t max_iters;
+  int32_t n_converged;
+  int32_t fell_back;
+  double seconds;              /* wall time of create + solve + destroy */
+} hh_problem_result;
+
+/* Solve every problem (largest N * nrhs first) with `nworkers` host threads,
+ * each problem on its own stream; the call returns when all are done.  A
+ * handle per worker thread; a failing problem reports its code and the call
+ * returns the first error after the others finished. */
+int hh_stream_solve(int nprob, const hh_problem* probs, int nworkers, hh_problem_result* results);
 
 #ifdef __cplusplus
 }
