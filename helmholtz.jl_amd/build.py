@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libhelmholtz_amd.so")
 OBJ = os.path.join(HERE, "_build")
-SOURCES = ["hh_stencil.hip", "hh_krylov.hip", "hh_api.hip", "hh_unet.hip", "hh_learn.hip"]
+SOURCES = ["hh_stencil.hip", "hh_krylov.hip", "hh_fused.hip", "hh_api.hip", "hh_unet.hip", "hh_learn.hip"]
 ARCH = os.environ.get("HH_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
@@ -35,7 +35,7 @@ def _compile(src: str) -> str:
     return obj
 
 
-def build(jobs: int = 5) -> str:
+def build(jobs: int = 6) -> str:
     os.makedirs(OBJ, exist_ok=True)
     with ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
         objs = list(ex.map(_compile, SOURCES))

@@ -368,10 +368,8 @@ void setup_learned(hh_solver* S) {
   HIPCK(hipGetLastError());
 }
 
-const cx<double>* coarse_inv(hh_solver* S) {
-  const int64_t n = S->lv.back().g.N;
-  return S->coarse_aug + n * n;
-}
+// row-major A^-1 of the coarsest level (left half of the Gauss-Jordan workspace)
+const cx<double>* coarse_inv(hh_solver* S) { return S->coarse_aug; }
 
 void ensure_work(hh_solver* S, int nrhs) {
   if (nrhs <= S->cap) return;
@@ -498,6 +496,11 @@ struct Cycle {
   hipStream_t st;
 
   void* other(Level& L) { return L.cur == L.xa ? L.xb : L.xa; }
+  // fused legs: 2-D mirror-ghost levels with a coarser level below and V/W(2,2)
+  bool use_fused(const Level& L) const {
+    const size_t l = &L - &S->lv[0];
+    return S->p.fused && l + 1 < S->lv.size() && S->p.pre == 2 && S->p.post == 2 && fused_ok(L.g);
+  }
   int rows_of(const Level& L) const {
     const size_t l = &L - &S->lv[0];
     return l < 16 ? S->rows_tuned[l] : 0;
@@ -544,6 +547,27 @@ struct Cycle {
       return;
     }
     const int pre = S->p.pre, post = S->p.post;
+    if (use_fused(L) && zero_init) {
+      Level& C = S->lv[l + 1];
+      const int ncorr = S->p.cycle == 'W' ? 2 : 1;
+      L.cur = first ? first : L.xa;
+      launch_down_fused<T>(L.g, C.g, nrhs, b, (const cx<T>*)L.d, S->p.relax, (cx<T>*)L.cur, (cx<T>*)C.b, skip, st);
+      C.cur = nullptr;
+      run(l + 1, (const cx<T>*)C.b, true, nullptr);
+      for (int c = 1; c < ncorr; ++c) {
+        void* o = other(L);
+        launch_mid_fused<T>(L.g, C.g, nrhs, (const cx<T>*)L.cur, (const cx<T>*)C.cur, b, (const cx<T>*)L.d,
+                            (cx<T>*)o, (cx<T>*)C.b, skip, st);
+        L.cur = o;
+        C.cur = nullptr;
+        run(l + 1, (const cx<T>*)C.b, true, nullptr);
+      }
+      void* o = other(L);
+      launch_up_fused<T>(L.g, C.g, nrhs, (const cx<T>*)L.cur, (const cx<T>*)C.cur, b, (const cx<T>*)L.d,
+                         S->p.relax, (cx<T>*)o, skip, st);
+      L.cur = o;
+      return;
+    }
     if (zero_init) {
       L.cur = first ? first : L.xa;
       if (pre > 0) {
@@ -580,8 +604,9 @@ struct Cycle {
       launch_dense_apply<T>(coarse_inv(S), L0.g.N, nrhs, b, out, skip, st);
       return;
     }
-    // number of buffer flips on level 0: pre (incl. the zero-start sweep) + post
-    const int flips = S->p.pre + S->p.post;
+    // number of buffer writes on level 0: pre (incl. the zero-start sweep) + post,
+    // or for the fused legs down + (corrections - 1) + up
+    const int flips = use_fused(L0) ? (S->p.cycle == 'W' ? 3 : 2) : S->p.pre + S->p.post;
     // choose the ping-pong pair (out, xa) so the last write lands in out
     void* saved_xb = L0.xb;
     L0.xb = out;
@@ -1140,6 +1165,7 @@ int hh_dataset_add_from_solver(hh_dataset* d, hh_solver* s, int sweeps) {
     tp.inner = 10;
     tp.krylov = HH_FGMRES;
     tp.prec = HH_C128;
+    tp.fused = 1;
     std::vector<double> shifts(tp.levels, 0.5);
     cx<double>* rb = nullptr;
     cx<double>* xb = nullptr;
@@ -1253,6 +1279,31 @@ int hh_stream_solve(int nprob, const hh_problem* probs, int nworkers, hh_problem
     for (int w = 0; w < std::min(nworkers, std::max(1, nprob)); ++w) th.emplace_back(worker);
     for (auto& t : th) t.join();
     if (first_code != HH_OK) throw HHError(first_code, "stream: " + first_err);
+  });
+}
+
+int hh_solver_precondition(hh_solver* s, const void* V, void* Z, int64_t nrhs, int loc) {
+  // One application of the multigrid preconditioner, Z = M^-1 V: V is
+  // complex128, Z has the preconditioner's precision.  Test/inspection hook.
+  return guarded([&] {
+    if (!s || !V || !Z) throw HHError(HH_ERR_ARG, "null argument");
+    if (nrhs < 1 || nrhs > 65535) throw HHError(HH_ERR_ARG, "nrhs must be in 1..65535");
+    if (loc != HH_HOST && loc != HH_DEVICE) throw HHError(HH_ERR_ARG, "bad loc");
+    if (!s->built) build_hierarchy(s);
+    ensure_work(s, (int)nrhs);
+    hipStream_t st = s->op->stream;
+    const int64_t N = s->op->g.N;
+    const size_t zb = csz(s->p.prec) * N * nrhs;
+    cx<double>* v = s->V;  // first Krylov block as staging
+    HIPCK(hipMemcpyAsync(v, V, sizeof(cx<double>) * N * nrhs,
+                         loc == HH_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, st));
+    if (s->p.prec == HH_C64)
+      precondition<float>(s, (int)nrhs, v, s->Z, nullptr, st);
+    else
+      precondition<double>(s, (int)nrhs, v, s->Z, nullptr, st);
+    HIPCK(hipMemcpyAsync(Z, s->Z, zb, loc == HH_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, st));
+    HIPCK(hipStreamSynchronize(st));
+    HIPCK(hipGetLastError());
   });
 }
 

@@ -1,0 +1,267 @@
+// Fused multigrid legs for 2-D levels with mirror ghosts (BC = 2) and
+// 2 pre- / 2 post-sweeps, LDS-tiled with halos so each leg reads its inputs
+// and writes its outputs once:
+//
+//   down : x1 = w D^-1 b, x2 = x1 + w D^-1 (b - A x1), r = b - A x2, b_c = R r
+//          reads b, d; writes x2 and b_c          (unfused: 4 kernels, ~9 passes)
+//   mid  : x' = x + P e_c, r = b - A x', b_c = R r
+//          reads x, b, d, e_c; writes x', b_c     (unfused: 3 kernels, ~6.5 passes)
+//   up   : x' = x + P e_c, two Jacobi sweeps
+//          reads x, b, d, e_c; writes x''         (unfused: 3 kernels, ~8.25 passes)
+//
+// A block owns a TX x TY tile of fine nodes (TX, TY even, so it also owns the
+// coarse nodes at even positions inside it) and stages a halo of H nodes in
+// LDS.  Halo positions outside the grid hold the mirrored node's values (x~_{-k}
+// = x_k): the mirror extension of a field is preserved exactly by the
+// operator, the smoother, and the transfers, so ghost positions evolve like
+// the nodes they mirror and every formula runs unchanged on the tile.
+// Per-node arithmetic is the same expression as the unfused kernels.
+#include "hh_kernels.h"
+
+namespace hh {
+
+constexpr int FTX = 64;
+
+// mirror ghost index; positions farther out than one mirror (only loaded by
+// the last tile, never used for an owned output) are clamped into the grid
+__device__ __forceinline__ int mirror_idx(int g, int n) {
+  g = g < 0 ? -g : (g >= n ? 2 * (n - 1) - g : g);
+  return g < 0 ? 0 : (g >= n ? n - 1 : g);
+}
+
+template <typename T>
+__device__ __forceinline__ cx<T> lap5(cx<T> xc, cx<T> xl, cx<T> xr, cx<T> xm, cx<T> xp, T ih1, T ih2) {
+  return ih1 * (((T)2 * xc) - xl - xr) + ih2 * (((T)2 * xc) - xm - xp);
+}
+
+// prolongation of the coarse correction at fine node (i, j) (nodal bilinear)
+template <typename T>
+__device__ __forceinline__ cx<T> prolong_at(const cx<T>* __restrict__ c, int nc1, int i, int j) {
+  const int i0 = i >> 1, j0 = j >> 1;
+  const int i1 = (i & 1) ? i0 + 1 : i0, j1 = (j & 1) ? j0 + 1 : j0;
+  const T wi = (i & 1) ? (T)0.5 : (T)1, wj = (j & 1) ? (T)0.5 : (T)1, wk = (T)1;
+  const int ik[2] = {i0, i1}, jk[2] = {j0, j1};
+  const int ni = (i & 1) ? 2 : 1, nj = (j & 1) ? 2 : 1;
+  cx<T> acc = mk<T>(0, 0);
+  for (int c2 = 0; c2 < nj; ++c2)
+    for (int c1 = 0; c1 < ni; ++c1) acc = acc + (wi * wj * wk) * c[(int64_t)jk[c2] * nc1 + ik[c1]];
+  return acc;
+}
+
+template <typename T, int TY, int H>
+struct Tile {
+  static constexpr int W = FTX + 2 * H;
+  static constexpr int HT = TY + 2 * H;
+  static constexpr int NPIX = W * HT;
+};
+
+// ------------------------------------------------------------------ down leg
+template <typename T, int TY>
+__global__ __launch_bounds__(256) void k_down_fused(Grid gf, Grid gc, const cx<T>* __restrict__ b,
+                                                    const cx<T>* __restrict__ d, T wj, cx<T>* __restrict__ xo,
+                                                    cx<T>* __restrict__ bco, const int* __restrict__ skip) {
+  constexpr int H = 3;
+  typedef Tile<T, TY, H> Tl;
+  __shared__ cx<T> sb[Tl::NPIX], sd[Tl::NPIX], sx[Tl::NPIX], sr[Tl::NPIX];
+  const int rhs = blockIdx.z;
+  if (skip && skip[rhs]) return;
+  const int n1 = gf.n1, n2 = gf.n2;
+  const int fx0 = blockIdx.x * FTX, fy0 = blockIdx.y * TY;
+  const cx<T>* bb = b + (int64_t)rhs * gf.N;
+  const T ih1 = (T)gf.ih1, ih2 = (T)gf.ih2;
+  const T ld0 = (T)(gf.ih1 * 2.0 + gf.ih2 * 2.0);        // first sweep (as k_jacobi0)
+  const T ldx = (T)(gf.ih1 * 2.0), ldy = (T)(gf.ih2 * 2.0);  // later sweeps (as k_stencil2d)
+  for (int q = threadIdx.x; q < Tl::NPIX; q += 256) {
+    const int li = q % Tl::W, lj = q / Tl::W;
+    const int64_t p = (int64_t)mirror_idx(fy0 - H + lj, n2) * n1 + mirror_idx(fx0 - H + li, n1);
+    sb[q] = bb[p];
+    sd[q] = d[p];
+  }
+  __syncthreads();
+  // x2 on the halo-2 region, x1 recomputed pointwise from b and d
+  auto x1 = [&](int q) {
+    cx<T> D = sd[q];
+    D.re += ld0;
+    return wj * (sb[q] * cinv(D));
+  };
+  for (int q = threadIdx.x; q < Tl::NPIX; q += 256) {
+    const int li = q % Tl::W, lj = q / Tl::W;
+    if (li < 1 || li >= Tl::W - 1 || lj < 1 || lj >= Tl::HT - 1) continue;
+    const cx<T> xc = x1(q);
+    const cx<T> Ax = lap5(xc, x1(q - 1), x1(q + 1), x1(q - Tl::W), x1(q + Tl::W), ih1, ih2) + sd[q] * xc;
+    cx<T> D = sd[q];
+    D.re += ldx + ldy;
+    sx[q] = xc + wj * ((sb[q] - Ax) * cinv(D));
+  }
+  __syncthreads();
+  // residual on the halo-1 region
+  for (int q = threadIdx.x; q < Tl::NPIX; q += 256) {
+    const int li = q % Tl::W, lj = q / Tl::W;
+    if (li < 2 || li >= Tl::W - 2 || lj < 2 || lj >= Tl::HT - 2) continue;
+    const cx<T> xc = sx[q];
+    const cx<T> Ax = lap5(xc, sx[q - 1], sx[q + 1], sx[q - Tl::W], sx[q + Tl::W], ih1, ih2) + sd[q] * xc;
+    sr[q] = sb[q] - Ax;
+  }
+  __syncthreads();
+  cx<T>* xb = xo + (int64_t)rhs * gf.N;
+  for (int q = threadIdx.x; q < FTX * TY; q += 256) {
+    const int i = fx0 + q % FTX, j = fy0 + q / FTX;
+    if (i < n1 && j < n2) xb[(int64_t)j * n1 + i] = sx[(q / FTX + H) * Tl::W + q % FTX + H];
+  }
+  // full weighting of r onto the owned coarse nodes
+  cx<T>* cb = bco + (int64_t)rhs * gc.N;
+  const T w[3] = {(T)0.25, (T)0.5, (T)0.25};
+  for (int q = threadIdx.x; q < (FTX / 2) * (TY / 2); q += 256) {
+    const int I = fx0 / 2 + q % (FTX / 2), J = fy0 / 2 + q / (FTX / 2);
+    if (I >= gc.n1 || J >= gc.n2) continue;
+    const int lc = (2 * J - fy0 + H) * Tl::W + (2 * I - fx0 + H);
+    cx<T> acc = mk<T>(0, 0);
+    for (int bb2 = 0; bb2 < 3; ++bb2)
+      for (int a = 0; a < 3; ++a) acc = acc + ((T)1 * w[bb2] * w[a]) * sr[lc + (bb2 - 1) * Tl::W + (a - 1)];
+    cb[(int64_t)J * gc.n1 + I] = acc;
+  }
+}
+
+// ------------------------------------------------------------------ middle (W)
+template <typename T, int TY>
+__global__ __launch_bounds__(256) void k_mid_fused(Grid gf, Grid gc, const cx<T>* __restrict__ x,
+                                                   const cx<T>* __restrict__ ec, const cx<T>* __restrict__ b,
+                                                   const cx<T>* __restrict__ d, cx<T>* __restrict__ xo,
+                                                   cx<T>* __restrict__ bco, const int* __restrict__ skip) {
+  constexpr int H = 2;
+  typedef Tile<T, TY, H> Tl;
+  __shared__ cx<T> sx[Tl::NPIX], sb[Tl::NPIX], sd[Tl::NPIX], sr[Tl::NPIX];
+  const int rhs = blockIdx.z;
+  if (skip && skip[rhs]) return;
+  const int n1 = gf.n1, n2 = gf.n2;
+  const int fx0 = blockIdx.x * FTX, fy0 = blockIdx.y * TY;
+  const cx<T>* xr = x + (int64_t)rhs * gf.N;
+  const cx<T>* bb = b + (int64_t)rhs * gf.N;
+  const cx<T>* cr = ec + (int64_t)rhs * gc.N;
+  const T ih1 = (T)gf.ih1, ih2 = (T)gf.ih2;
+  for (int q = threadIdx.x; q < Tl::NPIX; q += 256) {
+    const int li = q % Tl::W, lj = q / Tl::W;
+    const int gi = mirror_idx(fx0 - H + li, n1), gj = mirror_idx(fy0 - H + lj, n2);
+    const int64_t p = (int64_t)gj * n1 + gi;
+    sx[q] = xr[p] + prolong_at(cr, gc.n1, gi, gj);
+    sb[q] = bb[p];
+    sd[q] = d[p];
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < Tl::NPIX; q += 256) {
+    const int li = q % Tl::W, lj = q / Tl::W;
+    if (li < 1 || li >= Tl::W - 1 || lj < 1 || lj >= Tl::HT - 1) continue;
+    const cx<T> xc = sx[q];
+    const cx<T> Ax = lap5(xc, sx[q - 1], sx[q + 1], sx[q - Tl::W], sx[q + Tl::W], ih1, ih2) + sd[q] * xc;
+    sr[q] = sb[q] - Ax;
+  }
+  __syncthreads();
+  cx<T>* xb = xo + (int64_t)rhs * gf.N;
+  for (int q = threadIdx.x; q < FTX * TY; q += 256) {
+    const int i = fx0 + q % FTX, j = fy0 + q / FTX;
+    if (i < n1 && j < n2) xb[(int64_t)j * n1 + i] = sx[(q / FTX + H) * Tl::W + q % FTX + H];
+  }
+  cx<T>* cb = bco + (int64_t)rhs * gc.N;
+  const T w[3] = {(T)0.25, (T)0.5, (T)0.25};
+  for (int q = threadIdx.x; q < (FTX / 2) * (TY / 2); q += 256) {
+    const int I = fx0 / 2 + q % (FTX / 2), J = fy0 / 2 + q / (FTX / 2);
+    if (I >= gc.n1 || J >= gc.n2) continue;
+    const int lc = (2 * J - fy0 + H) * Tl::W + (2 * I - fx0 + H);
+    cx<T> acc = mk<T>(0, 0);
+    for (int bb2 = 0; bb2 < 3; ++bb2)
+      for (int a = 0; a < 3; ++a) acc = acc + ((T)1 * w[bb2] * w[a]) * sr[lc + (bb2 - 1) * Tl::W + (a - 1)];
+    cb[(int64_t)J * gc.n1 + I] = acc;
+  }
+}
+
+// ------------------------------------------------------------------ up leg
+template <typename T, int TY>
+__global__ __launch_bounds__(256) void k_up_fused(Grid gf, Grid gc, const cx<T>* __restrict__ x,
+                                                  const cx<T>* __restrict__ ec, const cx<T>* __restrict__ b,
+                                                  const cx<T>* __restrict__ d, T wj, cx<T>* __restrict__ xo,
+                                                  const int* __restrict__ skip) {
+  constexpr int H = 2;
+  typedef Tile<T, TY, H> Tl;
+  __shared__ cx<T> sx[Tl::NPIX], sb[Tl::NPIX], sd[Tl::NPIX], s3[Tl::NPIX];
+  const int rhs = blockIdx.z;
+  if (skip && skip[rhs]) return;
+  const int n1 = gf.n1, n2 = gf.n2;
+  const int fx0 = blockIdx.x * FTX, fy0 = blockIdx.y * TY;
+  const cx<T>* xr = x + (int64_t)rhs * gf.N;
+  const cx<T>* bb = b + (int64_t)rhs * gf.N;
+  const cx<T>* cr = ec + (int64_t)rhs * gc.N;
+  const T ih1 = (T)gf.ih1, ih2 = (T)gf.ih2;
+  const T ldx = (T)(gf.ih1 * 2.0), ldy = (T)(gf.ih2 * 2.0);
+  for (int q = threadIdx.x; q < Tl::NPIX; q += 256) {
+    const int li = q % Tl::W, lj = q / Tl::W;
+    const int gi = mirror_idx(fx0 - H + li, n1), gj = mirror_idx(fy0 - H + lj, n2);
+    const int64_t p = (int64_t)gj * n1 + gi;
+    sx[q] = xr[p] + prolong_at(cr, gc.n1, gi, gj);
+    sb[q] = bb[p];
+    sd[q] = d[p];
+  }
+  __syncthreads();
+  auto jac = [&](const cx<T>* s, int q) {
+    const cx<T> xc = s[q];
+    const cx<T> Ax = lap5(xc, s[q - 1], s[q + 1], s[q - Tl::W], s[q + Tl::W], ih1, ih2) + sd[q] * xc;
+    cx<T> D = sd[q];
+    D.re += ldx + ldy;
+    return xc + wj * ((sb[q] - Ax) * cinv(D));
+  };
+  for (int q = threadIdx.x; q < Tl::NPIX; q += 256) {
+    const int li = q % Tl::W, lj = q / Tl::W;
+    if (li < 1 || li >= Tl::W - 1 || lj < 1 || lj >= Tl::HT - 1) continue;
+    s3[q] = jac(sx, q);
+  }
+  __syncthreads();
+  cx<T>* xb = xo + (int64_t)rhs * gf.N;
+  for (int q = threadIdx.x; q < FTX * TY; q += 256) {
+    const int i = fx0 + q % FTX, j = fy0 + q / FTX;
+    if (i < n1 && j < n2) xb[(int64_t)j * n1 + i] = jac(s3, (q / FTX + H) * Tl::W + q % FTX + H);
+  }
+}
+
+// ------------------------------------------------------------------ launchers
+template <typename T>
+struct FusedTY {
+  static constexpr int v = sizeof(T) == 4 ? 16 : 8;
+};
+
+bool fused_ok(const Grid& gf) { return gf.dim == 2 && gf.bc == 2 && gf.n1 >= 16 && gf.n2 >= 16; }
+
+template <typename T>
+void launch_down_fused(const Grid& gf, const Grid& gc, int nrhs, const cx<T>* b, const cx<T>* d, double wj, cx<T>* xo,
+                       cx<T>* bco, const int* skip, hipStream_t s) {
+  constexpr int TY = FusedTY<T>::v;
+  dim3 grid((gf.n1 + FTX - 1) / FTX, (gf.n2 + TY - 1) / TY, nrhs);
+  hipLaunchKernelGGL((k_down_fused<T, TY>), grid, dim3(256), 0, s, gf, gc, b, d, (T)wj, xo, bco, skip);
+}
+
+template <typename T>
+void launch_mid_fused(const Grid& gf, const Grid& gc, int nrhs, const cx<T>* x, const cx<T>* ec, const cx<T>* b,
+                      const cx<T>* d, cx<T>* xo, cx<T>* bco, const int* skip, hipStream_t s) {
+  constexpr int TY = FusedTY<T>::v;
+  dim3 grid((gf.n1 + FTX - 1) / FTX, (gf.n2 + TY - 1) / TY, nrhs);
+  hipLaunchKernelGGL((k_mid_fused<T, TY>), grid, dim3(256), 0, s, gf, gc, x, ec, b, d, xo, bco, skip);
+}
+
+template <typename T>
+void launch_up_fused(const Grid& gf, const Grid& gc, int nrhs, const cx<T>* x, const cx<T>* ec, const cx<T>* b,
+                     const cx<T>* d, double wj, cx<T>* xo, const int* skip, hipStream_t s) {
+  constexpr int TY = FusedTY<T>::v;
+  dim3 grid((gf.n1 + FTX - 1) / FTX, (gf.n2 + TY - 1) / TY, nrhs);
+  hipLaunchKernelGGL((k_up_fused<T, TY>), grid, dim3(256), 0, s, gf, gc, x, ec, b, d, (T)wj, xo, skip);
+}
+
+#define HH_INST(T)                                                                                              \
+  template void launch_down_fused<T>(const Grid&, const Grid&, int, const cx<T>*, const cx<T>*, double, cx<T>*, \
+                                     cx<T>*, const int*, hipStream_t);                                          \
+  template void launch_mid_fused<T>(const Grid&, const Grid&, int, const cx<T>*, const cx<T>*, const cx<T>*,    \
+                                    const cx<T>*, cx<T>*, cx<T>*, const int*, hipStream_t);                     \
+  template void launch_up_fused<T>(const Grid&, const Grid&, int, const cx<T>*, const cx<T>*, const cx<T>*,     \
+                                   const cx<T>*, double, cx<T>*, const int*, hipStream_t);
+HH_INST(float)
+HH_INST(double)
+#undef HH_INST
+
+}  // namespace hh

@@ -30,11 +30,24 @@ template <typename T>
 void launch_diag(const Grid& g, const double* m, const double* gam, double wre, double wim, int neumann_top,
                  int sommerfeld, const double h[3], cx<T>* d, hipStream_t s);
 void launch_add_const(double* a, double c, int64_t n, hipStream_t s);
+// leaves A^-1 row-major in M[0, n*n) (and column-major in M[n*n, 2*n*n))
 void launch_dense_inverse(const Grid& g, const cx<double>* d, cx<double>* M, int64_t* piv, cx<double>* fac,
                           cx<double>* pinv, hipStream_t s);
 template <typename T>
 void launch_dense_apply(const cx<double>* Ainv, int64_t n, int nrhs, const cx<T>* r, cx<T>* x, const int* skip,
                         hipStream_t s);
+
+// ---- fused 2-D multigrid legs (hh_fused.hip): BC = 2, 2 pre/2 post sweeps ----
+bool fused_ok(const Grid& gf);
+template <typename T>
+void launch_down_fused(const Grid& gf, const Grid& gc, int nrhs, const cx<T>* b, const cx<T>* d, double wj, cx<T>* xo,
+                       cx<T>* bco, const int* skip, hipStream_t s);
+template <typename T>
+void launch_mid_fused(const Grid& gf, const Grid& gc, int nrhs, const cx<T>* x, const cx<T>* ec, const cx<T>* b,
+                      const cx<T>* d, cx<T>* xo, cx<T>* bco, const int* skip, hipStream_t s);
+template <typename T>
+void launch_up_fused(const Grid& gf, const Grid& gc, int nrhs, const cx<T>* x, const cx<T>* ec, const cx<T>* b,
+                     const cx<T>* d, double wj, cx<T>* xo, const int* skip, hipStream_t s);
 
 // ---- batched FGMRES (hh_krylov.hip) ----
 struct KrylovWork {

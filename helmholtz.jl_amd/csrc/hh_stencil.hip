@@ -406,47 +406,49 @@ __global__ __launch_bounds__(256) void k_gj_elim(cx<double>* __restrict__ M, int
   M[r + c * n] = M[r + c * n] - f * M[k + c * n];
 }
 
-// x_b = Ainv r_b in complex128 for every right-hand side.  Each block owns
-// 256 rows x DR right-hand sides, so one streamed column of Ainv serves DR
-// columns of r (the inverse is re-read nrhs/DR times instead of nrhs times).
-constexpr int DR = 8;
+// After elimination the left half of [A | I] is the identity and the right
+// half is A^-1 (column-major); copy A^-1 row-major into the left half so a
+// row is contiguous for the apply below.
+__global__ __launch_bounds__(256) void k_inv_rows(cx<double>* __restrict__ M, int64_t n) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (k >= n * n) return;
+  const int64_t i = k / n, j = k - i * n;
+  M[k] = M[n * n + j * n + i];
+}
+
+// x_b = Ainv r_b in complex128 for every right-hand side.  One wavefront per
+// row i and group of DR right-hand sides: lanes stride the columns of the
+// row-major inverse (coalesced), partial sums are combined by wave shuffles.
+constexpr int DR = 4;
 template <typename T>
-__global__ __launch_bounds__(256) void k_dense_apply(const cx<double>* __restrict__ Ainv, int64_t n, int nrhs,
+__global__ __launch_bounds__(256) void k_dense_apply(const cx<double>* __restrict__ Arow, int64_t n, int nrhs,
                                                      const cx<T>* __restrict__ r, cx<T>* __restrict__ x,
                                                      const int* __restrict__ skip) {
+  const int lane = threadIdx.x & 63;
+  const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   const int rhs0 = blockIdx.y * DR;
-  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  __shared__ cx<double> rs[DR][256];
-  bool live[DR];
-  int nlive = 0;
-#pragma unroll
-  for (int k = 0; k < DR; ++k) {
-    live[k] = rhs0 + k < nrhs && !(skip && skip[rhs0 + k]);
-    nlive += live[k];
-  }
-  if (nlive == 0) return;  // uniform per block
+  if (i >= n) return;  // uniform per wavefront
+  const int nr = min(DR, nrhs - rhs0);
+  const cx<double>* arow = Arow + i * n;
   cx<double> acc[DR];
 #pragma unroll
   for (int k = 0; k < DR; ++k) acc[k] = mk<double>(0.0, 0.0);
-  for (int64_t j0 = 0; j0 < n; j0 += 256) {
-    const int64_t jj = j0 + threadIdx.x;
+  for (int64_t j = lane; j < n; j += 64) {
+    const cx<double> a = arow[j];
 #pragma unroll
     for (int k = 0; k < DR; ++k)
-      rs[k][threadIdx.x] = (jj < n && live[k]) ? cvt<T, double>(r[(int64_t)(rhs0 + k) * n + jj]) : mk<double>(0.0, 0.0);
-    __syncthreads();
-    const int64_t jm = min((int64_t)256, n - j0);
-    if (i < n)
-      for (int64_t t = 0; t < jm; ++t) {
-        const cx<double> a = Ainv[i + (j0 + t) * n];
-#pragma unroll
-        for (int k = 0; k < DR; ++k) acc[k] = acc[k] + a * rs[k][t];
-      }
-    __syncthreads();
+      if (k < nr) acc[k] = acc[k] + a * cvt<T, double>(r[(int64_t)(rhs0 + k) * n + j]);
   }
-  if (i < n)
 #pragma unroll
-    for (int k = 0; k < DR; ++k)
-      if (live[k]) x[(int64_t)(rhs0 + k) * n + i] = cvt<double, T>(acc[k]);
+  for (int k = 0; k < DR; ++k) {
+    double re = acc[k].re, im = acc[k].im;
+    for (int o = 32; o > 0; o >>= 1) {
+      re += __shfl_down(re, o, 64);
+      im += __shfl_down(im, o, 64);
+    }
+    if (lane == 0 && k < nr && !(skip && skip[rhs0 + k]))
+      x[(int64_t)(rhs0 + k) * n + i] = cvt<double, T>(mk<double>(re, im));
+  }
 }
 
 // ---------------------------------------------------------------- launchers
@@ -591,12 +593,13 @@ void launch_dense_inverse(const Grid& g, const cx<double>* d, cx<double>* M, int
     hipLaunchKernelGGL(k_gj_elim, dim3((unsigned)((n + 255) / 256), (unsigned)(2 * n - k)), dim3(256), 0, s, M, n,
                        k, fac);
   }
+  hipLaunchKernelGGL(k_inv_rows, dim3((unsigned)((n * n + 255) / 256)), dim3(256), 0, s, M, n);
 }
 
 template <typename T>
 void launch_dense_apply(const cx<double>* Ainv, int64_t n, int nrhs, const cx<T>* r, cx<T>* x, const int* skip,
                         hipStream_t s) {
-  dim3 grid((unsigned)((n + 255) / 256), (unsigned)((nrhs + DR - 1) / DR));
+  dim3 grid((unsigned)((n + 3) / 4), (unsigned)((nrhs + DR - 1) / DR));
   hipLaunchKernelGGL((k_dense_apply<T>), grid, dim3(256), 0, s, Ainv, n, nrhs, r, x, skip);
 }
 template void launch_dense_apply<double>(const cx<double>*, int64_t, int, const cx<double>*, cx<double>*,

@@ -13,5 +13,5 @@ from .helmholtz import (  # noqa: F401
 )
 from .solver import (  # noqa: F401
     MGparam, ShiftedLaplacianMultigridSolver, clear, copySolver, getMGparam,
-    getShiftedLaplacianMultigridSolver, hierarchy_levels, solveLinearSystem,
+    getShiftedLaplacianMultigridSolver, hierarchy_levels, precondition, solveLinearSystem,
 )

@@ -24,7 +24,7 @@ EXPORTS = (
     "hh_unet_num_params", "hh_unet_create", "hh_unet_destroy", "hh_unet_get_params", "hh_unet_set_params",
     "hh_unet_forward", "hh_unet_loss_grad", "hh_unet_adam_step", "hh_dataset_create", "hh_dataset_destroy",
     "hh_dataset_size", "hh_solver_capture_begin", "hh_solver_capture_count", "hh_dataset_add_from_solver",
-    "hh_unet_train", "hh_unet_eval", "hh_solver_attach_unet", "hh_stream_solve",
+    "hh_unet_train", "hh_unet_eval", "hh_solver_attach_unet", "hh_stream_solve", "hh_solver_precondition",
 )
 
 
@@ -39,7 +39,7 @@ class MGParams(C.Structure):
         ("levels", C.c_int32), ("pre", C.c_int32), ("post", C.c_int32), ("relax", C.c_double),
         ("cycle", C.c_int32), ("max_outer", C.c_int32), ("tol", C.c_double), ("inner", C.c_int32),
         ("krylov", C.c_int32), ("prec", C.c_int32), ("learned", C.c_int32), ("verbose", C.c_int32),
-        ("graph", C.c_int32), ("autotune", C.c_int32),
+        ("graph", C.c_int32), ("autotune", C.c_int32), ("fused", C.c_int32),
     ]
 
 
@@ -102,6 +102,7 @@ def lib():
     L.hh_unet_train.argtypes = [vp, vp, i32, i32, f32, u64, pf]
     L.hh_unet_eval.argtypes = [vp, vp, i64, i64, pf]
     L.hh_solver_attach_unet.argtypes = [vp, vp, i32]
+    L.hh_solver_precondition.argtypes = [vp, vp, vp, i64, i32]
     for name in EXPORTS:
         if name == "hh_last_error":
             continue

@@ -45,6 +45,7 @@ class MGparam:
     prec: str = "c128"
     graph: bool = False
     autotune: bool = False
+    fused: bool = True
     verbose: bool = False
     _hierarchy_exists: bool = False
 
@@ -148,7 +149,7 @@ def _ensure_handle(s: ShiftedLaplacianMultigridSolver):
                          tol=float(mg.relativeTol), inner=int(s.inner), krylov=L.HH_FGMRES,
                          prec=L.HH_C64 if mg.prec == "c64" else L.HH_C128, learned=0,
                          verbose=int(bool(s.verbose)), graph=int(bool(mg.graph)),
-                         autotune=int(bool(mg.autotune)))
+                         autotune=int(bool(mg.autotune)), fused=int(bool(mg.fused)))
         shifts = np.ascontiguousarray(np.broadcast_to(np.asarray(s.shift, dtype=float), (int(mg.levels),)))
         h = C.c_void_p()
         L.check(L.lib().hh_solver_create(s._op.ptr, C.byref(prm), shifts.ctypes.data_as(C.POINTER(C.c_double)),
@@ -208,6 +209,19 @@ def solveLinearSystem(ShiftedHT, B, param: ShiftedLaplacianMultigridSolver, doTr
     if dev:
         return X, param
     return (X[:, 0] if vec else X), param
+
+
+def precondition(s: ShiftedLaplacianMultigridSolver, V):
+    """One multigrid preconditioner application Z = M^-1 V (inspection hook).
+    V: numpy N x nrhs complex128 (Fortran order); returns Z (complex64 or
+    complex128, the preconditioner's precision)."""
+    _validate(s)
+    _ensure_handle(s)
+    N = int(np.prod(s.helmParam.Mesh.nodes))
+    V2 = np.asfortranarray(np.asarray(V, dtype=np.complex128).reshape(N, -1, order="F"))
+    Z = np.empty(V2.shape, dtype=np.complex64 if s.MG.prec == "c64" else np.complex128, order="F")
+    L.check(L.lib().hh_solver_precondition(s._handle, V2.ctypes.data, Z.ctypes.data, V2.shape[1], L.HH_HOST))
+    return Z
 
 
 def hierarchy_levels(s: ShiftedLaplacianMultigridSolver):

@@ -59,6 +59,8 @@ typedef struct hh_mg_params {
   int32_t verbose;    /* solver.verbose                                          */
   int32_t graph;      /* 1: replay each FGMRES iteration as a captured hipGraph  */
   int32_t autotune;   /* 1: time stencil launch shapes per level at first solve  */
+  int32_t fused;      /* 1: LDS-tiled fused smoothing/transfer legs on 2-D levels
+                         (mirror ghosts, 2 pre / 2 post sweeps); 0: one kernel per step */
 } hh_mg_params;
 
 /* Per-call results.  The arrays are caller-owned with nrhs entries, or NULL. */
@@ -125,6 +127,11 @@ int hh_solver_destroy(hh_solver* s);
  * An all-zero B returns X = 0 (:40-43).  do_transpose must be 0.
  * stats may be NULL. */
 int hh_solve(hh_solver* s, const void* B, void* X, int64_t nrhs, int do_transpose, int loc, hh_solve_stats* stats);
+
+/* One preconditioner application Z = M^-1 V (V complex128, Z in the
+ * preconditioner's precision; N x nrhs).  Builds the hierarchy if needed.
+ * Inspection/test hook with no counterpart in the reference. */
+int hh_solver_precondition(hh_solver* s, const void* V, void* Z, int64_t nrhs, int loc);
 
 /* Number of grid levels the solver's hierarchy has (0 before the first solve)
  * and the node counts of level l. */

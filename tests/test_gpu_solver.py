@@ -254,3 +254,43 @@ def test_uneven_stream_matches_solo_solves():
         s = ha.getShiftedLaplacianMultigridSolver(hp, mg, 0.5, "GMRES", 10)
         Xs, s = ha.solveLinearSystem(None, pr["B"], s)
         assert np.array_equal(X, Xs.reshape(X.shape, order="F"))
+
+
+@pytest.mark.parametrize("cycle", ["V", "W"])
+@pytest.mark.parametrize("fused", [True, False])
+def test_one_cycle_matches_oracle(cycle, fused):
+    """One preconditioner application (complex128) equals the oracle's cycle
+    on the same hierarchy: rediscretised operators, full weighting, bilinear
+    prolongation, damped Jacobi, dense coarsest solve.  Bar 1e-12 relative."""
+    M, m, g, w = small_case((129, 97), seed=61, neumann=True)
+    rng = np.random.default_rng(62)
+    V = np.asfortranarray(rng.standard_normal((m.size, 3)) + 1j * rng.standard_normal((m.size, 3)))
+    s = _solver(M, m, g, w, True, 4, 0.5, cycle=cycle)
+    s.MG.fused = fused
+    Z = ha.precondition(s, V)
+    p = hr.HelmholtzParam(M, g, m, w, True, True)
+    hier = sr.build_hierarchy(p, 4, 0.5)
+    for c in range(3):
+        zo = sr.vcycle(hier, 0, V[:, c], 0.8, 2, 2, cycle)
+        err = np.linalg.norm(Z[:, c] - zo) / np.linalg.norm(zo)
+        assert err <= 1e-12, err
+
+
+@pytest.mark.parametrize("nodes", [(257, 129), (129, 65), (97, 33), (1025, 513)])
+@pytest.mark.parametrize("prec", ["c64", "c128"])
+def test_fused_legs_equal_unfused(nodes, prec):
+    """LDS-tiled fused down/middle/up legs give the same cycle as one kernel
+    per step (tiles that straddle the grid edge included)."""
+    M, m, g, w = small_case(nodes, seed=63, neumann=True)
+    rng = np.random.default_rng(64)
+    nr = 2
+    V = np.asfortranarray(rng.standard_normal((m.size, nr)) + 1j * rng.standard_normal((m.size, nr)))
+    zs = []
+    levels = 6 if nodes[0] > 600 else 4
+    for fused in (True, False):
+        s = _solver(M, m, g, w, True, levels, 0.5, cycle="W", prec=prec)
+        s.MG.fused = fused
+        zs.append(ha.precondition(s, V).astype(np.complex128))
+    err = np.linalg.norm(zs[0] - zs[1]) / np.linalg.norm(zs[1])
+    print(f"fused vs unfused {nodes} {prec}: {err:.2e}")
+    assert err <= (2e-6 if prec == "c64" else 1e-13)
