@@ -262,6 +262,7 @@ struct hh_solver {
     V = nullptr; Z = nullptr; xk = bk = wk = nullptr; b0 = nullptr;
     dfree(kw.part); dfree(kw.partn); dfree(kw.h1); dfree(kw.h2); dfree(kw.nrm); dfree(kw.H); dfree(kw.cs);
     dfree(kw.sn); dfree(kw.gv); dfree(kw.y); dfree(kw.bnorm); dfree(kw.est); dfree(kw.done); dfree(kw.kact);
+    dfree(kw.vscale); dfree(kw.fix);
     kw = KrylovWork{};
     dfree(skip_copy); dfree(conv_d); dfree(n2tmp); dfree(res_d);
     skip_copy = nullptr; conv_d = nullptr; n2tmp = nullptr; res_d = nullptr;
@@ -413,6 +414,9 @@ void ensure_work(hh_solver* S, int nrhs) {
   w.est = dalloc<double>(nrhs);
   w.done = dalloc<int>(nrhs);
   w.kact = dalloc<int>(nrhs);
+  w.lazy = pr == HH_C64;
+  w.vscale = dalloc<double>((size_t)(K + 1) * nrhs);
+  w.fix = dalloc<int>(nrhs);
   S->skip_copy = dalloc<int>(nrhs);
   S->conv_d = dalloc<int>(nrhs);
   S->n2tmp = dalloc<double>(nrhs);
@@ -496,6 +500,10 @@ struct Cycle {
   hipStream_t st;
 
   void* other(Level& L) { return L.cur == L.xa ? L.xb : L.xa; }
+  // coarse-grid corrections per visit of level l (W: 2 on the top wlevels levels)
+  int ncorr_at(int l) const {
+    return (S->p.cycle == 'W' && (S->p.wlevels <= 0 || l < S->p.wlevels)) ? 2 : 1;
+  }
   // fused legs: 2-D mirror-ghost levels with a coarser level below and V/W(2,2)
   bool use_fused(const Level& L) const {
     const size_t l = &L - &S->lv[0];
@@ -549,7 +557,7 @@ struct Cycle {
     const int pre = S->p.pre, post = S->p.post;
     if (use_fused(L) && zero_init) {
       Level& C = S->lv[l + 1];
-      const int ncorr = S->p.cycle == 'W' ? 2 : 1;
+      const int ncorr = ncorr_at(l);
       L.cur = first ? first : L.xa;
       launch_down_fused<T>(L.g, C.g, nrhs, b, (const cx<T>*)L.d, S->p.relax, (cx<T>*)L.cur, (cx<T>*)C.b, skip, st);
       C.cur = nullptr;
@@ -583,7 +591,7 @@ struct Cycle {
     // W: two coarse-grid corrections per visit, each from this level's
     // current residual (iterating the coarse cycle on the restricted residual
     // instead diverges with rediscretised coarse operators near kh ~ 2).
-    const int ncorr = S->p.cycle == 'W' ? 2 : 1;
+    const int ncorr = ncorr_at(l);
     for (int c = 0; c < ncorr; ++c) {
       launch_stencil<T, T, T, M_RESID>(L.g, nrhs, (const cx<T>*)L.cur, b, (cx<T>*)L.r, (const cx<T>*)L.d, nullptr,
                                        mk<double>(0, 0), 0.0, skip, st, rows_of(L));
@@ -606,7 +614,7 @@ struct Cycle {
     }
     // number of buffer writes on level 0: pre (incl. the zero-start sweep) + post,
     // or for the fused legs down + (corrections - 1) + up
-    const int flips = use_fused(L0) ? (S->p.cycle == 'W' ? 3 : 2) : S->p.pre + S->p.post;
+    const int flips = use_fused(L0) ? ncorr_at(0) + 1 : S->p.pre + S->p.post;
     // choose the ping-pong pair (out, xa) so the last write lands in out
     void* saved_xb = L0.xb;
     L0.xb = out;
@@ -618,12 +626,20 @@ struct Cycle {
   }
 };
 
+// z = M^-1 (scale * v); scale (per RHS, lazy basis normalisation) may be null
 template <typename T>
-void precondition(hh_solver* S, int nrhs, const cx<double>* vj, void* zj, const int* skip, hipStream_t st) {
+void precondition(hh_solver* S, int nrhs, const cx<double>* vj, void* zj, const int* skip, hipStream_t st,
+                  const double* scale = nullptr) {
   Cycle<T> c{S, nrhs, skip, st};
   const int64_t blk = S->lv[0].g.N * (int64_t)nrhs;
   if (std::is_same<T, float>::value) {
-    launch_cast<double, float>(vj, (cx<float>*)S->b0, blk, st);
+    if (scale) {
+      KrylovWork w = S->kw;
+      w.nrhs = nrhs;
+      kr_cast_scaled(w, vj, scale, (cx<float>*)S->b0, st);
+    } else {
+      launch_cast<double, float>(vj, (cx<float>*)S->b0, blk, st);
+    }
     c.apply((const cx<T>*)S->b0, (cx<T>*)zj);
   } else {
     c.apply((const cx<T>*)vj, (cx<T>*)zj);
@@ -683,7 +699,7 @@ void run_fgmres(hh_solver* S, int nrhs, hh_solve_stats* stats) {
       S->cap_now = S->cap_buf && nprec >= S->cap_lo && nprec < S->cap_hi && active == nrhs;
       auto body = [&] {
         if (S->p.prec == HH_C64) {
-          precondition<float>(S, nrhs, vj, zj, w.done, st);
+          precondition<float>(S, nrhs, vj, zj, w.done, st, w.lazy ? w.vscale + (int64_t)j * nrhs : nullptr);
           launch_stencil<float, double, double, M_APPLY>(S->op->g, nrhs, (const cx<float>*)zj, nullptr, S->wk,
                                                          S->op->d64, nullptr, mk<double>(0, 0), 0.0, w.done, st);
         } else {

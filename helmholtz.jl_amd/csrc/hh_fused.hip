@@ -71,11 +71,27 @@ __global__ __launch_bounds__(256) void k_down_fused(Grid gf, Grid gc, const cx<T
   const T ih1 = (T)gf.ih1, ih2 = (T)gf.ih2;
   const T ld0 = (T)(gf.ih1 * 2.0 + gf.ih2 * 2.0);        // first sweep (as k_jacobi0)
   const T ldx = (T)(gf.ih1 * 2.0), ldy = (T)(gf.ih2 * 2.0);  // later sweeps (as k_stencil2d)
-  for (int q = threadIdx.x; q < Tl::NPIX; q += 256) {
-    const int li = q % Tl::W, lj = q / Tl::W;
-    const int64_t p = (int64_t)mirror_idx(fy0 - H + lj, n2) * n1 + mirror_idx(fx0 - H + li, n1);
-    sb[q] = bb[p];
-    sd[q] = d[p];
+  {
+    constexpr int K = (Tl::NPIX + 255) / 256;
+    cx<T> vb[K], vd[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int q = threadIdx.x + 256 * k;
+      if (q < Tl::NPIX) {
+        const int li = q % Tl::W, lj = q / Tl::W;
+        const int64_t p = (int64_t)mirror_idx(fy0 - H + lj, n2) * n1 + mirror_idx(fx0 - H + li, n1);
+        vb[k] = bb[p];
+        vd[k] = d[p];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int q = threadIdx.x + 256 * k;
+      if (q < Tl::NPIX) {
+        sb[q] = vb[k];
+        sd[q] = vd[k];
+      }
+    }
   }
   __syncthreads();
   // x2 on the halo-2 region, x1 recomputed pointwise from b and d
@@ -139,13 +155,30 @@ __global__ __launch_bounds__(256) void k_mid_fused(Grid gf, Grid gc, const cx<T>
   const cx<T>* bb = b + (int64_t)rhs * gf.N;
   const cx<T>* cr = ec + (int64_t)rhs * gc.N;
   const T ih1 = (T)gf.ih1, ih2 = (T)gf.ih2;
-  for (int q = threadIdx.x; q < Tl::NPIX; q += 256) {
-    const int li = q % Tl::W, lj = q / Tl::W;
-    const int gi = mirror_idx(fx0 - H + li, n1), gj = mirror_idx(fy0 - H + lj, n2);
-    const int64_t p = (int64_t)gj * n1 + gi;
-    sx[q] = xr[p] + prolong_at(cr, gc.n1, gi, gj);
-    sb[q] = bb[p];
-    sd[q] = d[p];
+  {
+    constexpr int K = (Tl::NPIX + 255) / 256;
+    cx<T> vx[K], vb[K], vd[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int q = threadIdx.x + 256 * k;
+      if (q < Tl::NPIX) {
+        const int li = q % Tl::W, lj = q / Tl::W;
+        const int gi = mirror_idx(fx0 - H + li, n1), gj = mirror_idx(fy0 - H + lj, n2);
+        const int64_t p = (int64_t)gj * n1 + gi;
+        vx[k] = xr[p] + prolong_at(cr, gc.n1, gi, gj);
+        vb[k] = bb[p];
+        vd[k] = d[p];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int q = threadIdx.x + 256 * k;
+      if (q < Tl::NPIX) {
+        sx[q] = vx[k];
+        sb[q] = vb[k];
+        sd[q] = vd[k];
+      }
+    }
   }
   __syncthreads();
   for (int q = threadIdx.x; q < Tl::NPIX; q += 256) {
@@ -192,13 +225,30 @@ __global__ __launch_bounds__(256) void k_up_fused(Grid gf, Grid gc, const cx<T>*
   const cx<T>* cr = ec + (int64_t)rhs * gc.N;
   const T ih1 = (T)gf.ih1, ih2 = (T)gf.ih2;
   const T ldx = (T)(gf.ih1 * 2.0), ldy = (T)(gf.ih2 * 2.0);
-  for (int q = threadIdx.x; q < Tl::NPIX; q += 256) {
-    const int li = q % Tl::W, lj = q / Tl::W;
-    const int gi = mirror_idx(fx0 - H + li, n1), gj = mirror_idx(fy0 - H + lj, n2);
-    const int64_t p = (int64_t)gj * n1 + gi;
-    sx[q] = xr[p] + prolong_at(cr, gc.n1, gi, gj);
-    sb[q] = bb[p];
-    sd[q] = d[p];
+  {
+    constexpr int K = (Tl::NPIX + 255) / 256;
+    cx<T> vx[K], vb[K], vd[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int q = threadIdx.x + 256 * k;
+      if (q < Tl::NPIX) {
+        const int li = q % Tl::W, lj = q / Tl::W;
+        const int gi = mirror_idx(fx0 - H + li, n1), gj = mirror_idx(fy0 - H + lj, n2);
+        const int64_t p = (int64_t)gj * n1 + gi;
+        vx[k] = xr[p] + prolong_at(cr, gc.n1, gi, gj);
+        vb[k] = bb[p];
+        vd[k] = d[p];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int q = threadIdx.x + 256 * k;
+      if (q < Tl::NPIX) {
+        sx[q] = vx[k];
+        sb[q] = vb[k];
+        sd[q] = vd[k];
+      }
+    }
   }
   __syncthreads();
   auto jac = [&](const cx<T>* s, int q) {

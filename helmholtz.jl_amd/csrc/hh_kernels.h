@@ -69,7 +69,13 @@ struct KrylovWork {
   double* est;           // [nrhs] residual estimate / ||b||
   int* done;             // [nrhs] 1 = no more work in this cycle
   int* kact;             // [nrhs] Krylov steps used by this RHS in this cycle
+  // lazy normalisation: basis vector i of RHS b is vscale[i][b] * (stored V_i)
+  int lazy;
+  double* vscale;        // [kmax+1][nrhs]
+  int* fix;              // [nrhs] 1 = the second CGS correction is applied this step
 };
+// b0 = (complex64) vscale_j[b] * V_j  (preconditioner input in the lazy mode)
+void kr_cast_scaled(const KrylovWork& w, const cx<double>* Vj, const double* scale, cx<float>* b0, hipStream_t s);
 
 // ||v_b||^2 per RHS into out[b] (RHS with skip[b] != 0 untouched)
 void kr_norm2(const KrylovWork& w, const cx<double>* v, double* out, const int* skip, hipStream_t s);

@@ -61,8 +61,9 @@ __global__ __launch_bounds__(256) void k_dots(KrylovWork kw, const cx<double>* _
 #pragma unroll
   for (int i = 0; i < NV; ++i) {
     if (i > j) break;
-    const double re = block_sum(acc[i].re, sh);
-    const double im = block_sum(acc[i].im, sh);
+    const double si = kw.lazy ? kw.vscale[i * kw.nrhs + b] : 1.0;
+    const double re = block_sum(si * acc[i].re, sh);
+    const double im = block_sum(si * acc[i].im, sh);
     if (threadIdx.x == 0) kw.part[((int64_t)c * (kw.kmax + 1) + i) * kw.nrhs + b] = mk<double>(re, im);
   }
 }
@@ -85,7 +86,7 @@ __global__ __launch_bounds__(64) void k_reduce_dots(KrylovWork kw, cx<double>* _
 // pass 2: w' = w - V h1 ; part = V^H w' ; partn = ||w'||^2
 template <int NV>
 __global__ __launch_bounds__(256) void k_update_dots(KrylovWork kw, const cx<double>* __restrict__ V,
-                                                     cx<double>* __restrict__ w, int j) {
+                                                     const cx<double>* w, cx<double>* wout, int j) {
   const int c = blockIdx.x, b = blockIdx.y;
   if (kw.done[b]) return;
   __shared__ double sh[4];
@@ -97,7 +98,8 @@ __global__ __launch_bounds__(256) void k_update_dots(KrylovWork kw, const cx<dou
   cx<double> acc[NV];
 #pragma unroll
   for (int i = 0; i < NV; ++i) {
-    hc[i] = i <= j ? kw.h1[i * kw.nrhs + b] : mk<double>(0.0, 0.0);
+    const double si = (kw.lazy && i <= j) ? kw.vscale[i * kw.nrhs + b] : 1.0;
+    hc[i] = i <= j ? si * kw.h1[i * kw.nrhs + b] : mk<double>(0.0, 0.0);
     acc[i] = mk<double>(0.0, 0.0);
   }
   double nn = 0.0;
@@ -110,7 +112,7 @@ __global__ __launch_bounds__(256) void k_update_dots(KrylovWork kw, const cx<dou
         vv[i] = V[i * stride + base + p];
         wv = wv - hc[i] * vv[i];
       }
-    w[base + p] = wv;
+    wout[base + p] = wv;
     nn += cabs2(wv);
 #pragma unroll
     for (int i = 0; i < NV; ++i)
@@ -119,8 +121,9 @@ __global__ __launch_bounds__(256) void k_update_dots(KrylovWork kw, const cx<dou
 #pragma unroll
   for (int i = 0; i < NV; ++i) {
     if (i > j) break;
-    const double re = block_sum(acc[i].re, sh);
-    const double im = block_sum(acc[i].im, sh);
+    const double si = kw.lazy ? kw.vscale[i * kw.nrhs + b] : 1.0;
+    const double re = block_sum(si * acc[i].re, sh);
+    const double im = block_sum(si * acc[i].im, sh);
     if (threadIdx.x == 0) kw.part[((int64_t)c * (kw.kmax + 1) + i) * kw.nrhs + b] = mk<double>(re, im);
   }
   const double t = block_sum(nn, sh);
@@ -148,7 +151,7 @@ __global__ __launch_bounds__(64) void k_hess_givens(KrylovWork kw, int j, double
     if (lane == 0) {
       const cx<double> h2 = mk<double>(re, im);
       kw.h2[i * kw.nrhs + b] = h2;
-      col[i] = kw.h1[i * kw.nrhs + b] + h2;
+      col[i] = kw.h1[i * kw.nrhs + b];
       s2 += cabs2(h2);
     }
   }
@@ -156,9 +159,18 @@ __global__ __launch_bounds__(64) void k_hess_givens(KrylovWork kw, int j, double
   for (int c = lane; c < kw.nchunk; c += 64) nn += kw.partn[(int64_t)c * kw.nrhs + b];
   nn = wave_sum(nn);
   if (lane != 0) return;
-  const double hn2 = nn - s2;
+  // second CGS correction only where it is not negligible (lazy mode); the
+  // Hessenberg column then describes exactly the vector that is kept
+  const bool use_h2 = !kw.lazy || s2 > 1e-20 * nn;
+  if (use_h2)
+    for (int i = 0; i <= j; ++i) col[i] = col[i] + kw.h2[i * kw.nrhs + b];
+  const double hn2 = use_h2 ? nn - s2 : nn;
   const double hn = hn2 > 0.0 ? sqrt(hn2) : 0.0;
   kw.nrm[b] = hn > 0.0 ? 1.0 / hn : 0.0;
+  if (kw.lazy) {
+    kw.fix[b] = use_h2;
+    kw.vscale[(j + 1) * kw.nrhs + b] = use_h2 ? 1.0 : kw.nrm[b];
+  }
   col[j + 1] = mk<double>(hn, 0.0);
   const int K = kw.kmax;
   cx<double>* Hb = kw.H + (int64_t)b * (K + 1) * K;
@@ -199,17 +211,20 @@ __global__ __launch_bounds__(64) void k_hess_givens(KrylovWork kw, int j, double
 // pass 3: v_{j+1} = (w' - V h2) * nrm
 template <int NV>
 __global__ __launch_bounds__(256) void k_final(KrylovWork kw, const cx<double>* __restrict__ V,
-                                               const cx<double>* __restrict__ w, cx<double>* __restrict__ vout,
+                                               const cx<double>* w, cx<double>* vout,
                                                int j, const int* __restrict__ skip_before) {
   const int c = blockIdx.x, b = blockIdx.y;
-  if (skip_before[b]) return;
+  if (skip_before[b] || (kw.lazy && !kw.fix[b])) return;
   int64_t p0, p1;
   chunk_range(kw.N, kw.nchunk, c, &p0, &p1);
   const int64_t stride = kw.N * kw.nrhs;
   const int64_t base = (int64_t)b * kw.N;
   cx<double> hc[NV];
 #pragma unroll
-  for (int i = 0; i < NV; ++i) hc[i] = i <= j ? kw.h2[i * kw.nrhs + b] : mk<double>(0.0, 0.0);
+  for (int i = 0; i < NV; ++i) {
+    const double si = (kw.lazy && i <= j) ? kw.vscale[i * kw.nrhs + b] : 1.0;
+    hc[i] = i <= j ? si * kw.h2[i * kw.nrhs + b] : mk<double>(0.0, 0.0);
+  }
   const double s = kw.nrm[b];
   for (int64_t p = p0 + threadIdx.x; p < p1; p += 256) {
     cx<double> wv = w[base + p];
@@ -259,6 +274,7 @@ __global__ __launch_bounds__(64) void k_start_flags(KrylovWork kw, const double*
   kw.est[b] = bn > 0.0 ? beta / bn : 0.0;
   cx<double>* gb = kw.gv + (int64_t)b * (kw.kmax + 1);
   gb[0] = mk<double>(beta, 0.0);
+  if (kw.lazy) kw.vscale[b] = 1.0;
 }
 
 __global__ __launch_bounds__(256) void k_scale(KrylovWork kw, const cx<double>* __restrict__ r,
@@ -314,8 +330,22 @@ __global__ __launch_bounds__(64) void k_div_bnorm(KrylovWork kw, const double* _
   res[b] = bn > 0.0 ? sqrt(n2[b]) / bn : 0.0;
 }
 
+__global__ __launch_bounds__(256) void k_cast_scaled(const cx<double>* __restrict__ v, const double* __restrict__ sc,
+                                                     cx<float>* __restrict__ o, int64_t N, int nrhs) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (k >= N * nrhs) return;
+  const double s = sc[k / N];
+  const cx<double> a = v[k];
+  o[k] = mk<float>((float)(s * a.re), (float)(s * a.im));
+}
+
 // ------------------------------------------------------------------ host side
 static inline dim3 cgrid(const KrylovWork& w) { return dim3(w.nchunk, w.nrhs); }
+
+void kr_cast_scaled(const KrylovWork& w, const cx<double>* Vj, const double* scale, cx<float>* b0, hipStream_t s) {
+  const int64_t n = w.N * (int64_t)w.nrhs;
+  hipLaunchKernelGGL(k_cast_scaled, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, Vj, scale, b0, w.N, w.nrhs);
+}
 
 void kr_norm2(const KrylovWork& w, const cx<double>* v, double* out, const int* skip, hipStream_t s) {
   hipLaunchKernelGGL(k_norm_part, cgrid(w), dim3(256), 0, s, w, v, skip);
@@ -336,17 +366,23 @@ static void arnoldi_nv(const KrylovWork& w, const cx<double>* V, cx<double>* wv,
   const int64_t stride = w.N * w.nrhs;
   hipLaunchKernelGGL((k_dots<NV>), cgrid(w), dim3(256), 0, s, w, V, wv, j);
   hipLaunchKernelGGL(k_reduce_dots, dim3(j + 1, w.nrhs), dim3(64), 0, s, w, w.h1, j);
-  hipLaunchKernelGGL((k_update_dots<NV>), cgrid(w), dim3(256), 0, s, w, V, wv, j);
+  cx<double>* vnext = (cx<double>*)(V + (j + 1) * stride);
+  // lazy: w' goes straight into the next basis slot (kept unnormalised)
+  hipLaunchKernelGGL((k_update_dots<NV>), cgrid(w), dim3(256), 0, s, w, V, wv, w.lazy ? vnext : wv, j);
   hipMemcpyAsync(skip_copy, w.done, sizeof(int) * w.nrhs, hipMemcpyDeviceToDevice, s);
   hipLaunchKernelGGL(k_hess_givens, dim3(w.nrhs), dim3(64), 0, s, w, j, tol);
-  hipLaunchKernelGGL((k_final<NV>), cgrid(w), dim3(256), 0, s, w, V, wv, (cx<double>*)(V + (j + 1) * stride), j,
-                     skip_copy);
+  hipLaunchKernelGGL((k_final<NV>), cgrid(w), dim3(256), 0, s, w, V, w.lazy ? (const cx<double>*)vnext : wv, vnext,
+                     j, skip_copy);
 }
 
 void kr_arnoldi_impl(const KrylovWork& w, const cx<double>* V, cx<double>* wv, int j, double tol, int* skip_copy,
                      hipStream_t s) {
-  if (j < 8)
+  if (j < 4)
+    arnoldi_nv<4>(w, V, wv, j, tol, skip_copy, s);
+  else if (j < 8)
     arnoldi_nv<8>(w, V, wv, j, tol, skip_copy, s);
+  else if (j < 12)
+    arnoldi_nv<12>(w, V, wv, j, tol, skip_copy, s);
   else if (j < 16)
     arnoldi_nv<16>(w, V, wv, j, tol, skip_copy, s);
   else

@@ -417,38 +417,50 @@ __global__ __launch_bounds__(256) void k_inv_rows(cx<double>* __restrict__ M, in
 }
 
 // x_b = Ainv r_b in complex128 for every right-hand side.  One wavefront per
-// row i and group of DR right-hand sides: lanes stride the columns of the
-// row-major inverse (coalesced), partial sums are combined by wave shuffles.
-constexpr int DR = 4;
+// RT rows and DR right-hand sides: lanes stride the columns of the row-major
+// inverse (coalesced), each lane keeps RT x DR partial sums, wave shuffles
+// combine them.  The inverse is re-read nrhs/DR times and r n/RT times.
+constexpr int DR = 4, RT = 4;
 template <typename T>
 __global__ __launch_bounds__(256) void k_dense_apply(const cx<double>* __restrict__ Arow, int64_t n, int nrhs,
                                                      const cx<T>* __restrict__ r, cx<T>* __restrict__ x,
                                                      const int* __restrict__ skip) {
   const int lane = threadIdx.x & 63;
-  const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int64_t i0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * RT;
   const int rhs0 = blockIdx.y * DR;
-  if (i >= n) return;  // uniform per wavefront
+  if (i0 >= n) return;  // uniform per wavefront
   const int nr = min(DR, nrhs - rhs0);
-  const cx<double>* arow = Arow + i * n;
-  cx<double> acc[DR];
+  const int ni = (int)min((int64_t)RT, n - i0);
+  cx<double> acc[RT][DR];
 #pragma unroll
-  for (int k = 0; k < DR; ++k) acc[k] = mk<double>(0.0, 0.0);
+  for (int a = 0; a < RT; ++a)
+#pragma unroll
+    for (int k = 0; k < DR; ++k) acc[a][k] = mk<double>(0.0, 0.0);
   for (int64_t j = lane; j < n; j += 64) {
-    const cx<double> a = arow[j];
+    cx<double> rv[DR];
 #pragma unroll
     for (int k = 0; k < DR; ++k)
-      if (k < nr) acc[k] = acc[k] + a * cvt<T, double>(r[(int64_t)(rhs0 + k) * n + j]);
+      rv[k] = k < nr ? cvt<T, double>(r[(int64_t)(rhs0 + k) * n + j]) : mk<double>(0.0, 0.0);
+#pragma unroll
+    for (int a = 0; a < RT; ++a) {
+      if (a >= ni) break;
+      const cx<double> av = Arow[(i0 + a) * n + j];
+#pragma unroll
+      for (int k = 0; k < DR; ++k) acc[a][k] = acc[a][k] + av * rv[k];
+    }
   }
 #pragma unroll
-  for (int k = 0; k < DR; ++k) {
-    double re = acc[k].re, im = acc[k].im;
-    for (int o = 32; o > 0; o >>= 1) {
-      re += __shfl_down(re, o, 64);
-      im += __shfl_down(im, o, 64);
+  for (int a = 0; a < RT; ++a)
+#pragma unroll
+    for (int k = 0; k < DR; ++k) {
+      double re = acc[a][k].re, im = acc[a][k].im;
+      for (int o = 32; o > 0; o >>= 1) {
+        re += __shfl_down(re, o, 64);
+        im += __shfl_down(im, o, 64);
+      }
+      if (lane == 0 && a < ni && k < nr && !(skip && skip[rhs0 + k]))
+        x[(int64_t)(rhs0 + k) * n + i0 + a] = cvt<double, T>(mk<double>(re, im));
     }
-    if (lane == 0 && k < nr && !(skip && skip[rhs0 + k]))
-      x[(int64_t)(rhs0 + k) * n + i] = cvt<double, T>(mk<double>(re, im));
-  }
 }
 
 // ---------------------------------------------------------------- launchers
@@ -599,7 +611,7 @@ void launch_dense_inverse(const Grid& g, const cx<double>* d, cx<double>* M, int
 template <typename T>
 void launch_dense_apply(const cx<double>* Ainv, int64_t n, int nrhs, const cx<T>* r, cx<T>* x, const int* skip,
                         hipStream_t s) {
-  dim3 grid((unsigned)((n + 3) / 4), (unsigned)((nrhs + DR - 1) / DR));
+  dim3 grid((unsigned)((n + 4 * RT - 1) / (4 * RT)), (unsigned)((nrhs + DR - 1) / DR));
   hipLaunchKernelGGL((k_dense_apply<T>), grid, dim3(256), 0, s, Ainv, n, nrhs, r, x, skip);
 }
 template void launch_dense_apply<double>(const cx<double>*, int64_t, int, const cx<double>*, cx<double>*,

@@ -39,7 +39,7 @@ class MGParams(C.Structure):
         ("levels", C.c_int32), ("pre", C.c_int32), ("post", C.c_int32), ("relax", C.c_double),
         ("cycle", C.c_int32), ("max_outer", C.c_int32), ("tol", C.c_double), ("inner", C.c_int32),
         ("krylov", C.c_int32), ("prec", C.c_int32), ("learned", C.c_int32), ("verbose", C.c_int32),
-        ("graph", C.c_int32), ("autotune", C.c_int32), ("fused", C.c_int32),
+        ("graph", C.c_int32), ("autotune", C.c_int32), ("fused", C.c_int32), ("wlevels", C.c_int32),
     ]
 
 

@@ -46,6 +46,7 @@ class MGparam:
     graph: bool = False
     autotune: bool = False
     fused: bool = True
+    wlevels: int = 0
     verbose: bool = False
     _hierarchy_exists: bool = False
 
@@ -149,7 +150,8 @@ def _ensure_handle(s: ShiftedLaplacianMultigridSolver):
                          tol=float(mg.relativeTol), inner=int(s.inner), krylov=L.HH_FGMRES,
                          prec=L.HH_C64 if mg.prec == "c64" else L.HH_C128, learned=0,
                          verbose=int(bool(s.verbose)), graph=int(bool(mg.graph)),
-                         autotune=int(bool(mg.autotune)), fused=int(bool(mg.fused)))
+                         autotune=int(bool(mg.autotune)), fused=int(bool(mg.fused)),
+                         wlevels=int(mg.wlevels))
         shifts = np.ascontiguousarray(np.broadcast_to(np.asarray(s.shift, dtype=float), (int(mg.levels),)))
         h = C.c_void_p()
         L.check(L.lib().hh_solver_create(s._op.ptr, C.byref(prm), shifts.ctypes.data_as(C.POINTER(C.c_double)),

@@ -72,7 +72,7 @@ def solve_stream(problems, workers: int = 4):
                               inner=int(pr.get("inner", 10)), krylov=L.HH_FGMRES,
                               prec=L.HH_C64 if prec == "c64" else L.HH_C128, learned=0, verbose=0,
                               graph=int(bool(pr.get("graph", False))), autotune=int(bool(pr.get("autotune", False))),
-                              fused=int(bool(pr.get("fused", True))))
+                              fused=int(bool(pr.get("fused", True))), wlevels=int(pr.get("wlevels", 0)))
         a.shift_per_level = shifts.ctypes.data
         a.B, a.X, a.nrhs, a.loc = B.ctypes.data, X.ctypes.data, nrhs, L.HH_HOST
         a.iters, a.status, a.true_res = iters.ctypes.data, status.ctypes.data, tres.ctypes.data

@@ -20,7 +20,7 @@ const HH_FGMRES = Cint(0)
 struct HHMGParams
     levels::Int32; pre::Int32; post::Int32; relax::Float64; cycle::Int32; max_outer::Int32
     tol::Float64; inner::Int32; krylov::Int32; prec::Int32; learned::Int32; verbose::Int32
-    graph::Int32; autotune::Int32; fused::Int32
+    graph::Int32; autotune::Int32; fused::Int32; wlevels::Int32
 end
 
 mutable struct HHSolveStats
@@ -88,7 +88,7 @@ function ensure_handle!(s::ShiftedLaplacianMultigridSolverAMD)
     if s.handle == C_NULL
         mg = s.MG
         prm = Ref(HHMGParams(mg.levels, mg.relaxPre, mg.relaxPost, mg.relaxParam, Int32(mg.cycleType),
-                             mg.maxOuterIter, mg.relativeTol, s.inner, HH_FGMRES, s.prec, 0, s.verbose, 0, 0, 1))
+                             mg.maxOuterIter, mg.relativeTol, s.inner, HH_FGMRES, s.prec, 0, s.verbose, 0, 0, 1, 0))
         out = Ref{Ptr{Cvoid}}(C_NULL)
         check(ccall((:hh_solver_create, LIB), Cint, (Ptr{Cvoid}, Ref{HHMGParams}, Ptr{Float64}, Ptr{Ptr{Cvoid}}),
                     s.op.ptr, prm, s.shift, out))

@@ -61,6 +61,8 @@ typedef struct hh_mg_params {
   int32_t autotune;   /* 1: time stencil launch shapes per level at first solve  */
   int32_t fused;      /* 1: LDS-tiled fused smoothing/transfer legs on 2-D levels
                          (mirror ghosts, 2 pre / 2 post sweeps); 0: one kernel per step */
+  int32_t wlevels;    /* with cycle 'W': only levels 0..wlevels-1 apply the double
+                         correction, coarser levels a single one (0 = every level) */
 } hh_mg_params;
 
 /* Per-call results.  The arrays are caller-owned with nrhs entries, or NULL. */

@@ -120,8 +120,9 @@ def jacobi(lev: Level, x, b, w, sweeps):
     return x
 
 
-def vcycle(hier, l, b, w=0.8, pre=2, post=2, cycle="V"):
-    """One V- or W-cycle for A_l x = b from x = 0.
+def vcycle(hier, l, b, w=0.8, pre=2, post=2, cycle="V", wlevels=0):
+    """One V- or W-cycle for A_l x = b from x = 0 (``wlevels`` > 0: the W
+    double correction only on levels < wlevels, single below).
 
     W: the coarse-grid correction is applied twice per visit, the second time
     from the residual recomputed on this level.  With rediscretised (not
@@ -133,9 +134,9 @@ def vcycle(hier, l, b, w=0.8, pre=2, post=2, cycle="V"):
         return lev.lu.solve(b)
     x = np.zeros_like(b)
     x = jacobi(lev, x, b, w, pre)
-    for _ in range(2 if cycle == "W" else 1):
+    for _ in range(2 if (cycle == "W" and (wlevels <= 0 or l < wlevels)) else 1):
         r = b - lev.A @ x
-        xc = vcycle(hier, l + 1, lev.R @ r, w, pre, post, cycle)
+        xc = vcycle(hier, l + 1, lev.R @ r, w, pre, post, cycle, wlevels)
         x = x + lev.P @ xc
     x = jacobi(lev, x, b, w, post)
     return x
