@@ -681,6 +681,8 @@ void run_fgmres(hh_solver* S, int nrhs, hh_solve_stats* stats) {
   int64_t nprec = 0;
   S->use_learned = S->net != nullptr && S->p.prec == HH_C64;
   S->fell_back = false;
+  int trial = S->use_learned ? 0 : -1;
+  double rho_learned = 1.0;
   if (S->use_learned) setup_learned(S);
   for (int outer = 0; outer < S->p.max_outer; ++outer) {
     HIPCK(hipMemcpyAsync(S->conv_d, conv.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, st));
@@ -758,11 +760,41 @@ void run_fgmres(hh_solver* S, int nrhs, hh_solve_stats* stats) {
         else if (res[b] > 0.9 * prev[b]) stagnated = true;
       }
     }
+    // selection between the learned and the plain cycle, judged by the true
+    // residual: restart cycle 0 runs the network, cycle 1 the plain cycle; the
+    // network is kept only if its cycle reduced the residual clearly more
+    // (geometric mean over the active RHS, 0.8x margin since a first cycle
+    // from b reduces most)
+    double lred = 0.0;
+    int nact = 0;
+    for (int b = 0; b < nrhs; ++b)
+      if (!conv[b] && res[b] == res[b] && res[b] > 0.0 && prev[b] > 0.0) {
+        lred += std::log(res[b] / prev[b]);
+        ++nact;
+      }
+    const double rho = nact ? std::exp(lred / nact) : 0.0;
+    bool bad_any = false;
+    for (int b = 0; b < nrhs; ++b) bad_any |= bad[b] != 0;
+    if (trial == 0 && !bad_any) {
+      rho_learned = rho;
+      S->use_learned = false;  // cycle 1: plain
+      trial = 1;
+      stagnated = false;
+    } else if (trial == 1) {
+      S->use_learned = rho_learned < 0.8 * rho;
+      S->fell_back = !S->use_learned;
+      if (S->p.verbose)
+        fprintf(stderr, "[helmholtz_amd] restart reduction learned %.3f plain %.3f -> %s\n", rho_learned, rho,
+                S->use_learned ? "learned" : "plain");
+      trial = 2;
+      stagnated = false;
+    }
     if (S->use_learned && stagnated) {
       // the learned correction stopped helping: fall back to the plain cycle;
       // columns it drove to non-finite values restart from zero
       S->use_learned = false;
       S->fell_back = true;
+      trial = 2;
       if (S->p.verbose) fprintf(stderr, "[helmholtz_amd] learned correction stagnated; plain cycle from restart %d\n", outer);
       HIPCK(hipMemcpyAsync(S->conv_d, bad.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, st));
       hipLaunchKernelGGL(k_zero_cols, dim3((unsigned)((N + 255) / 256), nrhs), dim3(256), 0, st, S->xk, N, S->conv_d);

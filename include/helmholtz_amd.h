@@ -194,9 +194,13 @@ int hh_unet_train(hh_unet* u, hh_dataset* d, int epochs, int batch, float lr, ui
 /* mean relative error ||e_net - e|| / ||e|| over samples first..first+count-1 */
 int hh_unet_eval(hh_unet* u, hh_dataset* d, int64_t first, int64_t count, float* rel_err);
 /* Use the network as the coarse-grid correction at grid level `level` (its
- * output replaces the cycle below that level) inside the FGMRES solve; the
- * plain cycle takes over if a restart cycle stops reducing every residual by
- * 10% (or produces non-finite values).  net = NULL detaches. */
+ * output replaces the cycle below that level) inside the FGMRES solve.  The
+ * true residual decides: restart cycle 0 runs the network, cycle 1 the plain
+ * cycle, and the solve keeps the network only if its cycle reduced the
+ * residual (geometric mean over RHS) below 0.8x the plain cycle's reduction;
+ * afterwards the plain cycle also takes over if a restart cycle stops
+ * reducing every residual by 10% or produces non-finite values.
+ * hh_solve_stats.fell_back reports the plain cycle was used.  net = NULL detaches. */
 int hh_solver_attach_unet(hh_solver* s, hh_unet* net, int level);
 
 /* ---- Uneven stream of independent problems (no counterpart in the reference) ---- */

@@ -132,3 +132,6 @@ def test_training_lowers_loss_and_learned_solve_converges():
     print(f"ablation 257x257 8 ppw: plain W iters {sp.last_stats['max_iters']} "
           f"({sp.last_stats['solve_time']:.3f} s), learned iters {sl.last_stats['max_iters']} "
           f"({sl.last_stats['solve_time']:.3f} s), fell back: {sl.last_stats['fell_back']}")
+    # the restart-0/restart-1 trial bounds what a weak network can cost: two
+    # restart cycles of 10 iterations at most
+    assert sl.last_stats["max_iters"] <= sp.last_stats["max_iters"] + 20
