@@ -55,6 +55,29 @@ struct Tile {
   static constexpr int NPIX = W * HT;
 };
 
+// Each thread owns the tile positions q = threadIdx.x + 256 k (k < K) in every
+// phase, so b and d live in its registers; only the x fields (and the
+// residual) are staged in LDS, where neighbours are read.  Two LDS arrays per
+// block keep 6-7 blocks resident per CU to overlap one block's loads with
+// another's arithmetic.
+
+// load b and d at the owned positions of a tile whose first node is
+// (fx0 - H, fy0 - H)
+template <typename T, int W, int NPIX, int K, int H>
+__device__ __forceinline__ void load_bd(const cx<T>* __restrict__ bb, const cx<T>* __restrict__ d, int fx0, int fy0,
+                                        int n1, int n2, cx<T> (&rb)[K], cx<T> (&rd)[K]) {
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int q = threadIdx.x + 256 * k;
+    if (q < NPIX) {
+      const int li = q % W, lj = q / W;
+      const int64_t p = (int64_t)mirror_idx(fy0 - H + lj, n2) * n1 + mirror_idx(fx0 - H + li, n1);
+      rb[k] = bb[p];
+      rd[k] = d[p];
+    }
+  }
+}
+
 // ------------------------------------------------------------------ down leg
 template <typename T, int TY>
 __global__ __launch_bounds__(256) void k_down_fused(Grid gf, Grid gc, const cx<T>* __restrict__ b,
@@ -62,67 +85,58 @@ __global__ __launch_bounds__(256) void k_down_fused(Grid gf, Grid gc, const cx<T
                                                     cx<T>* __restrict__ bco, const int* __restrict__ skip) {
   constexpr int H = 3;
   typedef Tile<T, TY, H> Tl;
-  __shared__ cx<T> sb[Tl::NPIX], sd[Tl::NPIX], sx[Tl::NPIX], sr[Tl::NPIX];
+  constexpr int W = Tl::W, NPIX = Tl::NPIX, K = (NPIX + 255) / 256;
+  __shared__ cx<T> s1[NPIX], s2[NPIX];
   const int rhs = blockIdx.z;
   if (skip && skip[rhs]) return;
   const int n1 = gf.n1, n2 = gf.n2;
   const int fx0 = blockIdx.x * FTX, fy0 = blockIdx.y * TY;
-  const cx<T>* bb = b + (int64_t)rhs * gf.N;
   const T ih1 = (T)gf.ih1, ih2 = (T)gf.ih2;
   const T ld0 = (T)(gf.ih1 * 2.0 + gf.ih2 * 2.0);        // first sweep (as k_jacobi0)
   const T ldx = (T)(gf.ih1 * 2.0), ldy = (T)(gf.ih2 * 2.0);  // later sweeps (as k_stencil2d)
-  {
-    constexpr int K = (Tl::NPIX + 255) / 256;
-    cx<T> vb[K], vd[K];
+  cx<T> rb[K], rd[K];
+  load_bd<T, W, NPIX, K, H>(b + (int64_t)rhs * gf.N, d, fx0, fy0, n1, n2, rb, rd);
+  // x1 = w D0^-1 b on the whole tile
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const int q = threadIdx.x + 256 * k;
-      if (q < Tl::NPIX) {
-        const int li = q % Tl::W, lj = q / Tl::W;
-        const int64_t p = (int64_t)mirror_idx(fy0 - H + lj, n2) * n1 + mirror_idx(fx0 - H + li, n1);
-        vb[k] = bb[p];
-        vd[k] = d[p];
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const int q = threadIdx.x + 256 * k;
-      if (q < Tl::NPIX) {
-        sb[q] = vb[k];
-        sd[q] = vd[k];
-      }
+  for (int k = 0; k < K; ++k) {
+    const int q = threadIdx.x + 256 * k;
+    if (q < NPIX) {
+      cx<T> D = rd[k];
+      D.re += ld0;
+      s1[q] = wj * (rb[k] * cinv(D));
     }
   }
   __syncthreads();
-  // x2 on the halo-2 region, x1 recomputed pointwise from b and d
-  auto x1 = [&](int q) {
-    cx<T> D = sd[q];
-    D.re += ld0;
-    return wj * (sb[q] * cinv(D));
-  };
-  for (int q = threadIdx.x; q < Tl::NPIX; q += 256) {
-    const int li = q % Tl::W, lj = q / Tl::W;
-    if (li < 1 || li >= Tl::W - 1 || lj < 1 || lj >= Tl::HT - 1) continue;
-    const cx<T> xc = x1(q);
-    const cx<T> Ax = lap5(xc, x1(q - 1), x1(q + 1), x1(q - Tl::W), x1(q + Tl::W), ih1, ih2) + sd[q] * xc;
-    cx<T> D = sd[q];
-    D.re += ldx + ldy;
-    sx[q] = xc + wj * ((sb[q] - Ax) * cinv(D));
+  // x2 on the halo-2 region
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int q = threadIdx.x + 256 * k;
+    const int li = q % W, lj = q / W;
+    if (q < NPIX && li >= 1 && li < W - 1 && lj >= 1 && lj < Tl::HT - 1) {
+      const cx<T> xc = s1[q];
+      const cx<T> Ax = lap5(xc, s1[q - 1], s1[q + 1], s1[q - W], s1[q + W], ih1, ih2) + rd[k] * xc;
+      cx<T> D = rd[k];
+      D.re += ldx + ldy;
+      s2[q] = xc + wj * ((rb[k] - Ax) * cinv(D));
+    }
   }
   __syncthreads();
-  // residual on the halo-1 region
-  for (int q = threadIdx.x; q < Tl::NPIX; q += 256) {
-    const int li = q % Tl::W, lj = q / Tl::W;
-    if (li < 2 || li >= Tl::W - 2 || lj < 2 || lj >= Tl::HT - 2) continue;
-    const cx<T> xc = sx[q];
-    const cx<T> Ax = lap5(xc, sx[q - 1], sx[q + 1], sx[q - Tl::W], sx[q + Tl::W], ih1, ih2) + sd[q] * xc;
-    sr[q] = sb[q] - Ax;
+  // residual on the halo-1 region (over x1, which is no longer read)
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int q = threadIdx.x + 256 * k;
+    const int li = q % W, lj = q / W;
+    if (q < NPIX && li >= 2 && li < W - 2 && lj >= 2 && lj < Tl::HT - 2) {
+      const cx<T> xc = s2[q];
+      const cx<T> Ax = lap5(xc, s2[q - 1], s2[q + 1], s2[q - W], s2[q + W], ih1, ih2) + rd[k] * xc;
+      s1[q] = rb[k] - Ax;
+    }
   }
   __syncthreads();
   cx<T>* xb = xo + (int64_t)rhs * gf.N;
   for (int q = threadIdx.x; q < FTX * TY; q += 256) {
     const int i = fx0 + q % FTX, j = fy0 + q / FTX;
-    if (i < n1 && j < n2) xb[(int64_t)j * n1 + i] = sx[(q / FTX + H) * Tl::W + q % FTX + H];
+    if (i < n1 && j < n2) xb[(int64_t)j * n1 + i] = s2[(q / FTX + H) * W + q % FTX + H];
   }
   // full weighting of r onto the owned coarse nodes
   cx<T>* cb = bco + (int64_t)rhs * gc.N;
@@ -130,11 +144,32 @@ __global__ __launch_bounds__(256) void k_down_fused(Grid gf, Grid gc, const cx<T
   for (int q = threadIdx.x; q < (FTX / 2) * (TY / 2); q += 256) {
     const int I = fx0 / 2 + q % (FTX / 2), J = fy0 / 2 + q / (FTX / 2);
     if (I >= gc.n1 || J >= gc.n2) continue;
-    const int lc = (2 * J - fy0 + H) * Tl::W + (2 * I - fx0 + H);
+    const int lc = (2 * J - fy0 + H) * W + (2 * I - fx0 + H);
     cx<T> acc = mk<T>(0, 0);
     for (int bb2 = 0; bb2 < 3; ++bb2)
-      for (int a = 0; a < 3; ++a) acc = acc + ((T)1 * w[bb2] * w[a]) * sr[lc + (bb2 - 1) * Tl::W + (a - 1)];
+      for (int a = 0; a < 3; ++a) acc = acc + ((T)1 * w[bb2] * w[a]) * s1[lc + (bb2 - 1) * W + (a - 1)];
     cb[(int64_t)J * gc.n1 + I] = acc;
+  }
+}
+
+// x + P e_c on the owned positions, staged into sx
+template <typename T, int W, int NPIX, int K, int H>
+__device__ __forceinline__ void load_x_prolong(const cx<T>* __restrict__ xr, const cx<T>* __restrict__ cr, int nc1,
+                                               int fx0, int fy0, int n1, int n2, cx<T>* sx) {
+  cx<T> vx[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int q = threadIdx.x + 256 * k;
+    if (q < NPIX) {
+      const int li = q % W, lj = q / W;
+      const int gi = mirror_idx(fx0 - H + li, n1), gj = mirror_idx(fy0 - H + lj, n2);
+      vx[k] = xr[(int64_t)gj * n1 + gi] + prolong_at(cr, nc1, gi, gj);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int q = threadIdx.x + 256 * k;
+    if (q < NPIX) sx[q] = vx[k];
   }
 }
 
@@ -146,63 +181,42 @@ __global__ __launch_bounds__(256) void k_mid_fused(Grid gf, Grid gc, const cx<T>
                                                    cx<T>* __restrict__ bco, const int* __restrict__ skip) {
   constexpr int H = 2;
   typedef Tile<T, TY, H> Tl;
-  __shared__ cx<T> sx[Tl::NPIX], sb[Tl::NPIX], sd[Tl::NPIX], sr[Tl::NPIX];
+  constexpr int W = Tl::W, NPIX = Tl::NPIX, K = (NPIX + 255) / 256;
+  __shared__ cx<T> sx[NPIX], sr[NPIX];
   const int rhs = blockIdx.z;
   if (skip && skip[rhs]) return;
   const int n1 = gf.n1, n2 = gf.n2;
   const int fx0 = blockIdx.x * FTX, fy0 = blockIdx.y * TY;
-  const cx<T>* xr = x + (int64_t)rhs * gf.N;
-  const cx<T>* bb = b + (int64_t)rhs * gf.N;
-  const cx<T>* cr = ec + (int64_t)rhs * gc.N;
   const T ih1 = (T)gf.ih1, ih2 = (T)gf.ih2;
-  {
-    constexpr int K = (Tl::NPIX + 255) / 256;
-    cx<T> vx[K], vb[K], vd[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const int q = threadIdx.x + 256 * k;
-      if (q < Tl::NPIX) {
-        const int li = q % Tl::W, lj = q / Tl::W;
-        const int gi = mirror_idx(fx0 - H + li, n1), gj = mirror_idx(fy0 - H + lj, n2);
-        const int64_t p = (int64_t)gj * n1 + gi;
-        vx[k] = xr[p] + prolong_at(cr, gc.n1, gi, gj);
-        vb[k] = bb[p];
-        vd[k] = d[p];
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const int q = threadIdx.x + 256 * k;
-      if (q < Tl::NPIX) {
-        sx[q] = vx[k];
-        sb[q] = vb[k];
-        sd[q] = vd[k];
-      }
-    }
-  }
+  cx<T> rb[K], rd[K];
+  load_bd<T, W, NPIX, K, H>(b + (int64_t)rhs * gf.N, d, fx0, fy0, n1, n2, rb, rd);
+  load_x_prolong<T, W, NPIX, K, H>(x + (int64_t)rhs * gf.N, ec + (int64_t)rhs * gc.N, gc.n1, fx0, fy0, n1, n2, sx);
   __syncthreads();
-  for (int q = threadIdx.x; q < Tl::NPIX; q += 256) {
-    const int li = q % Tl::W, lj = q / Tl::W;
-    if (li < 1 || li >= Tl::W - 1 || lj < 1 || lj >= Tl::HT - 1) continue;
-    const cx<T> xc = sx[q];
-    const cx<T> Ax = lap5(xc, sx[q - 1], sx[q + 1], sx[q - Tl::W], sx[q + Tl::W], ih1, ih2) + sd[q] * xc;
-    sr[q] = sb[q] - Ax;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int q = threadIdx.x + 256 * k;
+    const int li = q % W, lj = q / W;
+    if (q < NPIX && li >= 1 && li < W - 1 && lj >= 1 && lj < Tl::HT - 1) {
+      const cx<T> xc = sx[q];
+      const cx<T> Ax = lap5(xc, sx[q - 1], sx[q + 1], sx[q - W], sx[q + W], ih1, ih2) + rd[k] * xc;
+      sr[q] = rb[k] - Ax;
+    }
   }
   __syncthreads();
   cx<T>* xb = xo + (int64_t)rhs * gf.N;
   for (int q = threadIdx.x; q < FTX * TY; q += 256) {
     const int i = fx0 + q % FTX, j = fy0 + q / FTX;
-    if (i < n1 && j < n2) xb[(int64_t)j * n1 + i] = sx[(q / FTX + H) * Tl::W + q % FTX + H];
+    if (i < n1 && j < n2) xb[(int64_t)j * n1 + i] = sx[(q / FTX + H) * W + q % FTX + H];
   }
   cx<T>* cb = bco + (int64_t)rhs * gc.N;
   const T w[3] = {(T)0.25, (T)0.5, (T)0.25};
   for (int q = threadIdx.x; q < (FTX / 2) * (TY / 2); q += 256) {
     const int I = fx0 / 2 + q % (FTX / 2), J = fy0 / 2 + q / (FTX / 2);
     if (I >= gc.n1 || J >= gc.n2) continue;
-    const int lc = (2 * J - fy0 + H) * Tl::W + (2 * I - fx0 + H);
+    const int lc = (2 * J - fy0 + H) * W + (2 * I - fx0 + H);
     cx<T> acc = mk<T>(0, 0);
     for (int bb2 = 0; bb2 < 3; ++bb2)
-      for (int a = 0; a < 3; ++a) acc = acc + ((T)1 * w[bb2] * w[a]) * sr[lc + (bb2 - 1) * Tl::W + (a - 1)];
+      for (int a = 0; a < 3; ++a) acc = acc + ((T)1 * w[bb2] * w[a]) * sr[lc + (bb2 - 1) * W + (a - 1)];
     cb[(int64_t)J * gc.n1 + I] = acc;
   }
 }
@@ -215,59 +229,40 @@ __global__ __launch_bounds__(256) void k_up_fused(Grid gf, Grid gc, const cx<T>*
                                                   const int* __restrict__ skip) {
   constexpr int H = 2;
   typedef Tile<T, TY, H> Tl;
-  __shared__ cx<T> sx[Tl::NPIX], sb[Tl::NPIX], sd[Tl::NPIX], s3[Tl::NPIX];
+  constexpr int W = Tl::W, NPIX = Tl::NPIX, K = (NPIX + 255) / 256;
+  __shared__ cx<T> sx[NPIX], s3[NPIX];
   const int rhs = blockIdx.z;
   if (skip && skip[rhs]) return;
   const int n1 = gf.n1, n2 = gf.n2;
   const int fx0 = blockIdx.x * FTX, fy0 = blockIdx.y * TY;
-  const cx<T>* xr = x + (int64_t)rhs * gf.N;
-  const cx<T>* bb = b + (int64_t)rhs * gf.N;
-  const cx<T>* cr = ec + (int64_t)rhs * gc.N;
   const T ih1 = (T)gf.ih1, ih2 = (T)gf.ih2;
   const T ldx = (T)(gf.ih1 * 2.0), ldy = (T)(gf.ih2 * 2.0);
-  {
-    constexpr int K = (Tl::NPIX + 255) / 256;
-    cx<T> vx[K], vb[K], vd[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const int q = threadIdx.x + 256 * k;
-      if (q < Tl::NPIX) {
-        const int li = q % Tl::W, lj = q / Tl::W;
-        const int gi = mirror_idx(fx0 - H + li, n1), gj = mirror_idx(fy0 - H + lj, n2);
-        const int64_t p = (int64_t)gj * n1 + gi;
-        vx[k] = xr[p] + prolong_at(cr, gc.n1, gi, gj);
-        vb[k] = bb[p];
-        vd[k] = d[p];
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const int q = threadIdx.x + 256 * k;
-      if (q < Tl::NPIX) {
-        sx[q] = vx[k];
-        sb[q] = vb[k];
-        sd[q] = vd[k];
-      }
-    }
-  }
+  cx<T> rb[K], rd[K];
+  load_bd<T, W, NPIX, K, H>(b + (int64_t)rhs * gf.N, d, fx0, fy0, n1, n2, rb, rd);
+  load_x_prolong<T, W, NPIX, K, H>(x + (int64_t)rhs * gf.N, ec + (int64_t)rhs * gc.N, gc.n1, fx0, fy0, n1, n2, sx);
   __syncthreads();
-  auto jac = [&](const cx<T>* s, int q) {
+  auto jac = [&](const cx<T>* s, int q, int k) {
     const cx<T> xc = s[q];
-    const cx<T> Ax = lap5(xc, s[q - 1], s[q + 1], s[q - Tl::W], s[q + Tl::W], ih1, ih2) + sd[q] * xc;
-    cx<T> D = sd[q];
+    const cx<T> Ax = lap5(xc, s[q - 1], s[q + 1], s[q - W], s[q + W], ih1, ih2) + rd[k] * xc;
+    cx<T> D = rd[k];
     D.re += ldx + ldy;
-    return xc + wj * ((sb[q] - Ax) * cinv(D));
+    return xc + wj * ((rb[k] - Ax) * cinv(D));
   };
-  for (int q = threadIdx.x; q < Tl::NPIX; q += 256) {
-    const int li = q % Tl::W, lj = q / Tl::W;
-    if (li < 1 || li >= Tl::W - 1 || lj < 1 || lj >= Tl::HT - 1) continue;
-    s3[q] = jac(sx, q);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int q = threadIdx.x + 256 * k;
+    const int li = q % W, lj = q / W;
+    if (q < NPIX && li >= 1 && li < W - 1 && lj >= 1 && lj < Tl::HT - 1) s3[q] = jac(sx, q, k);
   }
   __syncthreads();
   cx<T>* xb = xo + (int64_t)rhs * gf.N;
-  for (int q = threadIdx.x; q < FTX * TY; q += 256) {
-    const int i = fx0 + q % FTX, j = fy0 + q / FTX;
-    if (i < n1 && j < n2) xb[(int64_t)j * n1 + i] = jac(s3, (q / FTX + H) * Tl::W + q % FTX + H);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int q = threadIdx.x + 256 * k;
+    const int li = q % W, lj = q / W;
+    const int i = fx0 + li - H, j = fy0 + lj - H;
+    if (q < NPIX && li >= H && li < H + FTX && lj >= H && lj < H + TY && i < n1 && j < n2)
+      xb[(int64_t)j * n1 + i] = jac(s3, q, k);
   }
 }
 
