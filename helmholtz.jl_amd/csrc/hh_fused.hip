@@ -34,20 +34,6 @@ __device__ __forceinline__ cx<T> lap5(cx<T> xc, cx<T> xl, cx<T> xr, cx<T> xm, cx
   return ih1 * (((T)2 * xc) - xl - xr) + ih2 * (((T)2 * xc) - xm - xp);
 }
 
-// prolongation of the coarse correction at fine node (i, j) (nodal bilinear)
-template <typename T>
-__device__ __forceinline__ cx<T> prolong_at(const cx<T>* __restrict__ c, int nc1, int i, int j) {
-  const int i0 = i >> 1, j0 = j >> 1;
-  const int i1 = (i & 1) ? i0 + 1 : i0, j1 = (j & 1) ? j0 + 1 : j0;
-  const T wi = (i & 1) ? (T)0.5 : (T)1, wj = (j & 1) ? (T)0.5 : (T)1, wk = (T)1;
-  const int ik[2] = {i0, i1}, jk[2] = {j0, j1};
-  const int ni = (i & 1) ? 2 : 1, nj = (j & 1) ? 2 : 1;
-  cx<T> acc = mk<T>(0, 0);
-  for (int c2 = 0; c2 < nj; ++c2)
-    for (int c1 = 0; c1 < ni; ++c1) acc = acc + (wi * wj * wk) * c[(int64_t)jk[c2] * nc1 + ik[c1]];
-  return acc;
-}
-
 template <typename T, int TY, int H>
 struct Tile {
   static constexpr int W = FTX + 2 * H;
@@ -152,24 +138,53 @@ __global__ __launch_bounds__(256) void k_down_fused(Grid gf, Grid gc, const cx<T
   }
 }
 
-// x + P e_c on the owned positions, staged into sx
-template <typename T, int W, int NPIX, int K, int H>
-__device__ __forceinline__ void load_x_prolong(const cx<T>* __restrict__ xr, const cx<T>* __restrict__ cr, int nc1,
-                                               int fx0, int fy0, int n1, int n2, cx<T>* sx) {
+// x + P e_c on the owned positions, staged into sx.  The coarse patch the
+// tile interpolates from is first copied into LDS (sc) with coalesced loads.
+// Fine nodes that influence an owned output have mirrored indices in
+// [max(0, fx0 - H), fx0 + FTX - 1 + H], so their coarse neighbours lie in the
+// patch starting at max(0, fx0 - H) / 2; other halo positions are clamped
+// into the patch (their values are never used).
+template <typename T, int TY, int H>
+struct Patch {
+  static constexpr int PW = (FTX + 2 * H) / 2 + 2;
+  static constexpr int PH = (TY + 2 * H) / 2 + 2;
+};
+
+template <typename T, int TY, int W, int NPIX, int K, int H>
+__device__ __forceinline__ void load_x_prolong(const cx<T>* __restrict__ xr, const cx<T>* __restrict__ cr,
+                                               const Grid& gc, int fx0, int fy0, int n1, int n2, cx<T>* sx,
+                                               cx<T>* sc) {
+  constexpr int PW = Patch<T, TY, H>::PW, PH = Patch<T, TY, H>::PH;
+  const int cx0 = (fx0 - H > 0 ? fx0 - H : 0) >> 1, cy0 = (fy0 - H > 0 ? fy0 - H : 0) >> 1;
   cx<T> vx[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int q = threadIdx.x + 256 * k;
     if (q < NPIX) {
       const int li = q % W, lj = q / W;
-      const int gi = mirror_idx(fx0 - H + li, n1), gj = mirror_idx(fy0 - H + lj, n2);
-      vx[k] = xr[(int64_t)gj * n1 + gi] + prolong_at(cr, nc1, gi, gj);
+      vx[k] = xr[(int64_t)mirror_idx(fy0 - H + lj, n2) * n1 + mirror_idx(fx0 - H + li, n1)];
     }
   }
+  for (int q = threadIdx.x; q < PW * PH; q += 256) {
+    const int I = min(cx0 + q % PW, gc.n1 - 1), J = min(cy0 + q / PW, gc.n2 - 1);
+    sc[q] = cr[(int64_t)J * gc.n1 + I];
+  }
+  __syncthreads();
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int q = threadIdx.x + 256 * k;
-    if (q < NPIX) sx[q] = vx[k];
+    if (q < NPIX) {
+      const int li = q % W, lj = q / W;
+      const int gi = mirror_idx(fx0 - H + li, n1), gj = mirror_idx(fy0 - H + lj, n2);
+      const int i0 = min(max((gi >> 1) - cx0, 0), PW - 2), j0 = min(max((gj >> 1) - cy0, 0), PH - 2);
+      // nodal bilinear interpolation, same terms and order as k_prolong_add
+      const T wi = (gi & 1) ? (T)0.5 : (T)1, wj = (gj & 1) ? (T)0.5 : (T)1, wk = (T)1;
+      const int ni = (gi & 1) ? 2 : 1, nj = (gj & 1) ? 2 : 1;
+      cx<T> acc = mk<T>(0, 0);
+      for (int c2 = 0; c2 < nj; ++c2)
+        for (int c1 = 0; c1 < ni; ++c1) acc = acc + (wi * wj * wk) * sc[(j0 + c2) * PW + i0 + c1];
+      sx[q] = vx[k] + acc;
+    }
   }
 }
 
@@ -182,7 +197,7 @@ __global__ __launch_bounds__(256) void k_mid_fused(Grid gf, Grid gc, const cx<T>
   constexpr int H = 2;
   typedef Tile<T, TY, H> Tl;
   constexpr int W = Tl::W, NPIX = Tl::NPIX, K = (NPIX + 255) / 256;
-  __shared__ cx<T> sx[NPIX], sr[NPIX];
+  __shared__ cx<T> sx[NPIX], sr[NPIX], sc[Patch<T, TY, H>::PW * Patch<T, TY, H>::PH];
   const int rhs = blockIdx.z;
   if (skip && skip[rhs]) return;
   const int n1 = gf.n1, n2 = gf.n2;
@@ -190,7 +205,8 @@ __global__ __launch_bounds__(256) void k_mid_fused(Grid gf, Grid gc, const cx<T>
   const T ih1 = (T)gf.ih1, ih2 = (T)gf.ih2;
   cx<T> rb[K], rd[K];
   load_bd<T, W, NPIX, K, H>(b + (int64_t)rhs * gf.N, d, fx0, fy0, n1, n2, rb, rd);
-  load_x_prolong<T, W, NPIX, K, H>(x + (int64_t)rhs * gf.N, ec + (int64_t)rhs * gc.N, gc.n1, fx0, fy0, n1, n2, sx);
+  load_x_prolong<T, TY, W, NPIX, K, H>(x + (int64_t)rhs * gf.N, ec + (int64_t)rhs * gc.N, gc, fx0, fy0, n1, n2, sx,
+                                       sc);
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < K; ++k) {
@@ -230,7 +246,7 @@ __global__ __launch_bounds__(256) void k_up_fused(Grid gf, Grid gc, const cx<T>*
   constexpr int H = 2;
   typedef Tile<T, TY, H> Tl;
   constexpr int W = Tl::W, NPIX = Tl::NPIX, K = (NPIX + 255) / 256;
-  __shared__ cx<T> sx[NPIX], s3[NPIX];
+  __shared__ cx<T> sx[NPIX], s3[NPIX], sc[Patch<T, TY, H>::PW * Patch<T, TY, H>::PH];
   const int rhs = blockIdx.z;
   if (skip && skip[rhs]) return;
   const int n1 = gf.n1, n2 = gf.n2;
@@ -239,7 +255,8 @@ __global__ __launch_bounds__(256) void k_up_fused(Grid gf, Grid gc, const cx<T>*
   const T ldx = (T)(gf.ih1 * 2.0), ldy = (T)(gf.ih2 * 2.0);
   cx<T> rb[K], rd[K];
   load_bd<T, W, NPIX, K, H>(b + (int64_t)rhs * gf.N, d, fx0, fy0, n1, n2, rb, rd);
-  load_x_prolong<T, W, NPIX, K, H>(x + (int64_t)rhs * gf.N, ec + (int64_t)rhs * gc.N, gc.n1, fx0, fy0, n1, n2, sx);
+  load_x_prolong<T, TY, W, NPIX, K, H>(x + (int64_t)rhs * gf.N, ec + (int64_t)rhs * gc.N, gc, fx0, fy0, n1, n2, sx,
+                                       sc);
   __syncthreads();
   auto jac = [&](const cx<T>* s, int q, int k) {
     const cx<T> xc = s[q];
