@@ -10,14 +10,16 @@ damped Jacobi 0.8, shift 0.5 on every level;
 outer FGMRES(10) in complex128 to ||b - H x|| <= 1e-6 ||b||.
 
 A step = one solve of all 256 sources in one batched call (complex64
-preconditioner, inputs resident in HBM, FGMRES iterations replayed as
-hipGraphs, stencil launch shapes autotuned) with the learned U-Net coarse-grid
-correction at the 257x129 level and the plain cycle as its fallback; the
+preconditioner, inputs resident in HBM, stencil launch shapes autotuned) with
+the learned U-Net coarse-grid correction at the 257x129 level, chosen against
+the plain cycle by the true residual of the first two restart cycles; the
 network is trained first on pairs from plain SL-MG solves on seeded 513x513
 media (training time reported, not timed).  The same batched solve with the
 plain cycle, and the plain method (complex128 preconditioner, one source per
-call, eager launches) on a subset of sources, are reported beside it; the
-line's ``value`` is the learned+batched ms/RHS.
+call) on a subset of sources, are reported beside it; the line's ``value`` is
+the learned+batched ms/RHS.  ``roofline`` times every launch of the dominant
+kernel inside the timed solves with HIP events; ``cpu_baseline`` solves one
+source with the oracle's C/OpenMP restatement of the plain method.
 
 Usage: python bench.py [--gpus 1] [--steps 2] [--warmup 1] [--plain-rhs 4]
 """
@@ -52,24 +54,25 @@ def c2_problem(nsrc: int):
     return M, hp, H, B, w
 
 
-GRAPHS = True
-
-
 def make_solver(hp, prec: str, levels=6, cycle="W", tol=1e-6, fast=False):
     import helmholtz_amd as ha
     mg = ha.getMGparam(complex, int, levels, 1, 200, tol, "Jac", 0.8, 2, 2, cycle, "NoMUMPS", 0.5, 0.0)
     mg.prec = prec
-    mg.graph = fast and GRAPHS
+    # eager launches: at 256 RHS each launch runs for 0.1-7 ms, so graph replay
+    # measured no faster, and the in-solve kernel timing needs eager launches
+    mg.graph = False
     mg.autotune = fast
     mg.wlevels = 3  # W double correction on the three finest levels (kh < ~3.2), V below
     return ha.getShiftedLaplacianMultigridSolver(hp, mg, 0.5, "GMRES", 10, False)
 
 
-def timed_solves(ha, solver, Bd, warmup, steps, log, name):
+def timed_solves(ha, solver, Bd, warmup, steps, log, name, before_timed=None):
     for k in range(warmup):
         X, solver = ha.solveLinearSystem(None, Bd, solver)
         log(f"{name} warmup {k}: max iters {solver.last_stats['max_iters']}, "
             f"fell back {solver.last_stats['fell_back']}")
+    if before_timed is not None:
+        before_timed(solver)
     times = []
     for k in range(steps):
         t0 = time.perf_counter()
@@ -80,26 +83,50 @@ def timed_solves(ha, solver, Bd, warmup, steps, log, name):
     return float(np.mean(times)), solver.last_stats
 
 
-def roofline_jacobi(H, nrhs: int, reps: int = 20):
-    """Event-timed damped-Jacobi sweep (complex64, the batched smoother) at the
-    workload's fine grid with ``nrhs`` columns.  Algorithmic bytes per node:
-    read x, b, write x (3 * 8 B per RHS) + the complex64 diagonal (8 B)."""
-    import ctypes as C
-    from helmholtz_amd import _lib as L
+# The dominant kernel of the batched solve (rocprofv3 kernel trace of this
+# benchmark): the fused up leg of the multigrid cycle on the fine grid.
+ROOFLINE_KERNEL = ("up", 0)
+ROOFLINE_NAME = "k_up_fused<float,16> on the 1025x513 grid (x += P e_c, two damped-Jacobi sweeps)"
+
+
+def roofline_from_solver(ha, solver, nsrc: int, N: int, Nc: int):
+    """HIP-event timing of every launch of the roofline kernel inside the timed
+    solves (on the solver's stream), against its algorithmic bytes: per active
+    RHS read x, b (8 B each), the coarse correction (8 B per coarse node),
+    write x (8 B); plus the shared complex64 diagonal (8 B per node)."""
+    t = ha.kernel_time(solver)
+    ms, n, byt = t["ms"], t["launches"], t["bytes"]
+    achieved = byt / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    full = nsrc * (3 * N * 8 + Nc * 8) + N * 8
+    traffic = _pmc_traffic()
+    return {"bound": "hbm", "kernel": ROOFLINE_NAME, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "launches": n, "avg_launch_ms": round(ms / max(n, 1), 5),
+            "alg_bytes_per_launch": round(byt / max(n, 1)), "alg_bytes_full_batch_launch": full}
+
+
+def _pmc_traffic():
+    """HBM bytes per full-batch launch of the roofline kernel from the two
+    rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) summarised in
+    profiles/pmc_up_fused_c2_1025x513_256rhs.md, each scaled by the counter's
+    reading for a device copy of known size; None when that summary is absent."""
+    f = os.path.join(ROOT, "profiles", "pmc_up_fused_c2_1025x513_256rhs.md")
+    if not os.path.exists(f):
+        return None
+    for line in open(f):
+        if line.startswith("traffic_bytes_per_launch ="):
+            return float(line.split("=", 1)[1])
+    return None
+
+
+def calibration_copy(Bd, log):
+    """One device-to-device copy of the RHS block (bytes known exactly): the
+    scale for the HBM byte counters of the --pmc runs."""
     from helmholtz_amd.device import DeviceArray
-    N = H.shape[0]
-    rng = np.random.default_rng(0)
-    col = (rng.standard_normal(N) + 1j * rng.standard_normal(N)).astype(np.complex64)
-    X = DeviceArray.from_numpy(np.asfortranarray(np.repeat(col[:, None], nrhs, axis=1)))
-    Bd = DeviceArray.from_numpy(np.asfortranarray(np.repeat(col[:, None], nrhs, axis=1)))
-    Y = X.empty_like()
-    ms = C.c_double()
-    L.check(L.lib().hh_op_time_kernel(H._h.ptr, 2, 0.5, X.ptr, Bd.ptr, Y.ptr, nrhs, L.HH_C64, reps, C.byref(ms)))
-    bytes_alg = N * (3 * 8 * nrhs + 8)
-    achieved = bytes_alg / (ms.value * 1e-3) / 1e9
-    return {"bound": "hbm", "kernel": "k_stencil2d<float,float,float,M_JACOBI>", "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "bytes_per_launch": bytes_alg, "avg_launch_ms": round(ms.value, 5), "nrhs": nrhs}
+    tmp = DeviceArray(Bd.shape, Bd.dtype)
+    tmp.copy_from(Bd)
+    log(f"calibration copy: {Bd.nbytes} bytes read, {Bd.nbytes} bytes written (device to device)")
+    del tmp
 
 
 def cpu_baseline(hp, B, iters_per_rhs: int, budget_s: float):
@@ -146,11 +173,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--train-media", type=int, default=8)
     ap.add_argument("--train-epochs", type=int, default=8)
-    ap.add_argument("--no-graphs", action="store_true",
-                    help="launch FGMRES iterations eagerly (for profilers that do not trace hipGraph replays)")
     args = ap.parse_args()
-    global GRAPHS
-    GRAPHS = not args.no_graphs
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the solve shards by right-hand side "
               "with no communication, so N GPUs would each run this benchmark on nsrc/N sources.",
@@ -180,7 +203,11 @@ def main():
     log(f"training: {tinfo}")
     sl = make_solver(hp, "c64", fast=True)
     learned.attach(sl, net, 2)
-    t_l, st_l = timed_solves(ha, sl, Bd, args.warmup, args.steps, log, "learned")
+    t_l, st_l = timed_solves(ha, sl, Bd, args.warmup, args.steps, log, "learned",
+                             before_timed=lambda s: ha.time_kernel(s, *ROOFLINE_KERNEL))
+    lv = ha.hierarchy_levels(sl)
+    roof = roofline_from_solver(ha, sl, args.nsrc, int(np.prod(lv[0])), int(np.prod(lv[1])))
+    log(f"roofline: {roof}")
     res_l = np.asarray(st_l["true_res"])
     learned_ms_rhs = 1e3 * t_l / args.nsrc
     ms_step = 1e3 * t_l
@@ -204,8 +231,7 @@ def main():
         log(f"plain rhs {c}: {dt:.3f} s, {plain_it[-1]} iters, res {plain_res[-1]:.2e}")
     plain_ms_rhs = 1e3 * float(np.mean(plain_t))
 
-    roof = roofline_jacobi(H, args.nsrc)
-    log(f"roofline: {roof}")
+    calibration_copy(Bd, log)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(hp, B, int(np.mean(plain_it)), args.cpu_budget)
@@ -226,7 +252,7 @@ def main():
         "data": "synthetic (seeded layered+body medium, point sources)",
         "config": {"workload": "2-D 1025x513 Helmholtz, 256 point sources, 8 ppw, 1e-6 true residual",
                    "nsrc": args.nsrc, "levels": 6, "cycle": "W(2,2) on levels 0-2, V(2,2) below", "krylov": "FGMRES(10)",
-                   "learned": "U-Net at level 2 (257x129), plain-cycle fallback", "graphs": GRAPHS,
+                   "learned": "U-Net at level 2 (257x129), plain-cycle fallback", "graphs": False,
                    "autotune": True, "parallelism": "rhs-batched, 1 GPU"},
         "learned_ms_per_rhs": round(learned_ms_rhs, 4),
         "learned_max_iters": int(st_l["max_iters"]),

@@ -185,6 +185,7 @@ struct Level {
   double* m = nullptr;
   double* gam = nullptr;
   void* d = nullptr;      // diagonal of the shifted operator, preconditioner precision
+  void* f = nullptr;      // damped-Jacobi factor relax (d + Laplacian diagonal)^-1, same precision
   void* xa = nullptr;     // ping-pong iterate buffers (cap * N)
   void* xb = nullptr;
   void* b = nullptr;      // right-hand side of this level (levels >= 1)
@@ -250,6 +251,9 @@ struct hh_solver {
   int cap_lo = 0, cap_hi = 0;
   bool cap_now = false;
   cx<float>* cap_buf = nullptr;
+  // per-kernel timing (hh_solver_time_kernel); kt_active = RHS still iterating
+  KTimer kt;
+  int kt_active = 0;
 
   void free_work() {
     drop_graphs();
@@ -262,7 +266,7 @@ struct hh_solver {
     V = nullptr; Z = nullptr; xk = bk = wk = nullptr; b0 = nullptr;
     dfree(kw.part); dfree(kw.partn); dfree(kw.h1); dfree(kw.h2); dfree(kw.nrm); dfree(kw.H); dfree(kw.cs);
     dfree(kw.sn); dfree(kw.gv); dfree(kw.y); dfree(kw.bnorm); dfree(kw.est); dfree(kw.done); dfree(kw.kact);
-    dfree(kw.vscale); dfree(kw.fix);
+    dfree(kw.vscale); dfree(kw.fix); dfree(kw.wn2);
     kw = KrylovWork{};
     dfree(skip_copy); dfree(conv_d); dfree(n2tmp); dfree(res_d);
     skip_copy = nullptr; conv_d = nullptr; n2tmp = nullptr; res_d = nullptr;
@@ -274,7 +278,7 @@ struct hh_solver {
   }
   void free_hier() {
     free_work();
-    for (auto& L : lv) { dfree(L.m); dfree(L.gam); dfree(L.d); }
+    for (auto& L : lv) { dfree(L.m); dfree(L.gam); dfree(L.d); dfree(L.f); }
     lv.clear();
     dfree(coarse_aug); dfree(coarse_d); dfree(piv); dfree(fac); dfree(pinv);
     coarse_aug = nullptr; coarse_d = nullptr; piv = nullptr; fac = nullptr; pinv = nullptr;
@@ -322,10 +326,16 @@ void build_hierarchy(hh_solver* S) {
       lv.d = dalloc<cx<float>>(lv.g.N);
       launch_diag<float>(lv.g, lv.m, lv.gam, op->wre, op->wim, op->neumann_top, op->sommerfeld, lv.h,
                          (cx<float>*)lv.d, st);
+      lv.f = dalloc<cx<float>>(lv.g.N);
+      launch_jacobi_factor<float>(lv.g, lv.m, lv.gam, op->wre, op->wim, op->neumann_top, op->sommerfeld, lv.h,
+                                  S->p.relax, (cx<float>*)lv.f, st);
     } else {
       lv.d = dalloc<cx<double>>(lv.g.N);
       launch_diag<double>(lv.g, lv.m, lv.gam, op->wre, op->wim, op->neumann_top, op->sommerfeld, lv.h,
                           (cx<double>*)lv.d, st);
+      lv.f = dalloc<cx<double>>(lv.g.N);
+      launch_jacobi_factor<double>(lv.g, lv.m, lv.gam, op->wre, op->wim, op->neumann_top, op->sommerfeld, lv.h,
+                                   S->p.relax, (cx<double>*)lv.f, st);
     }
     if (l + 1 < L) {
       for (int d = 0; d < op->dim; ++d) {
@@ -417,6 +427,7 @@ void ensure_work(hh_solver* S, int nrhs) {
   w.lazy = pr == HH_C64;
   w.vscale = dalloc<double>((size_t)(K + 1) * nrhs);
   w.fix = dalloc<int>(nrhs);
+  w.wn2 = dalloc<double>(nrhs);
   S->skip_copy = dalloc<int>(nrhs);
   S->conv_d = dalloc<int>(nrhs);
   S->n2tmp = dalloc<double>(nrhs);
@@ -559,20 +570,31 @@ struct Cycle {
       Level& C = S->lv[l + 1];
       const int ncorr = ncorr_at(l);
       L.cur = first ? first : L.xa;
-      launch_down_fused<T>(L.g, C.g, nrhs, b, (const cx<T>*)L.d, S->p.relax, (cx<T>*)L.cur, (cx<T>*)C.b, skip, st);
+      // algorithmic bytes per launch: per active RHS the fields each leg
+      // reads and writes once (s bytes per value), plus the shared diagonal
+      const double s = sizeof(cx<T>), Nf = (double)L.g.N, Nc = (double)C.g.N, act = S->kt_active;
+      const bool td = S->kt.match(KT_DOWN, l), tm = S->kt.match(KT_MID, l), tu = S->kt.match(KT_UP, l);
+      if (td) S->kt.begin(st);
+      launch_down_fused<T>(L.g, C.g, nrhs, b, (const cx<T>*)L.d, (const cx<T>*)L.f, (cx<T>*)L.cur, (cx<T>*)C.b,
+                           skip, st);
+      if (td) S->kt.end(st, act * (2 * Nf * s + Nc * s) + Nf * s);
       C.cur = nullptr;
       run(l + 1, (const cx<T>*)C.b, true, nullptr);
       for (int c = 1; c < ncorr; ++c) {
         void* o = other(L);
+        if (tm) S->kt.begin(st);
         launch_mid_fused<T>(L.g, C.g, nrhs, (const cx<T>*)L.cur, (const cx<T>*)C.cur, b, (const cx<T>*)L.d,
                             (cx<T>*)o, (cx<T>*)C.b, skip, st);
+        if (tm) S->kt.end(st, act * (3 * Nf * s + 2 * Nc * s) + Nf * s);
         L.cur = o;
         C.cur = nullptr;
         run(l + 1, (const cx<T>*)C.b, true, nullptr);
       }
       void* o = other(L);
+      if (tu) S->kt.begin(st);
       launch_up_fused<T>(L.g, C.g, nrhs, (const cx<T>*)L.cur, (const cx<T>*)C.cur, b, (const cx<T>*)L.d,
-                         S->p.relax, (cx<T>*)o, skip, st);
+                         (const cx<T>*)L.f, (cx<T>*)o, skip, st);
+      if (tu) S->kt.end(st, act * (3 * Nf * s + Nc * s) + Nf * s);
       L.cur = o;
       return;
     }
@@ -699,21 +721,29 @@ void run_fgmres(hh_solver* S, int nrhs, hh_solve_stats* stats) {
       const cx<double>* vj = S->V + (int64_t)j * blk;
       void* zj = (char*)S->Z + (size_t)j * blk * zes;
       S->cap_now = S->cap_buf && nprec >= S->cap_lo && nprec < S->cap_hi && active == nrhs;
+      S->kt_active = active;
       auto body = [&] {
+        // H z: reads z (8 or 16 B per value), writes w (16 B), shared c128 diagonal
+        const bool ta = S->kt.match(KT_APPLY, 0);
+        const double Nf = (double)N;
         if (S->p.prec == HH_C64) {
           precondition<float>(S, nrhs, vj, zj, w.done, st, w.lazy ? w.vscale + (int64_t)j * nrhs : nullptr);
+          if (ta) S->kt.begin(st);
           launch_stencil<float, double, double, M_APPLY>(S->op->g, nrhs, (const cx<float>*)zj, nullptr, S->wk,
                                                          S->op->d64, nullptr, mk<double>(0, 0), 0.0, w.done, st);
+          if (ta) S->kt.end(st, active * Nf * 24.0 + Nf * 16.0);
         } else {
           precondition<double>(S, nrhs, vj, zj, w.done, st);
+          if (ta) S->kt.begin(st);
           h_apply(S, nrhs, (const cx<double>*)zj, S->wk, w.done, st);
+          if (ta) S->kt.end(st, active * Nf * 32.0 + Nf * 16.0);
         }
-        kr_arnoldi_impl(w, S->V, S->wk, j, tol, S->skip_copy, st);
+        kr_arnoldi_impl(w, S->V, S->wk, j, tol, S->skip_copy, &S->kt, active, st);
       };
       // hipGraph: the first restart cycle runs eagerly (it allocates the
       // network workspace and may capture training data); later cycles replay
       // one captured graph per Arnoldi step j.
-      if (S->p.graph && outer >= 1 && !S->cap_now) {
+      if (S->p.graph && outer >= 1 && !S->cap_now && S->kt.kernel == KT_NONE) {
         if (S->gkey_nrhs != nrhs || S->gkey_learned != S->use_learned) {
           S->drop_graphs();
           S->gkey_nrhs = nrhs;
@@ -812,6 +842,7 @@ void run_fgmres(hh_solver* S, int nrhs, hh_solve_stats* stats) {
     if (nconv == nrhs) break;
   }
   S->n_prec += nprec;
+  S->kt.flush(st);
   if (stats) {
     int mx = 0, nc = 0;
     for (int b = 0; b < nrhs; ++b) {
@@ -1352,6 +1383,29 @@ int hh_solver_precondition(hh_solver* s, const void* V, void* Z, int64_t nrhs, i
     HIPCK(hipMemcpyAsync(Z, s->Z, zb, loc == HH_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, st));
     HIPCK(hipStreamSynchronize(st));
     HIPCK(hipGetLastError());
+  });
+}
+
+int hh_solver_time_kernel(hh_solver* s, int kernel, int level) {
+  return guarded([&] {
+    if (!s) throw HHError(HH_ERR_ARG, "null argument");
+    if (kernel < HH_KT_NONE || kernel > HH_KT_UPDATE) throw HHError(HH_ERR_ARG, "unknown kernel id");
+    if (kernel >= HH_KT_APPLY && level != 0) throw HHError(HH_ERR_ARG, "H z and the Arnoldi passes run on level 0");
+    if (kernel >= HH_KT_DOWN && kernel <= HH_KT_UP && (level < 0 || level >= s->p.levels - 1))
+      throw HHError(HH_ERR_ARG, "the fused legs run on levels 0..levels-2");
+    s->kt.reset();
+    s->kt.kernel = kernel;
+    s->kt.level = level;
+  });
+}
+
+int hh_solver_kernel_time(hh_solver* s, double* total_ms, int64_t* launches, double* alg_bytes) {
+  return guarded([&] {
+    if (!s || !total_ms || !launches || !alg_bytes) throw HHError(HH_ERR_ARG, "null argument");
+    s->kt.flush(s->op->stream);
+    *total_ms = s->kt.ms;
+    *launches = s->kt.launches;
+    *alg_bytes = s->kt.bytes;
   });
 }
 

@@ -2,11 +2,11 @@
 // 2 pre- / 2 post-sweeps, LDS-tiled with halos so each leg reads its inputs
 // and writes its outputs once:
 //
-//   down : x1 = w D^-1 b, x2 = x1 + w D^-1 (b - A x1), r = b - A x2, b_c = R r
+//   down : x1 = F b, x2 = x1 + F (b - A x1), r = b - A x2, b_c = R r
 //          reads b, d; writes x2 and b_c          (unfused: 4 kernels, ~9 passes)
 //   mid  : x' = x + P e_c, r = b - A x', b_c = R r
 //          reads x, b, d, e_c; writes x', b_c     (unfused: 3 kernels, ~6.5 passes)
-//   up   : x' = x + P e_c, two Jacobi sweeps
+//   up   : x' = x + P e_c, two Jacobi sweeps x <- x + F (b - A x)
 //          reads x, b, d, e_c; writes x''         (unfused: 3 kernels, ~8.25 passes)
 //
 // A block owns a TX x TY tile of fine nodes (TX, TY even, so it also owns the
@@ -15,7 +15,8 @@
 // = x_k): the mirror extension of a field is preserved exactly by the
 // operator, the smoother, and the transfers, so ghost positions evolve like
 // the nodes they mirror and every formula runs unchanged on the tile.
-// Per-node arithmetic is the same expression as the unfused kernels.
+// F = relax (d + 2/h1^2 + 2/h2^2)^-1 is the level's precomputed damped-Jacobi
+// factor (one complex multiply per update instead of a complex reciprocal).
 #include "hh_kernels.h"
 
 namespace hh {
@@ -41,6 +42,31 @@ struct Tile {
   static constexpr int NPIX = W * HT;
 };
 
+// XCD-aware work order.  A launch is a 1-D grid of nxt * nyt * nrhs blocks;
+// the hardware hands block b to XCD b % 8, and each XCD has its own L2.  Block
+// b is given the (b / 8)-th item of a contiguous 1/8 slice of the work list
+// ordered (RHS group, tile row, RHS within the group, tile column), so each
+// XCD walks its own RHS slab row by row: a tile row's diagonal is reused by the
+// G RHS of the group and the halo rows of the previous tile row (same RHS) are
+// still in that XCD's L2.
+struct TileMap {
+  int nxt, nyt, G;
+  int64_t T;
+};
+
+__device__ __forceinline__ void tile_of(const TileMap& m, int& xt, int& yt, int& rhs) {
+  const int64_t b = blockIdx.x;
+  const int64_t per = m.T >> 3, rem = m.T & 7;
+  const int64_t xcd = b & 7;
+  int64_t L = xcd * per + (xcd < rem ? xcd : rem) + (b >> 3);
+  xt = (int)(L % m.nxt);
+  L /= m.nxt;
+  const int r_in = (int)(L % m.G);
+  L /= m.G;
+  yt = (int)(L % m.nyt);
+  rhs = (int)(L / m.nyt) * m.G + r_in;
+}
+
 // Each thread owns the tile positions q = threadIdx.x + 256 k (k < K) in every
 // phase, so b and d live in its registers; only the x fields (and the
 // residual) are staged in LDS, where neighbours are read.  Two LDS arrays per
@@ -49,9 +75,11 @@ struct Tile {
 
 // load b and d at the owned positions of a tile whose first node is
 // (fx0 - H, fy0 - H)
-template <typename T, int W, int NPIX, int K, int H>
-__device__ __forceinline__ void load_bd(const cx<T>* __restrict__ bb, const cx<T>* __restrict__ d, int fx0, int fy0,
-                                        int n1, int n2, cx<T> (&rb)[K], cx<T> (&rd)[K]) {
+// (and the Jacobi factor f when given)
+template <typename T, int W, int NPIX, int K, int H, bool WITH_F>
+__device__ __forceinline__ void load_bd(const cx<T>* __restrict__ bb, const cx<T>* __restrict__ d,
+                                        const cx<T>* __restrict__ f, int fx0, int fy0, int n1, int n2,
+                                        cx<T> (&rb)[K], cx<T> (&rd)[K], cx<T> (&rf)[K]) {
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int q = threadIdx.x + 256 * k;
@@ -60,37 +88,34 @@ __device__ __forceinline__ void load_bd(const cx<T>* __restrict__ bb, const cx<T
       const int64_t p = (int64_t)mirror_idx(fy0 - H + lj, n2) * n1 + mirror_idx(fx0 - H + li, n1);
       rb[k] = bb[p];
       rd[k] = d[p];
+      if (WITH_F) rf[k] = f[p];
     }
   }
 }
 
 // ------------------------------------------------------------------ down leg
 template <typename T, int TY>
-__global__ __launch_bounds__(256) void k_down_fused(Grid gf, Grid gc, const cx<T>* __restrict__ b,
-                                                    const cx<T>* __restrict__ d, T wj, cx<T>* __restrict__ xo,
-                                                    cx<T>* __restrict__ bco, const int* __restrict__ skip) {
+__global__ __launch_bounds__(256) void k_down_fused(TileMap tm, Grid gf, Grid gc, const cx<T>* __restrict__ b,
+                                                    const cx<T>* __restrict__ d, const cx<T>* __restrict__ f,
+                                                    cx<T>* __restrict__ xo, cx<T>* __restrict__ bco,
+                                                    const int* __restrict__ skip) {
   constexpr int H = 3;
   typedef Tile<T, TY, H> Tl;
   constexpr int W = Tl::W, NPIX = Tl::NPIX, K = (NPIX + 255) / 256;
   __shared__ cx<T> s1[NPIX], s2[NPIX];
-  const int rhs = blockIdx.z;
+  int xt, yt, rhs;
+  tile_of(tm, xt, yt, rhs);
   if (skip && skip[rhs]) return;
   const int n1 = gf.n1, n2 = gf.n2;
-  const int fx0 = blockIdx.x * FTX, fy0 = blockIdx.y * TY;
+  const int fx0 = xt * FTX, fy0 = yt * TY;
   const T ih1 = (T)gf.ih1, ih2 = (T)gf.ih2;
-  const T ld0 = (T)(gf.ih1 * 2.0 + gf.ih2 * 2.0);        // first sweep (as k_jacobi0)
-  const T ldx = (T)(gf.ih1 * 2.0), ldy = (T)(gf.ih2 * 2.0);  // later sweeps (as k_stencil2d)
-  cx<T> rb[K], rd[K];
-  load_bd<T, W, NPIX, K, H>(b + (int64_t)rhs * gf.N, d, fx0, fy0, n1, n2, rb, rd);
-  // x1 = w D0^-1 b on the whole tile
+  cx<T> rb[K], rd[K], rf[K];
+  load_bd<T, W, NPIX, K, H, true>(b + (int64_t)rhs * gf.N, d, f, fx0, fy0, n1, n2, rb, rd, rf);
+  // x1 = F b on the whole tile
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int q = threadIdx.x + 256 * k;
-    if (q < NPIX) {
-      cx<T> D = rd[k];
-      D.re += ld0;
-      s1[q] = wj * (rb[k] * cinv(D));
-    }
+    if (q < NPIX) s1[q] = rf[k] * rb[k];
   }
   __syncthreads();
   // x2 on the halo-2 region
@@ -101,9 +126,7 @@ __global__ __launch_bounds__(256) void k_down_fused(Grid gf, Grid gc, const cx<T
     if (q < NPIX && li >= 1 && li < W - 1 && lj >= 1 && lj < Tl::HT - 1) {
       const cx<T> xc = s1[q];
       const cx<T> Ax = lap5(xc, s1[q - 1], s1[q + 1], s1[q - W], s1[q + W], ih1, ih2) + rd[k] * xc;
-      cx<T> D = rd[k];
-      D.re += ldx + ldy;
-      s2[q] = xc + wj * ((rb[k] - Ax) * cinv(D));
+      s2[q] = xc + rf[k] * (rb[k] - Ax);
     }
   }
   __syncthreads();
@@ -190,7 +213,7 @@ __device__ __forceinline__ void load_x_prolong(const cx<T>* __restrict__ xr, con
 
 // ------------------------------------------------------------------ middle (W)
 template <typename T, int TY>
-__global__ __launch_bounds__(256) void k_mid_fused(Grid gf, Grid gc, const cx<T>* __restrict__ x,
+__global__ __launch_bounds__(256) void k_mid_fused(TileMap tm, Grid gf, Grid gc, const cx<T>* __restrict__ x,
                                                    const cx<T>* __restrict__ ec, const cx<T>* __restrict__ b,
                                                    const cx<T>* __restrict__ d, cx<T>* __restrict__ xo,
                                                    cx<T>* __restrict__ bco, const int* __restrict__ skip) {
@@ -198,13 +221,14 @@ __global__ __launch_bounds__(256) void k_mid_fused(Grid gf, Grid gc, const cx<T>
   typedef Tile<T, TY, H> Tl;
   constexpr int W = Tl::W, NPIX = Tl::NPIX, K = (NPIX + 255) / 256;
   __shared__ cx<T> sx[NPIX], sr[NPIX], sc[Patch<T, TY, H>::PW * Patch<T, TY, H>::PH];
-  const int rhs = blockIdx.z;
+  int xt, yt, rhs;
+  tile_of(tm, xt, yt, rhs);
   if (skip && skip[rhs]) return;
   const int n1 = gf.n1, n2 = gf.n2;
-  const int fx0 = blockIdx.x * FTX, fy0 = blockIdx.y * TY;
+  const int fx0 = xt * FTX, fy0 = yt * TY;
   const T ih1 = (T)gf.ih1, ih2 = (T)gf.ih2;
-  cx<T> rb[K], rd[K];
-  load_bd<T, W, NPIX, K, H>(b + (int64_t)rhs * gf.N, d, fx0, fy0, n1, n2, rb, rd);
+  cx<T> rb[K], rd[K], rf[K];  // rf unused (no Jacobi sweep in this leg)
+  load_bd<T, W, NPIX, K, H, false>(b + (int64_t)rhs * gf.N, d, nullptr, fx0, fy0, n1, n2, rb, rd, rf);
   load_x_prolong<T, TY, W, NPIX, K, H>(x + (int64_t)rhs * gf.N, ec + (int64_t)rhs * gc.N, gc, fx0, fy0, n1, n2, sx,
                                        sc);
   __syncthreads();
@@ -239,31 +263,29 @@ __global__ __launch_bounds__(256) void k_mid_fused(Grid gf, Grid gc, const cx<T>
 
 // ------------------------------------------------------------------ up leg
 template <typename T, int TY>
-__global__ __launch_bounds__(256) void k_up_fused(Grid gf, Grid gc, const cx<T>* __restrict__ x,
+__global__ __launch_bounds__(256) void k_up_fused(TileMap tm, Grid gf, Grid gc, const cx<T>* __restrict__ x,
                                                   const cx<T>* __restrict__ ec, const cx<T>* __restrict__ b,
-                                                  const cx<T>* __restrict__ d, T wj, cx<T>* __restrict__ xo,
-                                                  const int* __restrict__ skip) {
+                                                  const cx<T>* __restrict__ d, const cx<T>* __restrict__ f,
+                                                  cx<T>* __restrict__ xo, const int* __restrict__ skip) {
   constexpr int H = 2;
   typedef Tile<T, TY, H> Tl;
   constexpr int W = Tl::W, NPIX = Tl::NPIX, K = (NPIX + 255) / 256;
   __shared__ cx<T> sx[NPIX], s3[NPIX], sc[Patch<T, TY, H>::PW * Patch<T, TY, H>::PH];
-  const int rhs = blockIdx.z;
+  int xt, yt, rhs;
+  tile_of(tm, xt, yt, rhs);
   if (skip && skip[rhs]) return;
   const int n1 = gf.n1, n2 = gf.n2;
-  const int fx0 = blockIdx.x * FTX, fy0 = blockIdx.y * TY;
+  const int fx0 = xt * FTX, fy0 = yt * TY;
   const T ih1 = (T)gf.ih1, ih2 = (T)gf.ih2;
-  const T ldx = (T)(gf.ih1 * 2.0), ldy = (T)(gf.ih2 * 2.0);
-  cx<T> rb[K], rd[K];
-  load_bd<T, W, NPIX, K, H>(b + (int64_t)rhs * gf.N, d, fx0, fy0, n1, n2, rb, rd);
+  cx<T> rb[K], rd[K], rf[K];
+  load_bd<T, W, NPIX, K, H, true>(b + (int64_t)rhs * gf.N, d, f, fx0, fy0, n1, n2, rb, rd, rf);
   load_x_prolong<T, TY, W, NPIX, K, H>(x + (int64_t)rhs * gf.N, ec + (int64_t)rhs * gc.N, gc, fx0, fy0, n1, n2, sx,
                                        sc);
   __syncthreads();
   auto jac = [&](const cx<T>* s, int q, int k) {
     const cx<T> xc = s[q];
     const cx<T> Ax = lap5(xc, s[q - 1], s[q + 1], s[q - W], s[q + W], ih1, ih2) + rd[k] * xc;
-    cx<T> D = rd[k];
-    D.re += ldx + ldy;
-    return xc + wj * ((rb[k] - Ax) * cinv(D));
+    return xc + rf[k] * (rb[k] - Ax);
   };
 #pragma unroll
   for (int k = 0; k < K; ++k) {
@@ -292,36 +314,50 @@ struct FusedTY {
 bool fused_ok(const Grid& gf) { return gf.dim == 2 && gf.bc == 2 && gf.n1 >= 16 && gf.n2 >= 16; }
 
 template <typename T>
-void launch_down_fused(const Grid& gf, const Grid& gc, int nrhs, const cx<T>* b, const cx<T>* d, double wj, cx<T>* xo,
-                       cx<T>* bco, const int* skip, hipStream_t s) {
+TileMap tile_map(const Grid& gf, int nrhs) {
   constexpr int TY = FusedTY<T>::v;
-  dim3 grid((gf.n1 + FTX - 1) / FTX, (gf.n2 + TY - 1) / TY, nrhs);
-  hipLaunchKernelGGL((k_down_fused<T, TY>), grid, dim3(256), 0, s, gf, gc, b, d, (T)wj, xo, bco, skip);
+  TileMap m;
+  m.nxt = (gf.n1 + FTX - 1) / FTX;
+  m.nyt = (gf.n2 + TY - 1) / TY;
+  m.G = nrhs % 4 == 0 ? 4 : (nrhs % 2 == 0 ? 2 : 1);
+  m.T = (int64_t)m.nxt * m.nyt * nrhs;
+  return m;
+}
+
+template <typename T>
+void launch_down_fused(const Grid& gf, const Grid& gc, int nrhs, const cx<T>* b, const cx<T>* d, const cx<T>* f,
+                       cx<T>* xo, cx<T>* bco, const int* skip, hipStream_t s) {
+  constexpr int TY = FusedTY<T>::v;
+  const TileMap m = tile_map<T>(gf, nrhs);
+  hipLaunchKernelGGL((k_down_fused<T, TY>), dim3((unsigned)m.T), dim3(256), 0, s, m, gf, gc, b, d, f, xo, bco,
+                     skip);
 }
 
 template <typename T>
 void launch_mid_fused(const Grid& gf, const Grid& gc, int nrhs, const cx<T>* x, const cx<T>* ec, const cx<T>* b,
                       const cx<T>* d, cx<T>* xo, cx<T>* bco, const int* skip, hipStream_t s) {
   constexpr int TY = FusedTY<T>::v;
-  dim3 grid((gf.n1 + FTX - 1) / FTX, (gf.n2 + TY - 1) / TY, nrhs);
-  hipLaunchKernelGGL((k_mid_fused<T, TY>), grid, dim3(256), 0, s, gf, gc, x, ec, b, d, xo, bco, skip);
+  const TileMap m = tile_map<T>(gf, nrhs);
+  hipLaunchKernelGGL((k_mid_fused<T, TY>), dim3((unsigned)m.T), dim3(256), 0, s, m, gf, gc, x, ec, b, d, xo, bco,
+                     skip);
 }
 
 template <typename T>
 void launch_up_fused(const Grid& gf, const Grid& gc, int nrhs, const cx<T>* x, const cx<T>* ec, const cx<T>* b,
-                     const cx<T>* d, double wj, cx<T>* xo, const int* skip, hipStream_t s) {
+                     const cx<T>* d, const cx<T>* f, cx<T>* xo, const int* skip, hipStream_t s) {
   constexpr int TY = FusedTY<T>::v;
-  dim3 grid((gf.n1 + FTX - 1) / FTX, (gf.n2 + TY - 1) / TY, nrhs);
-  hipLaunchKernelGGL((k_up_fused<T, TY>), grid, dim3(256), 0, s, gf, gc, x, ec, b, d, (T)wj, xo, skip);
+  const TileMap m = tile_map<T>(gf, nrhs);
+  hipLaunchKernelGGL((k_up_fused<T, TY>), dim3((unsigned)m.T), dim3(256), 0, s, m, gf, gc, x, ec, b, d, f, xo,
+                     skip);
 }
 
 #define HH_INST(T)                                                                                              \
-  template void launch_down_fused<T>(const Grid&, const Grid&, int, const cx<T>*, const cx<T>*, double, cx<T>*, \
-                                     cx<T>*, const int*, hipStream_t);                                          \
+  template void launch_down_fused<T>(const Grid&, const Grid&, int, const cx<T>*, const cx<T>*, const cx<T>*,   \
+                                     cx<T>*, cx<T>*, const int*, hipStream_t);                                  \
   template void launch_mid_fused<T>(const Grid&, const Grid&, int, const cx<T>*, const cx<T>*, const cx<T>*,    \
                                     const cx<T>*, cx<T>*, cx<T>*, const int*, hipStream_t);                     \
   template void launch_up_fused<T>(const Grid&, const Grid&, int, const cx<T>*, const cx<T>*, const cx<T>*,     \
-                                   const cx<T>*, double, cx<T>*, const int*, hipStream_t);
+                                   const cx<T>*, const cx<T>*, cx<T>*, const int*, hipStream_t);
 HH_INST(float)
 HH_INST(double)
 #undef HH_INST

@@ -2,6 +2,8 @@
 #pragma once
 #include "hh_common.h"
 
+#include <vector>
+
 namespace hh {
 
 enum { M_APPLY = 0, M_RESID = 1, M_JACOBI = 2 };
@@ -29,6 +31,10 @@ void launch_cast(const cx<A>* a, cx<B>* b, int64_t n, hipStream_t s);
 template <typename T>
 void launch_diag(const Grid& g, const double* m, const double* gam, double wre, double wim, int neumann_top,
                  int sommerfeld, const double h[3], cx<T>* d, hipStream_t s);
+// f = relax * (d + Laplacian diagonal)^-1 (the damped-Jacobi factor), computed in double
+template <typename T>
+void launch_jacobi_factor(const Grid& g, const double* m, const double* gam, double wre, double wim, int neumann_top,
+                          int sommerfeld, const double h[3], double relax, cx<T>* f, hipStream_t s);
 void launch_add_const(double* a, double c, int64_t n, hipStream_t s);
 // leaves A^-1 row-major in M[0, n*n) (and column-major in M[n*n, 2*n*n))
 void launch_dense_inverse(const Grid& g, const cx<double>* d, cx<double>* M, int64_t* piv, cx<double>* fac,
@@ -40,14 +46,64 @@ void launch_dense_apply(const cx<double>* Ainv, int64_t n, int nrhs, const cx<T>
 // ---- fused 2-D multigrid legs (hh_fused.hip): BC = 2, 2 pre/2 post sweeps ----
 bool fused_ok(const Grid& gf);
 template <typename T>
-void launch_down_fused(const Grid& gf, const Grid& gc, int nrhs, const cx<T>* b, const cx<T>* d, double wj, cx<T>* xo,
-                       cx<T>* bco, const int* skip, hipStream_t s);
+void launch_down_fused(const Grid& gf, const Grid& gc, int nrhs, const cx<T>* b, const cx<T>* d, const cx<T>* f,
+                       cx<T>* xo, cx<T>* bco, const int* skip, hipStream_t s);
 template <typename T>
 void launch_mid_fused(const Grid& gf, const Grid& gc, int nrhs, const cx<T>* x, const cx<T>* ec, const cx<T>* b,
                       const cx<T>* d, cx<T>* xo, cx<T>* bco, const int* skip, hipStream_t s);
 template <typename T>
 void launch_up_fused(const Grid& gf, const Grid& gc, int nrhs, const cx<T>* x, const cx<T>* ec, const cx<T>* b,
-                     const cx<T>* d, double wj, cx<T>* xo, const int* skip, hipStream_t s);
+                     const cx<T>* d, const cx<T>* f, cx<T>* xo, const int* skip, hipStream_t s);
+
+// ---- per-kernel timing inside a solve ----
+// Brackets every launch of one chosen kernel (at one grid level) with HIP
+// events on the launching stream and accumulates the elapsed time, the launch
+// count and the launches' algorithmic bytes.  Off while a graph is captured.
+enum KtKernel { KT_NONE = 0, KT_DOWN = 1, KT_MID = 2, KT_UP = 3, KT_APPLY = 4, KT_DOTS = 5, KT_UPDATE = 6 };
+struct KTimer {
+  int kernel = KT_NONE, level = 0;
+  bool enabled = true;
+  std::vector<hipEvent_t> ev;
+  std::vector<double> pend;  // algorithmic bytes of the recorded launches
+  size_t used = 0;
+  double ms = 0.0, bytes = 0.0;
+  int64_t launches = 0;
+  bool match(int k, int l) const { return enabled && kernel != KT_NONE && k == kernel && l == level; }
+  void flush(hipStream_t s) {
+    if (!used) return;
+    (void)hipStreamSynchronize(s);
+    for (size_t i = 0; i + 1 < used; i += 2) {
+      float t = 0.f;
+      if (hipEventElapsedTime(&t, ev[i], ev[i + 1]) == hipSuccess) ms += t;
+      bytes += pend[i / 2];
+      ++launches;
+    }
+    used = 0;
+    pend.clear();
+  }
+  void begin(hipStream_t s) {
+    if (ev.empty()) {
+      ev.resize(512);
+      for (auto& e : ev) (void)hipEventCreate(&e);
+    }
+    if (used + 2 > ev.size()) flush(s);
+    (void)hipEventRecord(ev[used], s);
+  }
+  void end(hipStream_t s, double b) {
+    (void)hipEventRecord(ev[used + 1], s);
+    pend.push_back(b);
+    used += 2;
+  }
+  void reset() {
+    used = 0;
+    pend.clear();
+    ms = bytes = 0.0;
+    launches = 0;
+  }
+  ~KTimer() {
+    for (auto& e : ev) (void)hipEventDestroy(e);
+  }
+};
 
 // ---- batched FGMRES (hh_krylov.hip) ----
 struct KrylovWork {
@@ -73,6 +129,7 @@ struct KrylovWork {
   int lazy;
   double* vscale;        // [kmax+1][nrhs]
   int* fix;              // [nrhs] 1 = the second CGS correction is applied this step
+  double* wn2;           // [nrhs] ||w||^2 before projection (lazy mode)
 };
 // b0 = (complex64) vscale_j[b] * V_j  (preconditioner input in the lazy mode)
 void kr_cast_scaled(const KrylovWork& w, const cx<double>* Vj, const double* scale, cx<float>* b0, hipStream_t s);
@@ -87,7 +144,7 @@ void kr_start_impl(const KrylovWork& w, const cx<double>* r, cx<double>* V0, dou
 // fill column j of H, update Givens rotations and residual estimates.
 // skip_copy: nrhs ints of scratch.
 void kr_arnoldi_impl(const KrylovWork& w, const cx<double>* V, cx<double>* wv, int j, double tol, int* skip_copy,
-                     hipStream_t s);
+                     KTimer* kt, int active, hipStream_t s);
 // y = R^-1 g per RHS, x += Z y (Z in TZ)
 template <typename TZ>
 void kr_update_x(const KrylovWork& w, const cx<TZ>* Z, cx<double>* x, hipStream_t s);

@@ -8,6 +8,12 @@
 //   pass 1  h1 = V^H w
 //   pass 2  w' = w - V h1 (stored),  h2 = V^H w',  ||w'||^2
 //   pass 3  v_{j+1} = (w' - V h2) / sqrt(||w'||^2 - ||h2||^2)
+// Lazy mode (complex64 preconditioner): pass 1 also sums ||w||^2 and pass 2
+// only forms w' and ||w'||^2 (no basis values held for a second product);
+// the second projection h2 = V^H w' and its correction run only for RHS whose
+// projection cancelled severely, ||w'|| < 1e-4 ||w|| (a Daniel-Gragg-Kaufman-
+// Stewart test with a small threshold); otherwise w' is kept unnormalised and
+// its scale 1/||w'|| is applied where the basis vector is read.
 // Partial sums are per (chunk, RHS) and reduced in a fixed order, so results
 // are bit-reproducible run to run.  The small Hessenberg/Givens work runs on
 // the device (one wavefront per RHS) so a whole iteration needs no host sync.
@@ -39,11 +45,14 @@ static __device__ __forceinline__ void chunk_range(int64_t N, int nchunk, int c,
 }
 
 // pass 1: part[c][i][b] = sum_{p in chunk c} conj(V_i[p,b]) w[p,b], i <= j
-template <int NV>
+// NORM: also partn[c][b] = sum ||w||^2 over the chunk.  only (may be null):
+// skip RHS with only[b] == 0.
+template <int NV, bool NORM>
 __global__ __launch_bounds__(256) void k_dots(KrylovWork kw, const cx<double>* __restrict__ V,
-                                              const cx<double>* __restrict__ w, int j) {
+                                              const cx<double>* __restrict__ w, int j,
+                                              const int* __restrict__ only) {
   const int c = blockIdx.x, b = blockIdx.y;
-  if (kw.done[b]) return;
+  if (kw.done[b] || (only && !only[b])) return;
   __shared__ double sh[4];
   int64_t p0, p1;
   chunk_range(kw.N, kw.nchunk, c, &p0, &p1);
@@ -52,8 +61,10 @@ __global__ __launch_bounds__(256) void k_dots(KrylovWork kw, const cx<double>* _
   cx<double> acc[NV];
 #pragma unroll
   for (int i = 0; i < NV; ++i) acc[i] = mk<double>(0.0, 0.0);
+  double nn = 0.0;
   for (int64_t p = p0 + threadIdx.x; p < p1; p += 256) {
     const cx<double> wv = w[base + p];
+    if (NORM) nn += cabs2(wv);
 #pragma unroll
     for (int i = 0; i < NV; ++i)
       if (i <= j) acc[i] = acc[i] + cmulc(V[i * stride + base + p], wv);
@@ -66,12 +77,24 @@ __global__ __launch_bounds__(256) void k_dots(KrylovWork kw, const cx<double>* _
     const double im = block_sum(si * acc[i].im, sh);
     if (threadIdx.x == 0) kw.part[((int64_t)c * (kw.kmax + 1) + i) * kw.nrhs + b] = mk<double>(re, im);
   }
+  if (NORM) {
+    const double t = block_sum(nn, sh);
+    if (threadIdx.x == 0) kw.partn[(int64_t)c * kw.nrhs + b] = t;
+  }
 }
 
-// reduce part over chunks into h[i][b] (one wavefront per (i, b))
+// reduce part over chunks into h[i][b] (one wavefront per (i, b)); block
+// i = j + 1 (launched in lazy mode) reduces partn into wn2[b]
 __global__ __launch_bounds__(64) void k_reduce_dots(KrylovWork kw, cx<double>* __restrict__ h, int j) {
   const int i = blockIdx.x, b = blockIdx.y;
-  if (kw.done[b] || i > j) return;
+  if (kw.done[b] || i > j + 1) return;
+  if (i == j + 1) {
+    double nn = 0.0;
+    for (int c = threadIdx.x; c < kw.nchunk; c += 64) nn += kw.partn[(int64_t)c * kw.nrhs + b];
+    nn = wave_sum(nn);
+    if (threadIdx.x == 0) kw.wn2[b] = nn;
+    return;
+  }
   double re = 0.0, im = 0.0;
   for (int c = threadIdx.x; c < kw.nchunk; c += 64) {
     const cx<double> v = kw.part[((int64_t)c * (kw.kmax + 1) + i) * kw.nrhs + b];
@@ -130,6 +153,48 @@ __global__ __launch_bounds__(256) void k_update_dots(KrylovWork kw, const cx<dou
   if (threadIdx.x == 0) kw.partn[(int64_t)c * kw.nrhs + b] = t;
 }
 
+// lazy pass 2: w' = w - V h1 (basis vectors carry their scale), partn = ||w'||^2
+template <int NV>
+__global__ __launch_bounds__(256) void k_update_lazy(KrylovWork kw, const cx<double>* __restrict__ V,
+                                                     const cx<double>* __restrict__ w, cx<double>* __restrict__ wout,
+                                                     int j) {
+  const int c = blockIdx.x, b = blockIdx.y;
+  if (kw.done[b]) return;
+  __shared__ double sh[4];
+  __shared__ cx<double> hs[NV];
+  if (threadIdx.x < NV) {
+    const int i = threadIdx.x;
+    hs[i] = i <= j ? kw.vscale[i * kw.nrhs + b] * kw.h1[i * kw.nrhs + b] : mk<double>(0.0, 0.0);
+  }
+  __syncthreads();
+  int64_t p0, p1;
+  chunk_range(kw.N, kw.nchunk, c, &p0, &p1);
+  const int64_t stride = kw.N * kw.nrhs;
+  const int64_t base = (int64_t)b * kw.N;
+  double nn = 0.0;
+  for (int64_t p = p0 + threadIdx.x; p < p1; p += 256) {
+    cx<double> wv = w[base + p];
+#pragma unroll
+    for (int i = 0; i < NV; ++i)
+      if (i <= j) wv = wv - hs[i] * V[i * stride + base + p];
+    wout[base + p] = wv;
+    nn += cabs2(wv);
+  }
+  const double t = block_sum(nn, sh);
+  if (threadIdx.x == 0) kw.partn[(int64_t)c * kw.nrhs + b] = t;
+}
+
+// lazy mode: fix[b] = 1 where the projection cancelled severely,
+// ||w'||^2 < 1e-8 ||w||^2 (one wavefront per RHS)
+__global__ __launch_bounds__(64) void k_fix_flags(KrylovWork kw) {
+  const int b = blockIdx.x;
+  if (kw.done[b]) return;
+  double nn = 0.0;
+  for (int c = threadIdx.x; c < kw.nchunk; c += 64) nn += kw.partn[(int64_t)c * kw.nrhs + b];
+  nn = wave_sum(nn);
+  if (threadIdx.x == 0) kw.fix[b] = !(nn >= 1e-8 * kw.wn2[b]);
+}
+
 // Reduce pass-2 sums, form column j of H, apply/update Givens rotations and
 // the residual estimate; one wavefront per RHS.
 __global__ __launch_bounds__(64) void k_hess_givens(KrylovWork kw, int j, double tol) {
@@ -137,9 +202,14 @@ __global__ __launch_bounds__(64) void k_hess_givens(KrylovWork kw, int j, double
   if (kw.done[b]) return;
   const int lane = threadIdx.x;
   __shared__ cx<double> col[72];
-  // h2 and ||w'||^2
+  // h2 and ||w'||^2 (lazy mode: h2 exists only where fix[b] was raised)
+  const bool have_h2 = !kw.lazy || kw.fix[b];
   double s2 = 0.0;
   for (int i = 0; i <= j; ++i) {
+    if (!have_h2) {
+      if (lane == 0) col[i] = kw.h1[i * kw.nrhs + b];
+      continue;
+    }
     double re = 0.0, im = 0.0;
     for (int c = lane; c < kw.nchunk; c += 64) {
       const cx<double> v = kw.part[((int64_t)c * (kw.kmax + 1) + i) * kw.nrhs + b];
@@ -159,9 +229,8 @@ __global__ __launch_bounds__(64) void k_hess_givens(KrylovWork kw, int j, double
   for (int c = lane; c < kw.nchunk; c += 64) nn += kw.partn[(int64_t)c * kw.nrhs + b];
   nn = wave_sum(nn);
   if (lane != 0) return;
-  // second CGS correction only where it is not negligible (lazy mode); the
-  // Hessenberg column then describes exactly the vector that is kept
-  const bool use_h2 = !kw.lazy || s2 > 1e-20 * nn;
+  // the Hessenberg column describes exactly the vector that is kept
+  const bool use_h2 = have_h2;
   if (use_h2)
     for (int i = 0; i <= j; ++i) col[i] = col[i] + kw.h2[i * kw.nrhs + b];
   const double hn2 = use_h2 ? nn - s2 : nn;
@@ -360,15 +429,35 @@ void kr_start_impl(const KrylovWork& w, const cx<double>* r, cx<double>* V0, dou
   hipLaunchKernelGGL(k_scale, cgrid(w), dim3(256), 0, s, w, r, V0);
 }
 
+// kt (optional) times pass 1 / pass 2; algorithmic bytes per active RHS:
+// 16 N (j + 2) for pass 1 (reads v_0..v_j and w), 16 N (j + 3) for pass 2
+// (also writes w')
 template <int NV>
 static void arnoldi_nv(const KrylovWork& w, const cx<double>* V, cx<double>* wv, int j, double tol,
-                       int* skip_copy, hipStream_t s) {
+                       int* skip_copy, KTimer* kt, int active, hipStream_t s) {
   const int64_t stride = w.N * w.nrhs;
-  hipLaunchKernelGGL((k_dots<NV>), cgrid(w), dim3(256), 0, s, w, V, wv, j);
-  hipLaunchKernelGGL(k_reduce_dots, dim3(j + 1, w.nrhs), dim3(64), 0, s, w, w.h1, j);
+  const bool t1 = kt && kt->match(KT_DOTS, 0), t2 = kt && kt->match(KT_UPDATE, 0);
   cx<double>* vnext = (cx<double>*)(V + (j + 1) * stride);
-  // lazy: w' goes straight into the next basis slot (kept unnormalised)
-  hipLaunchKernelGGL((k_update_dots<NV>), cgrid(w), dim3(256), 0, s, w, V, wv, w.lazy ? vnext : wv, j);
+  if (t1) kt->begin(s);
+  if (w.lazy)
+    hipLaunchKernelGGL((k_dots<NV, true>), cgrid(w), dim3(256), 0, s, w, V, wv, j, nullptr);
+  else
+    hipLaunchKernelGGL((k_dots<NV, false>), cgrid(w), dim3(256), 0, s, w, V, wv, j, nullptr);
+  if (t1) kt->end(s, 16.0 * (double)w.N * (j + 2) * active);
+  hipLaunchKernelGGL(k_reduce_dots, dim3(j + (w.lazy ? 2 : 1), w.nrhs), dim3(64), 0, s, w, w.h1, j);
+  if (t2) kt->begin(s);
+  if (w.lazy) {
+    // w' goes straight into the next basis slot (kept unnormalised)
+    hipLaunchKernelGGL((k_update_lazy<NV>), cgrid(w), dim3(256), 0, s, w, V, wv, vnext, j);
+  } else {
+    hipLaunchKernelGGL((k_update_dots<NV>), cgrid(w), dim3(256), 0, s, w, V, wv, wv, j);
+  }
+  if (t2) kt->end(s, 16.0 * (double)w.N * (j + 3) * active);
+  if (w.lazy) {
+    // second projection only for RHS whose projection cancelled
+    hipLaunchKernelGGL(k_fix_flags, dim3(w.nrhs), dim3(64), 0, s, w);
+    hipLaunchKernelGGL((k_dots<NV, false>), cgrid(w), dim3(256), 0, s, w, V, (const cx<double>*)vnext, j, w.fix);
+  }
   hipMemcpyAsync(skip_copy, w.done, sizeof(int) * w.nrhs, hipMemcpyDeviceToDevice, s);
   hipLaunchKernelGGL(k_hess_givens, dim3(w.nrhs), dim3(64), 0, s, w, j, tol);
   hipLaunchKernelGGL((k_final<NV>), cgrid(w), dim3(256), 0, s, w, V, w.lazy ? (const cx<double>*)vnext : wv, vnext,
@@ -376,17 +465,17 @@ static void arnoldi_nv(const KrylovWork& w, const cx<double>* V, cx<double>* wv,
 }
 
 void kr_arnoldi_impl(const KrylovWork& w, const cx<double>* V, cx<double>* wv, int j, double tol, int* skip_copy,
-                     hipStream_t s) {
+                     KTimer* kt, int active, hipStream_t s) {
   if (j < 4)
-    arnoldi_nv<4>(w, V, wv, j, tol, skip_copy, s);
+    arnoldi_nv<4>(w, V, wv, j, tol, skip_copy, kt, active, s);
   else if (j < 8)
-    arnoldi_nv<8>(w, V, wv, j, tol, skip_copy, s);
+    arnoldi_nv<8>(w, V, wv, j, tol, skip_copy, kt, active, s);
   else if (j < 12)
-    arnoldi_nv<12>(w, V, wv, j, tol, skip_copy, s);
+    arnoldi_nv<12>(w, V, wv, j, tol, skip_copy, kt, active, s);
   else if (j < 16)
-    arnoldi_nv<16>(w, V, wv, j, tol, skip_copy, s);
+    arnoldi_nv<16>(w, V, wv, j, tol, skip_copy, kt, active, s);
   else
-    arnoldi_nv<32>(w, V, wv, j, tol, skip_copy, s);
+    arnoldi_nv<32>(w, V, wv, j, tol, skip_copy, kt, active, s);
 }
 
 template <typename TZ>

@@ -267,18 +267,16 @@ __global__ __launch_bounds__(256) void k_cast(const cx<A>* __restrict__ a, cx<B>
 
 // Nodal diagonal d = -w^2 m (1 - i g/Re w) + i Re(w) sqrt(m) sum_faces 2/h_d
 // (src/GetHelmholtz.jl:41-47, 222-247; Sommerfeld rows always use BC = 2).
-template <typename T>
-__global__ __launch_bounds__(256) void k_diag(Grid g, const double* __restrict__ m, const double* __restrict__ gam,
-                                              double wre, double wim, int neumann_top, int sommerfeld,
-                                              double h1, double h2, double h3, cx<T>* __restrict__ d) {
-  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (p >= g.N) return;
+// complex diagonal -(w^2) m (1 - i gamma / Re w) + Sommerfeld at node p, in double
+__device__ __forceinline__ void diag_at(const Grid& g, const double* __restrict__ m, const double* __restrict__ gam,
+                                        double wre, double wim, int neumann_top, int sommerfeld, double h1, double h2,
+                                        double h3, int64_t p, double& dr, double& di) {
   const double mv = m[p], gv = gam[p];
   const double w2r = wre * wre - wim * wim, w2i = 2.0 * wre * wim;
   // -(w^2) * m * (1 - i g / wre)
   const double ar = 1.0, ai = -gv / wre;
-  double dr = -(w2r * ar - w2i * ai) * mv;
-  double di = -(w2r * ai + w2i * ar) * mv;
+  dr = -(w2r * ar - w2i * ai) * mv;
+  di = -(w2r * ai + w2i * ar) * mv;
   if (sommerfeld) {
     const int i = (int)(p % g.n1);
     const int64_t q = p / g.n1;
@@ -298,7 +296,38 @@ __global__ __launch_bounds__(256) void k_diag(Grid g, const double* __restrict__
     }
     di += wre * s * sqrt(mv);
   }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_diag(Grid g, const double* __restrict__ m, const double* __restrict__ gam,
+                                              double wre, double wim, int neumann_top, int sommerfeld,
+                                              double h1, double h2, double h3, cx<T>* __restrict__ d) {
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= g.N) return;
+  double dr, di;
+  diag_at(g, m, gam, wre, wim, neumann_top, sommerfeld, h1, h2, h3, p, dr, di);
   d[p] = mk<T>((T)dr, (T)di);
+}
+
+// damped-Jacobi factor relax * (d + Laplacian diagonal)^-1 per node, in double,
+// rounded once to T
+template <typename T>
+__global__ __launch_bounds__(256) void k_jacobi_factor(Grid g, const double* __restrict__ m,
+                                                       const double* __restrict__ gam, double wre, double wim,
+                                                       int neumann_top, int sommerfeld, double h1, double h2,
+                                                       double h3, double relax, cx<T>* __restrict__ f) {
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= g.N) return;
+  double dr, di;
+  diag_at(g, m, gam, wre, wim, neumann_top, sommerfeld, h1, h2, h3, p, dr, di);
+  const int i = (int)(p % g.n1);
+  const int64_t q = p / g.n1;
+  const int j = (int)(q % g.n2);
+  const int k = (int)(q / g.n2);
+  dr += lap_diag1(i, g.n1, g.bc) * g.ih1 + lap_diag1(j, g.n2, g.bc) * g.ih2;
+  if (g.dim == 3) dr += lap_diag1(k, g.n3, g.bc) * g.ih3;
+  const double s = relax / (dr * dr + di * di);
+  f[p] = mk<T>((T)(dr * s), (T)(-di * s));
 }
 
 __global__ __launch_bounds__(256) void k_add_const(double* __restrict__ a, double c, int64_t n) {
@@ -587,6 +616,17 @@ template void launch_diag<double>(const Grid&, const double*, const double*, dou
                                   const double*, cx<double>*, hipStream_t);
 template void launch_diag<float>(const Grid&, const double*, const double*, double, double, int, int,
                                  const double*, cx<float>*, hipStream_t);
+
+template <typename T>
+void launch_jacobi_factor(const Grid& g, const double* m, const double* gam, double wre, double wim, int neumann_top,
+                          int sommerfeld, const double* h, double relax, cx<T>* f, hipStream_t s) {
+  hipLaunchKernelGGL((k_jacobi_factor<T>), dim3((unsigned)((g.N + 255) / 256)), dim3(256), 0, s, g, m, gam, wre, wim,
+                     neumann_top, sommerfeld, h[0], h[1], h[2], relax, f);
+}
+template void launch_jacobi_factor<double>(const Grid&, const double*, const double*, double, double, int, int,
+                                           const double*, double, cx<double>*, hipStream_t);
+template void launch_jacobi_factor<float>(const Grid&, const double*, const double*, double, double, int, int,
+                                          const double*, double, cx<float>*, hipStream_t);
 
 void launch_add_const(double* a, double c, int64_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_add_const, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, c, n);

@@ -13,5 +13,6 @@ from .helmholtz import (  # noqa: F401
 )
 from .solver import (  # noqa: F401
     MGparam, ShiftedLaplacianMultigridSolver, clear, copySolver, getMGparam,
-    getShiftedLaplacianMultigridSolver, hierarchy_levels, precondition, solveLinearSystem,
+    getShiftedLaplacianMultigridSolver, hierarchy_levels, kernel_time, precondition, solveLinearSystem,
+    time_kernel,
 )

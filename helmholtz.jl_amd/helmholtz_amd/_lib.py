@@ -25,7 +25,9 @@ EXPORTS = (
     "hh_unet_forward", "hh_unet_loss_grad", "hh_unet_adam_step", "hh_dataset_create", "hh_dataset_destroy",
     "hh_dataset_size", "hh_solver_capture_begin", "hh_solver_capture_count", "hh_dataset_add_from_solver",
     "hh_unet_train", "hh_unet_eval", "hh_solver_attach_unet", "hh_stream_solve", "hh_solver_precondition",
+    "hh_solver_time_kernel", "hh_solver_kernel_time",
 )
+HH_KT_NONE, HH_KT_DOWN, HH_KT_MID, HH_KT_UP, HH_KT_APPLY, HH_KT_DOTS, HH_KT_UPDATE = range(7)
 
 
 class HelmholtzError(RuntimeError):
@@ -103,6 +105,8 @@ def lib():
     L.hh_unet_eval.argtypes = [vp, vp, i64, i64, pf]
     L.hh_solver_attach_unet.argtypes = [vp, vp, i32]
     L.hh_solver_precondition.argtypes = [vp, vp, vp, i64, i32]
+    L.hh_solver_time_kernel.argtypes = [vp, i32, i32]
+    L.hh_solver_kernel_time.argtypes = [vp, pd, p64, pd]
     for name in EXPORTS:
         if name == "hh_last_error":
             continue

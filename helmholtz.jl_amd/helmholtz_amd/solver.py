@@ -234,3 +234,27 @@ def hierarchy_levels(s: ShiftedLaplacianMultigridSolver):
     arr = (C.c_int64 * (16 * 3))()
     L.check(L.lib().hh_solver_levels(s._handle, C.byref(n), arr, 16))
     return [tuple(arr[3 * l: 3 * l + 3]) for l in range(n.value)]
+
+
+KERNELS = {"none": L.HH_KT_NONE, "down": L.HH_KT_DOWN, "mid": L.HH_KT_MID, "up": L.HH_KT_UP,
+           "apply": L.HH_KT_APPLY, "dots": L.HH_KT_DOTS, "update": L.HH_KT_UPDATE}
+
+
+def time_kernel(s: ShiftedLaplacianMultigridSolver, kernel: str, level: int = 0):
+    """Bracket every launch of one kernel of later solves with HIP events on
+    the solver's stream (instrumentation).  kernel: 'down' | 'mid' | 'up'
+    (fused multigrid legs on grid level ``level``), 'apply' (H z), 'dots' /
+    'update' (Arnoldi passes 1 and 2), or 'none'.  Resets the totals."""
+    _validate(s)
+    _ensure_handle(s)
+    L.check(L.lib().hh_solver_time_kernel(s._handle, KERNELS[kernel], int(level)))
+
+
+def kernel_time(s: ShiftedLaplacianMultigridSolver) -> dict:
+    """Totals since ``time_kernel``: event-timed milliseconds, launches and
+    algorithmic bytes of the timed launches."""
+    if s._handle is None:
+        return {"ms": 0.0, "launches": 0, "bytes": 0.0}
+    ms, n, b = C.c_double(), C.c_int64(), C.c_double()
+    L.check(L.lib().hh_solver_kernel_time(s._handle, C.byref(ms), C.byref(n), C.byref(b)))
+    return {"ms": ms.value, "launches": int(n.value), "bytes": b.value}

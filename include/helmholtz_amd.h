@@ -135,6 +135,23 @@ int hh_solve(hh_solver* s, const void* B, void* X, int64_t nrhs, int do_transpos
  * Inspection/test hook with no counterpart in the reference. */
 int hh_solver_precondition(hh_solver* s, const void* V, void* Z, int64_t nrhs, int loc);
 
+/* Per-kernel timing inside hh_solve (instrumentation with no counterpart in
+ * the reference): every later launch of `kernel` on grid level `level` is
+ * bracketed by HIP events on the solver's stream, and its elapsed time,
+ * launch count and algorithmic bytes (the bytes each launch must move for the
+ * RHS still iterating) accumulate until the next hh_solver_time_kernel call.
+ * HH_KT_NONE switches it off.  Graph replay is not used while timing is on.
+ * Algorithmic bytes per launch (s = bytes per preconditioner value, N / Nc =
+ * nodes of the level / the next coarser level, a = active RHS):
+ *   DOWN   a (2 N s + Nc s) + N s      MID  a (3 N s + 2 Nc s) + N s
+ *   UP     a (3 N s + Nc s) + N s      APPLY a N (s_z + 16) + 16 N
+ *   DOTS   16 a N (j + 2)              UPDATE 16 a N (j + 3)  (Arnoldi step j) */
+enum {
+  HH_KT_NONE = 0, HH_KT_DOWN = 1, HH_KT_MID = 2, HH_KT_UP = 3, HH_KT_APPLY = 4, HH_KT_DOTS = 5, HH_KT_UPDATE = 6
+};
+int hh_solver_time_kernel(hh_solver* s, int kernel, int level);
+int hh_solver_kernel_time(hh_solver* s, double* total_ms, int64_t* launches, double* alg_bytes);
+
 /* Number of grid levels the solver's hierarchy has (0 before the first solve)
  * and the node counts of level l. */
 int hh_solver_levels(const hh_solver* s, int* levels, int64_t nnodes_of_level[][3], int max_levels);

@@ -300,3 +300,37 @@ def test_fused_legs_equal_unfused(nodes, prec):
     err = np.linalg.norm(zs[0] - zs[1]) / np.linalg.norm(zs[1])
     print(f"fused vs unfused {nodes} {prec}: {err:.2e}")
     assert err <= (2e-6 if prec == "c64" else 1e-13)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kernel,level", [("up", 0), ("down", 1), ("mid", 0), ("apply", 0), ("dots", 0),
+                                          ("update", 0)])
+def test_kernel_timing_counts_launches_and_keeps_results(kernel, level):
+    """Event timing of one kernel inside the solve: one timed launch per
+    preconditioner application (per Arnoldi step for H z and the passes), the
+    launches' algorithmic bytes as documented, and iterates identical to an
+    untimed solve."""
+    M, m, g, w = small_case((129, 129), seed=33, neumann=True)
+    rng = np.random.default_rng(10)
+    nr = 3
+    B = np.asfortranarray(rng.standard_normal((m.size, nr)) + 1j * rng.standard_normal((m.size, nr)))
+    s0 = _solver(M, m, g, w, True, 4, 0.5, cycle="W", prec="c64")
+    X0, s0 = ha.solveLinearSystem(None, B, s0)
+    s1 = _solver(M, m, g, w, True, 4, 0.5, cycle="W", prec="c64")
+    ha.time_kernel(s1, kernel, level)
+    X1, s1 = ha.solveLinearSystem(None, B, s1)
+    assert np.array_equal(X0, X1)
+    t = ha.kernel_time(s1)
+    nprec = s1.nPrec
+    per_visit = {"up": 1, "down": 1, "mid": 1, "apply": 1, "dots": 1, "update": 1}[kernel]
+    # level 1 is visited twice per cycle by the W cycle
+    visits = 2 if level == 1 else 1
+    assert t["launches"] == nprec * per_visit * visits
+    assert t["ms"] > 0.0 and t["bytes"] > 0.0
+    N = m.size
+    if kernel == "apply":
+        # every step with all RHS active except the last few: bounded by the full-batch figure
+        assert t["bytes"] <= t["launches"] * (nr * N * 24.0 + N * 16.0) + 1.0
+        assert t["bytes"] >= t["launches"] * (N * 24.0 + N * 16.0) - 1.0
+    ha.time_kernel(s1, "none")
+    assert ha.kernel_time(s1)["launches"] == 0
