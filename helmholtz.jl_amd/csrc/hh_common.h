@@ -77,4 +77,38 @@ __device__ __forceinline__ double lap_diag1(int i, int n, int bc) {
   return (bc == 1 && (i == 0 || i == n - 1)) ? 1.0 : 2.0;
 }
 
+// XCD-aware work order.  A launch is a 1-D grid of nxt * nyt * nrhs blocks;
+// the hardware hands block b to XCD b % 8, and each XCD has its own L2.  Block
+// b is given the (b / 8)-th item of a contiguous 1/8 slice of the work list
+// ordered (RHS group, tile row, RHS within the group, tile column), so each
+// XCD walks its own RHS slab row by row: a tile row's coefficients are reused
+// by the G RHS of the group and the halo rows of the previous tile row (same
+// RHS) are still in that XCD's L2.
+struct TileMap {
+  int nxt, nyt, G;
+  unsigned T;  // nxt * nyt * nrhs < 2^31
+};
+
+inline TileMap make_tile_map(int nxt, int nyt, int nrhs) {
+  TileMap m;
+  m.nxt = nxt;
+  m.nyt = nyt;
+  m.G = nrhs % 4 == 0 ? 4 : (nrhs % 2 == 0 ? 2 : 1);
+  m.T = (unsigned)((int64_t)nxt * nyt * nrhs);
+  return m;
+}
+
+__device__ __forceinline__ void tile_of(const TileMap& m, int& xt, int& yt, int& rhs) {
+  const unsigned b = blockIdx.x;
+  const unsigned per = m.T >> 3, rem = m.T & 7u;
+  const unsigned xcd = b & 7u;
+  unsigned L = xcd * per + (xcd < rem ? xcd : rem) + (b >> 3);
+  xt = (int)(L % (unsigned)m.nxt);
+  L /= (unsigned)m.nxt;
+  const int r_in = (int)(L % (unsigned)m.G);
+  L /= (unsigned)m.G;
+  yt = (int)(L % (unsigned)m.nyt);
+  rhs = (int)(L / (unsigned)m.nyt) * m.G + r_in;
+}
+
 }  // namespace hh

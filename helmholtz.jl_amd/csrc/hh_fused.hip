@@ -42,31 +42,6 @@ struct Tile {
   static constexpr int NPIX = W * HT;
 };
 
-// XCD-aware work order.  A launch is a 1-D grid of nxt * nyt * nrhs blocks;
-// the hardware hands block b to XCD b % 8, and each XCD has its own L2.  Block
-// b is given the (b / 8)-th item of a contiguous 1/8 slice of the work list
-// ordered (RHS group, tile row, RHS within the group, tile column), so each
-// XCD walks its own RHS slab row by row: a tile row's diagonal is reused by the
-// G RHS of the group and the halo rows of the previous tile row (same RHS) are
-// still in that XCD's L2.
-struct TileMap {
-  int nxt, nyt, G;
-  int64_t T;
-};
-
-__device__ __forceinline__ void tile_of(const TileMap& m, int& xt, int& yt, int& rhs) {
-  const int64_t b = blockIdx.x;
-  const int64_t per = m.T >> 3, rem = m.T & 7;
-  const int64_t xcd = b & 7;
-  int64_t L = xcd * per + (xcd < rem ? xcd : rem) + (b >> 3);
-  xt = (int)(L % m.nxt);
-  L /= m.nxt;
-  const int r_in = (int)(L % m.G);
-  L /= m.G;
-  yt = (int)(L % m.nyt);
-  rhs = (int)(L / m.nyt) * m.G + r_in;
-}
-
 // Each thread owns the tile positions q = threadIdx.x + 256 k (k < K) in every
 // phase, so b and d live in its registers; only the x fields (and the
 // residual) are staged in LDS, where neighbours are read.  Two LDS arrays per
@@ -316,12 +291,7 @@ bool fused_ok(const Grid& gf) { return gf.dim == 2 && gf.bc == 2 && gf.n1 >= 16 
 template <typename T>
 TileMap tile_map(const Grid& gf, int nrhs) {
   constexpr int TY = FusedTY<T>::v;
-  TileMap m;
-  m.nxt = (gf.n1 + FTX - 1) / FTX;
-  m.nyt = (gf.n2 + TY - 1) / TY;
-  m.G = nrhs % 4 == 0 ? 4 : (nrhs % 2 == 0 ? 2 : 1);
-  m.T = (int64_t)m.nxt * m.nyt * nrhs;
-  return m;
+  return make_tile_map((gf.n1 + FTX - 1) / FTX, (gf.n2 + TY - 1) / TY, nrhs);
 }
 
 template <typename T>
@@ -329,7 +299,7 @@ void launch_down_fused(const Grid& gf, const Grid& gc, int nrhs, const cx<T>* b,
                        cx<T>* xo, cx<T>* bco, const int* skip, hipStream_t s) {
   constexpr int TY = FusedTY<T>::v;
   const TileMap m = tile_map<T>(gf, nrhs);
-  hipLaunchKernelGGL((k_down_fused<T, TY>), dim3((unsigned)m.T), dim3(256), 0, s, m, gf, gc, b, d, f, xo, bco,
+  hipLaunchKernelGGL((k_down_fused<T, TY>), dim3(m.T), dim3(256), 0, s, m, gf, gc, b, d, f, xo, bco,
                      skip);
 }
 
@@ -338,7 +308,7 @@ void launch_mid_fused(const Grid& gf, const Grid& gc, int nrhs, const cx<T>* x, 
                       const cx<T>* d, cx<T>* xo, cx<T>* bco, const int* skip, hipStream_t s) {
   constexpr int TY = FusedTY<T>::v;
   const TileMap m = tile_map<T>(gf, nrhs);
-  hipLaunchKernelGGL((k_mid_fused<T, TY>), dim3((unsigned)m.T), dim3(256), 0, s, m, gf, gc, x, ec, b, d, xo, bco,
+  hipLaunchKernelGGL((k_mid_fused<T, TY>), dim3(m.T), dim3(256), 0, s, m, gf, gc, x, ec, b, d, xo, bco,
                      skip);
 }
 
@@ -347,7 +317,7 @@ void launch_up_fused(const Grid& gf, const Grid& gc, int nrhs, const cx<T>* x, c
                      const cx<T>* d, const cx<T>* f, cx<T>* xo, const int* skip, hipStream_t s) {
   constexpr int TY = FusedTY<T>::v;
   const TileMap m = tile_map<T>(gf, nrhs);
-  hipLaunchKernelGGL((k_up_fused<T, TY>), dim3((unsigned)m.T), dim3(256), 0, s, m, gf, gc, x, ec, b, d, f, xo,
+  hipLaunchKernelGGL((k_up_fused<T, TY>), dim3(m.T), dim3(256), 0, s, m, gf, gc, x, ec, b, d, f, xo,
                      skip);
 }
 

@@ -48,11 +48,13 @@ template <typename TI, typename TO, typename TD, int MODE>
 __global__ __launch_bounds__(256) void k_stencil2d(Grid g, const cx<TI>* __restrict__ x,
                                                    const cx<TO>* __restrict__ b, cx<TO>* __restrict__ y,
                                                    const cx<TD>* __restrict__ d, const double* __restrict__ m,
-                                                   cx<double> sfac, TD wj, int rows, const int* __restrict__ skip) {
-  const int rhs = blockIdx.z;
+                                                   cx<double> sfac, TD wj, int rows, TileMap tm,
+                                                   const int* __restrict__ skip) {
+  int bx, by, rhs;
+  tile_of(tm, bx, by, rhs);
   if (skip && skip[rhs]) return;
-  const int i = blockIdx.x * 64 + threadIdx.x;
-  const int j0 = (blockIdx.y * 4 + threadIdx.y) * rows;
+  const int i = bx * 64 + threadIdx.x;
+  const int j0 = (by * 4 + threadIdx.y) * rows;
   const int n1 = g.n1, n2 = g.n2;
   if (i >= n1 || j0 >= n2) return;
   const int j1 = min(j0 + rows, n2);
@@ -89,13 +91,14 @@ template <typename TI, typename TO, typename TD, int MODE>
 __global__ __launch_bounds__(256) void k_stencil3d(Grid g, const cx<TI>* __restrict__ x,
                                                    const cx<TO>* __restrict__ b, cx<TO>* __restrict__ y,
                                                    const cx<TD>* __restrict__ d, const double* __restrict__ m,
-                                                   cx<double> sfac, TD wj, int rows, int ytiles,
+                                                   cx<double> sfac, TD wj, int rows, int ytiles, TileMap tm,
                                                    const int* __restrict__ skip) {
-  const int rhs = blockIdx.z;
+  int bx, by, rhs;
+  tile_of(tm, bx, by, rhs);
   if (skip && skip[rhs]) return;
-  const int i = blockIdx.x * 64 + threadIdx.x;
-  const int yt = blockIdx.y % ytiles;
-  const int zc = blockIdx.y / ytiles;
+  const int i = bx * 64 + threadIdx.x;
+  const int yt = by % ytiles;
+  const int zc = by / ytiles;
   const int j = yt * 4 + threadIdx.y;
   const int k0 = zc * rows;
   const int n1 = g.n1, n2 = g.n2, n3 = g.n3;
@@ -518,13 +521,14 @@ void launch_stencil(const Grid& g, int nrhs, const cx<TI>* x, const cx<TO>* b, c
                     const double* m, cx<double> sfac, double wj, const int* skip, hipStream_t s, int rows_override) {
   int ytiles;
   const int rows = rows_override > 0 ? rows_override : pick_rows(g, nrhs);
-  dim3 grid = stencil_grid(g, nrhs, rows, &ytiles);
+  const dim3 grid = stencil_grid(g, nrhs, rows, &ytiles);
+  const TileMap tm = make_tile_map((int)grid.x, (int)grid.y, nrhs);
   if (g.dim == 2)
-    hipLaunchKernelGGL((k_stencil2d<TI, TO, TD, MODE>), grid, dim3(64, 4), 0, s, g, x, b, y, d, m, sfac, (TD)wj,
-                       rows, skip);
+    hipLaunchKernelGGL((k_stencil2d<TI, TO, TD, MODE>), dim3(tm.T), dim3(64, 4), 0, s, g, x, b, y, d, m, sfac,
+                       (TD)wj, rows, tm, skip);
   else
-    hipLaunchKernelGGL((k_stencil3d<TI, TO, TD, MODE>), grid, dim3(64, 4), 0, s, g, x, b, y, d, m, sfac, (TD)wj,
-                       rows, ytiles, skip);
+    hipLaunchKernelGGL((k_stencil3d<TI, TO, TD, MODE>), dim3(tm.T), dim3(64, 4), 0, s, g, x, b, y, d, m, sfac,
+                       (TD)wj, rows, ytiles, tm, skip);
 }
 
 template void launch_stencil<double, double, double, M_APPLY>(const Grid&, int, const cx<double>*,
