@@ -79,7 +79,7 @@ def timed_solves(ha, solver, Bd, warmup, steps, log, name, before_timed=None):
         X, solver = ha.solveLinearSystem(None, Bd, solver)
         times.append(time.perf_counter() - t0)
         log(f"{name} step {k}: {times[-1]:.3f} s, max iters {solver.last_stats['max_iters']}, "
-            f"fell back {solver.last_stats['fell_back']}")
+            f"learned used {solver.last_stats['learned_used']}, fell back {solver.last_stats['fell_back']}")
     return float(np.mean(times)), solver.last_stats
 
 
@@ -252,10 +252,11 @@ def main():
         "data": "synthetic (seeded layered+body medium, point sources)",
         "config": {"workload": "2-D 1025x513 Helmholtz, 256 point sources, 8 ppw, 1e-6 true residual",
                    "nsrc": args.nsrc, "levels": 6, "cycle": "W(2,2) on levels 0-2, V(2,2) below", "krylov": "FGMRES(10)",
-                   "learned": "U-Net at level 2 (257x129), plain-cycle fallback", "graphs": False,
+                   "learned": "U-Net at level 2 (257x129) vs plain cycle, chosen by a two-restart trial on the first solve", "graphs": False,
                    "autotune": True, "parallelism": "rhs-batched, 1 GPU"},
         "learned_ms_per_rhs": round(learned_ms_rhs, 4),
         "learned_max_iters": int(st_l["max_iters"]),
+        "learned_used": bool(st_l["learned_used"]),
         "learned_fell_back": bool(st_l["fell_back"]),
         "learned_max_true_res": float(res_l.max()),
         "training": tinfo,

@@ -230,6 +230,10 @@ struct hh_solver {
   int learn_level = -1;
   bool use_learned = false;
   bool fell_back = false;
+  bool learned_ran = false;  // the network was applied in the last solve
+  // outcome of the learned/plain trial on this hierarchy and network:
+  // -1 not run yet, 0 plain cycle kept, 1 network kept (later solves skip the trial)
+  int learn_verdict = -1;
   float* feat_kh2 = nullptr;
   float* feat_gw = nullptr;
   float* nn_scale = nullptr;
@@ -283,6 +287,7 @@ struct hh_solver {
     dfree(coarse_aug); dfree(coarse_d); dfree(piv); dfree(fac); dfree(pinv);
     coarse_aug = nullptr; coarse_d = nullptr; piv = nullptr; fac = nullptr; pinv = nullptr;
     built = false;
+    learn_verdict = -1;
   }
   void free_learned() {
     dfree(feat_kh2); dfree(feat_gw); dfree(nn_scale);
@@ -701,9 +706,11 @@ void run_fgmres(hh_solver* S, int nrhs, hh_solve_stats* stats) {
   HIPCK(hipMemcpyAsync(S->wk, S->bk, sizeof(cx<double>) * blk, hipMemcpyDeviceToDevice, st));
   std::vector<double> res(nrhs, 1.0), prev(nrhs, 1.0);
   int64_t nprec = 0;
-  S->use_learned = S->net != nullptr && S->p.prec == HH_C64;
-  S->fell_back = false;
-  int trial = S->use_learned ? 0 : -1;
+  const bool have_net = S->net != nullptr && S->p.prec == HH_C64;
+  S->use_learned = have_net && S->learn_verdict != 0;
+  S->fell_back = have_net && S->learn_verdict == 0;
+  S->learned_ran = S->use_learned;
+  int trial = S->use_learned && S->learn_verdict < 0 ? 0 : -1;
   double rho_learned = 1.0;
   if (S->use_learned) setup_learned(S);
   for (int outer = 0; outer < S->p.max_outer; ++outer) {
@@ -813,6 +820,7 @@ void run_fgmres(hh_solver* S, int nrhs, hh_solve_stats* stats) {
     } else if (trial == 1) {
       S->use_learned = rho_learned < 0.8 * rho;
       S->fell_back = !S->use_learned;
+      S->learn_verdict = S->use_learned ? 1 : 0;
       if (S->p.verbose)
         fprintf(stderr, "[helmholtz_amd] restart reduction learned %.3f plain %.3f -> %s\n", rho_learned, rho,
                 S->use_learned ? "learned" : "plain");
@@ -824,6 +832,7 @@ void run_fgmres(hh_solver* S, int nrhs, hh_solve_stats* stats) {
       // columns it drove to non-finite values restart from zero
       S->use_learned = false;
       S->fell_back = true;
+      S->learn_verdict = 0;
       trial = 2;
       if (S->p.verbose) fprintf(stderr, "[helmholtz_amd] learned correction stagnated; plain cycle from restart %d\n", outer);
       HIPCK(hipMemcpyAsync(S->conv_d, bad.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, st));
@@ -864,7 +873,7 @@ void run_fgmres(hh_solver* S, int nrhs, hh_solve_stats* stats) {
     stats->max_iters = mx;
     stats->n_converged = nc;
     stats->n_prec = nprec;
-    stats->learned_used = S->net != nullptr && S->p.prec == HH_C64;
+    stats->learned_used = S->learned_ran;
     stats->fell_back = S->fell_back;
   }
   bool warn = false;
@@ -1181,6 +1190,7 @@ int hh_solver_attach_unet(hh_solver* s, hh_unet* net, int level) {
         throw HHError(HH_ERR_ARG, "the learned level's grid needs 8k + 1 nodes per axis");
     }
     s->free_learned();
+    s->learn_verdict = -1;
     s->net = net;
     s->learn_level = net ? level : -1;
   });

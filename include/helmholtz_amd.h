@@ -75,8 +75,9 @@ typedef struct hh_solve_stats {
   double setup_time;   /* seconds spent building the hierarchy in this call     */
   double solve_time;   /* seconds in the Krylov solve                            */
   int64_t n_prec;      /* preconditioner applications (lock-step iterations)    */
-  int32_t learned_used; /* 1 if the solve started with the learned correction    */
-  int32_t fell_back;    /* 1 if it stagnated and switched to the plain cycle      */
+  int32_t learned_used; /* 1 if the learned correction was applied in this solve */
+  int32_t fell_back;    /* 1 if the plain cycle replaced it (trial lost, now or in
+                           an earlier solve on this hierarchy, or stagnation)     */
 } hh_solve_stats;
 
 const char* hh_last_error(void);
@@ -216,7 +217,9 @@ int hh_unet_eval(hh_unet* u, hh_dataset* d, int64_t first, int64_t count, float*
  * cycle, and the solve keeps the network only if its cycle reduced the
  * residual (geometric mean over RHS) below 0.8x the plain cycle's reduction;
  * afterwards the plain cycle also takes over if a restart cycle stops
- * reducing every residual by 10% or produces non-finite values.
+ * reducing every residual by 10% or produces non-finite values.  The verdict
+ * is kept for later solves on the same hierarchy (they skip the trial) until
+ * the network is re-attached or the hierarchy is rebuilt.
  * hh_solve_stats.fell_back reports the plain cycle was used.  net = NULL detaches. */
 int hh_solver_attach_unet(hh_solver* s, hh_unet* net, int level);
 

@@ -135,3 +135,12 @@ def test_training_lowers_loss_and_learned_solve_converges():
     # the restart-0/restart-1 trial bounds what a weak network can cost: two
     # restart cycles of 10 iterations at most
     assert sl.last_stats["max_iters"] <= sp.last_stats["max_iters"] + 20
+    # the verdict is kept: a second solve on the same hierarchy skips the trial
+    first_fell_back = sl.last_stats["fell_back"]
+    Xl2, sl = ha.solveLinearSystem(None, B, sl)
+    if first_fell_back:
+        assert sl.last_stats["fell_back"] and not sl.last_stats["learned_used"]
+        assert sl.last_stats["max_iters"] == sp.last_stats["max_iters"]
+        assert np.array_equal(Xl2, Xp)
+    else:
+        assert sl.last_stats["learned_used"]
