@@ -217,6 +217,7 @@ struct hh_solver {
   void* b0 = nullptr;       // fine-level V-cycle input in c64 mode
   KrylovWork kw{};
   int* skip_copy = nullptr;
+  cx<double>* bsc = nullptr;  // BiCGSTAB per-RHS scalars (rho, alpha, omega, beta)
   int* conv_d = nullptr;
   double* n2tmp = nullptr;
   double* res_d = nullptr;
@@ -270,7 +271,8 @@ struct hh_solver {
     V = nullptr; Z = nullptr; xk = bk = wk = nullptr; b0 = nullptr;
     dfree(kw.part); dfree(kw.partn); dfree(kw.h1); dfree(kw.h2); dfree(kw.nrm); dfree(kw.H); dfree(kw.cs);
     dfree(kw.sn); dfree(kw.gv); dfree(kw.y); dfree(kw.bnorm); dfree(kw.est); dfree(kw.done); dfree(kw.kact);
-    dfree(kw.vscale); dfree(kw.fix); dfree(kw.wn2);
+    dfree(kw.vscale); dfree(kw.fix); dfree(kw.wn2); dfree(bsc);
+    bsc = nullptr;
     kw = KrylovWork{};
     dfree(skip_copy); dfree(conv_d); dfree(n2tmp); dfree(res_d);
     skip_copy = nullptr; conv_d = nullptr; n2tmp = nullptr; res_d = nullptr;
@@ -403,8 +405,10 @@ void ensure_work(hh_solver* S, int nrhs) {
   const int64_t N = S->lv[0].g.N;
   const int K = S->p.inner;
   const size_t blk = (size_t)N * nrhs;
-  S->V = dalloc<cx<double>>(blk * (K + 1));
-  S->Z = dalloc<char>(blk * K * es);
+  // BiCGSTAB keeps r, r^, p, v, s, t in the basis blocks and p^, s^ in Z
+  const bool bicg = S->p.krylov == HH_BICGSTAB;
+  S->V = dalloc<cx<double>>(blk * std::max(K + 1, bicg ? 6 : 0));
+  S->Z = dalloc<char>(blk * std::max(K, bicg ? 2 : 0) * es);
   S->xk = dalloc<cx<double>>(blk);
   S->bk = dalloc<cx<double>>(blk);
   S->wk = dalloc<cx<double>>(blk);
@@ -433,6 +437,7 @@ void ensure_work(hh_solver* S, int nrhs) {
   w.vscale = dalloc<double>((size_t)(K + 1) * nrhs);
   w.fix = dalloc<int>(nrhs);
   w.wn2 = dalloc<double>(nrhs);
+  S->bsc = dalloc<cx<double>>((size_t)4 * nrhs);
   S->skip_copy = dalloc<int>(nrhs);
   S->conv_d = dalloc<int>(nrhs);
   S->n2tmp = dalloc<double>(nrhs);
@@ -881,6 +886,133 @@ void run_fgmres(hh_solver* S, int nrhs, hh_solve_stats* stats) {
   if (warn) fprintf(stderr, "WARNING: MG solver reached maximum iterations without convergence\n");
 }
 
+// Batched right-preconditioned BiCGSTAB (KrylovMethods.jl's bicgstb as called
+// by solveBiCGSTAB_MG, src/ShiftedLaplacianMultigridSolver.jl:91-93): two
+// preconditioner applications per iteration, at most max_outer iterations,
+// stop on the recursive residual ||r|| <= tol ||b||; then the true residual
+// b - H x is formed in complex128 and RHS that do not meet the tolerance
+// restart from it (r^ = r) while iterations remain.
+void run_bicgstab(hh_solver* S, int nrhs, hh_solve_stats* stats) {
+  hipStream_t st = S->op->stream;
+  KrylovWork w = S->kw;
+  w.nrhs = nrhs;
+  w.lazy = 0;
+  const int64_t N = S->lv[0].g.N;
+  const int64_t blk = N * (int64_t)nrhs;
+  const double tol = S->p.tol;
+  const int maxit = S->p.max_outer;
+  std::vector<int> iters(nrhs, 0), conv(nrhs, 0);
+  kr_norm2(w, S->bk, S->n2tmp, nullptr, st);
+  hipLaunchKernelGGL(k_sqrt_arr, dim3((nrhs + 63) / 64), dim3(64), 0, st, S->n2tmp, w.bnorm, nrhs);
+  HIPCK(hipMemcpyAsync(S->h_res, w.bnorm, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
+  HIPCK(hipStreamSynchronize(st));
+  std::vector<double> bn(S->h_res, S->h_res + nrhs), res(nrhs, 1.0);
+  for (int b = 0; b < nrhs; ++b) conv[b] = !(bn[b] > 0.0);
+  HIPCK(hipMemsetAsync(S->xk, 0, sizeof(cx<double>) * blk, st));
+  cx<double>* r = S->V;
+  cx<double>* rh = S->V + blk;
+  cx<double>* pv = S->V + 2 * blk;
+  cx<double>* v = S->V + 3 * blk;
+  cx<double>* sv = S->V + 4 * blk;
+  cx<double>* t = S->V + 5 * blk;
+  const size_t zes = csz(S->p.prec);
+  void* ph = S->Z;
+  void* shat = (char*)S->Z + (size_t)blk * zes;
+  S->use_learned = false;
+  S->learned_ran = false;
+  S->fell_back = false;
+  int64_t nprec = 0;
+  int total = 0;
+  auto prec_apply = [&](const cx<double>* in, void* z, cx<double>* out) {
+    if (S->p.prec == HH_C64) {
+      precondition<float>(S, nrhs, in, z, w.done, st);
+      launch_stencil<float, double, double, M_APPLY>(S->op->g, nrhs, (const cx<float>*)z, nullptr, out, S->op->d64,
+                                                     nullptr, mk<double>(0, 0), 0.0, w.done, st);
+    } else {
+      precondition<double>(S, nrhs, in, z, w.done, st);
+      h_apply(S, nrhs, (const cx<double>*)z, out, w.done, st);
+    }
+  };
+  for (int cycle = 0;; ++cycle) {
+    // true residual in complex128
+    h_resid(S, nrhs, S->xk, S->bk, r, st);
+    kr_true_res(w, r, S->res_d, S->n2tmp, st);
+    HIPCK(hipMemcpyAsync(S->h_res, S->res_d, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
+    HIPCK(hipStreamSynchronize(st));
+    int active = 0;
+    for (int b = 0; b < nrhs; ++b) {
+      res[b] = bn[b] > 0.0 ? S->h_res[b] : 0.0;
+      if (!conv[b] && (res[b] <= tol || !(res[b] == res[b]))) conv[b] = 1;
+      active += !conv[b];
+    }
+    if (S->p.verbose) fprintf(stderr, "[helmholtz_amd] BiCGSTAB cycle %d: %d/%d RHS converged\n", cycle, nrhs - active, nrhs);
+    if (active == 0 || total >= maxit || cycle >= 20) break;
+    HIPCK(hipMemcpyAsync(rh, r, sizeof(cx<double>) * blk, hipMemcpyDeviceToDevice, st));
+    HIPCK(hipMemcpyAsync(S->conv_d, conv.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, st));
+    bicg_start(w, S->conv_d, st);
+    const int limit = maxit - total;
+    int steps = 0;
+    for (int it = 0; it < limit; ++it) {
+      const int first = it == 0;
+      bicg_dots(w, rh, r, nullptr, nullptr, 1, st);
+      bicg_scalar(w, S->bsc, 0, first, tol, limit, st);
+      bicg_p(w, S->bsc, r, pv, v, first, st);
+      prec_apply(pv, ph, v);
+      bicg_dots(w, rh, v, nullptr, nullptr, 1, st);
+      bicg_scalar(w, S->bsc, 1, first, tol, limit, st);
+      bicg_s(w, S->bsc, r, v, sv, st);
+      prec_apply(sv, shat, t);
+      bicg_dots(w, t, sv, t, t, 2, st);
+      bicg_scalar(w, S->bsc, 2, first, tol, limit, st);
+      if (S->p.prec == HH_C64)
+        bicg_xr<float>(w, S->bsc, (const cx<float>*)ph, (const cx<float>*)shat, sv, t, S->xk, r, st);
+      else
+        bicg_xr<double>(w, S->bsc, (const cx<double>*)ph, (const cx<double>*)shat, sv, t, S->xk, r, st);
+      bicg_scalar(w, S->bsc, 3, first, tol, limit, st);
+      nprec += 2;
+      ++steps;
+      HIPCK(hipMemcpyAsync(S->h_done, w.done, sizeof(int) * nrhs, hipMemcpyDeviceToHost, st));
+      HIPCK(hipStreamSynchronize(st));
+      int left = 0;
+      for (int b = 0; b < nrhs; ++b) left += S->h_done[b] == 0;
+      if (left == 0) break;
+    }
+    HIPCK(hipMemcpyAsync(S->h_kact, w.kact, sizeof(int) * nrhs, hipMemcpyDeviceToHost, st));
+    HIPCK(hipStreamSynchronize(st));
+    HIPCK(hipGetLastError());
+    for (int b = 0; b < nrhs; ++b) iters[b] += conv[b] ? 0 : S->h_kact[b];
+    total += steps;
+  }
+  S->n_prec += nprec;
+  if (stats) {
+    int mx = 0, nc = 0;
+    for (int b = 0; b < nrhs; ++b) {
+      int status;
+      if (!(bn[b] > 0.0))
+        status = HH_ZERO_RHS;
+      else if (!(res[b] == res[b]))
+        status = HH_NONFINITE;
+      else if (res[b] <= tol)
+        status = HH_CONVERGED;
+      else
+        status = HH_MAXITER;
+      if (stats->status) stats->status[b] = status;
+      if (stats->iters) stats->iters[b] = iters[b];
+      if (stats->true_res) stats->true_res[b] = res[b];
+      mx = std::max(mx, iters[b]);
+      nc += (status == HH_CONVERGED || status == HH_ZERO_RHS);
+    }
+    stats->max_iters = mx;
+    stats->n_converged = nc;
+    stats->n_prec = nprec;
+    stats->learned_used = 0;
+    stats->fell_back = 0;
+  }
+  bool warn = false;
+  for (int b = 0; b < nrhs; ++b) warn |= (bn[b] > 0.0 && !(res[b] <= tol));
+  if (warn) fprintf(stderr, "WARNING: MG solver reached maximum iterations without convergence\n");
+}
+
 }  // namespace
 
 // ============================================================== C ABI
@@ -980,7 +1112,7 @@ int hh_solver_create(hh_op* op, const hh_mg_params* p, const double* shift_per_l
     if (p->inner < 1 || p->inner > 32) throw HHError(HH_ERR_ARG, "inner must be in 1..32");
     if (p->pre < 0 || p->post < 0 || p->pre > 64 || p->post > 64) throw HHError(HH_ERR_ARG, "bad sweep counts");
     if (p->cycle != 'V' && p->cycle != 'W') throw HHError(HH_ERR_ARG, "cycle must be 'V' or 'W'");
-    if (p->krylov != HH_FGMRES) throw HHError(HH_ERR_UNSUPPORTED, "only HH_FGMRES is implemented");
+    if (p->krylov != HH_FGMRES && p->krylov != HH_BICGSTAB) throw HHError(HH_ERR_ARG, "unknown krylov method");
     if (p->prec != HH_C128 && p->prec != HH_C64) throw HHError(HH_ERR_ARG, "prec must be HH_C128 or HH_C64");
     if (p->learned != 0) throw HHError(HH_ERR_UNSUPPORTED, "learned correction not available in this entry point");
     if (p->max_outer < 1 || !(p->tol > 0.0)) throw HHError(HH_ERR_ARG, "max_outer >= 1 and tol > 0 required");
@@ -1031,7 +1163,7 @@ int hh_solve(hh_solver* s, const void* B, void* X, int64_t nrhs, int do_transpos
   return guarded([&] {
     if (!s || !B || !X) throw HHError(HH_ERR_ARG, "null argument");
     if (nrhs < 1 || nrhs > 65535) throw HHError(HH_ERR_ARG, "nrhs must be in 1..65535");
-    if (do_transpose != 0) throw HHError(HH_ERR_UNSUPPORTED, "do_transpose = 1 is not implemented");
+    if (do_transpose != 0 && do_transpose != 1) throw HHError(HH_ERR_ARG, "do_transpose must be 0 or 1");
     if (loc != HH_HOST && loc != HH_DEVICE) throw HHError(HH_ERR_ARG, "loc must be HH_HOST or HH_DEVICE");
     hh_op* op = s->op;
     hipStream_t st = op->stream;
@@ -1045,7 +1177,55 @@ int hh_solve(hh_solver* s, const void* B, void* X, int64_t nrhs, int do_transpos
     s->setup_time += t1 - t0;
     if (stats) stats->setup_time = t1 - t0;
     HIPCK(hipMemcpyAsync(s->bk, B, bytes, loc == HH_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, st));
-    run_fgmres(s, (int)nrhs, stats);
+    const int nr = (int)nrhs;
+    KrylovWork w = s->kw;
+    w.nrhs = nr;
+    std::vector<double> bn0;
+    const double tol0 = s->p.tol;
+    if (do_transpose) {
+      // adjoint solve (doTranspose = 1, src/ShiftedLaplacianMultigridSolver.jl:68-80): H^H x = b
+      // <=> H y = W^-1 conj(b), x = W conj(y).  ||W^-1|| <= 2^dim, so the
+      // inner tolerance tol / 2^dim bounds ||H^H x - b|| <= tol ||b||.
+      kr_norm2(w, s->bk, s->n2tmp, nullptr, st);
+      bn0.resize(nr);
+      HIPCK(hipMemcpyAsync(bn0.data(), s->n2tmp, sizeof(double) * nr, hipMemcpyDeviceToHost, st));
+      adjoint_map(op->g, s->bk, nr, 0, st);
+      if (op->bc == 2) s->p.tol = tol0 / (double)(1 << op->dim);
+    }
+    try {
+      if (s->p.krylov == HH_BICGSTAB)
+        run_bicgstab(s, nr, stats);
+      else
+        run_fgmres(s, nr, stats);
+    } catch (...) {
+      s->p.tol = tol0;
+      throw;
+    }
+    s->p.tol = tol0;
+    if (do_transpose) {
+      // adjoint residual W conj(c - H y) and the solution x = W conj(y)
+      h_resid(s, nr, s->xk, s->bk, s->wk, st);
+      adjoint_map(op->g, s->wk, nr, 2, st);
+      kr_norm2(w, s->wk, s->n2tmp, nullptr, st);
+      std::vector<double> rn(nr);
+      HIPCK(hipMemcpyAsync(rn.data(), s->n2tmp, sizeof(double) * nr, hipMemcpyDeviceToHost, st));
+      adjoint_map(op->g, s->xk, nr, 1, st);
+      HIPCK(hipStreamSynchronize(st));
+      if (stats) {
+        int nc = 0;
+        for (int b = 0; b < nr; ++b) {
+          const double res = bn0[b] > 0.0 ? sqrt(rn[b] / bn0[b]) : 0.0;
+          int status = !(bn0[b] > 0.0) ? HH_ZERO_RHS
+                       : !(res == res) ? HH_NONFINITE
+                       : res <= tol0   ? HH_CONVERGED
+                                       : HH_MAXITER;
+          if (stats->true_res) stats->true_res[b] = res;
+          if (stats->status) stats->status[b] = status;
+          nc += (status == HH_CONVERGED || status == HH_ZERO_RHS);
+        }
+        stats->n_converged = nc;
+      }
+    }
     HIPCK(hipMemcpyAsync(X, s->xk, bytes, loc == HH_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, st));
     HIPCK(hipStreamSynchronize(st));
     double t2 = now_s();

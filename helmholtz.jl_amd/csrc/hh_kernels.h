@@ -151,4 +151,19 @@ void kr_update_x(const KrylovWork& w, const cx<TZ>* Z, cx<double>* x, hipStream_
 // res[b] = ||r_b|| / ||b_b||
 void kr_true_res(const KrylovWork& w, const cx<double>* r, double* res, double* n2tmp, hipStream_t s);
 
+// ---- batched BiCGSTAB (hh_krylov.hip); sc: 4 * nrhs per-RHS scalars ----
+void bicg_start(const KrylovWork& w, const int* conv, hipStream_t s);
+void bicg_dots(const KrylovWork& w, const cx<double>* a0, const cx<double>* b0, const cx<double>* a1,
+               const cx<double>* b1, int nd, hipStream_t s);
+void bicg_scalar(const KrylovWork& w, cx<double>* sc, int stage, int first, double tol, int limit, hipStream_t s);
+void bicg_p(const KrylovWork& w, const cx<double>* sc, const cx<double>* r, cx<double>* p, const cx<double>* v,
+            int first, hipStream_t s);
+void bicg_s(const KrylovWork& w, const cx<double>* sc, const cx<double>* r, const cx<double>* v, cx<double>* sv,
+            hipStream_t s);
+template <typename TZ>
+void bicg_xr(const KrylovWork& w, const cx<double>* sc, const cx<TZ>* ph, const cx<TZ>* shat, const cx<double>* sv,
+             const cx<double>* t, cx<double>* x, cx<double>* r, hipStream_t s);
+// adjoint-solve transforms: mode 0 v = W^-1 conj(v), 1 v = W conj(v), 2 v = W v
+void adjoint_map(const Grid& g, cx<double>* v, int nrhs, int mode, hipStream_t s);
+
 }  // namespace hh

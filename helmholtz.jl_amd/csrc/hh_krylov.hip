@@ -491,4 +491,215 @@ void kr_true_res(const KrylovWork& w, const cx<double>* r, double* res, double* 
   hipLaunchKernelGGL(k_div_bnorm, dim3((w.nrhs + 63) / 64), dim3(64), 0, s, w, n2tmp, res);
 }
 
+// ------------------------------------------------------------ BiCGSTAB
+// Batched right-preconditioned BiCGSTAB (van der Vorst), one independent
+// iteration per RHS advanced by the same launches.  Per-RHS scalars live in
+// sc[k * nrhs + b]: k = 0 rho, 1 alpha, 2 omega, 3 beta.
+
+// part[c][k][b] = sum over chunk c of conj(a_k) b_k for k < nd (nd <= 2)
+__global__ __launch_bounds__(256) void k_bdots(KrylovWork kw, const cx<double>* __restrict__ a0,
+                                               const cx<double>* __restrict__ b0, const cx<double>* __restrict__ a1,
+                                               const cx<double>* __restrict__ b1, int nd) {
+  const int c = blockIdx.x, b = blockIdx.y;
+  if (kw.done[b]) return;
+  __shared__ double sh[4];
+  int64_t p0, p1;
+  chunk_range(kw.N, kw.nchunk, c, &p0, &p1);
+  const int64_t base = (int64_t)b * kw.N;
+  cx<double> s0 = mk<double>(0.0, 0.0), s1 = mk<double>(0.0, 0.0);
+  for (int64_t p = p0 + threadIdx.x; p < p1; p += 256) {
+    s0 = s0 + cmulc(a0[base + p], b0[base + p]);
+    if (nd > 1) s1 = s1 + cmulc(a1[base + p], b1[base + p]);
+  }
+  for (int k = 0; k < nd; ++k) {
+    const cx<double> v = k == 0 ? s0 : s1;
+    const double re = block_sum(v.re, sh);
+    const double im = block_sum(v.im, sh);
+    if (threadIdx.x == 0) kw.part[((int64_t)c * (kw.kmax + 1) + k) * kw.nrhs + b] = mk<double>(re, im);
+  }
+}
+
+__device__ __forceinline__ cx<double> reduce_part(const KrylovWork& kw, int k, int b) {
+  double re = 0.0, im = 0.0;
+  for (int c = threadIdx.x; c < kw.nchunk; c += 64) {
+    const cx<double> v = kw.part[((int64_t)c * (kw.kmax + 1) + k) * kw.nrhs + b];
+    re += v.re;
+    im += v.im;
+  }
+  return mk<double>(wave_sum(re), wave_sum(im));
+}
+
+__device__ __forceinline__ cx<double> cdiv(cx<double> a, cx<double> b) { return a * cinv(b); }
+
+// scalar stages (one wavefront per RHS):
+//   0: rho' = <rhat, r>; beta = (rho'/rho)(alpha/omega) (0 on the first step)
+//   1: alpha = rho / <rhat, v>
+//   2: omega = <t, s> / <t, t>
+//   3: ||r|| estimate, step count, done flag
+// A zero denominator (breakdown) ends the RHS's iteration.
+__global__ __launch_bounds__(64) void k_bicg_scalar(KrylovWork kw, cx<double>* __restrict__ sc, int stage, int first,
+                                                    double tol, int limit) {
+  const int b = blockIdx.x;
+  if (kw.done[b]) return;
+  const int n = kw.nrhs;
+  if (stage == 3) {
+    double nn = 0.0;
+    for (int c = threadIdx.x; c < kw.nchunk; c += 64) nn += kw.partn[(int64_t)c * n + b];
+    nn = wave_sum(nn);
+    if (threadIdx.x != 0) return;
+    const double est = sqrt(nn) / kw.bnorm[b];
+    kw.est[b] = est;
+    kw.kact[b] += 1;
+    if (!(est > tol) || kw.kact[b] >= limit) kw.done[b] = 1;
+    return;
+  }
+  const cx<double> d0 = reduce_part(kw, 0, b);
+  const cx<double> d1 = stage == 2 ? reduce_part(kw, 1, b) : mk<double>(0.0, 0.0);
+  if (threadIdx.x != 0) return;
+  if (stage == 0) {
+    if (!(cabs2(d0) > 0.0)) {
+      kw.done[b] = 1;
+      return;
+    }
+    const cx<double> beta = first ? mk<double>(0.0, 0.0)
+                                  : cdiv(d0, sc[0 * n + b]) * cdiv(sc[1 * n + b], sc[2 * n + b]);
+    sc[0 * n + b] = d0;
+    sc[3 * n + b] = beta;
+  } else if (stage == 1) {
+    if (!(cabs2(d0) > 0.0)) {
+      kw.done[b] = 1;
+      return;
+    }
+    sc[1 * n + b] = cdiv(sc[0 * n + b], d0);
+  } else {
+    sc[2 * n + b] = d1.re > 0.0 ? mk<double>(d0.re / d1.re, d0.im / d1.re) : mk<double>(0.0, 0.0);
+  }
+}
+
+// p = r (first step) or r + beta (p - omega v)
+__global__ __launch_bounds__(256) void k_bicg_p(KrylovWork kw, const cx<double>* __restrict__ sc,
+                                                const cx<double>* __restrict__ r, cx<double>* __restrict__ p,
+                                                const cx<double>* __restrict__ v, int first) {
+  const int c = blockIdx.x, b = blockIdx.y;
+  if (kw.done[b]) return;
+  int64_t p0, p1;
+  chunk_range(kw.N, kw.nchunk, c, &p0, &p1);
+  const int64_t base = (int64_t)b * kw.N;
+  const int n = kw.nrhs;
+  const cx<double> beta = sc[3 * n + b], om = sc[2 * n + b];
+  for (int64_t q = p0 + threadIdx.x; q < p1; q += 256) {
+    const int64_t o = base + q;
+    p[o] = first ? r[o] : r[o] + beta * (p[o] - om * v[o]);
+  }
+}
+
+// s = r - alpha v
+__global__ __launch_bounds__(256) void k_bicg_s(KrylovWork kw, const cx<double>* __restrict__ sc,
+                                                const cx<double>* __restrict__ r, const cx<double>* __restrict__ v,
+                                                cx<double>* __restrict__ s) {
+  const int c = blockIdx.x, b = blockIdx.y;
+  if (kw.done[b]) return;
+  int64_t p0, p1;
+  chunk_range(kw.N, kw.nchunk, c, &p0, &p1);
+  const int64_t base = (int64_t)b * kw.N;
+  const cx<double> al = sc[1 * kw.nrhs + b];
+  for (int64_t q = p0 + threadIdx.x; q < p1; q += 256) s[base + q] = r[base + q] - al * v[base + q];
+}
+
+// x += alpha p^ + omega s^;  r = s - omega t;  partn = ||r||^2
+template <typename TZ>
+__global__ __launch_bounds__(256) void k_bicg_xr(KrylovWork kw, const cx<double>* __restrict__ sc,
+                                                 const cx<TZ>* __restrict__ ph, const cx<TZ>* __restrict__ shat,
+                                                 const cx<double>* __restrict__ s, const cx<double>* __restrict__ t,
+                                                 cx<double>* __restrict__ x, cx<double>* __restrict__ r) {
+  const int c = blockIdx.x, b = blockIdx.y;
+  if (kw.done[b]) return;
+  __shared__ double sh[4];
+  int64_t p0, p1;
+  chunk_range(kw.N, kw.nchunk, c, &p0, &p1);
+  const int64_t base = (int64_t)b * kw.N;
+  const int n = kw.nrhs;
+  const cx<double> al = sc[1 * n + b], om = sc[2 * n + b];
+  double nn = 0.0;
+  for (int64_t q = p0 + threadIdx.x; q < p1; q += 256) {
+    const int64_t o = base + q;
+    const cx<double> pv = mk<double>((double)ph[o].re, (double)ph[o].im);
+    const cx<double> sv = mk<double>((double)shat[o].re, (double)shat[o].im);
+    x[o] = x[o] + al * pv + om * sv;
+    const cx<double> rv = s[o] - om * t[o];
+    r[o] = rv;
+    nn += cabs2(rv);
+  }
+  const double tt = block_sum(nn, sh);
+  if (threadIdx.x == 0) kw.partn[(int64_t)c * n + b] = tt;
+}
+
+// cycle start: done = conv, step counts cleared
+__global__ __launch_bounds__(64) void k_bicg_start(KrylovWork kw, const int* __restrict__ conv) {
+  const int b = blockIdx.x * 64 + threadIdx.x;
+  if (b >= kw.nrhs) return;
+  kw.done[b] = conv[b];
+  kw.kact[b] = 0;
+}
+
+void bicg_start(const KrylovWork& w, const int* conv, hipStream_t s) {
+  hipLaunchKernelGGL(k_bicg_start, dim3((w.nrhs + 63) / 64), dim3(64), 0, s, w, conv);
+}
+void bicg_dots(const KrylovWork& w, const cx<double>* a0, const cx<double>* b0, const cx<double>* a1,
+               const cx<double>* b1, int nd, hipStream_t s) {
+  hipLaunchKernelGGL(k_bdots, cgrid(w), dim3(256), 0, s, w, a0, b0, a1, b1, nd);
+}
+void bicg_scalar(const KrylovWork& w, cx<double>* sc, int stage, int first, double tol, int limit, hipStream_t s) {
+  hipLaunchKernelGGL(k_bicg_scalar, dim3(w.nrhs), dim3(64), 0, s, w, sc, stage, first, tol, limit);
+}
+void bicg_p(const KrylovWork& w, const cx<double>* sc, const cx<double>* r, cx<double>* p, const cx<double>* v,
+            int first, hipStream_t s) {
+  hipLaunchKernelGGL(k_bicg_p, cgrid(w), dim3(256), 0, s, w, sc, r, p, v, first);
+}
+void bicg_s(const KrylovWork& w, const cx<double>* sc, const cx<double>* r, const cx<double>* v, cx<double>* sv,
+            hipStream_t s) {
+  hipLaunchKernelGGL(k_bicg_s, cgrid(w), dim3(256), 0, s, w, sc, r, v, sv);
+}
+template <typename TZ>
+void bicg_xr(const KrylovWork& w, const cx<double>* sc, const cx<TZ>* ph, const cx<TZ>* shat, const cx<double>* sv,
+             const cx<double>* t, cx<double>* x, cx<double>* r, hipStream_t s) {
+  hipLaunchKernelGGL((k_bicg_xr<TZ>), cgrid(w), dim3(256), 0, s, w, sc, ph, shat, sv, t, x, r);
+}
+template void bicg_xr<float>(const KrylovWork&, const cx<double>*, const cx<float>*, const cx<float>*,
+                             const cx<double>*, const cx<double>*, cx<double>*, cx<double>*, hipStream_t);
+template void bicg_xr<double>(const KrylovWork&, const cx<double>*, const cx<double>*, const cx<double>*,
+                              const cx<double>*, const cx<double>*, cx<double>*, cx<double>*, hipStream_t);
+
+// ------------------------------------------------------------ adjoint solves
+// With mirror ghosts the Laplacian is L = W^-1 S (S symmetric, W the nodal
+// trapezoid weights: 1/2 per boundary axis), so H^H = W conj(H) W^-1 and
+// H^H x = b  <=>  x = W conj(y),  H y = W^-1 conj(b).  With BC = 1, W = I.
+__device__ __forceinline__ double trap_weight(const Grid& g, int64_t p) {
+  if (g.bc != 2) return 1.0;
+  const int i = (int)(p % g.n1);
+  const int64_t q = p / g.n1;
+  const int j = (int)(q % g.n2);
+  const int k = (int)(q / g.n2);
+  double w = (i == 0 || i == g.n1 - 1) ? 0.5 : 1.0;
+  w *= (j == 0 || j == g.n2 - 1) ? 0.5 : 1.0;
+  if (g.dim == 3) w *= (k == 0 || k == g.n3 - 1) ? 0.5 : 1.0;
+  return w;
+}
+
+// mode 0: v = W^-1 conj(v);  mode 1: v = W conj(v);  mode 2: v = W v
+__global__ __launch_bounds__(256) void k_adjoint_map(Grid g, cx<double>* __restrict__ v, int nrhs, int mode) {
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= g.N) return;
+  const double w = trap_weight(g, p);
+  const double f = mode == 0 ? 1.0 / w : w;
+  for (int b = 0; b < nrhs; ++b) {
+    const cx<double> a = v[(int64_t)b * g.N + p];
+    v[(int64_t)b * g.N + p] = mk<double>(f * a.re, mode == 2 ? f * a.im : -f * a.im);
+  }
+}
+
+void adjoint_map(const Grid& g, cx<double>* v, int nrhs, int mode, hipStream_t s) {
+  hipLaunchKernelGGL(k_adjoint_map, dim3((unsigned)((g.N + 255) / 256)), dim3(256), 0, s, g, v, nrhs, mode);
+}
+
 }  // namespace hh

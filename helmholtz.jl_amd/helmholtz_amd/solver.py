@@ -124,8 +124,8 @@ def clear(s: ShiftedLaplacianMultigridSolver):
 
 def _validate(s: ShiftedLaplacianMultigridSolver):
     mg = s.MG
-    if s.Krylov != "GMRES":
-        raise NotImplementedError(f"Krylov={s.Krylov!r}: this path implements the FGMRES outer method ('GMRES')")
+    if s.Krylov not in ("GMRES", "BiCGSTAB"):
+        raise ValueError(f"Krylov={s.Krylov!r}: 'GMRES' (flexible GMRES) or 'BiCGSTAB'")
     if mg.relaxType != "Jac":
         raise NotImplementedError(f"relaxType={mg.relaxType!r}: damped Jacobi ('Jac') only")
     if str(mg.cycleType) not in ("V", "W"):
@@ -147,7 +147,8 @@ def _ensure_handle(s: ShiftedLaplacianMultigridSolver):
         mg = s.MG
         prm = L.MGParams(levels=int(mg.levels), pre=int(mg.relaxPre), post=int(mg.relaxPost),
                          relax=float(mg.relaxParam), cycle=ord(str(mg.cycleType)), max_outer=int(mg.maxOuterIter),
-                         tol=float(mg.relativeTol), inner=int(s.inner), krylov=L.HH_FGMRES,
+                         tol=float(mg.relativeTol), inner=int(s.inner),
+                         krylov=L.HH_BICGSTAB if s.Krylov == "BiCGSTAB" else L.HH_FGMRES,
                          prec=L.HH_C64 if mg.prec == "c64" else L.HH_C128, learned=0,
                          verbose=int(bool(s.verbose)), graph=int(bool(mg.graph)),
                          autotune=int(bool(mg.autotune)), fused=int(bool(mg.fused)),
@@ -169,10 +170,11 @@ def solveLinearSystem(ShiftedHT, B, param: ShiftedLaplacianMultigridSolver, doTr
     B: numpy (N,) or (N, nrhs) complex, or a complex128 DeviceArray (then X is
     a DeviceArray too and nothing crosses PCIe).  An all-zero host B returns
     zeros without touching the GPU (src/ShiftedLaplacianMultigridSolver.jl:40-43);
-    for device blocks the library reports zero columns per RHS."""
+    for device blocks the library reports zero columns per RHS.  doTranspose=1
+    solves the adjoint system H^H X = B (:68-80)."""
     _validate(param)
-    if doTranspose != 0:
-        raise NotImplementedError("doTranspose=1 (adjoint solve) is not implemented")
+    if doTranspose not in (0, 1):
+        raise ValueError("doTranspose must be 0 or 1")
     if param.doClear == 1:
         clear(param)
     N = int(np.prod(param.helmParam.Mesh.nodes))
@@ -200,7 +202,7 @@ def solveLinearSystem(ShiftedHT, B, param: ShiftedLaplacianMultigridSolver, doTr
     res = np.zeros(nrhs, dtype=np.float64)
     st = L.SolveStats(iters.ctypes.data_as(C.POINTER(C.c_int32)), status.ctypes.data_as(C.POINTER(C.c_int32)),
                       res.ctypes.data_as(C.POINTER(C.c_double)), 0, 0, 0.0, 0.0, 0, 0, 0)
-    L.check(L.lib().hh_solve(param._handle, bptr, xptr, nrhs, 0, loc, C.byref(st)))
+    L.check(L.lib().hh_solve(param._handle, bptr, xptr, nrhs, int(doTranspose), loc, C.byref(st)))
     param.MG._hierarchy_exists = True
     param.setupTime += st.setup_time
     param.solveTime += st.solve_time

@@ -16,6 +16,7 @@ const LIB = joinpath(@__DIR__, "..", "libhelmholtz_amd.so")
 const HH_C128 = Cint(0); const HH_C64 = Cint(1)
 const HH_HOST = Cint(0); const HH_DEVICE = Cint(1)
 const HH_FGMRES = Cint(0)
+const HH_BICGSTAB = Cint(1)
 
 struct HHMGParams
     levels::Int32; pre::Int32; post::Int32; relax::Float64; cycle::Int32; max_outer::Int32
@@ -88,7 +89,8 @@ function ensure_handle!(s::ShiftedLaplacianMultigridSolverAMD)
     if s.handle == C_NULL
         mg = s.MG
         prm = Ref(HHMGParams(mg.levels, mg.relaxPre, mg.relaxPost, mg.relaxParam, Int32(mg.cycleType),
-                             mg.maxOuterIter, mg.relativeTol, s.inner, HH_FGMRES, s.prec, 0, s.verbose, 0, 0, 1, 0))
+                             mg.maxOuterIter, mg.relativeTol, s.inner,
+                             s.Krylov == "BiCGSTAB" ? HH_BICGSTAB : HH_FGMRES, s.prec, 0, s.verbose, 0, 0, 1, 0))
         out = Ref{Ptr{Cvoid}}(C_NULL)
         check(ccall((:hh_solver_create, LIB), Cint, (Ptr{Cvoid}, Ref{HHMGParams}, Ptr{Float64}, Ptr{Ptr{Cvoid}}),
                     s.op.ptr, prm, s.shift, out))
@@ -98,8 +100,9 @@ function ensure_handle!(s::ShiftedLaplacianMultigridSolverAMD)
 end
 
 # solveLinearSystem(ShiftedHT, B, param, doTranspose) (src/ShiftedLaplacianMultigridSolver.jl:33-102)
+# doTranspose = 1 solves the adjoint system H^H X = B (:68-80)
 function solveLinearSystem(ShiftedHT, B, param::ShiftedLaplacianMultigridSolverAMD, doTranspose::Int64=0)
-    doTranspose == 0 || error("doTranspose = 1 is not implemented by libhelmholtz_amd")
+    doTranspose in (0, 1) || error("doTranspose must be 0 or 1")
     if param.doClear == 1
         clear!(param)
     end
@@ -115,7 +118,7 @@ function solveLinearSystem(ShiftedHT, B, param::ShiftedLaplacianMultigridSolverA
         st = Ref(HHSolveStats(pointer(iters), pointer(status), pointer(res), 0, 0, 0.0, 0.0, 0, 0, 0))
         check(ccall((:hh_solve, LIB), Cint,
                     (Ptr{Cvoid}, Ptr{ComplexF64}, Ptr{ComplexF64}, Int64, Cint, Cint, Ref{HHSolveStats}),
-                    param.handle, Bc, X, nrhs, 0, HH_HOST, st))
+                    param.handle, Bc, X, nrhs, doTranspose, HH_HOST, st))
         param.setupTime += st[].setup_time
         param.solveTime += st[].solve_time
         param.nPrec += st[].n_prec

@@ -53,7 +53,7 @@ typedef struct hh_mg_params {
   int32_t max_outer;  /* MG.maxOuterIter: FGMRES restarts                        */
   double tol;         /* MG.relativeTol: ||b - H x|| <= tol ||b||                */
   int32_t inner;      /* solver.inner: FGMRES restart length (1..32)             */
-  int32_t krylov;     /* HH_FGMRES                                               */
+  int32_t krylov;     /* HH_FGMRES (inner = restart length) or HH_BICGSTAB       */
   int32_t prec;       /* preconditioner precision: HH_C128 or HH_C64             */
   int32_t learned;    /* reserved: 0 (attach a network with hh_solver_attach_unet) */
   int32_t verbose;    /* solver.verbose                                          */
@@ -127,7 +127,12 @@ int hh_solver_destroy(hh_solver* s);
  * (src/ShiftedLaplacianMultigridSolver.jl:33-102): X = H^-1 B column by
  * column to ||b - H x|| <= tol ||b|| with FGMRES(inner) preconditioned by one
  * shifted-Laplacian MG cycle per iteration.  B and X are complex128, N x nrhs.
- * An all-zero B returns X = 0 (:40-43).  do_transpose must be 0.
+ * An all-zero B returns X = 0 (:40-43).  krylov = HH_BICGSTAB runs
+ * right-preconditioned BiCGSTAB instead (:91-93; two cycles per iteration, at
+ * most max_outer iterations).  do_transpose = 1 solves the adjoint system
+ * H^H X = B (:68-80), through the mirror-ghost similarity
+ * H^H = W conj(H) W^-1 (W: nodal trapezoid weights) with the same hierarchy;
+ * stats then report the adjoint system's true residuals.
  * stats may be NULL. */
 int hh_solve(hh_solver* s, const void* B, void* X, int64_t nrhs, int do_transpose, int loc, hh_solve_stats* stats);
 

@@ -201,6 +201,63 @@ def fgmres(Afun, b, M, inner=10, max_outer=50, tol=1e-6, x0=None):
     return x, total, hist
 
 
+def bicgstab(Afun, b, M, max_iter=50, tol=1e-6):
+    """Right-preconditioned BiCGSTAB (van der Vorst, SIAM J. Sci. Stat.
+    Comput. 13, 1992), x0 = 0, shadow residual r^ = r0, stopping on the
+    recursive residual ||r|| <= tol ||b||; the outer method behind
+    ``Krylov="BiCGSTAB"`` (src/ShiftedLaplacianMultigridSolver.jl:91-93).
+    Returns (x, iterations, relative-residual history)."""
+    n = b.shape[0]
+    x = np.zeros(n, dtype=complex)
+    bn = np.linalg.norm(b)
+    if bn == 0.0:
+        return x, 0, [0.0]
+    r = b.astype(complex).copy()
+    rh = r.copy()
+    p = np.zeros(n, dtype=complex)
+    v = np.zeros(n, dtype=complex)
+    rho = alpha = omega = 1.0 + 0j
+    hist = [1.0]
+    for it in range(max_iter):
+        rho_new = np.vdot(rh, r)
+        if rho_new == 0:
+            return x, it, hist
+        if it == 0:
+            p = r.copy()
+        else:
+            p = r + (rho_new / rho) * (alpha / omega) * (p - omega * v)
+        rho = rho_new
+        ph = M(p)
+        v = Afun(ph)
+        d = np.vdot(rh, v)
+        if d == 0:
+            return x, it, hist
+        alpha = rho / d
+        s = r - alpha * v
+        sh = M(s)
+        t = Afun(sh)
+        tt = np.vdot(t, t).real
+        omega = np.vdot(t, s) / tt if tt > 0 else 0.0
+        x = x + alpha * ph + omega * sh
+        r = s - omega * t
+        hist.append(np.linalg.norm(r) / bn)
+        if np.linalg.norm(r) <= tol * bn:
+            return x, it + 1, hist
+    return x, max_iter, hist
+
+
+def trapezoid_weights(mesh: hr.RegularMesh, order_neumann_bc: int = 2) -> np.ndarray:
+    """Nodal weights W (1/2 per boundary axis) with W L symmetric for the
+    mirror-ghost Laplacian (BC = 2); all ones for BC = 1, whose L is symmetric."""
+    w = np.ones(1)
+    for n in mesh.nodes:  # axis 0 fastest: kron(w_{d}, ..., w_0)
+        wd = np.ones(int(n))
+        if order_neumann_bc == 2:
+            wd[0] = wd[-1] = 0.5
+        w = np.kron(wd, w)
+    return w
+
+
 def solve_linear_system(param: hr.HelmholtzParam, B, levels=3, shift=0.5, inner=10, max_outer=50,
                         tol=1e-6, relax=0.8, pre=2, post=2, cycle="V", order_neumann_bc=2):
     """Plain SL-MG FGMRES solve of H X = B (column by column), returning (X, iterations)."""

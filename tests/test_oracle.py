@@ -17,6 +17,7 @@ import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 
 from oracle import helmholtz_ref as hr, slmg_ref as sr
+from tests.problems import small_case
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
@@ -335,3 +336,36 @@ def test_golden_fixtures_unchanged():
         m, g, w = f[key + "__m"], f[key + "__gamma"], complex(f[key + "__omega"])
         H = hr.get_helmholtz_operator7(M, m, w, g, neumann, somm, order)
         np.testing.assert_array_equal(H @ f[key + "__x"], f[key + "__y"])
+
+
+def test_bicgstab_exact_preconditioner_one_iteration():
+    M, m, g, w = small_case((17, 9), seed=5)
+    H = hr.get_helmholtz_operator7(M, m, w, g, True, True).tocsc()
+    lu = spla.splu(H)
+    b = np.zeros(m.size, dtype=complex)
+    b[20] = 1.0
+    x, it, hist = sr.bicgstab(lambda v: H @ v, b, lu.solve, 10, 1e-10)
+    assert it == 1
+    assert np.linalg.norm(H @ x - b) <= 1e-12 * np.linalg.norm(b)
+
+
+def test_bicgstab_identity_preconditioner_matches_direct():
+    rng = np.random.default_rng(3)
+    n = 40
+    A = np.eye(n) * 4 + rng.standard_normal((n, n)) * 0.3 + 1j * rng.standard_normal((n, n)) * 0.3
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    x, it, hist = sr.bicgstab(lambda v: A @ v, b, lambda v: v, 200, 1e-12)
+    assert np.linalg.norm(x - np.linalg.solve(A, b)) <= 1e-10 * np.linalg.norm(x)
+    assert hist[-1] <= 1e-12
+
+
+@pytest.mark.parametrize("nodes,order", [((9, 7), 2), ((9, 7), 1), ((5, 6, 4), 2), ((5, 6, 4), 1)])
+def test_adjoint_similarity_with_trapezoid_weights(nodes, order):
+    """H^H = W conj(H) W^-1 on the assembled operator: the identity the
+    adjoint (doTranspose) solve is built on."""
+    M, m, g, w = small_case(nodes, seed=8)
+    H = hr.get_helmholtz_operator7(M, m, w, g, True, True, order).toarray()
+    W = np.diag(sr.trapezoid_weights(M, order))
+    assert np.allclose(H.conj().T, W @ H.conj() @ np.linalg.inv(W), rtol=0, atol=1e-12 * np.abs(H).max())
+    if order == 2:
+        assert not np.allclose(H.T, H)  # the identity is not a triviality
