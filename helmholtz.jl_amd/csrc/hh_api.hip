@@ -271,7 +271,7 @@ struct hh_solver {
     V = nullptr; Z = nullptr; xk = bk = wk = nullptr; b0 = nullptr;
     dfree(kw.part); dfree(kw.partn); dfree(kw.h1); dfree(kw.h2); dfree(kw.nrm); dfree(kw.H); dfree(kw.cs);
     dfree(kw.sn); dfree(kw.gv); dfree(kw.y); dfree(kw.bnorm); dfree(kw.est); dfree(kw.done); dfree(kw.kact);
-    dfree(kw.vscale); dfree(kw.fix); dfree(kw.wn2); dfree(bsc);
+    dfree(kw.vscale); dfree(kw.fix); dfree(kw.wn2); dfree(kw.h1s); dfree(bsc);
     bsc = nullptr;
     kw = KrylovWork{};
     dfree(skip_copy); dfree(conv_d); dfree(n2tmp); dfree(res_d);
@@ -437,6 +437,7 @@ void ensure_work(hh_solver* S, int nrhs) {
   w.vscale = dalloc<double>((size_t)(K + 1) * nrhs);
   w.fix = dalloc<int>(nrhs);
   w.wn2 = dalloc<double>(nrhs);
+  w.h1s = dalloc<cx<double>>((size_t)(K + 1) * nrhs);
   S->bsc = dalloc<cx<double>>((size_t)4 * nrhs);
   S->skip_copy = dalloc<int>(nrhs);
   S->conv_d = dalloc<int>(nrhs);

@@ -130,6 +130,7 @@ struct KrylovWork {
   double* vscale;        // [kmax+1][nrhs]
   int* fix;              // [nrhs] 1 = the second CGS correction is applied this step
   double* wn2;           // [nrhs] ||w||^2 before projection (lazy mode)
+  cx<double>* h1s;       // [kmax+1][nrhs] vscale[i] * h1[i] (lazy mode: coefficients of the stored vectors)
 };
 // b0 = (complex64) vscale_j[b] * V_j  (preconditioner input in the lazy mode)
 void kr_cast_scaled(const KrylovWork& w, const cx<double>* Vj, const double* scale, cx<float>* b0, hipStream_t s);

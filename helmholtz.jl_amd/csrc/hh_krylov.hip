@@ -103,10 +103,17 @@ __global__ __launch_bounds__(64) void k_reduce_dots(KrylovWork kw, cx<double>* _
   }
   re = wave_sum(re);
   im = wave_sum(im);
-  if (threadIdx.x == 0) h[i * kw.nrhs + b] = mk<double>(re, im);
+  if (threadIdx.x == 0) {
+    h[i * kw.nrhs + b] = mk<double>(re, im);
+    if (kw.lazy) {
+      const double si = kw.vscale[i * kw.nrhs + b];
+      kw.h1s[i * kw.nrhs + b] = mk<double>(si * re, si * im);
+    }
+  }
 }
 
-// pass 2: w' = w - V h1 ; part = V^H w' ; partn = ||w'||^2
+// pass 2 (complex128 mode, normalised basis): w' = w - V h1 ; part = V^H w' ;
+// partn = ||w'||^2.  h1 is read with uniform (scalar) loads.
 template <int NV>
 __global__ __launch_bounds__(256) void k_update_dots(KrylovWork kw, const cx<double>* __restrict__ V,
                                                      const cx<double>* w, cx<double>* wout, int j) {
@@ -117,14 +124,11 @@ __global__ __launch_bounds__(256) void k_update_dots(KrylovWork kw, const cx<dou
   chunk_range(kw.N, kw.nchunk, c, &p0, &p1);
   const int64_t stride = kw.N * kw.nrhs;
   const int64_t base = (int64_t)b * kw.N;
-  cx<double> hc[NV];
+  const cx<double>* __restrict__ hc = kw.h1 + b;
+  const int hstride = kw.nrhs;
   cx<double> acc[NV];
 #pragma unroll
-  for (int i = 0; i < NV; ++i) {
-    const double si = (kw.lazy && i <= j) ? kw.vscale[i * kw.nrhs + b] : 1.0;
-    hc[i] = i <= j ? si * kw.h1[i * kw.nrhs + b] : mk<double>(0.0, 0.0);
-    acc[i] = mk<double>(0.0, 0.0);
-  }
+  for (int i = 0; i < NV; ++i) acc[i] = mk<double>(0.0, 0.0);
   double nn = 0.0;
   for (int64_t p = p0 + threadIdx.x; p < p1; p += 256) {
     cx<double> vv[NV];
@@ -133,7 +137,7 @@ __global__ __launch_bounds__(256) void k_update_dots(KrylovWork kw, const cx<dou
     for (int i = 0; i < NV; ++i)
       if (i <= j) {
         vv[i] = V[i * stride + base + p];
-        wv = wv - hc[i] * vv[i];
+        wv = wv - hc[i * hstride] * vv[i];
       }
     wout[base + p] = wv;
     nn += cabs2(wv);
@@ -144,9 +148,8 @@ __global__ __launch_bounds__(256) void k_update_dots(KrylovWork kw, const cx<dou
 #pragma unroll
   for (int i = 0; i < NV; ++i) {
     if (i > j) break;
-    const double si = kw.lazy ? kw.vscale[i * kw.nrhs + b] : 1.0;
-    const double re = block_sum(si * acc[i].re, sh);
-    const double im = block_sum(si * acc[i].im, sh);
+    const double re = block_sum(acc[i].re, sh);
+    const double im = block_sum(acc[i].im, sh);
     if (threadIdx.x == 0) kw.part[((int64_t)c * (kw.kmax + 1) + i) * kw.nrhs + b] = mk<double>(re, im);
   }
   const double t = block_sum(nn, sh);
@@ -162,10 +165,7 @@ __global__ __launch_bounds__(256) void k_update_lazy(KrylovWork kw, const cx<dou
   if (kw.done[b]) return;
   __shared__ double sh[4];
   __shared__ cx<double> hs[NV];
-  if (threadIdx.x < NV) {
-    const int i = threadIdx.x;
-    hs[i] = i <= j ? kw.vscale[i * kw.nrhs + b] * kw.h1[i * kw.nrhs + b] : mk<double>(0.0, 0.0);
-  }
+  if (threadIdx.x < NV) hs[threadIdx.x] = threadIdx.x <= j ? kw.h1s[threadIdx.x * kw.nrhs + b] : mk<double>(0.0, 0.0);
   __syncthreads();
   int64_t p0, p1;
   chunk_range(kw.N, kw.nchunk, c, &p0, &p1);
