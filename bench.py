@@ -171,8 +171,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=40.0,
                     help="seconds the host baseline may spend on a full single-source solve")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--train-media", type=int, default=8)
-    ap.add_argument("--train-epochs", type=int, default=8)
+    ap.add_argument("--train-media", type=int, default=40)
+    ap.add_argument("--train-epochs", type=int, default=6)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the solve shards by right-hand side "
