@@ -207,6 +207,7 @@ struct hh_solver {
   int64_t* piv = nullptr;
   cx<double>* fac = nullptr;
   cx<double>* pinv = nullptr;
+  float* coarse_pack = nullptr;      // fp32 real-ified inverse for the MFMA apply (complex64 mode)
   int cap = 0;  // RHS capacity of the work buffers
   // Krylov storage
   cx<double>* V = nullptr;  // (inner+1) blocks
@@ -286,8 +287,8 @@ struct hh_solver {
     free_work();
     for (auto& L : lv) { dfree(L.m); dfree(L.gam); dfree(L.d); dfree(L.f); }
     lv.clear();
-    dfree(coarse_aug); dfree(coarse_d); dfree(piv); dfree(fac); dfree(pinv);
-    coarse_aug = nullptr; coarse_d = nullptr; piv = nullptr; fac = nullptr; pinv = nullptr;
+    dfree(coarse_aug); dfree(coarse_d); dfree(piv); dfree(fac); dfree(pinv); dfree(coarse_pack);
+    coarse_aug = nullptr; coarse_d = nullptr; piv = nullptr; fac = nullptr; pinv = nullptr; coarse_pack = nullptr;
     built = false;
     learn_verdict = -1;
   }
@@ -366,6 +367,11 @@ void build_hierarchy(hh_solver* S) {
   S->fac = dalloc<cx<double>>(n);
   S->pinv = dalloc<cx<double>>(1);
   launch_dense_inverse(c.g, S->coarse_d, S->coarse_aug, S->piv, S->fac, S->pinv, st);
+  if (S->p.prec == HH_C64) {
+    // complex64 preconditioner: the coarsest apply runs on MFMA from an fp32 copy
+    S->coarse_pack = dalloc<float>((size_t)pack_inv_mfma_size(c.g.N));
+    pack_inv_mfma(S->coarse_aug, c.g.N, S->coarse_pack, st);
+  }
   HIPCK(hipGetLastError());
   HIPCK(hipStreamSynchronize(st));
   S->built = true;
@@ -388,6 +394,15 @@ void setup_learned(hh_solver* S) {
 
 // row-major A^-1 of the coarsest level (left half of the Gauss-Jordan workspace)
 const cx<double>* coarse_inv(hh_solver* S) { return S->coarse_aug; }
+
+// coarsest-level solve x = A^-1 r (MFMA in the complex64 mode)
+template <typename T>
+void coarse_apply(hh_solver* S, int64_t n, int nrhs, const cx<T>* r, cx<T>* x, const int* skip, hipStream_t st) {
+  if (std::is_same<T, float>::value && S->coarse_pack)
+    launch_dense_apply_mfma(S->coarse_pack, n, nrhs, (const cx<float>*)r, (cx<float>*)x, skip, st);
+  else
+    launch_dense_apply<T>(coarse_inv(S), n, nrhs, r, x, skip, st);
+}
 
 void ensure_work(hh_solver* S, int nrhs) {
   if (nrhs <= S->cap) return;
@@ -573,7 +588,7 @@ struct Cycle {
     }
     if (l == nl - 1) {
       if (!L.cur) L.cur = first ? first : L.xa;
-      launch_dense_apply<T>(coarse_inv(S), L.g.N, nrhs, b, (cx<T>*)L.cur, skip, st);
+      coarse_apply<T>(S, L.g.N, nrhs, b, (cx<T>*)L.cur, skip, st);
       return;
     }
     const int pre = S->p.pre, post = S->p.post;
@@ -642,7 +657,7 @@ struct Cycle {
     const int nl = (int)S->lv.size();
     if (nl == 1) {
       L0.cur = out;
-      launch_dense_apply<T>(coarse_inv(S), L0.g.N, nrhs, b, out, skip, st);
+      coarse_apply<T>(S, L0.g.N, nrhs, b, out, skip, st);
       return;
     }
     // number of buffer writes on level 0: pre (incl. the zero-start sweep) + post,

@@ -42,6 +42,11 @@ void launch_dense_inverse(const Grid& g, const cx<double>* d, cx<double>* M, int
 template <typename T>
 void launch_dense_apply(const cx<double>* Ainv, int64_t n, int nrhs, const cx<T>* r, cx<T>* x, const int* skip,
                         hipStream_t s);
+// complex64 coarsest apply on MFMA: P = the inverse packed by pack_inv_mfma (fp32, pack_inv_mfma_size(n) floats)
+void pack_inv_mfma(const cx<double>* Ainv, int64_t n, float* P, hipStream_t s);
+int64_t pack_inv_mfma_size(int64_t n);
+void launch_dense_apply_mfma(const float* P, int64_t n, int nrhs, const cx<float>* r, cx<float>* x, const int* skip,
+                             hipStream_t s);
 
 // ---- fused 2-D multigrid legs (hh_fused.hip): BC = 2, 2 pre/2 post sweeps ----
 bool fused_ok(const Grid& gf);

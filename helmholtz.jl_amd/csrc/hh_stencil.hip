@@ -495,7 +495,104 @@ __global__ __launch_bounds__(256) void k_dense_apply(const cx<double>* __restric
     }
 }
 
+// ---- coarsest apply on MFMA (complex64 preconditioner) ----
+// The complex product x = A^-1 r is one real GEMM with the real-ified matrix
+// A_R (2n x 2n; entry (i, j) of A^-1 becomes [[re, -im], [im, re]] at rows
+// 2i, 2i+1 and columns 2j, 2j+1) and r read as a real 2n x nrhs matrix, which
+// the interleaved complex64 block already is.  K runs in groups of 16; within
+// group g, step u (0..3) of v_mfma_f32_16x16x4_f32 gives lane l the column
+// k = 16g + 4(l/16) + u (a permutation of the group, the same for A and r), so
+// a lane reads its four A entries as one packed float4 and its four r entries
+// as two aligned float2.  A_R is packed once, in fp32: block (R, g), lane l
+// holds row 16R + l % 16, columns 16g + 4(l/16) + 0..3 (zero beyond 2n).
+__global__ __launch_bounds__(256) void k_pack_inv_mfma(const cx<double>* __restrict__ Arow, int n, int ng, int nrb,
+                                                       float4* __restrict__ P) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= (int64_t)nrb * ng * 64) return;
+  const int lane = (int)(t & 63);
+  const int64_t blk = t >> 6;
+  const int g = (int)(blk % ng), R = (int)(blk / ng);
+  const int row = R * 16 + (lane & 15), col0 = g * 16 + 4 * (lane >> 4);
+  float v[4];
+  for (int u = 0; u < 4; ++u) {
+    const int col = col0 + u;
+    v[u] = 0.f;
+    if (row < 2 * n && col < 2 * n) {
+      const cx<double> e = Arow[(int64_t)(row >> 1) * n + (col >> 1)];
+      const int ra = row & 1, cb = col & 1;
+      v[u] = (float)(ra == cb ? e.re : (ra == 0 ? -e.im : e.im));
+    }
+  }
+  P[t] = make_float4(v[0], v[1], v[2], v[3]);
+}
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// one block per 16 x 16 tile of the real-ified result (16 real rows, 16 RHS);
+// its four wavefronts take every fourth K-group and their partial tiles are
+// summed through LDS in a fixed order (deterministic)
+__global__ __launch_bounds__(256) void k_dense_apply_mfma(const float4* __restrict__ P, int n2, int ng, int nrhs,
+                                                          const float* __restrict__ r, float* __restrict__ x,
+                                                          const int* __restrict__ skip) {
+  __shared__ f32x4 part[3][64];
+  const int lane = threadIdx.x & 63, wv = (int)(threadIdx.x >> 6);
+  const int R = blockIdx.x, C = blockIdx.y;
+  int live = 0;
+  for (int q = 0; q < 16; ++q) {
+    const int cq = C * 16 + q;
+    live |= cq < nrhs && !(skip && skip[cq]);
+  }
+  if (!live) return;  // whole block: every RHS of the tile is done
+  const int col = C * 16 + (lane & 15);
+  const bool colok = col < nrhs;
+  const int kl = lane >> 4;
+  const float4* __restrict__ Pb = P + (int64_t)R * ng * 64 + lane;
+  // n2 is even, so every column starts 8-byte aligned
+  const float2* __restrict__ rc = (const float2*)(r + (int64_t)(colok ? col : 0) * n2);
+  f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+  for (int g = wv; g < ng; g += 4) {
+    const float4 av = Pb[(int64_t)g * 64];
+    const int k = g * 16 + 4 * kl;
+    const float2 z2 = make_float2(0.f, 0.f);
+    const float2 b01 = (colok && k < n2) ? rc[k >> 1] : z2;
+    const float2 b23 = (colok && k + 2 < n2) ? rc[(k >> 1) + 1] : z2;
+    acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(av.x, b01.x, acc0, 0, 0, 0);
+    acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(av.y, b01.y, acc1, 0, 0, 0);
+    acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(av.z, b23.x, acc0, 0, 0, 0);
+    acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(av.w, b23.y, acc1, 0, 0, 0);
+  }
+  const f32x4 acc = acc0 + acc1;
+  if (wv > 0) part[wv - 1][lane] = acc;
+  __syncthreads();
+  if (wv != 0) return;
+  const f32x4 sum = ((acc + part[0][lane]) + part[1][lane]) + part[2][lane];
+  if (colok && !(skip && skip[col])) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int row = R * 16 + 4 * kl + i;
+      if (row < n2) x[(int64_t)col * n2 + row] = sum[i];
+    }
+  }
+}
+
 // ---------------------------------------------------------------- launchers
+void pack_inv_mfma(const cx<double>* Ainv, int64_t n, float* P, hipStream_t s) {
+  const int n2 = (int)(2 * n), ng = (n2 + 15) / 16, nrb = (n2 + 15) / 16;
+  const int64_t total = (int64_t)nrb * ng * 64;
+  hipLaunchKernelGGL(k_pack_inv_mfma, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, Ainv, (int)n, ng, nrb,
+                     (float4*)P);
+}
+int64_t pack_inv_mfma_size(int64_t n) {
+  const int64_t b = (2 * n + 15) / 16;
+  return b * b * 64 * 4;
+}
+void launch_dense_apply_mfma(const float* P, int64_t n, int nrhs, const cx<float>* r, cx<float>* x, const int* skip,
+                             hipStream_t s) {
+  const int n2 = (int)(2 * n), ng = (n2 + 15) / 16, nrb = (n2 + 15) / 16;
+  dim3 grid((unsigned)nrb, (unsigned)((nrhs + 15) / 16));
+  hipLaunchKernelGGL(k_dense_apply_mfma, grid, dim3(256), 0, s, (const float4*)P, n2, ng, nrhs, (const float*)r,
+                     (float*)x, skip);
+}
 static inline dim3 stencil_grid(const Grid& g, int nrhs, int rows, int* ytiles) {
   if (g.dim == 2) {
     *ytiles = 1;

@@ -415,3 +415,26 @@ def test_adjoint_solve_3d():
     s = _solver(M, m, g, w, True, 2, 0.5, cycle="V")
     x, s = ha.solveLinearSystem(None, b, s, 1)
     assert np.linalg.norm(H.conj().T @ x - b) / np.linalg.norm(b) <= 1e-6
+
+
+@pytest.mark.parametrize("nodes", [(17, 9), (33, 17), (65, 33)])
+def test_mfma_coarsest_apply_matches_double(nodes):
+    """The complex64 coarsest solve runs on MFMA (fp32 real-ified inverse):
+    with a single level the preconditioner is that solve alone, and it must
+    equal the complex128 dense inverse to fp32 accuracy, and the exact solve
+    of the shifted operator (SciPy) likewise.  20 RHS: a partial 16-column tile."""
+    M, m, g, w = small_case(nodes, seed=37, neumann=True)
+    rng = np.random.default_rng(13)
+    V = np.asfortranarray(rng.standard_normal((m.size, 20)) + 1j * rng.standard_normal((m.size, 20)))
+    z = {}
+    for prec in ("c64", "c128"):
+        s = _solver(M, m, g, w, True, 1, 0.5, prec=prec)
+        z[prec] = ha.precondition(s, V).astype(np.complex128)
+    err = np.linalg.norm(z["c64"] - z["c128"]) / np.linalg.norm(z["c128"])
+    p = hr.HelmholtzParam(M, g, m, w, True, True)
+    A = sr.build_hierarchy(p, 1, 0.5)[0].A.tocsc()
+    xd = spla.splu(A).solve(V)
+    err_d = np.linalg.norm(z["c128"] - xd) / np.linalg.norm(xd)
+    print(f"coarsest apply {nodes}: complex64 (MFMA) vs complex128 {err:.2e}; complex128 vs SciPy {err_d:.2e}")
+    assert err_d <= 1e-12
+    assert err <= 1e-5
