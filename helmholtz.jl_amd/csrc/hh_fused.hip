@@ -42,7 +42,7 @@ struct Tile {
   static constexpr int NPIX = W * HT;
 };
 
-// Each thread owns the tile positions q = threadIdx.x + 256 k (k < K) in every
+// Each thread owns the tile positions q = threadIdx.x + NT k (k < K) in every
 // phase, so b and d live in its registers; only the x fields (and the
 // residual) are staged in LDS, where neighbours are read.  Two LDS arrays per
 // block keep 6-7 blocks resident per CU to overlap one block's loads with
@@ -51,13 +51,13 @@ struct Tile {
 // load b and d at the owned positions of a tile whose first node is
 // (fx0 - H, fy0 - H)
 // (and the Jacobi factor f when given)
-template <typename T, int W, int NPIX, int K, int H, bool WITH_F>
+template <typename T, int NT, int W, int NPIX, int K, int H, bool WITH_F>
 __device__ __forceinline__ void load_bd(const cx<T>* __restrict__ bb, const cx<T>* __restrict__ d,
                                         const cx<T>* __restrict__ f, int fx0, int fy0, int n1, int n2,
                                         cx<T> (&rb)[K], cx<T> (&rd)[K], cx<T> (&rf)[K]) {
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    const int q = threadIdx.x + 256 * k;
+    const int q = threadIdx.x + NT * k;
     if (q < NPIX) {
       const int li = q % W, lj = q / W;
       const int64_t p = (int64_t)mirror_idx(fy0 - H + lj, n2) * n1 + mirror_idx(fx0 - H + li, n1);
@@ -69,14 +69,14 @@ __device__ __forceinline__ void load_bd(const cx<T>* __restrict__ bb, const cx<T
 }
 
 // ------------------------------------------------------------------ down leg
-template <typename T, int TY>
-__global__ __launch_bounds__(256) void k_down_fused(TileMap tm, Grid gf, Grid gc, const cx<T>* __restrict__ b,
+template <typename T, int TY, int NT>
+__global__ __launch_bounds__(NT) void k_down_fused(TileMap tm, Grid gf, Grid gc, const cx<T>* __restrict__ b,
                                                     const cx<T>* __restrict__ d, const cx<T>* __restrict__ f,
                                                     cx<T>* __restrict__ xo, cx<T>* __restrict__ bco,
                                                     const int* __restrict__ skip) {
   constexpr int H = 3;
   typedef Tile<T, TY, H> Tl;
-  constexpr int W = Tl::W, NPIX = Tl::NPIX, K = (NPIX + 255) / 256;
+  constexpr int W = Tl::W, NPIX = Tl::NPIX, K = (NPIX + NT - 1) / NT;
   __shared__ cx<T> s1[NPIX], s2[NPIX];
   int xt, yt, rhs;
   tile_of(tm, xt, yt, rhs);
@@ -85,18 +85,18 @@ __global__ __launch_bounds__(256) void k_down_fused(TileMap tm, Grid gf, Grid gc
   const int fx0 = xt * FTX, fy0 = yt * TY;
   const T ih1 = (T)gf.ih1, ih2 = (T)gf.ih2;
   cx<T> rb[K], rd[K], rf[K];
-  load_bd<T, W, NPIX, K, H, true>(b + (int64_t)rhs * gf.N, d, f, fx0, fy0, n1, n2, rb, rd, rf);
+  load_bd<T, NT, W, NPIX, K, H, true>(b + (int64_t)rhs * gf.N, d, f, fx0, fy0, n1, n2, rb, rd, rf);
   // x1 = F b on the whole tile
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    const int q = threadIdx.x + 256 * k;
+    const int q = threadIdx.x + NT * k;
     if (q < NPIX) s1[q] = rf[k] * rb[k];
   }
   __syncthreads();
   // x2 on the halo-2 region
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    const int q = threadIdx.x + 256 * k;
+    const int q = threadIdx.x + NT * k;
     const int li = q % W, lj = q / W;
     if (q < NPIX && li >= 1 && li < W - 1 && lj >= 1 && lj < Tl::HT - 1) {
       const cx<T> xc = s1[q];
@@ -108,7 +108,7 @@ __global__ __launch_bounds__(256) void k_down_fused(TileMap tm, Grid gf, Grid gc
   // residual on the halo-1 region (over x1, which is no longer read)
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    const int q = threadIdx.x + 256 * k;
+    const int q = threadIdx.x + NT * k;
     const int li = q % W, lj = q / W;
     if (q < NPIX && li >= 2 && li < W - 2 && lj >= 2 && lj < Tl::HT - 2) {
       const cx<T> xc = s2[q];
@@ -118,14 +118,14 @@ __global__ __launch_bounds__(256) void k_down_fused(TileMap tm, Grid gf, Grid gc
   }
   __syncthreads();
   cx<T>* xb = xo + (int64_t)rhs * gf.N;
-  for (int q = threadIdx.x; q < FTX * TY; q += 256) {
+  for (int q = threadIdx.x; q < FTX * TY; q += NT) {
     const int i = fx0 + q % FTX, j = fy0 + q / FTX;
     if (i < n1 && j < n2) xb[(int64_t)j * n1 + i] = s2[(q / FTX + H) * W + q % FTX + H];
   }
   // full weighting of r onto the owned coarse nodes
   cx<T>* cb = bco + (int64_t)rhs * gc.N;
   const T w[3] = {(T)0.25, (T)0.5, (T)0.25};
-  for (int q = threadIdx.x; q < (FTX / 2) * (TY / 2); q += 256) {
+  for (int q = threadIdx.x; q < (FTX / 2) * (TY / 2); q += NT) {
     const int I = fx0 / 2 + q % (FTX / 2), J = fy0 / 2 + q / (FTX / 2);
     if (I >= gc.n1 || J >= gc.n2) continue;
     const int lc = (2 * J - fy0 + H) * W + (2 * I - fx0 + H);
@@ -148,7 +148,7 @@ struct Patch {
   static constexpr int PH = (TY + 2 * H) / 2 + 2;
 };
 
-template <typename T, int TY, int W, int NPIX, int K, int H>
+template <typename T, int NT, int TY, int W, int NPIX, int K, int H>
 __device__ __forceinline__ void load_x_prolong(const cx<T>* __restrict__ xr, const cx<T>* __restrict__ cr,
                                                const Grid& gc, int fx0, int fy0, int n1, int n2, cx<T>* sx,
                                                cx<T>* sc) {
@@ -157,20 +157,20 @@ __device__ __forceinline__ void load_x_prolong(const cx<T>* __restrict__ xr, con
   cx<T> vx[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    const int q = threadIdx.x + 256 * k;
+    const int q = threadIdx.x + NT * k;
     if (q < NPIX) {
       const int li = q % W, lj = q / W;
       vx[k] = xr[(int64_t)mirror_idx(fy0 - H + lj, n2) * n1 + mirror_idx(fx0 - H + li, n1)];
     }
   }
-  for (int q = threadIdx.x; q < PW * PH; q += 256) {
+  for (int q = threadIdx.x; q < PW * PH; q += NT) {
     const int I = min(cx0 + q % PW, gc.n1 - 1), J = min(cy0 + q / PW, gc.n2 - 1);
     sc[q] = cr[(int64_t)J * gc.n1 + I];
   }
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    const int q = threadIdx.x + 256 * k;
+    const int q = threadIdx.x + NT * k;
     if (q < NPIX) {
       const int li = q % W, lj = q / W;
       const int gi = mirror_idx(fx0 - H + li, n1), gj = mirror_idx(fy0 - H + lj, n2);
@@ -187,14 +187,14 @@ __device__ __forceinline__ void load_x_prolong(const cx<T>* __restrict__ xr, con
 }
 
 // ------------------------------------------------------------------ middle (W)
-template <typename T, int TY>
-__global__ __launch_bounds__(256) void k_mid_fused(TileMap tm, Grid gf, Grid gc, const cx<T>* __restrict__ x,
+template <typename T, int TY, int NT>
+__global__ __launch_bounds__(NT) void k_mid_fused(TileMap tm, Grid gf, Grid gc, const cx<T>* __restrict__ x,
                                                    const cx<T>* __restrict__ ec, const cx<T>* __restrict__ b,
                                                    const cx<T>* __restrict__ d, cx<T>* __restrict__ xo,
                                                    cx<T>* __restrict__ bco, const int* __restrict__ skip) {
   constexpr int H = 2;
   typedef Tile<T, TY, H> Tl;
-  constexpr int W = Tl::W, NPIX = Tl::NPIX, K = (NPIX + 255) / 256;
+  constexpr int W = Tl::W, NPIX = Tl::NPIX, K = (NPIX + NT - 1) / NT;
   __shared__ cx<T> sx[NPIX], sr[NPIX], sc[Patch<T, TY, H>::PW * Patch<T, TY, H>::PH];
   int xt, yt, rhs;
   tile_of(tm, xt, yt, rhs);
@@ -203,13 +203,13 @@ __global__ __launch_bounds__(256) void k_mid_fused(TileMap tm, Grid gf, Grid gc,
   const int fx0 = xt * FTX, fy0 = yt * TY;
   const T ih1 = (T)gf.ih1, ih2 = (T)gf.ih2;
   cx<T> rb[K], rd[K], rf[K];  // rf unused (no Jacobi sweep in this leg)
-  load_bd<T, W, NPIX, K, H, false>(b + (int64_t)rhs * gf.N, d, nullptr, fx0, fy0, n1, n2, rb, rd, rf);
-  load_x_prolong<T, TY, W, NPIX, K, H>(x + (int64_t)rhs * gf.N, ec + (int64_t)rhs * gc.N, gc, fx0, fy0, n1, n2, sx,
+  load_bd<T, NT, W, NPIX, K, H, false>(b + (int64_t)rhs * gf.N, d, nullptr, fx0, fy0, n1, n2, rb, rd, rf);
+  load_x_prolong<T, NT, TY, W, NPIX, K, H>(x + (int64_t)rhs * gf.N, ec + (int64_t)rhs * gc.N, gc, fx0, fy0, n1, n2, sx,
                                        sc);
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    const int q = threadIdx.x + 256 * k;
+    const int q = threadIdx.x + NT * k;
     const int li = q % W, lj = q / W;
     if (q < NPIX && li >= 1 && li < W - 1 && lj >= 1 && lj < Tl::HT - 1) {
       const cx<T> xc = sx[q];
@@ -219,13 +219,13 @@ __global__ __launch_bounds__(256) void k_mid_fused(TileMap tm, Grid gf, Grid gc,
   }
   __syncthreads();
   cx<T>* xb = xo + (int64_t)rhs * gf.N;
-  for (int q = threadIdx.x; q < FTX * TY; q += 256) {
+  for (int q = threadIdx.x; q < FTX * TY; q += NT) {
     const int i = fx0 + q % FTX, j = fy0 + q / FTX;
     if (i < n1 && j < n2) xb[(int64_t)j * n1 + i] = sx[(q / FTX + H) * W + q % FTX + H];
   }
   cx<T>* cb = bco + (int64_t)rhs * gc.N;
   const T w[3] = {(T)0.25, (T)0.5, (T)0.25};
-  for (int q = threadIdx.x; q < (FTX / 2) * (TY / 2); q += 256) {
+  for (int q = threadIdx.x; q < (FTX / 2) * (TY / 2); q += NT) {
     const int I = fx0 / 2 + q % (FTX / 2), J = fy0 / 2 + q / (FTX / 2);
     if (I >= gc.n1 || J >= gc.n2) continue;
     const int lc = (2 * J - fy0 + H) * W + (2 * I - fx0 + H);
@@ -237,14 +237,14 @@ __global__ __launch_bounds__(256) void k_mid_fused(TileMap tm, Grid gf, Grid gc,
 }
 
 // ------------------------------------------------------------------ up leg
-template <typename T, int TY>
-__global__ __launch_bounds__(256) void k_up_fused(TileMap tm, Grid gf, Grid gc, const cx<T>* __restrict__ x,
+template <typename T, int TY, int NT>
+__global__ __launch_bounds__(NT) void k_up_fused(TileMap tm, Grid gf, Grid gc, const cx<T>* __restrict__ x,
                                                   const cx<T>* __restrict__ ec, const cx<T>* __restrict__ b,
                                                   const cx<T>* __restrict__ d, const cx<T>* __restrict__ f,
                                                   cx<T>* __restrict__ xo, const int* __restrict__ skip) {
   constexpr int H = 2;
   typedef Tile<T, TY, H> Tl;
-  constexpr int W = Tl::W, NPIX = Tl::NPIX, K = (NPIX + 255) / 256;
+  constexpr int W = Tl::W, NPIX = Tl::NPIX, K = (NPIX + NT - 1) / NT;
   __shared__ cx<T> sx[NPIX], s3[NPIX], sc[Patch<T, TY, H>::PW * Patch<T, TY, H>::PH];
   int xt, yt, rhs;
   tile_of(tm, xt, yt, rhs);
@@ -253,8 +253,8 @@ __global__ __launch_bounds__(256) void k_up_fused(TileMap tm, Grid gf, Grid gc, 
   const int fx0 = xt * FTX, fy0 = yt * TY;
   const T ih1 = (T)gf.ih1, ih2 = (T)gf.ih2;
   cx<T> rb[K], rd[K], rf[K];
-  load_bd<T, W, NPIX, K, H, true>(b + (int64_t)rhs * gf.N, d, f, fx0, fy0, n1, n2, rb, rd, rf);
-  load_x_prolong<T, TY, W, NPIX, K, H>(x + (int64_t)rhs * gf.N, ec + (int64_t)rhs * gc.N, gc, fx0, fy0, n1, n2, sx,
+  load_bd<T, NT, W, NPIX, K, H, true>(b + (int64_t)rhs * gf.N, d, f, fx0, fy0, n1, n2, rb, rd, rf);
+  load_x_prolong<T, NT, TY, W, NPIX, K, H>(x + (int64_t)rhs * gf.N, ec + (int64_t)rhs * gc.N, gc, fx0, fy0, n1, n2, sx,
                                        sc);
   __syncthreads();
   auto jac = [&](const cx<T>* s, int q, int k) {
@@ -264,7 +264,7 @@ __global__ __launch_bounds__(256) void k_up_fused(TileMap tm, Grid gf, Grid gc, 
   };
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    const int q = threadIdx.x + 256 * k;
+    const int q = threadIdx.x + NT * k;
     const int li = q % W, lj = q / W;
     if (q < NPIX && li >= 1 && li < W - 1 && lj >= 1 && lj < Tl::HT - 1) s3[q] = jac(sx, q, k);
   }
@@ -272,7 +272,7 @@ __global__ __launch_bounds__(256) void k_up_fused(TileMap tm, Grid gf, Grid gc, 
   cx<T>* xb = xo + (int64_t)rhs * gf.N;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    const int q = threadIdx.x + 256 * k;
+    const int q = threadIdx.x + NT * k;
     const int li = q % W, lj = q / W;
     const int i = fx0 + li - H, j = fy0 + lj - H;
     if (q < NPIX && li >= H && li < H + FTX && lj >= H && lj < H + TY && i < n1 && j < n2)
@@ -281,9 +281,12 @@ __global__ __launch_bounds__(256) void k_up_fused(TileMap tm, Grid gf, Grid gc, 
 }
 
 // ------------------------------------------------------------------ launchers
+// tile height and threads per block: complex64 64 x 32 tiles on 512 threads,
+// complex128 64 x 8 on 256 (register and LDS budget)
 template <typename T>
 struct FusedTY {
-  static constexpr int v = sizeof(T) == 4 ? 16 : 8;
+  static constexpr int v = sizeof(T) == 4 ? 32 : 8;
+  static constexpr int nt = sizeof(T) == 4 ? 512 : 256;
 };
 
 bool fused_ok(const Grid& gf) { return gf.dim == 2 && gf.bc == 2 && gf.n1 >= 16 && gf.n2 >= 16; }
@@ -299,7 +302,7 @@ void launch_down_fused(const Grid& gf, const Grid& gc, int nrhs, const cx<T>* b,
                        cx<T>* xo, cx<T>* bco, const int* skip, hipStream_t s) {
   constexpr int TY = FusedTY<T>::v;
   const TileMap m = tile_map<T>(gf, nrhs);
-  hipLaunchKernelGGL((k_down_fused<T, TY>), dim3(m.T), dim3(256), 0, s, m, gf, gc, b, d, f, xo, bco,
+  hipLaunchKernelGGL((k_down_fused<T, TY, FusedTY<T>::nt>), dim3(m.T), dim3(FusedTY<T>::nt), 0, s, m, gf, gc, b, d, f, xo, bco,
                      skip);
 }
 
@@ -308,7 +311,7 @@ void launch_mid_fused(const Grid& gf, const Grid& gc, int nrhs, const cx<T>* x, 
                       const cx<T>* d, cx<T>* xo, cx<T>* bco, const int* skip, hipStream_t s) {
   constexpr int TY = FusedTY<T>::v;
   const TileMap m = tile_map<T>(gf, nrhs);
-  hipLaunchKernelGGL((k_mid_fused<T, TY>), dim3(m.T), dim3(256), 0, s, m, gf, gc, x, ec, b, d, xo, bco,
+  hipLaunchKernelGGL((k_mid_fused<T, TY, FusedTY<T>::nt>), dim3(m.T), dim3(FusedTY<T>::nt), 0, s, m, gf, gc, x, ec, b, d, xo, bco,
                      skip);
 }
 
@@ -317,7 +320,7 @@ void launch_up_fused(const Grid& gf, const Grid& gc, int nrhs, const cx<T>* x, c
                      const cx<T>* d, const cx<T>* f, cx<T>* xo, const int* skip, hipStream_t s) {
   constexpr int TY = FusedTY<T>::v;
   const TileMap m = tile_map<T>(gf, nrhs);
-  hipLaunchKernelGGL((k_up_fused<T, TY>), dim3(m.T), dim3(256), 0, s, m, gf, gc, x, ec, b, d, f, xo,
+  hipLaunchKernelGGL((k_up_fused<T, TY, FusedTY<T>::nt>), dim3(m.T), dim3(FusedTY<T>::nt), 0, s, m, gf, gc, x, ec, b, d, f, xo,
                      skip);
 }
 
