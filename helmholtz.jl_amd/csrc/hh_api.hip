@@ -145,6 +145,9 @@ struct hh_op {
   double h[3] = {1, 1, 1};
   double wre = 0, wim = 0;
   int neumann_top = 0, sommerfeld = 0, bc = 2;
+  // spread operator of GetHelmholtzOperatorHO (bc = 1 for its multigrid hierarchy)
+  int ho = 0;
+  double beta_lap = 1.0, beta_mass = 1.0;
   Grid g;
   double* m = nullptr;
   double* gam = nullptr;
@@ -694,16 +697,26 @@ void precondition(hh_solver* S, int nrhs, const cx<double>* vj, void* zj, const 
   }
 }
 
-void h_apply(hh_solver* S, int nrhs, const cx<double>* x, cx<double>* y, const int* skip, hipStream_t st) {
+// the outer operator: w = H z (z in the preconditioner precision, w complex128)
+template <typename TI>
+void h_apply(hh_solver* S, int nrhs, const cx<TI>* x, cx<double>* y, const int* skip, hipStream_t st) {
   hh_op* op = S->op;
-  launch_stencil<double, double, double, M_APPLY>(op->g, nrhs, x, nullptr, y, op->d64, nullptr, mk<double>(0, 0),
-                                                  0.0, skip, st);
+  if (op->ho)
+    launch_ho_stencil<TI, M_APPLY>(op->g, op->beta_lap, op->beta_mass, nrhs, x, nullptr, y, op->d64, nullptr,
+                                   mk<double>(0, 0), skip, st);
+  else
+    launch_stencil<TI, double, double, M_APPLY>(op->g, nrhs, x, nullptr, y, op->d64, nullptr, mk<double>(0, 0), 0.0,
+                                                skip, st);
 }
 
 void h_resid(hh_solver* S, int nrhs, const cx<double>* x, const cx<double>* b, cx<double>* r, hipStream_t st) {
   hh_op* op = S->op;
-  launch_stencil<double, double, double, M_RESID>(op->g, nrhs, x, b, r, op->d64, nullptr, mk<double>(0, 0), 0.0,
-                                                  nullptr, st);
+  if (op->ho)
+    launch_ho_stencil<double, M_RESID>(op->g, op->beta_lap, op->beta_mass, nrhs, x, b, r, op->d64, nullptr,
+                                       mk<double>(0, 0), nullptr, st);
+  else
+    launch_stencil<double, double, double, M_RESID>(op->g, nrhs, x, b, r, op->d64, nullptr, mk<double>(0, 0), 0.0,
+                                                    nullptr, st);
 }
 
 void run_fgmres(hh_solver* S, int nrhs, hh_solve_stats* stats) {
@@ -757,13 +770,12 @@ void run_fgmres(hh_solver* S, int nrhs, hh_solve_stats* stats) {
         if (S->p.prec == HH_C64) {
           precondition<float>(S, nrhs, vj, zj, w.done, st, w.lazy ? w.vscale + (int64_t)j * nrhs : nullptr);
           if (ta) S->kt.begin(st);
-          launch_stencil<float, double, double, M_APPLY>(S->op->g, nrhs, (const cx<float>*)zj, nullptr, S->wk,
-                                                         S->op->d64, nullptr, mk<double>(0, 0), 0.0, w.done, st);
+          h_apply<float>(S, nrhs, (const cx<float>*)zj, S->wk, w.done, st);
           if (ta) S->kt.end(st, active * Nf * 24.0 + Nf * 16.0);
         } else {
           precondition<double>(S, nrhs, vj, zj, w.done, st);
           if (ta) S->kt.begin(st);
-          h_apply(S, nrhs, (const cx<double>*)zj, S->wk, w.done, st);
+          h_apply<double>(S, nrhs, (const cx<double>*)zj, S->wk, w.done, st);
           if (ta) S->kt.end(st, active * Nf * 32.0 + Nf * 16.0);
         }
         kr_arnoldi_impl(w, S->V, S->wk, j, tol, S->skip_copy, &S->kt, active, st);
@@ -942,11 +954,10 @@ void run_bicgstab(hh_solver* S, int nrhs, hh_solve_stats* stats) {
   auto prec_apply = [&](const cx<double>* in, void* z, cx<double>* out) {
     if (S->p.prec == HH_C64) {
       precondition<float>(S, nrhs, in, z, w.done, st);
-      launch_stencil<float, double, double, M_APPLY>(S->op->g, nrhs, (const cx<float>*)z, nullptr, out, S->op->d64,
-                                                     nullptr, mk<double>(0, 0), 0.0, w.done, st);
+      h_apply<float>(S, nrhs, (const cx<float>*)z, out, w.done, st);
     } else {
       precondition<double>(S, nrhs, in, z, w.done, st);
-      h_apply(S, nrhs, (const cx<double>*)z, out, w.done, st);
+      h_apply<double>(S, nrhs, (const cx<double>*)z, out, w.done, st);
     }
   };
   for (int cycle = 0;; ++cycle) {
@@ -1085,6 +1096,23 @@ int hh_op_create(int dim, const int64_t nnodes[3], const double h[3], const doub
   });
 }
 
+int hh_op_create_ho(int dim, const int64_t nnodes[3], const double h[3], const double* m, const double* gamma,
+                    double omega_re, double omega_im, int neumann_top, int sommerfeld, double beta_lap,
+                    double beta_mass, hh_op** out) {
+  return guarded([&] {
+    if (!(beta_lap == beta_lap) || !(beta_mass == beta_mass)) throw HHError(HH_ERR_ARG, "beta must be finite");
+    hh_op* op = nullptr;
+    // the multigrid hierarchy of this operator uses the natural-Neumann (BC = 1)
+    // 5/7-point stencil, which is the spread operator at beta = 1
+    const int rc = hh_op_create(dim, nnodes, h, m, gamma, omega_re, omega_im, neumann_top, sommerfeld, 1, &op);
+    if (rc != HH_OK) throw HHError(rc, g_err);
+    op->ho = 1;
+    op->beta_lap = beta_lap;
+    op->beta_mass = dim == 2 ? beta_lap : beta_mass;
+    *out = op;
+  });
+}
+
 int hh_op_destroy(hh_op* op) {
   return guarded([&] { delete op; });
 }
@@ -1108,7 +1136,11 @@ int hh_op_apply(hh_op* op, double shift, const void* X, void* Y, int64_t nrhs, i
     const double w2r = op->wre * op->wre - op->wim * op->wim, w2i = 2.0 * op->wre * op->wim;
     const cx<double> sfac = mk<double>(-shift * w2i, shift * w2r);
     const double* mm = shift != 0.0 ? op->m : nullptr;
-    if (prec == HH_C128)
+    if (op->ho) {
+      if (prec != HH_C128) throw HHError(HH_ERR_UNSUPPORTED, "the spread (HO) operator applies in complex128");
+      launch_ho_stencil<double, M_APPLY>(op->g, op->beta_lap, op->beta_mass, (int)nrhs, (const cx<double>*)xin,
+                                         nullptr, (cx<double>*)yout, op->d64, mm, sfac, nullptr, op->stream);
+    } else if (prec == HH_C128)
       launch_stencil<double, double, double, M_APPLY>(op->g, (int)nrhs, (const cx<double>*)xin, nullptr,
                                                       (cx<double>*)yout, op->d64, mm, sfac, 0.0, nullptr,
                                                       op->stream);
@@ -1180,6 +1212,8 @@ int hh_solve(hh_solver* s, const void* B, void* X, int64_t nrhs, int do_transpos
     if (!s || !B || !X) throw HHError(HH_ERR_ARG, "null argument");
     if (nrhs < 1 || nrhs > 65535) throw HHError(HH_ERR_ARG, "nrhs must be in 1..65535");
     if (do_transpose != 0 && do_transpose != 1) throw HHError(HH_ERR_ARG, "do_transpose must be 0 or 1");
+    if (do_transpose && s->op->ho)
+      throw HHError(HH_ERR_UNSUPPORTED, "adjoint solves of the spread (HO) operator are not implemented");
     if (loc != HH_HOST && loc != HH_DEVICE) throw HHError(HH_ERR_ARG, "loc must be HH_HOST or HH_DEVICE");
     hh_op* op = s->op;
     hipStream_t st = op->stream;
@@ -1335,6 +1369,7 @@ int hh_op_time_kernel(hh_op* op, int kernel, double shift, const void* X, const 
     if (nrhs < 1 || nrhs > 65535) throw HHError(HH_ERR_ARG, "nrhs must be in 1..65535");
     if (kernel < 0 || kernel > 2) throw HHError(HH_ERR_ARG, "kernel must be 0 (apply), 1 (residual), 2 (Jacobi)");
     if (kernel > 0 && !B) throw HHError(HH_ERR_ARG, "residual/Jacobi need B");
+    if (op->ho) throw HHError(HH_ERR_UNSUPPORTED, "kernel timing covers the 5/7-point operator");
     const double w2r = op->wre * op->wre - op->wim * op->wim, w2i = 2.0 * op->wre * op->wim;
     const cx<double> sfac = mk<double>(-shift * w2i, shift * w2r);
     const double* mm = shift != 0.0 ? op->m : nullptr;

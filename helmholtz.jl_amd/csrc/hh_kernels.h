@@ -48,6 +48,12 @@ int64_t pack_inv_mfma_size(int64_t n);
 void launch_dense_apply_mfma(const float* P, int64_t n, int nrhs, const cx<float>* r, cx<float>* x, const int* skip,
                              hipStream_t s);
 
+// ---- spread (GetHelmholtzOperatorHO) operator (hh_ho.hip): M_APPLY y = H x (+ sfac m x), M_RESID y = b - H x ----
+template <typename TI, int MODE>
+void launch_ho_stencil(const Grid& g, double beta_lap, double beta_mass, int nrhs, const cx<TI>* x,
+                       const cx<double>* b, cx<double>* y, const cx<double>* d, const double* m, cx<double> sfac,
+                       const int* skip, hipStream_t s);
+
 // ---- fused 2-D multigrid legs (hh_fused.hip): BC = 2, 2 pre/2 post sweeps ----
 bool fused_ok(const Grid& gf);
 template <typename T>

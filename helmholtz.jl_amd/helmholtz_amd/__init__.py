@@ -7,7 +7,7 @@ reference interface.  Importing it does not touch the GPU.
 """
 from ._lib import HelmholtzError, LIB_PATH, lib  # noqa: F401
 from .helmholtz import (  # noqa: F401
-    GetHelmholtzOperator, GetHelmholtzShiftOP, HelmholtzOperator, HelmholtzParam, RegularMesh,
+    GetHelmholtzOperator, GetHelmholtzOperatorHO, GetHelmholtzShiftOP, HelmholtzOperator, HelmholtzParam, RegularMesh,
     getABL, getAcousticPointSource, getMaximalFrequency, getMidPointSrc, getRegularMesh,
     getShiftedHelmholtzParam, getTopPointSrc, loc2cs, point_sources,
 )

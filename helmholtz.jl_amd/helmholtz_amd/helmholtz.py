@@ -128,7 +128,8 @@ def GetHelmholtzShiftOP(mNodal, omega: float, shift: float) -> ShiftOp:
 class _OpHandle:
     """Owns one ``hh_op``; shared between the shifted views of an operator."""
 
-    def __init__(self, mesh: RegularMesh, m, omega, gamma, neumann: bool, sommerfeld: bool, order: int):
+    def __init__(self, mesh: RegularMesh, m, omega, gamma, neumann: bool, sommerfeld: bool, order: int,
+                 ho_beta=None):
         nn = [int(v) for v in mesh.nodes]
         N = int(np.prod(nn))
         self.m = np.ascontiguousarray(np.asarray(m, dtype=np.float64).ravel(order="F"))
@@ -139,9 +140,16 @@ class _OpHandle:
         self.N, self.nn, self.dim = N, nn, mesh.dim
         self.omega = w
         h = C.c_void_p()
-        L.check(L.lib().hh_op_create(mesh.dim, _i64x3(nn), _f64x3(mesh.h), self.m.ctypes.data,
-                                     self.gamma.ctypes.data, w.real, w.imag, int(bool(neumann)),
-                                     int(bool(sommerfeld)), int(order), C.byref(h)))
+        self.ho_beta = ho_beta
+        if ho_beta is None:
+            L.check(L.lib().hh_op_create(mesh.dim, _i64x3(nn), _f64x3(mesh.h), self.m.ctypes.data,
+                                         self.gamma.ctypes.data, w.real, w.imag, int(bool(neumann)),
+                                         int(bool(sommerfeld)), int(order), C.byref(h)))
+        else:
+            bl, bm = ho_beta
+            L.check(L.lib().hh_op_create_ho(mesh.dim, _i64x3(nn), _f64x3(mesh.h), self.m.ctypes.data,
+                                            self.gamma.ctypes.data, w.real, w.imag, int(bool(neumann)),
+                                            int(bool(sommerfeld)), float(bl), float(bm), C.byref(h)))
         self.ptr = h
 
     def __del__(self):
@@ -233,6 +241,31 @@ def GetHelmholtzOperator(*args, orderNeumannBC: int = 2):
     else:
         g = np.asarray(gamma, dtype=float).reshape(abl.shape, order="F") + abl
     return _build(mesh, m, omega, g, neumann, sommerfeld, order), g
+
+
+def GetHelmholtzOperatorHO(*args, beta=1.0):
+    """``GetHelmholtzOperatorHO`` (src/GetHelmholtz.jl:18-20, 54-72): the spread
+    Laplacian + averaged mass operator, matrix-free on the GPU.
+
+    * ``GetHelmholtzOperatorHO(Hparam[, beta])``
+    * ``GetHelmholtzOperatorHO(Msh, m, omega, gamma, NeumannAtFirstDim, Sommerfeld[, beta])``
+
+    beta: scalar in 2-D; in 3-D the reference's ``[beta_lap; beta_mass]`` (a
+    scalar sets both).  Passed as ``ShiftedHT`` to ``solveLinearSystem`` it
+    becomes the outer operator of the solve."""
+    if isinstance(args[0], HelmholtzParam):
+        p = args[0]
+        b = args[1] if len(args) > 1 else beta
+        mesh, m, omega, gamma, neumann, sommerfeld = p.Mesh, p.m, p.omega, p.gamma, p.NeumannOnTop, p.Sommerfeld
+        param = p
+    else:
+        mesh, m, omega, gamma, neumann, sommerfeld = args[:6]
+        b = args[6] if len(args) > 6 else beta
+        param = None
+    bb = np.broadcast_to(np.asarray(b, dtype=float).ravel(), (2,))
+    op = HelmholtzOperator(_OpHandle(mesh, m, omega, gamma, neumann, sommerfeld, 1, (bb[0], bb[1])), 0.0, mesh,
+                           param)
+    return op
 
 
 # ------------------------------------------------------------------ sources

@@ -22,7 +22,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib as L
-from .helmholtz import HelmholtzParam, _OpHandle, _is_device
+from .helmholtz import HelmholtzOperator, HelmholtzParam, _OpHandle, _is_device
 
 
 @dataclass
@@ -136,13 +136,23 @@ def _validate(s: ShiftedLaplacianMultigridSolver):
         raise ValueError("MG.prec must be 'c128' or 'c64'")
 
 
-def _ensure_handle(s: ShiftedLaplacianMultigridSolver):
+def _ensure_handle(s: ShiftedLaplacianMultigridSolver, outer=None):
+    """``outer``: a spread operator from GetHelmholtzOperatorHO passed as
+    ShiftedHT becomes the solve's outer operator; otherwise the 5/7-point
+    operator of helmParam (mirror-ghost BC) is built."""
     p = s.helmParam
-    key = (id(p), id(p.m), id(p.gamma), complex(p.omega), bool(p.NeumannOnTop), bool(p.Sommerfeld))
-    if s._op is None or s._op_key != key:
-        _drop_handle(s)
-        s._op = _OpHandle(p.Mesh, p.m, p.omega, p.gamma, p.NeumannOnTop, p.Sommerfeld, 2)
-        s._op_key = key
+    if outer is not None and getattr(outer._h, "ho_beta", None) is not None:
+        key = ("ho", id(outer._h))
+        if s._op is None or s._op_key != key:
+            _drop_handle(s)
+            s._op = outer._h
+            s._op_key = key
+    else:
+        key = (id(p), id(p.m), id(p.gamma), complex(p.omega), bool(p.NeumannOnTop), bool(p.Sommerfeld))
+        if s._op is None or s._op_key != key:
+            _drop_handle(s)
+            s._op = _OpHandle(p.Mesh, p.m, p.omega, p.gamma, p.NeumannOnTop, p.Sommerfeld, 2)
+            s._op_key = key
     if s._handle is None:
         mg = s.MG
         prm = L.MGParams(levels=int(mg.levels), pre=int(mg.relaxPre), post=int(mg.relaxPost),
@@ -196,7 +206,7 @@ def solveLinearSystem(ShiftedHT, B, param: ShiftedLaplacianMultigridSolver, doTr
             return np.zeros(Ba.shape, dtype=Ba.dtype), param
         X = np.empty_like(B2, order="F")
         bptr, xptr, loc = B2.ctypes.data, X.ctypes.data, L.HH_HOST
-    _ensure_handle(param)
+    _ensure_handle(param, ShiftedHT if isinstance(ShiftedHT, HelmholtzOperator) else None)
     iters = np.zeros(nrhs, dtype=np.int32)
     status = np.zeros(nrhs, dtype=np.int32)
     res = np.zeros(nrhs, dtype=np.float64)

@@ -100,6 +100,18 @@ int hh_abl(int dim, const int64_t nnodes[3], int neumann_at_first_dim, const int
  * h[d] = Msh.h[d]; bc_order = orderNeumannBC (1 or 2). */
 int hh_op_create(int dim, const int64_t nnodes[3], const double h[3], const double* m, const double* gamma,
                  double omega_re, double omega_im, int neumann_top, int sommerfeld, int bc_order, hh_op** out);
+/* GetHelmholtzOperatorHO(Msh, m, omega, gamma, NeumannAtFirstDim, Sommerfeld,
+ * beta) (src/GetHelmholtz.jl:18-20,54-72; src/PlainNodalLaplacian.jl:54-141):
+ * the spread Laplacian G'((1-b)Gs + bG) plus the averaged mass
+ * M diag(mass), matrix-free (9-point in 2-D, 19-point in 3-D).  2-D uses
+ * beta_lap for both (the reference's scalar beta); 3-D takes the reference's
+ * beta = [beta_lap; beta_mass].  Applies in complex128 (hh_op_apply); as the
+ * operator of hh_solve it is the outer operator and the preconditioner's
+ * hierarchy uses the natural-Neumann (BC = 1) 5/7-point stencil, which is
+ * this operator at beta = 1.  Adjoint solves are not implemented for it. */
+int hh_op_create_ho(int dim, const int64_t nnodes[3], const double h[3], const double* m, const double* gamma,
+                    double omega_re, double omega_im, int neumann_top, int sommerfeld, double beta_lap,
+                    double beta_mass, hh_op** out);
 int hh_op_destroy(hh_op* op);
 
 /* Y = (H + i*shift*w^2*diag(m)) X for nrhs columns.  shift = 0 gives H·x, the

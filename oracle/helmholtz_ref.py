@@ -88,6 +88,86 @@ def get_nodal_laplacian_matrix(mesh: RegularMesh, order_neumann_bc: int = 2) -> 
     return (sp.kron(I3, sp.kron(I2, D1) + sp.kron(D2, I1)) + sp.kron(D3, sp.kron(I2, I1))).tocsr()
 
 
+# --------------------------------------------------------------------------
+# spread ("HO") Laplacian and mass: src/PlainNodalLaplacian.jl:54-141,
+# src/GetHelmholtz.jl:18-20,54-72
+# --------------------------------------------------------------------------
+def _speye(n: int) -> sp.csr_matrix:
+    return sp.identity(int(n), format="csr")
+
+
+def ddx_cn(n: int, h: float) -> sp.csr_matrix:
+    """``ddxCN`` (src/PlainNodalLaplacian.jl:54-58): n x (n+1) cell-to-node
+    difference, row c = (-1/h at node c, +1/h at node c+1)."""
+    rows = np.repeat(np.arange(n), 2)
+    cols = np.stack([np.arange(n), np.arange(n) + 1], axis=1).ravel()
+    vals = np.tile([-1.0 / h, 1.0 / h], n)
+    return sp.csr_matrix((vals, (rows, cols)), shape=(n, n + 1))
+
+
+def av3term(n: int, alpha: float = 5.0 / 6.0) -> sp.csr_matrix:
+    """``av3term`` (src/PlainNodalLaplacian.jl:60-66): tridiagonal average,
+    diagonal alpha, off-diagonals (1-alpha)/2, end entries 1/2 + alpha/2."""
+    t = (1.0 - alpha) / 2.0
+    T = sp.diags([np.full(n - 1, t), np.full(n, alpha), np.full(n - 1, t)], [-1, 0, 1], format="lil")
+    T[0, 0] = 0.5 + alpha / 2.0
+    T[n - 1, n - 1] = 0.5 + alpha / 2.0
+    return T.tocsr()
+
+
+def get_nodal_spread_gradients(mesh: RegularMesh, av_func):
+    """``getNodalSpreadGradients`` (src/PlainNodalLaplacian.jl:69-100): the
+    nodal gradient G and its transversally averaged version Gs."""
+    n = [int(v) for v in mesh.n]
+    h = mesh.h
+    if mesh.dim == 2:
+        t = ddx_cn(n[0], h[0])
+        D1, D1s = sp.kron(_speye(n[1] + 1), t), sp.kron(av_func(n[1] + 1), t)
+        t = ddx_cn(n[1], h[1])
+        D2, D2s = sp.kron(t, _speye(n[0] + 1)), sp.kron(t, av_func(n[0] + 1))
+        return sp.vstack([D1, D2]).tocsr(), sp.vstack([D1s, D2s]).tocsr()
+    I1, I2, I3 = _speye(n[0] + 1), _speye(n[1] + 1), _speye(n[2] + 1)
+    t = ddx_cn(n[0], h[0])
+    D1 = sp.kron(I3, sp.kron(I2, t))
+    D1s = 0.5 * (sp.kron(I3, sp.kron(av_func(n[1] + 1), t)) + sp.kron(av_func(n[2] + 1), sp.kron(I2, t)))
+    t = ddx_cn(n[1], h[1])
+    D2 = sp.kron(I3, sp.kron(t, I1))
+    D2s = 0.5 * (sp.kron(I3, sp.kron(t, av_func(n[0] + 1))) + sp.kron(av_func(n[2] + 1), sp.kron(t, I1)))
+    t = ddx_cn(n[2], h[2])
+    D3 = sp.kron(t, sp.kron(I2, I1))
+    D3s = 0.5 * (sp.kron(t, sp.kron(I2, av_func(n[0] + 1))) + sp.kron(t, sp.kron(av_func(n[1] + 1), I1)))
+    return sp.vstack([D1, D2, D3]).tocsr(), sp.vstack([D1s, D2s, D3s]).tocsr()
+
+
+def get_spread_nodal_laplacian_and_mass(mesh: RegularMesh, beta):
+    """``getSpreadNodalLaplacianAndMass`` (src/PlainNodalLaplacian.jl:106-141):
+    Lap = G' ((1-b) Gs + b G) and the averaging mass matrix M.  2-D: scalar
+    beta for both; 3-D: beta = (b_lap, b_mass) (a scalar 1 means (1, 1))."""
+    n = [int(v) for v in mesh.n]
+    av = lambda k: av3term(k, 0.5)
+    G, Gs = get_nodal_spread_gradients(mesh, av)
+    if mesh.dim == 2:
+        b = float(beta)
+        Lap = (G.T @ ((1 - b) * Gs + b * G)).tocsr()
+        M = 0.5 * sp.kron(av3term(n[1] + 1, b), _speye(n[0] + 1)) + 0.5 * sp.kron(_speye(n[1] + 1), av3term(n[0] + 1, b))
+        return Lap, M.tocsr()
+    bb = np.broadcast_to(np.asarray(beta, dtype=float).ravel(), (2,))
+    Lap = (G.T @ ((1 - bb[0]) * Gs + bb[0] * G)).tocsr()
+    I1, I2, I3 = _speye(n[0] + 1), _speye(n[1] + 1), _speye(n[2] + 1)
+    M = (1.0 / 3.0) * (sp.kron(I3, sp.kron(av3term(n[1] + 1, bb[1]), I1))
+                       + sp.kron(I3, sp.kron(I2, av3term(n[0] + 1, bb[1])))
+                       + sp.kron(av3term(n[2] + 1, bb[1]), sp.kron(I2, I1)))
+    return Lap, M.tocsr()
+
+
+def get_helmholtz_operator_ho(mesh: RegularMesh, m_nodal, omega, gamma, neumann_at_first_dim: bool,
+                              sommerfeld: bool, beta=1.0) -> sp.csr_matrix:
+    """``GetHelmholtzOperatorHO`` (src/GetHelmholtz.jl:54-72): H = Lap + M diag(mass)."""
+    Lap, M = get_spread_nodal_laplacian_and_mass(mesh, beta)
+    mass = helmholtz_mass(mesh, m_nodal, omega, gamma, neumann_at_first_dim, sommerfeld)
+    return (Lap.astype(complex) + M @ sp.diags(mass)).tocsr()
+
+
 def lap2d_stencil(x1, x2, x3, x4, x5, h1invsq, h2invsq):
     """``Lap2DStencil`` (src/PlainNodalLaplacian.jl:150-152)."""
     return (2 * h1invsq + 2 * h2invsq) * x1 - h1invsq * (x2 + x3) - h2invsq * (x4 + x5)

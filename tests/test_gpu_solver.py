@@ -438,3 +438,28 @@ def test_mfma_coarsest_apply_matches_double(nodes):
     print(f"coarsest apply {nodes}: complex64 (MFMA) vs complex128 {err:.2e}; complex128 vs SciPy {err_d:.2e}")
     assert err_d <= 1e-12
     assert err <= 1e-5
+
+
+@pytest.mark.parametrize("krylov,prec", [("GMRES", "c64"), ("GMRES", "c128"), ("BiCGSTAB", "c128")])
+def test_ho_operator_solve(krylov, prec):
+    """The spread (HO) operator passed as ShiftedHT is the outer operator of
+    the solve (the preconditioner stays the 5-point SL-MG hierarchy): true
+    residual of the HO system from the oracle's CSR, and the direct solve."""
+    M, m, g, w = small_case((129, 65), seed=38, neumann=True)
+    beta = 2.0 / 3.0
+    H = hr.get_helmholtz_operator_ho(M, m, w, g, True, True, beta)
+    mesh = ha.getRegularMesh(M.domain, M.n)
+    Hg = ha.GetHelmholtzOperatorHO(mesh, m, w, g, True, True, beta)
+    rng = np.random.default_rng(14)
+    B = np.asfortranarray(rng.standard_normal((m.size, 2)) + 1j * rng.standard_normal((m.size, 2)))
+    s = _solver(M, m, g, w, True, 4, 0.5, cycle="W", prec=prec, krylov=krylov)
+    X, s = ha.solveLinearSystem(Hg, B, s)
+    xd = spla.splu(H.tocsc()).solve(B)
+    for c in range(2):
+        r = np.linalg.norm(H @ X[:, c] - B[:, c]) / np.linalg.norm(B[:, c])
+        err = np.linalg.norm(X[:, c] - xd[:, c]) / np.linalg.norm(xd[:, c])
+        print(f"HO {krylov} {prec}: iters {s.last_stats['iters'][c]} res {r:.2e} err vs direct {err:.2e}")
+        assert r <= 1e-6
+        assert err <= 1e-4
+    with pytest.raises(ha.HelmholtzError):
+        ha.solveLinearSystem(Hg, B, s, 1)

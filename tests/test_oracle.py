@@ -369,3 +369,33 @@ def test_adjoint_similarity_with_trapezoid_weights(nodes, order):
     assert np.allclose(H.conj().T, W @ H.conj() @ np.linalg.inv(W), rtol=0, atol=1e-12 * np.abs(H).max())
     if order == 2:
         assert not np.allclose(H.T, H)  # the identity is not a triviality
+
+
+@pytest.mark.parametrize("nodes", [(9, 7), (6, 5, 4)])
+def test_ho_beta_one_is_the_bc1_operator(nodes):
+    """GetHelmholtzOperatorHO with beta = 1: G'G is the natural-Neumann
+    Laplacian and av3term(., 1) = I, so H_HO equals the 7-argument operator
+    with orderNeumannBC = 1 (src/GetHelmholtz.jl:33-50 vs 54-72)."""
+    M, m, g, w = small_case(nodes, seed=9)
+    H1 = hr.get_helmholtz_operator_ho(M, m, w, g, True, True, 1.0)
+    H7 = hr.get_helmholtz_operator7(M, m, w, g, True, True, 1)
+    assert abs(H1 - H7).max() <= 1e-13 * abs(H7).max()
+
+
+@pytest.mark.parametrize("nodes,beta", [((9, 7), 0.3), ((9, 7), 2.0 / 3.0), ((6, 5, 4), (0.5, 0.25))])
+def test_ho_laplacian_and_mass_invariants(nodes, beta):
+    """Spread Laplacian: symmetric, kills constants, 9-point (2-D) / 19-point
+    (3-D) interior rows; averaging mass rows sum to 1."""
+    M, m, g, w = small_case(nodes, seed=10)
+    Lap, Mm = hr.get_spread_nodal_laplacian_and_mass(M, beta)
+    N = Lap.shape[0]
+    assert abs(Lap - Lap.T).max() <= 1e-12 * abs(Lap).max()
+    assert np.abs(Lap @ np.ones(N)).max() <= 1e-10 * abs(Lap).max()
+    assert np.allclose(Mm @ np.ones(N), 1.0, atol=1e-14)
+    counts = np.diff(Lap.tocsr().indptr)
+    assert counts.max() == (9 if M.dim == 2 else 19)
+    # the mass term multiplies before it averages: H = Lap + M diag(mass)
+    H = hr.get_helmholtz_operator_ho(M, m, w, g, True, True, beta)
+    mass = hr.helmholtz_mass(M, m, w, g, True, True)
+    x = np.random.default_rng(1).standard_normal(N) + 0j
+    assert np.allclose(H @ x, Lap @ x + Mm @ (mass * x), rtol=1e-13, atol=0)
