@@ -47,6 +47,24 @@ mutable struct HHOperator
         finalizer(o -> ccall((:hh_op_destroy, LIB), Cint, (Ptr{Cvoid},), o.ptr), op)
         return op
     end
+    HHOperator(ptr::Ptr{Cvoid}) = new(ptr)
+end
+
+# GetHelmholtzOperatorHO(Msh, m, omega, gamma, NeumannAtFirstDim, Sommerfeld, beta)
+# (src/GetHelmholtz.jl:54-72): the spread operator; beta scalar (2-D) or [beta_lap; beta_mass] (3-D)
+function HHOperatorHO(nnodes::Vector{Int}, h::Vector{Float64}, m::Vector{Float64}, gamma::Vector{Float64},
+                      omega::Union{Float64,ComplexF64}, neumann::Bool, sommerfeld::Bool, beta=1.0)
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    nn = Int64[nnodes; ones(Int64, 3 - length(nnodes))]
+    hh = Float64[h; ones(3 - length(h))]
+    bl, bm = length(beta) == 2 ? (beta[1], beta[2]) : (beta[1], beta[1])
+    GC.@preserve m gamma check(ccall((:hh_op_create_ho, LIB), Cint,
+        (Cint, Ptr{Int64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Float64, Float64, Cint, Cint, Float64,
+         Float64, Ptr{Ptr{Cvoid}}),
+        length(nnodes), nn, hh, m, gamma, real(omega), imag(omega), neumann, sommerfeld, bl, bm, out))
+    op = HHOperator(out[])
+    finalizer(o -> ccall((:hh_op_destroy, LIB), Cint, (Ptr{Cvoid},), o.ptr), op)
+    return op
 end
 
 # Y = (H + i*shift*w^2*diag(m)) X  — the product of getHelmholtzFun (src/GetHelmholtz.jl:85-95)
