@@ -194,6 +194,10 @@ struct Level {
   void* b = nullptr;      // right-hand side of this level (levels >= 1)
   void* r = nullptr;      // residual scratch
   void* cur = nullptr;    // buffer holding the current iterate
+  // K-cycle (levels that receive a Krylov-accelerated correction): c1, A c1, scalars
+  void* kc1 = nullptr;
+  void* kv1 = nullptr;
+  cx<double>* ksc = nullptr;
 };
 
 }  // namespace
@@ -222,6 +226,7 @@ struct hh_solver {
   KrylovWork kw{};
   int* skip_copy = nullptr;
   cx<double>* bsc = nullptr;  // BiCGSTAB per-RHS scalars (rho, alpha, omega, beta)
+  cx<double>* kpart = nullptr; // K-cycle partial inner products
   int* conv_d = nullptr;
   double* n2tmp = nullptr;
   double* res_d = nullptr;
@@ -268,9 +273,12 @@ struct hh_solver {
     drop_graphs();
     tuned_for = 0;
     for (auto& L : lv) {
-      dfree(L.xa); dfree(L.xb); dfree(L.b); dfree(L.r);
-      L.xa = L.xb = L.b = L.r = L.cur = nullptr;
+      dfree(L.xa); dfree(L.xb); dfree(L.b); dfree(L.r); dfree(L.kc1); dfree(L.kv1); dfree(L.ksc);
+      L.xa = L.xb = L.b = L.r = L.cur = L.kc1 = L.kv1 = nullptr;
+      L.ksc = nullptr;
     }
+    dfree(kpart);
+    kpart = nullptr;
     dfree(V); dfree(Z); dfree(xk); dfree(bk); dfree(wk); dfree(b0);
     V = nullptr; Z = nullptr; xk = bk = wk = nullptr; b0 = nullptr;
     dfree(kw.part); dfree(kw.partn); dfree(kw.h1); dfree(kw.h2); dfree(kw.nrm); dfree(kw.H); dfree(kw.cs);
@@ -419,7 +427,14 @@ void ensure_work(hh_solver* S, int nrhs) {
     L.xb = dalloc<char>(nb);
     if (l > 0) L.b = dalloc<char>(nb);
     L.r = dalloc<char>(nb);
+    // K-cycle: levels 1 .. levels-2 receive the two-step Krylov correction
+    if (S->p.cycle == 'K' && l >= 1 && l + 1 < S->lv.size()) {
+      L.kc1 = dalloc<char>(nb);
+      L.kv1 = dalloc<char>(nb);
+      L.ksc = dalloc<cx<double>>((size_t)4 * nrhs);
+    }
   }
+  if (S->p.cycle == 'K') S->kpart = dalloc<cx<double>>((size_t)64 * 2 * nrhs);
   const int64_t N = S->lv[0].g.N;
   const int K = S->p.inner;
   const size_t blk = (size_t)N * nrhs;
@@ -547,7 +562,8 @@ struct Cycle {
   // fused legs: 2-D mirror-ghost levels with a coarser level below and V/W(2,2)
   bool use_fused(const Level& L) const {
     const size_t l = &L - &S->lv[0];
-    return S->p.fused && l + 1 < S->lv.size() && S->p.pre == 2 && S->p.post == 2 && fused_ok(L.g);
+    return S->p.fused && S->p.cycle != 'K' && l + 1 < S->lv.size() && S->p.pre == 2 && S->p.post == 2 &&
+           fused_ok(L.g);
   }
   int rows_of(const Level& L) const {
     const size_t l = &L - &S->lv[0];
@@ -561,6 +577,39 @@ struct Cycle {
                                         mk<double>(0, 0), S->p.relax, skip, st, rows_of(L));
       L.cur = o;
     }
+  }
+
+  // K-cycle coarse correction on level lc (right-hand side in C.b, result in
+  // C.cur): two minimal-residual (GCR) steps preconditioned by the cycle on
+  // level lc; C.b is overwritten by the second step's residual, C.r holds A c2.
+  void krylov_correction(Level& C, int lc) {
+    const int64_t N = C.g.N;
+    const size_t bytes = sizeof(cx<T>) * N * nrhs;
+    auto A = [&](const void* x, void* y) {
+      launch_stencil<T, T, T, M_APPLY>(C.g, nrhs, (const cx<T>*)x, nullptr, (cx<T>*)y, (const cx<T>*)C.d, nullptr,
+                                       mk<double>(0, 0), 0.0, skip, st, rows_of(C));
+    };
+    cx<T>* rb = (cx<T>*)C.b;
+    cx<T>* c1 = (cx<T>*)C.kc1;
+    cx<T>* v1 = (cx<T>*)C.kv1;
+    C.cur = nullptr;
+    run(lc, rb, true, nullptr);
+    HIPCK(hipMemcpyAsync(c1, C.cur, bytes, hipMemcpyDeviceToDevice, st));
+    A(c1, v1);
+    kc_dots<T>(N, nrhs, v1, rb, v1, v1, 2, S->kpart, skip, st);
+    kc_scalar(N, nrhs, S->kpart, 1, C.ksc, skip, st);
+    kc_update<T>(N, nrhs, 0, C.ksc, rb, v1, nullptr, nullptr, skip, st);  // r2 = rc - a1 v1
+    C.cur = nullptr;
+    run(lc, rb, true, nullptr);
+    cx<T>* c2 = (cx<T>*)C.cur;
+    cx<T>* v2 = (cx<T>*)C.r;
+    A(c2, v2);
+    kc_dots<T>(N, nrhs, v1, v2, nullptr, nullptr, 1, S->kpart, skip, st);
+    kc_scalar(N, nrhs, S->kpart, 2, C.ksc, skip, st);
+    kc_update<T>(N, nrhs, 1, C.ksc, v2, v1, c2, c1, skip, st);  // v2 -= g v1, c2 -= g c1
+    kc_dots<T>(N, nrhs, v2, rb, v2, v2, 2, S->kpart, skip, st);
+    kc_scalar(N, nrhs, S->kpart, 3, C.ksc, skip, st);
+    kc_update<T>(N, nrhs, 2, C.ksc, c2, c1, nullptr, nullptr, skip, st);  // e = a1 c1 + a2 c2
   }
 
   // Solve approximately A_l x = b; zero_init: start from x = 0 (then L.cur is
@@ -639,6 +688,20 @@ struct Cycle {
       sweeps(L, b, pre);
     }
     Level& C = S->lv[l + 1];
+    if (S->p.cycle == 'K') {
+      launch_stencil<T, T, T, M_RESID>(L.g, nrhs, (const cx<T>*)L.cur, b, (cx<T>*)L.r, (const cx<T>*)L.d, nullptr,
+                                       mk<double>(0, 0), 0.0, skip, st, rows_of(L));
+      launch_restrict<T>(L.g, C.g, nrhs, (const cx<T>*)L.r, (cx<T>*)C.b, skip, st);
+      if (l + 2 == nl) {  // the coarse level is solved exactly: one step
+        C.cur = nullptr;
+        run(l + 1, (const cx<T>*)C.b, true, nullptr);
+      } else {
+        krylov_correction(C, l + 1);
+      }
+      launch_prolong_add<T>(L.g, C.g, nrhs, (const cx<T>*)C.cur, (cx<T>*)L.cur, skip, st);
+      sweeps(L, b, post);
+      return;
+    }
     // W: two coarse-grid corrections per visit, each from this level's
     // current residual (iterating the coarse cycle on the restricted residual
     // instead diverges with rediscretised coarse operators near kh ~ 2).
@@ -1159,7 +1222,7 @@ int hh_solver_create(hh_op* op, const hh_mg_params* p, const double* shift_per_l
     if (p->levels < 1 || p->levels > 16) throw HHError(HH_ERR_ARG, "levels must be in 1..16");
     if (p->inner < 1 || p->inner > 32) throw HHError(HH_ERR_ARG, "inner must be in 1..32");
     if (p->pre < 0 || p->post < 0 || p->pre > 64 || p->post > 64) throw HHError(HH_ERR_ARG, "bad sweep counts");
-    if (p->cycle != 'V' && p->cycle != 'W') throw HHError(HH_ERR_ARG, "cycle must be 'V' or 'W'");
+    if (p->cycle != 'V' && p->cycle != 'W' && p->cycle != 'K') throw HHError(HH_ERR_ARG, "cycle must be 'V', 'W' or 'K'");
     if (p->krylov != HH_FGMRES && p->krylov != HH_BICGSTAB) throw HHError(HH_ERR_ARG, "unknown krylov method");
     if (p->prec != HH_C128 && p->prec != HH_C64) throw HHError(HH_ERR_ARG, "prec must be HH_C128 or HH_C64");
     if (p->learned != 0) throw HHError(HH_ERR_UNSUPPORTED, "learned correction not available in this entry point");

@@ -54,6 +54,16 @@ void launch_ho_stencil(const Grid& g, double beta_lap, double beta_mass, int nrh
                        const cx<double>* b, cx<double>* y, const cx<double>* d, const double* m, cx<double> sfac,
                        const int* skip, hipStream_t s);
 
+// ---- K-cycle coarse correction (hh_kcycle.hip); part: 64*2*nrhs, sc: 4*nrhs ----
+template <typename T>
+void kc_dots(int64_t N, int nrhs, const cx<T>* a0, const cx<T>* b0, const cx<T>* a1, const cx<T>* b1, int nd,
+             cx<double>* part, const int* skip, hipStream_t s);
+void kc_scalar(int64_t N, int nrhs, const cx<double>* part, int stage, cx<double>* sc, const int* skip,
+               hipStream_t s);
+template <typename T>
+void kc_update(int64_t N, int nrhs, int mode, const cx<double>* sc, cx<T>* y, const cx<T>* x, cx<T>* y2,
+               const cx<T>* x2, const int* skip, hipStream_t s);
+
 // ---- fused 2-D multigrid legs (hh_fused.hip): BC = 2, 2 pre/2 post sweeps ----
 bool fused_ok(const Grid& gf);
 template <typename T>

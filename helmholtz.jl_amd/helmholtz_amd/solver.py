@@ -128,8 +128,8 @@ def _validate(s: ShiftedLaplacianMultigridSolver):
         raise ValueError(f"Krylov={s.Krylov!r}: 'GMRES' (flexible GMRES) or 'BiCGSTAB'")
     if mg.relaxType != "Jac":
         raise NotImplementedError(f"relaxType={mg.relaxType!r}: damped Jacobi ('Jac') only")
-    if str(mg.cycleType) not in ("V", "W"):
-        raise NotImplementedError(f"cycleType={mg.cycleType!r}: 'V' and 'W' only")
+    if str(mg.cycleType) not in ("V", "W", "K"):
+        raise NotImplementedError(f"cycleType={mg.cycleType!r}: 'V', 'W' and 'K' only")
     if callable(mg.relaxPre) or callable(mg.relaxPost):
         raise NotImplementedError("per-level sweep counts (functions) are not supported")
     if mg.prec not in ("c128", "c64"):

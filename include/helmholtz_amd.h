@@ -49,7 +49,8 @@ typedef struct hh_mg_params {
   int32_t pre;        /* MG.relaxPre: damped-Jacobi sweeps before the correction */
   int32_t post;       /* MG.relaxPost                                            */
   double relax;       /* MG.relaxParam: Jacobi weight                            */
-  int32_t cycle;      /* MG.cycleType: 'V' or 'W'                                */
+  int32_t cycle;      /* MG.cycleType: 'V', 'W' or 'K' (Krylov-accelerated: two
+                         minimal-residual steps per coarse visit)              */
   int32_t max_outer;  /* MG.maxOuterIter: FGMRES restarts                        */
   double tol;         /* MG.relativeTol: ||b - H x|| <= tol ||b||                */
   int32_t inner;      /* solver.inner: FGMRES restart length (1..32)             */

@@ -120,20 +120,54 @@ def jacobi(lev: Level, x, b, w, sweeps):
     return x
 
 
+def kcycle_correction(hier, l, rc, w, pre, post, wlevels):
+    """Coarse correction of the K-cycle (Notay & Vassilevski, Numer. Linear
+    Algebra Appl. 15, 2008): two steps of a minimal-residual (GCR) iteration
+    on A_{l+1} e = rc, each preconditioned by one K-cycle on level l+1.
+    c1 = B rc, v1 = A c1, a1 = <v1, rc>/<v1, v1>, r2 = rc - a1 v1;
+    c2 = B r2, v2 = A c2 made orthogonal to v1 (c2 alongside),
+    a2 = <v2, r2>/<v2, v2>; e = a1 c1 + a2 c2 (a zero <v, v> gives a zero step)."""
+    A = hier[l + 1].A
+    c1 = vcycle(hier, l + 1, rc, w, pre, post, "K", wlevels)
+    v1 = A @ c1
+    n1 = np.vdot(v1, v1).real
+    a1 = np.vdot(v1, rc) / n1 if n1 > 0 else 0.0
+    r2 = rc - a1 * v1
+    c2 = vcycle(hier, l + 1, r2, w, pre, post, "K", wlevels)
+    v2 = A @ c2
+    g = np.vdot(v1, v2) / n1 if n1 > 0 else 0.0
+    v2 = v2 - g * v1
+    c2 = c2 - g * c1
+    n2 = np.vdot(v2, v2).real
+    a2 = np.vdot(v2, r2) / n2 if n2 > 0 else 0.0
+    return a1 * c1 + a2 * c2
+
+
 def vcycle(hier, l, b, w=0.8, pre=2, post=2, cycle="V", wlevels=0):
-    """One V- or W-cycle for A_l x = b from x = 0 (``wlevels`` > 0: the W
+    """One V-, W- or K-cycle for A_l x = b from x = 0 (``wlevels`` > 0: the W
     double correction only on levels < wlevels, single below).
 
     W: the coarse-grid correction is applied twice per visit, the second time
     from the residual recomputed on this level.  With rediscretised (not
     Galerkin) coarse operators this differs from iterating the coarse cycle
     on the restricted residual; near kh ~ 2 the coarse two-grid cycle
-    diverges, and only the fine-residual form stays convergent."""
+    diverges, and only the fine-residual form stays convergent.
+    K: the coarse correction is ``kcycle_correction`` (below the next-to-last
+    level; there the coarse solve is exact and one step suffices)."""
     lev = hier[l]
     if l == len(hier) - 1:
         return lev.lu.solve(b)
     x = np.zeros_like(b)
     x = jacobi(lev, x, b, w, pre)
+    if cycle == "K":
+        r = b - lev.A @ x
+        rc = lev.R @ r
+        if l + 1 == len(hier) - 1:
+            xc = hier[l + 1].lu.solve(rc)
+        else:
+            xc = kcycle_correction(hier, l, rc, w, pre, post, wlevels)
+        x = x + lev.P @ xc
+        return jacobi(lev, x, b, w, post)
     for _ in range(2 if (cycle == "W" and (wlevels <= 0 or l < wlevels)) else 1):
         r = b - lev.A @ x
         xc = vcycle(hier, l + 1, lev.R @ r, w, pre, post, cycle, wlevels)

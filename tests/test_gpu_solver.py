@@ -261,7 +261,7 @@ def test_uneven_stream_matches_solo_solves():
         assert np.array_equal(X, Xs.reshape(X.shape, order="F"))
 
 
-@pytest.mark.parametrize("cycle,wlevels", [("V", 0), ("W", 0), ("W", 2)])
+@pytest.mark.parametrize("cycle,wlevels", [("V", 0), ("W", 0), ("W", 2), ("K", 0)])
 @pytest.mark.parametrize("fused", [True, False])
 def test_one_cycle_matches_oracle(cycle, wlevels, fused):
     """One preconditioner application (complex128) equals the oracle's cycle
@@ -463,3 +463,25 @@ def test_ho_operator_solve(krylov, prec):
         assert err <= 1e-4
     with pytest.raises(ha.HelmholtzError):
         ha.solveLinearSystem(Hg, B, s, 1)
+
+
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+def test_kcycle_solve_matches_oracle(prec):
+    """cycleType = 'K' (test/ShiftedLaplacianTest.jl:139-142,
+    examples/PointSourceADR/runExperiments.jl:118): the Krylov-accelerated
+    coarse correction, 4 levels; iteration count == oracle +-1 (complex128),
+    true residual for both precisions."""
+    M, m, g, w, H, q = c1_problem()
+    s = _solver(M, m, g, w, False, 4, 0.5, cycle="K", prec=prec)
+    x, s = ha.solveLinearSystem(None, q, s)
+    res = np.linalg.norm(H @ x - q) / np.linalg.norm(q)
+    p = hr.HelmholtzParam(M, g, m, w, False, True)
+    hier = sr.build_hierarchy(p, 4, 0.5)
+    _, ito, _ = sr.fgmres(lambda v: H @ v, q, lambda v: sr.vcycle(hier, 0, v, 0.8, 2, 2, "K"), 10, 100, 1e-6)
+    it = s.last_stats["iters"][0]
+    print(f"K-cycle {prec}: gpu iters {it} oracle {ito} res {res:.2e}")
+    assert res <= 1e-6
+    if prec == "c128":
+        assert abs(it - ito) <= 1
+    else:
+        assert abs(it - ito) <= 3

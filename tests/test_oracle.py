@@ -399,3 +399,41 @@ def test_ho_laplacian_and_mass_invariants(nodes, beta):
     mass = hr.helmholtz_mass(M, m, w, g, True, True)
     x = np.random.default_rng(1).standard_normal(N) + 0j
     assert np.allclose(H @ x, Lap @ x + Mm @ (mass * x), rtol=1e-13, atol=0)
+
+
+def test_kcycle_correction_is_minimal_residual():
+    """The K-cycle's coarse correction e minimises ||rc - A e|| over span{c1, c2}
+    (the two coarse cycles it applies): the residual is orthogonal to A c1 and
+    A c2, and random combinations do no better."""
+    M, m, g, w = small_case((65, 33), seed=11)
+    p = hr.HelmholtzParam(M, g, m, w, True, True)
+    hier = sr.build_hierarchy(p, 4, 0.5)
+    rng = np.random.default_rng(5)
+    rc = rng.standard_normal(hier[1].A.shape[0]) + 1j * rng.standard_normal(hier[1].A.shape[0])
+    e = sr.kcycle_correction(hier, 0, rc, 0.8, 2, 2, 0)
+    A = hier[1].A
+    c1 = sr.vcycle(hier, 1, rc, 0.8, 2, 2, "K")
+    v1 = A @ c1
+    a1 = np.vdot(v1, rc) / np.vdot(v1, v1).real
+    c2 = sr.vcycle(hier, 1, rc - a1 * v1, 0.8, 2, 2, "K")
+    res = rc - A @ e
+    for c in (c1, c2):
+        Ac = A @ c
+        assert abs(np.vdot(Ac, res)) <= 1e-10 * np.linalg.norm(Ac) * np.linalg.norm(res)
+    for _ in range(20):
+        al, be = rng.standard_normal(2) + 1j * rng.standard_normal(2)
+        assert np.linalg.norm(res) <= np.linalg.norm(rc - A @ (al * c1 + be * c2)) + 1e-12
+
+
+def test_kcycle_converges_like_w_and_beats_v():
+    M, m, g, w = small_case((129, 65), seed=12)
+    p = hr.HelmholtzParam(M, g, m, w, True, True)
+    H = hr.operator_from_param(p)
+    hier = sr.build_hierarchy(p, 4, 0.5)
+    b = np.zeros(m.size, complex)
+    b[64 + 129 * 2] = 1.0
+    its = {}
+    for cyc in ("V", "W", "K"):
+        x, its[cyc], _ = sr.fgmres(lambda v: H @ v, b, lambda v: sr.vcycle(hier, 0, v, 0.8, 2, 2, cyc), 10, 80, 1e-6)
+        assert np.linalg.norm(H @ x - b) <= 1e-6 * np.linalg.norm(b)
+    assert its["K"] < its["V"]
