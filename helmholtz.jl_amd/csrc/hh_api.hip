@@ -559,11 +559,14 @@ struct Cycle {
   int ncorr_at(int l) const {
     return (S->p.cycle == 'W' && (S->p.wlevels <= 0 || l < S->p.wlevels)) ? 2 : 1;
   }
+  // damped-Jacobi sweeps before / after the coarse correction on level l
+  int pre_at(int l) const { return S->p.per_level ? S->p.pre_l[l] : S->p.pre; }
+  int post_at(int l) const { return S->p.per_level ? S->p.post_l[l] : S->p.post; }
   // fused legs: 2-D mirror-ghost levels with a coarser level below and V/W(2,2)
   bool use_fused(const Level& L) const {
     const size_t l = &L - &S->lv[0];
-    return S->p.fused && S->p.cycle != 'K' && l + 1 < S->lv.size() && S->p.pre == 2 && S->p.post == 2 &&
-           fused_ok(L.g);
+    return S->p.fused && S->p.cycle != 'K' && l + 1 < S->lv.size() && pre_at((int)l) == 2 &&
+           post_at((int)l) == 2 && fused_ok(L.g);
   }
   int rows_of(const Level& L) const {
     const size_t l = &L - &S->lv[0];
@@ -643,7 +646,7 @@ struct Cycle {
       coarse_apply<T>(S, L.g.N, nrhs, b, (cx<T>*)L.cur, skip, st);
       return;
     }
-    const int pre = S->p.pre, post = S->p.post;
+    const int pre = pre_at(l), post = post_at(l);
     if (use_fused(L) && zero_init) {
       Level& C = S->lv[l + 1];
       const int ncorr = ncorr_at(l);
@@ -728,11 +731,11 @@ struct Cycle {
     }
     // number of buffer writes on level 0: pre (incl. the zero-start sweep) + post,
     // or for the fused legs down + (corrections - 1) + up
-    const int flips = use_fused(L0) ? ncorr_at(0) + 1 : S->p.pre + S->p.post;
+    const int flips = use_fused(L0) ? ncorr_at(0) + 1 : pre_at(0) + post_at(0);
     // choose the ping-pong pair (out, xa) so the last write lands in out
     void* saved_xb = L0.xb;
     L0.xb = out;
-    void* first = (S->p.pre == 0) ? (void*)out : ((flips % 2 == 1) ? (void*)out : L0.xa);
+    void* first = (pre_at(0) == 0) ? (void*)out : ((flips % 2 == 1) ? (void*)out : L0.xa);
     run(0, b, true, first);
     if (L0.cur != out)
       HIPCK(hipMemcpyAsync(out, L0.cur, sizeof(cx<T>) * L0.g.N * nrhs, hipMemcpyDeviceToDevice, st));
@@ -1222,6 +1225,10 @@ int hh_solver_create(hh_op* op, const hh_mg_params* p, const double* shift_per_l
     if (p->levels < 1 || p->levels > 16) throw HHError(HH_ERR_ARG, "levels must be in 1..16");
     if (p->inner < 1 || p->inner > 32) throw HHError(HH_ERR_ARG, "inner must be in 1..32");
     if (p->pre < 0 || p->post < 0 || p->pre > 64 || p->post > 64) throw HHError(HH_ERR_ARG, "bad sweep counts");
+    if (p->per_level)
+      for (int l = 0; l < p->levels; ++l)
+        if (p->pre_l[l] < 0 || p->post_l[l] < 0 || p->pre_l[l] > 64 || p->post_l[l] > 64)
+          throw HHError(HH_ERR_ARG, "bad per-level sweep counts");
     if (p->cycle != 'V' && p->cycle != 'W' && p->cycle != 'K') throw HHError(HH_ERR_ARG, "cycle must be 'V', 'W' or 'K'");
     if (p->krylov != HH_FGMRES && p->krylov != HH_BICGSTAB) throw HHError(HH_ERR_ARG, "unknown krylov method");
     if (p->prec != HH_C128 && p->prec != HH_C64) throw HHError(HH_ERR_ARG, "prec must be HH_C128 or HH_C64");

@@ -122,6 +122,15 @@ def clear(s: ShiftedLaplacianMultigridSolver):
     s.doClear = 0
 
 
+def _sweeps(mg: MGparam, l: int):
+    """Damped-Jacobi sweeps (pre, post) on level ``l`` (0 = finest).  relaxPre /
+    relaxPost are counts or functions of the 1-based level (finest = 1), as in
+    the reference's examples (examples/PointSourceADR/runExperiments.jl:121-122,
+    ``relaxPre = l->l+1``)."""
+    f = lambda v: int(v(l + 1)) if callable(v) else int(v)
+    return f(mg.relaxPre), f(mg.relaxPost)
+
+
 def _validate(s: ShiftedLaplacianMultigridSolver):
     mg = s.MG
     if s.Krylov not in ("GMRES", "BiCGSTAB"):
@@ -130,8 +139,12 @@ def _validate(s: ShiftedLaplacianMultigridSolver):
         raise NotImplementedError(f"relaxType={mg.relaxType!r}: damped Jacobi ('Jac') only")
     if str(mg.cycleType) not in ("V", "W", "K"):
         raise NotImplementedError(f"cycleType={mg.cycleType!r}: 'V', 'W' and 'K' only")
-    if callable(mg.relaxPre) or callable(mg.relaxPost):
-        raise NotImplementedError("per-level sweep counts (functions) are not supported")
+    if int(mg.levels) > 16:
+        raise ValueError("at most 16 levels")
+    for l in range(int(mg.levels)):
+        for c in _sweeps(mg, l):
+            if not 0 <= c <= 64:
+                raise ValueError(f"sweep count {c} on level {l + 1}: 0..64")
     if mg.prec not in ("c128", "c64"):
         raise ValueError("MG.prec must be 'c128' or 'c64'")
 
@@ -155,7 +168,8 @@ def _ensure_handle(s: ShiftedLaplacianMultigridSolver, outer=None):
             s._op_key = key
     if s._handle is None:
         mg = s.MG
-        prm = L.MGParams(levels=int(mg.levels), pre=int(mg.relaxPre), post=int(mg.relaxPost),
+        sw = [_sweeps(mg, l) for l in range(int(mg.levels))]
+        prm = L.MGParams(levels=int(mg.levels), pre=sw[0][0], post=sw[0][1],
                          relax=float(mg.relaxParam), cycle=ord(str(mg.cycleType)), max_outer=int(mg.maxOuterIter),
                          tol=float(mg.relativeTol), inner=int(s.inner),
                          krylov=L.HH_BICGSTAB if s.Krylov == "BiCGSTAB" else L.HH_FGMRES,
@@ -163,6 +177,10 @@ def _ensure_handle(s: ShiftedLaplacianMultigridSolver, outer=None):
                          verbose=int(bool(s.verbose)), graph=int(bool(mg.graph)),
                          autotune=int(bool(mg.autotune)), fused=int(bool(mg.fused)),
                          wlevels=int(mg.wlevels))
+        if callable(mg.relaxPre) or callable(mg.relaxPost):
+            prm.per_level = 1
+            for l, (npre, npost) in enumerate(sw):
+                prm.pre_l[l], prm.post_l[l] = npre, npost
         shifts = np.ascontiguousarray(np.broadcast_to(np.asarray(s.shift, dtype=float), (int(mg.levels),)))
         h = C.c_void_p()
         L.check(L.lib().hh_solver_create(s._op.ptr, C.byref(prm), shifts.ctypes.data_as(C.POINTER(C.c_double)),

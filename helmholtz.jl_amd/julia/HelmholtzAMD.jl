@@ -22,7 +22,12 @@ struct HHMGParams
     levels::Int32; pre::Int32; post::Int32; relax::Float64; cycle::Int32; max_outer::Int32
     tol::Float64; inner::Int32; krylov::Int32; prec::Int32; learned::Int32; verbose::Int32
     graph::Int32; autotune::Int32; fused::Int32; wlevels::Int32
+    per_level::Int32; pre_l::NTuple{16,Int32}; post_l::NTuple{16,Int32}
 end
+
+# relaxPre / relaxPost: a count or a function of the level (finest = 1)
+sweeps_at(v, l) = v isa Function ? Int32(v(l)) : Int32(v)
+per_level_sweeps(v, levels) = ntuple(l -> l <= levels ? sweeps_at(v, l) : Int32(0), 16)
 
 mutable struct HHSolveStats
     iters::Ptr{Int32}; status::Ptr{Int32}; true_res::Ptr{Float64}
@@ -106,9 +111,12 @@ function ensure_handle!(s::ShiftedLaplacianMultigridSolverAMD)
     end
     if s.handle == C_NULL
         mg = s.MG
-        prm = Ref(HHMGParams(mg.levels, mg.relaxPre, mg.relaxPost, mg.relaxParam, Int32(mg.cycleType),
-                             mg.maxOuterIter, mg.relativeTol, s.inner,
-                             s.Krylov == "BiCGSTAB" ? HH_BICGSTAB : HH_FGMRES, s.prec, 0, s.verbose, 0, 0, 1, 0))
+        pl = mg.relaxPre isa Function || mg.relaxPost isa Function
+        prm = Ref(HHMGParams(mg.levels, sweeps_at(mg.relaxPre, 1), sweeps_at(mg.relaxPost, 1), mg.relaxParam,
+                             Int32(mg.cycleType), mg.maxOuterIter, mg.relativeTol, s.inner,
+                             s.Krylov == "BiCGSTAB" ? HH_BICGSTAB : HH_FGMRES, s.prec, 0, s.verbose, 0, 0, 1, 0,
+                             pl ? 1 : 0, per_level_sweeps(mg.relaxPre, mg.levels),
+                             per_level_sweeps(mg.relaxPost, mg.levels)))
         out = Ref{Ptr{Cvoid}}(C_NULL)
         check(ccall((:hh_solver_create, LIB), Cint, (Ptr{Cvoid}, Ref{HHMGParams}, Ptr{Float64}, Ptr{Ptr{Cvoid}}),
                     s.op.ptr, prm, s.shift, out))

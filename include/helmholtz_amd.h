@@ -64,6 +64,10 @@ typedef struct hh_mg_params {
                          (mirror ghosts, 2 pre / 2 post sweeps); 0: one kernel per step */
   int32_t wlevels;    /* with cycle 'W': only levels 0..wlevels-1 apply the double
                          correction, coarser levels a single one (0 = every level) */
+  int32_t per_level;  /* 1: sweep counts per level from pre_l/post_l (MG.relaxPre
+                         and relaxPost given as functions of the level) */
+  int32_t pre_l[16];  /* per_level: sweeps before the correction on level l (0 = finest) */
+  int32_t post_l[16]; /* per_level: sweeps after it */
 } hh_mg_params;
 
 /* Per-call results.  The arrays are caller-owned with nrhs entries, or NULL. */

@@ -143,9 +143,21 @@ def kcycle_correction(hier, l, rc, w, pre, post, wlevels):
     return a1 * c1 + a2 * c2
 
 
+def sweeps_at(v, l):
+    """Sweep count on level ``l`` (0 = finest): an int, a sequence indexed by
+    level, or a function of the 1-based level as Multigrid.jl's relaxPre /
+    relaxPost (examples/PointSourceADR/runExperiments.jl:121-122)."""
+    if callable(v):
+        return int(v(l + 1))
+    if np.ndim(v):
+        return int(v[l])
+    return int(v)
+
+
 def vcycle(hier, l, b, w=0.8, pre=2, post=2, cycle="V", wlevels=0):
     """One V-, W- or K-cycle for A_l x = b from x = 0 (``wlevels`` > 0: the W
-    double correction only on levels < wlevels, single below).
+    double correction only on levels < wlevels, single below; ``pre`` /
+    ``post`` per level through ``sweeps_at``).
 
     W: the coarse-grid correction is applied twice per visit, the second time
     from the residual recomputed on this level.  With rediscretised (not
@@ -157,8 +169,9 @@ def vcycle(hier, l, b, w=0.8, pre=2, post=2, cycle="V", wlevels=0):
     lev = hier[l]
     if l == len(hier) - 1:
         return lev.lu.solve(b)
+    npre, npost = sweeps_at(pre, l), sweeps_at(post, l)
     x = np.zeros_like(b)
-    x = jacobi(lev, x, b, w, pre)
+    x = jacobi(lev, x, b, w, npre)
     if cycle == "K":
         r = b - lev.A @ x
         rc = lev.R @ r
@@ -167,12 +180,12 @@ def vcycle(hier, l, b, w=0.8, pre=2, post=2, cycle="V", wlevels=0):
         else:
             xc = kcycle_correction(hier, l, rc, w, pre, post, wlevels)
         x = x + lev.P @ xc
-        return jacobi(lev, x, b, w, post)
+        return jacobi(lev, x, b, w, npost)
     for _ in range(2 if (cycle == "W" and (wlevels <= 0 or l < wlevels)) else 1):
         r = b - lev.A @ x
         xc = vcycle(hier, l + 1, lev.R @ r, w, pre, post, cycle, wlevels)
         x = x + lev.P @ xc
-    x = jacobi(lev, x, b, w, post)
+    x = jacobi(lev, x, b, w, npost)
     return x
 
 

@@ -282,6 +282,47 @@ def test_one_cycle_matches_oracle(cycle, wlevels, fused):
         assert err <= 1e-12, err
 
 
+@pytest.mark.parametrize("cycle", ["V", "W", "K"])
+@pytest.mark.parametrize("pre,post", [(lambda l: l + 1, lambda l: l + 1), (lambda l: 2, lambda l: 3 - l % 2),
+                                      (lambda l: 0, lambda l: 3)])
+def test_per_level_sweeps_match_oracle(cycle, pre, post):
+    """relaxPre / relaxPost as functions of the level (finest = 1), as in
+    examples/PointSourceADR/runExperiments.jl:121-122: one complex128 cycle
+    equals the oracle's; levels whose counts are 2/2 take the fused legs,
+    the others one kernel per step.  Bar 1e-12 relative."""
+    M, m, g, w = small_case((129, 97), seed=65, neumann=True)
+    rng = np.random.default_rng(66)
+    V = np.asfortranarray(rng.standard_normal((m.size, 2)) + 1j * rng.standard_normal((m.size, 2)))
+    s = _solver(M, m, g, w, True, 4, 0.5, cycle=cycle)
+    s.MG.relaxPre, s.MG.relaxPost = pre, post
+    Z = ha.precondition(s, V)
+    p = hr.HelmholtzParam(M, g, m, w, True, True)
+    hier = sr.build_hierarchy(p, 4, 0.5)
+    for c in range(2):
+        zo = sr.vcycle(hier, 0, V[:, c], 0.8, pre, post, cycle)
+        err = np.linalg.norm(Z[:, c] - zo) / np.linalg.norm(zo)
+        assert err <= 1e-12, err
+
+
+def test_per_level_sweeps_solve():
+    """A complex64-preconditioned solve with relaxPre = relaxPost = l -> l+1
+    reaches the tolerance in the oracle's iteration count (within 2)."""
+    M, m, g, w = small_case((257, 129), seed=67, neumann=True)
+    p = hr.HelmholtzParam(M, g, m, w, True, True)
+    H = hr.operator_from_param(p)
+    q = np.zeros(m.size, complex)
+    q[128 + 257 * 3] = 1.0
+    s = _solver(M, m, g, w, True, 4, 0.5, cycle="W", prec="c64")
+    s.MG.relaxPre = s.MG.relaxPost = (lambda l: l + 1)
+    x, s = ha.solveLinearSystem(None, q, s)
+    assert np.linalg.norm(H @ x - q) / np.linalg.norm(q) <= 1e-6
+    hier = sr.build_hierarchy(p, 4, 0.5)
+    f = lambda l: l + 1
+    xo, ito, _ = sr.fgmres(lambda v: H @ v, q, lambda v: sr.vcycle(hier, 0, v, 0.8, f, f, "W"), 10, 100, 1e-6)
+    print(f"per-level sweeps: gpu iters {s.last_stats['iters'][0]} oracle {ito}")
+    assert abs(int(s.last_stats["iters"][0]) - ito) <= 2
+
+
 @pytest.mark.parametrize("nodes", [(257, 129), (129, 65), (97, 33), (1025, 513)])
 @pytest.mark.parametrize("prec", ["c64", "c128"])
 def test_fused_legs_equal_unfused(nodes, prec):

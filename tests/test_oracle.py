@@ -437,3 +437,37 @@ def test_kcycle_converges_like_w_and_beats_v():
         x, its[cyc], _ = sr.fgmres(lambda v: H @ v, b, lambda v: sr.vcycle(hier, 0, v, 0.8, 2, 2, cyc), 10, 80, 1e-6)
         assert np.linalg.norm(H @ x - b) <= 1e-6 * np.linalg.norm(b)
     assert its["K"] < its["V"]
+
+
+@pytest.mark.parametrize("cycle", ["V", "W"])
+def test_per_level_sweeps_equal_error_propagation(cycle):
+    """Per-level sweep counts (relaxPre/relaxPost as functions of the level):
+    the cycle's action equals (I - E_0) A_0^-1 built from the error-propagation
+    matrices E_l = S_l^post_l (I - P_l B_{l+1} R_l A_l)^k S_l^pre_l, with
+    S_l = I - w D_l^-1 A_l, B_{l+1} = (I - E_{l+1}) A_{l+1}^-1, B_last = A_last^-1
+    and k = 1 (V) or 2 (W).  Unequal pre/post counts make a swapped pair fail."""
+    M, m, g, w = small_case((17, 13), seed=31)
+    p = hr.HelmholtzParam(M, g, m, w, True, True)
+    hier = sr.build_hierarchy(p, 3, 0.5)
+    pre, post = (lambda l: 2 * l + 1), (lambda l: 4 - l)
+    k = 2 if cycle == "W" else 1
+    B = np.linalg.inv(hier[-1].A.toarray())
+    for l in range(len(hier) - 2, -1, -1):
+        lev = hier[l]
+        A = lev.A.toarray()
+        I = np.eye(A.shape[0])
+        S = I - 0.8 * lev.Dinv[:, None] * A
+        T = I - lev.P.toarray() @ B @ lev.R.toarray() @ A
+        E = (np.linalg.matrix_power(S, post(l + 1)) @ np.linalg.matrix_power(T, k)
+             @ np.linalg.matrix_power(S, pre(l + 1)))
+        B = (I - E) @ np.linalg.inv(A)
+    rng = np.random.default_rng(32)
+    b = rng.standard_normal(m.size) + 1j * rng.standard_normal(m.size)
+    z = sr.vcycle(hier, 0, b, 0.8, pre, post, cycle)
+    assert np.linalg.norm(z - B @ b) <= 1e-10 * np.linalg.norm(z)
+    # the same counts as sequences indexed by level
+    z2 = sr.vcycle(hier, 0, b, 0.8, [3, 5, 7], [3, 2, 1], cycle)
+    assert np.array_equal(z, z2)
+    # the swapped pair is a different cycle
+    zs = sr.vcycle(hier, 0, b, 0.8, post, pre, cycle)
+    assert np.linalg.norm(zs - z) > 1e-6 * np.linalg.norm(z)
