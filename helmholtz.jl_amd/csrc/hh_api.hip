@@ -833,6 +833,13 @@ void run_fgmres(hh_solver* S, int nrhs, hh_solve_stats* stats) {
         // H z: reads z (8 or 16 B per value), writes w (16 B), shared c128 diagonal
         const bool ta = S->kt.match(KT_APPLY, 0);
         const double Nf = (double)N;
+        if (S->p.prec == HH_C64 && w.lazy && !S->op->ho && !ta) {
+          // H z formed inside the Arnoldi passes (w is never stored)
+          precondition<float>(S, nrhs, vj, zj, w.done, st, w.vscale + (int64_t)j * nrhs);
+          const HzArgs hz{S->op->g, (const cx<float>*)zj, S->op->d64};
+          kr_arnoldi_impl(w, S->V, S->wk, j, tol, S->skip_copy, &S->kt, active, st, &hz);
+          return;
+        }
         if (S->p.prec == HH_C64) {
           precondition<float>(S, nrhs, vj, zj, w.done, st, w.lazy ? w.vscale + (int64_t)j * nrhs : nullptr);
           if (ta) S->kt.begin(st);

@@ -77,6 +77,36 @@ __device__ __forceinline__ double lap_diag1(int i, int n, int bc) {
   return (bc == 1 && (i == 0 || i == n - 1)) ? 1.0 : 2.0;
 }
 
+// A x at one node of the 5-point (2-D) or 7-point (3-D) operator from the
+// node value xc, its neighbours along each axis and the diagonal dv:
+//   sum_d ih_d (2 xc - x_{-d} - x_{+d}) + dv xc
+// with a fixed operation order (explicit fused multiply-adds), so every
+// kernel that forms it -- the stencil launches, the fused multigrid legs and
+// the Arnoldi passes that form H z on the fly -- gets the same bits.
+template <typename T>
+__device__ __forceinline__ T ax_dxc_re(cx<T> dv, cx<T> xc) {
+  return fma(dv.re, xc.re, -(dv.im * xc.im));
+}
+template <typename T>
+__device__ __forceinline__ T ax_dxc_im(cx<T> dv, cx<T> xc) {
+  return fma(dv.re, xc.im, dv.im * xc.re);
+}
+template <typename T>
+__device__ __forceinline__ cx<T> apply5(cx<T> xc, cx<T> xl, cx<T> xr, cx<T> xm, cx<T> xp, T ih1, T ih2, cx<T> dv) {
+  const T a1r = ((T)2 * xc.re - xl.re) - xr.re, a1i = ((T)2 * xc.im - xl.im) - xr.im;
+  const T a2r = ((T)2 * xc.re - xm.re) - xp.re, a2i = ((T)2 * xc.im - xm.im) - xp.im;
+  return mk<T>(fma(ih1, a1r, ih2 * a2r) + ax_dxc_re(dv, xc), fma(ih1, a1i, ih2 * a2i) + ax_dxc_im(dv, xc));
+}
+template <typename T>
+__device__ __forceinline__ cx<T> apply7(cx<T> xc, cx<T> x1l, cx<T> x1r, cx<T> x2l, cx<T> x2r, cx<T> x3l,
+                                        cx<T> x3r, T ih1, T ih2, T ih3, cx<T> dv) {
+  const T a1r = ((T)2 * xc.re - x1l.re) - x1r.re, a1i = ((T)2 * xc.im - x1l.im) - x1r.im;
+  const T a2r = ((T)2 * xc.re - x2l.re) - x2r.re, a2i = ((T)2 * xc.im - x2l.im) - x2r.im;
+  const T a3r = ((T)2 * xc.re - x3l.re) - x3r.re, a3i = ((T)2 * xc.im - x3l.im) - x3r.im;
+  return mk<T>(fma(ih1, a1r, fma(ih2, a2r, ih3 * a3r)) + ax_dxc_re(dv, xc),
+               fma(ih1, a1i, fma(ih2, a2i, ih3 * a3i)) + ax_dxc_im(dv, xc));
+}
+
 // XCD-aware work order.  A launch is a 1-D grid of nxt * nyt * nrhs blocks;
 // the hardware hands block b to XCD b % 8, and each XCD has its own L2.  Block
 // b is given the (b / 8)-th item of a contiguous 1/8 slice of the work list

@@ -32,11 +32,6 @@ __device__ __forceinline__ int mirror_idx(int g, int n) {
   return g < 0 ? 0 : (g >= n ? n - 1 : g);
 }
 
-template <typename T>
-__device__ __forceinline__ cx<T> lap5(cx<T> xc, cx<T> xl, cx<T> xr, cx<T> xm, cx<T> xp, T ih1, T ih2) {
-  return ih1 * (((T)2 * xc) - xl - xr) + ih2 * (((T)2 * xc) - xm - xp);
-}
-
 // tile rows: 40 (complex64, 512 threads = 8 wavefronts, 5 rows each) or
 // 16 (complex128, 256 threads = 4 wavefronts, 4 rows each)
 template <typename T>
@@ -155,8 +150,7 @@ __device__ __forceinline__ void restrict_tile(const cx<T>* sr, const TileCtx& c,
 // A x at tile position q (row stride 64)
 template <typename T>
 __device__ __forceinline__ cx<T> apply_at(const cx<T>* s, int q, cx<T> dq, T ih1, T ih2) {
-  const cx<T> xc = s[q];
-  return lap5(xc, s[q - 1], s[q + 1], s[q - 64], s[q + 64], ih1, ih2) + dq * xc;
+  return apply5(s[q], s[q - 1], s[q + 1], s[q - 64], s[q + 64], ih1, ih2, dq);
 }
 
 // ------------------------------------------------------------------ down leg

@@ -162,11 +162,19 @@ void kr_norm2(const KrylovWork& w, const cx<double>* v, double* out, const int* 
 // flags (conv[b] != 0 marks an RHS already converged).  n2tmp: nrhs doubles.
 void kr_start_impl(const KrylovWork& w, const cx<double>* r, cx<double>* V0, double tol, const int* conv,
                    double* n2tmp, hipStream_t s);
+// w = H z formed inside the lazy Arnoldi passes: the outer 5/7-point
+// operator's grid and complex128 diagonal, and the complex64 z of this step
+struct HzArgs {
+  Grid g;
+  const cx<float>* z;
+  const cx<double>* d;
+};
 // One CGS2 Arnoldi step j: orthogonalise wv against V_0..V_j, write V_{j+1},
 // fill column j of H, update Givens rotations and residual estimates.
-// skip_copy: nrhs ints of scratch.
+// skip_copy: nrhs ints of scratch.  hz (lazy mode only, may be null): take
+// w = H z from hz instead of wv.
 void kr_arnoldi_impl(const KrylovWork& w, const cx<double>* V, cx<double>* wv, int j, double tol, int* skip_copy,
-                     KTimer* kt, int active, hipStream_t s);
+                     KTimer* kt, int active, hipStream_t s, const HzArgs* hz = nullptr);
 // y = R^-1 g per RHS, x += Z y (Z in TZ)
 template <typename TZ>
 void kr_update_x(const KrylovWork& w, const cx<TZ>* Z, cx<double>* x, hipStream_t s);

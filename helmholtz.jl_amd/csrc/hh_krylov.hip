@@ -71,11 +71,12 @@ __global__ __launch_bounds__(256) void k_dots(KrylovWork kw, const cx<double>* _
   }
 #pragma unroll
   for (int i = 0; i < NV; ++i) {
-    if (i > j) break;
-    const double si = kw.lazy ? kw.vscale[i * kw.nrhs + b] : 1.0;
-    const double re = block_sum(si * acc[i].re, sh);
-    const double im = block_sum(si * acc[i].im, sh);
-    if (threadIdx.x == 0) kw.part[((int64_t)c * (kw.kmax + 1) + i) * kw.nrhs + b] = mk<double>(re, im);
+    if (i <= j) {  // (no early exit: keeps acc[] in registers)
+      const double si = kw.lazy ? kw.vscale[i * kw.nrhs + b] : 1.0;
+      const double re = block_sum(si * acc[i].re, sh);
+      const double im = block_sum(si * acc[i].im, sh);
+      if (threadIdx.x == 0) kw.part[((int64_t)c * (kw.kmax + 1) + i) * kw.nrhs + b] = mk<double>(re, im);
+    }
   }
   if (NORM) {
     const double t = block_sum(nn, sh);
@@ -147,10 +148,11 @@ __global__ __launch_bounds__(256) void k_update_dots(KrylovWork kw, const cx<dou
   }
 #pragma unroll
   for (int i = 0; i < NV; ++i) {
-    if (i > j) break;
-    const double re = block_sum(acc[i].re, sh);
-    const double im = block_sum(acc[i].im, sh);
-    if (threadIdx.x == 0) kw.part[((int64_t)c * (kw.kmax + 1) + i) * kw.nrhs + b] = mk<double>(re, im);
+    if (i <= j) {
+      const double re = block_sum(acc[i].re, sh);
+      const double im = block_sum(acc[i].im, sh);
+      if (threadIdx.x == 0) kw.part[((int64_t)c * (kw.kmax + 1) + i) * kw.nrhs + b] = mk<double>(re, im);
+    }
   }
   const double t = block_sum(nn, sh);
   if (threadIdx.x == 0) kw.partn[(int64_t)c * kw.nrhs + b] = t;
@@ -179,6 +181,133 @@ __global__ __launch_bounds__(256) void k_update_lazy(KrylovWork kw, const cx<dou
       if (i <= j) wv = wv - hs[i] * V[i * stride + base + p];
     wout[base + p] = wv;
     nn += cabs2(wv);
+  }
+  const double t = block_sum(nn, sh);
+  if (threadIdx.x == 0) kw.partn[(int64_t)c * kw.nrhs + b] = t;
+}
+
+// ---- lazy passes 1 and 2 with w = H z formed on the fly (complex64 z) ----
+// w is never stored: each pass recomputes it from z (8 B per value) and the
+// shared diagonal instead of reading a stored complex128 w (16 B) that a
+// separate stencil launch wrote (16 B).  Blocks are ordered RHS-fastest so the
+// blocks of one node chunk run together and share the diagonal in L2.
+
+// (H z)_p on a 2-D or 3-D grid: 5/7-point operator with mirror (BC 2) or
+// self (BC 1) ghosts and complex128 diagonal d, in float64 (apply5/apply7, as
+// in the stencil kernels' APPLY mode, so the values equal a stored H z bit for
+// bit).  zb: this RHS's block of z.
+__device__ __forceinline__ cx<double> hz_at(const Grid& g, const cx<float>* __restrict__ zb,
+                                            const cx<double>* __restrict__ d, int ci, int cj, int ck, int64_t p) {
+  const int n1 = g.n1, n2 = g.n2;
+  const cx<double> xc = cvt<float, double>(zb[p]);
+  if (g.dim == 2) {
+    const int64_t row = (int64_t)cj * n1;
+    const cx<double> xl = cvt<float, double>(zb[row + lo_nb(ci, g.bc)]);
+    const cx<double> xr = cvt<float, double>(zb[row + hi_nb(ci, n1, g.bc)]);
+    const cx<double> xm = cvt<float, double>(zb[(int64_t)lo_nb(cj, g.bc) * n1 + ci]);
+    const cx<double> xp = cvt<float, double>(zb[(int64_t)hi_nb(cj, n2, g.bc) * n1 + ci]);
+    return apply5(xc, xl, xr, xm, xp, g.ih1, g.ih2, d[p]);
+  }
+  const int64_t plane = (int64_t)n1 * n2, pl = (int64_t)ck * plane;
+  const cx<double> x1l = cvt<float, double>(zb[pl + (int64_t)cj * n1 + lo_nb(ci, g.bc)]);
+  const cx<double> x1r = cvt<float, double>(zb[pl + (int64_t)cj * n1 + hi_nb(ci, n1, g.bc)]);
+  const cx<double> x2l = cvt<float, double>(zb[pl + (int64_t)lo_nb(cj, g.bc) * n1 + ci]);
+  const cx<double> x2r = cvt<float, double>(zb[pl + (int64_t)hi_nb(cj, n2, g.bc) * n1 + ci]);
+  const int64_t col = (int64_t)cj * n1 + ci;
+  const cx<double> xm = cvt<float, double>(zb[(int64_t)lo_nb(ck, g.bc) * plane + col]);
+  const cx<double> xp = cvt<float, double>(zb[(int64_t)hi_nb(ck, g.n3, g.bc) * plane + col]);
+  return apply7(xc, x1l, x1r, x2l, x2r, xm, xp, g.ih1, g.ih2, g.ih3, d[p]);
+}
+
+// node coordinates of p and their advance by one block stride (256 nodes)
+struct NodeWalk {
+  int ci, cj, ck;
+  __device__ __forceinline__ void init(const Grid& g, int64_t p) {
+    ci = (int)(p % g.n1);
+    const int64_t q = p / g.n1;
+    cj = (int)(q % g.n2);
+    ck = (int)(q / g.n2);
+  }
+  __device__ __forceinline__ void step(const Grid& g) {
+    ci += 256;
+    while (ci >= g.n1) {
+      ci -= g.n1;
+      if (++cj == g.n2) {
+        cj = 0;
+        ++ck;
+      }
+    }
+  }
+};
+
+// pass 1: part = V^H (H z), partn = ||H z||^2
+template <int NV>
+__global__ __launch_bounds__(256) void k_dots_hz(KrylovWork kw, Grid g, const cx<double>* __restrict__ V,
+                                                 const cx<float>* __restrict__ Z, const cx<double>* __restrict__ d,
+                                                 int j) {
+  const int b = blockIdx.x, c = blockIdx.y;
+  if (kw.done[b]) return;
+  __shared__ double sh[4];
+  int64_t p0, p1;
+  chunk_range(kw.N, kw.nchunk, c, &p0, &p1);
+  const int64_t stride = kw.N * kw.nrhs;
+  const int64_t base = (int64_t)b * kw.N;
+  const cx<float>* __restrict__ zb = Z + base;
+  cx<double> acc[NV];
+#pragma unroll
+  for (int i = 0; i < NV; ++i) acc[i] = mk<double>(0.0, 0.0);
+  double nn = 0.0;
+  NodeWalk nw;
+  nw.init(g, p0 + threadIdx.x);
+  for (int64_t p = p0 + threadIdx.x; p < p1; p += 256) {
+    const cx<double> wv = hz_at(g, zb, d, nw.ci, nw.cj, nw.ck, p);
+    nn += cabs2(wv);
+#pragma unroll
+    for (int i = 0; i < NV; ++i)
+      if (i <= j) acc[i] = acc[i] + cmulc(V[i * stride + base + p], wv);
+    nw.step(g);
+  }
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    if (i <= j) {
+      const double si = kw.vscale[i * kw.nrhs + b];
+      const double re = block_sum(si * acc[i].re, sh);
+      const double im = block_sum(si * acc[i].im, sh);
+      if (threadIdx.x == 0) kw.part[((int64_t)c * (kw.kmax + 1) + i) * kw.nrhs + b] = mk<double>(re, im);
+    }
+  }
+  const double t = block_sum(nn, sh);
+  if (threadIdx.x == 0) kw.partn[(int64_t)c * kw.nrhs + b] = t;
+}
+
+// pass 2: w' = H z - V h1 (stored), partn = ||w'||^2
+template <int NV>
+__global__ __launch_bounds__(256) void k_update_lazy_hz(KrylovWork kw, Grid g, const cx<double>* __restrict__ V,
+                                                        const cx<float>* __restrict__ Z,
+                                                        const cx<double>* __restrict__ d,
+                                                        cx<double>* __restrict__ wout, int j) {
+  const int b = blockIdx.x, c = blockIdx.y;
+  if (kw.done[b]) return;
+  __shared__ double sh[4];
+  __shared__ cx<double> hs[NV];
+  if (threadIdx.x < NV) hs[threadIdx.x] = threadIdx.x <= j ? kw.h1s[threadIdx.x * kw.nrhs + b] : mk<double>(0.0, 0.0);
+  __syncthreads();
+  int64_t p0, p1;
+  chunk_range(kw.N, kw.nchunk, c, &p0, &p1);
+  const int64_t stride = kw.N * kw.nrhs;
+  const int64_t base = (int64_t)b * kw.N;
+  const cx<float>* __restrict__ zb = Z + base;
+  double nn = 0.0;
+  NodeWalk nw;
+  nw.init(g, p0 + threadIdx.x);
+  for (int64_t p = p0 + threadIdx.x; p < p1; p += 256) {
+    cx<double> wv = hz_at(g, zb, d, nw.ci, nw.cj, nw.ck, p);
+#pragma unroll
+    for (int i = 0; i < NV; ++i)
+      if (i <= j) wv = wv - hs[i] * V[i * stride + base + p];
+    wout[base + p] = wv;
+    nn += cabs2(wv);
+    nw.step(g);
   }
   const double t = block_sum(nn, sh);
   if (threadIdx.x == 0) kw.partn[(int64_t)c * kw.nrhs + b] = t;
@@ -432,27 +561,37 @@ void kr_start_impl(const KrylovWork& w, const cx<double>* r, cx<double>* V0, dou
 // kt (optional) times pass 1 / pass 2; algorithmic bytes per active RHS:
 // 16 N (j + 2) for pass 1 (reads v_0..v_j and w), 16 N (j + 3) for pass 2
 // (also writes w')
+//
+// hz (optional, lazy mode): w = H z is formed inside passes 1 and 2 from the
+// complex64 z; bytes per active RHS 16 N (j + 1) + 8 N and 16 N (j + 2) + 8 N,
+// plus 16 N for the shared diagonal.
 template <int NV>
 static void arnoldi_nv(const KrylovWork& w, const cx<double>* V, cx<double>* wv, int j, double tol,
-                       int* skip_copy, KTimer* kt, int active, hipStream_t s) {
+                       int* skip_copy, KTimer* kt, int active, const HzArgs* hz, hipStream_t s) {
   const int64_t stride = w.N * w.nrhs;
   const bool t1 = kt && kt->match(KT_DOTS, 0), t2 = kt && kt->match(KT_UPDATE, 0);
   cx<double>* vnext = (cx<double>*)(V + (j + 1) * stride);
+  const double Nf = (double)w.N;
   if (t1) kt->begin(s);
-  if (w.lazy)
+  if (hz)
+    hipLaunchKernelGGL((k_dots_hz<NV>), dim3(w.nrhs, w.nchunk), dim3(256), 0, s, w, hz->g, V, hz->z, hz->d, j);
+  else if (w.lazy)
     hipLaunchKernelGGL((k_dots<NV, true>), cgrid(w), dim3(256), 0, s, w, V, wv, j, nullptr);
   else
     hipLaunchKernelGGL((k_dots<NV, false>), cgrid(w), dim3(256), 0, s, w, V, wv, j, nullptr);
-  if (t1) kt->end(s, 16.0 * (double)w.N * (j + 2) * active);
+  if (t1) kt->end(s, hz ? (16.0 * (j + 1) + 8.0) * Nf * active + 16.0 * Nf : 16.0 * Nf * (j + 2) * active);
   hipLaunchKernelGGL(k_reduce_dots, dim3(j + (w.lazy ? 2 : 1), w.nrhs), dim3(64), 0, s, w, w.h1, j);
   if (t2) kt->begin(s);
-  if (w.lazy) {
+  if (hz) {
+    hipLaunchKernelGGL((k_update_lazy_hz<NV>), dim3(w.nrhs, w.nchunk), dim3(256), 0, s, w, hz->g, V, hz->z, hz->d,
+                       vnext, j);
+  } else if (w.lazy) {
     // w' goes straight into the next basis slot (kept unnormalised)
     hipLaunchKernelGGL((k_update_lazy<NV>), cgrid(w), dim3(256), 0, s, w, V, wv, vnext, j);
   } else {
     hipLaunchKernelGGL((k_update_dots<NV>), cgrid(w), dim3(256), 0, s, w, V, wv, wv, j);
   }
-  if (t2) kt->end(s, 16.0 * (double)w.N * (j + 3) * active);
+  if (t2) kt->end(s, hz ? (16.0 * (j + 2) + 8.0) * Nf * active + 16.0 * Nf : 16.0 * Nf * (j + 3) * active);
   if (w.lazy) {
     // second projection only for RHS whose projection cancelled
     hipLaunchKernelGGL(k_fix_flags, dim3(w.nrhs), dim3(64), 0, s, w);
@@ -465,17 +604,17 @@ static void arnoldi_nv(const KrylovWork& w, const cx<double>* V, cx<double>* wv,
 }
 
 void kr_arnoldi_impl(const KrylovWork& w, const cx<double>* V, cx<double>* wv, int j, double tol, int* skip_copy,
-                     KTimer* kt, int active, hipStream_t s) {
+                     KTimer* kt, int active, hipStream_t s, const HzArgs* hz) {
   if (j < 4)
-    arnoldi_nv<4>(w, V, wv, j, tol, skip_copy, kt, active, s);
+    arnoldi_nv<4>(w, V, wv, j, tol, skip_copy, kt, active, hz, s);
   else if (j < 8)
-    arnoldi_nv<8>(w, V, wv, j, tol, skip_copy, kt, active, s);
+    arnoldi_nv<8>(w, V, wv, j, tol, skip_copy, kt, active, hz, s);
   else if (j < 12)
-    arnoldi_nv<12>(w, V, wv, j, tol, skip_copy, kt, active, s);
+    arnoldi_nv<12>(w, V, wv, j, tol, skip_copy, kt, active, hz, s);
   else if (j < 16)
-    arnoldi_nv<16>(w, V, wv, j, tol, skip_copy, kt, active, s);
+    arnoldi_nv<16>(w, V, wv, j, tol, skip_copy, kt, active, hz, s);
   else
-    arnoldi_nv<32>(w, V, wv, j, tol, skip_copy, kt, active, s);
+    arnoldi_nv<32>(w, V, wv, j, tol, skip_copy, kt, active, hz, s);
 }
 
 template <typename TZ>

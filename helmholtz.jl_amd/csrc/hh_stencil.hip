@@ -75,8 +75,7 @@ __global__ __launch_bounds__(256) void k_stencil2d(Grid g, const cx<TI>* __restr
     cx<TD> xrr = cvt<TI, TD>(xr[row + ir]);
     const int64_t p = row + i;
     cx<TD> dv = node_diag<TD>(d, m, sfac, p);
-    cx<TD> lap = ih1 * (((TD)2 * xc) - xl - xrr) + ih2 * (((TD)2 * xc) - xm - xp);
-    cx<TD> Ax = lap + dv * xc;
+    cx<TD> Ax = apply5(xc, xl, xrr, xm, xp, ih1, ih2, dv);
     cx<TD> Dtot = dv;
     if (MODE == M_JACOBI) {
       Dtot.re += ldx + (TD)(g.ih2 * lap_diag1(j, n2, g.bc));
@@ -126,9 +125,7 @@ __global__ __launch_bounds__(256) void k_stencil3d(Grid g, const cx<TI>* __restr
     cx<TD> x2r = cvt<TI, TD>(xr[pl + (int64_t)jr * n1 + i]);
     const int64_t p = pl + col;
     cx<TD> dv = node_diag<TD>(d, m, sfac, p);
-    cx<TD> c2 = (TD)2 * xc;
-    cx<TD> lap = ih1 * (c2 - x1l - x1r) + ih2 * (c2 - x2l - x2r) + ih3 * (c2 - xm - xp);
-    cx<TD> Ax = lap + dv * xc;
+    cx<TD> Ax = apply7(xc, x1l, x1r, x2l, x2r, xm, xp, ih1, ih2, ih3, dv);
     cx<TD> Dtot = dv;
     if (MODE == M_JACOBI) {
       Dtot.re += ldxy + (TD)(g.ih3 * lap_diag1(k, n3, g.bc));
