@@ -450,7 +450,10 @@ void ensure_work(hh_solver* S, int nrhs) {
   w.nrhs = nrhs;
   w.N = N;
   w.kmax = K;
-  int nchunk = (int)std::max<int64_t>(1, std::min<int64_t>(2048 / nrhs + 1, (N + 4095) / 4096));
+  // node chunks per RHS for the Arnoldi/BiCGSTAB reductions: about 16K
+  // blocks in all (64 per CU, so the last wave of blocks is a small tail),
+  // chunks of at least 4096 nodes
+  int nchunk = (int)std::max<int64_t>(1, std::min<int64_t>((16384 + nrhs - 1) / nrhs, (N + 4095) / 4096));
   w.nchunk = nchunk;
   w.part = dalloc<cx<double>>((size_t)nchunk * (K + 1) * nrhs);
   w.partn = dalloc<double>((size_t)nchunk * nrhs);

@@ -607,8 +607,12 @@ void kr_arnoldi_impl(const KrylovWork& w, const cx<double>* V, cx<double>* wv, i
                      KTimer* kt, int active, hipStream_t s, const HzArgs* hz) {
   if (j < 4)
     arnoldi_nv<4>(w, V, wv, j, tol, skip_copy, kt, active, hz, s);
+  else if (j < 6)
+    arnoldi_nv<6>(w, V, wv, j, tol, skip_copy, kt, active, hz, s);
   else if (j < 8)
     arnoldi_nv<8>(w, V, wv, j, tol, skip_copy, kt, active, hz, s);
+  else if (j < 10)
+    arnoldi_nv<10>(w, V, wv, j, tol, skip_copy, kt, active, hz, s);
   else if (j < 12)
     arnoldi_nv<12>(w, V, wv, j, tol, skip_copy, kt, active, hz, s);
   else if (j < 16)
