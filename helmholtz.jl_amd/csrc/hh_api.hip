@@ -470,6 +470,8 @@ void ensure_work(hh_solver* S, int nrhs) {
   w.done = dalloc<int>(nrhs);
   w.kact = dalloc<int>(nrhs);
   w.lazy = pr == HH_C64;
+  w.zunscaled = 0;
+  w.stage = nullptr;
   w.vscale = dalloc<double>((size_t)(K + 1) * nrhs);
   w.fix = dalloc<int>(nrhs);
   w.wn2 = dalloc<double>(nrhs);
@@ -749,11 +751,13 @@ struct Cycle {
 // z = M^-1 (scale * v); scale (per RHS, lazy basis normalisation) may be null
 template <typename T>
 void precondition(hh_solver* S, int nrhs, const cx<double>* vj, void* zj, const int* skip, hipStream_t st,
-                  const double* scale = nullptr) {
+                  const double* scale = nullptr, bool staged = false) {
   Cycle<T> c{S, nrhs, skip, st};
   const int64_t blk = S->lv[0].g.N * (int64_t)nrhs;
   if (std::is_same<T, float>::value) {
-    if (scale) {
+    if (staged) {
+      // the complex64 input is already in b0 (written by the Arnoldi pass)
+    } else if (scale) {
       KrylovWork w = S->kw;
       w.nrhs = nrhs;
       kr_cast_scaled(w, vj, scale, (cx<float>*)S->b0, st);
@@ -818,6 +822,11 @@ void run_fgmres(hh_solver* S, int nrhs, hh_solve_stats* stats) {
   if (S->use_learned) setup_learned(S);
   for (int outer = 0; outer < S->p.max_outer; ++outer) {
     HIPCK(hipMemcpyAsync(S->conv_d, conv.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, st));
+    // plain complex64 cycle: the Arnoldi pass stages the next input and Z
+    // holds unscaled directions (the learned correction is not linear, so it
+    // gets normalised inputs)
+    w.zunscaled = w.lazy && !S->use_learned;
+    w.stage = w.zunscaled ? (cx<float>*)S->b0 : nullptr;
     kr_start_impl(w, S->wk, S->V, tol, S->conv_d, S->n2tmp, st);
     HIPCK(hipMemcpyAsync(S->h_done, w.done, sizeof(int) * nrhs, hipMemcpyDeviceToHost, st));
     HIPCK(hipStreamSynchronize(st));
@@ -836,18 +845,21 @@ void run_fgmres(hh_solver* S, int nrhs, hh_solve_stats* stats) {
         // H z: reads z (8 or 16 B per value), writes w (16 B), shared c128 diagonal
         const bool ta = S->kt.match(KT_APPLY, 0);
         const double Nf = (double)N;
+        const double* vsj = w.lazy ? w.vscale + (int64_t)j * nrhs : nullptr;
+        const bool staged = w.stage && j > 0;
         if (S->p.prec == HH_C64 && w.lazy && !S->op->ho && !ta) {
           // H z formed inside the Arnoldi passes (w is never stored)
-          precondition<float>(S, nrhs, vj, zj, w.done, st, w.vscale + (int64_t)j * nrhs);
+          precondition<float>(S, nrhs, vj, zj, w.done, st, vsj, staged);
           const HzArgs hz{S->op->g, (const cx<float>*)zj, S->op->d64};
           kr_arnoldi_impl(w, S->V, S->wk, j, tol, S->skip_copy, &S->kt, active, st, &hz);
           return;
         }
         if (S->p.prec == HH_C64) {
-          precondition<float>(S, nrhs, vj, zj, w.done, st, w.lazy ? w.vscale + (int64_t)j * nrhs : nullptr);
+          precondition<float>(S, nrhs, vj, zj, w.done, st, vsj, staged);
           if (ta) S->kt.begin(st);
           h_apply<float>(S, nrhs, (const cx<float>*)zj, S->wk, w.done, st);
           if (ta) S->kt.end(st, active * Nf * 24.0 + Nf * 16.0);
+          if (w.zunscaled) kr_scale_rhs(w, S->wk, vsj, st);
         } else {
           precondition<double>(S, nrhs, vj, zj, w.done, st);
           if (ta) S->kt.begin(st);

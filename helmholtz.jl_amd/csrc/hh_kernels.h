@@ -152,9 +152,17 @@ struct KrylovWork {
   int* fix;              // [nrhs] 1 = the second CGS correction is applied this step
   double* wn2;           // [nrhs] ||w||^2 before projection (lazy mode)
   cx<double>* h1s;       // [kmax+1][nrhs] vscale[i] * h1[i] (lazy mode: coefficients of the stored vectors)
+  // staged preconditioner input (lazy mode, plain cycle): pass 2 also writes
+  // the complex64 copy of the stored w' into stage, which the next step's
+  // cycle takes as its input; Z_j then holds M^-1 (stored V_j) and the true
+  // z_j = vscale[j] Z_j (scale applied to H z and to the x update)
+  int zunscaled;
+  cx<float>* stage;      // [nrhs][N] or null
 };
 // b0 = (complex64) vscale_j[b] * V_j  (preconditioner input in the lazy mode)
 void kr_cast_scaled(const KrylovWork& w, const cx<double>* Vj, const double* scale, cx<float>* b0, hipStream_t s);
+// w *= vscale_j[b] per RHS (H Z_j -> H z_j when Z holds unscaled directions)
+void kr_scale_rhs(const KrylovWork& w, cx<double>* wv, const double* scale, hipStream_t s);
 
 // ||v_b||^2 per RHS into out[b] (RHS with skip[b] != 0 untouched)
 void kr_norm2(const KrylovWork& w, const cx<double>* v, double* out, const int* skip, hipStream_t s);

@@ -180,6 +180,7 @@ __global__ __launch_bounds__(256) void k_update_lazy(KrylovWork kw, const cx<dou
     for (int i = 0; i < NV; ++i)
       if (i <= j) wv = wv - hs[i] * V[i * stride + base + p];
     wout[base + p] = wv;
+    if (kw.stage) kw.stage[base + p] = mk<float>((float)wv.re, (float)wv.im);
     nn += cabs2(wv);
   }
   const double t = block_sum(nn, sh);
@@ -257,10 +258,11 @@ __global__ __launch_bounds__(256) void k_dots_hz(KrylovWork kw, Grid g, const cx
 #pragma unroll
   for (int i = 0; i < NV; ++i) acc[i] = mk<double>(0.0, 0.0);
   double nn = 0.0;
+  const double zs = kw.zunscaled ? kw.vscale[j * kw.nrhs + b] : 1.0;
   NodeWalk nw;
   nw.init(g, p0 + threadIdx.x);
   for (int64_t p = p0 + threadIdx.x; p < p1; p += 256) {
-    const cx<double> wv = hz_at(g, zb, d, nw.ci, nw.cj, nw.ck, p);
+    const cx<double> wv = zs * hz_at(g, zb, d, nw.ci, nw.cj, nw.ck, p);
     nn += cabs2(wv);
 #pragma unroll
     for (int i = 0; i < NV; ++i)
@@ -298,14 +300,16 @@ __global__ __launch_bounds__(256) void k_update_lazy_hz(KrylovWork kw, Grid g, c
   const int64_t base = (int64_t)b * kw.N;
   const cx<float>* __restrict__ zb = Z + base;
   double nn = 0.0;
+  const double zs = kw.zunscaled ? kw.vscale[j * kw.nrhs + b] : 1.0;
   NodeWalk nw;
   nw.init(g, p0 + threadIdx.x);
   for (int64_t p = p0 + threadIdx.x; p < p1; p += 256) {
-    cx<double> wv = hz_at(g, zb, d, nw.ci, nw.cj, nw.ck, p);
+    cx<double> wv = zs * hz_at(g, zb, d, nw.ci, nw.cj, nw.ck, p);
 #pragma unroll
     for (int i = 0; i < NV; ++i)
       if (i <= j) wv = wv - hs[i] * V[i * stride + base + p];
     wout[base + p] = wv;
+    if (kw.stage) kw.stage[base + p] = mk<float>((float)wv.re, (float)wv.im);
     nn += cabs2(wv);
     nw.step(g);
   }
@@ -500,6 +504,9 @@ __global__ __launch_bounds__(64) void k_backsolve(KrylovWork kw) {
     for (int l = i + 1; l < k; ++l) s = s - Hb[i + (int64_t)l * (K + 1)] * yb[l];
     yb[i] = s * cinv(Hb[i + (int64_t)i * (K + 1)]);
   }
+  // x += sum y_i z_i with z_i = vscale_i Z_i
+  if (kw.zunscaled)
+    for (int i = 0; i < k; ++i) yb[i] = kw.vscale[i * kw.nrhs + b] * yb[i];
 }
 
 template <typename TZ>
@@ -537,12 +544,42 @@ __global__ __launch_bounds__(256) void k_cast_scaled(const cx<double>* __restric
   o[k] = mk<float>((float)(s * a.re), (float)(s * a.im));
 }
 
+// stage = (complex64) vout for RHS whose basis vector pass 3 rewrote
+__global__ __launch_bounds__(256) void k_restage(KrylovWork kw, const cx<double>* __restrict__ vout,
+                                                 const int* __restrict__ skip_before) {
+  const int c = blockIdx.x, b = blockIdx.y;
+  if (skip_before[b] || !kw.fix[b]) return;
+  int64_t p0, p1;
+  chunk_range(kw.N, kw.nchunk, c, &p0, &p1);
+  const int64_t base = (int64_t)b * kw.N;
+  for (int64_t p = p0 + threadIdx.x; p < p1; p += 256) {
+    const cx<double> v = vout[base + p];
+    kw.stage[base + p] = mk<float>((float)v.re, (float)v.im);
+  }
+}
+
+// wv *= sc[b] per RHS
+__global__ __launch_bounds__(256) void k_scale_rhs(KrylovWork kw, cx<double>* __restrict__ wv,
+                                                   const double* __restrict__ sc) {
+  const int c = blockIdx.x, b = blockIdx.y;
+  if (kw.done[b]) return;
+  int64_t p0, p1;
+  chunk_range(kw.N, kw.nchunk, c, &p0, &p1);
+  const int64_t base = (int64_t)b * kw.N;
+  const double s = sc[b];
+  for (int64_t p = p0 + threadIdx.x; p < p1; p += 256) wv[base + p] = s * wv[base + p];
+}
+
 // ------------------------------------------------------------------ host side
 static inline dim3 cgrid(const KrylovWork& w) { return dim3(w.nchunk, w.nrhs); }
 
 void kr_cast_scaled(const KrylovWork& w, const cx<double>* Vj, const double* scale, cx<float>* b0, hipStream_t s) {
   const int64_t n = w.N * (int64_t)w.nrhs;
   hipLaunchKernelGGL(k_cast_scaled, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, Vj, scale, b0, w.N, w.nrhs);
+}
+
+void kr_scale_rhs(const KrylovWork& w, cx<double>* wv, const double* scale, hipStream_t s) {
+  hipLaunchKernelGGL(k_scale_rhs, cgrid(w), dim3(256), 0, s, w, wv, scale);
 }
 
 void kr_norm2(const KrylovWork& w, const cx<double>* v, double* out, const int* skip, hipStream_t s) {
@@ -564,7 +601,7 @@ void kr_start_impl(const KrylovWork& w, const cx<double>* r, cx<double>* V0, dou
 //
 // hz (optional, lazy mode): w = H z is formed inside passes 1 and 2 from the
 // complex64 z; bytes per active RHS 16 N (j + 1) + 8 N and 16 N (j + 2) + 8 N,
-// plus 16 N for the shared diagonal.
+// plus 16 N for the shared diagonal.  A staged input adds 8 N to pass 2.
 template <int NV>
 static void arnoldi_nv(const KrylovWork& w, const cx<double>* V, cx<double>* wv, int j, double tol,
                        int* skip_copy, KTimer* kt, int active, const HzArgs* hz, hipStream_t s) {
@@ -591,7 +628,9 @@ static void arnoldi_nv(const KrylovWork& w, const cx<double>* V, cx<double>* wv,
   } else {
     hipLaunchKernelGGL((k_update_dots<NV>), cgrid(w), dim3(256), 0, s, w, V, wv, wv, j);
   }
-  if (t2) kt->end(s, hz ? (16.0 * (j + 2) + 8.0) * Nf * active + 16.0 * Nf : 16.0 * Nf * (j + 3) * active);
+  const double st8 = w.stage ? 8.0 : 0.0;  // staged complex64 copy of w'
+  if (t2)
+    kt->end(s, hz ? (16.0 * (j + 2) + 8.0 + st8) * Nf * active + 16.0 * Nf : (16.0 * (j + 3) + st8) * Nf * active);
   if (w.lazy) {
     // second projection only for RHS whose projection cancelled
     hipLaunchKernelGGL(k_fix_flags, dim3(w.nrhs), dim3(64), 0, s, w);
@@ -601,6 +640,7 @@ static void arnoldi_nv(const KrylovWork& w, const cx<double>* V, cx<double>* wv,
   hipLaunchKernelGGL(k_hess_givens, dim3(w.nrhs), dim3(64), 0, s, w, j, tol);
   hipLaunchKernelGGL((k_final<NV>), cgrid(w), dim3(256), 0, s, w, V, w.lazy ? (const cx<double>*)vnext : wv, vnext,
                      j, skip_copy);
+  if (w.lazy && w.stage) hipLaunchKernelGGL(k_restage, cgrid(w), dim3(256), 0, s, w, vnext, skip_copy);
 }
 
 void kr_arnoldi_impl(const KrylovWork& w, const cx<double>* V, cx<double>* wv, int j, double tol, int* skip_copy,
