@@ -86,18 +86,19 @@ def timed_solves(ha, solver, Bd, warmup, steps, log, name, before_timed=None):
 # The dominant kernel of the batched solve (rocprofv3 kernel trace of this
 # benchmark): the fused up leg of the multigrid cycle on the fine grid.
 ROOFLINE_KERNEL = ("up", 0)
-ROOFLINE_NAME = "k_up_fused<float,16> on the 1025x513 grid (x += P e_c, two damped-Jacobi sweeps)"
+ROOFLINE_NAME = "k_up_fused<float> on the 1025x513 grid (x += P e_c, two damped-Jacobi sweeps)"
 
 
 def roofline_from_solver(ha, solver, nsrc: int, N: int, Nc: int):
     """HIP-event timing of every launch of the roofline kernel inside the timed
     solves (on the solver's stream), against its algorithmic bytes: per active
     RHS read x, b (8 B each), the coarse correction (8 B per coarse node),
-    write x (8 B); plus the shared complex64 diagonal (8 B per node)."""
+    write x (8 B); plus the shared complex64 diagonal and Jacobi factor (8 B
+    each per node)."""
     t = ha.kernel_time(solver)
     ms, n, byt = t["ms"], t["launches"], t["bytes"]
     achieved = byt / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
-    full = nsrc * (3 * N * 8 + Nc * 8) + N * 8
+    full = nsrc * (3 * N * 8 + Nc * 8) + 2 * N * 8
     traffic = _pmc_traffic()
     return {"bound": "hbm", "kernel": ROOFLINE_NAME, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,

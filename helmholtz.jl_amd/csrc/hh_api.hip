@@ -658,12 +658,13 @@ struct Cycle {
       L.cur = first ? first : L.xa;
       // algorithmic bytes per launch: per active RHS the fields each leg
       // reads and writes once (s bytes per value), plus the shared diagonal
+      // (and, for the legs that smooth, the shared Jacobi factor)
       const double s = sizeof(cx<T>), Nf = (double)L.g.N, Nc = (double)C.g.N, act = S->kt_active;
       const bool td = S->kt.match(KT_DOWN, l), tm = S->kt.match(KT_MID, l), tu = S->kt.match(KT_UP, l);
       if (td) S->kt.begin(st);
       launch_down_fused<T>(L.g, C.g, nrhs, b, (const cx<T>*)L.d, (const cx<T>*)L.f, (cx<T>*)L.cur, (cx<T>*)C.b,
                            skip, st);
-      if (td) S->kt.end(st, act * (2 * Nf * s + Nc * s) + Nf * s);
+      if (td) S->kt.end(st, act * (2 * Nf * s + Nc * s) + 2 * Nf * s);
       C.cur = nullptr;
       run(l + 1, (const cx<T>*)C.b, true, nullptr);
       for (int c = 1; c < ncorr; ++c) {
@@ -680,7 +681,7 @@ struct Cycle {
       if (tu) S->kt.begin(st);
       launch_up_fused<T>(L.g, C.g, nrhs, (const cx<T>*)L.cur, (const cx<T>*)C.cur, b, (const cx<T>*)L.d,
                          (const cx<T>*)L.f, (cx<T>*)o, skip, st);
-      if (tu) S->kt.end(st, act * (3 * Nf * s + Nc * s) + Nf * s);
+      if (tu) S->kt.end(st, act * (3 * Nf * s + Nc * s) + 2 * Nf * s);
       L.cur = o;
       return;
     }

@@ -373,6 +373,12 @@ def test_kernel_timing_counts_launches_and_keeps_results(kernel, level):
         # every step with all RHS active except the last few: bounded by the full-batch figure
         assert t["bytes"] <= t["launches"] * (nr * N * 24.0 + N * 16.0) + 1.0
         assert t["bytes"] >= t["launches"] * (N * 24.0 + N * 16.0) - 1.0
+    if kernel == "up" and level == 0:
+        # per active RHS x, b, e_c read and x written (8 B values), plus the
+        # shared diagonal and Jacobi factor
+        Nc = 65 * 65
+        assert t["bytes"] <= t["launches"] * (nr * (3 * N * 8.0 + Nc * 8.0) + 2 * N * 8.0) + 1.0
+        assert t["bytes"] >= t["launches"] * ((3 * N * 8.0 + Nc * 8.0) + 2 * N * 8.0) - 1.0
     ha.time_kernel(s1, "none")
     assert ha.kernel_time(s1)["launches"] == 0
 
