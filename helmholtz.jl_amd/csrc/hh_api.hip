@@ -518,7 +518,8 @@ int tune_level(hh_solver* S, Level& L, int nrhs) {
     if (rows > len) continue;
     auto go = [&] {
       launch_stencil<T, T, T, M_JACOBI>(L.g, nrhs, (const cx<T>*)L.xa, (const cx<T>*)L.r, (cx<T>*)L.xb,
-                                        (const cx<T>*)L.d, nullptr, mk<double>(0, 0), 0.8, nullptr, st, rows);
+                                        (const cx<T>*)L.d, nullptr, mk<double>(0, 0), 0.8, nullptr, st, rows,
+                                        (const cx<T>*)L.f);
     };
     go();
     HIPCK(hipEventRecord(e0, st));
@@ -582,7 +583,7 @@ struct Cycle {
     for (int s = 0; s < n; ++s) {
       void* o = other(L);
       launch_stencil<T, T, T, M_JACOBI>(L.g, nrhs, (const cx<T>*)L.cur, b, (cx<T>*)o, (const cx<T>*)L.d, nullptr,
-                                        mk<double>(0, 0), S->p.relax, skip, st, rows_of(L));
+                                        mk<double>(0, 0), S->p.relax, skip, st, rows_of(L), (const cx<T>*)L.f);
       L.cur = o;
     }
   }
@@ -688,7 +689,7 @@ struct Cycle {
     if (zero_init) {
       L.cur = first ? first : L.xa;
       if (pre > 0) {
-        launch_jacobi0<T>(L.g, nrhs, b, (cx<T>*)L.cur, (const cx<T>*)L.d, S->p.relax, skip, st);
+        launch_jacobi0<T>(L.g, nrhs, b, (cx<T>*)L.cur, (const cx<T>*)L.f, skip, st);
         sweeps(L, b, pre - 1);
       } else {
         launch_zero<T>((cx<T>*)L.cur, L.g.N * nrhs, st);

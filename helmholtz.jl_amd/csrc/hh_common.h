@@ -67,6 +67,22 @@ struct Grid {
   int bc;           // 2: mirror ghost (second-order Neumann rows), 1: ghost = self
 };
 
+// Node coordinates of p (i fastest), in 32-bit arithmetic when the grid has
+// fewer than 2^31 nodes.
+__device__ __forceinline__ void node_coords(const Grid& g, int64_t p, int& i, int& j, int& k) {
+  if (g.N < ((int64_t)1 << 31)) {
+    const unsigned u = (unsigned)p, q = u / (unsigned)g.n1;
+    i = (int)(u - q * (unsigned)g.n1);
+    j = (int)(q % (unsigned)g.n2);
+    k = (int)(q / (unsigned)g.n2);
+  } else {
+    const int64_t q = p / g.n1;
+    i = (int)(p - q * g.n1);
+    j = (int)(q % g.n2);
+    k = (int)(q / g.n2);
+  }
+}
+
 // Mirror/self ghost index on one axis (src/PlainNodalLaplacian.jl:18-30 rows).
 __device__ __forceinline__ int lo_nb(int i, int bc) { return i > 0 ? i - 1 : (bc == 2 ? 1 : 0); }
 __device__ __forceinline__ int hi_nb(int i, int n, int bc) {

@@ -13,11 +13,11 @@ enum { M_APPLY = 0, M_RESID = 1, M_JACOBI = 2 };
 template <typename TI, typename TO, typename TD, int MODE>
 void launch_stencil(const Grid& g, int nrhs, const cx<TI>* x, const cx<TO>* b, cx<TO>* y, const cx<TD>* d,
                     const double* m, cx<double> sfac, double wj, const int* skip, hipStream_t s,
-                    int rows_override = 0);
+                    int rows_override = 0, const cx<TD>* fj = nullptr);
 template <typename T>
 void launch_zero(cx<T>* a, int64_t n, hipStream_t s);
 template <typename T>
-void launch_jacobi0(const Grid& g, int nrhs, const cx<T>* b, cx<T>* x, const cx<T>* d, double wj, const int* skip,
+void launch_jacobi0(const Grid& g, int nrhs, const cx<T>* b, cx<T>* x, const cx<T>* f, const int* skip,
                     hipStream_t s);
 template <typename T>
 void launch_restrict(const Grid& gf, const Grid& gc, int nrhs, const cx<T>* rf, cx<T>* bc, const int* skip,

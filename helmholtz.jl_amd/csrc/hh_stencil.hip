@@ -28,9 +28,11 @@ __device__ __forceinline__ cx<TD> node_diag(const cx<TD>* __restrict__ d, const 
   return dv;
 }
 
+// M_JACOBI: x + F (b - A x) with the level's precomputed damped-Jacobi factor
+// F = wj (d + diag of the Laplacian)^-1 when fj is given, else wj (b - A x) / Dtot
 template <typename TI, typename TO, typename TD, int MODE>
 __device__ __forceinline__ void finish(cx<TO>* __restrict__ yo, const cx<TO>* __restrict__ bo, int64_t p,
-                                       cx<TD> xc, cx<TD> Ax, cx<TD> Dtot, TD wj) {
+                                       cx<TD> xc, cx<TD> Ax, cx<TD> Dtot, TD wj, const cx<TD>* __restrict__ fj) {
   if (MODE == M_APPLY) {
     yo[p] = cvt<TD, TO>(Ax);
   } else if (MODE == M_RESID) {
@@ -39,8 +41,12 @@ __device__ __forceinline__ void finish(cx<TO>* __restrict__ yo, const cx<TO>* __
   } else {  // M_JACOBI
     cx<TD> bv = cvt<TO, TD>(bo[p]);
     cx<TD> r = bv - Ax;
-    cx<TD> up = r * cinv(Dtot);
-    yo[p] = cvt<TD, TO>(xc + wj * up);
+    if (fj) {
+      yo[p] = cvt<TD, TO>(xc + fj[p] * r);
+    } else {
+      cx<TD> up = r * cinv(Dtot);
+      yo[p] = cvt<TD, TO>(xc + wj * up);
+    }
   }
 }
 
@@ -49,7 +55,7 @@ __global__ __launch_bounds__(256) void k_stencil2d(Grid g, const cx<TI>* __restr
                                                    const cx<TO>* __restrict__ b, cx<TO>* __restrict__ y,
                                                    const cx<TD>* __restrict__ d, const double* __restrict__ m,
                                                    cx<double> sfac, TD wj, int rows, TileMap tm,
-                                                   const int* __restrict__ skip) {
+                                                   const int* __restrict__ skip, const cx<TD>* __restrict__ fj) {
   int bx, by, rhs;
   tile_of(tm, bx, by, rhs);
   if (skip && skip[rhs]) return;
@@ -77,10 +83,10 @@ __global__ __launch_bounds__(256) void k_stencil2d(Grid g, const cx<TI>* __restr
     cx<TD> dv = node_diag<TD>(d, m, sfac, p);
     cx<TD> Ax = apply5(xc, xl, xrr, xm, xp, ih1, ih2, dv);
     cx<TD> Dtot = dv;
-    if (MODE == M_JACOBI) {
+    if (MODE == M_JACOBI && !fj) {
       Dtot.re += ldx + (TD)(g.ih2 * lap_diag1(j, n2, g.bc));
     }
-    finish<TI, TO, TD, MODE>(yo, bo, p, xc, Ax, Dtot, wj);
+    finish<TI, TO, TD, MODE>(yo, bo, p, xc, Ax, Dtot, wj, fj);
     xm = xc;
     xc = xp;
   }
@@ -91,7 +97,7 @@ __global__ __launch_bounds__(256) void k_stencil3d(Grid g, const cx<TI>* __restr
                                                    const cx<TO>* __restrict__ b, cx<TO>* __restrict__ y,
                                                    const cx<TD>* __restrict__ d, const double* __restrict__ m,
                                                    cx<double> sfac, TD wj, int rows, int ytiles, TileMap tm,
-                                                   const int* __restrict__ skip) {
+                                                   const int* __restrict__ skip, const cx<TD>* __restrict__ fj) {
   int bx, by, rhs;
   tile_of(tm, bx, by, rhs);
   if (skip && skip[rhs]) return;
@@ -127,10 +133,10 @@ __global__ __launch_bounds__(256) void k_stencil3d(Grid g, const cx<TI>* __restr
     cx<TD> dv = node_diag<TD>(d, m, sfac, p);
     cx<TD> Ax = apply7(xc, x1l, x1r, x2l, x2r, xm, xp, ih1, ih2, ih3, dv);
     cx<TD> Dtot = dv;
-    if (MODE == M_JACOBI) {
+    if (MODE == M_JACOBI && !fj) {
       Dtot.re += ldxy + (TD)(g.ih3 * lap_diag1(k, n3, g.bc));
     }
-    finish<TI, TO, TD, MODE>(yo, bo, p, xc, Ax, Dtot, wj);
+    finish<TI, TO, TD, MODE>(yo, bo, p, xc, Ax, Dtot, wj, fj);
     xm = xc;
     xc = xp;
   }
@@ -139,21 +145,13 @@ __global__ __launch_bounds__(256) void k_stencil3d(Grid g, const cx<TI>* __restr
 // First Jacobi sweep from a zero guess: x = w D^-1 b (no stencil needed).
 template <typename T>
 __global__ __launch_bounds__(256) void k_jacobi0(Grid g, const cx<T>* __restrict__ b, cx<T>* __restrict__ x,
-                                                 const cx<T>* __restrict__ d, T wj, const int* __restrict__ skip) {
+                                                 const cx<T>* __restrict__ f, const int* __restrict__ skip) {
   const int rhs = blockIdx.y;
   if (skip && skip[rhs]) return;
   const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (p >= g.N) return;
-  const int i = (int)(p % g.n1);
-  const int64_t q = p / g.n1;
-  const int j = (int)(q % g.n2);
-  const int k = (int)(q / g.n2);
-  double ld = g.ih1 * lap_diag1(i, g.n1, g.bc) + g.ih2 * lap_diag1(j, g.n2, g.bc);
-  if (g.dim == 3) ld += g.ih3 * lap_diag1(k, g.n3, g.bc);
-  cx<T> D = d[p];
-  D.re += (T)ld;
   const int64_t o = (int64_t)rhs * g.N + p;
-  x[o] = wj * (b[o] * cinv(D));
+  x[o] = f[p] * b[o];
 }
 
 // Full weighting with mirror ghosts, fine residual -> coarse right-hand side.
@@ -164,10 +162,8 @@ __global__ __launch_bounds__(256) void k_restrict(Grid gf, Grid gc, const cx<T>*
   if (skip && skip[rhs]) return;
   const int64_t pc = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (pc >= gc.N) return;
-  const int I = (int)(pc % gc.n1);
-  const int64_t q = pc / gc.n1;
-  const int J = (int)(q % gc.n2);
-  const int K = (int)(q / gc.n2);
+  int I, J, K;
+  node_coords(gc, pc, I, J, K);
   const cx<T>* r = rf + (int64_t)rhs * gf.N;
   const T w[3] = {(T)0.25, (T)0.5, (T)0.25};
   cx<T> acc = mk<T>(0, 0);
@@ -234,10 +230,8 @@ __global__ __launch_bounds__(256) void k_prolong_add(Grid gf, Grid gc, const cx<
   if (skip && skip[rhs]) return;
   const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (p >= gf.N) return;
-  const int i = (int)(p % gf.n1);
-  const int64_t q = p / gf.n1;
-  const int j = (int)(q % gf.n2);
-  const int k = (int)(q / gf.n2);
+  int i, j, k;
+  node_coords(gf, p, i, j, k);
   const cx<T>* c = xc + (int64_t)rhs * gc.N;
   const int i0 = i >> 1, j0 = j >> 1, k0 = k >> 1;
   const int i1 = (i & 1) ? i0 + 1 : i0, j1 = (j & 1) ? j0 + 1 : j0, k1 = (k & 1) ? k0 + 1 : k0;
@@ -612,55 +606,63 @@ static inline int pick_rows(const Grid& g, int nrhs) {
 
 template <typename TI, typename TO, typename TD, int MODE>
 void launch_stencil(const Grid& g, int nrhs, const cx<TI>* x, const cx<TO>* b, cx<TO>* y, const cx<TD>* d,
-                    const double* m, cx<double> sfac, double wj, const int* skip, hipStream_t s, int rows_override) {
+                    const double* m, cx<double> sfac, double wj, const int* skip, hipStream_t s, int rows_override,
+                    const cx<TD>* fj) {
   int ytiles;
   const int rows = rows_override > 0 ? rows_override : pick_rows(g, nrhs);
   const dim3 grid = stencil_grid(g, nrhs, rows, &ytiles);
   const TileMap tm = make_tile_map((int)grid.x, (int)grid.y, nrhs);
   if (g.dim == 2)
     hipLaunchKernelGGL((k_stencil2d<TI, TO, TD, MODE>), dim3(tm.T), dim3(64, 4), 0, s, g, x, b, y, d, m, sfac,
-                       (TD)wj, rows, tm, skip);
+                       (TD)wj, rows, tm, skip, fj);
   else
     hipLaunchKernelGGL((k_stencil3d<TI, TO, TD, MODE>), dim3(tm.T), dim3(64, 4), 0, s, g, x, b, y, d, m, sfac,
-                       (TD)wj, rows, ytiles, tm, skip);
+                       (TD)wj, rows, ytiles, tm, skip, fj);
 }
 
 template void launch_stencil<double, double, double, M_APPLY>(const Grid&, int, const cx<double>*,
                                                                const cx<double>*, cx<double>*,
                                                                const cx<double>*, const double*, cx<double>,
-                                                               double, const int*, hipStream_t, int);
+                                                               double, const int*, hipStream_t, int,
+                                                              const cx<double>*);
 template void launch_stencil<double, double, double, M_RESID>(const Grid&, int, const cx<double>*,
                                                                const cx<double>*, cx<double>*,
                                                                const cx<double>*, const double*, cx<double>,
-                                                               double, const int*, hipStream_t, int);
+                                                               double, const int*, hipStream_t, int,
+                                                              const cx<double>*);
 template void launch_stencil<double, double, double, M_JACOBI>(const Grid&, int, const cx<double>*,
                                                                 const cx<double>*, cx<double>*,
                                                                 const cx<double>*, const double*, cx<double>,
-                                                                double, const int*, hipStream_t, int);
+                                                                double, const int*, hipStream_t, int,
+                                                              const cx<double>*);
 template void launch_stencil<float, float, float, M_APPLY>(const Grid&, int, const cx<float>*, const cx<float>*,
                                                             cx<float>*, const cx<float>*, const double*,
-                                                            cx<double>, double, const int*, hipStream_t, int);
+                                                            cx<double>, double, const int*, hipStream_t, int,
+                                                              const cx<float>*);
 template void launch_stencil<float, float, float, M_RESID>(const Grid&, int, const cx<float>*, const cx<float>*,
                                                             cx<float>*, const cx<float>*, const double*,
-                                                            cx<double>, double, const int*, hipStream_t, int);
+                                                            cx<double>, double, const int*, hipStream_t, int,
+                                                              const cx<float>*);
 template void launch_stencil<float, float, float, M_JACOBI>(const Grid&, int, const cx<float>*, const cx<float>*,
                                                              cx<float>*, const cx<float>*, const double*,
-                                                             cx<double>, double, const int*, hipStream_t, int);
+                                                             cx<double>, double, const int*, hipStream_t, int,
+                                                              const cx<float>*);
 template void launch_stencil<float, double, double, M_APPLY>(const Grid&, int, const cx<float>*,
                                                               const cx<double>*, cx<double>*,
                                                               const cx<double>*, const double*, cx<double>,
-                                                              double, const int*, hipStream_t, int);
+                                                              double, const int*, hipStream_t, int,
+                                                              const cx<double>*);
 
 template <typename T>
-void launch_jacobi0(const Grid& g, int nrhs, const cx<T>* b, cx<T>* x, const cx<T>* d, double wj, const int* skip,
+void launch_jacobi0(const Grid& g, int nrhs, const cx<T>* b, cx<T>* x, const cx<T>* f, const int* skip,
                     hipStream_t s) {
   dim3 grid((unsigned)((g.N + 255) / 256), nrhs);
-  hipLaunchKernelGGL((k_jacobi0<T>), grid, dim3(256), 0, s, g, b, x, d, (T)wj, skip);
+  hipLaunchKernelGGL((k_jacobi0<T>), grid, dim3(256), 0, s, g, b, x, f, skip);
 }
-template void launch_jacobi0<double>(const Grid&, int, const cx<double>*, cx<double>*, const cx<double>*, double,
+template void launch_jacobi0<double>(const Grid&, int, const cx<double>*, cx<double>*, const cx<double>*,
                                      const int*, hipStream_t);
-template void launch_jacobi0<float>(const Grid&, int, const cx<float>*, cx<float>*, const cx<float>*, double,
-                                    const int*, hipStream_t);
+template void launch_jacobi0<float>(const Grid&, int, const cx<float>*, cx<float>*, const cx<float>*, const int*,
+                                    hipStream_t);
 
 template <typename T>
 void launch_restrict(const Grid& gf, const Grid& gc, int nrhs, const cx<T>* rf, cx<T>* bc, const int* skip,
