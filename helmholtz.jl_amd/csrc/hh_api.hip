@@ -198,6 +198,8 @@ struct Level {
   void* kc1 = nullptr;
   void* kv1 = nullptr;
   cx<double>* ksc = nullptr;
+  // Jac-GMRES smoothing: Krylov basis of the level (steps + 1 blocks)
+  void* gv = nullptr;
 };
 
 }  // namespace
@@ -227,6 +229,13 @@ struct hh_solver {
   int* skip_copy = nullptr;
   cx<double>* bsc = nullptr;  // BiCGSTAB per-RHS scalars (rho, alpha, omega, beta)
   cx<double>* kpart = nullptr; // K-cycle partial inner products
+  // Jac-GMRES smoothing: per-RHS complex128 state shared by the levels
+  cx<double>* gm_part = nullptr;
+  cx<double>* gm_hc = nullptr;
+  cx<double>* gm_H = nullptr;
+  double* gm_sc = nullptr;
+  double* gm_beta = nullptr;
+  cx<double>* gm_y = nullptr;
   int* conv_d = nullptr;
   double* n2tmp = nullptr;
   double* res_d = nullptr;
@@ -273,12 +282,15 @@ struct hh_solver {
     drop_graphs();
     tuned_for = 0;
     for (auto& L : lv) {
-      dfree(L.xa); dfree(L.xb); dfree(L.b); dfree(L.r); dfree(L.kc1); dfree(L.kv1); dfree(L.ksc);
-      L.xa = L.xb = L.b = L.r = L.cur = L.kc1 = L.kv1 = nullptr;
+      dfree(L.xa); dfree(L.xb); dfree(L.b); dfree(L.r); dfree(L.kc1); dfree(L.kv1); dfree(L.ksc); dfree(L.gv);
+      L.xa = L.xb = L.b = L.r = L.cur = L.kc1 = L.kv1 = L.gv = nullptr;
       L.ksc = nullptr;
     }
     dfree(kpart);
     kpart = nullptr;
+    dfree(gm_part); dfree(gm_hc); dfree(gm_H); dfree(gm_sc); dfree(gm_beta); dfree(gm_y);
+    gm_part = gm_hc = gm_H = gm_y = nullptr;
+    gm_sc = gm_beta = nullptr;
     dfree(V); dfree(Z); dfree(xk); dfree(bk); dfree(wk); dfree(b0);
     V = nullptr; Z = nullptr; xk = bk = wk = nullptr; b0 = nullptr;
     dfree(kw.part); dfree(kw.partn); dfree(kw.h1); dfree(kw.h2); dfree(kw.nrm); dfree(kw.H); dfree(kw.cs);
@@ -435,6 +447,19 @@ void ensure_work(hh_solver* S, int nrhs) {
     }
   }
   if (S->p.cycle == 'K') S->kpart = dalloc<cx<double>>((size_t)64 * 2 * nrhs);
+  if (S->p.relax_type == HH_RELAX_JAC_GMRES) {
+    for (size_t l = 0; l + 1 < S->lv.size(); ++l) {
+      Level& L = S->lv[l];
+      const int pl = S->p.per_level ? std::max(S->p.pre_l[l], S->p.post_l[l]) : std::max(S->p.pre, S->p.post);
+      L.gv = dalloc<char>((size_t)L.g.N * nrhs * es * (pl + 1));
+    }
+    S->gm_part = dalloc<cx<double>>(gm_part_elems(nrhs));
+    S->gm_hc = dalloc<cx<double>>((size_t)(GM_MAX_STEPS + 1) * nrhs);
+    S->gm_H = dalloc<cx<double>>((size_t)nrhs * (GM_MAX_STEPS + 1) * GM_MAX_STEPS);
+    S->gm_sc = dalloc<double>(nrhs);
+    S->gm_beta = dalloc<double>(nrhs);
+    S->gm_y = dalloc<cx<double>>((size_t)nrhs * GM_MAX_STEPS);
+  }
   const int64_t N = S->lv[0].g.N;
   const int K = S->p.inner;
   const size_t blk = (size_t)N * nrhs;
@@ -571,7 +596,8 @@ struct Cycle {
   // fused legs: 2-D mirror-ghost levels with a coarser level below and V/W(2,2)
   bool use_fused(const Level& L) const {
     const size_t l = &L - &S->lv[0];
-    return S->p.fused && S->p.cycle != 'K' && l + 1 < S->lv.size() && pre_at((int)l) == 2 &&
+    return S->p.fused && S->p.cycle != 'K' && S->p.relax_type == HH_RELAX_JAC && l + 1 < S->lv.size() &&
+           pre_at((int)l) == 2 &&
            post_at((int)l) == 2 && fused_ok(L.g);
   }
   int rows_of(const Level& L) const {
@@ -579,6 +605,34 @@ struct Cycle {
     return l < 16 ? S->rows_tuned[l] : 0;
   }
 
+  // Jac-GMRES smoothing of L.cur in place (zero_x: L.cur is written from x = 0)
+  void gm(Level& L, const cx<T>* b, int steps, bool zero_x) {
+    GmArgs<T> a;
+    a.g = L.g;
+    a.nrhs = nrhs;
+    a.b = b;
+    a.x = (cx<T>*)L.cur;
+    a.zero_x = zero_x;
+    a.d = (const cx<T>*)L.d;
+    a.f = (const cx<T>*)L.f;
+    a.V = (cx<T>*)L.gv;
+    a.u = (cx<T>*)L.r;
+    a.part = S->gm_part;
+    a.hc = S->gm_hc;
+    a.H = S->gm_H;
+    a.sc = S->gm_sc;
+    a.beta = S->gm_beta;
+    a.y = S->gm_y;
+    a.skip = skip;
+    gm_smooth<T>(a, steps, st);
+  }
+  // post-smoothing with the configured relaxation
+  void post_smooth(Level& L, const cx<T>* b, int n) {
+    if (S->p.relax_type == HH_RELAX_JAC_GMRES)
+      gm(L, b, n, false);
+    else
+      sweeps(L, b, n);
+  }
   void sweeps(Level& L, const cx<T>* b, int n) {
     for (int s = 0; s < n; ++s) {
       void* o = other(L);
@@ -688,12 +742,16 @@ struct Cycle {
     }
     if (zero_init) {
       L.cur = first ? first : L.xa;
-      if (pre > 0) {
+      if (pre > 0 && S->p.relax_type == HH_RELAX_JAC_GMRES) {
+        gm(L, b, pre, true);
+      } else if (pre > 0) {
         launch_jacobi0<T>(L.g, nrhs, b, (cx<T>*)L.cur, (const cx<T>*)L.f, skip, st);
         sweeps(L, b, pre - 1);
       } else {
         launch_zero<T>((cx<T>*)L.cur, L.g.N * nrhs, st);
       }
+    } else if (S->p.relax_type == HH_RELAX_JAC_GMRES) {
+      gm(L, b, pre, false);
     } else {
       sweeps(L, b, pre);
     }
@@ -709,7 +767,7 @@ struct Cycle {
         krylov_correction(C, l + 1);
       }
       launch_prolong_add<T>(L.g, C.g, nrhs, (const cx<T>*)C.cur, (cx<T>*)L.cur, skip, st);
-      sweeps(L, b, post);
+      post_smooth(L, b, post);
       return;
     }
     // W: two coarse-grid corrections per visit, each from this level's
@@ -724,7 +782,7 @@ struct Cycle {
       run(l + 1, (const cx<T>*)C.b, true, nullptr);
       launch_prolong_add<T>(L.g, C.g, nrhs, (const cx<T>*)C.cur, (cx<T>*)L.cur, skip, st);
     }
-    sweeps(L, b, post);
+    post_smooth(L, b, post);
   }
 
   // z = M^-1 b on the fine level, result written to `out`
@@ -742,7 +800,9 @@ struct Cycle {
     // choose the ping-pong pair (out, xa) so the last write lands in out
     void* saved_xb = L0.xb;
     L0.xb = out;
-    void* first = (pre_at(0) == 0) ? (void*)out : ((flips % 2 == 1) ? (void*)out : L0.xa);
+    // (Jac-GMRES smoothing updates the iterate in place: start in out)
+    void* first = (pre_at(0) == 0 || S->p.relax_type == HH_RELAX_JAC_GMRES) ? (void*)out
+                                                                           : ((flips % 2 == 1) ? (void*)out : L0.xa);
     run(0, b, true, first);
     if (L0.cur != out)
       HIPCK(hipMemcpyAsync(out, L0.cur, sizeof(cx<T>) * L0.g.N * nrhs, hipMemcpyDeviceToDevice, st));
@@ -1253,6 +1313,14 @@ int hh_solver_create(hh_op* op, const hh_mg_params* p, const double* shift_per_l
       for (int l = 0; l < p->levels; ++l)
         if (p->pre_l[l] < 0 || p->post_l[l] < 0 || p->pre_l[l] > 64 || p->post_l[l] > 64)
           throw HHError(HH_ERR_ARG, "bad per-level sweep counts");
+    if (p->relax_type != HH_RELAX_JAC && p->relax_type != HH_RELAX_JAC_GMRES)
+      throw HHError(HH_ERR_ARG, "relax_type must be HH_RELAX_JAC or HH_RELAX_JAC_GMRES");
+    if (p->relax_type == HH_RELAX_JAC_GMRES) {
+      bool ok = p->pre <= GM_MAX_STEPS && p->post <= GM_MAX_STEPS;
+      if (p->per_level)
+        for (int l = 0; l < p->levels; ++l) ok = ok && p->pre_l[l] <= GM_MAX_STEPS && p->post_l[l] <= GM_MAX_STEPS;
+      if (!ok) throw HHError(HH_ERR_ARG, "Jac-GMRES smoothing takes at most 8 steps per call");
+    }
     if (p->cycle != 'V' && p->cycle != 'W' && p->cycle != 'K') throw HHError(HH_ERR_ARG, "cycle must be 'V', 'W' or 'K'");
     if (p->krylov != HH_FGMRES && p->krylov != HH_BICGSTAB) throw HHError(HH_ERR_ARG, "unknown krylov method");
     if (p->prec != HH_C128 && p->prec != HH_C64) throw HHError(HH_ERR_ARG, "prec must be HH_C128 or HH_C64");

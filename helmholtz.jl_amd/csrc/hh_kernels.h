@@ -204,4 +204,35 @@ void bicg_xr(const KrylovWork& w, const cx<double>* sc, const cx<TZ>* ph, const 
 // adjoint-solve transforms: mode 0 v = W^-1 conj(v), 1 v = W conj(v), 2 v = W v
 void adjoint_map(const Grid& g, cx<double>* v, int nrhs, int mode, hipStream_t s);
 
+// ---- Jac-GMRES smoothing (hh_gmsmooth.hip) ----
+constexpr int GM_MAX_STEPS = 8;  // GMRES steps per smoothing call
+// One level's operands and workspace: b, x (updated in place; zero_x: x = 0
+// on entry and is written), diagonal d and damped-Jacobi factor f, basis V
+// ((GM_MAX_STEPS + 1) blocks of N x nrhs), u (one block), and the per-RHS
+// complex128 state: part (gm_part_elems), hc ((GM_MAX_STEPS + 1) x nrhs),
+// H (nrhs x (GM_MAX_STEPS + 1) x GM_MAX_STEPS), sc, beta (nrhs), y
+// (nrhs x GM_MAX_STEPS).
+template <typename T>
+struct GmArgs {
+  Grid g;
+  int nrhs;
+  const cx<T>* b;
+  cx<T>* x;
+  bool zero_x;
+  const cx<T>* d;
+  const cx<T>* f;
+  cx<T>* V;
+  cx<T>* u;
+  cx<double>* part;
+  cx<double>* hc;
+  cx<double>* H;
+  double* sc;
+  double* beta;
+  cx<double>* y;
+  const int* skip;
+};
+size_t gm_part_elems(int nrhs);
+template <typename T>
+void gm_smooth(const GmArgs<T>& a, int steps, hipStream_t st);
+
 }  // namespace hh

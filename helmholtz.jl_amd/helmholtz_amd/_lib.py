@@ -27,6 +27,7 @@ EXPORTS = (
     "hh_unet_train", "hh_unet_eval", "hh_solver_attach_unet", "hh_stream_solve", "hh_solver_precondition",
     "hh_solver_time_kernel", "hh_solver_kernel_time", "hh_op_create_ho",
 )
+HH_RELAX_JAC, HH_RELAX_JAC_GMRES = 0, 1
 HH_KT_NONE, HH_KT_DOWN, HH_KT_MID, HH_KT_UP, HH_KT_APPLY, HH_KT_DOTS, HH_KT_UPDATE = range(7)
 
 
@@ -43,6 +44,7 @@ class MGParams(C.Structure):
         ("krylov", C.c_int32), ("prec", C.c_int32), ("learned", C.c_int32), ("verbose", C.c_int32),
         ("graph", C.c_int32), ("autotune", C.c_int32), ("fused", C.c_int32), ("wlevels", C.c_int32),
         ("per_level", C.c_int32), ("pre_l", C.c_int32 * 16), ("post_l", C.c_int32 * 16),
+        ("relax_type", C.c_int32),
     ]
 
 

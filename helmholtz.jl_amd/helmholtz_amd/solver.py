@@ -135,16 +135,17 @@ def _validate(s: ShiftedLaplacianMultigridSolver):
     mg = s.MG
     if s.Krylov not in ("GMRES", "BiCGSTAB"):
         raise ValueError(f"Krylov={s.Krylov!r}: 'GMRES' (flexible GMRES) or 'BiCGSTAB'")
-    if mg.relaxType != "Jac":
-        raise NotImplementedError(f"relaxType={mg.relaxType!r}: damped Jacobi ('Jac') only")
+    if mg.relaxType not in ("Jac", "Jac-GMRES"):
+        raise NotImplementedError(f"relaxType={mg.relaxType!r}: 'Jac' or 'Jac-GMRES'")
     if str(mg.cycleType) not in ("V", "W", "K"):
         raise NotImplementedError(f"cycleType={mg.cycleType!r}: 'V', 'W' and 'K' only")
     if int(mg.levels) > 16:
         raise ValueError("at most 16 levels")
     for l in range(int(mg.levels)):
         for c in _sweeps(mg, l):
-            if not 0 <= c <= 64:
-                raise ValueError(f"sweep count {c} on level {l + 1}: 0..64")
+            cmax = 8 if mg.relaxType == "Jac-GMRES" else 64
+            if not 0 <= c <= cmax:
+                raise ValueError(f"sweep count {c} on level {l + 1}: 0..{cmax}")
     if mg.prec not in ("c128", "c64"):
         raise ValueError("MG.prec must be 'c128' or 'c64'")
 
@@ -176,7 +177,8 @@ def _ensure_handle(s: ShiftedLaplacianMultigridSolver, outer=None):
                          prec=L.HH_C64 if mg.prec == "c64" else L.HH_C128, learned=0,
                          verbose=int(bool(s.verbose)), graph=int(bool(mg.graph)),
                          autotune=int(bool(mg.autotune)), fused=int(bool(mg.fused)),
-                         wlevels=int(mg.wlevels))
+                         wlevels=int(mg.wlevels),
+                         relax_type=L.HH_RELAX_JAC_GMRES if mg.relaxType == "Jac-GMRES" else L.HH_RELAX_JAC)
         if callable(mg.relaxPre) or callable(mg.relaxPost):
             prm.per_level = 1
             for l, (npre, npost) in enumerate(sw):

@@ -23,7 +23,9 @@ struct HHMGParams
     tol::Float64; inner::Int32; krylov::Int32; prec::Int32; learned::Int32; verbose::Int32
     graph::Int32; autotune::Int32; fused::Int32; wlevels::Int32
     per_level::Int32; pre_l::NTuple{16,Int32}; post_l::NTuple{16,Int32}
+    relax_type::Int32
 end
+const HH_RELAX_JAC = Cint(0); const HH_RELAX_JAC_GMRES = Cint(1)
 
 # relaxPre / relaxPost: a count or a function of the level (finest = 1)
 sweeps_at(v, l) = v isa Function ? Int32(v(l)) : Int32(v)
@@ -116,7 +118,8 @@ function ensure_handle!(s::ShiftedLaplacianMultigridSolverAMD)
                              Int32(mg.cycleType), mg.maxOuterIter, mg.relativeTol, s.inner,
                              s.Krylov == "BiCGSTAB" ? HH_BICGSTAB : HH_FGMRES, s.prec, 0, s.verbose, 0, 0, 1, 0,
                              pl ? 1 : 0, per_level_sweeps(mg.relaxPre, mg.levels),
-                             per_level_sweeps(mg.relaxPost, mg.levels)))
+                             per_level_sweeps(mg.relaxPost, mg.levels),
+                             mg.relaxType == "Jac-GMRES" ? HH_RELAX_JAC_GMRES : HH_RELAX_JAC))
         out = Ref{Ptr{Cvoid}}(C_NULL)
         check(ccall((:hh_solver_create, LIB), Cint, (Ptr{Cvoid}, Ref{HHMGParams}, Ptr{Float64}, Ptr{Ptr{Cvoid}}),
                     s.op.ptr, prm, s.shift, out))

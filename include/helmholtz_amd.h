@@ -33,6 +33,9 @@ enum { HH_C128 = 0, HH_C64 = 1 };
 enum { HH_HOST = 0, HH_DEVICE = 1 };
 /* outer Krylov method (ShiftedLaplacianMultigridSolver.Krylov) */
 enum { HH_FGMRES = 0, HH_BICGSTAB = 1 };
+/* smoother (MG.relaxType): "Jac" damped Jacobi, "Jac-GMRES" GMRES steps
+ * right-preconditioned by damped Jacobi (test/ShiftedLaplacianTest.jl:138) */
+enum { HH_RELAX_JAC = 0, HH_RELAX_JAC_GMRES = 1 };
 /* per right-hand-side outcome of hh_solve */
 enum { HH_CONVERGED = 0, HH_MAXITER = 1, HH_NONFINITE = 2, HH_ZERO_RHS = 3 };
 
@@ -46,7 +49,8 @@ typedef struct hh_dataset hh_dataset;
  * plus the solver's Krylov/inner (src/ShiftedLaplacianMultigridSolver.jl:8-9). */
 typedef struct hh_mg_params {
   int32_t levels;     /* MG.levels (>= 1; every coarsening halves the cells)     */
-  int32_t pre;        /* MG.relaxPre: damped-Jacobi sweeps before the correction */
+  int32_t pre;        /* MG.relaxPre: damped-Jacobi sweeps (or Jac-GMRES steps,
+                         at most 8) before the correction                      */
   int32_t post;       /* MG.relaxPost                                            */
   double relax;       /* MG.relaxParam: Jacobi weight                            */
   int32_t cycle;      /* MG.cycleType: 'V', 'W' or 'K' (Krylov-accelerated: two
@@ -68,6 +72,8 @@ typedef struct hh_mg_params {
                          and relaxPost given as functions of the level) */
   int32_t pre_l[16];  /* per_level: sweeps before the correction on level l (0 = finest) */
   int32_t post_l[16]; /* per_level: sweeps after it */
+  int32_t relax_type; /* MG.relaxType: HH_RELAX_JAC or HH_RELAX_JAC_GMRES (the
+                         fused legs run only with HH_RELAX_JAC) */
 } hh_mg_params;
 
 /* Per-call results.  The arrays are caller-owned with nrhs entries, or NULL. */

@@ -17,7 +17,8 @@ the control flow of ``solveLinearSystem`` (src/ShiftedLaplacianMultigridSolver.j
   ``GetHelmholtzOperator`` on the coarse mesh;
 * transfers: nodal bi/tri-linear prolongation P and full-weighting restriction
   R with mirror ghosts (the operator's own boundary convention), R 1 = 1;
-* smoother: damped Jacobi x <- x + w D^-1 (b - A x);
+* smoother: damped Jacobi x <- x + w D^-1 (b - A x), or ("Jac-GMRES") a few
+  GMRES steps right-preconditioned by w D^-1 (``jac_gmres``);
 * coarsest level: sparse direct solve (SuperLU);
 * outer: right-preconditioned flexible GMRES(inner) with restarts, x0 = 0,
   stopping on ||r|| <= tol ||b|| (the Arnoldi estimate), modified Gram-Schmidt.
@@ -120,7 +121,50 @@ def jacobi(lev: Level, x, b, w, sweeps):
     return x
 
 
-def kcycle_correction(hier, l, rc, w, pre, post, wlevels):
+def jac_gmres(lev: Level, x, b, w, steps):
+    """GMRES-accelerated damped-Jacobi smoothing ("Jac-GMRES", a relaxType of
+    Multigrid.jl, used by test/ShiftedLaplacianTest.jl:137-142): ``steps``
+    steps of GMRES on A x = b from the current x, right-preconditioned by the
+    damped Jacobi matrix M = w D^-1, i.e. the iterate minimising ||b - A x||
+    over x0 + M K_steps(A M, r0).  Arnoldi with modified Gram-Schmidt; the
+    small least-squares problem by numpy; an exact (zero-norm) Krylov
+    direction ends the iteration early."""
+    if steps <= 0:
+        return x
+    Minv = w * lev.Dinv
+    r = b - lev.A @ x
+    beta = np.linalg.norm(r)
+    if beta == 0.0:
+        return x
+    n = r.shape[0]
+    V = np.zeros((n, steps + 1), dtype=complex)
+    H = np.zeros((steps + 1, steps), dtype=complex)
+    V[:, 0] = r / beta
+    k = steps
+    for i in range(steps):
+        wv = lev.A @ (Minv * V[:, i])
+        for q in range(i + 1):
+            H[q, i] = np.vdot(V[:, q], wv)
+            wv = wv - H[q, i] * V[:, q]
+        H[i + 1, i] = np.linalg.norm(wv)
+        if H[i + 1, i] == 0.0:
+            k = i + 1
+            break
+        V[:, i + 1] = wv / H[i + 1, i]
+    e1 = np.zeros(k + 1, dtype=complex)
+    e1[0] = beta
+    y = np.linalg.lstsq(H[:k + 1, :k], e1, rcond=None)[0]
+    return x + Minv * (V[:, :k] @ y)
+
+
+def smooth(lev: Level, x, b, w, steps, relax="Jac"):
+    """The level's smoother: damped Jacobi sweeps or Jac-GMRES steps."""
+    if relax == "Jac-GMRES":
+        return jac_gmres(lev, x, b, w, steps)
+    return jacobi(lev, x, b, w, steps)
+
+
+def kcycle_correction(hier, l, rc, w, pre, post, wlevels, relax="Jac"):
     """Coarse correction of the K-cycle (Notay & Vassilevski, Numer. Linear
     Algebra Appl. 15, 2008): two steps of a minimal-residual (GCR) iteration
     on A_{l+1} e = rc, each preconditioned by one K-cycle on level l+1.
@@ -128,12 +172,12 @@ def kcycle_correction(hier, l, rc, w, pre, post, wlevels):
     c2 = B r2, v2 = A c2 made orthogonal to v1 (c2 alongside),
     a2 = <v2, r2>/<v2, v2>; e = a1 c1 + a2 c2 (a zero <v, v> gives a zero step)."""
     A = hier[l + 1].A
-    c1 = vcycle(hier, l + 1, rc, w, pre, post, "K", wlevels)
+    c1 = vcycle(hier, l + 1, rc, w, pre, post, "K", wlevels, relax)
     v1 = A @ c1
     n1 = np.vdot(v1, v1).real
     a1 = np.vdot(v1, rc) / n1 if n1 > 0 else 0.0
     r2 = rc - a1 * v1
-    c2 = vcycle(hier, l + 1, r2, w, pre, post, "K", wlevels)
+    c2 = vcycle(hier, l + 1, r2, w, pre, post, "K", wlevels, relax)
     v2 = A @ c2
     g = np.vdot(v1, v2) / n1 if n1 > 0 else 0.0
     v2 = v2 - g * v1
@@ -154,7 +198,7 @@ def sweeps_at(v, l):
     return int(v)
 
 
-def vcycle(hier, l, b, w=0.8, pre=2, post=2, cycle="V", wlevels=0):
+def vcycle(hier, l, b, w=0.8, pre=2, post=2, cycle="V", wlevels=0, relax="Jac"):
     """One V-, W- or K-cycle for A_l x = b from x = 0 (``wlevels`` > 0: the W
     double correction only on levels < wlevels, single below; ``pre`` /
     ``post`` per level through ``sweeps_at``).
@@ -171,21 +215,21 @@ def vcycle(hier, l, b, w=0.8, pre=2, post=2, cycle="V", wlevels=0):
         return lev.lu.solve(b)
     npre, npost = sweeps_at(pre, l), sweeps_at(post, l)
     x = np.zeros_like(b)
-    x = jacobi(lev, x, b, w, npre)
+    x = smooth(lev, x, b, w, npre, relax)
     if cycle == "K":
         r = b - lev.A @ x
         rc = lev.R @ r
         if l + 1 == len(hier) - 1:
             xc = hier[l + 1].lu.solve(rc)
         else:
-            xc = kcycle_correction(hier, l, rc, w, pre, post, wlevels)
+            xc = kcycle_correction(hier, l, rc, w, pre, post, wlevels, relax)
         x = x + lev.P @ xc
-        return jacobi(lev, x, b, w, npost)
+        return smooth(lev, x, b, w, npost, relax)
     for _ in range(2 if (cycle == "W" and (wlevels <= 0 or l < wlevels)) else 1):
         r = b - lev.A @ x
-        xc = vcycle(hier, l + 1, lev.R @ r, w, pre, post, cycle, wlevels)
+        xc = vcycle(hier, l + 1, lev.R @ r, w, pre, post, cycle, wlevels, relax)
         x = x + lev.P @ xc
-    x = jacobi(lev, x, b, w, npost)
+    x = smooth(lev, x, b, w, npost, relax)
     return x
 
 

@@ -47,13 +47,17 @@ def test_argument_errors_are_reported_without_touching_the_gpu():
 
 
 def test_solver_rejects_unsupported_settings():
-    mg = ha.getMGparam(complex, int, 3, 1, 30, 1e-6, "Jac-GMRES", 0.75, 2, 2, "K", "NoMUMPS", 0.5, 0.0)
+    mg = ha.getMGparam(complex, int, 3, 1, 30, 1e-6, "GS", 0.75, 2, 2, "K", "NoMUMPS", 0.5, 0.0)
     M = ha.getRegularMesh([0, 1, 0, 1], [8, 8])
     import numpy as np
     hp = ha.HelmholtzParam(M, np.zeros(81), np.ones(81), 1.0, False, True)
     s = ha.getShiftedLaplacianMultigridSolver(hp, mg, 0.5, "GMRES", 5)
     with pytest.raises(NotImplementedError):
         ha.solveLinearSystem(None, np.ones(81, complex), s)
+    # Jac-GMRES takes at most 8 GMRES steps per smoothing call
+    mg9 = ha.getMGparam(complex, int, 3, 1, 30, 1e-6, "Jac-GMRES", 0.75, 9, 2, "K", "NoMUMPS", 0.5, 0.0)
+    with pytest.raises(ValueError):
+        ha.solveLinearSystem(None, np.ones(81, complex), ha.getShiftedLaplacianMultigridSolver(hp, mg9, 0.5, "GMRES", 5))
     s2 = ha.getShiftedLaplacianMultigridSolver(hp, ha.getMGparam(3, 1, 30, 1e-6, "Jac", 0.8, 2, 2, "V", "NoMUMPS"),
                                                0.5)
     assert s2.Krylov == "BiCGSTAB" and s2.inner == 5 and list(s2.shift) == [0.5] * 3

@@ -323,6 +323,55 @@ def test_per_level_sweeps_solve():
     assert abs(int(s.last_stats["iters"][0]) - ito) <= 2
 
 
+@pytest.mark.parametrize("cycle", ["V", "W", "K"])
+@pytest.mark.parametrize("steps", [(2, 2), (1, 3), ("l+1", "l+1")])
+def test_jac_gmres_cycle_matches_oracle(cycle, steps):
+    """relaxType = "Jac-GMRES" (test/ShiftedLaplacianTest.jl:138): GMRES steps
+    right-preconditioned by damped Jacobi as the smoother.  One complex128
+    cycle equals the oracle's (modified Gram-Schmidt there, classical
+    Gram-Schmidt twice here: 1e-10 relative); a zero column stays zero."""
+    M, m, g, w = small_case((129, 97), seed=71, neumann=True)
+    rng = np.random.default_rng(72)
+    V = np.asfortranarray(rng.standard_normal((m.size, 3)) + 1j * rng.standard_normal((m.size, 3)))
+    V[:, 1] = 0.0
+    s = _solver(M, m, g, w, True, 4, 0.5, cycle=cycle)
+    s.MG.relaxType = "Jac-GMRES"
+    pre, post = [(lambda l: l + 1) if c == "l+1" else c for c in steps]
+    s.MG.relaxPre, s.MG.relaxPost = pre, post
+    Z = ha.precondition(s, V)
+    p = hr.HelmholtzParam(M, g, m, w, True, True)
+    hier = sr.build_hierarchy(p, 4, 0.5)
+    assert np.all(Z[:, 1] == 0)
+    for c in (0, 2):
+        zo = sr.vcycle(hier, 0, V[:, c], 0.8, pre, post, cycle, 0, "Jac-GMRES")
+        err = np.linalg.norm(Z[:, c] - zo) / np.linalg.norm(zo)
+        assert err <= 1e-10, err
+
+
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+def test_jac_gmres_kcycle_solve(prec):
+    """The reference test's last solve (test/ShiftedLaplacianTest.jl:137-142):
+    K-cycle, Jac-GMRES relaxation, FGMRES(5), two random RHS.  True residual
+    <= 1e-6 and iterations within 1 (c128) / 2 (c64) of the oracle's."""
+    M, m, g, w = small_case((129, 129), seed=73, neumann=True)
+    p = hr.HelmholtzParam(M, g, m, w, True, True)
+    H = hr.operator_from_param(p)
+    rng = np.random.default_rng(74)
+    B = np.asfortranarray(rng.standard_normal((m.size, 2)) + 1j * rng.standard_normal((m.size, 2)))
+    s = _solver(M, m, g, w, True, 3, 0.5, cycle="K", prec=prec, inner=5)
+    s.MG.relaxType = "Jac-GMRES"
+    X, s = ha.solveLinearSystem(None, B, s)
+    hier = sr.build_hierarchy(p, 3, 0.5)
+    for c in range(2):
+        res = np.linalg.norm(H @ X[:, c] - B[:, c]) / np.linalg.norm(B[:, c])
+        assert res <= 1e-6, res
+        xo, ito, _ = sr.fgmres(lambda v: H @ v, B[:, c],
+                               lambda v: sr.vcycle(hier, 0, v, 0.8, 2, 2, "K", 0, "Jac-GMRES"), 5, 100, 1e-6)
+        it = int(s.last_stats["iters"][c])
+        print(f"Jac-GMRES K-cycle {prec} rhs {c}: gpu {it} oracle {ito} res {res:.2e}")
+        assert abs(it - ito) <= (1 if prec == "c128" else 2)
+
+
 @pytest.mark.parametrize("nodes", [(257, 129), (129, 65), (97, 33), (1025, 513)])
 @pytest.mark.parametrize("prec", ["c64", "c128"])
 def test_fused_legs_equal_unfused(nodes, prec):

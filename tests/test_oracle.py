@@ -471,3 +471,53 @@ def test_per_level_sweeps_equal_error_propagation(cycle):
     # the swapped pair is a different cycle
     zs = sr.vcycle(hier, 0, b, 0.8, post, pre, cycle)
     assert np.linalg.norm(zs - z) > 1e-6 * np.linalg.norm(z)
+
+
+@pytest.mark.parametrize("steps", [1, 2, 3])
+def test_jac_gmres_is_minimal_residual_over_preconditioned_krylov_space(steps):
+    """Jac-GMRES smoothing: the new iterate minimises ||b - A x|| over
+    x0 + M span{r0, (A M) r0, ..., (A M)^(s-1) r0} with M = w D^-1.  Checked
+    against a brute-force least-squares solve on the explicit Krylov basis;
+    one step has the closed form alpha = <A M r0, r0> / ||A M r0||^2."""
+    M, m, g, w = small_case((17, 13), seed=41)
+    p = hr.HelmholtzParam(M, g, m, w, True, True)
+    lev = sr.build_hierarchy(p, 2, 0.5)[0]
+    rng = np.random.default_rng(42)
+    n = lev.A.shape[0]
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    x0 = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    Mi = 0.8 * lev.Dinv
+    r0 = b - lev.A @ x0
+    K = [Mi * r0]
+    for _ in range(steps - 1):
+        K.append(Mi * (lev.A @ K[-1]))
+    K = np.stack(K, axis=1)
+    c = np.linalg.lstsq(lev.A @ K, r0, rcond=None)[0]
+    xb = x0 + K @ c
+    x = sr.jac_gmres(lev, x0, b, 0.8, steps)
+    rb, rx = np.linalg.norm(b - lev.A @ xb), np.linalg.norm(b - lev.A @ x)
+    assert rx <= rb * (1 + 1e-9)
+    assert np.linalg.norm(x - xb) <= 1e-8 * np.linalg.norm(xb)
+    if steps == 1:
+        q = lev.A @ (Mi * r0)
+        alpha = np.vdot(q, r0) / np.vdot(q, q)
+        assert np.linalg.norm(x - (x0 + alpha * Mi * r0)) <= 1e-12 * np.linalg.norm(x)
+    # the smoother never increases the residual and zero steps change nothing
+    assert rx <= np.linalg.norm(r0)
+    assert np.array_equal(sr.jac_gmres(lev, x0, b, 0.8, 0), x0)
+
+
+def test_kcycle_with_jac_gmres_converges():
+    """The reference test's last configuration (test/ShiftedLaplacianTest.jl:137-142):
+    K-cycle with Jac-GMRES relaxation inside FGMRES(5) reaches 1e-6."""
+    M, m, g, w = small_case((65, 65), seed=43)
+    p = hr.HelmholtzParam(M, g, m, w, True, True)
+    H = hr.operator_from_param(p)
+    hier = sr.build_hierarchy(p, 3, 0.5)
+    b = np.zeros(m.size, complex)
+    b[32 + 65 * 32] = 1.0
+    x, its, _ = sr.fgmres(lambda v: H @ v, b, lambda v: sr.vcycle(hier, 0, v, 0.8, 2, 2, "K", 0, "Jac-GMRES"),
+                          5, 60, 1e-6)
+    assert np.linalg.norm(H @ x - b) <= 1e-6 * np.linalg.norm(b)
+    xj, itsj, _ = sr.fgmres(lambda v: H @ v, b, lambda v: sr.vcycle(hier, 0, v, 0.8, 2, 2, "K", 0, "Jac"), 5, 60, 1e-6)
+    print(f"K-cycle FGMRES(5): Jac-GMRES {its} iterations, Jac {itsj}")
