@@ -19,7 +19,9 @@ plain cycle, and the plain method (complex128 preconditioner, one source per
 call) on a subset of sources, are reported beside it; the line's ``value`` is
 the learned+batched ms/RHS.  ``roofline`` times every launch of the dominant
 kernel inside the timed solves with HIP events; ``cpu_baseline`` solves one
-source with the oracle's C/OpenMP restatement of the plain method.
+source with the oracle's C/OpenMP restatement of the plain method;
+``three_d`` times the 3-D case (193 x 193 x 97, 32 sources) with a complex64
+and a complex128 preconditioner (``--no-3d`` skips it).
 
 Usage: python bench.py [--gpus 1] [--steps 2] [--warmup 1] [--plain-rhs 4]
 """
@@ -58,6 +60,44 @@ def c2_problem(nsrc: int):
 # workload a uniform 0.5 needs 261 iterations, this 245; a uniform 0.4 or a
 # 0.45 on the third level does not converge in 2000.
 SHIFTS = (0.4, 0.5, 0.5, 0.5, 0.5, 0.5, 0.5, 0.5)
+
+
+def c3_problem():
+    """The 3-D case: 193 x 193 x 97 nodes, h = 10 m, the layered-plus-body
+    medium extruded with seeded lateral shifts, 8 ppw at 1500 m/s, a 16-cell
+    absorbing layer, 32 sources on depth index 2 in a 4 x 8 lattice."""
+    import helmholtz_amd as ha
+    from helmholtz_amd import media
+    n, h = (193, 193, 97), 10.0
+    v = media.layered_body_velocity_3d(n, seed=3)
+    m = (1.0 / v**2).ravel(order="F")
+    M = ha.getRegularMesh([0, h * (n[0] - 1), 0, h * (n[1] - 1), 0, h * (n[2] - 1)], [n[0] - 1, n[1] - 1, n[2] - 1])
+    w = media.omega_for_ppw(1500.0, h, 8)
+    _, gamma = ha.GetHelmholtzOperator(M, m, w, 0.01 * w * np.ones(m.size), True, [16, 16, 16], w, True)
+    hp = ha.HelmholtzParam(M, gamma.ravel(order="F"), m, w, True, True)
+    subs = [(int(round(1 + (n[0] - 1) * (i + 0.5) / 8)), int(round(1 + (n[1] - 1) * (j + 0.5) / 4)), 2)
+            for j in range(4) for i in range(8)]
+    return hp, ha.point_sources(M, subs)
+
+
+def c3_lines(log):
+    """Complex64 vs complex128 preconditioner on the 3-D case: 5-level W(2,2)
+    cycle (the coarsest level must have at most 4096 nodes for the dense
+    solve, so 4 levels do not fit), FGMRES(8) in complex128 to 1e-6."""
+    import helmholtz_amd as ha
+    from helmholtz_amd.device import DeviceArray
+    hp, B = c3_problem()
+    Bd = DeviceArray.from_numpy(B)
+    out = {"workload": "3-D 193x193x97, 32 sources, 8 ppw, 5-level W(2,2), FGMRES(8), 1e-6 true residual"}
+    for prec in ("c64", "c128"):
+        mg = ha.getMGparam(complex, int, 5, 1, 100, 1e-6, "Jac", 0.8, 2, 2, "W", "NoMUMPS", 0.5, 0.0)
+        mg.prec = prec
+        s = ha.getShiftedLaplacianMultigridSolver(hp, mg, 0.5, "GMRES", 8)
+        t, st = timed_solves(ha, s, Bd, 1, 1, log, f"3-D {prec}")
+        out[f"{prec}_ms_per_rhs"] = round(1e3 * t / B.shape[1], 3)
+        out[f"{prec}_max_iters"] = int(st["max_iters"])
+        out[f"{prec}_max_true_res"] = float(max(st["true_res"]))
+    return out
 
 
 def make_solver(hp, prec: str, levels=6, cycle="W", tol=1e-6, fast=False):
@@ -178,6 +218,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=40.0,
                     help="seconds the host baseline may spend on a full single-source solve")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-3d", action="store_true", help="skip the 3-D complex64 vs complex128 lines")
     ap.add_argument("--train-media", type=int, default=40)
     ap.add_argument("--train-epochs", type=int, default=6)
     args = ap.parse_args()
@@ -243,6 +284,7 @@ def main():
         log(f"plain rhs {c}: {dt:.3f} s, {plain_it[-1]} iters, res {plain_res[-1]:.2e}")
     plain_ms_rhs = 1e3 * float(np.mean(plain_t))
 
+    c3 = None if args.no_3d else c3_lines(log)
     calibration_copy(Bd, log)
     cpu = None
     if not args.no_cpu:
@@ -285,6 +327,7 @@ def main():
         "speedup_batched_vs_plain": round(plain_ms_rhs / batched_ms_rhs, 2),
         "roofline": roof,
         "cpu_baseline": cpu,
+        "three_d": c3,
     }
     print(json.dumps(line), flush=True)
 
