@@ -7,11 +7,12 @@
 // per-node complex diagonal d_p = -w^2 m_p (1 - i g_p / Re w) + Sommerfeld
 // terms (+ i s w^2 m_p for a shifted operator).
 //
-// Launch shape: 64 x 4 threads.  threadIdx.x walks the fastest axis (one
-// coalesced 64-wide row segment per wavefront), each thread marches a run of
-// rows/planes along the slowest axis keeping the previous/current/next values
-// in registers, so x is read from HBM once; x-neighbours come from the same
-// cache lines.  blockIdx.z is the right-hand side.
+// Launch shape: 256 threads over consecutive positions of a row (2-D) or of
+// a plane (3-D), numbered across the grid so no lane idles at a ragged row
+// end; each thread marches a run of rows/planes along the slowest axis keeping
+// the previous/current/next values in registers, so x is read from HBM once;
+// in-plane neighbours come from the same or nearby cache lines.  The XCD-aware
+// block order groups right-hand sides that share the diagonal in L2.
 #include "hh_kernels.h"
 
 namespace hh {
@@ -50,6 +51,11 @@ __device__ __forceinline__ void finish(cx<TO>* __restrict__ yo, const cx<TO>* __
   }
 }
 
+// One thread per (column position, run of `rows` rows/planes): the positions
+// of a row (2-D) or of a plane (3-D) are numbered consecutively across the
+// whole grid, so a block of 256 threads covers 256 consecutive nodes with no
+// idle lanes at ragged row ends; each thread then marches its run along the
+// slowest axis keeping the previous/current/next values in registers.
 template <typename TI, typename TO, typename TD, int MODE>
 __global__ __launch_bounds__(256) void k_stencil2d(Grid g, const cx<TI>* __restrict__ x,
                                                    const cx<TO>* __restrict__ b, cx<TO>* __restrict__ y,
@@ -59,10 +65,11 @@ __global__ __launch_bounds__(256) void k_stencil2d(Grid g, const cx<TI>* __restr
   int bx, by, rhs;
   tile_of(tm, bx, by, rhs);
   if (skip && skip[rhs]) return;
-  const int i = bx * 64 + threadIdx.x;
-  const int j0 = (by * 4 + threadIdx.y) * rows;
   const int n1 = g.n1, n2 = g.n2;
-  if (i >= n1 || j0 >= n2) return;
+  const unsigned item = (unsigned)bx * 256u + threadIdx.x, band = item / (unsigned)n1;
+  const int i = (int)(item - band * (unsigned)n1);
+  const int j0 = (int)band * rows;
+  if (j0 >= n2) return;
   const int j1 = min(j0 + rows, n2);
   const int64_t off = (int64_t)rhs * g.N;
   const cx<TI>* xr = x + off;
@@ -96,18 +103,16 @@ template <typename TI, typename TO, typename TD, int MODE>
 __global__ __launch_bounds__(256) void k_stencil3d(Grid g, const cx<TI>* __restrict__ x,
                                                    const cx<TO>* __restrict__ b, cx<TO>* __restrict__ y,
                                                    const cx<TD>* __restrict__ d, const double* __restrict__ m,
-                                                   cx<double> sfac, TD wj, int rows, int ytiles, TileMap tm,
+                                                   cx<double> sfac, TD wj, int rows, TileMap tm,
                                                    const int* __restrict__ skip, const cx<TD>* __restrict__ fj) {
   int bx, by, rhs;
   tile_of(tm, bx, by, rhs);
   if (skip && skip[rhs]) return;
-  const int i = bx * 64 + threadIdx.x;
-  const int yt = by % ytiles;
-  const int zc = by / ytiles;
-  const int j = yt * 4 + threadIdx.y;
-  const int k0 = zc * rows;
   const int n1 = g.n1, n2 = g.n2, n3 = g.n3;
-  if (i >= n1 || j >= n2 || k0 >= n3) return;
+  const unsigned item = (unsigned)bx * 256u + threadIdx.x, jj = item / (unsigned)n1;
+  const int i = (int)(item - jj * (unsigned)n1), j = (int)jj;
+  const int k0 = by * rows;
+  if (j >= n2 || k0 >= n3) return;
   const int k1 = min(k0 + rows, n3);
   const int64_t plane = (int64_t)n1 * n2;
   const int64_t off = (int64_t)rhs * g.N;
@@ -119,16 +124,18 @@ __global__ __launch_bounds__(256) void k_stencil3d(Grid g, const cx<TI>* __restr
   const TD ih1 = (TD)g.ih1, ih2 = (TD)g.ih2, ih3 = (TD)g.ih3;
   const TD ldxy = (TD)(g.ih1 * lap_diag1(i, n1, g.bc) + g.ih2 * lap_diag1(j, n2, g.bc));
   const int64_t col = (int64_t)j * n1 + i;
+  const int64_t c1l = (int64_t)j * n1 + il, c1r = (int64_t)j * n1 + ir;
+  const int64_t c2l = (int64_t)jl * n1 + i, c2r = (int64_t)jr * n1 + i;
   cx<TD> xm = cvt<TI, TD>(xr[(int64_t)lo_nb(k0, g.bc) * plane + col]);
   cx<TD> xc = cvt<TI, TD>(xr[(int64_t)k0 * plane + col]);
   for (int k = k0; k < k1; ++k) {
     const int64_t pl = (int64_t)k * plane;
     const int kp = hi_nb(k, n3, g.bc);
     cx<TD> xp = cvt<TI, TD>(xr[(int64_t)kp * plane + col]);
-    cx<TD> x1l = cvt<TI, TD>(xr[pl + (int64_t)j * n1 + il]);
-    cx<TD> x1r = cvt<TI, TD>(xr[pl + (int64_t)j * n1 + ir]);
-    cx<TD> x2l = cvt<TI, TD>(xr[pl + (int64_t)jl * n1 + i]);
-    cx<TD> x2r = cvt<TI, TD>(xr[pl + (int64_t)jr * n1 + i]);
+    cx<TD> x1l = cvt<TI, TD>(xr[pl + c1l]);
+    cx<TD> x1r = cvt<TI, TD>(xr[pl + c1r]);
+    cx<TD> x2l = cvt<TI, TD>(xr[pl + c2l]);
+    cx<TD> x2r = cvt<TI, TD>(xr[pl + c2r]);
     const int64_t p = pl + col;
     cx<TD> dv = node_diag<TD>(d, m, sfac, p);
     cx<TD> Ax = apply7(xc, x1l, x1r, x2l, x2r, xm, xp, ih1, ih2, ih3, dv);
@@ -142,7 +149,6 @@ __global__ __launch_bounds__(256) void k_stencil3d(Grid g, const cx<TI>* __restr
   }
 }
 
-// First Jacobi sweep from a zero guess: x = w D^-1 b (no stencil needed).
 template <typename T>
 __global__ __launch_bounds__(256) void k_jacobi0(Grid g, const cx<T>* __restrict__ b, cx<T>* __restrict__ x,
                                                  const cx<T>* __restrict__ f, const int* __restrict__ skip) {
@@ -584,20 +590,19 @@ void launch_dense_apply_mfma(const float* P, int64_t n, int nrhs, const cx<float
   hipLaunchKernelGGL(k_dense_apply_mfma, grid, dim3(256), 0, s, (const float4*)P, n2, ng, nrhs, (const float*)r,
                      (float*)x, skip);
 }
-static inline dim3 stencil_grid(const Grid& g, int nrhs, int rows, int* ytiles) {
+// blocks: (positions of a row/plane x runs) / 256 along x, runs of planes along y (3-D)
+static inline TileMap stencil_map(const Grid& g, int nrhs, int rows) {
   if (g.dim == 2) {
-    *ytiles = 1;
-    const int rowgroups = (g.n2 + rows - 1) / rows;
-    return dim3((g.n1 + 63) / 64, (rowgroups + 3) / 4, nrhs);
+    const int64_t items = (int64_t)g.n1 * ((g.n2 + rows - 1) / rows);
+    return make_tile_map((int)((items + 255) / 256), 1, nrhs);
   }
-  *ytiles = (g.n2 + 3) / 4;
-  const int zc = (g.n3 + rows - 1) / rows;
-  return dim3((g.n1 + 63) / 64, (*ytiles) * zc, nrhs);
+  const int64_t items = (int64_t)g.n1 * g.n2;
+  return make_tile_map((int)((items + 255) / 256), (g.n3 + rows - 1) / rows, nrhs);
 }
 
 static inline int pick_rows(const Grid& g, int nrhs) {
   // enough independent wavefronts to fill 256 CUs, long runs otherwise
-  const int64_t cols = (int64_t)((g.n1 + 63) / 64) * (g.dim == 3 ? g.n2 : 4) * nrhs;
+  const int64_t cols = ((int64_t)g.n1 * (g.dim == 3 ? g.n2 : 1) + 63) / 64 * nrhs;
   const int len = g.dim == 3 ? g.n3 : g.n2;
   int rows = 16;
   while (rows > 2 && cols * ((len + rows - 1) / rows) < 8192) rows >>= 1;
@@ -608,16 +613,15 @@ template <typename TI, typename TO, typename TD, int MODE>
 void launch_stencil(const Grid& g, int nrhs, const cx<TI>* x, const cx<TO>* b, cx<TO>* y, const cx<TD>* d,
                     const double* m, cx<double> sfac, double wj, const int* skip, hipStream_t s, int rows_override,
                     const cx<TD>* fj) {
-  int ytiles;
-  const int rows = rows_override > 0 ? rows_override : pick_rows(g, nrhs);
-  const dim3 grid = stencil_grid(g, nrhs, rows, &ytiles);
-  const TileMap tm = make_tile_map((int)grid.x, (int)grid.y, nrhs);
+  static const int env_rows = getenv("HH_ROWS") ? atoi(getenv("HH_ROWS")) : 0;
+  const int rows = env_rows > 0 ? env_rows : (rows_override > 0 ? rows_override : pick_rows(g, nrhs));
+  const TileMap tm = stencil_map(g, nrhs, rows);
   if (g.dim == 2)
-    hipLaunchKernelGGL((k_stencil2d<TI, TO, TD, MODE>), dim3(tm.T), dim3(64, 4), 0, s, g, x, b, y, d, m, sfac,
+    hipLaunchKernelGGL((k_stencil2d<TI, TO, TD, MODE>), dim3(tm.T), dim3(256), 0, s, g, x, b, y, d, m, sfac,
                        (TD)wj, rows, tm, skip, fj);
   else
-    hipLaunchKernelGGL((k_stencil3d<TI, TO, TD, MODE>), dim3(tm.T), dim3(64, 4), 0, s, g, x, b, y, d, m, sfac,
-                       (TD)wj, rows, ytiles, tm, skip, fj);
+    hipLaunchKernelGGL((k_stencil3d<TI, TO, TD, MODE>), dim3(tm.T), dim3(256), 0, s, g, x, b, y, d, m, sfac,
+                       (TD)wj, rows, tm, skip, fj);
 }
 
 template void launch_stencil<double, double, double, M_APPLY>(const Grid&, int, const cx<double>*,
