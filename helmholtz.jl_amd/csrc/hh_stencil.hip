@@ -253,6 +253,58 @@ __global__ __launch_bounds__(256) void k_prolong_add(Grid gf, Grid gc, const cx<
   xf[o] = xf[o] + acc;
 }
 
+// 3-D prolongation marching along z: a thread owns one fine (i, j) column
+// over a run of fine planes starting at an even plane.  The in-plane bilinear
+// value of each coarse plane K at (i, j) is formed once (weights 1 or 1/2 per
+// axis) and reused: fine plane 2K adds c_K, fine plane 2K + 1 adds
+// (c_K + c_{K+1}) / 2.
+template <typename T>
+__global__ __launch_bounds__(256) void k_prolong_add3(Grid gf, Grid gc, int rows, TileMap tm,
+                                                      const cx<T>* __restrict__ xc, cx<T>* __restrict__ xf,
+                                                      const int* __restrict__ skip) {
+  int bx, by, rhs;
+  tile_of(tm, bx, by, rhs);
+  if (skip && skip[rhs]) return;
+  const int n1 = gf.n1, n2 = gf.n2;
+  const unsigned item = (unsigned)bx * 256u + threadIdx.x, jj = item / (unsigned)n1;
+  const int i = (int)(item - jj * (unsigned)n1), j = (int)jj;
+  const int k0 = by * rows;
+  if (j >= n2 || k0 >= gf.n3) return;
+  const int k1 = min(k0 + rows, gf.n3);
+  const cx<T>* c = xc + (int64_t)rhs * gc.N;
+  cx<T>* x = xf + (int64_t)rhs * gf.N;
+  const int i0 = i >> 1, j0 = j >> 1;
+  const int i1 = (i & 1) ? i0 + 1 : i0, j1 = (j & 1) ? j0 + 1 : j0;
+  const T wi = (i & 1) ? (T)0.5 : (T)1, wj = (j & 1) ? (T)0.5 : (T)1;
+  const int64_t P1 = gc.n1, P2 = (int64_t)gc.n1 * gc.n2;
+  const int64_t a00 = j0 * P1 + i0, a01 = j0 * P1 + i1, a10 = j1 * P1 + i0, a11 = j1 * P1 + i1;
+  const bool two_i = (i & 1) != 0, two_j = (j & 1) != 0;
+  auto plane_val = [&](int K) {
+    const cx<T>* cp = c + (int64_t)K * P2;
+    cx<T> acc = (wi * wj) * cp[a00];
+    if (two_i) acc = acc + (wi * wj) * cp[a01];
+    if (two_j) {
+      acc = acc + (wi * wj) * cp[a10];
+      if (two_i) acc = acc + (wi * wj) * cp[a11];
+    }
+    return acc;
+  };
+  const int64_t plane = (int64_t)n1 * n2, col = (int64_t)j * n1 + i;
+  cx<T> cK = plane_val(k0 >> 1);
+  for (int k = k0; k < k1; ++k) {
+    cx<T> e;
+    if (k & 1) {
+      const cx<T> cK1 = plane_val((k >> 1) + 1);
+      e = (T)0.5 * (cK + cK1);
+      cK = cK1;
+    } else {
+      e = cK;
+    }
+    const int64_t o = (int64_t)k * plane + col;
+    x[o] = x[o] + e;
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_zero(cx<T>* __restrict__ a, int64_t n) {
   const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -686,6 +738,13 @@ void launch_restrict_real(const Grid& gf, const Grid& gc, const double* f, doubl
 template <typename T>
 void launch_prolong_add(const Grid& gf, const Grid& gc, int nrhs, const cx<T>* xc, cx<T>* xf, const int* skip,
                         hipStream_t s) {
+  if (gf.dim == 3) {
+    const int rows = 8;  // even: every run starts on a fine plane that coincides with a coarse one
+    const int64_t items = (int64_t)gf.n1 * gf.n2;
+    const TileMap tm = make_tile_map((int)((items + 255) / 256), (gf.n3 + rows - 1) / rows, nrhs);
+    hipLaunchKernelGGL((k_prolong_add3<T>), dim3(tm.T), dim3(256), 0, s, gf, gc, rows, tm, xc, xf, skip);
+    return;
+  }
   dim3 grid((unsigned)((gf.N + 255) / 256), nrhs);
   hipLaunchKernelGGL((k_prolong_add<T>), grid, dim3(256), 0, s, gf, gc, xc, xf, skip);
 }
