@@ -195,6 +195,53 @@ __global__ __launch_bounds__(256) void k_restrict(Grid gf, Grid gc, const cx<T>*
   bc[(int64_t)rhs * gc.N + pc] = acc;
 }
 
+// 3-D full weighting marching along z: a thread owns one coarse (I, J) column
+// over a run of coarse planes; the in-plane 9-point weighted sum s_f of each
+// fine plane f at (I, J) is formed once and the coarse value is
+// s_{2K-1} / 4 + s_{2K} / 2 + s_{2K+1} / 4 (mirror ghosts on every axis).
+template <typename T>
+__global__ __launch_bounds__(256) void k_restrict3(Grid gf, Grid gc, int rows, TileMap tm,
+                                                   const cx<T>* __restrict__ rf, cx<T>* __restrict__ bc,
+                                                   const int* __restrict__ skip) {
+  int bx, by, rhs;
+  tile_of(tm, bx, by, rhs);
+  if (skip && skip[rhs]) return;
+  const unsigned item = (unsigned)bx * 256u + threadIdx.x, JJ = item / (unsigned)gc.n1;
+  const int I = (int)(item - JJ * (unsigned)gc.n1), J = (int)JJ;
+  const int K0 = by * rows;
+  if (J >= gc.n2 || K0 >= gc.n3) return;
+  const int K1 = min(K0 + rows, gc.n3);
+  const cx<T>* r = rf + (int64_t)rhs * gf.N;
+  cx<T>* out = bc + (int64_t)rhs * gc.N;
+  const T w[3] = {(T)0.25, (T)0.5, (T)0.25};
+  int64_t cols[3][3];
+  for (int bb = 0; bb < 3; ++bb) {
+    const int f2 = 2 * J + bb - 1;
+    const int fj = f2 < 0 ? 1 : (f2 >= gf.n2 ? gf.n2 - 2 : f2);
+    for (int aa = 0; aa < 3; ++aa) {
+      const int f1 = 2 * I + aa - 1;
+      const int fi = f1 < 0 ? 1 : (f1 >= gf.n1 ? gf.n1 - 2 : f1);
+      cols[bb][aa] = (int64_t)fj * gf.n1 + fi;
+    }
+  }
+  const int64_t plane = (int64_t)gf.n1 * gf.n2;
+  auto plane_sum = [&](int f) {
+    const int fk = f < 0 ? 1 : (f >= gf.n3 ? gf.n3 - 2 : f);
+    const cx<T>* rp = r + (int64_t)fk * plane;
+    cx<T> acc = mk<T>(0, 0);
+    for (int bb = 0; bb < 3; ++bb)
+      for (int aa = 0; aa < 3; ++aa) acc = acc + (w[bb] * w[aa]) * rp[cols[bb][aa]];
+    return acc;
+  };
+  const int64_t ccol = (int64_t)J * gc.n1 + I, cplane = (int64_t)gc.n1 * gc.n2;
+  cx<T> sm = plane_sum(2 * K0 - 1);
+  for (int K = K0; K < K1; ++K) {
+    const cx<T> s0 = plane_sum(2 * K), sp = plane_sum(2 * K + 1);
+    out[(int64_t)K * cplane + ccol] = w[0] * sm + w[1] * s0 + w[2] * sp;
+    sm = sp;
+  }
+}
+
 // Same full weighting on a real nodal coefficient field (m, gamma).
 __global__ __launch_bounds__(256) void k_restrict_real(Grid gf, Grid gc, const double* __restrict__ f,
                                                        double* __restrict__ c) {
@@ -723,6 +770,13 @@ template void launch_jacobi0<float>(const Grid&, int, const cx<float>*, cx<float
 template <typename T>
 void launch_restrict(const Grid& gf, const Grid& gc, int nrhs, const cx<T>* rf, cx<T>* bc, const int* skip,
                      hipStream_t s) {
+  if (gf.dim == 3) {
+    const int rows = 4;
+    const int64_t items = (int64_t)gc.n1 * gc.n2;
+    const TileMap tm = make_tile_map((int)((items + 255) / 256), (gc.n3 + rows - 1) / rows, nrhs);
+    hipLaunchKernelGGL((k_restrict3<T>), dim3(tm.T), dim3(256), 0, s, gf, gc, rows, tm, rf, bc, skip);
+    return;
+  }
   dim3 grid((unsigned)((gc.N + 255) / 256), nrhs);
   hipLaunchKernelGGL((k_restrict<T>), grid, dim3(256), 0, s, gf, gc, rf, bc, skip);
 }
