@@ -712,8 +712,7 @@ template <typename TI, typename TO, typename TD, int MODE>
 void launch_stencil(const Grid& g, int nrhs, const cx<TI>* x, const cx<TO>* b, cx<TO>* y, const cx<TD>* d,
                     const double* m, cx<double> sfac, double wj, const int* skip, hipStream_t s, int rows_override,
                     const cx<TD>* fj) {
-  static const int env_rows = getenv("HH_ROWS") ? atoi(getenv("HH_ROWS")) : 0;
-  const int rows = env_rows > 0 ? env_rows : (rows_override > 0 ? rows_override : pick_rows(g, nrhs));
+  const int rows = rows_override > 0 ? rows_override : pick_rows(g, nrhs);
   const TileMap tm = stencil_map(g, nrhs, rows);
   if (g.dim == 2)
     hipLaunchKernelGGL((k_stencil2d<TI, TO, TD, MODE>), dim3(tm.T), dim3(256), 0, s, g, x, b, y, d, m, sfac,
