@@ -21,7 +21,8 @@ the learned+batched ms/RHS.  ``roofline`` times every launch of the dominant
 kernel inside the timed solves with HIP events; ``cpu_baseline`` solves one
 source with the oracle's C/OpenMP restatement of the plain method;
 ``three_d`` times the 3-D case (193 x 193 x 97, 32 sources) with a complex64
-and a complex128 preconditioner (``--no-3d`` skips it).
+and a complex128 preconditioner (``--no-3d`` skips it); ``stream`` solves an
+uneven stream of 200 independent problems (``--no-stream`` skips it).
 
 Usage: python bench.py [--gpus 1] [--steps 2] [--warmup 1] [--plain-rhs 4]
 """
@@ -98,6 +99,44 @@ def c3_lines(log):
         out[f"{prec}_max_iters"] = int(st["max_iters"])
         out[f"{prec}_max_true_res"] = float(max(st["true_res"]))
     return out
+
+
+def stream_line(log, count=200, workers=4):
+    """The uneven stream: ``count`` independent 2-D problems, grids drawn from
+    {129, 257, 513, 1025} x {129, 257, 513}, 6-12 points per wavelength,
+    1-64 sources each, seeded layered+body media, complex64 preconditioner,
+    complex128 FGMRES(10) to 1e-6, solved by hh_stream_solve (native worker
+    threads, one stream per problem, host buffers in and out)."""
+    import helmholtz_amd as ha
+    from helmholtz_amd import media, stream
+    rng = np.random.default_rng(5)
+    t0 = time.perf_counter()
+    probs, nodes = [], 0
+    for k in range(count):
+        n = (int(rng.choice([129, 257, 513, 1025])), int(rng.choice([129, 257, 513])))
+        v = media.layered_body_velocity_2d(n, seed=5000 + k)
+        m = (1.0 / v**2).ravel(order="F")
+        M = ha.getRegularMesh([0.0, 10.0 * (n[0] - 1), 0.0, 10.0 * (n[1] - 1)], [n[0] - 1, n[1] - 1])
+        w = media.omega_for_ppw(1500.0, 10.0, float(rng.uniform(6, 12)))
+        _, gam = ha.GetHelmholtzOperator(M, m, w, 0.01 * w * np.ones(m.size), True, [10, 10], w, True)
+        hp = ha.HelmholtzParam(M, gam.ravel(order="F"), m, w, True, True)
+        nrhs = int(rng.integers(1, 65))
+        xs = rng.integers(5, n[0] - 5, size=nrhs)
+        probs.append({"param": hp, "B": ha.point_sources(M, [(int(x), 2) for x in xs]), "prec": "c64",
+                      "max_outer": 200})
+        nodes += m.size * nrhs
+    log(f"stream: {count} problems built in {time.perf_counter() - t0:.1f} s")
+    t0 = time.perf_counter()
+    out = stream.solve_stream(probs, workers=workers)
+    dt = time.perf_counter() - t0
+    rhs = sum(int(p["B"].shape[1]) for p in probs)
+    res = max(float(np.max(info["true_res"])) for _, info in out)
+    ok = all(info["code"] == 0 for _, info in out) and res <= 1e-6
+    log(f"stream: {rhs} right-hand sides in {dt:.2f} s, max true residual {res:.2e}")
+    return {"workload": f"{count} 2-D problems, grids {{129..1025}} x {{129..513}}, 6-12 ppw, 1-64 sources each",
+            "problems": count, "rhs": rhs, "node_rhs": int(nodes), "seconds": round(dt, 3),
+            "ms_per_rhs": round(1e3 * dt / rhs, 3), "max_true_res": res, "all_converged": bool(ok),
+            "workers": workers, "buffers": "host (PCIe-inclusive)"}
 
 
 def make_solver(hp, prec: str, levels=6, cycle="W", tol=1e-6, fast=False):
@@ -219,6 +258,7 @@ def main():
                     help="seconds the host baseline may spend on a full single-source solve")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-3d", action="store_true", help="skip the 3-D complex64 vs complex128 lines")
+    ap.add_argument("--no-stream", action="store_true", help="skip the uneven-stream line")
     ap.add_argument("--train-media", type=int, default=40)
     ap.add_argument("--train-epochs", type=int, default=6)
     args = ap.parse_args()
@@ -285,6 +325,7 @@ def main():
     plain_ms_rhs = 1e3 * float(np.mean(plain_t))
 
     c3 = None if args.no_3d else c3_lines(log)
+    st5 = None if args.no_stream else stream_line(log)
     calibration_copy(Bd, log)
     cpu = None
     if not args.no_cpu:
@@ -328,6 +369,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "three_d": c3,
+        "stream": st5,
     }
     print(json.dumps(line), flush=True)
 
