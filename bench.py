@@ -206,13 +206,25 @@ def _pmc_traffic():
 
 
 def calibration_copy(Bd, log):
-    """One device-to-device copy of the RHS block (bytes known exactly): the
-    scale for the HBM byte counters of the --pmc runs."""
+    """Device-to-device copies of the RHS block (bytes known exactly): the
+    scale for the HBM byte counters of the --pmc runs (the last copy), and the
+    measured copy bandwidth (read + written bytes over the best of three
+    synchronous copies) that the roofline kernel is also compared with."""
+    from helmholtz_amd import _lib as L
     from helmholtz_amd.device import DeviceArray
     tmp = DeviceArray(Bd.shape, Bd.dtype)
-    tmp.copy_from(Bd)
-    log(f"calibration copy: {Bd.nbytes} bytes read, {Bd.nbytes} bytes written (device to device)")
+    probe = np.zeros(1, dtype=np.complex128)
+    best = float("inf")
+    for _ in range(3):
+        t0 = time.perf_counter()
+        tmp.copy_from(Bd)
+        # a device-to-host read of one value completes only after the copy
+        L.check(L.lib().hh_memcpy(probe.ctypes.data, tmp.ptr, probe.nbytes, 1))
+        best = min(best, time.perf_counter() - t0)
+    gbs = 2.0 * Bd.nbytes / best / 1e9
+    log(f"calibration copy: {Bd.nbytes} bytes read, {Bd.nbytes} bytes written (device to device), {gbs:.0f} GB/s")
     del tmp
+    return gbs
 
 
 def cpu_baseline(hp, B, iters_per_rhs: int, budget_s: float):
@@ -326,7 +338,9 @@ def main():
 
     c3 = None if args.no_3d else c3_lines(log)
     st5 = None if args.no_stream else stream_line(log)
-    calibration_copy(Bd, log)
+    copy_gbs = calibration_copy(Bd, log)
+    roof["copy_gbs"] = round(copy_gbs, 1)
+    roof["copy_frac"] = round(roof["achieved"] / copy_gbs, 4)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(hp, B, int(np.mean(plain_it)), args.cpu_budget)
