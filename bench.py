@@ -101,7 +101,7 @@ def c3_lines(log):
     return out
 
 
-def stream_line(log, count=200, workers=4):
+def stream_line(log, count=200, workers=8):
     """The uneven stream: ``count`` independent 2-D problems, grids drawn from
     {129, 257, 513, 1025} x {129, 257, 513}, 6-12 points per wavelength,
     1-64 sources each, seeded layered+body media, complex64 preconditioner,
