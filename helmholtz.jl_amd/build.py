@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libhelmholtz_amd.so")
 OBJ = os.path.join(HERE, "_build")
-SOURCES = ["hh_stencil.hip", "hh_krylov.hip", "hh_fused.hip", "hh_ho.hip", "hh_kcycle.hip", "hh_gmsmooth.hip", "hh_api.hip", "hh_unet.hip", "hh_learn.hip"]
+SOURCES = ["hh_stencil.hip", "hh_krylov.hip", "hh_fused.hip", "hh_fused3.hip", "hh_ho.hip", "hh_kcycle.hip", "hh_gmsmooth.hip", "hh_api.hip", "hh_unet.hip", "hh_learn.hip"]
 ARCH = os.environ.get("HH_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

@@ -590,6 +590,11 @@ struct Cycle {
   int ncorr_at(int l) const {
     return (S->p.cycle == 'W' && (S->p.wlevels <= 0 || l < S->p.wlevels)) ? 2 : 1;
   }
+  // fused 3-D up leg (last coarse correction + 2 post sweeps), complex64
+  bool use_up3(const Level& L, int l) const {
+    return std::is_same<T, float>::value && S->p.fused && S->p.cycle != 'K' &&
+           S->p.relax_type == HH_RELAX_JAC && l + 1 < (int)S->lv.size() && post_at(l) == 2 && fused3_ok(L.g);
+  }
   // damped-Jacobi sweeps before / after the coarse correction on level l
   int pre_at(int l) const { return S->p.per_level ? S->p.pre_l[l] : S->p.pre; }
   int post_at(int l) const { return S->p.per_level ? S->p.post_l[l] : S->p.post; }
@@ -774,13 +779,22 @@ struct Cycle {
     // current residual (iterating the coarse cycle on the restricted residual
     // instead diverges with rediscretised coarse operators near kh ~ 2).
     const int ncorr = ncorr_at(l);
+    const bool up3 = use_up3(L, l);
     for (int c = 0; c < ncorr; ++c) {
       launch_stencil<T, T, T, M_RESID>(L.g, nrhs, (const cx<T>*)L.cur, b, (cx<T>*)L.r, (const cx<T>*)L.d, nullptr,
                                        mk<double>(0, 0), 0.0, skip, st, rows_of(L));
       launch_restrict<T>(L.g, C.g, nrhs, (const cx<T>*)L.r, (cx<T>*)C.b, skip, st);
       C.cur = nullptr;
       run(l + 1, (const cx<T>*)C.b, true, nullptr);
+      if (up3 && c + 1 == ncorr) break;  // the last correction is added by the fused up leg
       launch_prolong_add<T>(L.g, C.g, nrhs, (const cx<T>*)C.cur, (cx<T>*)L.cur, skip, st);
+    }
+    if (up3) {
+      void* o = other(L);
+      launch_up_fused3(L.g, C.g, nrhs, (const cx<float>*)L.cur, (const cx<float>*)C.cur, (const cx<float>*)b,
+                       (const cx<float>*)L.d, (const cx<float>*)L.f, (cx<float>*)o, skip, st);
+      L.cur = o;
+      return;
     }
     post_smooth(L, b, post);
   }
@@ -796,7 +810,7 @@ struct Cycle {
     }
     // number of buffer writes on level 0: pre (incl. the zero-start sweep) + post,
     // or for the fused legs down + (corrections - 1) + up
-    const int flips = use_fused(L0) ? ncorr_at(0) + 1 : pre_at(0) + post_at(0);
+    const int flips = use_fused(L0) ? ncorr_at(0) + 1 : pre_at(0) + (use_up3(L0, 0) ? 1 : post_at(0));
     // choose the ping-pong pair (out, xa) so the last write lands in out
     void* saved_xb = L0.xb;
     L0.xb = out;

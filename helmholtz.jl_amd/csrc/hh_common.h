@@ -83,6 +83,14 @@ __device__ __forceinline__ void node_coords(const Grid& g, int64_t p, int& i, in
   }
 }
 
+// mirror ghost index for tiled kernels; positions farther out than one
+// mirror (only loaded by a last tile, never used for an owned output) are
+// clamped into the grid
+__device__ __forceinline__ int mirror_idx(int g, int n) {
+  g = g < 0 ? -g : (g >= n ? 2 * (n - 1) - g : g);
+  return g < 0 ? 0 : (g >= n ? n - 1 : g);
+}
+
 // Mirror/self ghost index on one axis (src/PlainNodalLaplacian.jl:18-30 rows).
 __device__ __forceinline__ int lo_nb(int i, int bc) { return i > 0 ? i - 1 : (bc == 2 ? 1 : 0); }
 __device__ __forceinline__ int hi_nb(int i, int n, int bc) {

@@ -25,13 +25,6 @@
 
 namespace hh {
 
-// mirror ghost index; positions farther out than one mirror (only loaded by
-// the last tile, never used for an owned output) are clamped into the grid
-__device__ __forceinline__ int mirror_idx(int g, int n) {
-  g = g < 0 ? -g : (g >= n ? 2 * (n - 1) - g : g);
-  return g < 0 ? 0 : (g >= n ? n - 1 : g);
-}
-
 // tile rows: 40 (complex64, 512 threads = 8 wavefronts, 5 rows each) or
 // 16 (complex128, 256 threads = 4 wavefronts, 4 rows each)
 template <typename T>

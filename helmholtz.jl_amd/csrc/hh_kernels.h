@@ -66,6 +66,11 @@ void kc_update(int64_t N, int nrhs, int mode, const cx<double>* sc, cx<T>* y, co
 
 // ---- fused 2-D multigrid legs (hh_fused.hip): BC = 2, 2 pre/2 post sweeps ----
 bool fused_ok(const Grid& gf);
+// ---- fused 3-D up leg (hh_fused3.hip): x' = x + P e_c, two Jacobi sweeps, complex64 ----
+bool fused3_ok(const Grid& gf);
+void launch_up_fused3(const Grid& gf, const Grid& gc, int nrhs, const cx<float>* x, const cx<float>* ec,
+                      const cx<float>* b, const cx<float>* d, const cx<float>* f, cx<float>* xo, const int* skip,
+                      hipStream_t s);
 template <typename T>
 void launch_down_fused(const Grid& gf, const Grid& gc, int nrhs, const cx<T>* b, const cx<T>* d, const cx<T>* f,
                        cx<T>* xo, cx<T>* bco, const int* skip, hipStream_t s);

@@ -391,6 +391,30 @@ def test_jac_gmres_kcycle_solve(prec):
         assert abs(it - ito) <= (1 if prec == "c128" else 2)
 
 
+@pytest.mark.parametrize("nodes", [(17, 17, 9), (33, 17, 17), (65, 49, 33), (129, 65, 17)])
+@pytest.mark.parametrize("cycle", ["V", "W"])
+def test_fused_up3_equals_unfused(nodes, cycle):
+    """The fused 3-D up leg (z-marching: x + P e_c and two Jacobi sweeps in
+    one kernel) gives the complex64 cycle of the separate kernels (tiles that
+    straddle the grid edge and the z boundaries included); also checked
+    against the complex128 oracle cycle at complex64 accuracy."""
+    M, m, g, w = small_case(nodes, seed=77, neumann=True)
+    rng = np.random.default_rng(78)
+    V = np.asfortranarray(rng.standard_normal((m.size, 3)) + 1j * rng.standard_normal((m.size, 3)))
+    zs = []
+    for fused in (True, False):
+        s = _solver(M, m, g, w, True, 3, 0.5, cycle=cycle, prec="c64")
+        s.MG.fused = fused
+        zs.append(ha.precondition(s, V).astype(np.complex128))
+    err = np.linalg.norm(zs[0] - zs[1]) / np.linalg.norm(zs[1])
+    print(f"fused up3 vs unfused {nodes} {cycle}: {err:.2e}")
+    assert err <= 2e-6
+    p = hr.HelmholtzParam(M, g, m, w, True, True)
+    hier = sr.build_hierarchy(p, 3, 0.5)
+    zo = sr.vcycle(hier, 0, V[:, 0], 0.8, 2, 2, cycle)
+    assert np.linalg.norm(zs[0][:, 0] - zo) / np.linalg.norm(zo) <= 1e-5
+
+
 @pytest.mark.parametrize("nodes", [(257, 129), (129, 65), (97, 33), (1025, 513)])
 @pytest.mark.parametrize("prec", ["c64", "c128"])
 def test_fused_legs_equal_unfused(nodes, prec):
