@@ -68,3 +68,39 @@ def test_getmgparam_both_signatures():
     b = ha.getMGparam(5, 4, 50, 1e-5, "Jac", 0.8, 2, 2, "K", "NoMUMPS", 0.5, 0.0, "FullWeighting")
     assert (a.levels, a.numCores, a.maxOuterIter, a.cycleType) == (2, 2, 30, "W")
     assert (b.levels, b.relaxParam, b.extra) == (5, 0.8, (0.5, 0.0, "FullWeighting"))
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """The ctypes mirrors of the header's structs (hh_mg_params,
+    hh_solve_stats, hh_problem, hh_problem_result) have the C compiler's
+    size and field offsets: compiled from include/helmholtz_amd.h with gcc."""
+    import ctypes as C
+    import shutil
+    import subprocess
+    from helmholtz_amd import _lib as L
+    from helmholtz_amd import stream
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    structs = {"hh_mg_params": L.MGParams, "hh_solve_stats": L.SolveStats, "hh_problem": stream.Problem,
+               "hh_problem_result": stream.ProblemResult}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "helmholtz_amd.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'  printf("{cname} sizeof %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'  printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", inc, str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {}
+    for line in out:
+        if line:
+            s, f, v = line.split()
+            got[(s, f)] = int(v)
+    for cname, py in structs.items():
+        assert got[(cname, "sizeof")] == C.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
