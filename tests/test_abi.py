@@ -104,3 +104,39 @@ def test_struct_layouts_match_the_header(tmp_path):
         assert got[(cname, "sizeof")] == C.sizeof(py), cname
         for fname, _ in py._fields_:
             assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
+
+
+def _c_fields(header: str, name: str):
+    """(field, julia type) of a C struct in the header, in order."""
+    body = re.search(r"typedef struct " + name + r" \{(.*?)\} " + name + ";", header, re.S).group(1)
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    cmap = {"int32_t": "Int32", "int64_t": "Int64", "double": "Float64"}
+    out = []
+    for decl in body.split(";"):
+        decl = decl.strip()
+        if not decl:
+            continue
+        m = re.match(r"(int32_t|int64_t|double)\s*(\*?)\s*(\w+)(\[(\d+)\])?$", decl)
+        assert m, decl
+        t = cmap[m.group(1)]
+        if m.group(2):
+            t = f"Ptr{{{t}}}"
+        if m.group(5):
+            t = f"NTuple{{{m.group(5)},{t}}}"
+        out.append((m.group(3), t))
+    return out
+
+
+def _julia_fields(src: str, name: str):
+    body = re.search(r"struct " + name + r"\n(.*?)\nend", src, re.S).group(1)
+    return [(f.strip(), t.strip()) for f, t in re.findall(r"(\w+)::([\w{},]+)", body)]
+
+
+def test_julia_shim_structs_match_the_header():
+    """The Julia ccall shim's HHMGParams / HHSolveStats have the header's
+    fields, in order, with the matching Julia types."""
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    header = open(os.path.join(root, "include", "helmholtz_amd.h")).read()
+    jl = open(os.path.join(root, "helmholtz.jl_amd", "julia", "HelmholtzAMD.jl")).read()
+    assert _julia_fields(jl, "HHMGParams") == _c_fields(header, "hh_mg_params")
+    assert _julia_fields(jl, "HHSolveStats") == _c_fields(header, "hh_solve_stats")
