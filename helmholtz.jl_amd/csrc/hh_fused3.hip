@@ -8,7 +8,7 @@
 // row per wavefront as in the 2-D legs) and marches it along z.  Step z forms
 // x' on plane z, the first sweep on plane z - 1 and the second on plane z - 2;
 // x' and the first sweep live in LDS rings of three planes, and b, d, F of the
-// three planes in flight in registers.  Each block covers a run of 16 output
+// three planes in flight in registers; one barrier per step.  Each block covers a run of 16 output
 // planes.  Ghost planes mirror (plane -1 = 1,
 // n3 = n3 - 2) like the in-plane halos, so every formula is the unfused one:
 // the result equals prolongation + two sweeps.
@@ -84,16 +84,33 @@ __global__ __launch_bounds__(512) void k_up_fused3(TileMap tm, int ytiles, int z
   // in-plane neighbours from LDS)
   cx<T> b0[KR], d0[KR], f0[KR], b1[KR], d1[KR], f1[KR], b2[KR], d2[KR], f2[KR];
   cx<T> xa0[KR], xa1[KR], xa2[KR], sa0[KR], sa1[KR], sa2[KR];
+  // x, b, d, F of the next plane, loaded a step before they are used
+  cx<T> xn[KR], bn[KR], dn[KR], fn[KR];
+  auto load_plane = [&](int zz) {
+    const int64_t pz = (int64_t)zz * plane;
+#pragma unroll
+    for (int k = 0; k < KR; ++k) {
+      const int64_t o = pz + (int64_t)gj[k] * n1 + gi;
+      xn[k] = xr[o];
+      bn[k] = br[o];
+      dn[k] = d[o];
+      fn[k] = f[o];
+    }
+  };
+  load_plane(za);
   for (int z = za; z <= z1 + 1; ++z) {
-    if (z < zb) {  // x' on plane z
-      const int64_t pz = (int64_t)z * plane;
+    if (z < zb) {  // x' on plane z (its x, b, d, F were loaded one step ahead)
+      cx<T> xv[KR];
 #pragma unroll
       for (int k = 0; k < KR; ++k) {
-        const int64_t o = pz + (int64_t)gj[k] * n1 + gi;
-        const cx<T> xv = xr[o];
-        b0[k] = br[o];
-        d0[k] = d[o];
-        f0[k] = f[o];
+        xv[k] = xn[k];
+        b0[k] = bn[k];
+        d0[k] = dn[k];
+        f0[k] = fn[k];
+      }
+      if (z + 1 < zb) load_plane(z + 1);
+#pragma unroll
+      for (int k = 0; k < KR; ++k) {
         cx<T> e;
         if (z & 1) {
           const cx<T> cN = cval((z >> 1) + 1, k);
@@ -102,11 +119,10 @@ __global__ __launch_bounds__(512) void k_up_fused3(TileMap tm, int ytiles, int z
         } else {
           e = cK[k];
         }
-        xa0[k] = xv + e;
+        xa0[k] = xv[k] + e;
         ra[z % 3][lj[k] * 64 + lane] = xa0[k];
       }
     }
-    __syncthreads();
     if (z - 1 >= max(0, z0 - 1) && z - 1 < min(n3, z1 + 1)) {  // first sweep on plane p = z - 1
       const int p = z - 1;
       const cx<T>* a0 = ra[p % 3];
@@ -124,7 +140,6 @@ __global__ __launch_bounds__(512) void k_up_fused3(TileMap tm, int ytiles, int z
         }
       }
     }
-    __syncthreads();
     if (z - 2 >= z0 && z - 2 < z1) {  // second sweep on plane r = z - 2, owned nodes stored
       const int r = z - 2;
       const cx<T>* s0 = rs[r % 3];
@@ -155,6 +170,10 @@ __global__ __launch_bounds__(512) void k_up_fused3(TileMap tm, int ytiles, int z
       sa2[k] = sa1[k];
       sa1[k] = sa0[k];
     }
+    // each stage reads in LDS only planes written in earlier steps (the
+    // z-neighbours it needs from this step are the thread's own registers),
+    // so one barrier per step separates every write from its readers
+    __syncthreads();
   }
 }
 
