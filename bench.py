@@ -7,7 +7,9 @@ seeded layered medium (1500-4500 m/s, 12 interfaces, elliptic 4500 m/s body,
 sides, 256 point sources on depth row 2 (x = 3 + 4k).  Preconditioner: 6-level
 shifted-Laplacian cycle, W(2,2) on the three finest levels and V(2,2) below,
 damped Jacobi 0.8, shift 0.4 on the finest level and 0.5 below (``SHIFTS``);
-outer FGMRES(10) in complex128 to ||b - H x|| <= 1e-6 ||b||.
+outer FGMRES(10) to ||b - H x|| <= 1e-6 ||b|| (iterate and restart residual
+in complex128; the complex64-cycle solves store the Krylov basis in complex64,
+``--basis``, with every product and sum in float64).
 
 A step = one solve of all 256 sources in one batched call (complex64
 preconditioner, inputs resident in HBM, stencil launch shapes autotuned) with
@@ -81,10 +83,11 @@ def c3_problem():
     return hp, ha.point_sources(M, subs)
 
 
-def c3_lines(log):
+def c3_lines(log, basis="c128"):
     """Complex64 vs complex128 preconditioner on the 3-D case: 5-level W(2,2)
     cycle (the coarsest level must have at most 4096 nodes for the dense
-    solve, so 4 levels do not fit), FGMRES(8) in complex128 to 1e-6."""
+    solve, so 4 levels do not fit), FGMRES(8) to 1e-6; the complex64 line
+    stores the basis as ``basis`` says, the complex128 line in complex128."""
     import helmholtz_amd as ha
     from helmholtz_amd.device import DeviceArray
     hp, B = c3_problem()
@@ -93,15 +96,17 @@ def c3_lines(log):
     for prec in ("c64", "c128"):
         mg = ha.getMGparam(complex, int, 5, 1, 100, 1e-6, "Jac", 0.8, 2, 2, "W", "NoMUMPS", 0.5, 0.0)
         mg.prec = prec
+        mg.basis = basis if prec == "c64" else "c128"
         s = ha.getShiftedLaplacianMultigridSolver(hp, mg, 0.5, "GMRES", 8)
         t, st = timed_solves(ha, s, Bd, 1, 1, log, f"3-D {prec}")
         out[f"{prec}_ms_per_rhs"] = round(1e3 * t / B.shape[1], 3)
         out[f"{prec}_max_iters"] = int(st["max_iters"])
         out[f"{prec}_max_true_res"] = float(max(st["true_res"]))
+    out["c64_basis"] = basis
     return out
 
 
-def stream_line(log, count=200, workers=8):
+def stream_line(log, count=200, workers=8, basis="c128"):
     """The uneven stream: ``count`` independent 2-D problems, grids drawn from
     {129, 257, 513, 1025} x {129, 257, 513}, 6-12 points per wavelength,
     1-64 sources each, seeded layered+body media, complex64 preconditioner,
@@ -123,7 +128,7 @@ def stream_line(log, count=200, workers=8):
         nrhs = int(rng.integers(1, 65))
         xs = rng.integers(5, n[0] - 5, size=nrhs)
         probs.append({"param": hp, "B": ha.point_sources(M, [(int(x), 2) for x in xs]), "prec": "c64",
-                      "max_outer": 200})
+                      "max_outer": 200, "basis": basis})
         nodes += m.size * nrhs
     log(f"stream: {count} problems built in {time.perf_counter() - t0:.1f} s")
     t0 = time.perf_counter()
@@ -136,13 +141,14 @@ def stream_line(log, count=200, workers=8):
     return {"workload": f"{count} 2-D problems, grids {{129..1025}} x {{129..513}}, 6-12 ppw, 1-64 sources each",
             "problems": count, "rhs": rhs, "node_rhs": int(nodes), "seconds": round(dt, 3),
             "ms_per_rhs": round(1e3 * dt / rhs, 3), "max_true_res": res, "all_converged": bool(ok),
-            "workers": workers, "buffers": "host (PCIe-inclusive)"}
+            "workers": workers, "basis": basis, "buffers": "host (PCIe-inclusive)"}
 
 
-def make_solver(hp, prec: str, levels=6, cycle="W", tol=1e-6, fast=False):
+def make_solver(hp, prec: str, levels=6, cycle="W", tol=1e-6, fast=False, basis="c128"):
     import helmholtz_amd as ha
     mg = ha.getMGparam(complex, int, levels, 1, 200, tol, "Jac", 0.8, 2, 2, cycle, "NoMUMPS", 0.5, 0.0)
     mg.prec = prec
+    mg.basis = basis
     # eager launches: at 256 RHS each launch runs for 0.1-7 ms, so graph replay
     # measured no faster, and the in-solve kernel timing needs eager launches
     mg.graph = False
@@ -273,6 +279,8 @@ def main():
     ap.add_argument("--no-stream", action="store_true", help="skip the uneven-stream line")
     ap.add_argument("--train-media", type=int, default=40)
     ap.add_argument("--train-epochs", type=int, default=6)
+    ap.add_argument("--basis", choices=["c64", "c128"], default="c64",
+                    help="FGMRES basis storage of the complex64-cycle solves (the plain leg stays complex128)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the solve shards by right-hand side "
@@ -290,8 +298,8 @@ def main():
     log(f"problem built in {time.perf_counter() - t:.1f} s: N={H.shape[0]} nsrc={args.nsrc} omega={w:.3f}")
     Bd = DeviceArray.from_numpy(B)
 
-    # ---- batched, complex64 preconditioner, plain W cycle (graphs + autotuned launches)
-    sb = make_solver(hp, "c64", fast=True)
+    # ---- batched, complex64 preconditioner, plain W cycle (autotuned launches)
+    sb = make_solver(hp, "c64", fast=True, basis=args.basis)
     t_b, st_b = timed_solves(ha, sb, Bd, args.warmup, args.steps, log, "batched")
     res_b = np.asarray(st_b["true_res"])
     batched_ms_rhs = 1e3 * t_b / args.nsrc
@@ -306,7 +314,7 @@ def main():
     net, tinfo = train.train_coarse_correction(n=(513, 513), media_count=args.train_media, level=2,
                                                epochs=args.train_epochs, log=log)
     log(f"training: {tinfo}")
-    sl = make_solver(hp, "c64", fast=True)
+    sl = make_solver(hp, "c64", fast=True, basis=args.basis)
     learned.attach(sl, net, 2)
     t_l, st_l = timed_solves(ha, sl, Bd, args.warmup, args.steps, log, "learned",
                              before_timed=lambda s: ha.time_kernel(s, *ROOFLINE_KERNEL))
@@ -336,8 +344,8 @@ def main():
         log(f"plain rhs {c}: {dt:.3f} s, {plain_it[-1]} iters, res {plain_res[-1]:.2e}")
     plain_ms_rhs = 1e3 * float(np.mean(plain_t))
 
-    c3 = None if args.no_3d else c3_lines(log)
-    st5 = None if args.no_stream else stream_line(log)
+    c3 = None if args.no_3d else c3_lines(log, args.basis)
+    st5 = None if args.no_stream else stream_line(log, basis=args.basis)
     copy_gbs = calibration_copy(Bd, log)
     roof["copy_gbs"] = round(copy_gbs, 1)
     roof["copy_frac"] = round(roof["achieved"] / copy_gbs, 4)
@@ -357,12 +365,12 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c64 preconditioner / c128 outer",
+        "dtype": f"c64 preconditioner, {args.basis} Krylov basis, f64 products / c128 iterate and residual",
         "data": "synthetic (seeded layered+body medium, point sources)",
         "config": {"workload": "2-D 1025x513 Helmholtz, 256 point sources, 8 ppw, 1e-6 true residual",
                    "nsrc": args.nsrc, "levels": 6, "cycle": "W(2,2) on levels 0-2, V(2,2) below", "krylov": "FGMRES(10)",
                    "learned": "U-Net at level 2 (257x129) vs plain cycle, chosen by a two-restart trial on the first solve", "graphs": False,
-                   "autotune": True, "parallelism": "rhs-batched, 1 GPU"},
+                   "autotune": True, "basis": args.basis, "parallelism": "rhs-batched, 1 GPU"},
         "learned_ms_per_rhs": round(learned_ms_rhs, 4),
         "learned_max_iters": int(st_l["max_iters"]),
         "learned_used": bool(st_l["learned_used"]),

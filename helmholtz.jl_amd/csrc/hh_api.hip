@@ -465,7 +465,9 @@ void ensure_work(hh_solver* S, int nrhs) {
   const size_t blk = (size_t)N * nrhs;
   // BiCGSTAB keeps r, r^, p, v, s, t in the basis blocks and p^, s^ in Z
   const bool bicg = S->p.krylov == HH_BICGSTAB;
-  S->V = dalloc<cx<double>>(blk * std::max(K + 1, bicg ? 6 : 0));
+  // (a complex64 basis takes half the complex128 blocks)
+  const bool vb32 = S->p.basis == HH_C64 && pr == HH_C64 && !S->op->ho;
+  S->V = dalloc<cx<double>>(vb32 ? (blk * (K + 1) + 1) / 2 : blk * std::max(K + 1, bicg ? 6 : 0));
   S->Z = dalloc<char>(blk * std::max(K, bicg ? 2 : 0) * es);
   S->xk = dalloc<cx<double>>(blk);
   S->bk = dalloc<cx<double>>(blk);
@@ -836,7 +838,7 @@ void precondition(hh_solver* S, int nrhs, const cx<double>* vj, void* zj, const 
     } else if (scale) {
       KrylovWork w = S->kw;
       w.nrhs = nrhs;
-      kr_cast_scaled(w, vj, scale, (cx<float>*)S->b0, st);
+      kr_cast_scaled<double>(w, vj, scale, (cx<float>*)S->b0, st);
     } else {
       launch_cast<double, float>(vj, (cx<float>*)S->b0, blk, st);
     }
@@ -877,6 +879,8 @@ void run_fgmres(hh_solver* S, int nrhs, hh_solve_stats* stats) {
   const int K = S->p.inner;
   const double tol = S->p.tol;
   const size_t zes = csz(S->p.prec);
+  // complex64 basis (validated at create: FGMRES, complex64 cycle, 5/7-point)
+  const bool vb32 = S->p.basis == HH_C64 && w.lazy && !S->op->ho;
   std::vector<int> iters(nrhs, 0), conv(nrhs, 0);
   // ||b|| per RHS
   kr_norm2(w, S->bk, S->n2tmp, nullptr, st);
@@ -901,9 +905,13 @@ void run_fgmres(hh_solver* S, int nrhs, hh_solve_stats* stats) {
     // plain complex64 cycle: the Arnoldi pass stages the next input and Z
     // holds unscaled directions (the learned correction is not linear, so it
     // gets normalised inputs)
+    // (a complex64 basis is that input itself: no staged copy)
     w.zunscaled = w.lazy && !S->use_learned;
-    w.stage = w.zunscaled ? (cx<float>*)S->b0 : nullptr;
-    kr_start_impl(w, S->wk, S->V, tol, S->conv_d, S->n2tmp, st);
+    w.stage = w.zunscaled && !vb32 ? (cx<float>*)S->b0 : nullptr;
+    if (vb32)
+      kr_start_impl<float>(w, S->wk, (cx<float>*)S->V, tol, S->conv_d, S->n2tmp, st);
+    else
+      kr_start_impl<double>(w, S->wk, S->V, tol, S->conv_d, S->n2tmp, st);
     HIPCK(hipMemcpyAsync(S->h_done, w.done, sizeof(int) * nrhs, hipMemcpyDeviceToHost, st));
     HIPCK(hipStreamSynchronize(st));
     int active = 0;
@@ -913,7 +921,8 @@ void run_fgmres(hh_solver* S, int nrhs, hh_solve_stats* stats) {
       break;
     }
     for (int j = 0; j < K; ++j) {
-      const cx<double>* vj = S->V + (int64_t)j * blk;
+      const cx<double>* vj = vb32 ? nullptr : S->V + (int64_t)j * blk;
+      const cx<float>* vj32 = (const cx<float>*)S->V + (int64_t)j * blk;
       void* zj = (char*)S->Z + (size_t)j * blk * zes;
       S->cap_now = S->cap_buf && nprec >= S->cap_lo && nprec < S->cap_hi && active == nrhs;
       S->kt_active = active;
@@ -923,11 +932,25 @@ void run_fgmres(hh_solver* S, int nrhs, hh_solve_stats* stats) {
         const double Nf = (double)N;
         const double* vsj = w.lazy ? w.vscale + (int64_t)j * nrhs : nullptr;
         const bool staged = w.stage && j > 0;
-        if (S->p.prec == HH_C64 && w.lazy && !S->op->ho && !ta) {
+        if (S->p.prec == HH_C64 && w.lazy && !S->op->ho && (!ta || vb32)) {
           // H z formed inside the Arnoldi passes (w is never stored)
-          precondition<float>(S, nrhs, vj, zj, w.done, st, vsj, staged);
           const HzArgs hz{S->op->g, (const cx<float>*)zj, S->op->d64};
-          kr_arnoldi_impl(w, S->V, S->wk, j, tol, S->skip_copy, &S->kt, active, st, &hz);
+          if (vb32) {
+            // the stored complex64 V_j is the cycle's input (scaled first
+            // for the learned correction, which is not linear)
+            const cx<float>* in = vj32;
+            if (!w.zunscaled) {
+              kr_cast_scaled<float>(w, vj32, vsj, (cx<float>*)S->b0, st);
+              in = (const cx<float>*)S->b0;
+            }
+            Cycle<float> c{S, nrhs, w.done, st};
+            c.apply(in, (cx<float>*)zj);
+            kr_arnoldi_impl<float>(w, (const cx<float>*)S->V, S->wk, j, tol, S->skip_copy, &S->kt, active, st,
+                                   &hz);
+          } else {
+            precondition<float>(S, nrhs, vj, zj, w.done, st, vsj, staged);
+            kr_arnoldi_impl<double>(w, S->V, S->wk, j, tol, S->skip_copy, &S->kt, active, st, &hz);
+          }
           return;
         }
         if (S->p.prec == HH_C64) {
@@ -942,7 +965,7 @@ void run_fgmres(hh_solver* S, int nrhs, hh_solve_stats* stats) {
           h_apply<double>(S, nrhs, (const cx<double>*)zj, S->wk, w.done, st);
           if (ta) S->kt.end(st, active * Nf * 32.0 + Nf * 16.0);
         }
-        kr_arnoldi_impl(w, S->V, S->wk, j, tol, S->skip_copy, &S->kt, active, st);
+        kr_arnoldi_impl<double>(w, S->V, S->wk, j, tol, S->skip_copy, &S->kt, active, st);
       };
       // hipGraph: the first restart cycle runs eagerly (it allocates the
       // network workspace and may capture training data); later cycles replay
@@ -1338,6 +1361,9 @@ int hh_solver_create(hh_op* op, const hh_mg_params* p, const double* shift_per_l
     if (p->cycle != 'V' && p->cycle != 'W' && p->cycle != 'K') throw HHError(HH_ERR_ARG, "cycle must be 'V', 'W' or 'K'");
     if (p->krylov != HH_FGMRES && p->krylov != HH_BICGSTAB) throw HHError(HH_ERR_ARG, "unknown krylov method");
     if (p->prec != HH_C128 && p->prec != HH_C64) throw HHError(HH_ERR_ARG, "prec must be HH_C128 or HH_C64");
+    if (p->basis != HH_C128 && p->basis != HH_C64) throw HHError(HH_ERR_ARG, "basis must be HH_C128 or HH_C64");
+    if (p->basis == HH_C64 && (p->krylov != HH_FGMRES || p->prec != HH_C64 || op->ho))
+      throw HHError(HH_ERR_ARG, "basis = HH_C64 needs krylov = HH_FGMRES, prec = HH_C64 and the 5/7-point operator");
     if (p->learned != 0) throw HHError(HH_ERR_UNSUPPORTED, "learned correction not available in this entry point");
     if (p->max_outer < 1 || !(p->tol > 0.0)) throw HHError(HH_ERR_ARG, "max_outer >= 1 and tol > 0 required");
     hh_solver* s = new hh_solver();

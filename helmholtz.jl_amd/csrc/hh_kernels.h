@@ -164,8 +164,10 @@ struct KrylovWork {
   int zunscaled;
   cx<float>* stage;      // [nrhs][N] or null
 };
-// b0 = (complex64) vscale_j[b] * V_j  (preconditioner input in the lazy mode)
-void kr_cast_scaled(const KrylovWork& w, const cx<double>* Vj, const double* scale, cx<float>* b0, hipStream_t s);
+// b0 = (complex64) vscale_j[b] * V_j  (preconditioner input in the lazy mode;
+// TV: basis element type)
+template <typename TV>
+void kr_cast_scaled(const KrylovWork& w, const cx<TV>* Vj, const double* scale, cx<float>* b0, hipStream_t s);
 // w *= vscale_j[b] per RHS (H Z_j -> H z_j when Z holds unscaled directions)
 void kr_scale_rhs(const KrylovWork& w, cx<double>* wv, const double* scale, hipStream_t s);
 
@@ -173,7 +175,8 @@ void kr_scale_rhs(const KrylovWork& w, cx<double>* wv, const double* scale, hipS
 void kr_norm2(const KrylovWork& w, const cx<double>* v, double* out, const int* skip, hipStream_t s);
 // Cycle start: beta_b = ||r_b||, V0 = r / beta, g = (beta, 0, ...), done/kact
 // flags (conv[b] != 0 marks an RHS already converged).  n2tmp: nrhs doubles.
-void kr_start_impl(const KrylovWork& w, const cx<double>* r, cx<double>* V0, double tol, const int* conv,
+template <typename TV>
+void kr_start_impl(const KrylovWork& w, const cx<double>* r, cx<TV>* V0, double tol, const int* conv,
                    double* n2tmp, hipStream_t s);
 // w = H z formed inside the lazy Arnoldi passes: the outer 5/7-point
 // operator's grid and complex128 diagonal, and the complex64 z of this step
@@ -185,8 +188,9 @@ struct HzArgs {
 // One CGS2 Arnoldi step j: orthogonalise wv against V_0..V_j, write V_{j+1},
 // fill column j of H, update Givens rotations and residual estimates.
 // skip_copy: nrhs ints of scratch.  hz (lazy mode only, may be null): take
-// w = H z from hz instead of wv.
-void kr_arnoldi_impl(const KrylovWork& w, const cx<double>* V, cx<double>* wv, int j, double tol, int* skip_copy,
+// w = H z from hz instead of wv.  TV = float (complex64 basis) requires hz.
+template <typename TV>
+void kr_arnoldi_impl(const KrylovWork& w, const cx<TV>* V, cx<double>* wv, int j, double tol, int* skip_copy,
                      KTimer* kt, int active, hipStream_t s, const HzArgs* hz = nullptr);
 // y = R^-1 g per RHS, x += Z y (Z in TZ)
 template <typename TZ>

@@ -14,6 +14,13 @@
 // projection cancelled severely, ||w'|| < 1e-4 ||w|| (a Daniel-Gragg-Kaufman-
 // Stewart test with a small threshold); otherwise w' is kept unnormalised and
 // its scale 1/||w'|| is applied where the basis vector is read.
+// The lazy mode can keep the basis in complex64 (TV = float): every product
+// and sum is still formed in float64, only the stored basis values are
+// rounded (about 6e-8 relative), which perturbs the Arnoldi relation and the
+// orthogonality at that level while the outer iterate, the residual that
+// each restart recomputes and the Hessenberg/Givens work stay complex128 —
+// the compressed-basis GMRES of Aliaga, Anzt et al. (2022).  The stored
+// complex64 w' is then the preconditioner's next input as it is.
 // Partial sums are per (chunk, RHS) and reduced in a fixed order, so results
 // are bit-reproducible run to run.  The small Hessenberg/Givens work runs on
 // the device (one wavefront per RHS) so a whole iteration needs no host sync.
@@ -47,9 +54,9 @@ static __device__ __forceinline__ void chunk_range(int64_t N, int nchunk, int c,
 // pass 1: part[c][i][b] = sum_{p in chunk c} conj(V_i[p,b]) w[p,b], i <= j
 // NORM: also partn[c][b] = sum ||w||^2 over the chunk.  only (may be null):
 // skip RHS with only[b] == 0.
-template <int NV, bool NORM>
-__global__ __launch_bounds__(256) void k_dots(KrylovWork kw, const cx<double>* __restrict__ V,
-                                              const cx<double>* __restrict__ w, int j,
+template <int NV, bool NORM, typename TV>
+__global__ __launch_bounds__(256) void k_dots(KrylovWork kw, const cx<TV>* __restrict__ V,
+                                              const cx<TV>* __restrict__ w, int j,
                                               const int* __restrict__ only) {
   const int c = blockIdx.x, b = blockIdx.y;
   if (kw.done[b] || (only && !only[b])) return;
@@ -63,11 +70,11 @@ __global__ __launch_bounds__(256) void k_dots(KrylovWork kw, const cx<double>* _
   for (int i = 0; i < NV; ++i) acc[i] = mk<double>(0.0, 0.0);
   double nn = 0.0;
   for (int64_t p = p0 + threadIdx.x; p < p1; p += 256) {
-    const cx<double> wv = w[base + p];
+    const cx<double> wv = cvt<TV, double>(w[base + p]);
     if (NORM) nn += cabs2(wv);
 #pragma unroll
     for (int i = 0; i < NV; ++i)
-      if (i <= j) acc[i] = acc[i] + cmulc(V[i * stride + base + p], wv);
+      if (i <= j) acc[i] = acc[i] + cmulc(cvt<TV, double>(V[i * stride + base + p]), wv);
   }
 #pragma unroll
   for (int i = 0; i < NV; ++i) {
@@ -242,8 +249,8 @@ struct NodeWalk {
 };
 
 // pass 1: part = V^H (H z), partn = ||H z||^2
-template <int NV>
-__global__ __launch_bounds__(256) void k_dots_hz(KrylovWork kw, Grid g, const cx<double>* __restrict__ V,
+template <int NV, typename TV>
+__global__ __launch_bounds__(256) void k_dots_hz(KrylovWork kw, Grid g, const cx<TV>* __restrict__ V,
                                                  const cx<float>* __restrict__ Z, const cx<double>* __restrict__ d,
                                                  int j) {
   const int b = blockIdx.x, c = blockIdx.y;
@@ -266,7 +273,7 @@ __global__ __launch_bounds__(256) void k_dots_hz(KrylovWork kw, Grid g, const cx
     nn += cabs2(wv);
 #pragma unroll
     for (int i = 0; i < NV; ++i)
-      if (i <= j) acc[i] = acc[i] + cmulc(V[i * stride + base + p], wv);
+      if (i <= j) acc[i] = acc[i] + cmulc(cvt<TV, double>(V[i * stride + base + p]), wv);
     nw.step(g);
   }
 #pragma unroll
@@ -282,12 +289,13 @@ __global__ __launch_bounds__(256) void k_dots_hz(KrylovWork kw, Grid g, const cx
   if (threadIdx.x == 0) kw.partn[(int64_t)c * kw.nrhs + b] = t;
 }
 
-// pass 2: w' = H z - V h1 (stored), partn = ||w'||^2
-template <int NV>
-__global__ __launch_bounds__(256) void k_update_lazy_hz(KrylovWork kw, Grid g, const cx<double>* __restrict__ V,
+// pass 2: w' = H z - V h1 (stored in the basis precision), partn = ||w'||^2
+// of the unrounded w'
+template <int NV, typename TV>
+__global__ __launch_bounds__(256) void k_update_lazy_hz(KrylovWork kw, Grid g, const cx<TV>* __restrict__ V,
                                                         const cx<float>* __restrict__ Z,
                                                         const cx<double>* __restrict__ d,
-                                                        cx<double>* __restrict__ wout, int j) {
+                                                        cx<TV>* __restrict__ wout, int j) {
   const int b = blockIdx.x, c = blockIdx.y;
   if (kw.done[b]) return;
   __shared__ double sh[4];
@@ -307,8 +315,8 @@ __global__ __launch_bounds__(256) void k_update_lazy_hz(KrylovWork kw, Grid g, c
     cx<double> wv = zs * hz_at(g, zb, d, nw.ci, nw.cj, nw.ck, p);
 #pragma unroll
     for (int i = 0; i < NV; ++i)
-      if (i <= j) wv = wv - hs[i] * V[i * stride + base + p];
-    wout[base + p] = wv;
+      if (i <= j) wv = wv - hs[i] * cvt<TV, double>(V[i * stride + base + p]);
+    wout[base + p] = cvt<double, TV>(wv);
     if (kw.stage) kw.stage[base + p] = mk<float>((float)wv.re, (float)wv.im);
     nn += cabs2(wv);
     nw.step(g);
@@ -411,9 +419,9 @@ __global__ __launch_bounds__(64) void k_hess_givens(KrylovWork kw, int j, double
 }
 
 // pass 3: v_{j+1} = (w' - V h2) * nrm
-template <int NV>
-__global__ __launch_bounds__(256) void k_final(KrylovWork kw, const cx<double>* __restrict__ V,
-                                               const cx<double>* w, cx<double>* vout,
+template <int NV, typename TV>
+__global__ __launch_bounds__(256) void k_final(KrylovWork kw, const cx<TV>* __restrict__ V,
+                                               const cx<TV>* w, cx<TV>* vout,
                                                int j, const int* __restrict__ skip_before) {
   const int c = blockIdx.x, b = blockIdx.y;
   if (skip_before[b] || (kw.lazy && !kw.fix[b])) return;
@@ -429,11 +437,11 @@ __global__ __launch_bounds__(256) void k_final(KrylovWork kw, const cx<double>* 
   }
   const double s = kw.nrm[b];
   for (int64_t p = p0 + threadIdx.x; p < p1; p += 256) {
-    cx<double> wv = w[base + p];
+    cx<double> wv = cvt<TV, double>(w[base + p]);
 #pragma unroll
     for (int i = 0; i < NV; ++i)
-      if (i <= j) wv = wv - hc[i] * V[i * stride + base + p];
-    vout[base + p] = s * wv;
+      if (i <= j) wv = wv - hc[i] * cvt<TV, double>(V[i * stride + base + p]);
+    vout[base + p] = cvt<double, TV>(s * wv);
   }
 }
 
@@ -479,15 +487,16 @@ __global__ __launch_bounds__(64) void k_start_flags(KrylovWork kw, const double*
   if (kw.lazy) kw.vscale[b] = 1.0;
 }
 
+template <typename TV>
 __global__ __launch_bounds__(256) void k_scale(KrylovWork kw, const cx<double>* __restrict__ r,
-                                               cx<double>* __restrict__ v) {
+                                               cx<TV>* __restrict__ v) {
   const int c = blockIdx.x, b = blockIdx.y;
   if (kw.done[b]) return;
   int64_t p0, p1;
   chunk_range(kw.N, kw.nchunk, c, &p0, &p1);
   const int64_t base = (int64_t)b * kw.N;
   const double s = kw.nrm[b];
-  for (int64_t p = p0 + threadIdx.x; p < p1; p += 256) v[base + p] = s * r[base + p];
+  for (int64_t p = p0 + threadIdx.x; p < p1; p += 256) v[base + p] = cvt<double, TV>(s * r[base + p]);
 }
 
 // y = R^-1 g (back substitution), one thread per RHS
@@ -535,12 +544,13 @@ __global__ __launch_bounds__(64) void k_div_bnorm(KrylovWork kw, const double* _
   res[b] = bn > 0.0 ? sqrt(n2[b]) / bn : 0.0;
 }
 
-__global__ __launch_bounds__(256) void k_cast_scaled(const cx<double>* __restrict__ v, const double* __restrict__ sc,
+template <typename TV>
+__global__ __launch_bounds__(256) void k_cast_scaled(const cx<TV>* __restrict__ v, const double* __restrict__ sc,
                                                      cx<float>* __restrict__ o, int64_t N, int nrhs) {
   const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (k >= N * nrhs) return;
   const double s = sc[k / N];
-  const cx<double> a = v[k];
+  const cx<double> a = cvt<TV, double>(v[k]);
   o[k] = mk<float>((float)(s * a.re), (float)(s * a.im));
 }
 
@@ -573,10 +583,14 @@ __global__ __launch_bounds__(256) void k_scale_rhs(KrylovWork kw, cx<double>* __
 // ------------------------------------------------------------------ host side
 static inline dim3 cgrid(const KrylovWork& w) { return dim3(w.nchunk, w.nrhs); }
 
-void kr_cast_scaled(const KrylovWork& w, const cx<double>* Vj, const double* scale, cx<float>* b0, hipStream_t s) {
+template <typename TV>
+void kr_cast_scaled(const KrylovWork& w, const cx<TV>* Vj, const double* scale, cx<float>* b0, hipStream_t s) {
   const int64_t n = w.N * (int64_t)w.nrhs;
-  hipLaunchKernelGGL(k_cast_scaled, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, Vj, scale, b0, w.N, w.nrhs);
+  hipLaunchKernelGGL((k_cast_scaled<TV>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, Vj, scale, b0, w.N,
+                     w.nrhs);
 }
+template void kr_cast_scaled<double>(const KrylovWork&, const cx<double>*, const double*, cx<float>*, hipStream_t);
+template void kr_cast_scaled<float>(const KrylovWork&, const cx<float>*, const double*, cx<float>*, hipStream_t);
 
 void kr_scale_rhs(const KrylovWork& w, cx<double>* wv, const double* scale, hipStream_t s) {
   hipLaunchKernelGGL(k_scale_rhs, cgrid(w), dim3(256), 0, s, w, wv, scale);
@@ -588,78 +602,101 @@ void kr_norm2(const KrylovWork& w, const cx<double>* v, double* out, const int* 
 }
 
 // r: residual block, conv: per-RHS converged flags (device).  Uses w.nrm as scratch.
-void kr_start_impl(const KrylovWork& w, const cx<double>* r, cx<double>* V0, double tol, const int* conv,
+template <typename TV>
+void kr_start_impl(const KrylovWork& w, const cx<double>* r, cx<TV>* V0, double tol, const int* conv,
                    double* n2tmp, hipStream_t s) {
   kr_norm2(w, r, n2tmp, conv, s);
   hipLaunchKernelGGL(k_start_flags, dim3((w.nrhs + 63) / 64), dim3(64), 0, s, w, n2tmp, tol, conv);
-  hipLaunchKernelGGL(k_scale, cgrid(w), dim3(256), 0, s, w, r, V0);
+  hipLaunchKernelGGL((k_scale<TV>), cgrid(w), dim3(256), 0, s, w, r, V0);
 }
+template void kr_start_impl<double>(const KrylovWork&, const cx<double>*, cx<double>*, double, const int*, double*,
+                                    hipStream_t);
+template void kr_start_impl<float>(const KrylovWork&, const cx<double>*, cx<float>*, double, const int*, double*,
+                                   hipStream_t);
 
 // kt (optional) times pass 1 / pass 2; algorithmic bytes per active RHS:
 // 16 N (j + 2) for pass 1 (reads v_0..v_j and w), 16 N (j + 3) for pass 2
 // (also writes w')
 //
 // hz (optional, lazy mode): w = H z is formed inside passes 1 and 2 from the
-// complex64 z; bytes per active RHS 16 N (j + 1) + 8 N and 16 N (j + 2) + 8 N,
-// plus 16 N for the shared diagonal.  A staged input adds 8 N to pass 2.
-template <int NV>
-static void arnoldi_nv(const KrylovWork& w, const cx<double>* V, cx<double>* wv, int j, double tol,
+// complex64 z; bytes per active RHS sv N (j + 1) + 8 N and sv N (j + 2) + 8 N
+// (sv = 16 or 8 B per stored basis value), plus 16 N for the shared
+// diagonal.  A staged input adds 8 N to pass 2.  A complex64 basis (TV =
+// float) needs hz.
+template <int NV, typename TV>
+static void arnoldi_nv(const KrylovWork& w, const cx<TV>* V, cx<double>* wv, int j, double tol,
                        int* skip_copy, KTimer* kt, int active, const HzArgs* hz, hipStream_t s) {
   const int64_t stride = w.N * w.nrhs;
   const bool t1 = kt && kt->match(KT_DOTS, 0), t2 = kt && kt->match(KT_UPDATE, 0);
-  cx<double>* vnext = (cx<double>*)(V + (j + 1) * stride);
-  const double Nf = (double)w.N;
+  cx<TV>* vnext = (cx<TV>*)(V + (j + 1) * stride);
+  const double Nf = (double)w.N, sv = (double)sizeof(cx<TV>);
   if (t1) kt->begin(s);
-  if (hz)
-    hipLaunchKernelGGL((k_dots_hz<NV>), dim3(w.nrhs, w.nchunk), dim3(256), 0, s, w, hz->g, V, hz->z, hz->d, j);
-  else if (w.lazy)
-    hipLaunchKernelGGL((k_dots<NV, true>), cgrid(w), dim3(256), 0, s, w, V, wv, j, nullptr);
-  else
-    hipLaunchKernelGGL((k_dots<NV, false>), cgrid(w), dim3(256), 0, s, w, V, wv, j, nullptr);
-  if (t1) kt->end(s, hz ? (16.0 * (j + 1) + 8.0) * Nf * active + 16.0 * Nf : 16.0 * Nf * (j + 2) * active);
+  if (hz) {
+    hipLaunchKernelGGL((k_dots_hz<NV, TV>), dim3(w.nrhs, w.nchunk), dim3(256), 0, s, w, hz->g, V, hz->z, hz->d, j);
+  } else if constexpr (std::is_same<TV, double>::value) {
+    if (w.lazy)
+      hipLaunchKernelGGL((k_dots<NV, true, double>), cgrid(w), dim3(256), 0, s, w, V, wv, j, nullptr);
+    else
+      hipLaunchKernelGGL((k_dots<NV, false, double>), cgrid(w), dim3(256), 0, s, w, V, wv, j, nullptr);
+  }
+  if (t1) kt->end(s, hz ? (sv * (j + 1) + 8.0) * Nf * active + 16.0 * Nf : 16.0 * Nf * (j + 2) * active);
   hipLaunchKernelGGL(k_reduce_dots, dim3(j + (w.lazy ? 2 : 1), w.nrhs), dim3(64), 0, s, w, w.h1, j);
   if (t2) kt->begin(s);
   if (hz) {
-    hipLaunchKernelGGL((k_update_lazy_hz<NV>), dim3(w.nrhs, w.nchunk), dim3(256), 0, s, w, hz->g, V, hz->z, hz->d,
-                       vnext, j);
-  } else if (w.lazy) {
-    // w' goes straight into the next basis slot (kept unnormalised)
-    hipLaunchKernelGGL((k_update_lazy<NV>), cgrid(w), dim3(256), 0, s, w, V, wv, vnext, j);
-  } else {
-    hipLaunchKernelGGL((k_update_dots<NV>), cgrid(w), dim3(256), 0, s, w, V, wv, wv, j);
+    hipLaunchKernelGGL((k_update_lazy_hz<NV, TV>), dim3(w.nrhs, w.nchunk), dim3(256), 0, s, w, hz->g, V, hz->z,
+                       hz->d, vnext, j);
+  } else if constexpr (std::is_same<TV, double>::value) {
+    if (w.lazy) {
+      // w' goes straight into the next basis slot (kept unnormalised)
+      hipLaunchKernelGGL((k_update_lazy<NV>), cgrid(w), dim3(256), 0, s, w, V, wv, vnext, j);
+    } else {
+      hipLaunchKernelGGL((k_update_dots<NV>), cgrid(w), dim3(256), 0, s, w, V, wv, wv, j);
+    }
   }
   const double st8 = w.stage ? 8.0 : 0.0;  // staged complex64 copy of w'
   if (t2)
-    kt->end(s, hz ? (16.0 * (j + 2) + 8.0 + st8) * Nf * active + 16.0 * Nf : (16.0 * (j + 3) + st8) * Nf * active);
+    kt->end(s, hz ? (sv * (j + 2) + 8.0 + st8) * Nf * active + 16.0 * Nf : (16.0 * (j + 3) + st8) * Nf * active);
   if (w.lazy) {
     // second projection only for RHS whose projection cancelled
     hipLaunchKernelGGL(k_fix_flags, dim3(w.nrhs), dim3(64), 0, s, w);
-    hipLaunchKernelGGL((k_dots<NV, false>), cgrid(w), dim3(256), 0, s, w, V, (const cx<double>*)vnext, j, w.fix);
+    hipLaunchKernelGGL((k_dots<NV, false, TV>), cgrid(w), dim3(256), 0, s, w, V, (const cx<TV>*)vnext, j, w.fix);
   }
   hipMemcpyAsync(skip_copy, w.done, sizeof(int) * w.nrhs, hipMemcpyDeviceToDevice, s);
   hipLaunchKernelGGL(k_hess_givens, dim3(w.nrhs), dim3(64), 0, s, w, j, tol);
-  hipLaunchKernelGGL((k_final<NV>), cgrid(w), dim3(256), 0, s, w, V, w.lazy ? (const cx<double>*)vnext : wv, vnext,
-                     j, skip_copy);
-  if (w.lazy && w.stage) hipLaunchKernelGGL(k_restage, cgrid(w), dim3(256), 0, s, w, vnext, skip_copy);
+  if constexpr (std::is_same<TV, double>::value) {
+    hipLaunchKernelGGL((k_final<NV, double>), cgrid(w), dim3(256), 0, s, w, V,
+                       w.lazy ? (const cx<double>*)vnext : wv, vnext, j, skip_copy);
+    if (w.lazy && w.stage) hipLaunchKernelGGL(k_restage, cgrid(w), dim3(256), 0, s, w, vnext, skip_copy);
+  } else {
+    // the corrected vector is stored in complex64 in place: it is the next
+    // preconditioner input as it is (no staged copy)
+    hipLaunchKernelGGL((k_final<NV, TV>), cgrid(w), dim3(256), 0, s, w, V, (const cx<TV>*)vnext, vnext, j,
+                       skip_copy);
+  }
 }
 
-void kr_arnoldi_impl(const KrylovWork& w, const cx<double>* V, cx<double>* wv, int j, double tol, int* skip_copy,
+template <typename TV>
+void kr_arnoldi_impl(const KrylovWork& w, const cx<TV>* V, cx<double>* wv, int j, double tol, int* skip_copy,
                      KTimer* kt, int active, hipStream_t s, const HzArgs* hz) {
   if (j < 4)
-    arnoldi_nv<4>(w, V, wv, j, tol, skip_copy, kt, active, hz, s);
+    arnoldi_nv<4, TV>(w, V, wv, j, tol, skip_copy, kt, active, hz, s);
   else if (j < 6)
-    arnoldi_nv<6>(w, V, wv, j, tol, skip_copy, kt, active, hz, s);
+    arnoldi_nv<6, TV>(w, V, wv, j, tol, skip_copy, kt, active, hz, s);
   else if (j < 8)
-    arnoldi_nv<8>(w, V, wv, j, tol, skip_copy, kt, active, hz, s);
+    arnoldi_nv<8, TV>(w, V, wv, j, tol, skip_copy, kt, active, hz, s);
   else if (j < 10)
-    arnoldi_nv<10>(w, V, wv, j, tol, skip_copy, kt, active, hz, s);
+    arnoldi_nv<10, TV>(w, V, wv, j, tol, skip_copy, kt, active, hz, s);
   else if (j < 12)
-    arnoldi_nv<12>(w, V, wv, j, tol, skip_copy, kt, active, hz, s);
+    arnoldi_nv<12, TV>(w, V, wv, j, tol, skip_copy, kt, active, hz, s);
   else if (j < 16)
-    arnoldi_nv<16>(w, V, wv, j, tol, skip_copy, kt, active, hz, s);
+    arnoldi_nv<16, TV>(w, V, wv, j, tol, skip_copy, kt, active, hz, s);
   else
-    arnoldi_nv<32>(w, V, wv, j, tol, skip_copy, kt, active, hz, s);
+    arnoldi_nv<32, TV>(w, V, wv, j, tol, skip_copy, kt, active, hz, s);
 }
+template void kr_arnoldi_impl<double>(const KrylovWork&, const cx<double>*, cx<double>*, int, double, int*, KTimer*,
+                                      int, hipStream_t, const HzArgs*);
+template void kr_arnoldi_impl<float>(const KrylovWork&, const cx<float>*, cx<double>*, int, double, int*, KTimer*,
+                                     int, hipStream_t, const HzArgs*);
 
 template <typename TZ>
 void kr_update_x(const KrylovWork& w, const cx<TZ>* Z, cx<double>* x, hipStream_t s) {

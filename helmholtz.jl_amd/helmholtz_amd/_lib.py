@@ -44,7 +44,7 @@ class MGParams(C.Structure):
         ("krylov", C.c_int32), ("prec", C.c_int32), ("learned", C.c_int32), ("verbose", C.c_int32),
         ("graph", C.c_int32), ("autotune", C.c_int32), ("fused", C.c_int32), ("wlevels", C.c_int32),
         ("per_level", C.c_int32), ("pre_l", C.c_int32 * 16), ("post_l", C.c_int32 * 16),
-        ("relax_type", C.c_int32),
+        ("relax_type", C.c_int32), ("basis", C.c_int32),
     ]
 
 

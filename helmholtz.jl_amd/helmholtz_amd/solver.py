@@ -30,7 +30,10 @@ class MGparam:
     """Settings of Multigrid.jl's MGparam that this path honours.
 
     ``prec`` ("c128" | "c64") selects the preconditioner precision (an
-    extension: the reference runs the type of the matrix)."""
+    extension: the reference runs the type of the matrix).  ``basis``
+    ("c128" | "c64") is the storage of the FGMRES basis; "c64" needs
+    prec="c64", FGMRES and the 5/7-point operator (products, the iterate and
+    the restart residual stay complex128)."""
     levels: int = 3
     numCores: int = 1
     maxOuterIter: int = 50
@@ -43,6 +46,7 @@ class MGparam:
     coarseSolveType: str = "NoMUMPS"
     extra: tuple = field(default_factory=tuple)
     prec: str = "c128"
+    basis: str = "c128"
     graph: bool = False
     autotune: bool = False
     fused: bool = True
@@ -148,6 +152,8 @@ def _validate(s: ShiftedLaplacianMultigridSolver):
                 raise ValueError(f"sweep count {c} on level {l + 1}: 0..{cmax}")
     if mg.prec not in ("c128", "c64"):
         raise ValueError("MG.prec must be 'c128' or 'c64'")
+    if mg.basis not in ("c128", "c64"):
+        raise ValueError("MG.basis must be 'c128' or 'c64'")
 
 
 def _ensure_handle(s: ShiftedLaplacianMultigridSolver, outer=None):
@@ -178,7 +184,8 @@ def _ensure_handle(s: ShiftedLaplacianMultigridSolver, outer=None):
                          verbose=int(bool(s.verbose)), graph=int(bool(mg.graph)),
                          autotune=int(bool(mg.autotune)), fused=int(bool(mg.fused)),
                          wlevels=int(mg.wlevels),
-                         relax_type=L.HH_RELAX_JAC_GMRES if mg.relaxType == "Jac-GMRES" else L.HH_RELAX_JAC)
+                         relax_type=L.HH_RELAX_JAC_GMRES if mg.relaxType == "Jac-GMRES" else L.HH_RELAX_JAC,
+                         basis=L.HH_C64 if mg.basis == "c64" else L.HH_C128)
         if callable(mg.relaxPre) or callable(mg.relaxPost):
             prm.per_level = 1
             for l, (npre, npost) in enumerate(sw):

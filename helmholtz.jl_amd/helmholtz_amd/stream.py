@@ -33,7 +33,7 @@ def solve_stream(problems, workers: int = 4):
     """problems: list of dicts with keys ``param`` (HelmholtzParam), ``B``
     (N x nrhs complex), and optional ``levels`` (default: coarsen to <= 561
     nodes), ``cycle`` ('W'), ``prec`` ('c64'), ``shift`` (0.5), ``inner`` (10),
-    ``tol`` (1e-6), ``max_outer`` (100), ``graph``, ``autotune``, ``net``
+    ``tol`` (1e-6), ``max_outer`` (100), ``basis`` ('c128'), ``graph``, ``autotune``, ``net``
     (a learned.UNet) and ``learn_level`` (2).  Returns [(X, info)]."""
     from .train import levels_for
     lib = L.lib()
@@ -72,7 +72,8 @@ def solve_stream(problems, workers: int = 4):
                               inner=int(pr.get("inner", 10)), krylov=L.HH_FGMRES,
                               prec=L.HH_C64 if prec == "c64" else L.HH_C128, learned=0, verbose=0,
                               graph=int(bool(pr.get("graph", False))), autotune=int(bool(pr.get("autotune", False))),
-                              fused=int(bool(pr.get("fused", True))), wlevels=int(pr.get("wlevels", 0)))
+                              fused=int(bool(pr.get("fused", True))), wlevels=int(pr.get("wlevels", 0)),
+                              basis=L.HH_C64 if pr.get("basis", "c128") == "c64" else L.HH_C128)
         a.shift_per_level = shifts.ctypes.data
         a.B, a.X, a.nrhs, a.loc = B.ctypes.data, X.ctypes.data, nrhs, L.HH_HOST
         a.iters, a.status, a.true_res = iters.ctypes.data, status.ctypes.data, tres.ctypes.data

@@ -24,6 +24,7 @@ struct HHMGParams
     graph::Int32; autotune::Int32; fused::Int32; wlevels::Int32
     per_level::Int32; pre_l::NTuple{16,Int32}; post_l::NTuple{16,Int32}
     relax_type::Int32
+    basis::Int32
 end
 const HH_RELAX_JAC = Cint(0); const HH_RELAX_JAC_GMRES = Cint(1)
 
@@ -98,12 +99,13 @@ mutable struct ShiftedLaplacianMultigridSolverAMD
     op::Union{Nothing,HHOperator}
     handle::Ptr{Cvoid}
     prec::Cint              # HH_C128 (plain) or HH_C64 (mixed precision preconditioner)
+    basis::Cint             # FGMRES basis storage: HH_C128, or HH_C64 (needs prec = HH_C64)
 end
 
 getShiftedLaplacianMultigridSolverAMD(helmParam, MG, shift::Float64, Krylov="GMRES", inner=10, verbose=false;
-                                      prec=HH_C128) =
+                                      prec=HH_C128, basis=HH_C128) =
     ShiftedLaplacianMultigridSolverAMD(helmParam, MG, fill(shift, MG.levels), Krylov, inner, 0, verbose,
-                                       0.0, 0, 0.0, nothing, C_NULL, prec)
+                                       0.0, 0, 0.0, nothing, C_NULL, prec, basis)
 
 function ensure_handle!(s::ShiftedLaplacianMultigridSolverAMD)
     p = s.helmParam
@@ -119,7 +121,7 @@ function ensure_handle!(s::ShiftedLaplacianMultigridSolverAMD)
                              s.Krylov == "BiCGSTAB" ? HH_BICGSTAB : HH_FGMRES, s.prec, 0, s.verbose, 0, 0, 1, 0,
                              pl ? 1 : 0, per_level_sweeps(mg.relaxPre, mg.levels),
                              per_level_sweeps(mg.relaxPost, mg.levels),
-                             mg.relaxType == "Jac-GMRES" ? HH_RELAX_JAC_GMRES : HH_RELAX_JAC))
+                             mg.relaxType == "Jac-GMRES" ? HH_RELAX_JAC_GMRES : HH_RELAX_JAC, s.basis))
         out = Ref{Ptr{Cvoid}}(C_NULL)
         check(ccall((:hh_solver_create, LIB), Cint, (Ptr{Cvoid}, Ref{HHMGParams}, Ptr{Float64}, Ptr{Ptr{Cvoid}}),
                     s.op.ptr, prm, s.shift, out))
@@ -162,6 +164,6 @@ end
 
 copySolver(s::ShiftedLaplacianMultigridSolverAMD) =
     ShiftedLaplacianMultigridSolverAMD(s.helmParam, s.MG, copy(s.shift), s.Krylov, s.inner, 0, s.verbose,
-                                       0.0, 0, 0.0, nothing, C_NULL, s.prec)
+                                       0.0, 0, 0.0, nothing, C_NULL, s.prec, s.basis)
 
 end # module

@@ -74,6 +74,10 @@ typedef struct hh_mg_params {
   int32_t post_l[16]; /* per_level: sweeps after it */
   int32_t relax_type; /* MG.relaxType: HH_RELAX_JAC or HH_RELAX_JAC_GMRES (the
                          fused legs run only with HH_RELAX_JAC) */
+  int32_t basis;      /* storage of the FGMRES basis: HH_C128, or HH_C64 (with
+                         prec = HH_C64 and the 5/7-point operator: products and
+                         sums stay float64, the iterate and the restart residual
+                         complex128; halves the Arnoldi passes' traffic) */
 } hh_mg_params;
 
 /* Per-call results.  The arrays are caller-owned with nrhs entries, or NULL. */

@@ -23,9 +23,10 @@ def _oracle_operator(M, m, w, pad, neumann=True):
     return H, g.ravel(order="F")
 
 
-def _solver(hp, levels, prec, tol, inner, cycle="W", wlevels=0, max_outer=200):
+def _solver(hp, levels, prec, tol, inner, cycle="W", wlevels=0, max_outer=200, basis="c128"):
     mg = ha.getMGparam(complex, int, levels, 1, max_outer, tol, "Jac", 0.8, 2, 2, cycle, "NoMUMPS", 0.5, 0.0)
     mg.prec = prec
+    mg.basis = basis
     mg.wlevels = wlevels
     return ha.getShiftedLaplacianMultigridSolver(hp, mg, 0.5, "GMRES", inner)
 
@@ -35,10 +36,11 @@ def _true_res(H, X, B):
     return np.linalg.norm(R, axis=0) / np.linalg.norm(B, axis=0)
 
 
-def test_batched_2d_full_size():
-    """1025 x 513, 256 sources, complex64 preconditioner, complex128 FGMRES(10)
-    to 1e-6: all 256 true residuals from the oracle's CSR; four columns against
-    a complex128 solve to 1e-9."""
+@pytest.mark.parametrize("basis", ["c128", "c64"])
+def test_batched_2d_full_size(basis):
+    """1025 x 513, 256 sources, complex64 preconditioner, FGMRES(10) with a
+    complex128 or complex64 basis to 1e-6: all 256 true residuals from the
+    oracle's CSR; four columns against a complex128 solve to 1e-9."""
     n, h = (1025, 513), 10.0
     v = media.layered_body_velocity_2d(n, seed=2)
     m = (1.0 / v**2).ravel(order="F")
@@ -47,23 +49,23 @@ def test_batched_2d_full_size():
     H, g = _oracle_operator(M, m, w, [20, 20])
     hp = ha.HelmholtzParam(M, g, m, w, True, True)
     B = ha.point_sources(M, media.surface_sources_2d(n, 256))
-    s = _solver(hp, 6, "c64", 1e-6, 10, wlevels=3)
+    s = _solver(hp, 6, "c64", 1e-6, 10, wlevels=3, basis=basis)
     Xd, s = ha.solveLinearSystem(None, DeviceArray.from_numpy(B), s)
     X = Xd.to_numpy()
     res = _true_res(H, X, B)
-    print(f"2-D full size: iterations {s.last_stats['max_iters']}, max true residual {res.max():.2e}")
+    print(f"2-D full size, {basis} basis: iterations {s.last_stats['max_iters']}, max true residual {res.max():.2e}")
     assert res.max() <= 1e-6
-    _agreement(hp, B, X, [0, 85, 170, 255], 6, 10, 3)
+    _agreement(hp, B, X, [0, 85, 170, 255], 6, 10, 3, basis)
 
 
-def _agreement(hp, B, X, cols, levels, inner, wlevels):
+def _agreement(hp, B, X, cols, levels, inner, wlevels, basis="c128"):
     """Reference: the complex128 preconditioner to 1e-9.  The 1e-6 solution's
     error against it is stated (the operator's conditioning makes it ~1e-4);
     the complex64-preconditioned solve taken to the same 1e-9 must agree with
     the complex128 one to 1e-5."""
     sr = _solver(hp, levels, "c128", 1e-9, inner, wlevels=wlevels, max_outer=400)
     Xr, sr = ha.solveLinearSystem(None, B[:, cols], sr)
-    s9 = _solver(hp, levels, "c64", 1e-9, inner, wlevels=wlevels, max_outer=400)
+    s9 = _solver(hp, levels, "c64", 1e-9, inner, wlevels=wlevels, max_outer=400, basis=basis)
     X9, s9 = ha.solveLinearSystem(None, B[:, cols], s9)
     for k, c in enumerate(cols):
         ref = np.linalg.norm(Xr[:, k])
@@ -94,16 +96,19 @@ def test_3d_full_size_c64_vs_c128():
     B = ha.point_sources(M, subs)
     Bd = DeviceArray.from_numpy(B)
     X = {}
-    for prec in ("c64", "c128"):
-        s = _solver(hp, 5, prec, 1e-6, 8)
+    for prec, basis in (("c64", "c128"), ("c128", "c128"), ("c64", "c64")):
+        s = _solver(hp, 5, prec, 1e-6, 8, basis=basis)
         Xd, s = ha.solveLinearSystem(None, Bd, s)
-        X[prec] = Xd.to_numpy()
-        res = _true_res(H, X[prec], B)
-        print(f"3-D full size {prec}: iterations {s.last_stats['max_iters']}, max true residual {res.max():.2e}, "
-              f"solve {s.last_stats['solve_time']:.2f} s")
+        X[prec + basis] = Xd.to_numpy()
+        res = _true_res(H, X[prec + basis], B)
+        print(f"3-D full size {prec} cycle, {basis} basis: iterations {s.last_stats['max_iters']}, "
+              f"max true residual {res.max():.2e}, solve {s.last_stats['solve_time']:.2f} s")
         assert res.max() <= 1e-6
+    X["c64"], X["c128"] = X["c64c128"], X["c128c128"]
     d = np.linalg.norm(X["c64"] - X["c128"], axis=0) / np.linalg.norm(X["c128"], axis=0)
     print(f"  complex64 vs complex128 preconditioner at 1e-6: max solution difference {d.max():.2e}")
+    d = np.linalg.norm(X["c64c64"] - X["c64"], axis=0) / np.linalg.norm(X["c64"], axis=0)
+    print(f"  complex64 vs complex128 basis at 1e-6: max solution difference {d.max():.2e}")
     _agreement(hp, B, X["c64"], [0, 13, 31], 5, 8, 0)
 
 

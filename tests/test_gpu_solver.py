@@ -33,11 +33,12 @@ def c1_problem():
 
 
 def _solver(M, m, g, w, neumann, levels, shift, inner=10, tol=1e-6, cycle="V", prec="c128", max_outer=100,
-            relax=0.8, krylov="GMRES"):
+            relax=0.8, krylov="GMRES", basis="c128"):
     mesh = ha.getRegularMesh(M.domain, M.n)
     hp = ha.HelmholtzParam(mesh, g, m, w, neumann, True)
     mg = ha.getMGparam(complex, int, levels, 1, max_outer, tol, "Jac", relax, 2, 2, cycle, "NoMUMPS", 0.5, 0.0)
     mg.prec = prec
+    mg.basis = basis
     return ha.getShiftedLaplacianMultigridSolver(hp, mg, shift, krylov, inner, False)
 
 
@@ -166,23 +167,24 @@ def test_maxiter_status():
     assert s.last_stats["iters"][0] == 2
 
 
-def _c64_solver(M, m, g, w, neumann, levels, graph=False, autotune=False, cycle="W"):
-    s = _solver(M, m, g, w, neumann, levels, 0.5, cycle=cycle, prec="c64")
+def _c64_solver(M, m, g, w, neumann, levels, graph=False, autotune=False, cycle="W", basis="c128"):
+    s = _solver(M, m, g, w, neumann, levels, 0.5, cycle=cycle, prec="c64", basis=basis)
     s.MG.graph = graph
     s.MG.autotune = autotune
     return s
 
 
-def test_graph_replay_equals_eager():
+@pytest.mark.parametrize("basis", ["c128", "c64"])
+def test_graph_replay_equals_eager(basis):
     """Each FGMRES iteration captured as a hipGraph from the second restart
     cycle on and replayed (several times per solve, and again in a second
     call) gives bit-identical iterates to eager launches."""
     M, m, g, w = small_case((129, 129), seed=31, neumann=True)
     rng = np.random.default_rng(8)
     B = np.asfortranarray(rng.standard_normal((m.size, 6)) + 1j * rng.standard_normal((m.size, 6)))
-    se = _c64_solver(M, m, g, w, True, 4)
+    se = _c64_solver(M, m, g, w, True, 4, basis=basis)
     Xe, se = ha.solveLinearSystem(None, B, se)
-    sg = _c64_solver(M, m, g, w, True, 4, graph=True)
+    sg = _c64_solver(M, m, g, w, True, 4, graph=True, basis=basis)
     Xg, sg = ha.solveLinearSystem(None, B, sg)
     assert se.last_stats["max_iters"] > 20  # several restart cycles replayed
     assert np.array_equal(Xe, Xg)
@@ -624,3 +626,68 @@ def test_kcycle_solve_matches_oracle(prec):
         assert abs(it - ito) <= 1
     else:
         assert abs(it - ito) <= 3
+
+
+# ---- complex64 Krylov basis (MG.basis = "c64") ----
+# Products, sums, the iterate and each restart's residual stay complex128;
+# only the stored basis is rounded to complex64.  Bars: the true residual
+# from the oracle's CSR operator (north-star 1e-6), the iteration count of
+# the complex128-basis solve within 2, and at a tight tolerance the direct
+# solve (the rounding of the basis must not stall the restarts).
+
+
+def test_c64_basis_reaches_fp64_residual():
+    M, m, g, w, H, q = c1_problem()
+    it = {}
+    for basis in ("c128", "c64"):
+        s = _solver(M, m, g, w, False, 3, 0.5, prec="c64", basis=basis)
+        x, s = ha.solveLinearSystem(None, q, s)
+        res = np.linalg.norm(H @ x - q) / np.linalg.norm(q)
+        it[basis] = int(s.last_stats["iters"][0])
+        print(f"C1 c64 precond, {basis} basis: iters {it[basis]} true_res {res:.3e}")
+        assert res <= 1e-6
+    assert abs(it["c64"] - it["c128"]) <= 2
+    s = _solver(M, m, g, w, False, 3, 0.5, prec="c64", basis="c64", tol=1e-10, max_outer=200)
+    x, s = ha.solveLinearSystem(None, q, s)
+    xd = spla.splu(H.tocsc()).solve(q)
+    res = np.linalg.norm(H @ x - q) / np.linalg.norm(q)
+    err = np.linalg.norm(x - xd) / np.linalg.norm(xd)
+    print(f"C1 c64 basis tol=1e-10: iters {s.last_stats['iters'][0]} true_res {res:.3e} err_vs_direct {err:.3e}")
+    assert res <= 1e-10
+    assert err <= 1e-5
+
+
+@pytest.mark.parametrize("nodes,levels,cycle", [((129, 65), 4, "W"), ((257, 129), 5, "V"), ((33, 17, 17), 3, "W")])
+def test_c64_basis_batched(nodes, levels, cycle):
+    """Several RHS per call, 2-D and 3-D (w = H z formed inside the Arnoldi
+    passes on both), complex64 basis against the complex128 basis."""
+    M, m, g, w = small_case(nodes, seed=41, neumann=True)
+    H = hr.get_helmholtz_operator7(M, m, w, g, True, True, 2)
+    rng = np.random.default_rng(42)
+    B = np.asfortranarray(rng.standard_normal((m.size, 5)) + 1j * rng.standard_normal((m.size, 5)))
+    X, it = {}, {}
+    for basis in ("c128", "c64"):
+        s = _c64_solver(M, m, g, w, True, levels, cycle=cycle, basis=basis)
+        X[basis], s = ha.solveLinearSystem(None, B, s)
+        it[basis] = np.asarray(s.last_stats["iters"])
+        res = np.linalg.norm(H @ X[basis] - B, axis=0) / np.linalg.norm(B, axis=0)
+        print(f"{nodes} {cycle} {basis} basis: iters {it[basis].tolist()} max true_res {res.max():.2e}")
+        assert res.max() <= 1e-6
+        assert all(st == "converged" for st in s.last_stats["status"])
+    assert np.abs(it["c64"] - it["c128"]).max() <= 2
+
+
+def test_c64_basis_rejected_where_unsupported():
+    """basis = "c64" needs FGMRES, the complex64 cycle and the 5/7-point
+    operator: the library refuses the other combinations at create."""
+    M, m, g, w = small_case((33, 17), seed=43, neumann=True)
+    q = np.ones(m.size, complex)
+    for prec, krylov in (("c128", "GMRES"), ("c64", "BiCGSTAB")):
+        s = _solver(M, m, g, w, True, 2, 0.5, prec=prec, krylov=krylov, basis="c64")
+        with pytest.raises(ha.HelmholtzError):
+            ha.solveLinearSystem(None, q, s)
+    mesh = ha.getRegularMesh(M.domain, M.n)
+    Hg = ha.GetHelmholtzOperatorHO(mesh, m, w, g, True, True, 2.0 / 3.0)
+    s = _solver(M, m, g, w, True, 2, 0.5, prec="c64", basis="c64")
+    with pytest.raises(ha.HelmholtzError):
+        ha.solveLinearSystem(Hg, q, s)
