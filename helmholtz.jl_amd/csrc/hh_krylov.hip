@@ -51,6 +51,21 @@ static __device__ __forceinline__ void chunk_range(int64_t N, int nchunk, int c,
   *b = min(N, *a + L);
 }
 
+// The lazy passes that read a stored w (k_dots, k_update_lazy) and those that
+// form w = H z in place (k_dots_hz, k_update_lazy_hz) accumulate through these
+// helpers, written as explicit fmas in a fixed order, so both forms give
+// identical sums whatever the compiler contracts in either kernel.
+// acc + conj(v) w
+__device__ __forceinline__ cx<double> cmac_conj(cx<double> acc, cx<double> v, cx<double> w) {
+  return mk<double>(fma(v.im, w.im, fma(v.re, w.re, acc.re)), fma(-v.im, w.re, fma(v.re, w.im, acc.im)));
+}
+// w - h v
+__device__ __forceinline__ cx<double> csub_mul(cx<double> w, cx<double> h, cx<double> v) {
+  return mk<double>(fma(h.im, v.im, fma(-h.re, v.re, w.re)), fma(-h.im, v.re, fma(-h.re, v.im, w.im)));
+}
+// nn + |w|^2
+__device__ __forceinline__ double nrm_acc(double nn, cx<double> w) { return fma(w.im, w.im, fma(w.re, w.re, nn)); }
+
 // pass 1: part[c][i][b] = sum_{p in chunk c} conj(V_i[p,b]) w[p,b], i <= j
 // NORM: also partn[c][b] = sum ||w||^2 over the chunk.  only (may be null):
 // skip RHS with only[b] == 0.
@@ -71,10 +86,10 @@ __global__ __launch_bounds__(256) void k_dots(KrylovWork kw, const cx<TV>* __res
   double nn = 0.0;
   for (int64_t p = p0 + threadIdx.x; p < p1; p += 256) {
     const cx<double> wv = cvt<TV, double>(w[base + p]);
-    if (NORM) nn += cabs2(wv);
+    if (NORM) nn = nrm_acc(nn, wv);
 #pragma unroll
     for (int i = 0; i < NV; ++i)
-      if (i <= j) acc[i] = acc[i] + cmulc(cvt<TV, double>(V[i * stride + base + p]), wv);
+      if (i <= j) acc[i] = cmac_conj(acc[i], cvt<TV, double>(V[i * stride + base + p]), wv);
   }
 #pragma unroll
   for (int i = 0; i < NV; ++i) {
@@ -185,10 +200,10 @@ __global__ __launch_bounds__(256) void k_update_lazy(KrylovWork kw, const cx<dou
     cx<double> wv = w[base + p];
 #pragma unroll
     for (int i = 0; i < NV; ++i)
-      if (i <= j) wv = wv - hs[i] * V[i * stride + base + p];
+      if (i <= j) wv = csub_mul(wv, hs[i], V[i * stride + base + p]);
     wout[base + p] = wv;
     if (kw.stage) kw.stage[base + p] = mk<float>((float)wv.re, (float)wv.im);
-    nn += cabs2(wv);
+    nn = nrm_acc(nn, wv);
   }
   const double t = block_sum(nn, sh);
   if (threadIdx.x == 0) kw.partn[(int64_t)c * kw.nrhs + b] = t;
@@ -203,31 +218,46 @@ __global__ __launch_bounds__(256) void k_update_lazy(KrylovWork kw, const cx<dou
 // (H z)_p on a 2-D or 3-D grid: 5/7-point operator with mirror (BC 2) or
 // self (BC 1) ghosts and complex128 diagonal d, in float64 (apply5/apply7, as
 // in the stencil kernels' APPLY mode, so the values equal a stored H z bit for
-// bit).  zb: this RHS's block of z.
-__device__ __forceinline__ cx<double> hz_at(const Grid& g, const cx<float>* __restrict__ zb,
-                                            const cx<double>* __restrict__ d, int ci, int cj, int ck, int64_t p) {
+// bit).  zb: this RHS's block of z.  hz_load issues the loads, hz_eval forms
+// the value, so a caller can put every load of an iteration in flight first.
+template <int DIM>
+struct HzOps {
+  cx<float> c, x1l, x1r, x2l, x2r, x3l, x3r;
+  cx<double> d;
+};
+
+template <int DIM>
+__device__ __forceinline__ void hz_load(const Grid& g, const cx<float>* __restrict__ zb,
+                                        const cx<double>* __restrict__ d, int ci, int cj, int ck, int64_t p,
+                                        HzOps<DIM>& o) {
   const int n1 = g.n1, n2 = g.n2;
-  const cx<double> xc = cvt<float, double>(zb[p]);
-  if (g.dim == 2) {
-    const int64_t row = (int64_t)cj * n1;
-    const cx<double> xl = cvt<float, double>(zb[row + lo_nb(ci, g.bc)]);
-    const cx<double> xr = cvt<float, double>(zb[row + hi_nb(ci, n1, g.bc)]);
-    const cx<double> xm = cvt<float, double>(zb[(int64_t)lo_nb(cj, g.bc) * n1 + ci]);
-    const cx<double> xp = cvt<float, double>(zb[(int64_t)hi_nb(cj, n2, g.bc) * n1 + ci]);
-    return apply5(xc, xl, xr, xm, xp, g.ih1, g.ih2, d[p]);
+  const int64_t plane = (int64_t)n1 * n2, pl = DIM == 3 ? (int64_t)ck * plane : 0;
+  const int64_t row = pl + (int64_t)cj * n1;
+  o.c = zb[p];
+  o.x1l = zb[row + lo_nb(ci, g.bc)];
+  o.x1r = zb[row + hi_nb(ci, n1, g.bc)];
+  o.x2l = zb[pl + (int64_t)lo_nb(cj, g.bc) * n1 + ci];
+  o.x2r = zb[pl + (int64_t)hi_nb(cj, n2, g.bc) * n1 + ci];
+  if (DIM == 3) {
+    const int64_t col = (int64_t)cj * n1 + ci;
+    o.x3l = zb[(int64_t)lo_nb(ck, g.bc) * plane + col];
+    o.x3r = zb[(int64_t)hi_nb(ck, g.n3, g.bc) * plane + col];
   }
-  const int64_t plane = (int64_t)n1 * n2, pl = (int64_t)ck * plane;
-  const cx<double> x1l = cvt<float, double>(zb[pl + (int64_t)cj * n1 + lo_nb(ci, g.bc)]);
-  const cx<double> x1r = cvt<float, double>(zb[pl + (int64_t)cj * n1 + hi_nb(ci, n1, g.bc)]);
-  const cx<double> x2l = cvt<float, double>(zb[pl + (int64_t)lo_nb(cj, g.bc) * n1 + ci]);
-  const cx<double> x2r = cvt<float, double>(zb[pl + (int64_t)hi_nb(cj, n2, g.bc) * n1 + ci]);
-  const int64_t col = (int64_t)cj * n1 + ci;
-  const cx<double> xm = cvt<float, double>(zb[(int64_t)lo_nb(ck, g.bc) * plane + col]);
-  const cx<double> xp = cvt<float, double>(zb[(int64_t)hi_nb(ck, g.n3, g.bc) * plane + col]);
-  return apply7(xc, x1l, x1r, x2l, x2r, xm, xp, g.ih1, g.ih2, g.ih3, d[p]);
+  o.d = d[p];
 }
 
-// node coordinates of p and their advance by one block stride (256 nodes)
+template <int DIM>
+__device__ __forceinline__ cx<double> hz_eval(const Grid& g, const HzOps<DIM>& o) {
+  const cx<double> xc = cvt<float, double>(o.c);
+  const cx<double> x1l = cvt<float, double>(o.x1l), x1r = cvt<float, double>(o.x1r);
+  const cx<double> x2l = cvt<float, double>(o.x2l), x2r = cvt<float, double>(o.x2r);
+  if (DIM == 2) return apply5(xc, x1l, x1r, x2l, x2r, g.ih1, g.ih2, o.d);
+  return apply7(xc, x1l, x1r, x2l, x2r, cvt<float, double>(o.x3l), cvt<float, double>(o.x3r), g.ih1, g.ih2, g.ih3,
+                o.d);
+}
+
+// node coordinates of p and their advance by `by` positions (a multiple of
+// the block's 256 threads)
 struct NodeWalk {
   int ci, cj, ck;
   __device__ __forceinline__ void init(const Grid& g, int64_t p) {
@@ -236,8 +266,8 @@ struct NodeWalk {
     cj = (int)(q % g.n2);
     ck = (int)(q / g.n2);
   }
-  __device__ __forceinline__ void step(const Grid& g) {
-    ci += 256;
+  __device__ __forceinline__ void step(const Grid& g, int by) {
+    ci += by;
     while (ci >= g.n1) {
       ci -= g.n1;
       if (++cj == g.n2) {
@@ -248,11 +278,25 @@ struct NodeWalk {
   }
 };
 
+// Both passes walk their chunk with U nodes per thread per iteration (p and
+// p + 256 for U = 2) and issue every load of an iteration — the stencil
+// operands, the diagonal and the NV basis values — before using any, with
+// no branch between them: basis vector i is read at min(i, j), the values
+// past j (re-reads of v_j from cache) feed only accumulators that are never
+// stored (pass 1) or are scaled by a zero coefficient (pass 2).  Written
+// with a load-use chain per basis vector, the compiler waited on each load
+// in turn and the passes were bound by load latency, not HBM bandwidth.  A
+// node past the chunk end (U = 2) loads node p's values and contributes 0.
+// U = 1 keeps the summation order of the stored-w kernels.  EARLY (all NV
+// basis values held in registers at once) for NV <= 10; longer restarts
+// read one basis value at a time within their register budget.
+
 // pass 1: part = V^H (H z), partn = ||H z||^2
-template <int NV, typename TV>
+template <int NV, typename TV, int U, int DIM>
 __global__ __launch_bounds__(256) void k_dots_hz(KrylovWork kw, Grid g, const cx<TV>* __restrict__ V,
                                                  const cx<float>* __restrict__ Z, const cx<double>* __restrict__ d,
                                                  int j) {
+  constexpr bool EARLY = NV <= 10;
   const int b = blockIdx.x, c = blockIdx.y;
   if (kw.done[b]) return;
   __shared__ double sh[4];
@@ -266,15 +310,41 @@ __global__ __launch_bounds__(256) void k_dots_hz(KrylovWork kw, Grid g, const cx
   for (int i = 0; i < NV; ++i) acc[i] = mk<double>(0.0, 0.0);
   double nn = 0.0;
   const double zs = kw.zunscaled ? kw.vscale[j * kw.nrhs + b] : 1.0;
-  NodeWalk nw;
-  nw.init(g, p0 + threadIdx.x);
-  for (int64_t p = p0 + threadIdx.x; p < p1; p += 256) {
-    const cx<double> wv = zs * hz_at(g, zb, d, nw.ci, nw.cj, nw.ck, p);
-    nn += cabs2(wv);
+  NodeWalk nw[U];
 #pragma unroll
-    for (int i = 0; i < NV; ++i)
-      if (i <= j) acc[i] = acc[i] + cmulc(cvt<TV, double>(V[i * stride + base + p]), wv);
-    nw.step(g);
+  for (int u = 0; u < U; ++u) nw[u].init(g, p0 + threadIdx.x + 256 * u);
+  for (int64_t p = p0 + threadIdx.x; p < p1; p += 256 * U) {
+    HzOps<DIM> ho[U];
+    int64_t qq[U];
+    bool ok[U];
+    cx<TV> vv[EARLY ? U : 1][EARLY ? NV : 1];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      ok[u] = p + 256 * u < p1;
+      qq[u] = ok[u] ? p + 256 * u : p;
+      hz_load<DIM>(g, zb, d, ok[u] ? nw[u].ci : nw[0].ci, ok[u] ? nw[u].cj : nw[0].cj,
+                   ok[u] ? nw[u].ck : nw[0].ck, qq[u], ho[u]);
+    }
+    if constexpr (EARLY) {
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int i = 0; i < NV; ++i) vv[u][i] = V[(int64_t)min(i, j) * stride + base + qq[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const cx<double> wv = ok[u] ? zs * hz_eval<DIM>(g, ho[u]) : mk<double>(0.0, 0.0);
+      nn = nrm_acc(nn, wv);
+#pragma unroll
+      for (int i = 0; i < NV; ++i) {
+        if constexpr (EARLY)
+          acc[i] = cmac_conj(acc[i], cvt<TV, double>(vv[u][i]), wv);
+        else if (i <= j)  // (register budget: one basis value at a time)
+          acc[i] = cmac_conj(acc[i], cvt<TV, double>(V[(int64_t)i * stride + base + qq[u]]), wv);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) nw[u].step(g, 256 * U);
   }
 #pragma unroll
   for (int i = 0; i < NV; ++i) {
@@ -291,15 +361,17 @@ __global__ __launch_bounds__(256) void k_dots_hz(KrylovWork kw, Grid g, const cx
 
 // pass 2: w' = H z - V h1 (stored in the basis precision), partn = ||w'||^2
 // of the unrounded w'
-template <int NV, typename TV>
+template <int NV, typename TV, int U, int DIM>
 __global__ __launch_bounds__(256) void k_update_lazy_hz(KrylovWork kw, Grid g, const cx<TV>* __restrict__ V,
                                                         const cx<float>* __restrict__ Z,
                                                         const cx<double>* __restrict__ d,
                                                         cx<TV>* __restrict__ wout, int j) {
+  constexpr bool EARLY = NV <= 10;
   const int b = blockIdx.x, c = blockIdx.y;
   if (kw.done[b]) return;
   __shared__ double sh[4];
   __shared__ cx<double> hs[NV];
+  // coefficients past j are 0: those terms subtract exact zeros
   if (threadIdx.x < NV) hs[threadIdx.x] = threadIdx.x <= j ? kw.h1s[threadIdx.x * kw.nrhs + b] : mk<double>(0.0, 0.0);
   __syncthreads();
   int64_t p0, p1;
@@ -309,17 +381,45 @@ __global__ __launch_bounds__(256) void k_update_lazy_hz(KrylovWork kw, Grid g, c
   const cx<float>* __restrict__ zb = Z + base;
   double nn = 0.0;
   const double zs = kw.zunscaled ? kw.vscale[j * kw.nrhs + b] : 1.0;
-  NodeWalk nw;
-  nw.init(g, p0 + threadIdx.x);
-  for (int64_t p = p0 + threadIdx.x; p < p1; p += 256) {
-    cx<double> wv = zs * hz_at(g, zb, d, nw.ci, nw.cj, nw.ck, p);
+  NodeWalk nw[U];
 #pragma unroll
-    for (int i = 0; i < NV; ++i)
-      if (i <= j) wv = wv - hs[i] * cvt<TV, double>(V[i * stride + base + p]);
-    wout[base + p] = cvt<double, TV>(wv);
-    if (kw.stage) kw.stage[base + p] = mk<float>((float)wv.re, (float)wv.im);
-    nn += cabs2(wv);
-    nw.step(g);
+  for (int u = 0; u < U; ++u) nw[u].init(g, p0 + threadIdx.x + 256 * u);
+  for (int64_t p = p0 + threadIdx.x; p < p1; p += 256 * U) {
+    HzOps<DIM> ho[U];
+    int64_t qq[U];
+    bool ok[U];
+    cx<TV> vv[EARLY ? U : 1][EARLY ? NV : 1];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      ok[u] = p + 256 * u < p1;
+      qq[u] = ok[u] ? p + 256 * u : p;
+      hz_load<DIM>(g, zb, d, ok[u] ? nw[u].ci : nw[0].ci, ok[u] ? nw[u].cj : nw[0].cj,
+                   ok[u] ? nw[u].ck : nw[0].ck, qq[u], ho[u]);
+    }
+    if constexpr (EARLY) {
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int i = 0; i < NV; ++i) vv[u][i] = V[(int64_t)min(i, j) * stride + base + qq[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      cx<double> wv = zs * hz_eval<DIM>(g, ho[u]);
+#pragma unroll
+      for (int i = 0; i < NV; ++i) {
+        if constexpr (EARLY)
+          wv = csub_mul(wv, hs[i], cvt<TV, double>(vv[u][i]));
+        else if (i <= j)
+          wv = csub_mul(wv, hs[i], cvt<TV, double>(V[(int64_t)i * stride + base + qq[u]]));
+      }
+      if (ok[u]) {
+        wout[base + qq[u]] = cvt<double, TV>(wv);
+        if (kw.stage) kw.stage[base + qq[u]] = mk<float>((float)wv.re, (float)wv.im);
+        nn = nrm_acc(nn, wv);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) nw[u].step(g, 256 * U);
   }
   const double t = block_sum(nn, sh);
   if (threadIdx.x == 0) kw.partn[(int64_t)c * kw.nrhs + b] = t;
@@ -631,8 +731,15 @@ static void arnoldi_nv(const KrylovWork& w, const cx<TV>* V, cx<double>* wv, int
   cx<TV>* vnext = (cx<TV>*)(V + (j + 1) * stride);
   const double Nf = (double)w.N, sv = (double)sizeof(cx<TV>);
   if (t1) kt->begin(s);
+  // nodes per thread per iteration: 2 for a complex64 basis up to NV = 10
+  constexpr int U = std::is_same<TV, float>::value && NV <= 10 ? 2 : 1;
   if (hz) {
-    hipLaunchKernelGGL((k_dots_hz<NV, TV>), dim3(w.nrhs, w.nchunk), dim3(256), 0, s, w, hz->g, V, hz->z, hz->d, j);
+    if (hz->g.dim == 2)
+      hipLaunchKernelGGL((k_dots_hz<NV, TV, U, 2>), dim3(w.nrhs, w.nchunk), dim3(256), 0, s, w, hz->g, V, hz->z,
+                         hz->d, j);
+    else
+      hipLaunchKernelGGL((k_dots_hz<NV, TV, U, 3>), dim3(w.nrhs, w.nchunk), dim3(256), 0, s, w, hz->g, V, hz->z,
+                         hz->d, j);
   } else if constexpr (std::is_same<TV, double>::value) {
     if (w.lazy)
       hipLaunchKernelGGL((k_dots<NV, true, double>), cgrid(w), dim3(256), 0, s, w, V, wv, j, nullptr);
@@ -643,8 +750,12 @@ static void arnoldi_nv(const KrylovWork& w, const cx<TV>* V, cx<double>* wv, int
   hipLaunchKernelGGL(k_reduce_dots, dim3(j + (w.lazy ? 2 : 1), w.nrhs), dim3(64), 0, s, w, w.h1, j);
   if (t2) kt->begin(s);
   if (hz) {
-    hipLaunchKernelGGL((k_update_lazy_hz<NV, TV>), dim3(w.nrhs, w.nchunk), dim3(256), 0, s, w, hz->g, V, hz->z,
-                       hz->d, vnext, j);
+    if (hz->g.dim == 2)
+      hipLaunchKernelGGL((k_update_lazy_hz<NV, TV, U, 2>), dim3(w.nrhs, w.nchunk), dim3(256), 0, s, w, hz->g, V,
+                         hz->z, hz->d, vnext, j);
+    else
+      hipLaunchKernelGGL((k_update_lazy_hz<NV, TV, U, 3>), dim3(w.nrhs, w.nchunk), dim3(256), 0, s, w, hz->g, V,
+                         hz->z, hz->d, vnext, j);
   } else if constexpr (std::is_same<TV, double>::value) {
     if (w.lazy) {
       // w' goes straight into the next basis slot (kept unnormalised)
