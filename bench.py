@@ -7,9 +7,11 @@ seeded layered medium (1500-4500 m/s, 12 interfaces, elliptic 4500 m/s body,
 sides, 256 point sources on depth row 2 (x = 3 + 4k).  Preconditioner: 6-level
 shifted-Laplacian cycle, W(2,2) on the three finest levels and V(2,2) below,
 damped Jacobi 0.8, shift 0.4 on the finest level and 0.5 below (``SHIFTS``);
-outer FGMRES(10) to ||b - H x|| <= 1e-6 ||b|| (iterate and restart residual
-in complex128; the complex64-cycle solves store the Krylov basis in complex64,
-``--basis``, with every product and sum in float64).
+outer FGMRES(10) in complex128 to ||b - H x|| <= 1e-6 ||b||.  The same batched
+solve with the FGMRES basis stored in complex64 (products, sums, iterate and
+restart residual still float64/complex128; ``basis = "c64"``, an option beyond
+the configured complex128 outer iteration) is reported beside it as
+``c64_basis``, not as ``value`` (``--no-c64-basis`` skips it).
 
 A step = one solve of all 256 sources in one batched call (complex64
 preconditioner, inputs resident in HBM, stencil launch shapes autotuned) with
@@ -83,26 +85,29 @@ def c3_problem():
     return hp, ha.point_sources(M, subs)
 
 
-def c3_lines(log, basis="c128"):
+def c3_lines(log, c64_basis=True):
     """Complex64 vs complex128 preconditioner on the 3-D case: 5-level W(2,2)
     cycle (the coarsest level must have at most 4096 nodes for the dense
-    solve, so 4 levels do not fit), FGMRES(8) to 1e-6; the complex64 line
-    stores the basis as ``basis`` says, the complex128 line in complex128."""
+    solve, so 4 levels do not fit), complex128 FGMRES(8) to 1e-6; with
+    ``c64_basis`` also the complex64 preconditioner with the FGMRES basis
+    stored in complex64 (``c64_c64basis_*``)."""
     import helmholtz_amd as ha
     from helmholtz_amd.device import DeviceArray
     hp, B = c3_problem()
     Bd = DeviceArray.from_numpy(B)
     out = {"workload": "3-D 193x193x97, 32 sources, 8 ppw, 5-level W(2,2), FGMRES(8), 1e-6 true residual"}
-    for prec in ("c64", "c128"):
+    runs = [("c64", "c128", "c64"), ("c128", "c128", "c128")]
+    if c64_basis:
+        runs.append(("c64", "c64", "c64_c64basis"))
+    for prec, basis, key in runs:
         mg = ha.getMGparam(complex, int, 5, 1, 100, 1e-6, "Jac", 0.8, 2, 2, "W", "NoMUMPS", 0.5, 0.0)
         mg.prec = prec
-        mg.basis = basis if prec == "c64" else "c128"
+        mg.basis = basis
         s = ha.getShiftedLaplacianMultigridSolver(hp, mg, 0.5, "GMRES", 8)
-        t, st = timed_solves(ha, s, Bd, 1, 1, log, f"3-D {prec}")
-        out[f"{prec}_ms_per_rhs"] = round(1e3 * t / B.shape[1], 3)
-        out[f"{prec}_max_iters"] = int(st["max_iters"])
-        out[f"{prec}_max_true_res"] = float(max(st["true_res"]))
-    out["c64_basis"] = basis
+        t, st = timed_solves(ha, s, Bd, 1, 1, log, f"3-D {key}")
+        out[f"{key}_ms_per_rhs"] = round(1e3 * t / B.shape[1], 3)
+        out[f"{key}_max_iters"] = int(st["max_iters"])
+        out[f"{key}_max_true_res"] = float(max(st["true_res"]))
     return out
 
 
@@ -279,8 +284,8 @@ def main():
     ap.add_argument("--no-stream", action="store_true", help="skip the uneven-stream line")
     ap.add_argument("--train-media", type=int, default=40)
     ap.add_argument("--train-epochs", type=int, default=6)
-    ap.add_argument("--basis", choices=["c64", "c128"], default="c64",
-                    help="FGMRES basis storage of the complex64-cycle solves (the plain leg stays complex128)")
+    ap.add_argument("--no-c64-basis", action="store_true",
+                    help="skip the solves with the FGMRES basis stored in complex64")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the solve shards by right-hand side "
@@ -299,7 +304,7 @@ def main():
     Bd = DeviceArray.from_numpy(B)
 
     # ---- batched, complex64 preconditioner, plain W cycle (autotuned launches)
-    sb = make_solver(hp, "c64", fast=True, basis=args.basis)
+    sb = make_solver(hp, "c64", fast=True)
     t_b, st_b = timed_solves(ha, sb, Bd, args.warmup, args.steps, log, "batched")
     res_b = np.asarray(st_b["true_res"])
     batched_ms_rhs = 1e3 * t_b / args.nsrc
@@ -309,12 +314,23 @@ def main():
     host_ms_rhs = 1e3 * (time.perf_counter() - t0) / args.nsrc
     log(f"batched from host buffers: {host_ms_rhs:.3f} ms/RHS")
 
+    # ---- the same batched solve with the FGMRES basis stored in complex64
+    c64b = None
+    if not args.no_c64_basis:
+        sc = make_solver(hp, "c64", fast=True, basis="c64")
+        t_c, st_c = timed_solves(ha, sc, Bd, args.warmup, args.steps, log, "batched c64 basis")
+        c64b = {"ms_per_rhs": round(1e3 * t_c / args.nsrc, 4), "max_iters": int(st_c["max_iters"]),
+                "max_true_res": float(max(st_c["true_res"])),
+                "note": "FGMRES basis stored in complex64; products, sums, iterate and restart residual "
+                        "float64/complex128; plain cycle"}
+        del sc
+
     # ---- learned + batched: U-Net coarse-grid correction at the 257x129 level
     from helmholtz_amd import learned, train
     net, tinfo = train.train_coarse_correction(n=(513, 513), media_count=args.train_media, level=2,
                                                epochs=args.train_epochs, log=log)
     log(f"training: {tinfo}")
-    sl = make_solver(hp, "c64", fast=True, basis=args.basis)
+    sl = make_solver(hp, "c64", fast=True)
     learned.attach(sl, net, 2)
     t_l, st_l = timed_solves(ha, sl, Bd, args.warmup, args.steps, log, "learned",
                              before_timed=lambda s: ha.time_kernel(s, *ROOFLINE_KERNEL))
@@ -344,8 +360,8 @@ def main():
         log(f"plain rhs {c}: {dt:.3f} s, {plain_it[-1]} iters, res {plain_res[-1]:.2e}")
     plain_ms_rhs = 1e3 * float(np.mean(plain_t))
 
-    c3 = None if args.no_3d else c3_lines(log, args.basis)
-    st5 = None if args.no_stream else stream_line(log, basis=args.basis)
+    c3 = None if args.no_3d else c3_lines(log, not args.no_c64_basis)
+    st5 = None if args.no_stream else stream_line(log)
     copy_gbs = calibration_copy(Bd, log)
     roof["copy_gbs"] = round(copy_gbs, 1)
     roof["copy_frac"] = round(roof["achieved"] / copy_gbs, 4)
@@ -365,12 +381,12 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": f"c64 preconditioner, {args.basis} Krylov basis, f64 products / c128 iterate and residual",
+        "dtype": "c64 preconditioner / c128 outer",
         "data": "synthetic (seeded layered+body medium, point sources)",
         "config": {"workload": "2-D 1025x513 Helmholtz, 256 point sources, 8 ppw, 1e-6 true residual",
                    "nsrc": args.nsrc, "levels": 6, "cycle": "W(2,2) on levels 0-2, V(2,2) below", "krylov": "FGMRES(10)",
                    "learned": "U-Net at level 2 (257x129) vs plain cycle, chosen by a two-restart trial on the first solve", "graphs": False,
-                   "autotune": True, "basis": args.basis, "parallelism": "rhs-batched, 1 GPU"},
+                   "autotune": True, "parallelism": "rhs-batched, 1 GPU"},
         "learned_ms_per_rhs": round(learned_ms_rhs, 4),
         "learned_max_iters": int(st_l["max_iters"]),
         "learned_used": bool(st_l["learned_used"]),
@@ -381,6 +397,7 @@ def main():
         "batched_max_iters": int(st_b["max_iters"]),
         "batched_max_true_res": float(res_b.max()),
         "pcie_inclusive_ms_per_rhs": round(host_ms_rhs, 4),
+        "c64_basis": c64b,
         "all_converged": bool(np.all(res_b <= 1e-6)) and bool(np.all(res_l <= 1e-6)) and bool(max(plain_res) <= 1e-6),
         "plain_ms_per_rhs": round(plain_ms_rhs, 3),
         "plain_iters": plain_it,

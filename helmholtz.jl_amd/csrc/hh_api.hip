@@ -1834,6 +1834,8 @@ int hh_solver_time_kernel(hh_solver* s, int kernel, int level) {
     if (!s) throw HHError(HH_ERR_ARG, "null argument");
     if (kernel < HH_KT_NONE || kernel > HH_KT_UPDATE) throw HHError(HH_ERR_ARG, "unknown kernel id");
     if (kernel >= HH_KT_APPLY && level != 0) throw HHError(HH_ERR_ARG, "H z and the Arnoldi passes run on level 0");
+    if (kernel == HH_KT_APPLY && s->p.basis == HH_C64)
+      throw HHError(HH_ERR_UNSUPPORTED, "with the complex64 basis H z is formed inside the Arnoldi passes only");
     if (kernel >= HH_KT_DOWN && kernel <= HH_KT_UP && (level < 0 || level >= s->p.levels - 1))
       throw HHError(HH_ERR_ARG, "the fused legs run on levels 0..levels-2");
     s->kt.reset();

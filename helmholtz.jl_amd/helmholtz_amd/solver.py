@@ -282,8 +282,10 @@ KERNELS = {"none": L.HH_KT_NONE, "down": L.HH_KT_DOWN, "mid": L.HH_KT_MID, "up":
 def time_kernel(s: ShiftedLaplacianMultigridSolver, kernel: str, level: int = 0):
     """Bracket every launch of one kernel of later solves with HIP events on
     the solver's stream (instrumentation).  kernel: 'down' | 'mid' | 'up'
-    (fused multigrid legs on grid level ``level``), 'apply' (H z), 'dots' /
-    'update' (Arnoldi passes 1 and 2), or 'none'.  Resets the totals."""
+    (fused multigrid legs on grid level ``level``), 'apply' (H z; not with
+    MG.basis = "c64", where H z is formed inside the Arnoldi passes only),
+    'dots' / 'update' (Arnoldi passes 1 and 2), or 'none'.  Resets the
+    totals."""
     _validate(s)
     _ensure_handle(s)
     L.check(L.lib().hh_solver_time_kernel(s._handle, KERNELS[kernel], int(level)))

@@ -175,10 +175,18 @@ int hh_solver_precondition(hh_solver* s, const void* V, void* Z, int64_t nrhs, i
  * RHS still iterating) accumulate until the next hh_solver_time_kernel call.
  * HH_KT_NONE switches it off.  Graph replay is not used while timing is on.
  * Algorithmic bytes per launch (s = bytes per preconditioner value, N / Nc =
- * nodes of the level / the next coarser level, a = active RHS):
- *   DOWN   a (2 N s + Nc s) + N s      MID  a (3 N s + 2 Nc s) + N s
- *   UP     a (3 N s + Nc s) + N s      APPLY a N (s_z + 16) + 16 N
- *   DOTS   16 a N (j + 2)              UPDATE 16 a N (j + 3)  (Arnoldi step j) */
+ * nodes of the level / the next coarser level, a = active RHS; the shared
+ * diagonal and Jacobi factor count once per launch):
+ *   DOWN   a (2 N s + Nc s) + 2 N s    MID  a (3 N s + 2 Nc s) + N s
+ *   UP     a (3 N s + Nc s) + 2 N s    APPLY a N (s_z + 16) + 16 N
+ *   DOTS   16 a N (j + 2)              UPDATE 16 a N (j + 3)  (Arnoldi step j)
+ * With prec = HH_C64 on the 5/7-point operator the passes form H z from the
+ * complex64 z (v = 16 or 8 B per basis value, t = 8 B for a staged copy with
+ * the complex128 basis, else 0):
+ *   DOTS   a N (v (j + 1) + 8) + 16 N  UPDATE a N (v (j + 2) + 8 + t) + 16 N
+ * and APPLY (a separate H z launch) exists only with the complex128 basis:
+ * timing it switches that solve to the stored-w passes, with the same
+ * iterates; with basis = HH_C64 it is refused. */
 enum {
   HH_KT_NONE = 0, HH_KT_DOWN = 1, HH_KT_MID = 2, HH_KT_UP = 3, HH_KT_APPLY = 4, HH_KT_DOTS = 5, HH_KT_UPDATE = 6
 };

@@ -691,3 +691,10 @@ def test_c64_basis_rejected_where_unsupported():
     s = _solver(M, m, g, w, True, 2, 0.5, prec="c64", basis="c64")
     with pytest.raises(ha.HelmholtzError):
         ha.solveLinearSystem(Hg, q, s)
+    # no separate H z launch exists to time with the complex64 basis
+    s = _solver(M, m, g, w, True, 2, 0.5, prec="c64", basis="c64")
+    with pytest.raises(ha.HelmholtzError):
+        ha.time_kernel(s, "apply", 0)
+    ha.time_kernel(s, "dots", 0)
+    x, s = ha.solveLinearSystem(None, q, s)
+    assert ha.kernel_time(s)["launches"] == s.nPrec
