@@ -58,6 +58,12 @@ def test_solver_rejects_unsupported_settings():
     mg9 = ha.getMGparam(complex, int, 3, 1, 30, 1e-6, "Jac-GMRES", 0.75, 9, 2, "K", "NoMUMPS", 0.5, 0.0)
     with pytest.raises(ValueError):
         ha.solveLinearSystem(None, np.ones(81, complex), ha.getShiftedLaplacianMultigridSolver(hp, mg9, 0.5, "GMRES", 5))
+    # the FGMRES basis is stored in complex128 or complex64 only
+    mgb = ha.getMGparam(complex, int, 3, 1, 30, 1e-6, "Jac", 0.75, 2, 2, "V", "NoMUMPS", 0.5, 0.0)
+    mgb.basis = "bf16"
+    with pytest.raises(ValueError):
+        ha.solveLinearSystem(None, np.ones(81, complex), ha.getShiftedLaplacianMultigridSolver(hp, mgb, 0.5, "GMRES", 5))
+    assert ha.getMGparam(3, 1, 30, 1e-6, "Jac", 0.8, 2, 2, "V", "NoMUMPS").basis == "c128"
     s2 = ha.getShiftedLaplacianMultigridSolver(hp, ha.getMGparam(3, 1, 30, 1e-6, "Jac", 0.8, 2, 2, "V", "NoMUMPS"),
                                                0.5)
     assert s2.Krylov == "BiCGSTAB" and s2.inner == 5 and list(s2.shift) == [0.5] * 3
