@@ -1,7 +1,7 @@
 """GPU parity of the matrix-free stencil (hh_op_apply) against the oracle's
 assembled CSR operator (the reference's kron recipe, src/GetHelmholtz.jl:33-50).
 
-Bar: complex128 relative error <= 1e-13 (north-star bar for the stencil
+Bar: complex128 relative error <= 1e-13 (the parity bar for the stencil
 apply); complex64 <= 2e-6.  Relative error is ||y_gpu - y_ref|| / ||y_ref||.
 """
 import numpy as np

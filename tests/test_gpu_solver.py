@@ -1,7 +1,7 @@
 """GPU SL-MG/FGMRES solves against the oracle.
 
 Bars: every solve reaches true ||H x - b|| / ||b|| <= tol recomputed by the
-oracle's CSR H in float64 (north-star: 1e-6); the solution agrees with a
+oracle's CSR H in float64 (the solve tolerance, 1e-6); the solution agrees with a
 SciPy sparse direct solve (the role UMFPACK plays in test/HelmholtzTest.jl:42)
 to <= 1e-5 relative when the tolerance is tightened to 1e-9 (the error at
 tol = 1e-6 is printed per case); iteration counts equal the oracle's textbook
@@ -631,7 +631,7 @@ def test_kcycle_solve_matches_oracle(prec):
 # ---- complex64 Krylov basis (MG.basis = "c64") ----
 # Products, sums, the iterate and each restart's residual stay complex128;
 # only the stored basis is rounded to complex64.  Bars: the true residual
-# from the oracle's CSR operator (north-star 1e-6), the iteration count of
+# from the oracle's CSR operator (tolerance 1e-6), the iteration count of
 # the complex128-basis solve within 2, and at a tight tolerance the direct
 # solve (the rounding of the basis must not stall the restarts).
 
