@@ -264,6 +264,9 @@ struct hh_solver {
   hipGraphExec_t gexec[33] = {};
   int gkey_nrhs = 0;
   bool gkey_learned = false;
+  bool gkey_tr = false;
+  // the outer operator is the spread operator's transpose (adjoint solve in progress)
+  bool ho_tr = false;
   void drop_graphs() {
     for (auto& g : gexec)
       if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
@@ -854,7 +857,7 @@ void h_apply(hh_solver* S, int nrhs, const cx<TI>* x, cx<double>* y, const int* 
   hh_op* op = S->op;
   if (op->ho)
     launch_ho_stencil<TI, M_APPLY>(op->g, op->beta_lap, op->beta_mass, nrhs, x, nullptr, y, op->d64, nullptr,
-                                   mk<double>(0, 0), skip, st);
+                                   mk<double>(0, 0), skip, st, S->ho_tr ? 1 : 0);
   else
     launch_stencil<TI, double, double, M_APPLY>(op->g, nrhs, x, nullptr, y, op->d64, nullptr, mk<double>(0, 0), 0.0,
                                                 skip, st);
@@ -864,7 +867,7 @@ void h_resid(hh_solver* S, int nrhs, const cx<double>* x, const cx<double>* b, c
   hh_op* op = S->op;
   if (op->ho)
     launch_ho_stencil<double, M_RESID>(op->g, op->beta_lap, op->beta_mass, nrhs, x, b, r, op->d64, nullptr,
-                                       mk<double>(0, 0), nullptr, st);
+                                       mk<double>(0, 0), nullptr, st, S->ho_tr ? 1 : 0);
   else
     launch_stencil<double, double, double, M_RESID>(op->g, nrhs, x, b, r, op->d64, nullptr, mk<double>(0, 0), 0.0,
                                                     nullptr, st);
@@ -971,10 +974,11 @@ void run_fgmres(hh_solver* S, int nrhs, hh_solve_stats* stats) {
       // network workspace and may capture training data); later cycles replay
       // one captured graph per Arnoldi step j.
       if (S->p.graph && outer >= 1 && !S->cap_now && S->kt.kernel == KT_NONE) {
-        if (S->gkey_nrhs != nrhs || S->gkey_learned != S->use_learned) {
+        if (S->gkey_nrhs != nrhs || S->gkey_learned != S->use_learned || S->gkey_tr != S->ho_tr) {
           S->drop_graphs();
           S->gkey_nrhs = nrhs;
           S->gkey_learned = S->use_learned;
+          S->gkey_tr = S->ho_tr;
         }
         if (!S->gexec[j]) {
           hipGraph_t g = nullptr;
@@ -1414,8 +1418,6 @@ int hh_solve(hh_solver* s, const void* B, void* X, int64_t nrhs, int do_transpos
     if (!s || !B || !X) throw HHError(HH_ERR_ARG, "null argument");
     if (nrhs < 1 || nrhs > 65535) throw HHError(HH_ERR_ARG, "nrhs must be in 1..65535");
     if (do_transpose != 0 && do_transpose != 1) throw HHError(HH_ERR_ARG, "do_transpose must be 0 or 1");
-    if (do_transpose && s->op->ho)
-      throw HHError(HH_ERR_UNSUPPORTED, "adjoint solves of the spread (HO) operator are not implemented");
     if (loc != HH_HOST && loc != HH_DEVICE) throw HHError(HH_ERR_ARG, "loc must be HH_HOST or HH_DEVICE");
     hh_op* op = s->op;
     hipStream_t st = op->stream;
@@ -1432,6 +1434,7 @@ int hh_solve(hh_solver* s, const void* B, void* X, int64_t nrhs, int do_transpos
     const int nr = (int)nrhs;
     KrylovWork w = s->kw;
     w.nrhs = nr;
+    s->ho_tr = false;
     std::vector<double> bn0;
     const double tol0 = s->p.tol;
     if (do_transpose) {
@@ -1443,6 +1446,11 @@ int hh_solve(hh_solver* s, const void* B, void* X, int64_t nrhs, int do_transpos
       HIPCK(hipMemcpyAsync(bn0.data(), s->n2tmp, sizeof(double) * nr, hipMemcpyDeviceToHost, st));
       adjoint_map(op->g, s->bk, nr, 0, st);
       if (op->bc == 2) s->p.tol = tol0 / (double)(1 << op->dim);
+      // spread operator (its hierarchy has bc = 1, so W = I and the map is a
+      // conjugation): H^H x = b <=> H^T y = conj(b), x = conj(y), and
+      // ||H^T y - conj(b)|| = ||H^H x - b||.  The 5-point preconditioner is
+      // complex symmetric, so it serves H^T as it serves H.
+      s->ho_tr = op->ho != 0;
     }
     try {
       if (s->p.krylov == HH_BICGSTAB)
@@ -1451,6 +1459,7 @@ int hh_solve(hh_solver* s, const void* B, void* X, int64_t nrhs, int do_transpos
         run_fgmres(s, nr, stats);
     } catch (...) {
       s->p.tol = tol0;
+      s->ho_tr = false;
       throw;
     }
     s->p.tol = tol0;
@@ -1462,6 +1471,7 @@ int hh_solve(hh_solver* s, const void* B, void* X, int64_t nrhs, int do_transpos
       std::vector<double> rn(nr);
       HIPCK(hipMemcpyAsync(rn.data(), s->n2tmp, sizeof(double) * nr, hipMemcpyDeviceToHost, st));
       adjoint_map(op->g, s->xk, nr, 1, st);
+      s->ho_tr = false;
       HIPCK(hipStreamSynchronize(st));
       if (stats) {
         int nc = 0;

@@ -13,6 +13,11 @@
 //   av  = av3term(n, 1/2), avb = av3term(n, b_m), A = (1 - b_l) av + b_l I.
 // The separable forms follow from G' Gs with G, Gs kron products of ddxCN and
 // av3term.  One thread per node and RHS; the arithmetic is float64.
+//
+// Lap and M are symmetric (sums of kron products of symmetric tridiagonals), so
+// the transpose is H^T x = Lap x + d o (M x): with tr = 1 the mass term takes
+// the diagonal at the row's own node instead of at each neighbour.  Adjoint
+// solves use it: H^H x = b <=> H^T conj(x) = conj(b).
 #include "hh_kernels.h"
 
 namespace hh {
@@ -39,12 +44,12 @@ __device__ __forceinline__ cx<double> ld(const cx<TI>* __restrict__ x, int64_t p
   return mk<double>((double)v.re, (double)v.im);
 }
 
-// MODE: M_APPLY y = H x (+ sfac m x), M_RESID y = b - H x
+// MODE: M_APPLY y = H x (+ sfac m x), M_RESID y = b - H x; tr = 1: H^T in place of H
 template <typename TI, int MODE>
 __global__ __launch_bounds__(256) void k_ho_stencil(Grid g, double bl, double bm, const cx<TI>* __restrict__ x,
                                                     const cx<double>* __restrict__ b, cx<double>* __restrict__ y,
                                                     const cx<double>* __restrict__ d, const double* __restrict__ m,
-                                                    cx<double> sfac, const int* __restrict__ skip) {
+                                                    cx<double> sfac, const int* __restrict__ skip, int tr) {
   const int rhs = blockIdx.y;
   if (skip && skip[rhs]) return;
   const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -56,6 +61,7 @@ __global__ __launch_bounds__(256) void k_ho_stencil(Grid g, double bl, double bm
   const int64_t q = p / n1;
   const int j = (int)(q % n2), k = (int)(q / n2);
   const int64_t s1 = 1, s2 = n1, s3 = (int64_t)n1 * n2;
+  const cx<double> dp = d[p];
   cx<double> acc = mk<double>(0.0, 0.0);
   if (g.dim == 2) {
     double L1[3], L2[3], A1[3], A2[3], V1[3], V2[3];
@@ -81,7 +87,7 @@ __global__ __launch_bounds__(256) void k_ho_stencil(Grid g, double bl, double bm
         if (ai == 0) cm += 0.5 * V2[bj + 1];
         if (bj == 0) cm += 0.5 * V1[ai + 1];
         acc = acc + c * xv;
-        if (cm != 0.0) acc = acc + cm * (d[pq] * xv);
+        if (cm != 0.0) acc = acc + cm * ((tr ? dp : d[pq]) * xv);
       }
     }
   } else {
@@ -116,7 +122,7 @@ __global__ __launch_bounds__(256) void k_ho_stencil(Grid g, double bl, double bm
       for (int u = -1; u <= 1; ++u) {
         if ((u < 0 && ii[ax] == 0) || (u > 0 && ii[ax] == nn[ax] - 1)) continue;
         const int64_t pu = p + u * st[ax];
-        acc = acc + ((1.0 / 3.0) * VB[ax][u + 1]) * (d[pu] * ld(xr, pu));
+        acc = acc + ((1.0 / 3.0) * VB[ax][u + 1]) * ((tr ? dp : d[pu]) * ld(xr, pu));
       }
     }
   }
@@ -130,18 +136,18 @@ __global__ __launch_bounds__(256) void k_ho_stencil(Grid g, double bl, double bm
 template <typename TI, int MODE>
 void launch_ho_stencil(const Grid& g, double bl, double bm, int nrhs, const cx<TI>* x, const cx<double>* b,
                        cx<double>* y, const cx<double>* d, const double* m, cx<double> sfac, const int* skip,
-                       hipStream_t s) {
+                       hipStream_t s, int tr) {
   dim3 grid((unsigned)((g.N + 255) / 256), nrhs);
-  hipLaunchKernelGGL((k_ho_stencil<TI, MODE>), grid, dim3(256), 0, s, g, bl, bm, x, b, y, d, m, sfac, skip);
+  hipLaunchKernelGGL((k_ho_stencil<TI, MODE>), grid, dim3(256), 0, s, g, bl, bm, x, b, y, d, m, sfac, skip, tr);
 }
 template void launch_ho_stencil<double, M_APPLY>(const Grid&, double, double, int, const cx<double>*,
                                                  const cx<double>*, cx<double>*, const cx<double>*, const double*,
-                                                 cx<double>, const int*, hipStream_t);
+                                                 cx<double>, const int*, hipStream_t, int);
 template void launch_ho_stencil<float, M_APPLY>(const Grid&, double, double, int, const cx<float>*,
                                                 const cx<double>*, cx<double>*, const cx<double>*, const double*,
-                                                cx<double>, const int*, hipStream_t);
+                                                cx<double>, const int*, hipStream_t, int);
 template void launch_ho_stencil<double, M_RESID>(const Grid&, double, double, int, const cx<double>*,
                                                  const cx<double>*, cx<double>*, const cx<double>*, const double*,
-                                                 cx<double>, const int*, hipStream_t);
+                                                 cx<double>, const int*, hipStream_t, int);
 
 }  // namespace hh

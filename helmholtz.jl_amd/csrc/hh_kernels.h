@@ -49,10 +49,11 @@ void launch_dense_apply_mfma(const float* P, int64_t n, int nrhs, const cx<float
                              hipStream_t s);
 
 // ---- spread (GetHelmholtzOperatorHO) operator (hh_ho.hip): M_APPLY y = H x (+ sfac m x), M_RESID y = b - H x ----
+// tr = 1 applies the transpose H^T = Lap + D M (Lap and M are symmetric) in place of H = Lap + M D.
 template <typename TI, int MODE>
 void launch_ho_stencil(const Grid& g, double beta_lap, double beta_mass, int nrhs, const cx<TI>* x,
                        const cx<double>* b, cx<double>* y, const cx<double>* d, const double* m, cx<double> sfac,
-                       const int* skip, hipStream_t s);
+                       const int* skip, hipStream_t s, int tr = 0);
 
 // ---- K-cycle coarse correction (hh_kcycle.hip); part: 64*2*nrhs, sc: 4*nrhs ----
 template <typename T>

@@ -123,7 +123,8 @@ int hh_op_create(int dim, const int64_t nnodes[3], const double h[3], const doub
  * beta = [beta_lap; beta_mass].  Applies in complex128 (hh_op_apply); as the
  * operator of hh_solve it is the outer operator and the preconditioner's
  * hierarchy uses the natural-Neumann (BC = 1) 5/7-point stencil, which is
- * this operator at beta = 1.  Adjoint solves are not implemented for it. */
+ * this operator at beta = 1.  Adjoint solves (hh_solve, do_transpose = 1)
+ * apply its transpose Lap + diag(mass) M. */
 int hh_op_create_ho(int dim, const int64_t nnodes[3], const double h[3], const double* m, const double* gamma,
                     double omega_re, double omega_im, int neumann_top, int sommerfeld, double beta_lap,
                     double beta_mass, hh_op** out);
@@ -158,8 +159,10 @@ int hh_solver_destroy(hh_solver* s);
  * right-preconditioned BiCGSTAB instead (:91-93; two cycles per iteration, at
  * most max_outer iterations).  do_transpose = 1 solves the adjoint system
  * H^H X = B (:68-80), through the mirror-ghost similarity
- * H^H = W conj(H) W^-1 (W: nodal trapezoid weights) with the same hierarchy;
- * stats then report the adjoint system's true residuals.
+ * H^H = W conj(H) W^-1 (W: nodal trapezoid weights) with the same hierarchy
+ * (for the spread operator of hh_op_create_ho: H^T conj(X) = conj(B), its
+ * transpose applied by the same kernel); stats then report the adjoint
+ * system's true residuals.
  * stats may be NULL. */
 int hh_solve(hh_solver* s, const void* B, void* X, int64_t nrhs, int do_transpose, int loc, hh_solve_stats* stats);
 

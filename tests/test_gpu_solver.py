@@ -602,8 +602,43 @@ def test_ho_operator_solve(krylov, prec):
         print(f"HO {krylov} {prec}: iters {s.last_stats['iters'][c]} res {r:.2e} err vs direct {err:.2e}")
         assert r <= 1e-6
         assert err <= 1e-4
-    with pytest.raises(ha.HelmholtzError):
-        ha.solveLinearSystem(Hg, B, s, 1)
+    # adjoint solve (doTranspose = 1): H^H x = b through H^T conj(x) = conj(b),
+    # the spread operator's transpose applied by the same kernel
+    HH = H.conj().T.tocsc()
+    Xa, s = ha.solveLinearSystem(Hg, B, s, 1)
+    xa = spla.splu(HH).solve(B)
+    for c in range(2):
+        r = np.linalg.norm(HH @ Xa[:, c] - B[:, c]) / np.linalg.norm(B[:, c])
+        err = np.linalg.norm(Xa[:, c] - xa[:, c]) / np.linalg.norm(xa[:, c])
+        print(f"HO adjoint {krylov} {prec}: iters {s.last_stats['iters'][c]} res {r:.2e} err vs direct {err:.2e} "
+              f"reported {s.last_stats['true_res'][c]:.2e}")
+        assert r <= 1e-6
+        assert err <= 1e-4
+        assert abs(s.last_stats["true_res"][c] - r) <= 1e-3 * r + 1e-14
+    # a forward solve after the adjoint one applies H again
+    X2, s = ha.solveLinearSystem(Hg, B, s)
+    assert np.linalg.norm(H @ X2[:, 0] - B[:, 0]) / np.linalg.norm(B[:, 0]) <= 1e-6
+
+
+def test_ho_adjoint_solve_3d():
+    """Adjoint solve of the 3-D spread operator (19-point Laplacian, averaged
+    mass), checked with the oracle's CSR H^H and a direct solve."""
+    M, m, g, w = small_case((17, 17, 9), seed=39, neumann=True)
+    beta = (0.5, 0.25)
+    H = hr.get_helmholtz_operator_ho(M, m, w, g, True, True, beta)
+    HH = H.conj().T.tocsc()
+    mesh = ha.getRegularMesh(M.domain, M.n)
+    Hg = ha.GetHelmholtzOperatorHO(mesh, m, w, g, True, True, beta)
+    rng = np.random.default_rng(15)
+    b = rng.standard_normal(m.size) + 1j * rng.standard_normal(m.size)
+    s = _solver(M, m, g, w, True, 2, 0.5, cycle="V")
+    x, s = ha.solveLinearSystem(Hg, b, s, 1)
+    r = np.linalg.norm(HH @ x - b) / np.linalg.norm(b)
+    xd = spla.splu(HH).solve(b)
+    err = np.linalg.norm(x - xd) / np.linalg.norm(xd)
+    print(f"HO adjoint 3-D: res {r:.2e} err vs direct {err:.2e}")
+    assert r <= 1e-6
+    assert err <= 1e-4
 
 
 @pytest.mark.parametrize("prec", ["c128", "c64"])

@@ -399,6 +399,12 @@ def test_ho_laplacian_and_mass_invariants(nodes, beta):
     mass = hr.helmholtz_mass(M, m, w, g, True, True)
     x = np.random.default_rng(1).standard_normal(N) + 0j
     assert np.allclose(H @ x, Lap @ x + Mm @ (mass * x), rtol=1e-13, atol=0)
+    # the averaging mass matrix is symmetric, so the transpose the adjoint solve
+    # applies is H^T = Lap + diag(mass) M; H itself is not symmetric (the
+    # diagonal varies with the medium), so the two orders differ
+    assert abs(Mm - Mm.T).max() <= 1e-15
+    assert np.allclose(H.T @ x, Lap @ x + mass * (Mm @ x), rtol=1e-13, atol=0)
+    assert abs(H - H.T).max() > 1e-6 * abs(H).max()
 
 
 def test_kcycle_correction_is_minimal_residual():
