@@ -27,3 +27,34 @@ def small_case(nodes, seed=0, dim=None, neumann=True, sommerfeld=True, order=2, 
     g0 = 0.01 * abs(complex(w)) * np.ones(int(np.prod(nodes)))
     abl = hr.get_abl(nodes, neumann, pad, abs(complex(w)).real).ravel(order="F")
     return M, m, g0 + abl, w
+
+
+def attenuation_case(nn=129):
+    """The flow of test/AttenuationTest.jl at 129x129: unit slowness on
+    [0, 10]^2, 20 points per wavelength, a centred point source 1/h^2, no
+    Sommerfeld, no Neumann top; attenuation alpha = 0.2 w either added to gamma
+    at real w (s1) or carried by the complex frequency w - i alpha/2 (s2)."""
+    n = (nn, nn)
+    M = hr.get_regular_mesh([0.0, 10.0, 0.0, 10.0], [nn - 1, nn - 1])
+    m = np.ones(nn * nn)
+    wmax = hr.get_maximal_frequency(m, M)
+    w = wmax / 2.0
+    alpha = 0.2 * w
+    pad = nn * 35 // 512 + 1
+    gam = hr.get_abl(n, False, [pad, pad], wmax).ravel(order="F")
+    src = [nn // 2, nn // 2]
+    q = np.zeros(nn * nn, complex)
+    q[hr.loc2cs(n, src)] = 1.0 / M.h[0] ** 2
+    return M, m, w, alpha, gam, pad, src, q
+
+
+def attenuation_decay_slope(s, nn, src, pad, h):
+    """Slope of log(|s| sqrt(r)) along the source row between the near field
+    and the absorbing layer: the 2-D Green's function decays as
+    r^(-1/2) exp(-Im(kappa) r)."""
+    S = np.asarray(s).reshape((nn, nn), order="F")
+    rs = np.arange(5, nn // 2 - pad - 2)
+    r = rs * h
+    a = np.abs(S[src[0] + rs, src[1]])
+    return np.polyfit(r, np.log(a * np.sqrt(r)), 1)[0]
+

@@ -12,7 +12,7 @@ import pytest
 import scipy.sparse.linalg as spla
 
 from oracle import helmholtz_ref as hr, slmg_ref as sr
-from problems import small_case
+from problems import attenuation_case, attenuation_decay_slope, small_case
 
 pytestmark = pytest.mark.gpu
 
@@ -733,3 +733,40 @@ def test_c64_basis_rejected_where_unsupported():
     ha.time_kernel(s, "dots", 0)
     x, s = ha.solveLinearSystem(None, q, s)
     assert ha.kernel_time(s)["launches"] == s.nPrec
+
+
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+def test_attenuation_real_and_complex_frequency(prec):
+    """The flow of test/AttenuationTest.jl solved on the GPU: attenuation alpha
+    added to gamma at real w (s1) and the complex frequency w - i alpha/2 (s2),
+    no Sommerfeld, no Neumann top.  Each solve reaches the true residual of the
+    oracle's CSR operator and agrees with its direct solve; the two fields
+    nearly agree and both decay at Im sqrt(w^2 + i w alpha) along the source
+    row (the bars of the oracle's own test of this flow)."""
+    M, m, w, alpha, gam, pad, src, q = attenuation_case()
+    mesh = ha.getRegularMesh(M.domain, M.n)
+    cases = ((w, gam + alpha), (complex(w, -alpha / 2), gam))
+    sol = []
+    for omega, g in cases:
+        print(f"attenuation case omega={omega} prec={prec}", flush=True)
+        H = hr.get_helmholtz_operator7(M, m, omega, g, False, False)
+        hp = ha.HelmholtzParam(mesh, g, m, omega, False, False)
+        mg = ha.getMGparam(complex, int, 3, 1, 100, 1e-6, "Jac", 0.8, 2, 2, "V", "NoMUMPS", 0.5, 0.0)
+        mg.prec = prec
+        s = ha.getShiftedLaplacianMultigridSolver(hp, mg, 0.5, "GMRES", 10)
+        x, s = ha.solveLinearSystem(None, q, s)
+        r = np.linalg.norm(H @ x - q) / np.linalg.norm(q)
+        xd = spla.splu(H.tocsc()).solve(q)
+        err = np.linalg.norm(x - xd) / np.linalg.norm(xd)
+        print(f"  iters {s.last_stats['iters'][0]} res {r:.2e} err vs direct {err:.2e}")
+        assert r <= 1e-6
+        assert err <= 1e-4
+        sol.append(x)
+    nn = int(M.n[0]) + 1
+    rel = np.linalg.norm(sol[0] - sol[1]) / np.linalg.norm(sol[0])
+    kappa_im = np.sqrt(w**2 + 1j * w * alpha).imag
+    for x in sol:
+        d = attenuation_decay_slope(x, nn, src, pad, M.h[0])
+        assert abs(-d - kappa_im) <= 0.05 * kappa_im
+    assert rel <= 0.05
+

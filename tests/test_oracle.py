@@ -17,7 +17,7 @@ import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 
 from oracle import helmholtz_ref as hr, slmg_ref as sr
-from tests.problems import small_case
+from tests.problems import attenuation_case, attenuation_decay_slope, small_case
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
@@ -527,3 +527,30 @@ def test_kcycle_with_jac_gmres_converges():
     assert np.linalg.norm(H @ x - b) <= 1e-6 * np.linalg.norm(b)
     xj, itsj, _ = sr.fgmres(lambda v: H @ v, b, lambda v: sr.vcycle(hier, 0, v, 0.8, 2, 2, "K", 0, "Jac"), 5, 60, 1e-6)
     print(f"K-cycle FGMRES(5): Jac-GMRES {its} iterations, Jac {itsj}")
+
+
+def test_attenuation_real_and_complex_frequency_agree():
+    """test/AttenuationTest.jl: attenuation alpha added to gamma at real w and
+    the complex frequency w - i alpha/2 give nearly the same field (they differ
+    by the alpha^2/4 and alpha*gamma/2 terms of the mass, O((alpha/2w)^2)), and
+    both decay at Im(kappa) = Im sqrt(w^2 + i w alpha) ~ alpha/2 (unit
+    slowness).  A dropped attenuation term, a wrong sign of Im(w^2) or Re(w)
+    in place of w in the mass each fail one of these."""
+    M, m, w, alpha, gam, pad, src, q = attenuation_case()
+    H1 = hr.get_helmholtz_operator7(M, m, w, gam + alpha, False, False)
+    H2 = hr.get_helmholtz_operator7(M, m, w - 1j * alpha / 2, gam, False, False)
+    s1 = spla.splu(H1.tocsc()).solve(q)
+    s2 = spla.splu(H2.tocsc()).solve(q)
+    rel = np.linalg.norm(s1 - s2) / np.linalg.norm(s1)
+    kappa_im = np.sqrt(w**2 + 1j * w * alpha).imag
+    nn = int(M.n[0]) + 1
+    d1 = attenuation_decay_slope(s1, nn, src, pad, M.h[0])
+    d2 = attenuation_decay_slope(s2, nn, src, pad, M.h[0])
+    print(f"attenuation: rel diff {rel:.3e}; decay {d1:.4f} / {d2:.4f}, Im kappa {kappa_im:.4f}")
+    assert rel <= 0.05
+    assert abs(-d1 - kappa_im) <= 0.05 * kappa_im
+    assert abs(-d2 - kappa_im) <= 0.05 * kappa_im
+    # without the attenuation the field does not decay at that rate
+    H0 = hr.get_helmholtz_operator7(M, m, w, gam, False, False)
+    d0 = attenuation_decay_slope(spla.splu(H0.tocsc()).solve(q), nn, src, pad, M.h[0])
+    assert -d0 < 0.25 * kappa_im
