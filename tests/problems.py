@@ -58,3 +58,31 @@ def attenuation_decay_slope(s, nn, src, pad, h):
     a = np.abs(S[src[0] + rs, src[1]])
     return np.polyfit(r, np.log(a * np.sqrt(r)), 1)[0]
 
+
+
+def seg_salt_case():
+    """The medium of the commented-out line of test/ShiftedLaplacianTest.jl:14
+    (examples/SEGmodel2Dsalt.dat, committed as tests/golden/seg_salt_2d.npz)
+    on that test's set-up: velocity in km/s transposed to x-fastest nodal
+    order (256 x 128), extended by one node on the far x and bottom edges to
+    the test's own 257 x 129 grid (even cell counts, so it coarsens) on
+    [0, 13.5] x [0, 4.2]; f = 2.5; gamma = 0.01 w plus an absorbing layer of 16
+    cells with amplitude getMaximalFrequency; Neumann top, Sommerfeld; the
+    test's source (div(n, 2), 1) scaled by 1/h1^2.
+    Returns (mesh, m, gamma, w, H, q)."""
+    import hashlib
+    import os
+
+    f = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "seg_salt_2d.npz"))
+    v = f["v"].astype(np.float64)
+    assert hashlib.sha256(v.tobytes()).hexdigest() == str(f["sha256"])
+    v = np.pad(v.T * 1e-3, ((0, 1), (0, 1)), mode="edge")
+    n = v.shape
+    M = hr.get_regular_mesh([0.0, 13.5, 0.0, 4.2], [n[0] - 1, n[1] - 1])
+    m = (1.0 / v**2).ravel(order="F")
+    w = 2 * np.pi * 2.5
+    H, g = hr.get_helmholtz_operator(M, m, w, 0.01 * w * np.ones(m.size), True, [16, 16],
+                                     hr.get_maximal_frequency(m, M), True)
+    q = np.zeros(m.size, complex)
+    q[hr.loc2cs(n, [n[0] // 2, 1])] = 1.0 / M.h[0] ** 2
+    return M, m, g.ravel(order="F"), w, H, q

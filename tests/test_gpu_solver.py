@@ -12,7 +12,7 @@ import pytest
 import scipy.sparse.linalg as spla
 
 from oracle import helmholtz_ref as hr, slmg_ref as sr
-from problems import attenuation_case, attenuation_decay_slope, small_case
+from problems import attenuation_case, attenuation_decay_slope, seg_salt_case, small_case
 
 pytestmark = pytest.mark.gpu
 
@@ -769,4 +769,40 @@ def test_attenuation_real_and_complex_frequency(prec):
         d = attenuation_decay_slope(x, nn, src, pad, M.h[0])
         assert abs(-d - kappa_im) <= 0.05 * kappa_im
     assert rel <= 0.05
+
+
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+def test_seg_salt_model_solve(prec):
+    """The SEG salt slice (examples/SEGmodel2Dsalt.dat) on the set-up of
+    test/ShiftedLaplacianTest.jl (257 x 129, f = 2.5, W(2,2) Jacobi 0.75,
+    FGMRES(20)), with 3 levels and shift 0.5 (the 2-level hierarchy of the
+    reference test leaves a 129 x 65 coarsest grid, above the dense coarsest
+    solve's 4096 nodes).  The test's source and a second surface source: true
+    residual by the oracle's CSR, agreement with the direct solve, and the
+    oracle cycle's iteration counts (+-1 in complex128, +-2 in complex64)."""
+    M, m, g, w, H, q = seg_salt_case()
+    n = (257, 129)
+    q2 = np.zeros_like(q)
+    q2[hr.loc2cs(n, [64, 1])] = q.max()
+    B = np.asfortranarray(np.stack([q, q2], axis=1))
+    mesh = ha.getRegularMesh(M.domain, M.n)
+    hp = ha.HelmholtzParam(mesh, g, m, w, True, True)
+    mg = ha.getMGparam(complex, int, 3, 1, 30, 1e-6, "Jac", 0.75, 2, 2, "W", "NoMUMPS", 0.5, 0.0)
+    mg.prec = prec
+    s = ha.getShiftedLaplacianMultigridSolver(hp, mg, 0.5, "GMRES", 20)
+    X, s = ha.solveLinearSystem(None, B, s)
+    p = hr.HelmholtzParam(M, g, m, w, True, True)
+    hier = sr.build_hierarchy(p, 3, 0.5)
+    lu = spla.splu(H.tocsc())
+    for c in range(2):
+        res = np.linalg.norm(H @ X[:, c] - B[:, c]) / np.linalg.norm(B[:, c])
+        xd = lu.solve(B[:, c])
+        err = np.linalg.norm(X[:, c] - xd) / np.linalg.norm(xd)
+        _, ito, _ = sr.fgmres(lambda u: H @ u, B[:, c], lambda u: sr.vcycle(hier, 0, u, 0.75, 2, 2, "W"),
+                              20, 30, 1e-6)
+        it = int(s.last_stats["iters"][c])
+        print(f"SEG salt {prec} rhs {c}: gpu {it} oracle {ito} res {res:.2e} err vs direct {err:.2e}")
+        assert res <= 1e-6
+        assert err <= 1e-4
+        assert abs(it - ito) <= (1 if prec == "c128" else 2)
 

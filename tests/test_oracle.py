@@ -17,7 +17,7 @@ import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 
 from oracle import helmholtz_ref as hr, slmg_ref as sr
-from tests.problems import attenuation_case, attenuation_decay_slope, small_case
+from tests.problems import attenuation_case, attenuation_decay_slope, seg_salt_case, small_case
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
@@ -554,3 +554,22 @@ def test_attenuation_real_and_complex_frequency_agree():
     H0 = hr.get_helmholtz_operator7(M, m, w, gam, False, False)
     d0 = attenuation_decay_slope(spla.splu(H0.tocsc()).solve(q), nn, src, pad, M.h[0])
     assert -d0 < 0.25 * kappa_im
+
+
+def test_seg_salt_fixture_and_reference_setup_converges():
+    """The SEG salt slice (examples/SEGmodel2Dsalt.dat) loads with its
+    checksum, water on top (1.5 km/s along the first depth row) and the salt
+    body's 4.48 km/s inside; with the reference test's own solver settings
+    (test/ShiftedLaplacianTest.jl:22-66: 2 levels, shift 0.02, W(2,2) Jacobi
+    0.75, FGMRES(20)) the oracle's SL-MG reaches 1e-6 and the direct solve."""
+    M, m, g, w, H, q = seg_salt_case()
+    v = 1.0 / np.sqrt(m.reshape((257, 129), order="F"))
+    assert np.allclose(v[:, 0], 1.5) and abs(v.max() - 4.48) < 1e-12 and abs(v.min() - 1.5) < 1e-12
+    p = hr.HelmholtzParam(M, g, m, w, True, True)
+    hier = sr.build_hierarchy(p, 2, 0.02)
+    x, it, _ = sr.fgmres(lambda u: H @ u, q, lambda u: sr.vcycle(hier, 0, u, 0.75, 2, 2, "W"), 20, 30, 1e-6)
+    res = np.linalg.norm(H @ x - q) / np.linalg.norm(q)
+    xd = spla.splu(H.tocsc()).solve(q)
+    err = np.linalg.norm(x - xd) / np.linalg.norm(xd)
+    print(f"SEG salt, reference settings: {it} iterations, res {res:.2e}, err vs direct {err:.2e}")
+    assert res <= 1e-6 and it <= 30 and err <= 1e-4
