@@ -333,6 +333,8 @@ namespace {
 
 size_t csz(int prec) { return prec == HH_C64 ? sizeof(cx<float>) : sizeof(cx<double>); }
 
+constexpr int64_t kMaxCoarsest = 10000;  // nodes of the dense coarsest solve
+
 void build_hierarchy(hh_solver* S) {
   hh_op* op = S->op;
   hipStream_t st = op->stream;
@@ -382,10 +384,14 @@ void build_hierarchy(hh_solver* S) {
       }
     }
   }
-  // coarsest level: dense complex128 inverse of the shifted operator
+  // coarsest level: dense complex128 inverse of the shifted operator (Gauss-Jordan,
+  // O(n^3) set-up and 32 n^2 bytes; 10^4 nodes admit the 129 x 65 coarsest grid of a
+  // 2-level hierarchy on 257 x 129)
   Level& c = S->lv[L - 1];
   const int64_t n = c.g.N;
-  if (n > 4096) throw HHError(HH_ERR_ARG, "coarsest grid has " + std::to_string(n) + " nodes (> 4096); add levels");
+  if (n > kMaxCoarsest)
+    throw HHError(HH_ERR_ARG, "coarsest grid has " + std::to_string(n) + " nodes (> " + std::to_string(kMaxCoarsest) +
+                                  "); add levels");
   S->coarse_d = dalloc<cx<double>>(n);
   launch_diag<double>(c.g, c.m, c.gam, op->wre, op->wim, op->neumann_top, op->sommerfeld, c.h, S->coarse_d, st);
   S->coarse_aug = dalloc<cx<double>>((size_t)n * 2 * n);

@@ -48,7 +48,9 @@ typedef struct hh_dataset hh_dataset;
  * that the reference sets through getMGparam (test/ShiftedLaplacianTest.jl:51-64)
  * plus the solver's Krylov/inner (src/ShiftedLaplacianMultigridSolver.jl:8-9). */
 typedef struct hh_mg_params {
-  int32_t levels;     /* MG.levels (>= 1; every coarsening halves the cells)     */
+  int32_t levels;     /* MG.levels (>= 1; every coarsening halves the cells; the
+                         coarsest grid, solved by a dense inverse, has at most
+                         10000 nodes)                                            */
   int32_t pre;        /* MG.relaxPre: damped-Jacobi sweeps (or Jac-GMRES steps,
                          at most 8) before the correction                      */
   int32_t post;       /* MG.relaxPost                                            */

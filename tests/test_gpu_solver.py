@@ -771,15 +771,16 @@ def test_attenuation_real_and_complex_frequency(prec):
     assert rel <= 0.05
 
 
+@pytest.mark.parametrize("levels,shift", [(2, 0.02), (3, 0.5)])
 @pytest.mark.parametrize("prec", ["c128", "c64"])
-def test_seg_salt_model_solve(prec):
+def test_seg_salt_model_solve(prec, levels, shift):
     """The SEG salt slice (examples/SEGmodel2Dsalt.dat) on the set-up of
     test/ShiftedLaplacianTest.jl (257 x 129, f = 2.5, W(2,2) Jacobi 0.75,
-    FGMRES(20)), with 3 levels and shift 0.5 (the 2-level hierarchy of the
-    reference test leaves a 129 x 65 coarsest grid, above the dense coarsest
-    solve's 4096 nodes).  The test's source and a second surface source: true
-    residual by the oracle's CSR, agreement with the direct solve, and the
-    oracle cycle's iteration counts (+-1 in complex128, +-2 in complex64)."""
+    FGMRES(20)): the test's own 2 levels and shift 0.02 (a 129 x 65 = 8385-node
+    dense coarsest solve), and 3 levels with shift 0.5.  The test's source and
+    a second surface source: true residual by the oracle's CSR, agreement with
+    the direct solve, and the oracle cycle's iteration counts (+-1 in
+    complex128, +-2 in complex64)."""
     M, m, g, w, H, q = seg_salt_case()
     n = (257, 129)
     q2 = np.zeros_like(q)
@@ -787,12 +788,14 @@ def test_seg_salt_model_solve(prec):
     B = np.asfortranarray(np.stack([q, q2], axis=1))
     mesh = ha.getRegularMesh(M.domain, M.n)
     hp = ha.HelmholtzParam(mesh, g, m, w, True, True)
-    mg = ha.getMGparam(complex, int, 3, 1, 30, 1e-6, "Jac", 0.75, 2, 2, "W", "NoMUMPS", 0.5, 0.0)
+    mg = ha.getMGparam(complex, int, levels, 1, 30, 1e-6, "Jac", 0.75, 2, 2, "W", "NoMUMPS", 0.5, 0.0)
     mg.prec = prec
-    s = ha.getShiftedLaplacianMultigridSolver(hp, mg, 0.5, "GMRES", 20)
+    s = ha.getShiftedLaplacianMultigridSolver(hp, mg, shift, "GMRES", 20)
+    print(f"SEG salt {prec} levels {levels} shift {shift}", flush=True)
     X, s = ha.solveLinearSystem(None, B, s)
+    print(f"  set-up {s.setupTime:.2f} s, solve {s.solveTime:.2f} s", flush=True)
     p = hr.HelmholtzParam(M, g, m, w, True, True)
-    hier = sr.build_hierarchy(p, 3, 0.5)
+    hier = sr.build_hierarchy(p, levels, shift)
     lu = spla.splu(H.tocsc())
     for c in range(2):
         res = np.linalg.norm(H @ X[:, c] - B[:, c]) / np.linalg.norm(B[:, c])
