@@ -87,8 +87,9 @@ def c3_problem():
 
 def c3_lines(log, c64_basis=True):
     """Complex64 vs complex128 preconditioner on the 3-D case: 5-level W(2,2)
-    cycle (the coarsest level must have at most 4096 nodes for the dense
-    solve, so 4 levels do not fit), complex128 FGMRES(8) to 1e-6; with
+    cycle (a 4-level V cycle does not converge at 8 ppw; 4-level W saves about
+    an eighth of the iterations but adds eight dense 8125-node coarsest
+    solves per cycle), complex128 FGMRES(8) to 1e-6; with
     ``c64_basis`` also the complex64 preconditioner with the FGMRES basis
     stored in complex64 (``c64_c64basis_*``)."""
     import helmholtz_amd as ha
