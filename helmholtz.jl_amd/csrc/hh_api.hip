@@ -109,6 +109,33 @@ __global__ void k_abl(int dim, int n1, int n2, int n3, const double* p1, const d
   }
 }
 
+// Sommerfeld boundary terms getSommerfeldBC (src/GetHelmholtz.jl:222-247):
+// S = -i w (bc / h_d) sqrt(m) summed over the faces a node lies on, in the
+// reference's face order (last axis first, its low face skipped with a Neumann
+// top), each term formed as (w (bc / h_d)) sqrt(m).
+__global__ void k_sommerfeld(int dim, int n1, int n2, int n3, double h1, double h2, double h3,
+                             const double* __restrict__ m, double w, int neumann_top, double bc,
+                             cx<double>* __restrict__ out) {
+  const int64_t N = (int64_t)n1 * n2 * n3;
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= N) return;
+  const int i = (int)(p % n1);
+  const int64_t q = p / n1;
+  const int j = (int)(q % n2);
+  const int k = (int)(q / n2);
+  const double sm = sqrt(m[p]);
+  const int ii[3] = {i, j, k}, nn[3] = {n1, n2, n3};
+  const double hh[3] = {h1, h2, h3};
+  double s = 0.0;
+  for (int d = dim - 1; d >= 0; --d) {
+    const double t = (w * (bc / hh[d])) * sm;
+    const bool top = d == dim - 1;
+    if (ii[d] == 0 && !(top && neumann_top)) s += t;
+    if (ii[d] == nn[d] - 1) s += t;
+  }
+  out[p] = mk<double>(0.0, -s);
+}
+
 __global__ void k_sqrt_arr(const double* a, double* b, int n) {
   int i = blockIdx.x * 64 + threadIdx.x;
   if (i < n) b[i] = sqrt(a[i]);
@@ -1499,6 +1526,55 @@ int hh_solve(hh_solver* s, const void* B, void* X, int64_t nrhs, int do_transpos
     double t2 = now_s();
     s->solve_time += t2 - t1;
     if (stats) stats->solve_time = t2 - t1;
+  });
+}
+
+int hh_sommerfeld_bc(int dim, const int64_t nnodes[3], const double h[3], const double* m, double omega,
+                     int neumann_top, int bc_order, void* out, int loc) {
+  return guarded([&] {
+    if (!nnodes || !h || !m || !out) throw HHError(HH_ERR_ARG, "null argument");
+    if (dim != 2 && dim != 3) throw HHError(HH_ERR_ARG, "dim must be 2 or 3");
+    if (bc_order != 1 && bc_order != 2) throw HHError(HH_ERR_ARG, "bc_order must be 1 or 2");
+    if (loc != HH_HOST && loc != HH_DEVICE) throw HHError(HH_ERR_ARG, "loc must be HH_HOST or HH_DEVICE");
+    int64_t nn[3] = {1, 1, 1};
+    double hv[3] = {1.0, 1.0, 1.0};
+    int64_t N = 1;
+    for (int d = 0; d < dim; ++d) {
+      if (nnodes[d] < 3 || nnodes[d] > (1 << 24)) throw HHError(HH_ERR_ARG, "each axis needs 3..2^24 nodes");
+      if (!(h[d] > 0.0)) throw HHError(HH_ERR_ARG, "h must be positive");
+      nn[d] = nnodes[d];
+      hv[d] = h[d];
+      N *= nn[d];
+    }
+    if (N > ((int64_t)1 << 31)) throw HHError(HH_ERR_ARG, "grid too large");
+    const double* dm = m;
+    cx<double>* dout = (cx<double>*)out;
+    if (loc == HH_HOST) {
+      double* t = dalloc<double>(N);
+      dm = t;
+      try {
+        dout = dalloc<cx<double>>(N);
+      } catch (...) {
+        dfree(t);
+        throw;
+      }
+      hipError_t e = hipMemcpy(t, m, sizeof(double) * N, hipMemcpyHostToDevice);
+      if (e != hipSuccess) {
+        dfree(t);
+        dfree(dout);
+        hipck(e, "hh_sommerfeld_bc");
+      }
+    }
+    hipLaunchKernelGGL(k_sommerfeld, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, nullptr, dim, (int)nn[0],
+                       (int)nn[1], (int)nn[2], hv[0], hv[1], hv[2], dm, omega, neumann_top, (double)bc_order, dout);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess && loc == HH_HOST) e = hipMemcpy(out, dout, sizeof(cx<double>) * N, hipMemcpyDeviceToHost);
+    if (loc == HH_HOST) {
+      dfree((double*)dm);
+      dfree(dout);
+    }
+    hipck(e, "hh_sommerfeld_bc");
   });
 }
 

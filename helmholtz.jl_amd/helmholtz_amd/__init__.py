@@ -9,7 +9,7 @@ from ._lib import HelmholtzError, LIB_PATH, lib  # noqa: F401
 from .helmholtz import (  # noqa: F401
     GetHelmholtzOperator, GetHelmholtzOperatorHO, GetHelmholtzShiftOP, HelmholtzOperator, HelmholtzParam, RegularMesh,
     getABL, getAcousticPointSource, getMaximalFrequency, getMidPointSrc, getRegularMesh,
-    getShiftedHelmholtzParam, getTopPointSrc, loc2cs, point_sources,
+    getShiftedHelmholtzParam, getSommerfeldBC, getTopPointSrc, loc2cs, point_sources,
 )
 from .solver import (  # noqa: F401
     MGparam, ShiftedLaplacianMultigridSolver, clear, copySolver, getMGparam,

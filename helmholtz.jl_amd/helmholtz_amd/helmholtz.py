@@ -107,6 +107,24 @@ def getABL(n, NeumannAtFirstDim: bool, ABLpad, ABLamp: float):
     return out.reshape(tuple(n), order="F")
 
 
+def getSommerfeldBC(Msh: RegularMesh, mNodal, omega: float, NeumannOnTop: bool, orderNeumannBC: int = 2):
+    """src/GetHelmholtz.jl:222-247 computed on the GPU (``hh_sommerfeld_bc``):
+    -i omega (BC / h_d) sqrt(m) on every boundary face (the top face skipped
+    with NeumannOnTop), summed at edges and corners.  ``omega`` is real, as
+    the reference's ``omega::Float64``.  Returns a complex128 array shaped
+    like the nodal grid, in Fortran order."""
+    if isinstance(omega, complex) or np.iscomplexobj(omega):
+        raise TypeError("getSommerfeldBC takes a real omega")
+    nodes = [int(v) for v in Msh.nodes]
+    m = np.ascontiguousarray(np.asarray(mNodal, dtype=np.float64).ravel(order="F"))
+    if m.size != int(np.prod(nodes)):
+        raise ValueError("mNodal must have one value per node")
+    out = np.zeros(m.size, dtype=np.complex128)
+    L.check(L.lib().hh_sommerfeld_bc(len(nodes), _i64x3(nodes), _f64x3(Msh.h), m.ctypes.data, float(omega),
+                                     int(bool(NeumannOnTop)), int(orderNeumannBC), out.ctypes.data, L.HH_HOST))
+    return out.reshape(tuple(nodes), order="F")
+
+
 def getMaximalFrequency(m, M: RegularMesh) -> float:
     """src/GetHelmholtz.jl:75-79 (slowness squared in): 10 points per wavelength."""
     return (0.1 * 2 * math.pi) / (float(np.max(M.h)) * math.sqrt(float(np.max(m))))

@@ -107,6 +107,15 @@ int hh_version(void);
 int hh_abl(int dim, const int64_t nnodes[3], int neumann_at_first_dim, const int64_t pad[3], double amp,
            double* out, int loc);
 
+/* Sommerfeld boundary terms getSommerfeldBC(Msh, mNodal, omega, NeumannOnTop,
+ * orderNeumannBC) (src/GetHelmholtz.jl:222-247): -i omega (BC / h_d) sqrt(m)
+ * summed over the faces of the grid a node lies on (BC = 2 for bc_order 2,
+ * 1 for bc_order 1), the top face (low end of the last axis) skipped when
+ * neumann_top.  m: prod(nnodes) doubles; out: prod(nnodes) complex128
+ * (interleaved), column-major; both host (loc = HH_HOST) or both device. */
+int hh_sommerfeld_bc(int dim, const int64_t nnodes[3], const double h[3], const double* m, double omega,
+                     int neumann_top, int bc_order, void* out, int loc);
+
 /* Operator H = -Lap_h - w^2 m (1 - i gamma / Re w) + Sommerfeld terms on a
  * nodal grid.  Replaces GetHelmholtzOperator(Msh, m, omega, gamma,
  * NeumannAtFirstDim, Sommerfeld, orderNeumannBC) (src/GetHelmholtz.jl:33-50)

@@ -44,6 +44,12 @@ def test_argument_errors_are_reported_without_touching_the_gpu():
     rc = L.hh_op_create(2, nn, h, C.c_void_p(1), C.c_void_p(1), 1.0, 0.0, 0, 0, 3, C.byref(out))
     assert rc == _lib.HH_ERR_ARG
     assert b"bc_order" in L.hh_last_error()
+    # getSommerfeldBC: a bad dimension, bc order or location is refused before any device call
+    m1 = C.c_void_p(1)
+    assert L.hh_sommerfeld_bc(4, nn, h, m1, 1.0, 0, 2, m1, 0) == _lib.HH_ERR_ARG
+    assert L.hh_sommerfeld_bc(2, nn, h, m1, 1.0, 0, 3, m1, 0) == _lib.HH_ERR_ARG
+    assert b"bc_order" in L.hh_last_error()
+    assert L.hh_sommerfeld_bc(2, nn, h, m1, 1.0, 0, 2, m1, 7) == _lib.HH_ERR_ARG
 
 
 def test_solver_rejects_unsupported_settings():

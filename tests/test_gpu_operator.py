@@ -98,6 +98,22 @@ def test_abl_matches_oracle(n, neumann, pad):
     np.testing.assert_allclose(g, r, rtol=1e-14, atol=1e-14)
 
 
+@pytest.mark.parametrize("nodes,neumann,order", [((12, 10), False, 2), ((12, 10), True, 2), ((33, 17), True, 1),
+                                                  ((9, 8, 7), False, 2), ((9, 8, 7), True, 1)])
+def test_sommerfeld_bc_matches_oracle(nodes, neumann, order):
+    """getSommerfeldBC (src/GetHelmholtz.jl:222-247) on the GPU against the
+    oracle's face-by-face sums: zero in the interior, -i w BC/h sqrt(m) on
+    faces, sums at edges and corners, no top face with a Neumann top."""
+    M, m, g, w = small_case(nodes, seed=12, neumann=neumann)
+    r = hr.get_sommerfeld_bc(M, m, w, neumann, order)
+    s = ha.getSommerfeldBC(_hh_mesh(M), m, w, neumann, order)
+    assert s.shape == r.shape
+    np.testing.assert_allclose(s, r, rtol=1e-15, atol=0)
+    assert np.count_nonzero(s) == np.count_nonzero(r)
+    with pytest.raises(TypeError):
+        ha.getSommerfeldBC(_hh_mesh(M), m, complex(w, 0.1), neumann, order)
+
+
 def test_eight_argument_form_returns_gamma():
     M, m, g, w = small_case((33, 17), seed=9, neumann=True)
     g0 = 0.01 * w * np.ones(m.size)
