@@ -8,7 +8,8 @@ import pytest
 import helmholtz_amd as ha
 from helmholtz_amd import _lib
 
-HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "helmholtz_amd.h")
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "helmholtz_amd.h")
 
 
 def _declared():
@@ -152,3 +153,39 @@ def test_julia_shim_structs_match_the_header():
     jl = open(os.path.join(root, "helmholtz.jl_amd", "julia", "HelmholtzAMD.jl")).read()
     assert _julia_fields(jl, "HHMGParams") == _c_fields(header, "hh_mg_params")
     assert _julia_fields(jl, "HHSolveStats") == _c_fields(header, "hh_solve_stats")
+
+
+def test_no_cpu_fallback_without_the_library(tmp_path):
+    """Without libhelmholtz_amd.so every compute entry of the package raises:
+    there is no host fallback (run in a fresh interpreter so that the
+    library this process loaded stays untouched)."""
+    import subprocess
+    import sys
+    import textwrap
+
+    script = textwrap.dedent(f"""
+        import sys
+        sys.path.insert(0, {os.path.join(ROOT, "helmholtz.jl_amd")!r})
+        import numpy as np
+        from helmholtz_amd import _lib
+        _lib.LIB_PATH = {str(tmp_path / "absent.so")!r}
+        import helmholtz_amd as ha
+        M = ha.getRegularMesh([0.0, 1.0, 0.0, 1.0], [8, 8])
+        m = np.ones(81)
+        calls = [
+            lambda: ha.getABL([9, 9], False, [2, 2], 1.0),
+            lambda: ha.getSommerfeldBC(M, m, 1.0, False),
+            lambda: ha.GetHelmholtzOperator(M, m, 1.0, np.zeros(81), False, True) @ np.ones(81, complex),
+        ]
+        for c in calls:
+            try:
+                c()
+            except ha.HelmholtzError as e:
+                assert "not built" in str(e), e
+            else:
+                raise SystemExit("a compute call ran without the library")
+        print("all raised")
+    """)
+    r = subprocess.run([sys.executable, "-c", script], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "all raised" in r.stdout
