@@ -189,3 +189,15 @@ def test_no_cpu_fallback_without_the_library(tmp_path):
     r = subprocess.run([sys.executable, "-c", script], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "all raised" in r.stdout
+
+
+def test_sommerfeld_bc_host_checks():
+    """getSommerfeldBC takes a real omega (the reference's omega::Float64) and
+    one slowness value per node; both are checked before the library runs."""
+    import numpy as np
+
+    M = ha.getRegularMesh([0.0, 1.0, 0.0, 1.0], [8, 8])
+    with pytest.raises(TypeError):
+        ha.getSommerfeldBC(M, np.ones(81), complex(1.0, 0.5), False)
+    with pytest.raises(ValueError):
+        ha.getSommerfeldBC(M, np.ones(80), 1.0, False)
