@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "../../include/helmholtz_amd.h"
+#include "hh_abl.h"
 #include "hh_kernels.h"
 #include "hh_unet.h"
 
@@ -1264,6 +1265,43 @@ void run_bicgstab(hh_solver* S, int nrhs, hh_solve_stats* stats) {
   if (warn) fprintf(stderr, "WARNING: MG solver reached maximum iterations without convergence\n");
 }
 
+// getABL on the device: host ramps (hh_abl.h), combined per node by k_abl
+void abl_impl(int dim, const int64_t nnodes[3], int neumann_at_first_dim, const int64_t pad[3], double amp,
+              const int32_t* faces, double* out, int loc) {
+  if (!nnodes || !pad || !out) throw HHError(HH_ERR_ARG, "null argument");
+  if (dim != 2 && dim != 3) throw HHError(HH_ERR_ARG, "dim must be 2 or 3");
+  if (loc != HH_HOST && loc != HH_DEVICE) throw HHError(HH_ERR_ARG, "loc must be HH_HOST or HH_DEVICE");
+  std::vector<std::vector<double>> prof;
+  try {
+    prof = abl_profiles(dim, nnodes, neumann_at_first_dim, pad, faces);
+  } catch (const std::invalid_argument& e) {
+    throw HHError(HH_ERR_ARG, e.what());
+  }
+  const int64_t N = (int64_t)prof[0].size() * (int64_t)prof[1].size() * (int64_t)prof[2].size();
+  double* dp[3] = {nullptr, nullptr, nullptr};
+  double* dout = nullptr;
+  hipError_t e = hipSuccess;
+  try {
+    for (int d = 0; d < 3; ++d) dp[d] = dalloc<double>(prof[d].size());
+    dout = loc == HH_DEVICE ? out : dalloc<double>(N);
+  } catch (...) {
+    for (int d = 0; d < 3; ++d) dfree(dp[d]);
+    throw;
+  }
+  for (int d = 0; d < 3 && e == hipSuccess; ++d)
+    e = hipMemcpy(dp[d], prof[d].data(), sizeof(double) * prof[d].size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_abl, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, nullptr, dim, (int)prof[0].size(),
+                       (int)prof[1].size(), (int)prof[2].size(), dp[0], dp[1], dp[2], amp, dout);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess && loc == HH_HOST) e = hipMemcpy(out, dout, sizeof(double) * N, hipMemcpyDeviceToHost);
+  for (int d = 0; d < 3; ++d) dfree(dp[d]);
+  if (loc == HH_HOST) dfree(dout);
+  hipck(e, "hh_abl");
+}
+
 }  // namespace
 
 // ============================================================== C ABI
@@ -1580,59 +1618,17 @@ int hh_sommerfeld_bc(int dim, const int64_t nnodes[3], const double h[3], const 
 
 int hh_abl(int dim, const int64_t nnodes[3], int neumann_at_first_dim, const int64_t pad[3], double amp,
            double* out, int loc) {
+  return guarded([&] { abl_impl(dim, nnodes, neumann_at_first_dim, pad, amp, nullptr, out, loc); });
+}
+
+int hh_abl_faces(int dim, const int64_t nnodes[3], int neumann_at_first_dim, const int64_t pad[3], double amp,
+                 const int32_t faces[6], double* out, int loc) {
   return guarded([&] {
-    if (!nnodes || !pad || !out) throw HHError(HH_ERR_ARG, "null argument");
+    if (!faces) throw HHError(HH_ERR_ARG, "null argument");
     if (dim != 2 && dim != 3) throw HHError(HH_ERR_ARG, "dim must be 2 or 3");
-    if (loc != HH_HOST && loc != HH_DEVICE) throw HHError(HH_ERR_ARG, "loc must be HH_HOST or HH_DEVICE");
-    std::vector<std::vector<double>> prof(3);
-    int64_t N = 1;
-    for (int d = 0; d < 3; ++d) {
-      const int64_t n = d < dim ? nnodes[d] : 1;
-      if (n < 1 || n > (1 << 24)) throw HHError(HH_ERR_ARG, "bad node count");
-      N *= n;
-      prof[d].assign(n, 0.0);
-      if (d >= dim) continue;
-      const int64_t pd = pad[d];
-      if (pd < 1 || pd > n) throw HHError(HH_ERR_ARG, "pad must be in 1..n");
-      std::vector<double>& P = prof[d];
-      if (dim == 2) {
-        // quadratic ramps ((pad - k)/pad)^2 at the low end, ((k + 1)/pad)^2 at the high end
-        const bool low_on = !(d == 1 && neumann_at_first_dim);
-        for (int64_t k = 0; k < pd; ++k) {
-          if (low_on) P[k] += (double)((pd - k) * (pd - k)) / (double)(pd * pd);
-          P[n - pd + k] += (double)((k + 1) * (k + 1)) / (double)(pd * pd);
-        }
-      } else {
-        // squared distance to the inner edge of the pad on a [-1,1] (x, y) or [0,1] (z) axis
-        const double lo = d == 2 ? 0.0 : -1.0, len = d == 2 ? 1.0 : 2.0;
-        auto xc = [&](int64_t k) { return lo + len * (double)k / (double)(n - 1); };
-        const bool low_on = !(d == 2 && neumann_at_first_dim);
-        for (int64_t k = 0; k < n; ++k) {
-          double v = 0.0;
-          if (low_on && k < pd) v += (xc(k) - xc(pd - 1)) * (xc(k) - xc(pd - 1));
-          if (k >= n - pd) v += (xc(k) - xc(n - pd)) * (xc(k) - xc(n - pd));
-          P[k] = v;
-        }
-        double mx = 0.0;
-        for (double v : P) mx = v > mx ? v : mx;
-        for (double& v : P) v /= (mx + 1e-5);
-      }
-    }
-    hipStream_t st = nullptr;
-    double* dp[3];
-    for (int d = 0; d < 3; ++d) {
-      dp[d] = dalloc<double>(prof[d].size());
-      HIPCK(hipMemcpy(dp[d], prof[d].data(), sizeof(double) * prof[d].size(), hipMemcpyHostToDevice));
-    }
-    double* dout = loc == HH_DEVICE ? out : dalloc<double>(N);
-    hipLaunchKernelGGL(k_abl, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, dim, (int)prof[0].size(),
-                       (int)prof[1].size(), (int)prof[2].size(), dp[0], dp[1], dp[2], amp, dout);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hipDeviceSynchronize();
-    if (e == hipSuccess && loc == HH_HOST) e = hipMemcpy(out, dout, sizeof(double) * N, hipMemcpyDeviceToHost);
-    for (int d = 0; d < 3; ++d) dfree(dp[d]);
-    if (loc == HH_HOST) dfree(dout);
-    hipck(e, "hh_abl");
+    for (int q = 0; q < 2 * dim; ++q)
+      if (faces[q] != 0 && faces[q] != 1) throw HHError(HH_ERR_ARG, "faces entries must be 0 or 1");
+    abl_impl(dim, nnodes, neumann_at_first_dim, pad, amp, faces, out, loc);
   });
 }
 

@@ -26,6 +26,7 @@ EXPORTS = (
     "hh_dataset_size", "hh_solver_capture_begin", "hh_solver_capture_count", "hh_dataset_add_from_solver",
     "hh_unet_train", "hh_unet_eval", "hh_solver_attach_unet", "hh_stream_solve", "hh_solver_precondition",
     "hh_solver_time_kernel", "hh_solver_kernel_time", "hh_op_create_ho", "hh_sommerfeld_bc",
+    "hh_abl_faces",
 )
 HH_RELAX_JAC, HH_RELAX_JAC_GMRES = 0, 1
 HH_KT_NONE, HH_KT_DOWN, HH_KT_MID, HH_KT_UP, HH_KT_APPLY, HH_KT_DOTS, HH_KT_UPDATE = range(7)
@@ -75,6 +76,7 @@ def lib():
     L.hh_last_error.argtypes = []
     L.hh_version.restype = i32
     L.hh_abl.argtypes = [i32, p64, i32, p64, dbl, vp, i32]
+    L.hh_abl_faces.argtypes = [i32, p64, i32, p64, dbl, C.POINTER(C.c_int32), vp, i32]
     L.hh_sommerfeld_bc.argtypes = [i32, p64, pd, vp, dbl, i32, i32, vp, i32]
     L.hh_op_create.argtypes = [i32, p64, pd, vp, vp, dbl, dbl, i32, i32, i32, C.POINTER(vp)]
     L.hh_op_create_ho.argtypes = [i32, p64, pd, vp, vp, dbl, dbl, i32, i32, dbl, dbl, C.POINTER(vp)]

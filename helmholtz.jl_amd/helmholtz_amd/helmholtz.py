@@ -96,14 +96,25 @@ def _is_device(x) -> bool:
     return isinstance(x, DeviceArray)
 
 
-def getABL(n, NeumannAtFirstDim: bool, ABLpad, ABLamp: float):
-    """src/GetHelmholtz.jl:97-220 computed on the GPU (``hh_abl``).  ``n`` are node
-    counts; returns a float64 array shaped ``tuple(n)`` in Fortran order."""
+def getABL(n, NeumannAtFirstDim: bool, ABLpad, ABLamp: float, code=None):
+    """src/GetHelmholtz.jl:97-220 computed on the GPU (``hh_abl``; with ``code``,
+    the reference's length(n) x 2 Bool face mask, ``hh_abl_faces``: it switches
+    the low/high face of each axis in 3-D and is not read in 2-D, as in the
+    reference).  ``n`` are node counts; returns a float64 array shaped
+    ``tuple(n)`` in Fortran order."""
     n = [int(v) for v in np.asarray(n).ravel()]
     pad = [int(v) for v in np.asarray(ABLpad).ravel()]
     out = np.zeros(int(np.prod(n)), dtype=np.float64)
-    L.check(L.lib().hh_abl(len(n), _i64x3(n), int(bool(NeumannAtFirstDim)), _i64x3(pad), float(ABLamp),
-                           out.ctypes.data, L.HH_HOST))
+    if code is None:
+        L.check(L.lib().hh_abl(len(n), _i64x3(n), int(bool(NeumannAtFirstDim)), _i64x3(pad), float(ABLamp),
+                               out.ctypes.data, L.HH_HOST))
+    else:
+        c = np.asarray(code, dtype=bool)
+        if c.shape != (len(n), 2):
+            raise ValueError("code must be a length(n) x 2 Bool matrix")
+        faces = (C.c_int32 * 6)(*([int(v) for v in c.ravel()] + [1] * (6 - c.size)))
+        L.check(L.lib().hh_abl_faces(len(n), _i64x3(n), int(bool(NeumannAtFirstDim)), _i64x3(pad), float(ABLamp),
+                                     faces, out.ctypes.data, L.HH_HOST))
     return out.reshape(tuple(n), order="F")
 
 

@@ -107,6 +107,15 @@ int hh_version(void);
 int hh_abl(int dim, const int64_t nnodes[3], int neumann_at_first_dim, const int64_t pad[3], double amp,
            double* out, int loc);
 
+/* getABL(n, NeumannAtFirstDim, ABLpad, ABLamp, code) with the reference's face
+ * mask `code` (src/GetHelmholtz.jl:97; a length(n) x 2 Bool matrix):
+ * faces[2 d] = code[d+1, 1] (low face of axis d), faces[2 d + 1] = code[d+1, 2]
+ * (high face), each 0 or 1.  The mask acts in 3-D (:164-218); the 2-D branch
+ * (impl = 1, :141-163) does not read it, so 2-D results equal hh_abl's.  A
+ * Neumann top still removes the low face of the last axis. */
+int hh_abl_faces(int dim, const int64_t nnodes[3], int neumann_at_first_dim, const int64_t pad[3], double amp,
+                 const int32_t faces[6], double* out, int loc);
+
 /* Sommerfeld boundary terms getSommerfeldBC(Msh, mNodal, omega, NeumannOnTop,
  * orderNeumannBC) (src/GetHelmholtz.jl:222-247): -i omega (BC / h_d) sqrt(m)
  * summed over the faces of the grid a node lies on (BC = 2 for bc_order 2,

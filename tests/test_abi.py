@@ -51,6 +51,15 @@ def test_argument_errors_are_reported_without_touching_the_gpu():
     assert L.hh_sommerfeld_bc(2, nn, h, m1, 1.0, 0, 3, m1, 0) == _lib.HH_ERR_ARG
     assert b"bc_order" in L.hh_last_error()
     assert L.hh_sommerfeld_bc(2, nn, h, m1, 1.0, 0, 2, m1, 7) == _lib.HH_ERR_ARG
+    # getABL with a face mask: entries must be 0 or 1, and the mask must be given
+    pad = (C.c_int64 * 3)(2, 2, 2)
+    nn3 = (C.c_int64 * 3)(9, 8, 7)
+    bad = (C.c_int32 * 6)(1, 1, 2, 1, 1, 1)
+    assert L.hh_abl_faces(3, nn3, 0, pad, 1.0, bad, m1, 0) == _lib.HH_ERR_ARG
+    assert b"faces" in L.hh_last_error()
+    assert L.hh_abl_faces(3, nn3, 0, pad, 1.0, None, m1, 0) == _lib.HH_ERR_ARG
+    with pytest.raises(ValueError):
+        ha.getABL([9, 8, 7], False, [2, 2, 2], 1.0, [[1, 1], [1, 1]])
 
 
 def test_solver_rejects_unsupported_settings():
