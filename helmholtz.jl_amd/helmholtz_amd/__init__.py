@@ -8,6 +8,7 @@ reference interface.  Importing it does not touch the GPU.
 from ._lib import HelmholtzError, LIB_PATH, lib  # noqa: F401
 from .helmholtz import (  # noqa: F401
     GetHelmholtzOperator, GetHelmholtzOperatorHO, GetHelmholtzShiftOP, HelmholtzOperator, HelmholtzParam, RegularMesh,
+    getHelmholtzFun,
     getABL, getAcousticPointSource, getMaximalFrequency, getMidPointSrc, getRegularMesh,
     getShiftedHelmholtzParam, getSommerfeldBC, getTopPointSrc, loc2cs, point_sources,
 )

@@ -242,6 +242,29 @@ class HelmholtzOperator:
         return self.apply(X)
 
 
+def getHelmholtzFun(ShiftedHT, ShiftMat, y, numCores: int = 1):
+    """src/GetHelmholtz.jl:85-95: ``Hfun(x)`` writes (ShiftedHT + ShiftMat) x
+    into ``y`` and returns ``y`` (the reference's two SpMatMul calls).  Both
+    operands are matrix-free here: ``ShiftedHT`` a HelmholtzOperator (the
+    shifted operator, e.g. ``H + GetHelmholtzShiftOP(m, w, s)``) and
+    ``ShiftMat`` a ShiftOp (``GetHelmholtzShiftOP(m, w, -s)`` takes the shift
+    back out), so their sum is one operator and ``Hfun`` is one
+    ``hh_op_apply``.  ``y`` is a numpy array or a DeviceArray of x's shape;
+    ``numCores`` is accepted for the signature.  Sparse matrices are refused:
+    there is no host SpMV path."""
+    if not isinstance(ShiftedHT, HelmholtzOperator) or not isinstance(ShiftMat, ShiftOp):
+        raise TypeError("getHelmholtzFun takes a HelmholtzOperator and a ShiftOp (matrix-free operators only)")
+    op = ShiftedHT + ShiftMat
+
+    def Hfun(x):
+        if _is_device(y):
+            return op.apply(x, out=y)
+        y[...] = np.asarray(op.apply(x)).reshape(y.shape, order="F")
+        return y
+
+    return Hfun
+
+
 def _build(mesh, m, omega, gamma, neumann, sommerfeld, order, param=None):
     return HelmholtzOperator(_OpHandle(mesh, m, omega, gamma, neumann, sommerfeld, order), 0.0, mesh, param)
 

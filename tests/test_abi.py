@@ -210,3 +210,14 @@ def test_sommerfeld_bc_host_checks():
         ha.getSommerfeldBC(M, np.ones(81), complex(1.0, 0.5), False)
     with pytest.raises(ValueError):
         ha.getSommerfeldBC(M, np.ones(80), 1.0, False)
+
+
+def test_helmholtz_fun_refuses_sparse_matrices():
+    """getHelmholtzFun composes matrix-free operators only: the reference's
+    CSR arguments are refused (there is no host SpMV path)."""
+    import numpy as np
+    import scipy.sparse as sp
+
+    with pytest.raises(TypeError):
+        ha.getHelmholtzFun(sp.eye(4, format="csc"), sp.eye(4, format="csc"), np.zeros(4, complex), 1)
+
