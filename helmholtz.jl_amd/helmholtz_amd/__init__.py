@@ -10,7 +10,7 @@ from .helmholtz import (  # noqa: F401
     GetHelmholtzOperator, GetHelmholtzOperatorHO, GetHelmholtzShiftOP, HelmholtzOperator, HelmholtzParam, RegularMesh,
     getHelmholtzFun,
     getABL, getAcousticPointSource, getMaximalFrequency, getMidPointSrc, getRegularMesh,
-    getShiftedHelmholtzParam, getSommerfeldBC, getTopPointSrc, loc2cs, point_sources,
+    getShiftedHelmholtzParam, getSommerfeldBC, getTopPointSrc, loc2cs, loc2cs3D, point_sources,
 )
 from .solver import (  # noqa: F401
     MGparam, ShiftedLaplacianMultigridSolver, clear, copySolver, getMGparam,

@@ -333,12 +333,21 @@ def getMidPointSrc(M: RegularMesh):
     return [int(v + 1) // 2 for v in M.n]
 
 
-def loc2cs(n, sub) -> int:
-    """src/getPointSource.jl:82-100: 1-based column-major linear index."""
+def loc2cs(*args) -> int:
+    """src/getPointSource.jl:82-100: 1-based column-major linear index, in the
+    reference's two methods ``loc2cs(n, sub)`` and ``loc2cs(loc1, loc2, n)``."""
+    if len(args) == 3:
+        return loc2cs(args[2], [args[0], args[1]])
+    n, sub = args
     n = [int(v) for v in n]
     if len(sub) == 2:
         return int(sub[0] + (sub[1] - 1) * n[0])
     return int(sub[0] + (sub[1] - 1) * n[0] + (sub[2] - 1) * n[0] * n[1])
+
+
+def loc2cs3D(loc1: int, loc2: int, loc3: int, n) -> int:
+    """src/getPointSource.jl:87-90."""
+    return loc2cs(n, [loc1, loc2, loc3])
 
 
 def getAcousticPointSource(M: RegularMesh, TYPE=np.complex128, src=None):

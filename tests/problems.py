@@ -31,7 +31,8 @@ def small_case(nodes, seed=0, dim=None, neumann=True, sommerfeld=True, order=2, 
 
 def attenuation_case(nn=129):
     """The flow of test/AttenuationTest.jl at 129x129: unit slowness on
-    [0, 10]^2, 20 points per wavelength, a centred point source 1/h^2, no
+    [0, 10]^2, 20 points per wavelength, a point source 1/h^2 at node
+    div(n, 2) on both axes (1-based, as the reference test's src), no
     Sommerfeld, no Neumann top; attenuation alpha = 0.2 w either added to gamma
     at real w (s1) or carried by the complex frequency w - i alpha/2 (s2)."""
     n = (nn, nn)
@@ -44,7 +45,7 @@ def attenuation_case(nn=129):
     gam = hr.get_abl(n, False, [pad, pad], wmax).ravel(order="F")
     src = [nn // 2, nn // 2]
     q = np.zeros(nn * nn, complex)
-    q[hr.loc2cs(n, src)] = 1.0 / M.h[0] ** 2
+    q[hr.loc2cs(n, src) - 1] = 1.0 / M.h[0] ** 2
     return M, m, w, alpha, gam, pad, src, q
 
 
@@ -55,7 +56,7 @@ def attenuation_decay_slope(s, nn, src, pad, h):
     S = np.asarray(s).reshape((nn, nn), order="F")
     rs = np.arange(5, nn // 2 - pad - 2)
     r = rs * h
-    a = np.abs(S[src[0] + rs, src[1]])
+    a = np.abs(S[src[0] - 1 + rs, src[1] - 1])
     return np.polyfit(r, np.log(a * np.sqrt(r)), 1)[0]
 
 
@@ -84,5 +85,5 @@ def seg_salt_case():
     H, g = hr.get_helmholtz_operator(M, m, w, 0.01 * w * np.ones(m.size), True, [16, 16],
                                      hr.get_maximal_frequency(m, M), True)
     q = np.zeros(m.size, complex)
-    q[hr.loc2cs(n, [n[0] // 2, 1])] = 1.0 / M.h[0] ** 2
+    q[hr.loc2cs(n, [n[0] // 2, 1]) - 1] = 1.0 / M.h[0] ** 2
     return M, m, g.ravel(order="F"), w, H, q

@@ -221,3 +221,17 @@ def test_helmholtz_fun_refuses_sparse_matrices():
     with pytest.raises(TypeError):
         ha.getHelmholtzFun(sp.eye(4, format="csc"), sp.eye(4, format="csc"), np.zeros(4, complex), 1)
 
+
+def test_loc2cs_methods():
+    """Both loc2cs methods and loc2cs3D (src/getPointSource.jl:82-100) give the
+    1-based column-major index, which numpy's ravel_multi_index confirms."""
+    import numpy as np
+
+    n = [7, 5, 4]
+    for sub in ([3, 2], [7, 5], [1, 1]):
+        want = int(np.ravel_multi_index([v - 1 for v in sub], n[:2], order="F")) + 1
+        assert ha.loc2cs(n[:2], sub) == want == ha.loc2cs(sub[0], sub[1], n[:2])
+    for sub in ([3, 2, 4], [7, 5, 1], [1, 1, 1]):
+        want = int(np.ravel_multi_index([v - 1 for v in sub], n, order="F")) + 1
+        assert ha.loc2cs(n, sub) == want == ha.loc2cs3D(sub[0], sub[1], sub[2], n)
+

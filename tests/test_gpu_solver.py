@@ -784,7 +784,7 @@ def test_seg_salt_model_solve(prec, levels, shift):
     M, m, g, w, H, q = seg_salt_case()
     n = (257, 129)
     q2 = np.zeros_like(q)
-    q2[hr.loc2cs(n, [64, 1])] = q.max()
+    q2[hr.loc2cs(n, [64, 1]) - 1] = q.max()
     B = np.asfortranarray(np.stack([q, q2], axis=1))
     mesh = ha.getRegularMesh(M.domain, M.n)
     hp = ha.HelmholtzParam(mesh, g, m, w, True, True)
