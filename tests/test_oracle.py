@@ -573,3 +573,14 @@ def test_seg_salt_fixture_and_reference_setup_converges():
     err = np.linalg.norm(x - xd) / np.linalg.norm(xd)
     print(f"SEG salt, reference settings: {it} iterations, res {res:.2e}, err vs direct {err:.2e}")
     assert res <= 1e-6 and it <= 30 and err <= 1e-4
+
+
+def test_seg_fixture_matches_the_reference_file():
+    """tests/golden/seg_salt_2d.npz holds exactly the values of the
+    reference's examples/SEGmodel2Dsalt.dat (checked where the reference
+    checkout is present; the GPU tests read only the fixture)."""
+    src = "/root/reference/examples/SEGmodel2Dsalt.dat"
+    if not os.path.exists(src):
+        pytest.skip("reference checkout not present")
+    f = np.load(os.path.join(GOLDEN, "seg_salt_2d.npz"))
+    np.testing.assert_array_equal(f["v"].astype(np.float64), np.loadtxt(src))
