@@ -235,3 +235,54 @@ def test_loc2cs_methods():
         want = int(np.ravel_multi_index([v - 1 for v in sub], n, order="F")) + 1
         assert ha.loc2cs(n, sub) == want == ha.loc2cs3D(sub[0], sub[1], sub[2], n)
 
+
+
+def _split_top(s: str):
+    """Split at commas outside (), {} and [] (for Julia type tuples and C
+    parameter lists)."""
+    out, depth, cur = [], 0, ""
+    for ch in s:
+        if ch in "({[":
+            depth += 1
+        elif ch in ")}]":
+            depth -= 1
+        if ch == "," and depth == 0:
+            out.append(cur.strip())
+            cur = ""
+        else:
+            cur += ch
+    if cur.strip():
+        out.append(cur.strip())
+    return out
+
+
+def _balanced(s: str, start: int) -> str:
+    """The text inside the parenthesis that opens at s[start]."""
+    depth = 0
+    for i in range(start, len(s)):
+        depth += s[i] == "("
+        depth -= s[i] == ")"
+        if depth == 0:
+            return s[start + 1:i]
+    raise ValueError("unbalanced")
+
+
+def test_julia_ccalls_match_the_header():
+    """Every ccall of the Julia shim names a function the header declares and
+    passes as many argument types as the C declaration has parameters."""
+    header = open(HEADER).read()
+    decl = {}
+    for mt in re.finditer(r"^\s*(?:const\s+)?\w+\s*\*?\s*(hh_\w+)\s*\(", header, re.M):
+        params = _split_top(_balanced(header, mt.end() - 1))
+        decl[mt.group(1)] = 0 if params in ([], ["void"]) else len(params)
+    jl = open(os.path.join(ROOT, "helmholtz.jl_amd", "julia", "HelmholtzAMD.jl")).read()
+    calls = list(re.finditer(r"ccall\(\(:(hh_\w+), LIB\)", jl))
+    assert calls
+    for mc in calls:
+        name = mc.group(1)
+        assert name in decl, name
+        args = _split_top(_balanced(jl, mc.start() + len("ccall")))
+        types = args[2].strip()
+        assert types.startswith("(") and types.endswith(")"), (name, types)
+        n = len(_split_top(types[1:-1]))
+        assert n == decl[name], (name, n, decl[name])
