@@ -286,3 +286,20 @@ def test_julia_ccalls_match_the_header():
         assert types.startswith("(") and types.endswith(")"), (name, types)
         n = len(_split_top(types[1:-1]))
         assert n == decl[name], (name, n, decl[name])
+
+
+def test_ctypes_argtypes_match_the_header():
+    """The ctypes binding declares, for every header function it sets
+    argtypes for, as many argument types as the C declaration has
+    parameters."""
+    header = open(HEADER).read()
+    L = ha.lib()
+    checked = 0
+    for mt in re.finditer(r"^\s*(?:const\s+)?\w+\s*\*?\s*(hh_\w+)\s*\(", header, re.M):
+        params = _split_top(_balanced(header, mt.end() - 1))
+        n = 0 if params in ([], ["void"]) else len(params)
+        at = getattr(L, mt.group(1)).argtypes
+        if at is not None:
+            assert len(at) == n, (mt.group(1), len(at), n)
+            checked += 1
+    assert checked >= 30
