@@ -283,8 +283,15 @@ struct hh_solver {
   int learn_verdict = -1;
   float* feat_kh2 = nullptr;
   float* feat_gw = nullptr;
+  int feat_level = -1;       // level the feature fields were built for
   float* nn_scale = nullptr;
   float h2_learn = 1.f;
+  // this solver's own inference workspace: activations, input and output
+  // staging (the network's weights are shared read-only with other solvers)
+  hh::UNetWork nn_ws;
+  hh::bf16* nn_x16 = nullptr;
+  float* nn_y = nullptr;
+  int64_t nn_pix = 0;
   // launch shapes picked by the autotuner (0 = heuristic), per level
   int rows_tuned[16] = {0};
   int tuned_for = 0;
@@ -347,8 +354,20 @@ struct hh_solver {
     learn_verdict = -1;
   }
   void free_learned() {
-    dfree(feat_kh2); dfree(feat_gw); dfree(nn_scale);
-    feat_kh2 = nullptr; feat_gw = nullptr; nn_scale = nullptr;
+    dfree(feat_kh2); dfree(feat_gw); dfree(nn_scale); dfree(nn_x16); dfree(nn_y);
+    feat_kh2 = nullptr; feat_gw = nullptr; nn_scale = nullptr; nn_x16 = nullptr; nn_y = nullptr;
+    nn_pix = 0;
+    feat_level = -1;
+    nn_ws.release();
+  }
+  // input/output staging of the network for npix pixels (all images)
+  void ensure_nn_stage(int64_t npix) {
+    if (npix <= nn_pix) return;
+    dfree(nn_x16); dfree(nn_y);
+    nn_x16 = nullptr; nn_y = nullptr; nn_pix = 0;
+    nn_x16 = dalloc<hh::bf16>((size_t)npix * hh::UNET_CIN);
+    nn_y = dalloc<float>((size_t)npix * 2);
+    nn_pix = npix;
   }
   ~hh_solver() {
     free_hier();
@@ -437,19 +456,32 @@ void build_hierarchy(hh_solver* S) {
   S->built = true;
 }
 
-// network features of the learned level (after the hierarchy exists)
-void setup_learned(hh_solver* S) {
-  if (!S->net || S->feat_kh2) return;
-  if (S->learn_level < 1 || S->learn_level >= (int)S->lv.size())
-    throw HHError(HH_ERR_ARG, "learned level must be in 1..levels-1");
-  Level& L = S->lv[S->learn_level];
+// feature fields (k^2 h^2 and gamma / Re w) of grid level `level`, kept until
+// another level asks for them (network inputs and training-pair features share
+// the buffers, so they are rebuilt whenever the level differs)
+void ensure_level_feats(hh_solver* S, int level) {
+  if (S->feat_kh2 && S->feat_level == level) return;
+  Level& L = S->lv[level];
+  dfree(S->feat_kh2); dfree(S->feat_gw);
+  S->feat_kh2 = S->feat_gw = nullptr;
+  S->feat_level = -1;
   S->feat_kh2 = dalloc<float>(L.g.N);
   S->feat_gw = dalloc<float>(L.g.N);
-  S->nn_scale = dalloc<float>(65536);
-  S->h2_learn = (float)(L.h[0] * L.h[0]);
   hipLaunchKernelGGL(k_level_feats, dim3((unsigned)((L.g.N + 255) / 256)), dim3(256), 0, S->op->stream, L.m, L.gam,
                      L.g.N, S->op->wre, L.h[0] * L.h[0], S->feat_kh2, S->feat_gw);
   HIPCK(hipGetLastError());
+  S->feat_level = level;
+}
+
+// network features of the learned level (after the hierarchy exists)
+void setup_learned(hh_solver* S) {
+  if (!S->net) return;
+  if (S->learn_level < 1 || S->learn_level >= (int)S->lv.size())
+    throw HHError(HH_ERR_ARG, "learned level must be in 1..levels-1");
+  Level& L = S->lv[S->learn_level];
+  ensure_level_feats(S, S->learn_level);
+  if (!S->nn_scale) S->nn_scale = dalloc<float>(65536);
+  S->h2_learn = (float)(L.h[0] * L.h[0]);
 }
 
 // row-major A^-1 of the coarsest level (left half of the Gauss-Jordan workspace)
@@ -735,13 +767,12 @@ struct Cycle {
       }
       if (S->use_learned && l == S->learn_level) {
         L.cur = first ? first : L.xa;
-        hh_unet* u = S->net;
-        u->ensure_stage(L.g.N * nrhs);
-        unet_prep_input(nrhs, L.g.N, (const cx<float>*)b, S->feat_kh2, S->feat_gw, u->x16, S->nn_scale, skip, st);
-        u->net.stream = st;
-        unet_forward(u->net, nrhs, L.g.n1, L.g.n2, u->x16, u->y, false);
-        u->net.stream = u->stream;
-        unet_post_output(nrhs, L.g.N, u->y, S->nn_scale, S->h2_learn, (cx<float>*)L.cur, skip, st);
+        // the network's weights are read-only here; activations and staging
+        // are this solver's own, launches go to this solver's stream
+        S->ensure_nn_stage(L.g.N * nrhs);
+        unet_prep_input(nrhs, L.g.N, (const cx<float>*)b, S->feat_kh2, S->feat_gw, S->nn_x16, S->nn_scale, skip, st);
+        unet_forward(S->net->net, S->nn_ws, st, nrhs, (int)L.g.n1, (int)L.g.n2, S->nn_x16, S->nn_y);
+        unet_post_output(nrhs, L.g.N, S->nn_y, S->nn_scale, S->h2_learn, (cx<float>*)L.cur, skip, st);
         return;
       }
     }
@@ -850,8 +881,17 @@ struct Cycle {
     // number of buffer writes on level 0: pre (incl. the zero-start sweep) + post,
     // or for the fused legs down + (corrections - 1) + up
     const int flips = use_fused(L0) ? ncorr_at(0) + 1 : pre_at(0) + (use_up3(L0, 0) ? 1 : post_at(0));
-    // choose the ping-pong pair (out, xa) so the last write lands in out
-    void* saved_xb = L0.xb;
+    // choose the ping-pong pair (out, xa) so the last write lands in out;
+    // the level's own xb is put back on every exit, exceptions included, so
+    // free_work never sees the caller's block in its place
+    struct Restore {
+      Level& L;
+      void* xb;
+      ~Restore() {
+        L.xb = xb;
+        L.cur = nullptr;
+      }
+    } restore{L0, L0.xb};
     L0.xb = out;
     // (Jac-GMRES smoothing updates the iterate in place: start in out)
     void* first = (pre_at(0) == 0 || S->p.relax_type == HH_RELAX_JAC_GMRES) ? (void*)out
@@ -859,7 +899,6 @@ struct Cycle {
     run(0, b, true, first);
     if (L0.cur != out)
       HIPCK(hipMemcpyAsync(out, L0.cur, sizeof(cx<T>) * L0.g.N * nrhs, hipMemcpyDeviceToDevice, st));
-    L0.xb = saved_xb;
   }
 };
 
@@ -1785,13 +1824,8 @@ int hh_dataset_add_from_solver(hh_dataset* d, hh_solver* s, int sweeps) {
       const int chunk = 256;
       rb = dalloc<cx<double>>((size_t)N * chunk);
       xb = dalloc<cx<double>>((size_t)N * chunk);
-      // features of this level (network inputs) once
-      if (!s->feat_kh2) {
-        s->feat_kh2 = dalloc<float>(N);
-        s->feat_gw = dalloc<float>(N);
-        hipLaunchKernelGGL(k_level_feats, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, L.m, L.gam, N,
-                           s->op->wre, L.h[0] * L.h[0], s->feat_kh2, s->feat_gw);
-      }
+      // features of the captured level (network inputs)
+      ensure_level_feats(s, s->cap_level);
       for (int64_t k0 = 0; k0 < s->cap_count && d->n < d->cap;) {
         const int m = (int)std::min<int64_t>({(int64_t)chunk, s->cap_count - k0, d->cap - d->n});
         launch_cast<float, double>(s->cap_buf + k0 * N, rb, N * m, st);

@@ -204,7 +204,7 @@ int hh_unet_forward(hh_unet* u, int nimg, int nx, int ny, const float* feats, fl
     float* df = dmal2<float>((size_t)npix * UNET_FEATS);
     ck2(hipMemcpy(df, feats, sizeof(float) * npix * UNET_FEATS, hipMemcpyHostToDevice), "upload");
     hipLaunchKernelGGL(k_feats_to_x16, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, u->stream, df, npix, u->x16);
-    unet_forward(u->net, nimg, nx, ny, u->x16, u->y, false);
+    unet_forward(u->net, u->ws, u->stream, nimg, nx, ny, u->x16, u->y);
     ck2(hipStreamSynchronize(u->stream), "forward");
     ck2(hipMemcpy(y, u->y, sizeof(float) * npix * 2, hipMemcpyDeviceToHost), "download");
     (void)hipFree(df);
@@ -222,9 +222,9 @@ int hh_unet_loss_grad(hh_unet* u, int nimg, int nx, int ny, const float* feats, 
     ck2(hipMemcpy(df, feats, sizeof(float) * npix * UNET_FEATS, hipMemcpyHostToDevice), "upload");
     ck2(hipMemcpy(u->tgt, target, sizeof(float) * npix * 2, hipMemcpyHostToDevice), "upload");
     hipLaunchKernelGGL(k_feats_to_x16, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, u->stream, df, npix, u->x16);
-    unet_forward(u->net, nimg, nx, ny, u->x16, u->y, true);
+    unet_forward(u->net, u->ws, u->stream, nimg, nx, ny, u->x16, u->y);
     unet_mse(npix * 2, u->y, u->tgt, u->dy, u->loss_parts, 64, u->stream);
-    unet_backward(u->net, nimg, nx, ny, u->x16, u->dy);
+    unet_backward(u->net, u->ws, u->stream, nimg, nx, ny, u->x16, u->dy);
     ck2(hipStreamSynchronize(u->stream), "backward");
     float parts[64];
     ck2(hipMemcpy(parts, u->loss_parts, sizeof(parts), hipMemcpyDeviceToHost), "loss");
@@ -301,9 +301,9 @@ int hh_unet_train(hh_unet* u, hh_dataset* d, int epochs, int batch, float lr, ui
         hipLaunchKernelGGL(k_batch_rms, dim3(nb), dim3(256), 0, u->stream, d->r, N, didx, dscale);
         hipLaunchKernelGGL(k_batch_prep, dim3((unsigned)((N * nb + 255) / 256)), dim3(256), 0, u->stream, d->r, d->e,
                            d->kh2, d->gw, d->h2, N, didx, dscale, nb, u->x16, u->tgt);
-        unet_forward(u->net, nb, d->nx, d->ny, u->x16, u->y, true);
+        unet_forward(u->net, u->ws, u->stream, nb, d->nx, d->ny, u->x16, u->y);
         unet_mse(N * nb * 2, u->y, u->tgt, u->dy, u->loss_parts, 64, u->stream);
-        unet_backward(u->net, nb, d->nx, d->ny, u->x16, u->dy);
+        unet_backward(u->net, u->ws, u->stream, nb, d->nx, d->ny, u->x16, u->dy);
         unet_adam(u->net, lr, 0.9f, 0.999f, 1e-8f);
         ck2(hipMemcpyAsync(parts, u->loss_parts, sizeof(parts), hipMemcpyDeviceToHost, u->stream), "loss");
         ck2(hipStreamSynchronize(u->stream), "train step");
@@ -339,7 +339,7 @@ int hh_unet_eval(hh_unet* u, hh_dataset* d, int64_t first, int64_t count, float*
       hipLaunchKernelGGL(k_batch_rms, dim3(m), dim3(256), 0, u->stream, d->r, N, didx, dscale);
       hipLaunchKernelGGL(k_batch_prep, dim3((unsigned)((N * m + 255) / 256)), dim3(256), 0, u->stream, d->r, d->e,
                          d->kh2, d->gw, d->h2, N, didx, dscale, m, u->x16, u->tgt);
-      unet_forward(u->net, m, d->nx, d->ny, u->x16, u->y, false);
+      unet_forward(u->net, u->ws, u->stream, m, d->nx, d->ny, u->x16, u->y);
       hipLaunchKernelGGL(k_relerr_parts, dim3(m), dim3(256), 0, u->stream, u->y, u->tgt, N, derr);
       ck2(hipMemcpyAsync(err.data(), derr, sizeof(float) * m, hipMemcpyDeviceToHost, u->stream), "err");
       ck2(hipStreamSynchronize(u->stream), "eval");

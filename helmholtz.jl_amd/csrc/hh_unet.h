@@ -18,6 +18,22 @@ struct UNetLevelDims {
   int nx[4], ny[4];
 };
 
+// Activation workspace of one forward/backward context: the network's own
+// training hooks, or one solver's inference.  The weights (UNet) are shared
+// read-only between contexts; a workspace never is, so solvers on different
+// streams that hold one network do not touch each other's activations.
+struct UNetWork {
+  int nimg = 0, nx = 0, ny = 0;
+  std::vector<void*> bufs;
+  float* scratch = nullptr;       // wgrad partials
+  int64_t scratch_n = 0;
+  UNetWork() = default;
+  UNetWork(const UNetWork&) = delete;
+  UNetWork& operator=(const UNetWork&) = delete;
+  void release();
+  ~UNetWork() { release(); }
+};
+
 struct UNet {
   // parameters (device, fp32 master + bf16 copies)
   float* w[UNET_NCONV] = {};      // [9][co][ci]
@@ -32,22 +48,18 @@ struct UNet {
   float* adam_v = nullptr;
   int64_t nparam = 0;
   int64_t adam_t = 0;
-  // activation workspace for the current (nimg, nx, ny)
-  int ws_nimg = 0, ws_nx = 0, ws_ny = 0;
-  std::vector<void*> bufs;
-  float* scratch = nullptr;       // wgrad partials
-  int64_t scratch_n = 0;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;   // stream of set-up and Adam updates
 };
 
 int64_t unet_param_count();
 void unet_init(UNet& net, uint64_t seed, hipStream_t s);
 void unet_free(UNet& net);
 void unet_sync_bf16(UNet& net);
-// x: [nimg][nx*ny][16] bf16 input; y: [nimg][nx*ny][2] fp32 output
-void unet_forward(UNet& net, int nimg, int nx, int ny, const bf16* x, float* y, bool keep);
-// after unet_forward(keep=true): dy -> net.grad (overwritten), returns nothing
-void unet_backward(UNet& net, int nimg, int nx, int ny, const bf16* x, const float* dy);
+// x: [nimg][nx*ny][16] bf16 input; y: [nimg][nx*ny][2] fp32 output.  Reads
+// only the weights; activations go to ws, launches to stream s.
+void unet_forward(const UNet& net, UNetWork& ws, hipStream_t s, int nimg, int nx, int ny, const bf16* x, float* y);
+// after unet_forward with the same ws and input: dy -> net.grad (overwritten)
+void unet_backward(UNet& net, UNetWork& ws, hipStream_t s, int nimg, int nx, int ny, const bf16* x, const float* dy);
 void unet_adam(UNet& net, float lr, float b1, float b2, float eps);
 
 // solver-side helpers: complex residual block -> network input, and back
@@ -65,6 +77,7 @@ void unet_mse(int64_t n, const float* y, const float* t, float* dy, float* loss_
 // ---- C-ABI objects (opaque to callers) ----
 struct hh_unet {
   hh::UNet net;
+  hh::UNetWork ws;  // workspace of this handle's own hooks (training, forward, eval)
   hipStream_t stream = nullptr;
   // staging for the host-pointer hooks and the solver's inference
   hh::bf16* x16 = nullptr;

@@ -469,27 +469,39 @@ static Dims dims_of(int nimg, int nx, int ny) {
   return d;
 }
 
-// buffers: [0, A_NUM) activations, [A_NUM, 2 A_NUM) gradients, then dpre, head out, dy
-static void ensure_ws(UNet& net, int nimg, int nx, int ny) {
-  if (net.ws_nimg == nimg && net.ws_nx == nx && net.ws_ny == ny) return;
-  for (void* p : net.bufs)
+void UNetWork::release() {
+  for (void* p : bufs)
     if (p) (void)hipFree(p);
-  net.bufs.assign(2 * A_NUM + 3, nullptr);
-  Dims d = dims_of(nimg, nx, ny);
-  for (int a = 0; a < A_NUM; ++a) {
-    net.bufs[a] = dmal<bf16>((size_t)d.T[kActLevel[a]] * kActC[a]);
-    net.bufs[A_NUM + a] = dmal<bf16>((size_t)d.T[kActLevel[a]] * kActC[a]);
-  }
-  net.bufs[2 * A_NUM] = dmal<bf16>((size_t)d.T[0] * 64);      // dpre scratch
-  net.bufs[2 * A_NUM + 1] = dmal<bf16>((size_t)d.T[0] * 16);  // d(head input)
-  net.bufs[2 * A_NUM + 2] = dmal<float>((size_t)d.T[0] * 2);  // unused spare
-  net.ws_nimg = nimg;
-  net.ws_nx = nx;
-  net.ws_ny = ny;
+  bufs.clear();
+  if (scratch) (void)hipFree(scratch);
+  scratch = nullptr;
+  scratch_n = 0;
+  nimg = nx = ny = 0;
 }
 
-static bf16* act(UNet& net, int a) { return (bf16*)net.bufs[a]; }
-static bf16* grd(UNet& net, int a) { return (bf16*)net.bufs[A_NUM + a]; }
+// buffers: [0, A_NUM) activations, [A_NUM, 2 A_NUM) gradients, then dpre and
+// d(head input).  Reallocated when the shape grows; a smaller batch of the same
+// grid reuses them (launches index with their own nimg).
+static void ensure_ws(UNetWork& ws, int nimg, int nx, int ny) {
+  if (ws.nx == nx && ws.ny == ny && nimg <= ws.nimg && !ws.bufs.empty()) return;
+  for (void* p : ws.bufs)
+    if (p) (void)hipFree(p);
+  ws.bufs.assign(2 * A_NUM + 2, nullptr);
+  ws.nimg = ws.nx = ws.ny = 0;
+  Dims d = dims_of(nimg, nx, ny);
+  for (int a = 0; a < A_NUM; ++a) {
+    ws.bufs[a] = dmal<bf16>((size_t)d.T[kActLevel[a]] * kActC[a]);
+    ws.bufs[A_NUM + a] = dmal<bf16>((size_t)d.T[kActLevel[a]] * kActC[a]);
+  }
+  ws.bufs[2 * A_NUM] = dmal<bf16>((size_t)d.T[0] * 64);      // dpre scratch
+  ws.bufs[2 * A_NUM + 1] = dmal<bf16>((size_t)d.T[0] * 16);  // d(head input)
+  ws.nimg = nimg;
+  ws.nx = nx;
+  ws.ny = ny;
+}
+
+static bf16* act(UNetWork& ws, int a) { return (bf16*)ws.bufs[a]; }
+static bf16* grd(UNetWork& ws, int a) { return (bf16*)ws.bufs[A_NUM + a]; }
 
 static void conv_launch(const bf16* in, int cin, const bf16* w, const float* bias, bf16* out, int cout, int nimg, int nx,
                         int ny, int relu, hipStream_t s) {
@@ -547,41 +559,37 @@ void unet_init(UNet& net, uint64_t seed, hipStream_t s) {
 }
 
 void unet_free(UNet& net) {
-  for (void* p : net.bufs)
-    if (p) (void)hipFree(p);
-  net.bufs.clear();
   for (int l = 0; l < UNET_NCONV; ++l) {
     if (net.wb[l]) (void)hipFree(net.wb[l]);
     if (net.wt[l]) (void)hipFree(net.wt[l]);
   }
-  for (float* p : {net.flat, net.grad, net.adam_m, net.adam_v, net.scratch})
+  for (float* p : {net.flat, net.grad, net.adam_m, net.adam_v})
     if (p) (void)hipFree(p);
   net = UNet{};
 }
 
-void unet_forward(UNet& net, int nimg, int nx, int ny, const bf16* x, float* y, bool /*keep*/) {
-  ensure_ws(net, nimg, nx, ny);
-  hipStream_t s = net.stream;
+void unet_forward(const UNet& net, UNetWork& ws, hipStream_t s, int nimg, int nx, int ny, const bf16* x, float* y) {
+  ensure_ws(ws, nimg, nx, ny);
   Dims d = dims_of(nimg, nx, ny);
   auto conv = [&](int l) {
     const ConvSpec& c = kSpec[l];
     const int lv = c.level;
-    const bf16* in = kIn[l] < 0 ? x : act(net, kIn[l]);
-    conv_launch(in, c.ci, net.wb[l], net.b[l], act(net, kOut[l]), c.co, nimg, d.nx[lv], d.ny[lv], 1, s);
+    const bf16* in = kIn[l] < 0 ? x : act(ws, kIn[l]);
+    conv_launch(in, c.ci, net.wb[l], net.b[l], act(ws, kOut[l]), c.co, nimg, d.nx[lv], d.ny[lv], 1, s);
   };
   auto down = [&](int a_in, int a_out, int lf) {
     const int C = kActC[a_in];
-    hipLaunchKernelGGL(k_down, dim3(nblk(d.T[lf + 1] * C)), dim3(256), 0, s, act(net, a_in), C, d.nx[lf], d.ny[lf],
-                       act(net, a_out), d.nx[lf + 1], d.ny[lf + 1], nimg, 1.f);
+    hipLaunchKernelGGL(k_down, dim3(nblk(d.T[lf + 1] * C)), dim3(256), 0, s, act(ws, a_in), C, d.nx[lf], d.ny[lf],
+                       act(ws, a_out), d.nx[lf + 1], d.ny[lf + 1], nimg, 1.f);
   };
   auto up = [&](int a_in, int a_out, int lc) {
     const int C = kActC[a_in];
-    hipLaunchKernelGGL(k_up, dim3(nblk(d.T[lc - 1] * C)), dim3(256), 0, s, act(net, a_in), C, d.nx[lc], d.ny[lc],
-                       act(net, a_out), d.nx[lc - 1], d.ny[lc - 1], nimg, 1.f, 0);
+    hipLaunchKernelGGL(k_up, dim3(nblk(d.T[lc - 1] * C)), dim3(256), 0, s, act(ws, a_in), C, d.nx[lc], d.ny[lc],
+                       act(ws, a_out), d.nx[lc - 1], d.ny[lc - 1], nimg, 1.f, 0);
   };
   auto cat = [&](int a, int b, int c, int lv) {
-    hipLaunchKernelGGL(k_concat, dim3(nblk(d.T[lv] * kActC[c])), dim3(256), 0, s, act(net, a), kActC[a], act(net, b),
-                       kActC[b], act(net, c), d.T[lv]);
+    hipLaunchKernelGGL(k_concat, dim3(nblk(d.T[lv] * kActC[c])), dim3(256), 0, s, act(ws, a), kActC[a], act(ws, b),
+                       kActC[b], act(ws, c), d.T[lv]);
   };
   conv(0); conv(1); down(A_S0, A_D1, 0);
   conv(2); conv(3); down(A_S1, A_D2, 1);
@@ -593,7 +601,7 @@ void unet_forward(UNet& net, int nimg, int nx, int ny, const bf16* x, float* y, 
   conv(10); conv(11);
   up(A_B5, A_U0, 1); cat(A_U0, A_S0, A_C0, 0);
   conv(12); conv(13);
-  hipLaunchKernelGGL(k_head_fwd, dim3(nblk(d.T[0])), dim3(256), 0, s, act(net, A_B6), net.hw, net.hb, y, d.T[0]);
+  hipLaunchKernelGGL(k_head_fwd, dim3(nblk(d.T[0])), dim3(256), 0, s, act(ws, A_B6), net.hw, net.hb, y, d.T[0]);
 }
 
 static int64_t pick_chunks(int64_t total, int64_t* chunk) {
@@ -606,27 +614,28 @@ static int64_t pick_chunks(int64_t total, int64_t* chunk) {
   return (total + c - 1) / c;
 }
 
-static void ensure_scratch(UNet& net, int64_t n) {
-  if (net.scratch_n >= n) return;
-  if (net.scratch) (void)hipFree(net.scratch);
-  net.scratch = dmal<float>(n);
-  net.scratch_n = n;
+static void ensure_scratch(UNetWork& ws, int64_t n) {
+  if (ws.scratch_n >= n) return;
+  if (ws.scratch) (void)hipFree(ws.scratch);
+  ws.scratch = nullptr;
+  ws.scratch_n = 0;
+  ws.scratch = dmal<float>(n);
+  ws.scratch_n = n;
 }
 
-void unet_backward(UNet& net, int nimg, int nx, int ny, const bf16* x, const float* dy) {
-  hipStream_t s = net.stream;
+void unet_backward(UNet& net, UNetWork& ws, hipStream_t s, int nimg, int nx, int ny, const bf16* x, const float* dy) {
   Dims d = dims_of(nimg, nx, ny);
-  bf16* dpre = (bf16*)net.bufs[2 * A_NUM];
+  bf16* dpre = (bf16*)ws.bufs[2 * A_NUM];
   // scratch: wgrad partials (64 chunks x largest layer) or head partials
-  ensure_scratch(net, 64LL * 9 * 128 * 64 + 64LL * 34);
+  ensure_scratch(ws, 64LL * 9 * 128 * 64 + 64LL * 34);
   const int64_t head_off = unet_param_count() - 34;
   // head
   {
     int64_t chunk;
     const int64_t nch = pick_chunks(d.T[0], &chunk);
-    hipLaunchKernelGGL(k_head_bwd, dim3((unsigned)nch), dim3(256), 0, s, act(net, A_B6), net.hw, dy, grd(net, A_B6),
-                       d.T[0], chunk, net.scratch);
-    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, s, net.scratch, (int)nch, (int64_t)34, net.grad + head_off);
+    hipLaunchKernelGGL(k_head_bwd, dim3((unsigned)nch), dim3(256), 0, s, act(ws, A_B6), net.hw, dy, grd(ws, A_B6),
+                       d.T[0], chunk, ws.scratch);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, s, ws.scratch, (int)nch, (int64_t)34, net.grad + head_off);
   }
   int64_t off[UNET_NCONV];
   {
@@ -641,36 +650,36 @@ void unet_backward(UNet& net, int nimg, int nx, int ny, const bf16* x, const flo
     const ConvSpec& c = kSpec[l];
     const int lv = c.level;
     const int64_t T = d.T[lv];
-    hipLaunchKernelGGL(k_relu_bwd, dim3(nblk(T * c.co)), dim3(256), 0, s, grd(net, kOut[l]), act(net, kOut[l]), dpre,
+    hipLaunchKernelGGL(k_relu_bwd, dim3(nblk(T * c.co)), dim3(256), 0, s, grd(ws, kOut[l]), act(ws, kOut[l]), dpre,
                        T * c.co);
-    const bf16* in = kIn[l] < 0 ? x : act(net, kIn[l]);
+    const bf16* in = kIn[l] < 0 ? x : act(ws, kIn[l]);
     int64_t chunk;
     const int64_t nch = pick_chunks(T, &chunk);
     const int64_t nw = 9LL * c.co * c.ci;
     hipLaunchKernelGGL(k_conv_wgrad, dim3((unsigned)nch, 9 * (c.co / 16) * (c.ci / 16)), dim3(64), 0, s, dpre, c.co,
-                       in, c.ci, d.nx[lv], d.ny[lv], T, chunk, net.scratch);
-    hipLaunchKernelGGL(k_sum_parts, dim3(nblk(nw)), dim3(256), 0, s, net.scratch, (int)nch, nw, net.grad + off[l]);
-    hipLaunchKernelGGL(k_colsum, dim3((unsigned)nch), dim3(256), 0, s, dpre, c.co, T, chunk, net.scratch);
-    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, s, net.scratch, (int)nch, (int64_t)c.co,
+                       in, c.ci, d.nx[lv], d.ny[lv], T, chunk, ws.scratch);
+    hipLaunchKernelGGL(k_sum_parts, dim3(nblk(nw)), dim3(256), 0, s, ws.scratch, (int)nch, nw, net.grad + off[l]);
+    hipLaunchKernelGGL(k_colsum, dim3((unsigned)nch), dim3(256), 0, s, dpre, c.co, T, chunk, ws.scratch);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, s, ws.scratch, (int)nch, (int64_t)c.co,
                        net.grad + off[l] + nw);
     if (need_din)
-      conv_launch(dpre, c.co, net.wt[l], nullptr, grd(net, kIn[l]), c.ci, nimg, d.nx[lv], d.ny[lv], 0, s);
+      conv_launch(dpre, c.co, net.wt[l], nullptr, grd(ws, kIn[l]), c.ci, nimg, d.nx[lv], d.ny[lv], 0, s);
   };
   auto split = [&](int ac, int au, int askip, int lv) {
-    hipLaunchKernelGGL(k_split, dim3(nblk(d.T[lv] * kActC[ac])), dim3(256), 0, s, grd(net, ac), kActC[au],
-                       kActC[askip], grd(net, au), grd(net, askip), 0, d.T[lv]);
+    hipLaunchKernelGGL(k_split, dim3(nblk(d.T[lv] * kActC[ac])), dim3(256), 0, s, grd(ws, ac), kActC[au],
+                       kActC[askip], grd(ws, au), grd(ws, askip), 0, d.T[lv]);
   };
   // g(b) = P^T g(u) = 4 R g(u)
   auto up_b = [&](int au, int ab, int lc) {
     const int C = kActC[au];
-    hipLaunchKernelGGL(k_down, dim3(nblk(d.T[lc] * C)), dim3(256), 0, s, grd(net, au), C, d.nx[lc - 1], d.ny[lc - 1],
-                       grd(net, ab), d.nx[lc], d.ny[lc], nimg, 4.f);
+    hipLaunchKernelGGL(k_down, dim3(nblk(d.T[lc] * C)), dim3(256), 0, s, grd(ws, au), C, d.nx[lc - 1], d.ny[lc - 1],
+                       grd(ws, ab), d.nx[lc], d.ny[lc], nimg, 4.f);
   };
   // g(s) += R^T g(d) = P g(d) / 4
   auto down_b = [&](int ad, int as, int lf) {
     const int C = kActC[ad];
-    hipLaunchKernelGGL(k_up, dim3(nblk(d.T[lf] * C)), dim3(256), 0, s, grd(net, ad), C, d.nx[lf + 1], d.ny[lf + 1],
-                       grd(net, as), d.nx[lf], d.ny[lf], nimg, 0.25f, 1);
+    hipLaunchKernelGGL(k_up, dim3(nblk(d.T[lf] * C)), dim3(256), 0, s, grd(ws, ad), C, d.nx[lf + 1], d.ny[lf + 1],
+                       grd(ws, as), d.nx[lf], d.ny[lf], nimg, 0.25f, 1);
   };
   conv_b(13, true); conv_b(12, true);
   split(A_C0, A_U0, A_S0, 0);

@@ -168,6 +168,8 @@ class _OpHandle:
         w = complex(omega)
         self.N, self.nn, self.dim = N, nn, mesh.dim
         self.omega = w
+        self.h = np.array(mesh.h, dtype=np.float64)
+        self.neumann, self.sommerfeld, self.order = bool(neumann), bool(sommerfeld), int(order)
         h = C.c_void_p()
         self.ho_beta = ho_beta
         if ho_beta is None:
@@ -180,6 +182,20 @@ class _OpHandle:
                                             self.gamma.ctypes.data, w.real, w.imag, int(bool(neumann)),
                                             int(bool(sommerfeld)), float(bl), float(bm), C.byref(h)))
         self.ptr = h
+
+    def matches(self, p: HelmholtzParam) -> bool:
+        """True if this operator was built from the fields of ``p`` (mesh,
+        slowness, attenuation, frequency and both boundary flags)."""
+        nn = [int(v) for v in p.Mesh.nodes]
+        if self.nn != nn or not np.allclose(self.h, p.Mesh.h, rtol=1e-14, atol=0.0):
+            return False
+        if self.neumann != bool(p.NeumannOnTop) or self.sommerfeld != bool(p.Sommerfeld):
+            return False
+        if self.omega != complex(p.omega):
+            return False
+        m = np.asarray(p.m, dtype=np.float64).ravel(order="F")
+        g = np.asarray(p.gamma, dtype=np.float64).ravel(order="F")
+        return m.size == self.N and g.size == self.N and np.array_equal(m, self.m) and np.array_equal(g, self.gamma)
 
     def __del__(self):
         p = getattr(self, "ptr", None)

@@ -10,9 +10,10 @@ The multigrid hierarchy is the rediscretised shifted operator on every level
 (the reference's commented geometric path, :51-62), with full-weighting
 restriction, bilinear/trilinear prolongation, damped Jacobi and an FP64
 direct coarsest solve; the outer method is flexible GMRES(inner).  The
-``ShiftedHT`` argument of ``solveLinearSystem`` is accepted for signature
-compatibility and only checked for shape: the GPU path is matrix-free from
-``helmParam``.
+``ShiftedHT`` argument of ``solveLinearSystem`` selects the operator as the
+reference's does (its first solve builds from it): a matrix-free operator is
+used as it is, a sparse matrix is checked against ``helmParam``, and an
+operator that does not belong to ``helmParam`` is refused.
 """
 from __future__ import annotations
 
@@ -156,23 +157,68 @@ def _validate(s: ShiftedLaplacianMultigridSolver):
         raise ValueError("MG.basis must be 'c128' or 'c64'")
 
 
-def _ensure_handle(s: ShiftedLaplacianMultigridSolver, outer=None):
-    """``outer``: a spread operator from GetHelmholtzOperatorHO passed as
-    ShiftedHT becomes the solve's outer operator; otherwise the 5/7-point
-    operator of helmParam (mirror-ghost BC) is built."""
+def _resolve_outer(s: ShiftedLaplacianMultigridSolver, ShiftedHT):
+    """The operator a solve runs on, from the ``ShiftedHT`` argument.
+
+    The reference builds its hierarchy from the matrix it is given
+    (``MGsetup(ShiftedHT, ...)``, src/ShiftedLaplacianMultigridSolver.jl:65), so
+    the argument is not ignored here either:
+
+    * a matrix-free operator from ``GetHelmholtzOperator`` (H, or the shifted
+      H + GetHelmholtzShiftOP(m, w, shift[1]) of the reference's tests) or
+      ``GetHelmholtzOperatorHO`` is used as it is, whatever its orderNeumannBC;
+      it must have been built from helmParam's fields, and a shift other than
+      0 or the solver's finest-level shift is refused;
+    * a sparse matrix (the reference's ``sparse(SH')``) is checked with one
+      product against helmParam's shifted operator (orderNeumannBC 2, then 1);
+      a matrix that matches neither is refused;
+    * ``None``: helmParam's 5/7-point operator with orderNeumannBC = 2.
+
+    Returns (key, factory) where factory() builds or returns the _OpHandle."""
     p = s.helmParam
-    if outer is not None and getattr(outer._h, "ho_beta", None) is not None:
-        key = ("ho", id(outer._h))
-        if s._op is None or s._op_key != key:
-            _drop_handle(s)
-            s._op = outer._h
-            s._op_key = key
-    else:
-        key = (id(p), id(p.m), id(p.gamma), complex(p.omega), bool(p.NeumannOnTop), bool(p.Sommerfeld))
-        if s._op is None or s._op_key != key:
-            _drop_handle(s)
-            s._op = _OpHandle(p.Mesh, p.m, p.omega, p.gamma, p.NeumannOnTop, p.Sommerfeld, 2)
-            s._op_key = key
+    if isinstance(ShiftedHT, HelmholtzOperator):
+        h = ShiftedHT._h
+        if not h.matches(p):
+            raise ValueError("ShiftedHT was built from other fields than helmParam (mesh, m, gamma, omega or "
+                             "boundary flags differ)")
+        sh0 = float(np.asarray(s.shift, dtype=float).ravel()[0])
+        if ShiftedHT.shift != 0.0 and abs(ShiftedHT.shift - sh0) > 1e-12 * max(1.0, abs(sh0)):
+            raise ValueError(f"ShiftedHT carries shift {ShiftedHT.shift}; expected 0 or the solver's shift {sh0}")
+        return ("op", id(h)), (lambda: h)
+    if ShiftedHT is not None and hasattr(ShiftedHT, "tocsr") and hasattr(ShiftedHT, "shape"):
+        key = ("sparse", id(ShiftedHT), id(p), id(p.m), id(p.gamma))
+        return key, (lambda: _match_sparse(s, ShiftedHT))
+    key = ("param", id(p), id(p.m), id(p.gamma), complex(p.omega), bool(p.NeumannOnTop), bool(p.Sommerfeld))
+    return key, (lambda: _OpHandle(p.Mesh, p.m, p.omega, p.gamma, p.NeumannOnTop, p.Sommerfeld, 2))
+
+
+def _match_sparse(s: ShiftedLaplacianMultigridSolver, A):
+    """helmParam's operator whose shifted form equals A^H (A = sparse(SH'))
+    on a random vector, trying orderNeumannBC 2 then 1."""
+    p = s.helmParam
+    N = int(np.prod(p.Mesh.nodes))
+    rng = np.random.default_rng(12345)
+    x = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    want = A.tocsr().conj().T @ x
+    sh0 = float(np.asarray(s.shift, dtype=float).ravel()[0])
+    for order in (2, 1):
+        h = _OpHandle(p.Mesh, p.m, p.omega, p.gamma, p.NeumannOnTop, p.Sommerfeld, order)
+        for shift in (sh0, 0.0):
+            got = HelmholtzOperator(h, shift).apply(x)
+            if np.linalg.norm(got - want) <= 1e-10 * np.linalg.norm(want):
+                return h
+    raise ValueError("ShiftedHT does not match helmParam's shifted operator (sparse(SH') with SH = H + "
+                     "GetHelmholtzShiftOP(m, w, shift[1]) and orderNeumannBC 1 or 2)")
+
+
+def _ensure_handle(s: ShiftedLaplacianMultigridSolver, ShiftedHT=None):
+    """Bind the solver to the operator ``ShiftedHT`` resolves to
+    (``_resolve_outer``) and create the library solver on it."""
+    key, factory = _resolve_outer(s, ShiftedHT)
+    if s._op is None or s._op_key != key:
+        _drop_handle(s)
+        s._op = factory()
+        s._op_key = key
     if s._handle is None:
         mg = s.MG
         sw = [_sweeps(mg, l) for l in range(int(mg.levels))]
@@ -233,7 +279,7 @@ def solveLinearSystem(ShiftedHT, B, param: ShiftedLaplacianMultigridSolver, doTr
             return np.zeros(Ba.shape, dtype=Ba.dtype), param
         X = np.empty_like(B2, order="F")
         bptr, xptr, loc = B2.ctypes.data, X.ctypes.data, L.HH_HOST
-    _ensure_handle(param, ShiftedHT if isinstance(ShiftedHT, HelmholtzOperator) else None)
+    _ensure_handle(param, ShiftedHT)
     iters = np.zeros(nrhs, dtype=np.int32)
     status = np.zeros(nrhs, dtype=np.int32)
     res = np.zeros(nrhs, dtype=np.float64)

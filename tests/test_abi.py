@@ -303,3 +303,41 @@ def test_ctypes_argtypes_match_the_header():
             assert len(at) == n, (mt.group(1), len(at), n)
             checked += 1
     assert checked >= 30
+
+
+def test_julia_shim_is_a_jinv_plugin():
+    """The shim plugs into jInv as the reference's solver does
+    (src/Helmholtz.jl:5-8, src/ShiftedLaplacianMultigridSolver.jl:4,17,32,104):
+    the solver type subtypes jInv.LinearSolvers.AbstractSolver, and
+    solveLinearSystem / copySolver / clear! are imported from jInv before the
+    shim defines its methods, so they extend jInv's generic functions (a
+    module-local function of the same name would not be reached by a jInv
+    caller's dispatch)."""
+    jl = open(os.path.join(ROOT, "helmholtz.jl_amd", "julia", "HelmholtzAMD.jl")).read()
+    code = "\n".join(l.split("#", 1)[0] for l in jl.splitlines())
+    assert re.search(r"^mutable struct ShiftedLaplacianMultigridSolverAMD <: AbstractSolver$", code, re.M)
+    imports = {
+        "AbstractSolver": r"^import jInv\.LinearSolvers\.AbstractSolver$",
+        "solveLinearSystem": r"^import jInv\.LinearSolvers\.solveLinearSystem$",
+        "copySolver": r"^import jInv\.LinearSolvers\.copySolver$",
+        "clear!": r"^import jInv\.Utils\.clear!$",
+    }
+    pos = {}
+    for name, pat in imports.items():
+        m = re.search(pat, code, re.M)
+        assert m, f"missing import of jInv's {name}"
+        pos[name] = m.start()
+    defs = {
+        "solveLinearSystem": r"^function solveLinearSystem\(ShiftedHT, B, param::ShiftedLaplacianMultigridSolverAMD,"
+                             r" doTranspose::Int64=0\)",
+        "clear!": r"^function clear!\(s::ShiftedLaplacianMultigridSolverAMD\)",
+        "copySolver": r"^copySolver\(s::ShiftedLaplacianMultigridSolverAMD\) =",
+    }
+    for name, pat in defs.items():
+        m = re.search(pat, code, re.M)
+        assert m, f"no method of {name} for the AMD solver"
+        assert pos[name] < m.start(), f"{name} is defined before it is imported from jInv"
+    # ShiftedHT is read when the hierarchy is built (checked, not ignored)
+    assert re.search(r"ensure_handle!\(param, ShiftedHT\)", code)
+    assert re.search(r"s\.op = matching_operator\(s, ShiftedHT\)", code)
+    assert "does not match helmParam's shifted operator" in code

@@ -584,3 +584,40 @@ def test_seg_fixture_matches_the_reference_file():
         pytest.skip("reference checkout not present")
     f = np.load(os.path.join(GOLDEN, "seg_salt_2d.npz"))
     np.testing.assert_array_equal(f["v"].astype(np.float64), np.loadtxt(src))
+
+
+def test_spread_operator_interior_weights_known_answer():
+    """Interior 9-point row of the spread Laplacian G'((1-b) Gs + b G) and
+    5-point row of the mass average M at b = 2/3, typed from the algebra of
+    src/PlainNodalLaplacian.jl:59-66,106-120: Gs averages transversally with
+    av3term(., 0.5) = [1/4, 1/2, 1/4], so the transverse weights are
+    w(+-1) = (1 - b)/4 = 1/12 and w(0) = (1 - b)/2 + b = 5/6.  With
+    A = 1/h1^2, B = 1/h2^2 the row is: centre (5/3)(A + B); x-neighbours
+    -(5/6)A + (1/6)B; y-neighbours (1/6)A - (5/6)B; corners -(1/12)(A + B).
+    M = (av3term_y(b) (x) I + I (x) av3term_x(b))/2: centre 2/3, the four
+    neighbours 1/12.  (av3term's default weight 5/6 in Gs would make the
+    corners -(1/36)(A + B).)"""
+    mesh = hr.get_regular_mesh([0.0, 1.0, 0.0, 1.3], [8, 6])
+    h1, h2 = mesh.h
+    A, B = 1.0 / h1**2, 1.0 / h2**2
+    Lap, M = hr.get_spread_nodal_laplacian_and_mass(mesh, 2.0 / 3.0)
+    n1 = int(mesh.nodes[0])
+    i, j = 4, 3  # 0-based interior node
+    row = Lap.getrow(i + n1 * j).toarray().ravel()
+    mrow = M.getrow(i + n1 * j).toarray().ravel()
+    want = {(0, 0): 5.0 / 3.0 * (A + B), (1, 0): -5.0 / 6.0 * A + B / 6.0, (-1, 0): -5.0 / 6.0 * A + B / 6.0,
+            (0, 1): A / 6.0 - 5.0 / 6.0 * B, (0, -1): A / 6.0 - 5.0 / 6.0 * B}
+    for di in (-1, 1):
+        for dj in (-1, 1):
+            want[(di, dj)] = -(A + B) / 12.0
+    got = {}
+    for k in np.flatnonzero(row):
+        got[(int(k % n1) - i, int(k // n1) - j)] = row[k]
+    assert set(got) == set(want)
+    for key, v in want.items():
+        assert got[key] == pytest.approx(v, rel=1e-13), key
+    mwant = {(0, 0): 2.0 / 3.0, (1, 0): 1.0 / 12.0, (-1, 0): 1.0 / 12.0, (0, 1): 1.0 / 12.0, (0, -1): 1.0 / 12.0}
+    mgot = {(int(k % n1) - i, int(k // n1) - j): mrow[k] for k in np.flatnonzero(mrow)}
+    assert set(mgot) == set(mwant)
+    for key, v in mwant.items():
+        assert mgot[key] == pytest.approx(v, rel=1e-14), key
