@@ -14,19 +14,28 @@ the configured complex128 outer iteration) is reported beside it as
 ``c64_basis``, not as ``value`` (``--no-c64-basis`` skips it).
 
 A step = one solve of all 256 sources in one batched call (complex64
-preconditioner, inputs resident in HBM, stencil launch shapes autotuned) with
-the learned U-Net coarse-grid correction at the 257x129 level, chosen against
-the plain cycle by the true residual of the first two restart cycles; the
-network is trained first on pairs from plain SL-MG solves on seeded 513x513
-media (training time reported, not timed).  The same batched solve with the
-plain cycle, and the plain method (complex128 preconditioner, one source per
-call) on a subset of sources, are reported beside it; the line's ``value`` is
-the learned+batched ms/RHS.  ``roofline`` times every launch of the dominant
-kernel inside the timed solves with HIP events; ``cpu_baseline`` solves one
-source with the oracle's C/OpenMP restatement of the plain method;
-``three_d`` times the 3-D case (193 x 193 x 97, 32 sources) with a complex64
-and a complex128 preconditioner (``--no-3d`` skips it); ``stream`` solves an
-uneven stream of 200 independent problems (``--no-stream`` skips it).
+preconditioner, inputs resident in HBM, stencil launch shapes autotuned) by
+the learned+batched solver: the U-Net coarse-grid correction at the 257x129
+level is trained first on pairs from plain SL-MG solves on seeded 513x513
+media (training time reported, not timed) and chosen against the plain cycle
+by the true residual of the first solve's restart cycles 0 and 1; ``value`` is
+that solver's timed ms/RHS and ``value_pipeline`` says which cycle ran (the
+plain cycle when the network lost its trial).  Reported beside it:
+``learned_trial`` and ``learned_every_cycle`` (the network forced into every
+restart cycle, the ablation); the batched plain-cycle solve; its set-up time;
+the plain method (complex128 preconditioner, one source per call) on sampled
+sources and on the small 2-D workload (``c1_plain``); ``accuracy``, the error
+of each of the 256 results against a complex128 1e-10 reference checked
+against SciPy's sparse direct solve, with that solve's time as context;
+``roofline`` (the Arnoldi pass 2 family, the largest share of an iteration)
+and ``roofline_kernels`` (pass 1 and the fine-level multigrid legs), every
+launch event-timed inside full solves against its algorithmic bytes;
+``cpu_baseline`` (the oracle's C/OpenMP restatement of the plain method on
+the host's cores, one source) with ``host`` (CPU model, core counts);
+``three_d`` (193 x 193 x 97, 32 sources, complex64 vs complex128
+preconditioner, and the plain method one source per call; ``--no-3d`` skips
+it); ``stream`` and ``stream_learned``, an uneven stream of 200 independent
+problems without and with the trained network (``--no-stream`` skips them).
 
 Usage: python bench.py [--gpus 1] [--steps 2] [--warmup 1] [--plain-rhs 4]
 """
@@ -105,14 +114,19 @@ def c3_lines(log, c64_basis=True):
         mg.prec = prec
         mg.basis = basis
         s = ha.getShiftedLaplacianMultigridSolver(hp, mg, 0.5, "GMRES", 8)
-        t, st = timed_solves(ha, s, Bd, 1, 1, log, f"3-D {key}")
+        t, st, setup, _ = timed_solves(ha, s, Bd, 1, 1, log, f"3-D {key}")
         out[f"{key}_ms_per_rhs"] = round(1e3 * t / B.shape[1], 3)
         out[f"{key}_max_iters"] = int(st["max_iters"])
         out[f"{key}_max_true_res"] = float(max(st["true_res"]))
+        out[f"{key}_setup_s"] = round(setup, 3)
+    # the plain method: complex128 preconditioner, one source per call
+    ms, its, res, setup, _ = plain_single_rhs(ha, hp, B, [0, 13], 5, "W", 8, [0.5] * 5, log=log, name="3-D plain")
+    out["plain_c128_single_rhs"] = {"ms_per_rhs": round(ms, 2), "iters": its, "max_true_res": max(res),
+                                    "setup_s": round(setup, 3), "sources": [0, 13]}
     return out
 
 
-def stream_line(log, count=200, workers=8, basis="c128"):
+def stream_line(log, count=200, workers=8, basis="c128", graph=False, net=None):
     """The uneven stream: ``count`` independent 2-D problems, grids drawn from
     {129, 257, 513, 1025} x {129, 257, 513}, 6-12 points per wavelength,
     1-64 sources each, seeded layered+body media, complex64 preconditioner,
@@ -134,7 +148,7 @@ def stream_line(log, count=200, workers=8, basis="c128"):
         nrhs = int(rng.integers(1, 65))
         xs = rng.integers(5, n[0] - 5, size=nrhs)
         probs.append({"param": hp, "B": ha.point_sources(M, [(int(x), 2) for x in xs]), "prec": "c64",
-                      "max_outer": 200, "basis": basis})
+                      "max_outer": 200, "basis": basis, "graph": graph, "net": net, "learn_level": 2})
         nodes += m.size * nrhs
     log(f"stream: {count} problems built in {time.perf_counter() - t0:.1f} s")
     t0 = time.perf_counter()
@@ -143,11 +157,14 @@ def stream_line(log, count=200, workers=8, basis="c128"):
     rhs = sum(int(p["B"].shape[1]) for p in probs)
     res = max(float(np.max(info["true_res"])) for _, info in out)
     ok = all(info["code"] == 0 for _, info in out) and res <= 1e-6
-    log(f"stream: {rhs} right-hand sides in {dt:.2f} s, max true residual {res:.2e}")
+    fell = sum(1 for _, info in out if info["fell_back"])
+    log(f"stream: {rhs} right-hand sides in {dt:.2f} s, max true residual {res:.2e}"
+        + (f", network kept by {count - fell} of {count} problems" if net is not None else ""))
     return {"workload": f"{count} 2-D problems, grids {{129..1025}} x {{129..513}}, 6-12 ppw, 1-64 sources each",
             "problems": count, "rhs": rhs, "node_rhs": int(nodes), "seconds": round(dt, 3),
             "ms_per_rhs": round(1e3 * dt / rhs, 3), "max_true_res": res, "all_converged": bool(ok),
-            "workers": workers, "basis": basis, "buffers": "host (PCIe-inclusive)"}
+            "workers": workers, "basis": basis, "graphs": bool(graph), "buffers": "host (PCIe-inclusive)",
+            "learned": None if net is None else {"level": 2, "fell_back": fell, "kept": count - fell}}
 
 
 def make_solver(hp, prec: str, levels=6, cycle="W", tol=1e-6, fast=False, basis="c128"):
@@ -164,8 +181,16 @@ def make_solver(hp, prec: str, levels=6, cycle="W", tol=1e-6, fast=False, basis=
 
 
 def timed_solves(ha, solver, Bd, warmup, steps, log, name, before_timed=None):
+    """``warmup`` untimed solves (the first builds the hierarchy: its set-up
+    time is returned separately), then ``steps`` timed solves.  Returns (mean
+    seconds per timed solve, the last solve's stats, set-up seconds of the
+    first call, the last solution block)."""
+    setup = None
+    X = None
     for k in range(warmup):
         X, solver = ha.solveLinearSystem(None, Bd, solver)
+        if setup is None:
+            setup = float(solver.last_stats["setup_time"])
         log(f"{name} warmup {k}: max iters {solver.last_stats['max_iters']}, "
             f"fell back {solver.last_stats['fell_back']}")
     if before_timed is not None:
@@ -175,46 +200,149 @@ def timed_solves(ha, solver, Bd, warmup, steps, log, name, before_timed=None):
         t0 = time.perf_counter()
         X, solver = ha.solveLinearSystem(None, Bd, solver)
         times.append(time.perf_counter() - t0)
+        if setup is None:
+            setup = float(solver.last_stats["setup_time"])
         log(f"{name} step {k}: {times[-1]:.3f} s, max iters {solver.last_stats['max_iters']}, "
             f"learned used {solver.last_stats['learned_used']}, fell back {solver.last_stats['fell_back']}")
-    return float(np.mean(times)), solver.last_stats
+    return float(np.mean(times)), solver.last_stats, setup, X
 
 
-# The dominant kernel of the batched solve (rocprofv3 kernel trace of this
-# benchmark): the fused up leg of the multigrid cycle on the fine grid.
-ROOFLINE_KERNEL = ("up", 0)
-ROOFLINE_NAME = "k_up_fused<float> on the 1025x513 grid (x += P e_c, two damped-Jacobi sweeps)"
+# Kernels whose launches inside the timed solves are event-timed against their
+# algorithmic bytes (hh_solver_time_kernel).  By share of a FGMRES iteration
+# of the batched solve (rocprofv3 kernel trace) the Arnoldi passes lead: pass 2
+# (w' = w - V h, the k_update_lazy_hz family) then pass 1 (the dots,
+# k_dots_hz); the largest multigrid kernel is the fine-level up leg.
+ROOFLINE_KERNELS = [
+    ("update", 0, "k_update_lazy_hz<*> (Arnoldi pass 2: w = H z formed again, w' = w - V h, ||w'||)"),
+    ("dots", 0, "k_dots_hz<*> (Arnoldi pass 1: w = H z from the complex64 z, h = V^H w, ||w||)"),
+    ("up", 0, "k_up_fused<float> on the 1025x513 grid (x += P e_c, two damped-Jacobi sweeps)"),
+    ("mid", 0, "k_mid_fused<float> on the 1025x513 grid (x2 recomputed, x' = x2 + P e_c, r, R r)"),
+    ("down", 0, "k_down_fused<float> on the 1025x513 grid (two sweeps from zero, r, R r)"),
+]
 
 
-def roofline_from_solver(ha, solver, nsrc: int, N: int, Nc: int):
-    """HIP-event timing of every launch of the roofline kernel inside the timed
-    solves (on the solver's stream), against its algorithmic bytes: per active
-    RHS read x, b (8 B each), the coarse correction (8 B per coarse node),
-    write x (8 B); plus the shared complex64 diagonal and Jacobi factor (8 B
-    each per node)."""
+def kernel_roofline(ha, solver, Bd, kernel, level, name):
+    """Every launch of one kernel in a full batched solve, bracketed by HIP
+    events on the solver's stream: Σ algorithmic bytes / Σ duration."""
+    ha.time_kernel(solver, kernel, level)
+    _, solver = ha.solveLinearSystem(None, Bd, solver)
     t = ha.kernel_time(solver)
+    ha.time_kernel(solver, "none", 0)
     ms, n, byt = t["ms"], t["launches"], t["bytes"]
     achieved = byt / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
-    full = nsrc * (3 * N * 8 + Nc * 8) + 2 * N * 8
-    traffic = _pmc_traffic()
-    return {"bound": "hbm", "kernel": ROOFLINE_NAME, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "launches": n, "avg_launch_ms": round(ms / max(n, 1), 5),
-            "alg_bytes_per_launch": round(byt / max(n, 1)), "alg_bytes_full_batch_launch": full}
+    traffic = _pmc_traffic(name.split(" ")[0])
+    return {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "launches": n,
+            "avg_launch_ms": round(ms / max(n, 1), 5), "alg_bytes_per_launch": round(byt / max(n, 1))}
 
 
-def _pmc_traffic():
-    """HBM bytes per full-batch launch of the roofline kernel from the two
-    rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) summarised in
-    profiles/pmc_up_fused_c2_1025x513_256rhs.md, each scaled by the counter's
-    reading for a device copy of known size; None when that summary is absent."""
-    f = os.path.join(ROOT, "profiles", "pmc_up_fused_c2_1025x513_256rhs.md")
-    if not os.path.exists(f):
-        return None
-    for line in open(f):
-        if line.startswith("traffic_bytes_per_launch ="):
-            return float(line.split("=", 1)[1])
+def _pmc_traffic(kernel: str):
+    """HBM bytes per launch of ``kernel`` from the two rocprofv3 --pmc runs
+    (FETCH_SIZE, WRITE_SIZE) summarised in profiles/, each scaled by the
+    counter's reading for a device copy of known size; None when no summary
+    gives it.  Summary lines read ``traffic_bytes_per_launch[<kernel>] = <B>``."""
+    for f in sorted(os.listdir(os.path.join(ROOT, "profiles"))):
+        if not f.endswith(".md"):
+            continue
+        for line in open(os.path.join(ROOT, "profiles", f)):
+            if line.startswith(f"traffic_bytes_per_launch[{kernel}] ="):
+                return float(line.split("=", 1)[1])
     return None
+
+
+def host_info():
+    """The host the CPU lines ran on: CPU model, logical CPUs, CPUs this process may use."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
+
+
+def plain_single_rhs(ha, hp, B, cols, levels, cycle, inner, shifts, wlevels=0, log=None, name="plain"):
+    """The plain method: complex128 preconditioner, one source per call, the
+    first call (hierarchy set-up) untimed.  Returns (ms/RHS, iterations, true
+    residuals, set-up seconds, solutions)."""
+    from helmholtz_amd.device import DeviceArray
+    mg = ha.getMGparam(complex, int, levels, 1, 400, 1e-6, "Jac", 0.8, 2, 2, cycle, "NoMUMPS", 0.5, 0.0)
+    mg.prec = "c128"
+    mg.wlevels = wlevels
+    s = ha.getShiftedLaplacianMultigridSolver(hp, mg, np.array(shifts[:levels]), "GMRES", inner, False)
+    _, s = ha.solveLinearSystem(None, DeviceArray.from_numpy(B[:, [cols[0]]]), s)
+    setup = float(s.last_stats["setup_time"])
+    ts, its, res, xs = [], [], [], []
+    for c in cols:
+        bcol = DeviceArray.from_numpy(B[:, [c]])
+        t0 = time.perf_counter()
+        X, s = ha.solveLinearSystem(None, bcol, s)
+        ts.append(time.perf_counter() - t0)
+        its.append(int(s.last_stats["max_iters"]))
+        res.append(float(s.last_stats["true_res"][0]))
+        xs.append(X.to_numpy()[:, 0])
+        if log:
+            log(f"{name} rhs {c}: {ts[-1]:.3f} s, {its[-1]} iters, res {res[-1]:.2e}")
+    return 1e3 * float(np.mean(ts)), its, res, setup, xs
+
+
+def c1_problem():
+    """The plain-method workload: 129 x 129 on the unit square, v = 2 + 0.3 G
+    (G a unit-std Gaussian random field, correlation length 0.1, seed 1),
+    w = getMaximalFrequency (10 points per wavelength), attenuation 0.01 w
+    plus a 10-cell absorbing layer of amplitude w, one source at the centre."""
+    import helmholtz_amd as ha
+    from helmholtz_amd import media
+    v = media.grf_velocity_2d((129, 129), seed=1)
+    m = (1.0 / v**2).ravel(order="F")
+    M = ha.getRegularMesh([0.0, 1.0, 0.0, 1.0], [128, 128])
+    w = ha.getMaximalFrequency(m, M)
+    _, gamma = ha.GetHelmholtzOperator(M, m, w, 0.01 * w * np.ones(m.size), False, [10, 10], w, True)
+    hp = ha.HelmholtzParam(M, gamma.ravel(order="F"), m, w, False, True)
+    return hp, ha.point_sources(M, [(65, 65)])
+
+
+def accuracy(ha, hp, B, Xb, cols, plain_x, log):
+    """Error of every batched result against a float64 reference: the
+    complex128 preconditioner with FGMRES(10) to a 1e-10 true residual on all
+    right-hand sides, itself checked against SciPy's sparse direct solve
+    (splu, one thread) on the sampled columns ``cols``; the plain method's
+    sampled solutions against splu too.  Also times splu as context."""
+    import scipy.sparse.linalg as spla
+    sys.path.insert(0, ROOT)
+    from oracle import helmholtz_ref as hr
+    from helmholtz_amd.device import DeviceArray
+    M = hr.get_regular_mesh(hp.Mesh.domain, hp.Mesh.n)
+    H = hr.get_helmholtz_operator7(M, hp.m, hp.omega, hp.gamma, True, True).tocsc()
+    t0 = time.perf_counter()
+    lu = spla.splu(H)
+    t_fac = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    Xs = lu.solve(np.ascontiguousarray(B[:, cols]))
+    t_sol = (time.perf_counter() - t0) / len(cols)
+    mg = ha.getMGparam(complex, int, 6, 1, 400, 1e-10, "Jac", 0.8, 2, 2, "W", "NoMUMPS", 0.5, 0.0)
+    mg.prec = "c128"
+    mg.wlevels = 3
+    sr = ha.getShiftedLaplacianMultigridSolver(hp, mg, np.array(SHIFTS[:6]), "GMRES", 10, False)
+    Xr, sr = ha.solveLinearSystem(None, DeviceArray.from_numpy(B), sr)
+    Xr = Xr.to_numpy()
+    Xb = Xb.to_numpy()
+    nref = np.linalg.norm(Xr, axis=0)
+    err = np.linalg.norm(Xb - Xr, axis=0) / nref
+    ref_vs_splu = [float(np.linalg.norm(Xr[:, c] - Xs[:, k]) / np.linalg.norm(Xs[:, k])) for k, c in enumerate(cols)]
+    plain_err = [float(np.linalg.norm(x - Xs[:, k]) / np.linalg.norm(Xs[:, k])) for k, x in enumerate(plain_x)]
+    log(f"accuracy: batched vs 1e-10 reference max {err.max():.2e}; reference vs splu max {max(ref_vs_splu):.2e}")
+    return {"reference": "complex128 preconditioner, FGMRES(10) to 1e-10 true residual, all 256 sources; "
+                         "checked against scipy splu on the sampled sources",
+            "batched_err_max": float(err.max()), "batched_err_median": float(np.median(err)),
+            "batched_err_per_rhs": [float(f"{e:.3e}") for e in err],
+            "reference_vs_splu": ref_vs_splu, "sampled_sources": [int(c) for c in cols],
+            "plain_err_vs_splu": plain_err,
+            "splu": {"factor_s": round(t_fac, 2), "solve_s_per_rhs": round(t_sol, 3), "threads": 1,
+                     "ms_per_rhs_256": round(1e3 * (t_fac + B.shape[1] * t_sol) / B.shape[1], 2),
+                     "note": "SciPy SuperLU direct solve of the same 1025x513 operator on the host (context)"}}
 
 
 def calibration_copy(Bd, log):
@@ -282,11 +410,17 @@ def main():
                     help="seconds the host baseline may spend on a full single-source solve")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-3d", action="store_true", help="skip the 3-D complex64 vs complex128 lines")
-    ap.add_argument("--no-stream", action="store_true", help="skip the uneven-stream line")
+    ap.add_argument("--no-stream", action="store_true", help="skip the uneven-stream lines")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the per-result error and splu lines")
     ap.add_argument("--train-media", type=int, default=40)
     ap.add_argument("--train-epochs", type=int, default=6)
     ap.add_argument("--no-c64-basis", action="store_true",
                     help="skip the solves with the FGMRES basis stored in complex64")
+    ap.add_argument("--profile-batched", type=int, default=0, metavar="CYCLES",
+                    help="profiling mode: only the batched complex64 solve of the 1025x513 workload, stopped "
+                         "after CYCLES FGMRES restart cycles (no warmup, no other lines)")
+    ap.add_argument("--profile-training", action="store_true",
+                    help="profiling mode: only the training of the learned correction (pairs from 513x513 media)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the solve shards by right-hand side "
@@ -294,19 +428,34 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     import helmholtz_amd as ha
+    from helmholtz_amd import learned, train
     from helmholtz_amd.device import DeviceArray
 
     def log(msg):
         print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
+    if args.profile_training:
+        _, info = train.train_coarse_correction(n=(513, 513), media_count=args.train_media, level=2,
+                                                epochs=args.train_epochs, log=log)
+        log(f"training: {info}")
+        return
     t = time.perf_counter()
     M, hp, H, B, w = c2_problem(args.nsrc)
     log(f"problem built in {time.perf_counter() - t:.1f} s: N={H.shape[0]} nsrc={args.nsrc} omega={w:.3f}")
     Bd = DeviceArray.from_numpy(B)
+    if args.profile_batched:
+        sp = make_solver(hp, "c64", fast=True)
+        sp.MG.maxOuterIter = args.profile_batched
+        t0 = time.perf_counter()
+        _, sp = ha.solveLinearSystem(None, Bd, sp)
+        log(f"profiling solve: {args.profile_batched} restart cycles, {sp.last_stats['max_iters']} iterations, "
+            f"{time.perf_counter() - t0:.2f} s")
+        calibration_copy(Bd, log)  # a device copy of known size: the scale of the byte counters
+        return
 
-    # ---- batched, complex64 preconditioner, plain W cycle (autotuned launches)
+    # ---- batched, complex64 preconditioner, plain cycle (autotuned launches)
     sb = make_solver(hp, "c64", fast=True)
-    t_b, st_b = timed_solves(ha, sb, Bd, args.warmup, args.steps, log, "batched")
+    t_b, st_b, setup_b, Xb = timed_solves(ha, sb, Bd, args.warmup, args.steps, log, "batched")
     res_b = np.asarray(st_b["true_res"])
     batched_ms_rhs = 1e3 * t_b / args.nsrc
     # the same solve from host buffers: B and X cross PCIe inside the call
@@ -314,66 +463,86 @@ def main():
     _, sb = ha.solveLinearSystem(None, B, sb)
     host_ms_rhs = 1e3 * (time.perf_counter() - t0) / args.nsrc
     log(f"batched from host buffers: {host_ms_rhs:.3f} ms/RHS")
+    # every launch of the leading kernels, event-timed inside full solves
+    roofs = [kernel_roofline(ha, sb, Bd, k, l, name) for k, l, name in ROOFLINE_KERNELS]
+    for r in roofs:
+        log(f"roofline: {r}")
+    del sb
 
     # ---- the same batched solve with the FGMRES basis stored in complex64
     c64b = None
     if not args.no_c64_basis:
         sc = make_solver(hp, "c64", fast=True, basis="c64")
-        t_c, st_c = timed_solves(ha, sc, Bd, args.warmup, args.steps, log, "batched c64 basis")
+        t_c, st_c, _, _ = timed_solves(ha, sc, Bd, args.warmup, args.steps, log, "batched c64 basis")
         c64b = {"ms_per_rhs": round(1e3 * t_c / args.nsrc, 4), "max_iters": int(st_c["max_iters"]),
                 "max_true_res": float(max(st_c["true_res"])),
                 "note": "FGMRES basis stored in complex64; products, sums, iterate and restart residual "
                         "float64/complex128; plain cycle"}
         del sc
 
-    # ---- learned + batched: U-Net coarse-grid correction at the 257x129 level
-    from helmholtz_amd import learned, train
+    # ---- learned + batched: U-Net coarse-grid correction at the 257x129 level,
+    # trained on pairs from plain SL-MG solves on seeded 513x513 media
+    t0 = time.perf_counter()
     net, tinfo = train.train_coarse_correction(n=(513, 513), media_count=args.train_media, level=2,
                                                epochs=args.train_epochs, log=log)
-    log(f"training: {tinfo}")
+    log(f"training ({time.perf_counter() - t0:.1f} s): {tinfo}")
     sl = make_solver(hp, "c64", fast=True)
     learned.attach(sl, net, 2)
-    t_l, st_l = timed_solves(ha, sl, Bd, args.warmup, args.steps, log, "learned",
-                             before_timed=lambda s: ha.time_kernel(s, *ROOFLINE_KERNEL))
-    lv = ha.hierarchy_levels(sl)
-    roof = roofline_from_solver(ha, sl, args.nsrc, int(np.prod(lv[0])), int(np.prod(lv[1])))
-    log(f"roofline: {roof}")
-    res_l = np.asarray(st_l["true_res"])
-    learned_ms_rhs = 1e3 * t_l / args.nsrc
+    t0 = time.perf_counter()
+    _, sl = ha.solveLinearSystem(None, Bd, sl)  # builds the hierarchy; restart cycles 0/1 are the trial
+    trial = {"seconds": round(time.perf_counter() - t0, 3), "setup_s": round(sl.last_stats["setup_time"], 3),
+             "max_iters": int(sl.last_stats["max_iters"]), "network_ran": bool(sl.last_stats["learned_used"]),
+             "kept": not sl.last_stats["fell_back"]}
+    log(f"learned trial: {trial}")
+    t_l, st_l, _, _ = timed_solves(ha, sl, Bd, 0, args.steps, log, "learned+batched")
+    learned_used = bool(st_l["learned_used"])
+    # ablation: the network in every restart cycle (no trial), at most 30
+    # restart cycles (set-up time subtracted)
+    sa = make_solver(hp, "c64", fast=True)
+    sa.MG.maxOuterIter = 30
+    learned.attach(sa, net, 2)
+    learned.policy(sa, always=True)
+    t_a, st_a, setup_a, _ = timed_solves(ha, sa, Bd, 0, 1, log, "network every cycle")
+    t_a -= setup_a
+    ablation = {"seconds": round(t_a, 3), "max_iters": int(st_a["max_iters"]),
+                "max_true_res": float(max(st_a["true_res"])), "network_ran": bool(st_a["learned_used"]),
+                "converged": bool(max(st_a["true_res"]) <= 1e-6), "max_restart_cycles": 30}
+    ablation["ms_per_rhs"] = round(1e3 * t_a / args.nsrc, 4) if ablation["converged"] else None
+    del sa
+    pipeline_ms_rhs = 1e3 * t_l / args.nsrc
     ms_step = 1e3 * t_l
-    st = st_l
+    del sl
 
-    # ---- plain FP64, one source per call
-    sp_ = make_solver(hp, "c128")
-    plain_t, plain_it, plain_res = [], [], []
+    # ---- plain FP64, one source per call, and the small 2-D plain workload
     step = max(1, args.nsrc // max(1, args.plain_rhs))
-    for c in range(args.plain_rhs + 1):
-        col = 0 if c == 0 else min(args.nsrc - 1, (c - 1) * step + step // 2)
-        bcol = DeviceArray.from_numpy(B[:, [col]])
-        t0 = time.perf_counter()
-        Xp, sp_ = ha.solveLinearSystem(None, bcol, sp_)
-        dt = time.perf_counter() - t0
-        if c == 0:
-            continue  # first call builds the hierarchy
-        plain_t.append(dt)
-        plain_it.append(int(sp_.last_stats["max_iters"]))
-        plain_res.append(float(sp_.last_stats["true_res"][0]))
-        log(f"plain rhs {c}: {dt:.3f} s, {plain_it[-1]} iters, res {plain_res[-1]:.2e}")
-    plain_ms_rhs = 1e3 * float(np.mean(plain_t))
+    cols = [min(args.nsrc - 1, c * step + step // 2) for c in range(args.plain_rhs)]
+    plain_ms_rhs, plain_it, plain_res, setup_p, plain_x = plain_single_rhs(ha, hp, B, cols, 6, "W", 10, SHIFTS,
+                                                                            wlevels=3, log=log)
+    hp1, B1 = c1_problem()
+    c1_ms, c1_it, c1_res, c1_setup, _ = plain_single_rhs(ha, hp1, B1, [0], 3, "V", 10, [0.5] * 3, log=log,
+                                                           name="C1 plain")
+    c1 = {"workload": "2-D 129x129 unit square, v = 2 + 0.3 GRF, 10 ppw, 3-level V(2,2), FGMRES(10)",
+          "ms_per_rhs": round(c1_ms, 3), "iters": c1_it[0], "true_res": c1_res[0], "setup_s": round(c1_setup, 4)}
 
+    acc = None if args.no_accuracy else accuracy(ha, hp, B, Xb, cols, plain_x, log)
+    del Xb
     c3 = None if args.no_3d else c3_lines(log, not args.no_c64_basis)
-    st5 = None if args.no_stream else stream_line(log)
+    st5 = st5l = None
+    if not args.no_stream:
+        st5 = stream_line(log)
+        st5l = stream_line(log, net=net)
     copy_gbs = calibration_copy(Bd, log)
-    roof["copy_gbs"] = round(copy_gbs, 1)
-    roof["copy_frac"] = round(roof["achieved"] / copy_gbs, 4)
+    for r in roofs:
+        r["copy_frac"] = round(r["achieved"] / copy_gbs, 4)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(hp, B, int(np.mean(plain_it)), args.cpu_budget)
         log(f"cpu baseline: {cpu}")
+    host = host_info()
 
     line = {
         "metric": "ms/RHS to 1e-6 true residual: learned+batched vs plain SL-MG, same GPU",
-        "value": round(learned_ms_rhs, 4),
+        "value": round(pipeline_ms_rhs, 4),
         "unit": "ms/RHS",
         "n_gpus": 1,
         "steps": args.steps,
@@ -385,31 +554,43 @@ def main():
         "dtype": "c64 preconditioner / c128 outer",
         "data": "synthetic (seeded layered+body medium, point sources)",
         "config": {"workload": "2-D 1025x513 Helmholtz, 256 point sources, 8 ppw, 1e-6 true residual",
-                   "nsrc": args.nsrc, "levels": 6, "cycle": "W(2,2) on levels 0-2, V(2,2) below", "krylov": "FGMRES(10)",
-                   "learned": "U-Net at level 2 (257x129) vs plain cycle, chosen by a two-restart trial on the first solve", "graphs": False,
-                   "autotune": True, "parallelism": "rhs-batched, 1 GPU"},
-        "learned_ms_per_rhs": round(learned_ms_rhs, 4),
-        "learned_max_iters": int(st_l["max_iters"]),
-        "learned_used": bool(st_l["learned_used"]),
-        "learned_fell_back": bool(st_l["fell_back"]),
-        "learned_max_true_res": float(res_l.max()),
+                   "nsrc": args.nsrc, "levels": 6, "cycle": "W(2,2) on levels 0-2, V(2,2) below",
+                   "krylov": "FGMRES(10)",
+                   "learned": "U-Net at level 2 (257x129) vs plain cycle, chosen by a two-restart trial on the "
+                              "first solve", "graphs": False, "autotune": True, "parallelism": "rhs-batched, 1 GPU"},
+        # value is the learned+batched solver's timed solves: with the network
+        # when it won its trial, else the plain cycle it fell back to
+        "value_pipeline": "batched, learned correction" if learned_used
+                          else "batched, plain cycle (the network lost its trial and the solver fell back)",
+        "learned_used": learned_used,
+        "learned_ms_per_rhs": round(pipeline_ms_rhs, 4) if learned_used else None,
+        "learned_trial": trial,
+        "learned_every_cycle": ablation,
         "training": tinfo,
         "batched_ms_per_rhs": round(batched_ms_rhs, 4),
         "batched_max_iters": int(st_b["max_iters"]),
         "batched_max_true_res": float(res_b.max()),
+        "batched_setup_s": round(setup_b, 3),
         "pcie_inclusive_ms_per_rhs": round(host_ms_rhs, 4),
         "c64_basis": c64b,
-        "all_converged": bool(np.all(res_b <= 1e-6)) and bool(np.all(res_l <= 1e-6)) and bool(max(plain_res) <= 1e-6),
+        "all_converged": bool(np.all(res_b <= 1e-6)) and bool(max(st_l["true_res"]) <= 1e-6)
+                         and bool(max(plain_res) <= 1e-6),
         "plain_ms_per_rhs": round(plain_ms_rhs, 3),
         "plain_iters": plain_it,
         "plain_max_true_res": float(max(plain_res)),
         "plain_rhs_sampled": args.plain_rhs,
-        "speedup_vs_plain": round(plain_ms_rhs / learned_ms_rhs, 2),
+        "plain_setup_s": round(setup_p, 3),
+        "speedup_vs_plain": round(plain_ms_rhs / pipeline_ms_rhs, 2),
         "speedup_batched_vs_plain": round(plain_ms_rhs / batched_ms_rhs, 2),
-        "roofline": roof,
+        "accuracy": acc,
+        "c1_plain": c1,
+        "roofline": roofs[0],
+        "roofline_kernels": roofs[1:],
         "cpu_baseline": cpu,
+        "host": host,
         "three_d": c3,
         "stream": st5,
+        "stream_learned": st5l,
     }
     print(json.dumps(line), flush=True)
 

@@ -281,6 +281,9 @@ struct hh_solver {
   // outcome of the learned/plain trial on this hierarchy and network:
   // -1 not run yet, 0 plain cycle kept, 1 network kept (later solves skip the trial)
   int learn_verdict = -1;
+  // HH_LEARN_TRIAL: the two-restart trial decides; HH_LEARN_ALWAYS: the network
+  // runs in every restart cycle (only non-finite results switch to the plain cycle)
+  int learn_policy = HH_LEARN_TRIAL;
   float* feat_kh2 = nullptr;
   float* feat_gw = nullptr;
   int feat_level = -1;       // level the feature fields were built for
@@ -295,16 +298,17 @@ struct hh_solver {
   // launch shapes picked by the autotuner (0 = heuristic), per level
   int rows_tuned[16] = {0};
   int tuned_for = 0;
-  // hipGraph per Arnoldi step j (replayed from the second restart cycle on)
-  hipGraphExec_t gexec[33] = {};
+  // hipGraph of one FGMRES restart cycle (replayed from the second cycle on)
+  hipGraphExec_t gcycle = nullptr;
   int gkey_nrhs = 0;
   bool gkey_learned = false;
   bool gkey_tr = false;
+  int gkey_zun = 0;
   // the outer operator is the spread operator's transpose (adjoint solve in progress)
   bool ho_tr = false;
   void drop_graphs() {
-    for (auto& g : gexec)
-      if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+    if (gcycle) (void)hipGraphExecDestroy(gcycle);
+    gcycle = nullptr;
   }
   // capture of level-l right-hand sides for training pairs
   int cap_level = -1;
@@ -791,18 +795,21 @@ struct Cycle {
       // (and, for the legs that smooth, the shared Jacobi factor)
       const double s = sizeof(cx<T>), Nf = (double)L.g.N, Nc = (double)C.g.N, act = S->kt_active;
       const bool td = S->kt.match(KT_DOWN, l), tm = S->kt.match(KT_MID, l), tu = S->kt.match(KT_UP, l);
+      // with two corrections (W) the down leg stores no iterate: the middle
+      // leg forms it again from b (L.cur only fixes the ping-pong parity)
+      const bool w2 = ncorr == 2;
       if (td) S->kt.begin(st);
-      launch_down_fused<T>(L.g, C.g, nrhs, b, (const cx<T>*)L.d, (const cx<T>*)L.f, (cx<T>*)L.cur, (cx<T>*)C.b,
-                           skip, st);
-      if (td) S->kt.end(st, act * (2 * Nf * s + Nc * s) + 2 * Nf * s);
+      launch_down_fused<T>(L.g, C.g, nrhs, b, (const cx<T>*)L.d, (const cx<T>*)L.f, w2 ? nullptr : (cx<T>*)L.cur,
+                           (cx<T>*)C.b, skip, st);
+      if (td) S->kt.end(st, act * ((w2 ? 1 : 2) * Nf * s + Nc * s) + 2 * Nf * s);
       C.cur = nullptr;
       run(l + 1, (const cx<T>*)C.b, true, nullptr);
-      for (int c = 1; c < ncorr; ++c) {
+      if (w2) {
         void* o = other(L);
         if (tm) S->kt.begin(st);
-        launch_mid_fused<T>(L.g, C.g, nrhs, (const cx<T>*)L.cur, (const cx<T>*)C.cur, b, (const cx<T>*)L.d,
-                            (cx<T>*)o, (cx<T>*)C.b, skip, st);
-        if (tm) S->kt.end(st, act * (3 * Nf * s + 2 * Nc * s) + Nf * s);
+        launch_mid_fused<T>(L.g, C.g, nrhs, (const cx<T>*)C.cur, b, (const cx<T>*)L.d, (const cx<T>*)L.f, (cx<T>*)o,
+                            (cx<T>*)C.b, skip, st);
+        if (tm) S->kt.end(st, act * (2 * Nf * s + 2 * Nc * s) + 2 * Nf * s);
         L.cur = o;
         C.cur = nullptr;
         run(l + 1, (const cx<T>*)C.b, true, nullptr);
@@ -970,10 +977,11 @@ void run_fgmres(hh_solver* S, int nrhs, hh_solve_stats* stats) {
   std::vector<double> res(nrhs, 1.0), prev(nrhs, 1.0);
   int64_t nprec = 0;
   const bool have_net = S->net != nullptr && S->p.prec == HH_C64;
-  S->use_learned = have_net && S->learn_verdict != 0;
-  S->fell_back = have_net && S->learn_verdict == 0;
+  const bool always = S->learn_policy == HH_LEARN_ALWAYS;
+  S->use_learned = have_net && (always || S->learn_verdict != 0);
+  S->fell_back = have_net && !always && S->learn_verdict == 0;
   S->learned_ran = S->use_learned;
-  int trial = S->use_learned && S->learn_verdict < 0 ? 0 : -1;
+  int trial = S->use_learned && !always && S->learn_verdict < 0 ? 0 : -1;
   double rho_learned = 1.0;
   if (S->use_learned) setup_learned(S);
   for (int outer = 0; outer < S->p.max_outer; ++outer) {
@@ -996,96 +1004,117 @@ void run_fgmres(hh_solver* S, int nrhs, hh_solve_stats* stats) {
       // every RHS already meets the tolerance at the cycle start
       break;
     }
-    for (int j = 0; j < K; ++j) {
+    const int64_t nprec_cycle0 = nprec;
+    // one Arnoldi step j: preconditioner cycle, H z and the Arnoldi passes;
+    // the kernels skip RHS whose device-side done flag is set
+    auto step = [&](int j) {
       const cx<double>* vj = vb32 ? nullptr : S->V + (int64_t)j * blk;
       const cx<float>* vj32 = (const cx<float>*)S->V + (int64_t)j * blk;
       void* zj = (char*)S->Z + (size_t)j * blk * zes;
-      S->cap_now = S->cap_buf && nprec >= S->cap_lo && nprec < S->cap_hi && active == nrhs;
-      S->kt_active = active;
-      auto body = [&] {
-        // H z: reads z (8 or 16 B per value), writes w (16 B), shared c128 diagonal
-        const bool ta = S->kt.match(KT_APPLY, 0);
-        const double Nf = (double)N;
-        const double* vsj = w.lazy ? w.vscale + (int64_t)j * nrhs : nullptr;
-        const bool staged = w.stage && j > 0;
-        if (S->p.prec == HH_C64 && w.lazy && !S->op->ho && (!ta || vb32)) {
-          // H z formed inside the Arnoldi passes (w is never stored)
-          const HzArgs hz{S->op->g, (const cx<float>*)zj, S->op->d64};
-          if (vb32) {
-            // the stored complex64 V_j is the cycle's input (scaled first
-            // for the learned correction, which is not linear)
-            const cx<float>* in = vj32;
-            if (!w.zunscaled) {
-              kr_cast_scaled<float>(w, vj32, vsj, (cx<float>*)S->b0, st);
-              in = (const cx<float>*)S->b0;
-            }
-            Cycle<float> c{S, nrhs, w.done, st};
-            c.apply(in, (cx<float>*)zj);
-            kr_arnoldi_impl<float>(w, (const cx<float>*)S->V, S->wk, j, tol, S->skip_copy, &S->kt, active, st,
-                                   &hz);
-          } else {
-            precondition<float>(S, nrhs, vj, zj, w.done, st, vsj, staged);
-            kr_arnoldi_impl<double>(w, S->V, S->wk, j, tol, S->skip_copy, &S->kt, active, st, &hz);
+      // H z: reads z (8 or 16 B per value), writes w (16 B), shared c128 diagonal
+      const bool ta = S->kt.match(KT_APPLY, 0);
+      const double Nf = (double)N;
+      const double* vsj = w.lazy ? w.vscale + (int64_t)j * nrhs : nullptr;
+      const bool staged = w.stage && j > 0;
+      if (S->p.prec == HH_C64 && w.lazy && !S->op->ho && (!ta || vb32)) {
+        // H z formed inside the Arnoldi passes (w is never stored)
+        const HzArgs hz{S->op->g, (const cx<float>*)zj, S->op->d64};
+        if (vb32) {
+          // the stored complex64 V_j is the cycle's input (scaled first
+          // for the learned correction, which is not linear)
+          const cx<float>* in = vj32;
+          if (!w.zunscaled) {
+            kr_cast_scaled<float>(w, vj32, vsj, (cx<float>*)S->b0, st);
+            in = (const cx<float>*)S->b0;
           }
-          return;
-        }
-        if (S->p.prec == HH_C64) {
-          precondition<float>(S, nrhs, vj, zj, w.done, st, vsj, staged);
-          if (ta) S->kt.begin(st);
-          h_apply<float>(S, nrhs, (const cx<float>*)zj, S->wk, w.done, st);
-          if (ta) S->kt.end(st, active * Nf * 24.0 + Nf * 16.0);
-          if (w.zunscaled) kr_scale_rhs(w, S->wk, vsj, st);
+          Cycle<float> c{S, nrhs, w.done, st};
+          c.apply(in, (cx<float>*)zj);
+          kr_arnoldi_impl<float>(w, (const cx<float>*)S->V, S->wk, j, tol, S->skip_copy, &S->kt, active, st,
+                                 &hz);
         } else {
-          precondition<double>(S, nrhs, vj, zj, w.done, st);
-          if (ta) S->kt.begin(st);
-          h_apply<double>(S, nrhs, (const cx<double>*)zj, S->wk, w.done, st);
-          if (ta) S->kt.end(st, active * Nf * 32.0 + Nf * 16.0);
+          precondition<float>(S, nrhs, vj, zj, w.done, st, vsj, staged);
+          kr_arnoldi_impl<double>(w, S->V, S->wk, j, tol, S->skip_copy, &S->kt, active, st, &hz);
         }
-        kr_arnoldi_impl<double>(w, S->V, S->wk, j, tol, S->skip_copy, &S->kt, active, st);
-      };
-      // hipGraph: the first restart cycle runs eagerly (it allocates the
-      // network workspace and may capture training data); later cycles replay
-      // one captured graph per Arnoldi step j.
-      if (S->p.graph && outer >= 1 && !S->cap_now && S->kt.kernel == KT_NONE) {
-        if (S->gkey_nrhs != nrhs || S->gkey_learned != S->use_learned || S->gkey_tr != S->ho_tr) {
-          S->drop_graphs();
-          S->gkey_nrhs = nrhs;
-          S->gkey_learned = S->use_learned;
-          S->gkey_tr = S->ho_tr;
-        }
-        if (!S->gexec[j]) {
-          hipGraph_t g = nullptr;
-          HIPCK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-          body();
-          HIPCK(hipStreamEndCapture(st, &g));
-          HIPCK(hipGraphInstantiate(&S->gexec[j], g, nullptr, nullptr, 0));
-          (void)hipGraphDestroy(g);
-        }
-        HIPCK(hipGraphLaunch(S->gexec[j], st));
-      } else {
-        body();
+        return;
       }
-      ++nprec;
-      HIPCK(hipMemcpyAsync(S->h_done, w.done, sizeof(int) * nrhs, hipMemcpyDeviceToHost, st));
-      HIPCK(hipStreamSynchronize(st));
-      active = 0;
-      for (int b = 0; b < nrhs; ++b) active += S->h_done[b] == 0;
-      if (active == 0) break;
+      if (S->p.prec == HH_C64) {
+        precondition<float>(S, nrhs, vj, zj, w.done, st, vsj, staged);
+        if (ta) S->kt.begin(st);
+        h_apply<float>(S, nrhs, (const cx<float>*)zj, S->wk, w.done, st);
+        if (ta) S->kt.end(st, active * Nf * 24.0 + Nf * 16.0);
+        if (w.zunscaled) kr_scale_rhs(w, S->wk, vsj, st);
+      } else {
+        precondition<double>(S, nrhs, vj, zj, w.done, st);
+        if (ta) S->kt.begin(st);
+        h_apply<double>(S, nrhs, (const cx<double>*)zj, S->wk, w.done, st);
+        if (ta) S->kt.end(st, active * Nf * 32.0 + Nf * 16.0);
+      }
+      kr_arnoldi_impl<double>(w, S->V, S->wk, j, tol, S->skip_copy, &S->kt, active, st);
+    };
+    // end of a restart cycle: x += Z y, then the true residual b - H x in
+    // complex128 and its norm per RHS
+    auto finish = [&] {
+      if (S->p.prec == HH_C64)
+        kr_update_x<float>(w, (const cx<float>*)S->Z, S->xk, st);
+      else
+        kr_update_x<double>(w, (const cx<double>*)S->Z, S->xk, st);
+      h_resid(S, nrhs, S->xk, S->bk, S->wk, st);
+      kr_true_res(w, S->wk, S->res_d, S->n2tmp, st);
+    };
+    // hipGraph mode: from the second restart cycle on, the whole cycle (K
+    // steps, x update, true residual) is one captured graph replayed with a
+    // single host synchronisation per restart; RHS that converge inside the
+    // cycle are skipped by the kernels' done flags.  The first cycle runs
+    // eagerly (it allocates the network workspace and may capture training
+    // data), as do timed solves and solves that capture training pairs.
+    const bool graph_cycle = S->p.graph && outer >= 1 && !S->cap_buf && S->kt.kernel == KT_NONE;
+    if (graph_cycle) {
+      S->cap_now = false;
+      if (S->gkey_nrhs != nrhs || S->gkey_learned != S->use_learned || S->gkey_tr != S->ho_tr ||
+          S->gkey_zun != w.zunscaled) {
+        S->drop_graphs();
+        S->gkey_nrhs = nrhs;
+        S->gkey_learned = S->use_learned;
+        S->gkey_tr = S->ho_tr;
+        S->gkey_zun = w.zunscaled;
+      }
+      if (!S->gcycle) {
+        hipGraph_t g = nullptr;
+        HIPCK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        for (int j = 0; j < K; ++j) step(j);
+        finish();
+        HIPCK(hipStreamEndCapture(st, &g));
+        const hipError_t ie = hipGraphInstantiate(&S->gcycle, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIPCK(ie);
+      }
+      HIPCK(hipGraphLaunch(S->gcycle, st));
+    } else {
+      for (int j = 0; j < K; ++j) {
+        S->cap_now = S->cap_buf && nprec_cycle0 + j >= S->cap_lo && nprec_cycle0 + j < S->cap_hi && active == nrhs;
+        S->kt_active = active;
+        step(j);
+        HIPCK(hipMemcpyAsync(S->h_done, w.done, sizeof(int) * nrhs, hipMemcpyDeviceToHost, st));
+        HIPCK(hipStreamSynchronize(st));
+        active = 0;
+        for (int b = 0; b < nrhs; ++b) active += S->h_done[b] == 0;
+        if (active == 0) break;
+      }
+      finish();
     }
-    if (S->p.prec == HH_C64)
-      kr_update_x<float>(w, (const cx<float>*)S->Z, S->xk, st);
-    else
-      kr_update_x<double>(w, (const cx<double>*)S->Z, S->xk, st);
     HIPCK(hipMemcpyAsync(S->h_kact, w.kact, sizeof(int) * nrhs, hipMemcpyDeviceToHost, st));
-    // true residual in complex128
-    h_resid(S, nrhs, S->xk, S->bk, S->wk, st);
-    kr_true_res(w, S->wk, S->res_d, S->n2tmp, st);
     HIPCK(hipMemcpyAsync(S->h_res, S->res_d, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
     HIPCK(hipStreamSynchronize(st));
     HIPCK(hipGetLastError());
     int nconv = 0;
     bool stagnated = false;
     std::vector<int> bad(nrhs, 0);
+    // lock-step preconditioner applications of this cycle: the steps some RHS
+    // still took
+    int kmax_cycle = 0;
+    for (int b = 0; b < nrhs; ++b)
+      if (!conv[b]) kmax_cycle = std::max(kmax_cycle, S->h_kact[b]);
+    nprec += kmax_cycle;
     for (int b = 0; b < nrhs; ++b) {
       iters[b] += conv[b] ? 0 : S->h_kact[b];
       res[b] = S->h_res[b];
@@ -1124,7 +1153,7 @@ void run_fgmres(hh_solver* S, int nrhs, hh_solve_stats* stats) {
       trial = 2;
       stagnated = false;
     }
-    if (S->use_learned && stagnated) {
+    if (S->use_learned && stagnated && (!always || bad_any)) {
       // the learned correction stopped helping: fall back to the plain cycle;
       // columns it drove to non-finite values restart from zero
       S->use_learned = false;
@@ -1753,6 +1782,25 @@ int hh_solver_attach_unet(hh_solver* s, hh_unet* net, int level) {
     s->learn_verdict = -1;
     s->net = net;
     s->learn_level = net ? level : -1;
+  });
+}
+
+int hh_solver_learned_policy(hh_solver* s, int policy) {
+  return guarded([&] {
+    if (!s) throw HHError(HH_ERR_ARG, "null argument");
+    if (policy != HH_LEARN_TRIAL && policy != HH_LEARN_ALWAYS)
+      throw HHError(HH_ERR_ARG, "policy must be HH_LEARN_TRIAL or HH_LEARN_ALWAYS");
+    s->learn_policy = policy;
+    s->learn_verdict = -1;
+    s->drop_graphs();
+  });
+}
+
+int hh_solver_learned_workspace(const hh_solver* s, const void** activations, const void** staging) {
+  return guarded([&] {
+    if (!s || !activations || !staging) throw HHError(HH_ERR_ARG, "null argument");
+    *activations = s->nn_ws.bufs.empty() ? nullptr : s->nn_ws.bufs[0];
+    *staging = s->nn_x16;
   });
 }
 

@@ -3,9 +3,14 @@
 // and writes its outputs once:
 //
 //   down : x1 = F b, x2 = x1 + F (b - A x1), r = b - A x2, b_c = R r
-//          reads b, d; writes x2 and b_c          (unfused: 4 kernels, ~9 passes)
-//   mid  : x' = x + P e_c, r = b - A x', b_c = R r
-//          reads x, b, d, e_c; writes x', b_c     (unfused: 3 kernels, ~6.5 passes)
+//          reads b, d, F; writes b_c, and x2 only where a single coarse
+//          correction follows (V levels)         (unfused: 4 kernels, ~9 passes)
+//   mid  : W levels: x2 recomputed from b exactly as the down leg forms it,
+//          x' = x2 + P e_c, r = b - A x', b_c = R r
+//          reads b, d, F, e_c; writes x', b_c    (unfused: 3 kernels, ~6.5 passes)
+//          (recomputing x2 costs ~40 flops per node and saves storing and
+//          re-reading it: 16 of the 72 bytes per node and RHS of a fine-level
+//          W visit)
 //   up   : x' = x + P e_c, two Jacobi sweeps x <- x + F (b - A x)
 //          reads x, b, d, e_c; writes x''         (unfused: 3 kernels, ~8.25 passes)
 //
@@ -147,7 +152,9 @@ __device__ __forceinline__ cx<T> apply_at(const cx<T>* s, int q, cx<T> dq, T ih1
 }
 
 // ------------------------------------------------------------------ down leg
-template <typename T>
+// STOREX: store the smoothed iterate x2 (V levels); on W levels the middle
+// leg recomputes it, so only b_c is written
+template <typename T, bool STOREX>
 __global__ __launch_bounds__(FusedCfg<T>::NT) void k_down_fused(TileMap tm, Grid gf, Grid gc,
                                                                 const cx<T>* __restrict__ b,
                                                                 const cx<T>* __restrict__ d,
@@ -181,7 +188,7 @@ __global__ __launch_bounds__(FusedCfg<T>::NT) void k_down_fused(TileMap tm, Grid
       const cx<T> v = xc + rf[k] * (rb[k] - apply_at(s1, q, rd[k], ih1, ih2));
       s2[q] = v;
       const int j = c.fy0 - H + lj;
-      if (lj >= H && lj < H + G::TY && j < gf.n2 && own_c) xb[(int64_t)j * gf.n1 + (c.fx0 - H + c.lane)] = v;
+      if (STOREX && lj >= H && lj < H + G::TY && j < gf.n2 && own_c) xb[(int64_t)j * gf.n1 + (c.fx0 - H + c.lane)] = v;
     }
   }
   __syncthreads();
@@ -197,41 +204,61 @@ __global__ __launch_bounds__(FusedCfg<T>::NT) void k_down_fused(TileMap tm, Grid
 }
 
 // ------------------------------------------------------------------ middle (W)
+// x2 = F b + F (b - A F b) is formed again on the halo-2 region of an H = 3
+// tile with the down leg's expressions (so it equals the x2 that leg would
+// have stored, bit for bit), then x' = x2 + P e_c, the residual on the halo-1
+// region, and its full weighting.
 template <typename T>
 __global__ __launch_bounds__(FusedCfg<T>::NT) void k_mid_fused(TileMap tm, Grid gf, Grid gc,
-                                                               const cx<T>* __restrict__ x,
                                                                const cx<T>* __restrict__ ec,
                                                                const cx<T>* __restrict__ b,
-                                                               const cx<T>* __restrict__ d, cx<T>* __restrict__ xo,
+                                                               const cx<T>* __restrict__ d,
+                                                               const cx<T>* __restrict__ f, cx<T>* __restrict__ xo,
                                                                cx<T>* __restrict__ bco,
                                                                const int* __restrict__ skip) {
-  constexpr int H = 2;
+  constexpr int H = 3;
   typedef Geo<T, H> G;
   constexpr int NW = G::NW, KR = G::KR, HT = G::HT;
-  // the coarse patch is staged in sr, which is first written after the
-  // prolongation's last read of the patch (barrier between)
-  __shared__ cx<T> sx[G::NPIX], sr[G::NPIX];
+  __shared__ cx<T> s1[G::NPIX], s2[G::NPIX];
   TileCtx c;
   if (!tile_ctx<H, G::FTX, G::TY>(tm, gf, skip, c)) return;
   const T ih1 = (T)gf.ih1, ih2 = (T)gf.ih2;
   const int64_t off = (int64_t)c.rhs * gf.N;
-  cx<T> rb[KR], rd[KR], vx[KR];
-  load_rows<T, H, NW, KR>(x + off, c, gf, vx);
+  cx<T> rb[KR], rd[KR], rf[KR], vx[KR];
   load_rows<T, H, NW, KR>(b + off, c, gf, rb);
   load_rows<T, H, NW, KR>(d, c, gf, rd);
-  stage_prolong<T, H>(ec + (int64_t)c.rhs * gc.N, gc, gf, c, vx, sx, sr);
+  load_rows<T, H, NW, KR>(f, c, gf, rf);
+  // x1 = F b on the whole tile
+#pragma unroll
+  for (int k = 0; k < KR; ++k) s1[(c.wv + NW * k) * 64 + c.lane] = rf[k] * rb[k];
   __syncthreads();
-  const bool in1 = c.lane >= 1 && c.lane < 63, own_c = c.lane >= H && c.lane < H + G::FTX && c.fx0 - H + c.lane < gf.n1;
+  // x2 on the halo-2 region (the down leg's second sweep)
+  const bool in1 = c.lane >= 1 && c.lane < 63;
+#pragma unroll
+  for (int k = 0; k < KR; ++k) {
+    const int lj = c.wv + NW * k, q = lj * 64 + c.lane;
+    if (lj >= 1 && lj < HT - 1 && in1) s2[q] = s1[q] + rf[k] * (rb[k] - apply_at(s1, q, rd[k], ih1, ih2));
+  }
+  __syncthreads();
+  // x2 into registers (the outer ring holds no x2; its x' never reaches an
+  // owned output), then the coarse patch may overwrite s2
+#pragma unroll
+  for (int k = 0; k < KR; ++k) vx[k] = s2[(c.wv + NW * k) * 64 + c.lane];
+  __syncthreads();
+  stage_prolong<T, H>(ec + (int64_t)c.rhs * gc.N, gc, gf, c, vx, s1, s2);
+  __syncthreads();
+  // residual on the halo-1 region over x' (s1); owned x' is stored
+  const bool in2 = c.lane >= 2 && c.lane < 62, own_c = c.lane >= H && c.lane < H + G::FTX && c.fx0 - H + c.lane < gf.n1;
   cx<T>* xb = xo + off;
 #pragma unroll
   for (int k = 0; k < KR; ++k) {
     const int lj = c.wv + NW * k, q = lj * 64 + c.lane;
-    if (lj >= 1 && lj < HT - 1 && in1) sr[q] = rb[k] - apply_at(sx, q, rd[k], ih1, ih2);
+    if (lj >= 2 && lj < HT - 2 && in2) s2[q] = rb[k] - apply_at(s1, q, rd[k], ih1, ih2);
     const int j = c.fy0 - H + lj;
-    if (lj >= H && lj < H + G::TY && j < gf.n2 && own_c) xb[(int64_t)j * gf.n1 + (c.fx0 - H + c.lane)] = sx[q];
+    if (lj >= H && lj < H + G::TY && j < gf.n2 && own_c) xb[(int64_t)j * gf.n1 + (c.fx0 - H + c.lane)] = s1[q];
   }
   __syncthreads();
-  restrict_tile<T, H>(sr, c, gc, bco);
+  restrict_tile<T, H>(s2, c, gc, bco);
 }
 
 // ------------------------------------------------------------------ up leg
@@ -290,14 +317,19 @@ template <typename T>
 void launch_down_fused(const Grid& gf, const Grid& gc, int nrhs, const cx<T>* b, const cx<T>* d, const cx<T>* f,
                        cx<T>* xo, cx<T>* bco, const int* skip, hipStream_t s) {
   const TileMap m = tile_map<T, 3>(gf, nrhs);
-  hipLaunchKernelGGL((k_down_fused<T>), dim3(m.T), dim3(FusedCfg<T>::NT), 0, s, m, gf, gc, b, d, f, xo, bco, skip);
+  if (xo)
+    hipLaunchKernelGGL((k_down_fused<T, true>), dim3(m.T), dim3(FusedCfg<T>::NT), 0, s, m, gf, gc, b, d, f, xo, bco,
+                       skip);
+  else
+    hipLaunchKernelGGL((k_down_fused<T, false>), dim3(m.T), dim3(FusedCfg<T>::NT), 0, s, m, gf, gc, b, d, f, xo, bco,
+                       skip);
 }
 
 template <typename T>
-void launch_mid_fused(const Grid& gf, const Grid& gc, int nrhs, const cx<T>* x, const cx<T>* ec, const cx<T>* b,
-                      const cx<T>* d, cx<T>* xo, cx<T>* bco, const int* skip, hipStream_t s) {
-  const TileMap m = tile_map<T, 2>(gf, nrhs);
-  hipLaunchKernelGGL((k_mid_fused<T>), dim3(m.T), dim3(FusedCfg<T>::NT), 0, s, m, gf, gc, x, ec, b, d, xo, bco, skip);
+void launch_mid_fused(const Grid& gf, const Grid& gc, int nrhs, const cx<T>* ec, const cx<T>* b, const cx<T>* d,
+                      const cx<T>* f, cx<T>* xo, cx<T>* bco, const int* skip, hipStream_t s) {
+  const TileMap m = tile_map<T, 3>(gf, nrhs);
+  hipLaunchKernelGGL((k_mid_fused<T>), dim3(m.T), dim3(FusedCfg<T>::NT), 0, s, m, gf, gc, ec, b, d, f, xo, bco, skip);
 }
 
 template <typename T>

@@ -72,12 +72,14 @@ bool fused3_ok(const Grid& gf);
 void launch_up_fused3(const Grid& gf, const Grid& gc, int nrhs, const cx<float>* x, const cx<float>* ec,
                       const cx<float>* b, const cx<float>* d, const cx<float>* f, cx<float>* xo, const int* skip,
                       hipStream_t s);
+// xo = null: the smoothed iterate is not stored (the middle leg recomputes it)
 template <typename T>
 void launch_down_fused(const Grid& gf, const Grid& gc, int nrhs, const cx<T>* b, const cx<T>* d, const cx<T>* f,
                        cx<T>* xo, cx<T>* bco, const int* skip, hipStream_t s);
+// the middle leg of a W visit: recomputes the down leg's x2 from b
 template <typename T>
-void launch_mid_fused(const Grid& gf, const Grid& gc, int nrhs, const cx<T>* x, const cx<T>* ec, const cx<T>* b,
-                      const cx<T>* d, cx<T>* xo, cx<T>* bco, const int* skip, hipStream_t s);
+void launch_mid_fused(const Grid& gf, const Grid& gc, int nrhs, const cx<T>* ec, const cx<T>* b, const cx<T>* d,
+                      const cx<T>* f, cx<T>* xo, cx<T>* bco, const int* skip, hipStream_t s);
 template <typename T>
 void launch_up_fused(const Grid& gf, const Grid& gc, int nrhs, const cx<T>* x, const cx<T>* ec, const cx<T>* b,
                      const cx<T>* d, const cx<T>* f, cx<T>* xo, const int* skip, hipStream_t s);

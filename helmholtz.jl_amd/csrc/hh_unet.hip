@@ -111,45 +111,112 @@ __global__ __launch_bounds__(256) void k_conv3x3(const bf16* __restrict__ in, in
     }
 }
 
-// part[chunk][tap][co][ci] = sum over the chunk's pixels of dpre[q][co] * in[q + off(tap)][ci]
-__global__ __launch_bounds__(64) void k_conv_wgrad(const bf16* __restrict__ dpre, int cout, const bf16* __restrict__ in,
-                                                   int cin, int nx, int ny, int64_t total, int64_t chunk,
-                                                   float* __restrict__ part) {
-  const int ncib = cin / 16, ncob = cout / 16;
-  int t = blockIdx.y;
-  const int cib = t % ncib;
-  t /= ncib;
-  const int cob = t % ncob;
-  const int tap = t / ncob;
-  const int di = tap % 3 - 1, dj = tap / 3 - 1;
-  const int lane = threadIdx.x;
+// Weight and bias gradients of a 3x3 layer as one MFMA GEMM per tap over
+// pixels: dW[tap][co][ci] = sum_q dpre[q][co] in[q + off(tap)][ci] and
+// db[co] = sum_q dpre[q][co].  A block takes the pixel range [q0, q0 + per) of
+// the flattened (image, pixel) index and one 16-channel block of ci (grid.y),
+// for all 9 taps and all output channels.  Each chunk of 64 pixels is staged
+// once in LDS with 16-byte global loads, transposed so that the MFMA's
+// reduction index (pixels) is contiguous: sA[co][px] = dpre, and the
+// im2col of the input sB[tap][ci][px] (zero outside the image).  Every
+// operand of v_mfma_f32_16x16x16bf16 (lane: 4 consecutive pixels of one
+// channel) is then one aligned 8-byte LDS read; rows are padded to 68
+// entries so the 16 lanes of a group hit distinct banks.  The 9 NCO tiles of
+// 16x16 outputs are spread over the 4 wavefronts; partial sums per block are
+// reduced in a fixed order afterwards (deterministic).
+constexpr int WG_PX = 64;         // pixels per staged chunk
+constexpr int WG_LD = WG_PX + 4;  // padded LDS row
+template <int NCO>
+__global__ __launch_bounds__(256) void k_conv_wgrad(const bf16* __restrict__ dpre, const bf16* __restrict__ in,
+                                                   int cin, int nx, int ny, int64_t total, int64_t per,
+                                                   float* __restrict__ pw, float* __restrict__ pb) {
+  constexpr int COUT = 16 * NCO;
+  constexpr int NTILE = 9 * NCO;
+  constexpr int TPW = (NTILE + 3) / 4;
+  __shared__ __align__(16) bf16 sA[COUT * WG_LD];
+  __shared__ __align__(16) bf16 sB[9 * 16 * WG_LD];
+  __shared__ int sbase[WG_PX], si[WG_PX], sj[WG_PX];
+  const int cib = blockIdx.y;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int m = lane & 15, kq = (lane >> 4) * 4;
   const int npix = nx * ny;
-  const int64_t q0 = (int64_t)blockIdx.x * chunk;
-  const int64_t q1 = q0 + chunk < total ? q0 + chunk : total;
-  f4 acc = f4{0.f, 0.f, 0.f, 0.f};
-  for (int64_t q = q0; q < q1; q += 16) {
-    s4 a, b;
+  const int64_t qb0 = (int64_t)blockIdx.x * per;
+  const int64_t qb1 = qb0 + per < total ? qb0 + per : total;
+  f4 acc[TPW];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int64_t qq = q + kq + i;
-      short av = 0, bvv = 0;
-      if (qq < q1) {
-        av = (short)dpre[qq * cout + cob * 16 + m];
-        const int64_t img = qq / npix;
-        const int p = (int)(qq - img * npix);
-        const int i0 = p % nx + di, j0 = p / nx + dj;
-        if (i0 >= 0 && i0 < nx && j0 >= 0 && j0 < ny)
-          bvv = (short)in[(img * npix + (int64_t)j0 * nx + i0) * cin + cib * 16 + m];
+  for (int t = 0; t < TPW; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+  float bsum = 0.f;
+  for (int64_t q0 = qb0; q0 < qb1; q0 += WG_PX) {
+    __syncthreads();  // the previous chunk's operands are consumed
+    if (threadIdx.x < WG_PX) {
+      const int64_t q = q0 + threadIdx.x;
+      if (q < qb1) {
+        const int64_t img = q / npix;
+        const int p = (int)(q - img * npix);
+        sbase[threadIdx.x] = (int)img;
+        si[threadIdx.x] = p % nx;
+        sj[threadIdx.x] = p / nx;
+      } else {
+        sbase[threadIdx.x] = -1;
       }
-      a[i] = av;
-      b[i] = bvv;
     }
-    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
-  }
-  float* pb = part + ((int64_t)blockIdx.x * 9 + tap) * cout * cin;
+    // dpre: 64 pixels x COUT channels, 8 channels (16 B) per load
+    for (int e = threadIdx.x; e < WG_PX * COUT / 8; e += 256) {
+      const int px = e / (COUT / 8), c8 = (e % (COUT / 8)) * 8;
+      const int64_t q = q0 + px;
+      uint4 v = make_uint4(0u, 0u, 0u, 0u);
+      if (q < qb1) v = *(const uint4*)(dpre + q * COUT + c8);
+      const bf16* hv = (const bf16*)&v;
 #pragma unroll
-  for (int i = 0; i < 4; ++i) pb[(int64_t)(cob * 16 + kq + i) * cin + cib * 16 + m] = acc[i];
+      for (int k = 0; k < 8; ++k) sA[(c8 + k) * WG_LD + px] = hv[k];
+    }
+    __syncthreads();  // pixel coordinates
+    // im2col of the input for this ci block: tap t, pixel px, 8 channels per load
+    for (int e = threadIdx.x; e < 9 * WG_PX * 2; e += 256) {
+      const int t = e / (WG_PX * 2), r = e % (WG_PX * 2);
+      const int px = r >> 1, c8 = (r & 1) * 8;
+      uint4 v = make_uint4(0u, 0u, 0u, 0u);
+      const int img = sbase[px];
+      if (img >= 0) {
+        const int ii = si[px] + t % 3 - 1, jj = sj[px] + t / 3 - 1;
+        if (ii >= 0 && ii < nx && jj >= 0 && jj < ny)
+          v = *(const uint4*)(in + ((int64_t)img * npix + (int64_t)jj * nx + ii) * cin + cib * 16 + c8);
+      }
+      const bf16* hv = (const bf16*)&v;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) sB[(t * 16 + c8 + k) * WG_LD + px] = hv[k];
+    }
+    __syncthreads();
+    // bias gradient: one thread per output channel (ci block 0 only)
+    if (cib == 0 && threadIdx.x < COUT)
+      for (int px = 0; px < WG_PX; ++px) bsum += bf2f(sA[threadIdx.x * WG_LD + px]);
+#pragma unroll
+    for (int k0 = 0; k0 < WG_PX; k0 += 16) {
+#pragma unroll
+      for (int ti = 0; ti < TPW; ++ti) {
+        const int tile = wv + 4 * ti;
+        if (tile < NTILE) {
+          const int t = tile / NCO, cob = tile % NCO;
+          const s4 a = *(const s4*)(sA + (cob * 16 + m) * WG_LD + k0 + kq);
+          const s4 b = *(const s4*)(sB + (t * 16 + m) * WG_LD + k0 + kq);
+          acc[ti] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[ti], 0, 0, 0);
+        }
+      }
+    }
+  }
+  // partials: pw[block][tap][co][ci] (C row = co, column = ci), pb[block][co]
+  float* out = pw + (int64_t)blockIdx.x * 9 * COUT * cin;
+#pragma unroll
+  for (int ti = 0; ti < TPW; ++ti) {
+    const int tile = wv + 4 * ti;
+    if (tile < NTILE) {
+      const int t = tile / NCO, cob = tile % NCO;
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        out[((int64_t)t * COUT + cob * 16 + kq + i) * cin + cib * 16 + m] = acc[ti][i];
+    }
+  }
+  if (cib == 0 && threadIdx.x < COUT) pb[(int64_t)blockIdx.x * COUT + threadIdx.x] = bsum;
 }
 
 // out[k] = sum_c part[c][k] (fixed order)
@@ -160,27 +227,6 @@ __global__ __launch_bounds__(256) void k_sum_parts(const float* __restrict__ par
   float s = 0.f;
   for (int c = 0; c < nparts; ++c) s += part[(int64_t)c * n + k];
   out[k] = s;
-}
-
-// per-chunk column sums of an NHWC bf16 tensor: part[chunk][c]
-__global__ __launch_bounds__(256) void k_colsum(const bf16* __restrict__ g, int C, int64_t total, int64_t chunk,
-                                                float* __restrict__ part) {
-  __shared__ float sh[256];
-  const int64_t q0 = (int64_t)blockIdx.x * chunk;
-  const int64_t q1 = q0 + chunk < total ? q0 + chunk : total;
-  const int c = threadIdx.x % C;
-  const int rows = 256 / C;
-  const int r0 = threadIdx.x / C;
-  float s = 0.f;
-  if (r0 < rows)
-    for (int64_t q = q0 + r0; q < q1; q += rows) s += bf2f(g[q * C + c]);
-  sh[threadIdx.x] = (r0 < rows) ? s : 0.f;
-  __syncthreads();
-  if (threadIdx.x < C) {
-    float t = 0.f;
-    for (int r = 0; r < rows; ++r) t += sh[r * C + threadIdx.x];
-    part[(int64_t)blockIdx.x * C + threadIdx.x] = t;
-  }
 }
 
 __global__ __launch_bounds__(256) void k_relu_bwd(const bf16* __restrict__ dout, const bf16* __restrict__ out,
@@ -614,6 +660,19 @@ static int64_t pick_chunks(int64_t total, int64_t* chunk) {
   return (total + c - 1) / c;
 }
 
+// pixel blocks of the weight-gradient GEMM: at most WG_MAXBLK blocks of a
+// multiple of WG_PX pixels, at least 16 chunks each
+constexpr int64_t WG_MAXBLK = 256;
+static int64_t pick_wgrad_blocks(int64_t total, int64_t* per) {
+  int64_t nb = (total + 16 * WG_PX - 1) / (16 * WG_PX);
+  if (nb > WG_MAXBLK) nb = WG_MAXBLK;
+  if (nb < 1) nb = 1;
+  int64_t p = (total + nb - 1) / nb;
+  p = (p + WG_PX - 1) / WG_PX * WG_PX;
+  *per = p;
+  return (total + p - 1) / p;
+}
+
 static void ensure_scratch(UNetWork& ws, int64_t n) {
   if (ws.scratch_n >= n) return;
   if (ws.scratch) (void)hipFree(ws.scratch);
@@ -626,8 +685,9 @@ static void ensure_scratch(UNetWork& ws, int64_t n) {
 void unet_backward(UNet& net, UNetWork& ws, hipStream_t s, int nimg, int nx, int ny, const bf16* x, const float* dy) {
   Dims d = dims_of(nimg, nx, ny);
   bf16* dpre = (bf16*)ws.bufs[2 * A_NUM];
-  // scratch: wgrad partials (64 chunks x largest layer) or head partials
-  ensure_scratch(ws, 64LL * 9 * 128 * 64 + 64LL * 34);
+  // scratch: weight-gradient partials (at most WG_MAXBLK pixel blocks x the
+  // largest layer) and bias partials, or head partials
+  ensure_scratch(ws, (int64_t)WG_MAXBLK * (9 * 128 * 64 + 64) + 64LL * 34);
   const int64_t head_off = unet_param_count() - 34;
   // head
   {
@@ -653,15 +713,20 @@ void unet_backward(UNet& net, UNetWork& ws, hipStream_t s, int nimg, int nx, int
     hipLaunchKernelGGL(k_relu_bwd, dim3(nblk(T * c.co)), dim3(256), 0, s, grd(ws, kOut[l]), act(ws, kOut[l]), dpre,
                        T * c.co);
     const bf16* in = kIn[l] < 0 ? x : act(ws, kIn[l]);
-    int64_t chunk;
-    const int64_t nch = pick_chunks(T, &chunk);
+    int64_t per;
+    const int64_t nb = pick_wgrad_blocks(T, &per);
     const int64_t nw = 9LL * c.co * c.ci;
-    hipLaunchKernelGGL(k_conv_wgrad, dim3((unsigned)nch, 9 * (c.co / 16) * (c.ci / 16)), dim3(64), 0, s, dpre, c.co,
-                       in, c.ci, d.nx[lv], d.ny[lv], T, chunk, ws.scratch);
-    hipLaunchKernelGGL(k_sum_parts, dim3(nblk(nw)), dim3(256), 0, s, ws.scratch, (int)nch, nw, net.grad + off[l]);
-    hipLaunchKernelGGL(k_colsum, dim3((unsigned)nch), dim3(256), 0, s, dpre, c.co, T, chunk, ws.scratch);
-    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, s, ws.scratch, (int)nch, (int64_t)c.co,
-                       net.grad + off[l] + nw);
+    float* pw = ws.scratch;
+    float* pb = ws.scratch + nb * nw;
+    const dim3 g((unsigned)nb, (unsigned)(c.ci / 16));
+    switch (c.co / 16) {
+      case 1: hipLaunchKernelGGL(k_conv_wgrad<1>, g, dim3(256), 0, s, dpre, in, c.ci, d.nx[lv], d.ny[lv], T, per, pw, pb); break;
+      case 2: hipLaunchKernelGGL(k_conv_wgrad<2>, g, dim3(256), 0, s, dpre, in, c.ci, d.nx[lv], d.ny[lv], T, per, pw, pb); break;
+      case 4: hipLaunchKernelGGL(k_conv_wgrad<4>, g, dim3(256), 0, s, dpre, in, c.ci, d.nx[lv], d.ny[lv], T, per, pw, pb); break;
+      default: throw std::runtime_error("unsupported output channel count");
+    }
+    hipLaunchKernelGGL(k_sum_parts, dim3(nblk(nw)), dim3(256), 0, s, pw, (int)nb, nw, net.grad + off[l]);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, s, pb, (int)nb, (int64_t)c.co, net.grad + off[l] + nw);
     if (need_din)
       conv_launch(dpre, c.co, net.wt[l], nullptr, grd(ws, kIn[l]), c.ci, nimg, d.nx[lv], d.ny[lv], 0, s);
   };

@@ -115,3 +115,20 @@ def attach(solver, net: UNet | None, level: int = 2):
     _ensure_handle(solver)
     solver._net_ref = net
     L.check(L.lib().hh_solver_attach_unet(solver._handle, None if net is None else net.ptr, level))
+
+
+def policy(solver, always: bool):
+    """How the attached network is used: the two-restart trial (always=False,
+    the default) or every restart cycle with no trial (always=True, the
+    ablation: only non-finite results switch to the plain cycle)."""
+    _validate(solver)
+    _ensure_handle(solver)
+    L.check(L.lib().hh_solver_learned_policy(solver._handle, L.HH_LEARN_ALWAYS if always else L.HH_LEARN_TRIAL))
+
+
+def workspace(solver):
+    """(activations, staging) device addresses of the solver's own network
+    workspace (0 where the network has not run on it yet): inspection hook."""
+    a, b = C.c_void_p(), C.c_void_p()
+    L.check(L.lib().hh_solver_learned_workspace(solver._handle, C.byref(a), C.byref(b)))
+    return int(a.value or 0), int(b.value or 0)

@@ -64,7 +64,9 @@ typedef struct hh_mg_params {
   int32_t prec;       /* preconditioner precision: HH_C128 or HH_C64             */
   int32_t learned;    /* reserved: 0 (attach a network with hh_solver_attach_unet) */
   int32_t verbose;    /* solver.verbose                                          */
-  int32_t graph;      /* 1: replay each FGMRES iteration as a captured hipGraph  */
+  int32_t graph;      /* 1: from the second FGMRES restart cycle on, replay each
+                         whole cycle (inner steps, x update, true residual) as
+                         one captured hipGraph, one host sync per cycle      */
   int32_t autotune;   /* 1: time stencil launch shapes per level at first solve  */
   int32_t fused;      /* 1: LDS-tiled fused smoothing/transfer legs on 2-D levels
                          (mirror ghosts, 2 pre / 2 post sweeps); 0: one kernel per step */
@@ -200,7 +202,10 @@ int hh_solver_precondition(hh_solver* s, const void* V, void* Z, int64_t nrhs, i
  * Algorithmic bytes per launch (s = bytes per preconditioner value, N / Nc =
  * nodes of the level / the next coarser level, a = active RHS; the shared
  * diagonal and Jacobi factor count once per launch):
- *   DOWN   a (2 N s + Nc s) + 2 N s    MID  a (3 N s + 2 Nc s) + N s
+ *   DOWN   a (2 N s + Nc s) + 2 N s  (a (N s + Nc s) + 2 N s on W levels,
+ *          where the smoothed iterate is not stored)
+ *   MID    a (2 N s + 2 Nc s) + 2 N s  (W levels: b and e_c read, the
+ *          smoothed iterate recomputed from b, x' and b_c written)
  *   UP     a (3 N s + Nc s) + 2 N s    APPLY a N (s_z + 16) + 16 N
  *   DOTS   16 a N (j + 2)              UPDATE 16 a N (j + 3)  (Arnoldi step j)
  * With prec = HH_C64 on the 5/7-point operator the passes form H z from the
@@ -285,6 +290,18 @@ int hh_unet_eval(hh_unet* u, hh_dataset* d, int64_t first, int64_t count, float*
  * the network is re-attached or the hierarchy is rebuilt.
  * hh_solve_stats.fell_back reports the plain cycle was used.  net = NULL detaches. */
 int hh_solver_attach_unet(hh_solver* s, hh_unet* net, int level);
+/* How an attached network is used: HH_LEARN_TRIAL (default) is the trial
+ * described above; HH_LEARN_ALWAYS runs the network in every restart cycle of
+ * every solve (no trial, no stagnation switch; only non-finite results bring
+ * in the plain cycle) — the ablation of the learned correction.  Resets the
+ * kept verdict. */
+enum { HH_LEARN_TRIAL = 0, HH_LEARN_ALWAYS = 1 };
+int hh_solver_learned_policy(hh_solver* s, int policy);
+/* Inspection hook (no counterpart in the reference): device addresses of the
+ * solver's own network workspace — its first activation buffer and its input
+ * staging block — or NULL where the network has not run on this solver yet.
+ * Solvers holding one network share only its read-only weights, never these. */
+int hh_solver_learned_workspace(const hh_solver* s, const void** activations, const void** staging);
 
 /* ---- Uneven stream of independent problems (no counterpart in the reference) ---- */
 typedef struct hh_problem {

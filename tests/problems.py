@@ -87,3 +87,24 @@ def seg_salt_case():
     q = np.zeros(m.size, complex)
     q[hr.loc2cs(n, [n[0] // 2, 1]) - 1] = 1.0 / M.h[0] ** 2
     return M, m, g.ravel(order="F"), w, H, q
+
+
+def c2_case(nsrc=256):
+    """The batched 2-D workload: 1025 x 513 nodes, h = 10 m, seeded layered
+    medium with an elliptic body (seed 2), 8 points per wavelength at
+    1500 m/s, attenuation 0.01 w plus a 20-cell absorbing layer of amplitude
+    w, Neumann top, Sommerfeld sides; nsrc point sources on depth row 2
+    (x = 3 + 4k).  Returns (oracle mesh, m, gamma, w, CSR H, B)."""
+    import sys, os
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "helmholtz.jl_amd"))
+    from helmholtz_amd import media
+    n, h = (1025, 513), 10.0
+    v = media.layered_body_velocity_2d(n, seed=2)
+    m = (1.0 / v**2).ravel(order="F")
+    M = hr.get_regular_mesh([0.0, h * (n[0] - 1), 0.0, h * (n[1] - 1)], [n[0] - 1, n[1] - 1])
+    w = media.omega_for_ppw(1500.0, h, 8)
+    H, g = hr.get_helmholtz_operator(M, m, w, 0.01 * w * np.ones(m.size), True, [20, 20], w, True)
+    B = np.zeros((m.size, nsrc), dtype=complex, order="F")
+    for k, (ix, iz) in enumerate(media.surface_sources_2d(n, nsrc)):
+        B[hr.loc2cs(M.nodes, [ix, iz]) - 1, k] = 1.0 / np.linalg.norm(M.h) ** 2
+    return M, m, g.ravel(order="F"), w, H.tocsr(), B

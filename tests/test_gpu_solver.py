@@ -174,11 +174,33 @@ def _c64_solver(M, m, g, w, neumann, levels, graph=False, autotune=False, cycle=
     return s
 
 
+def test_graph_replay_equals_eager_plain_single_rhs():
+    """The plain complex128 method with one right-hand side (the
+    launch-latency-bound case): every FGMRES restart cycle from the second on
+    replayed as one hipGraph gives bit-identical iterates, iteration counts
+    and preconditioner counts to eager launches, in two calls in a row."""
+    M, m, g, w = small_case((257, 129), seed=33, neumann=True)
+    rng = np.random.default_rng(10)
+    b = rng.standard_normal(m.size) + 1j * rng.standard_normal(m.size)
+    se = _solver(M, m, g, w, True, 4, 0.5, cycle="W")
+    xe, se = ha.solveLinearSystem(None, b, se)
+    sg = _solver(M, m, g, w, True, 4, 0.5, cycle="W")
+    sg.MG.graph = True
+    for _ in range(2):
+        xg, sg = ha.solveLinearSystem(None, b, sg)
+        assert np.array_equal(xe, xg)
+        assert list(sg.last_stats["iters"]) == list(se.last_stats["iters"])
+    assert se.last_stats["max_iters"] > 20
+    assert sg.nPrec == 2 * se.nPrec
+
+
 @pytest.mark.parametrize("basis", ["c128", "c64"])
 def test_graph_replay_equals_eager(basis):
-    """Each FGMRES iteration captured as a hipGraph from the second restart
-    cycle on and replayed (several times per solve, and again in a second
-    call) gives bit-identical iterates to eager launches."""
+    """Each FGMRES restart cycle (K Arnoldi steps, x update, true residual)
+    captured as a hipGraph from the second cycle on and replayed (several
+    times per solve, and again in a second call) gives bit-identical iterates
+    to eager launches; RHS that converge inside a replayed cycle are skipped
+    by the device-side done flags."""
     M, m, g, w = small_case((129, 129), seed=31, neumann=True)
     rng = np.random.default_rng(8)
     B = np.asfortranarray(rng.standard_normal((m.size, 6)) + 1j * rng.standard_normal((m.size, 6)))
@@ -188,6 +210,8 @@ def test_graph_replay_equals_eager(basis):
     Xg, sg = ha.solveLinearSystem(None, B, sg)
     assert se.last_stats["max_iters"] > 20  # several restart cycles replayed
     assert np.array_equal(Xe, Xg)
+    assert list(sg.last_stats["iters"]) == list(se.last_stats["iters"])
+    assert sg.nPrec == se.nPrec
     Xg2, sg = ha.solveLinearSystem(None, B, sg)
     assert np.array_equal(Xe, Xg2)
 

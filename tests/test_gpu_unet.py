@@ -64,6 +64,32 @@ def test_gradient_matches_autograd():
         off += n
 
 
+def test_gradient_across_pixel_chunks_matches_autograd():
+    """Three 65 x 33 images (2145 pixels each): the weight-gradient GEMM's
+    64-pixel chunks and its pixel blocks straddle image boundaries, and every
+    layer's gradient (all channel counts, every tap) must still match float64
+    autograd of the independent torch statement."""
+    pytest.importorskip("torch")
+    net = learned.UNet(seed=6)
+    theta = ur.bf16_round(net.get_params())
+    net.set_params(theta)
+    feats, target = _case(10, n=3, nx=65, ny=33)
+    loss, g = net.loss_grad(feats, target, 65, 33)
+    lref, gref = torch_loss_grad(theta.astype(np.float64), feats.astype(np.float64), target.astype(np.float64), 65, 33)
+    assert loss == pytest.approx(lref, rel=2e-2)
+    err = np.linalg.norm(g - gref) / np.linalg.norm(gref)
+    print(f"gradient (3 x 65 x 33) rel err {err:.2e}")
+    assert err <= 6e-2
+    off = 0
+    for l, (ci, co) in enumerate(ur.SPEC):
+        nw = 9 * co * ci
+        for part, (a, b) in (("weights", (g[off:off + nw], gref[off:off + nw])),
+                             ("bias", (g[off + nw:off + nw + co], gref[off + nw:off + nw + co]))):
+            cos = float(a @ b / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-30))
+            assert cos >= 0.99, (l, part, cos)
+        off += nw + co
+
+
 def test_adam_step_closed_form():
     net = learned.UNet(seed=1)
     theta = net.get_params().astype(np.float64)
