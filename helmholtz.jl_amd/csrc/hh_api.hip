@@ -168,6 +168,7 @@ __global__ void k_zero_cols(hh::cx<double>* __restrict__ x, int64_t N, const int
 
 // ============================================================== operator
 struct hh_op {
+  int dev = 0;  // the GPU the operator's memory and stream live on
   int dim = 2;
   int64_t nn[3] = {1, 1, 1};
   double h[3] = {1, 1, 1};
@@ -1405,6 +1406,7 @@ int hh_op_create(int dim, const int64_t nnodes[3], const double h[3], const doub
       op->sommerfeld = sommerfeld;
       op->bc = bc_order;
       op->g = make_grid(dim, op->nn, op->h, bc_order);
+      HIPCK(hipGetDevice(&op->dev));
       HIPCK(hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking));
       op->m = dalloc<double>(N);
       op->gam = dalloc<double>(N);
@@ -1447,6 +1449,59 @@ int hh_op_destroy(hh_op* op) {
   return guarded([&] { delete op; });
 }
 
+// Staging scratch of the allocation-free operator apply: a fixed device array,
+// input half then output half.  Each GPU the library runs on holds its own
+// copy until the process ends.  In host code `scratch` names a host-side
+// stand-in, never a device address: kernels and copies take the per-device
+// address from scratch_ptr().
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+constexpr size_t kScratchHalf = kScratchBytes / 2;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+constexpr int kMaxDevices = 64;
+void* g_scratch_dev[kMaxDevices] = {};
+
+// device address of `scratch` on GPU dev, looked up once per device with that
+// device current (every GPU holds the array at its own address)
+void* scratch_ptr(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) throw HHError(HH_ERR_ARG, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    int cur = 0;
+    HIPCK(hipGetDevice(&cur));
+    if (cur != dev) HIPCK(hipSetDevice(dev));
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (cur != dev) (void)hipSetDevice(cur);
+    HIPCK(e);
+    if (!ptr) throw HHError(HH_ERR_HIP, "no device address for the scratch array");
+    g_scratch_dev[dev] = ptr;
+  }
+  return g_scratch_dev[dev];
+}
+
+// Y = (H + i shift w^2 m) X on the operator's stream, X and Y device blocks
+void op_apply_device(hh_op* op, double shift, const void* xin, void* yout, int64_t nrhs, int prec) {
+  // i * shift * omega^2
+  const double w2r = op->wre * op->wre - op->wim * op->wim, w2i = 2.0 * op->wre * op->wim;
+  const cx<double> sfac = mk<double>(-shift * w2i, shift * w2r);
+  const double* mm = shift != 0.0 ? op->m : nullptr;
+  if (op->ho) {
+    if (prec != HH_C128) throw HHError(HH_ERR_UNSUPPORTED, "the spread (HO) operator applies in complex128");
+    launch_ho_stencil<double, M_APPLY>(op->g, op->beta_lap, op->beta_mass, (int)nrhs, (const cx<double>*)xin,
+                                       nullptr, (cx<double>*)yout, op->d64, mm, sfac, nullptr, op->stream);
+  } else if (prec == HH_C128)
+    launch_stencil<double, double, double, M_APPLY>(op->g, (int)nrhs, (const cx<double>*)xin, nullptr,
+                                                    (cx<double>*)yout, op->d64, mm, sfac, 0.0, nullptr, op->stream);
+  else
+    launch_stencil<float, float, float, M_APPLY>(op->g, (int)nrhs, (const cx<float>*)xin, nullptr,
+                                                 (cx<float>*)yout, op->d32, mm, sfac, 0.0, nullptr, op->stream);
+  HIPCK(hipGetLastError());
+}
+}  // namespace
+
 int hh_op_apply(hh_op* op, double shift, const void* X, void* Y, int64_t nrhs, int prec, int loc) {
   return guarded([&] {
     if (!op || !X || !Y) throw HHError(HH_ERR_ARG, "null argument");
@@ -1462,23 +1517,31 @@ int hh_op_apply(hh_op* op, double shift, const void* X, void* Y, int64_t nrhs, i
       xin = op->stage_in;
       yout = op->stage_out;
     }
-    // i * shift * omega^2
-    const double w2r = op->wre * op->wre - op->wim * op->wim, w2i = 2.0 * op->wre * op->wim;
-    const cx<double> sfac = mk<double>(-shift * w2i, shift * w2r);
-    const double* mm = shift != 0.0 ? op->m : nullptr;
-    if (op->ho) {
-      if (prec != HH_C128) throw HHError(HH_ERR_UNSUPPORTED, "the spread (HO) operator applies in complex128");
-      launch_ho_stencil<double, M_APPLY>(op->g, op->beta_lap, op->beta_mass, (int)nrhs, (const cx<double>*)xin,
-                                         nullptr, (cx<double>*)yout, op->d64, mm, sfac, nullptr, op->stream);
-    } else if (prec == HH_C128)
-      launch_stencil<double, double, double, M_APPLY>(op->g, (int)nrhs, (const cx<double>*)xin, nullptr,
-                                                      (cx<double>*)yout, op->d64, mm, sfac, 0.0, nullptr,
-                                                      op->stream);
-    else
-      launch_stencil<float, float, float, M_APPLY>(op->g, (int)nrhs, (const cx<float>*)xin, nullptr,
-                                                   (cx<float>*)yout, op->d32, mm, sfac, 0.0, nullptr, op->stream);
-    HIPCK(hipGetLastError());
+    op_apply_device(op, shift, xin, yout, nrhs, prec);
     if (loc == HH_HOST) HIPCK(hipMemcpyAsync(Y, yout, bytes, hipMemcpyDeviceToHost, op->stream));
+    HIPCK(hipStreamSynchronize(op->stream));
+  });
+}
+
+int hh_op_apply_noalloc(hh_op* op, double shift, const void* X, void* Y, int64_t nrhs, int prec) {
+  // hh_op_apply's host path with the staging in the fixed scratch array: no
+  // hipMalloc / hipFree here or in anything it calls.  The stencil writes
+  // every output value, so nothing in the scratch needs clearing first.
+  return guarded([&] {
+    if (!op || !X || !Y) throw HHError(HH_ERR_ARG, "null argument");
+    if (nrhs < 1 || nrhs > 65535) throw HHError(HH_ERR_ARG, "nrhs must be in 1..65535");
+    if (prec != HH_C128 && prec != HH_C64) throw HHError(HH_ERR_ARG, "prec must be HH_C128 or HH_C64");
+    const size_t bytes = (size_t)op->g.N * nrhs * csz(prec);
+    if (bytes > kScratchHalf)
+      throw HHError(HH_ERR_ARG, "N * nrhs * element size = " + std::to_string(bytes) +
+                                    " bytes exceeds the allocation-free apply's limit of " +
+                                    std::to_string(kScratchHalf) + " bytes per block");
+    char* base = (char*)scratch_ptr(op->dev);
+    void* xin = base;
+    void* yout = base + kScratchHalf;
+    HIPCK(hipMemcpyAsync(xin, X, bytes, hipMemcpyHostToDevice, op->stream));
+    op_apply_device(op, shift, xin, yout, nrhs, prec);
+    HIPCK(hipMemcpyAsync(Y, yout, bytes, hipMemcpyDeviceToHost, op->stream));
     HIPCK(hipStreamSynchronize(op->stream));
   });
 }

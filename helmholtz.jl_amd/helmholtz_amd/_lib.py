@@ -26,7 +26,7 @@ EXPORTS = (
     "hh_dataset_size", "hh_solver_capture_begin", "hh_solver_capture_count", "hh_dataset_add_from_solver",
     "hh_unet_train", "hh_unet_eval", "hh_solver_attach_unet", "hh_stream_solve", "hh_solver_precondition",
     "hh_solver_time_kernel", "hh_solver_kernel_time", "hh_op_create_ho", "hh_sommerfeld_bc",
-    "hh_abl_faces", "hh_solver_learned_policy", "hh_solver_learned_workspace",
+    "hh_abl_faces", "hh_solver_learned_policy", "hh_solver_learned_workspace", "hh_op_apply_noalloc",
 )
 HH_LEARN_TRIAL, HH_LEARN_ALWAYS = 0, 1
 HH_RELAX_JAC, HH_RELAX_JAC_GMRES = 0, 1
@@ -113,6 +113,7 @@ def lib():
     L.hh_unet_eval.argtypes = [vp, vp, i64, i64, pf]
     L.hh_solver_attach_unet.argtypes = [vp, vp, i32]
     L.hh_solver_learned_policy.argtypes = [vp, i32]
+    L.hh_op_apply_noalloc.argtypes = [vp, C.c_double, vp, vp, i64, i32]
     L.hh_solver_learned_workspace.argtypes = [vp, C.POINTER(vp), C.POINTER(vp)]
     L.hh_solver_precondition.argtypes = [vp, vp, vp, i64, i32]
     L.hh_solver_time_kernel.argtypes = [vp, i32, i32]

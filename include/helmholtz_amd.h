@@ -160,6 +160,18 @@ int hh_op_destroy(hh_op* op);
  * prec = HH_C128 | HH_C64 (element type of X and Y), loc = HH_HOST | HH_DEVICE. */
 int hh_op_apply(hh_op* op, double shift, const void* X, void* Y, int64_t nrhs, int prec, int loc);
 
+/* hh_op_apply on host blocks X, Y that allocates nothing: no counterpart in
+ * the reference; for callers that cannot allocate device memory (code under
+ * hipGraph capture, tight serving loops, harnesses that forbid hipMalloc).  X
+ * and Y are staged through a fixed scratch array of 64 MiB in device memory
+ * (32 MiB in, 32 MiB out), which each GPU holds from the library's load to
+ * the end of the process; a call with N * nrhs * (16 | 8) bytes above
+ * 32 MiB (33,554,432) is refused with HH_ERR_ARG.  The operator itself is
+ * created beforehand (hh_op_create allocates).  Every call on one GPU shares
+ * that GPU's scratch: calls that overlap on one GPU race.  Results equal
+ * hh_op_apply's bit for bit. */
+int hh_op_apply_noalloc(hh_op* op, double shift, const void* X, void* Y, int64_t nrhs, int prec);
+
 /* Shifted-Laplacian multigrid solver bound to op (which must outlive it).
  * Replaces getShiftedLaplacianMultigridSolver(helmParam, MG, shift, Krylov,
  * inner, verbose) (src/ShiftedLaplacianMultigridSolver.jl:24-30).
