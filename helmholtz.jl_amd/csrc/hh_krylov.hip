@@ -256,6 +256,36 @@ __device__ __forceinline__ cx<double> hz_eval(const Grid& g, const HzOps<DIM>& o
                 o.d);
 }
 
+// 2-D: the same operands from a sliding window of z in LDS (position q mod R,
+// R a power of two): every operand of node p lies in [p - n1, p + n1]
+__device__ __forceinline__ void hz_load_ring(const Grid& g, const cx<float>* zr, int rmask,
+                                             const cx<double>* __restrict__ d, int ci, int cj, int64_t p,
+                                             HzOps<2>& o) {
+  const int n1 = g.n1, n2 = g.n2;
+  const int64_t row = (int64_t)cj * n1;
+  o.c = zr[p & rmask];
+  o.x1l = zr[(row + lo_nb(ci, g.bc)) & rmask];
+  o.x1r = zr[(row + hi_nb(ci, n1, g.bc)) & rmask];
+  o.x2l = zr[((int64_t)lo_nb(cj, g.bc) * n1 + ci) & rmask];
+  o.x2r = zr[((int64_t)hi_nb(cj, n2, g.bc) * n1 + ci) & rmask];
+  o.d = d[p];
+}
+
+// The window: before the block's step over [pb, pb + S) (S = 256 U nodes) it
+// holds z[q] for q in [pb - n1, pb + S + n1); during the step every thread
+// loads its share of the next S values [pb + S + n1, pb + 2S + n1) from HBM
+// and stores them after its own reads.  With R >= 2 n1 + 2 S a stored
+// position and a read one differ by less than R, so they never share a
+// slot: one barrier per step orders the stores before the next step's reads.
+// Each z value crosses HBM once instead of up to three times (the row above
+// has left the caches when the block comes back to it).
+template <int U>
+__device__ __forceinline__ void zring_fill(cx<float>* zr, int rmask, const cx<float>* __restrict__ zb, int64_t N,
+                                           int64_t lo, int64_t hi) {
+  for (int64_t q = lo + threadIdx.x; q < hi; q += 256)
+    if (q >= 0 && q < N) zr[q & rmask] = zb[q];
+}
+
 // node coordinates of p and their advance by `by` positions (a multiple of
 // the block's 256 threads)
 struct NodeWalk {
@@ -287,19 +317,25 @@ struct NodeWalk {
 // with a load-use chain per basis vector, the compiler waited on each load
 // in turn and the passes were bound by load latency, not HBM bandwidth.  A
 // node past the chunk end (U = 2) loads node p's values and contributes 0.
-// U = 1 keeps the summation order of the stored-w kernels.  EARLY (all NV
+// U = 1 keeps the summation order of the stored-w kernels.  With the z
+// window (RING, 2-D) every thread runs every step of its block; a thread
+// whose nodes have ended reads node p0's basis values and diagonal, window
+// slots of no meaning, and contributes nothing.  EARLY (all NV
 // basis values held in registers at once) for NV <= 10; longer restarts
 // read one basis value at a time within their register budget.
 
 // pass 1: part = V^H (H z), partn = ||H z||^2
-template <int NV, typename TV, int U, int DIM>
+template <int NV, typename TV, int U, int DIM, bool RING>
 __global__ __launch_bounds__(256) void k_dots_hz(KrylovWork kw, Grid g, const cx<TV>* __restrict__ V,
                                                  const cx<float>* __restrict__ Z, const cx<double>* __restrict__ d,
-                                                 int j) {
+                                                 int j, int rmask) {
   constexpr bool EARLY = NV <= 10;
+  constexpr int S = 256 * U;
+  static_assert(!RING || DIM == 2, "the z window serves the 2-D stencil");
   const int b = blockIdx.x, c = blockIdx.y;
   if (kw.done[b]) return;
   __shared__ double sh[4];
+  extern __shared__ cx<float> zr[];
   int64_t p0, p1;
   chunk_range(kw.N, kw.nchunk, c, &p0, &p1);
   const int64_t stride = kw.N * kw.nrhs;
@@ -313,17 +349,38 @@ __global__ __launch_bounds__(256) void k_dots_hz(KrylovWork kw, Grid g, const cx
   NodeWalk nw[U];
 #pragma unroll
   for (int u = 0; u < U; ++u) nw[u].init(g, p0 + threadIdx.x + 256 * u);
-  for (int64_t p = p0 + threadIdx.x; p < p1; p += 256 * U) {
+  if constexpr (RING) {
+    zring_fill<U>(zr, rmask, zb, kw.N, p0 - g.n1, p0 + S + g.n1);
+    __syncthreads();
+  }
+  // block-uniform steps (the window's barrier needs every thread in each step)
+  for (int64_t pb = p0; pb < p1; pb += S) {
+    const int64_t p = pb + threadIdx.x;
+    if constexpr (!RING) {
+      if (p >= p1) break;  // no barrier in this loop: a thread leaves when its nodes end
+    }
     HzOps<DIM> ho[U];
     int64_t qq[U];
     bool ok[U];
     cx<TV> vv[EARLY ? U : 1][EARLY ? NV : 1];
+    cx<float> zn[U];
+    int64_t qn[U];
+    if constexpr (RING) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        qn[u] = pb + S + g.n1 + threadIdx.x + 256 * u;
+        zn[u] = qn[u] < kw.N ? zb[qn[u]] : mk<float>(0.f, 0.f);
+      }
+    }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       ok[u] = p + 256 * u < p1;
-      qq[u] = ok[u] ? p + 256 * u : p;
-      hz_load<DIM>(g, zb, d, ok[u] ? nw[u].ci : nw[0].ci, ok[u] ? nw[u].cj : nw[0].cj,
-                   ok[u] ? nw[u].ck : nw[0].ck, qq[u], ho[u]);
+      qq[u] = ok[u] ? p + 256 * u : (p < p1 ? p : p0);
+      const int ci = ok[u] ? nw[u].ci : nw[0].ci, cj = ok[u] ? nw[u].cj : nw[0].cj;
+      if constexpr (RING)
+        hz_load_ring(g, zr, rmask, d, p < p1 ? ci : nw[0].ci, p < p1 ? cj : nw[0].cj, qq[u], ho[u]);
+      else
+        hz_load<DIM>(g, zb, d, ci, cj, ok[u] ? nw[u].ck : nw[0].ck, qq[u], ho[u]);
     }
     if constexpr (EARLY) {
 #pragma unroll
@@ -345,6 +402,12 @@ __global__ __launch_bounds__(256) void k_dots_hz(KrylovWork kw, Grid g, const cx
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) nw[u].step(g, 256 * U);
+    if constexpr (RING) {
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (qn[u] < kw.N) zr[qn[u] & rmask] = zn[u];
+      __syncthreads();
+    }
   }
 #pragma unroll
   for (int i = 0; i < NV; ++i) {
@@ -361,16 +424,19 @@ __global__ __launch_bounds__(256) void k_dots_hz(KrylovWork kw, Grid g, const cx
 
 // pass 2: w' = H z - V h1 (stored in the basis precision), partn = ||w'||^2
 // of the unrounded w'
-template <int NV, typename TV, int U, int DIM>
+template <int NV, typename TV, int U, int DIM, bool RING>
 __global__ __launch_bounds__(256) void k_update_lazy_hz(KrylovWork kw, Grid g, const cx<TV>* __restrict__ V,
                                                         const cx<float>* __restrict__ Z,
                                                         const cx<double>* __restrict__ d,
-                                                        cx<TV>* __restrict__ wout, int j) {
+                                                        cx<TV>* __restrict__ wout, int j, int rmask) {
   constexpr bool EARLY = NV <= 10;
+  constexpr int S = 256 * U;
+  static_assert(!RING || DIM == 2, "the z window serves the 2-D stencil");
   const int b = blockIdx.x, c = blockIdx.y;
   if (kw.done[b]) return;
   __shared__ double sh[4];
   __shared__ cx<double> hs[NV];
+  extern __shared__ cx<float> zr[];
   // coefficients past j are 0: those terms subtract exact zeros
   if (threadIdx.x < NV) hs[threadIdx.x] = threadIdx.x <= j ? kw.h1s[threadIdx.x * kw.nrhs + b] : mk<double>(0.0, 0.0);
   __syncthreads();
@@ -384,17 +450,38 @@ __global__ __launch_bounds__(256) void k_update_lazy_hz(KrylovWork kw, Grid g, c
   NodeWalk nw[U];
 #pragma unroll
   for (int u = 0; u < U; ++u) nw[u].init(g, p0 + threadIdx.x + 256 * u);
-  for (int64_t p = p0 + threadIdx.x; p < p1; p += 256 * U) {
+  if constexpr (RING) {
+    zring_fill<U>(zr, rmask, zb, kw.N, p0 - g.n1, p0 + S + g.n1);
+    __syncthreads();
+  }
+  // block-uniform steps (the window's barrier needs every thread in each step)
+  for (int64_t pb = p0; pb < p1; pb += S) {
+    const int64_t p = pb + threadIdx.x;
+    if constexpr (!RING) {
+      if (p >= p1) break;  // no barrier in this loop: a thread leaves when its nodes end
+    }
     HzOps<DIM> ho[U];
     int64_t qq[U];
     bool ok[U];
     cx<TV> vv[EARLY ? U : 1][EARLY ? NV : 1];
+    cx<float> zn[U];
+    int64_t qn[U];
+    if constexpr (RING) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        qn[u] = pb + S + g.n1 + threadIdx.x + 256 * u;
+        zn[u] = qn[u] < kw.N ? zb[qn[u]] : mk<float>(0.f, 0.f);
+      }
+    }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       ok[u] = p + 256 * u < p1;
-      qq[u] = ok[u] ? p + 256 * u : p;
-      hz_load<DIM>(g, zb, d, ok[u] ? nw[u].ci : nw[0].ci, ok[u] ? nw[u].cj : nw[0].cj,
-                   ok[u] ? nw[u].ck : nw[0].ck, qq[u], ho[u]);
+      qq[u] = ok[u] ? p + 256 * u : (p < p1 ? p : p0);
+      const int ci = ok[u] ? nw[u].ci : nw[0].ci, cj = ok[u] ? nw[u].cj : nw[0].cj;
+      if constexpr (RING)
+        hz_load_ring(g, zr, rmask, d, p < p1 ? ci : nw[0].ci, p < p1 ? cj : nw[0].cj, qq[u], ho[u]);
+      else
+        hz_load<DIM>(g, zb, d, ci, cj, ok[u] ? nw[u].ck : nw[0].ck, qq[u], ho[u]);
     }
     if constexpr (EARLY) {
 #pragma unroll
@@ -420,6 +507,12 @@ __global__ __launch_bounds__(256) void k_update_lazy_hz(KrylovWork kw, Grid g, c
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) nw[u].step(g, 256 * U);
+    if constexpr (RING) {
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (qn[u] < kw.N) zr[qn[u] & rmask] = zn[u];
+      __syncthreads();
+    }
   }
   const double t = block_sum(nn, sh);
   if (threadIdx.x == 0) kw.partn[(int64_t)c * kw.nrhs + b] = t;
@@ -733,13 +826,26 @@ static void arnoldi_nv(const KrylovWork& w, const cx<TV>* V, cx<double>* wv, int
   if (t1) kt->begin(s);
   // nodes per thread per iteration: 2 for a complex64 basis up to NV = 10
   constexpr int U = std::is_same<TV, float>::value && NV <= 10 ? 2 : 1;
+  // 2-D: z through the LDS window when 2 n1 + 2 S nodes (S = 256 U) fit in
+  // a power-of-two ring of at most 8192 complex64 values (64 KiB)
+  int rnodes = 0;
+  if (hz && hz->g.dim == 2) {
+    const int64_t need = 2 * (int64_t)hz->g.n1 + 2 * 256 * U;
+    int64_t r = 1;
+    while (r < need) r <<= 1;
+    if (r <= 8192) rnodes = (int)r;
+  }
+  const size_t rbytes = sizeof(cx<float>) * (size_t)rnodes;
   if (hz) {
-    if (hz->g.dim == 2)
-      hipLaunchKernelGGL((k_dots_hz<NV, TV, U, 2>), dim3(w.nrhs, w.nchunk), dim3(256), 0, s, w, hz->g, V, hz->z,
-                         hz->d, j);
+    if (rnodes)
+      hipLaunchKernelGGL((k_dots_hz<NV, TV, U, 2, true>), dim3(w.nrhs, w.nchunk), dim3(256), rbytes, s, w, hz->g,
+                         V, hz->z, hz->d, j, rnodes - 1);
+    else if (hz->g.dim == 2)
+      hipLaunchKernelGGL((k_dots_hz<NV, TV, U, 2, false>), dim3(w.nrhs, w.nchunk), dim3(256), 0, s, w, hz->g, V,
+                         hz->z, hz->d, j, 0);
     else
-      hipLaunchKernelGGL((k_dots_hz<NV, TV, U, 3>), dim3(w.nrhs, w.nchunk), dim3(256), 0, s, w, hz->g, V, hz->z,
-                         hz->d, j);
+      hipLaunchKernelGGL((k_dots_hz<NV, TV, U, 3, false>), dim3(w.nrhs, w.nchunk), dim3(256), 0, s, w, hz->g, V,
+                         hz->z, hz->d, j, 0);
   } else if constexpr (std::is_same<TV, double>::value) {
     if (w.lazy)
       hipLaunchKernelGGL((k_dots<NV, true, double>), cgrid(w), dim3(256), 0, s, w, V, wv, j, nullptr);
@@ -750,12 +856,15 @@ static void arnoldi_nv(const KrylovWork& w, const cx<TV>* V, cx<double>* wv, int
   hipLaunchKernelGGL(k_reduce_dots, dim3(j + (w.lazy ? 2 : 1), w.nrhs), dim3(64), 0, s, w, w.h1, j);
   if (t2) kt->begin(s);
   if (hz) {
-    if (hz->g.dim == 2)
-      hipLaunchKernelGGL((k_update_lazy_hz<NV, TV, U, 2>), dim3(w.nrhs, w.nchunk), dim3(256), 0, s, w, hz->g, V,
-                         hz->z, hz->d, vnext, j);
+    if (rnodes)
+      hipLaunchKernelGGL((k_update_lazy_hz<NV, TV, U, 2, true>), dim3(w.nrhs, w.nchunk), dim3(256), rbytes, s, w,
+                         hz->g, V, hz->z, hz->d, vnext, j, rnodes - 1);
+    else if (hz->g.dim == 2)
+      hipLaunchKernelGGL((k_update_lazy_hz<NV, TV, U, 2, false>), dim3(w.nrhs, w.nchunk), dim3(256), 0, s, w,
+                         hz->g, V, hz->z, hz->d, vnext, j, 0);
     else
-      hipLaunchKernelGGL((k_update_lazy_hz<NV, TV, U, 3>), dim3(w.nrhs, w.nchunk), dim3(256), 0, s, w, hz->g, V,
-                         hz->z, hz->d, vnext, j);
+      hipLaunchKernelGGL((k_update_lazy_hz<NV, TV, U, 3, false>), dim3(w.nrhs, w.nchunk), dim3(256), 0, s, w,
+                         hz->g, V, hz->z, hz->d, vnext, j, 0);
   } else if constexpr (std::is_same<TV, double>::value) {
     if (w.lazy) {
       // w' goes straight into the next basis slot (kept unnormalised)
