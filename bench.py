@@ -96,21 +96,21 @@ def c3_problem():
 
 def c3_lines(log, c64_basis=True):
     """Complex64 vs complex128 preconditioner on the 3-D case: 5-level W(2,2)
-    cycle (a 4-level V cycle does not converge at 8 ppw; 4-level W saves about
-    an eighth of the iterations but adds eight dense 8125-node coarsest
-    solves per cycle), complex128 FGMRES(8) to 1e-6; with
-    ``c64_basis`` also the complex64 preconditioner with the FGMRES basis
-    stored in complex64 (``c64_c64basis_*``)."""
+    cycle, complex128 FGMRES(8) to 1e-6; the complex64 preconditioner also
+    with a 4-level W(2,2) cycle (``c64_4levelW_*``: fewer iterations, eight
+    dense 8125-node coarsest solves per cycle; a 4-level V cycle does not
+    converge at 8 ppw); with ``c64_basis`` also the complex64 preconditioner
+    with the FGMRES basis stored in complex64 (``c64_c64basis_*``)."""
     import helmholtz_amd as ha
     from helmholtz_amd.device import DeviceArray
     hp, B = c3_problem()
     Bd = DeviceArray.from_numpy(B)
     out = {"workload": "3-D 193x193x97, 32 sources, 8 ppw, 5-level W(2,2), FGMRES(8), 1e-6 true residual"}
-    runs = [("c64", "c128", "c64"), ("c128", "c128", "c128")]
+    runs = [("c64", "c128", "c64", 5), ("c128", "c128", "c128", 5), ("c64", "c128", "c64_4levelW", 4)]
     if c64_basis:
-        runs.append(("c64", "c64", "c64_c64basis"))
-    for prec, basis, key in runs:
-        mg = ha.getMGparam(complex, int, 5, 1, 100, 1e-6, "Jac", 0.8, 2, 2, "W", "NoMUMPS", 0.5, 0.0)
+        runs.append(("c64", "c64", "c64_c64basis", 5))
+    for prec, basis, key, levels in runs:
+        mg = ha.getMGparam(complex, int, levels, 1, 100, 1e-6, "Jac", 0.8, 2, 2, "W", "NoMUMPS", 0.5, 0.0)
         mg.prec = prec
         mg.basis = basis
         s = ha.getShiftedLaplacianMultigridSolver(hp, mg, 0.5, "GMRES", 8)
