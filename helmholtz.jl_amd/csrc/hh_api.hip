@@ -800,16 +800,16 @@ struct Cycle {
       // leg forms it again from b (L.cur only fixes the ping-pong parity)
       const bool w2 = ncorr == 2;
       if (td) S->kt.begin(st);
-      launch_down_fused<T>(L.g, C.g, nrhs, b, (const cx<T>*)L.d, (const cx<T>*)L.f, w2 ? nullptr : (cx<T>*)L.cur,
-                           (cx<T>*)C.b, skip, st);
+      launch_down_fused<T>(L.g, C.g, nrhs, b, (const cx<T>*)L.d, (const cx<T>*)L.f, S->p.relax,
+                           w2 ? nullptr : (cx<T>*)L.cur, (cx<T>*)C.b, skip, st);
       if (td) S->kt.end(st, act * ((w2 ? 1 : 2) * Nf * s + Nc * s) + 2 * Nf * s);
       C.cur = nullptr;
       run(l + 1, (const cx<T>*)C.b, true, nullptr);
       if (w2) {
         void* o = other(L);
         if (tm) S->kt.begin(st);
-        launch_mid_fused<T>(L.g, C.g, nrhs, (const cx<T>*)C.cur, b, (const cx<T>*)L.d, (const cx<T>*)L.f, (cx<T>*)o,
-                            (cx<T>*)C.b, skip, st);
+        launch_mid_fused<T>(L.g, C.g, nrhs, (const cx<T>*)C.cur, b, (const cx<T>*)L.d, (const cx<T>*)L.f,
+                            S->p.relax, (cx<T>*)o, (cx<T>*)C.b, skip, st);
         if (tm) S->kt.end(st, act * (2 * Nf * s + 2 * Nc * s) + 2 * Nf * s);
         L.cur = o;
         C.cur = nullptr;
@@ -817,9 +817,9 @@ struct Cycle {
       }
       void* o = other(L);
       if (tu) S->kt.begin(st);
-      launch_up_fused<T>(L.g, C.g, nrhs, (const cx<T>*)L.cur, (const cx<T>*)C.cur, b, (const cx<T>*)L.d,
-                         (const cx<T>*)L.f, (cx<T>*)o, skip, st);
-      if (tu) S->kt.end(st, act * (3 * Nf * s + Nc * s) + 2 * Nf * s);
+      launch_up_fused<T>(L.g, C.g, nrhs, (const cx<T>*)L.cur, (const cx<T>*)C.cur, b, (const cx<T>*)L.f,
+                         S->p.relax, (cx<T>*)o, skip, st);
+      if (tu) S->kt.end(st, act * (3 * Nf * s + Nc * s) + Nf * s);  // the up leg reads no diagonal
       L.cur = o;
       return;
     }

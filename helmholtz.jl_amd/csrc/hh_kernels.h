@@ -72,17 +72,19 @@ bool fused3_ok(const Grid& gf);
 void launch_up_fused3(const Grid& gf, const Grid& gc, int nrhs, const cx<float>* x, const cx<float>* ec,
                       const cx<float>* b, const cx<float>* d, const cx<float>* f, cx<float>* xo, const int* skip,
                       hipStream_t s);
+// f = relax D^-1 (the level's Jacobi factor) and relax itself;
 // xo = null: the smoothed iterate is not stored (the middle leg recomputes it)
 template <typename T>
 void launch_down_fused(const Grid& gf, const Grid& gc, int nrhs, const cx<T>* b, const cx<T>* d, const cx<T>* f,
-                       cx<T>* xo, cx<T>* bco, const int* skip, hipStream_t s);
+                       double relax, cx<T>* xo, cx<T>* bco, const int* skip, hipStream_t s);
 // the middle leg of a W visit: recomputes the down leg's x2 from b
 template <typename T>
 void launch_mid_fused(const Grid& gf, const Grid& gc, int nrhs, const cx<T>* ec, const cx<T>* b, const cx<T>* d,
-                      const cx<T>* f, cx<T>* xo, cx<T>* bco, const int* skip, hipStream_t s);
+                      const cx<T>* f, double relax, cx<T>* xo, cx<T>* bco, const int* skip, hipStream_t s);
+// the up leg's two sweeps need only f and relax (no diagonal)
 template <typename T>
 void launch_up_fused(const Grid& gf, const Grid& gc, int nrhs, const cx<T>* x, const cx<T>* ec, const cx<T>* b,
-                     const cx<T>* d, const cx<T>* f, cx<T>* xo, const int* skip, hipStream_t s);
+                     const cx<T>* f, double relax, cx<T>* xo, const int* skip, hipStream_t s);
 
 // ---- per-kernel timing inside a solve ----
 // Brackets every launch of one chosen kernel (at one grid level) with HIP

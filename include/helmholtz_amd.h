@@ -218,7 +218,9 @@ int hh_solver_precondition(hh_solver* s, const void* V, void* Z, int64_t nrhs, i
  *          where the smoothed iterate is not stored)
  *   MID    a (2 N s + 2 Nc s) + 2 N s  (W levels: b and e_c read, the
  *          smoothed iterate recomputed from b, x' and b_c written)
- *   UP     a (3 N s + Nc s) + 2 N s    APPLY a N (s_z + 16) + 16 N
+ *   UP     a (3 N s + Nc s) + N s  (x, b, e_c read, x'' written; the
+ *          sweeps need the Jacobi factor, not the diagonal)
+ *   APPLY  a N (s_z + 16) + 16 N
  *   DOTS   16 a N (j + 2)              UPDATE 16 a N (j + 3)  (Arnoldi step j)
  * With prec = HH_C64 on the 5/7-point operator the passes form H z from the
  * complex64 z (v = 16 or 8 B per basis value, t = 8 B for a staged copy with

@@ -445,7 +445,13 @@ def test_fused_up3_equals_unfused(nodes, cycle):
 @pytest.mark.parametrize("prec", ["c64", "c128"])
 def test_fused_legs_equal_unfused(nodes, prec):
     """LDS-tiled fused down/middle/up legs give the same cycle as one kernel
-    per step (tiles that straddle the grid edge included)."""
+    per step (tiles that straddle the grid edge included).  The fused legs
+    write a sweep as (1 - relax) x + F (b + ih1 sx + ih2 sy) (D the
+    diagonal, F D = relax), the separate kernels as x + F (b - A x): equal in
+    exact arithmetic, so the two cycles agree to rounding (bars: 2e-6 in
+    complex64, 1e-13 in complex128; measured about 2e-7 and 6e-16), and the
+    complex64 fused cycle stays within 1e-6 of the complex128 oracle cycle
+    (measured 1.2e-7 to 2e-7, as the separate kernels)."""
     M, m, g, w = small_case(nodes, seed=63, neumann=True)
     rng = np.random.default_rng(64)
     nr = 2
@@ -459,6 +465,13 @@ def test_fused_legs_equal_unfused(nodes, prec):
     err = np.linalg.norm(zs[0] - zs[1]) / np.linalg.norm(zs[1])
     print(f"fused vs unfused {nodes} {prec}: {err:.2e}")
     assert err <= (2e-6 if prec == "c64" else 1e-13)
+    if prec == "c64" and nodes[0] < 600:
+        p = hr.HelmholtzParam(M, g, m, w, True, True)
+        hier = sr.build_hierarchy(p, levels, 0.5)
+        zo = sr.vcycle(hier, 0, V[:, 0], 0.8, 2, 2, "W")
+        eo = np.linalg.norm(zs[0][:, 0] - zo) / np.linalg.norm(zo)
+        print(f"fused c64 vs oracle {nodes}: {eo:.2e}")
+        assert eo <= 1e-6
 
 
 @pytest.mark.gpu
@@ -495,8 +508,8 @@ def test_kernel_timing_counts_launches_and_keeps_results(kernel, level):
         # per active RHS x, b, e_c read and x written (8 B values), plus the
         # shared diagonal and Jacobi factor
         Nc = 65 * 65
-        assert t["bytes"] <= t["launches"] * (nr * (3 * N * 8.0 + Nc * 8.0) + 2 * N * 8.0) + 1.0
-        assert t["bytes"] >= t["launches"] * ((3 * N * 8.0 + Nc * 8.0) + 2 * N * 8.0) - 1.0
+        assert t["bytes"] <= t["launches"] * (nr * (3 * N * 8.0 + Nc * 8.0) + N * 8.0) + 1.0
+        assert t["bytes"] >= t["launches"] * ((3 * N * 8.0 + Nc * 8.0) + N * 8.0) - 1.0
     ha.time_kernel(s1, "none")
     assert ha.kernel_time(s1)["launches"] == 0
 
