@@ -56,6 +56,7 @@ struct Geo {
   static_assert(HT % NW == 0, "rows per wavefront");
   static_assert(FTX % 2 == 0 && TY % 2 == 0, "owned tile must be even");
   static_assert(PW * PH <= NPIX, "coarse patch must fit in one tile array");
+  static_assert(PW * PH >= NT, "every thread stores a patch value");
 };
 
 // Per-block tile coordinates and per-thread column data
@@ -94,22 +95,48 @@ __device__ __forceinline__ void load_rows(const cx<T>* __restrict__ fld, const T
   }
 }
 
-// x + P e_c on this thread's positions, staged into sx.  The coarse patch is
-// first copied into LDS (sc) with coalesced loads; positions whose values
-// never reach an owned output are clamped into the patch.  Bilinear weights
-// per axis are (1) at even fine indices and (1/2, 1/2) at odd ones; the even
-// case adds a zero-weighted second term so all lanes run the same code.
+// The coarse patch a block's prolongation reads: PW x PH values, NP per
+// thread, loaded with coalesced loads into registers (load_patch) — the
+// middle leg issues them with its first loads, so their latency hides behind
+// its two smoothing phases — and copied into LDS (sc) where it is used.
+// Positions whose values never reach an owned output are clamped into the
+// patch.
+template <typename T>
+struct Patch {  // two values per thread (PW x PH <= 2 NT in both tile shapes)
+  cx<T> v0, v1;
+};
+
 template <typename T, int H>
-__device__ __forceinline__ void stage_prolong(const cx<T>* __restrict__ cr, const Grid& gc, const Grid& gf,
-                                              const TileCtx& c, const cx<T> (&vx)[Geo<T, H>::KR], cx<T>* sx,
-                                              cx<T>* sc) {
+__device__ __forceinline__ cx<T> patch_value(const cx<T>* __restrict__ cr, const Grid& gc, int cx0, int cy0, int q) {
+  constexpr int PW = Geo<T, H>::PW, PH = Geo<T, H>::PH;
+  const int qq = q < PW * PH ? q : 0;
+  const int I = min(cx0 + qq % PW, gc.n1 - 1), J = min(cy0 + qq / PW, gc.n2 - 1);
+  return cr[(int64_t)J * gc.n1 + I];
+}
+
+template <typename T, int H>
+__device__ __forceinline__ Patch<T> load_patch(const cx<T>* __restrict__ cr, const Grid& gc, const TileCtx& c) {
+  typedef Geo<T, H> G;
+  static_assert(G::PW * G::PH <= 2 * G::NT, "two patch values per thread");
+  const int cx0 = (c.fx0 - H > 0 ? c.fx0 - H : 0) >> 1, cy0 = (c.fy0 - H > 0 ? c.fy0 - H : 0) >> 1;
+  Patch<T> p;
+  p.v0 = patch_value<T, H>(cr, gc, cx0, cy0, threadIdx.x);
+  p.v1 = patch_value<T, H>(cr, gc, cx0, cy0, threadIdx.x + G::NT);
+  return p;
+}
+
+// x + P e_c on this thread's positions, staged into sx, from the patch in
+// registers (copied into sc first).  Bilinear weights per axis are (1) at
+// even fine indices and (1/2, 1/2) at odd ones; the even case adds a
+// zero-weighted second term so all lanes run the same code.
+template <typename T, int H>
+__device__ __forceinline__ void prolong_patch(const Patch<T>& pv, const Grid& gf, const TileCtx& c,
+                                              const cx<T> (&vx)[Geo<T, H>::KR], cx<T>* sx, cx<T>* sc) {
   typedef Geo<T, H> G;
   constexpr int PW = G::PW, PH = G::PH;
   const int cx0 = (c.fx0 - H > 0 ? c.fx0 - H : 0) >> 1, cy0 = (c.fy0 - H > 0 ? c.fy0 - H : 0) >> 1;
-  for (int q = threadIdx.x; q < PW * PH; q += G::NT) {
-    const int I = min(cx0 + q % PW, gc.n1 - 1), J = min(cy0 + q / PW, gc.n2 - 1);
-    sc[q] = cr[(int64_t)J * gc.n1 + I];
-  }
+  sc[threadIdx.x] = pv.v0;  // PW x PH >= NT
+  if (threadIdx.x + G::NT < PW * PH) sc[threadIdx.x + G::NT] = pv.v1;
   __syncthreads();
   const int i0 = min(max((c.gi >> 1) - cx0, 0), PW - 2);
   const T wi0 = (c.gi & 1) ? (T)0.5 : (T)1, wi1 = (c.gi & 1) ? (T)0.5 : (T)0;
@@ -250,6 +277,7 @@ __global__ __launch_bounds__(FusedCfg<T>::NT) void k_mid_fused(TileMap tm, Grid 
   load_rows<T, H, NW, KR>(b + off, c, gf, rb);
   load_rows<T, H, NW, KR>(d, c, gf, rd);
   load_rows<T, H, NW, KR>(f, c, gf, rf);
+  const Patch<T> pv = load_patch<T, H>(ec + (int64_t)c.rhs * gc.N, gc, c);
   // x1 = F b on the whole tile
 #pragma unroll
   for (int k = 0; k < KR; ++k) s1[(c.wv + NW * k) * 64 + c.lane] = rf[k] * rb[k];
@@ -267,7 +295,7 @@ __global__ __launch_bounds__(FusedCfg<T>::NT) void k_mid_fused(TileMap tm, Grid 
 #pragma unroll
   for (int k = 0; k < KR; ++k) vx[k] = s2[(c.wv + NW * k) * 64 + c.lane];
   __syncthreads();
-  stage_prolong<T, H>(ec + (int64_t)c.rhs * gc.N, gc, gf, c, vx, s1, s2);
+  prolong_patch<T, H>(pv, gf, c, vx, s1, s2);
   __syncthreads();
   // residual on the halo-1 region over x' (s1); owned x' is stored
   const bool in2 = c.lane >= 2 && c.lane < 62, own_c = c.lane >= H && c.lane < H + G::FTX && c.fx0 - H + c.lane < gf.n1;
@@ -305,7 +333,8 @@ __global__ __launch_bounds__(FusedCfg<T>::NT) void k_up_fused(TileMap tm, Grid g
   load_rows<T, H, NW, KR>(x + off, c, gf, vx);
   load_rows<T, H, NW, KR>(b + off, c, gf, rb);
   load_rows<T, H, NW, KR>(f, c, gf, rf);
-  stage_prolong<T, H>(ec + (int64_t)c.rhs * gc.N, gc, gf, c, vx, sx, s3);
+  const Patch<T> pv = load_patch<T, H>(ec + (int64_t)c.rhs * gc.N, gc, c);
+  prolong_patch<T, H>(pv, gf, c, vx, sx, s3);
   __syncthreads();
   const bool in1 = c.lane >= 1 && c.lane < 63, own_c = c.lane >= H && c.lane < H + G::FTX && c.fx0 - H + c.lane < gf.n1;
 #pragma unroll
