@@ -419,6 +419,9 @@ def main():
     ap.add_argument("--profile-batched", type=int, default=0, metavar="CYCLES",
                     help="profiling mode: only the batched complex64 solve of the 1025x513 workload, stopped "
                          "after CYCLES FGMRES restart cycles (no warmup, no other lines)")
+    ap.add_argument("--profile-3d", type=int, default=0, metavar="CYCLES",
+                    help="profiling mode: only the 3-D complex64 solve (193x193x97, 32 sources), stopped after "
+                         "CYCLES FGMRES(8) restart cycles")
     ap.add_argument("--profile-training", action="store_true",
                     help="profiling mode: only the training of the learned correction (pairs from 513x513 media)")
     args = ap.parse_args()
@@ -434,6 +437,16 @@ def main():
     def log(msg):
         print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
+    if args.profile_3d:
+        hp3, B3 = c3_problem()
+        mg = ha.getMGparam(complex, int, 5, 1, args.profile_3d, 1e-6, "Jac", 0.8, 2, 2, "W", "NoMUMPS", 0.5, 0.0)
+        mg.prec = "c64"
+        s3 = ha.getShiftedLaplacianMultigridSolver(hp3, mg, 0.5, "GMRES", 8)
+        t0 = time.perf_counter()
+        _, s3 = ha.solveLinearSystem(None, DeviceArray.from_numpy(B3), s3)
+        log(f"profiling 3-D solve: {args.profile_3d} restart cycles, {s3.last_stats['max_iters']} iterations, "
+            f"{time.perf_counter() - t0:.2f} s")
+        return
     if args.profile_training:
         _, info = train.train_coarse_correction(n=(513, 513), media_count=args.train_media, level=2,
                                                 epochs=args.train_epochs, log=log)

@@ -43,6 +43,13 @@ template <typename T>
 __host__ __device__ __forceinline__ cx<T> operator*(T s, cx<T> a) {
   return mk<T>(s * a.re, s * a.im);
 }
+// a * b with explicit fused multiply-adds: the plain operator leaves the
+// compiler free to contract either product of each component into an fma,
+// and its choice can differ between two instantiations of one kernel
+template <typename T>
+__host__ __device__ __forceinline__ cx<T> cmul_fma(cx<T> a, cx<T> b) {
+  return mk<T>(fma(a.re, b.re, -(a.im * b.im)), fma(a.re, b.im, a.im * b.re));
+}
 // conj(a) * b
 template <typename T>
 __host__ __device__ __forceinline__ cx<T> cmulc(cx<T> a, cx<T> b) {

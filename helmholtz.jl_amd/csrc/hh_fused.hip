@@ -203,33 +203,26 @@ __device__ __forceinline__ cx<T> res_at(const cx<T>* s, int q, cx<T> bq, cx<T> d
 }
 // ------------------------------------------------------------------ down leg
 // STOREX: store the smoothed iterate x2 (V levels); on W levels the middle
-// leg recomputes it, so only b_c is written
+// leg recomputes it, so only b_c is written.  A block smooths R right-hand
+// sides of one tile in turn: the diagonal and the factor are loaded once and
+// every RHS's b with the first loads, so the R tiles share one load latency
+// (the leg waits at its barriers, not on VALU or bytes).
 template <typename T, bool STOREX>
-__global__ __launch_bounds__(FusedCfg<T>::NT) void k_down_fused(TileMap tm, Grid gf, Grid gc,
-                                                                const cx<T>* __restrict__ b,
-                                                                const cx<T>* __restrict__ d,
-                                                                const cx<T>* __restrict__ f, cx<T>* __restrict__ xo,
-                                                                cx<T>* __restrict__ bco,
-                                                                const int* __restrict__ skip, T omr) {
+__device__ __forceinline__ void down_tile(const TileCtx& c, const Grid& gf, const Grid& gc, int rhs,
+                                          const cx<T> (&rb)[Geo<T, 3>::KR], const cx<T> (&rd)[Geo<T, 3>::KR],
+                                          const cx<T> (&rf)[Geo<T, 3>::KR], cx<T>* __restrict__ xo,
+                                          cx<T>* __restrict__ bco, T omr, cx<T>* s1, cx<T>* s2) {
   constexpr int H = 3;
   typedef Geo<T, H> G;
   constexpr int NW = G::NW, KR = G::KR, HT = G::HT;
-  __shared__ cx<T> s1[G::NPIX], s2[G::NPIX];
-  TileCtx c;
-  if (!tile_ctx<H, G::FTX, G::TY>(tm, gf, skip, c)) return;
   const T ih1 = (T)gf.ih1, ih2 = (T)gf.ih2;
-  const int64_t off = (int64_t)c.rhs * gf.N;
-  cx<T> rb[KR], rd[KR], rf[KR];
-  load_rows<T, H, NW, KR>(b + off, c, gf, rb);
-  load_rows<T, H, NW, KR>(d, c, gf, rd);
-  load_rows<T, H, NW, KR>(f, c, gf, rf);
   // x1 = F b on the whole tile
 #pragma unroll
-  for (int k = 0; k < KR; ++k) s1[(c.wv + NW * k) * 64 + c.lane] = rf[k] * rb[k];
+  for (int k = 0; k < KR; ++k) s1[(c.wv + NW * k) * 64 + c.lane] = cmul_fma(rf[k], rb[k]);
   __syncthreads();
   // x2 on the halo-2 region; owned nodes are stored
   const bool in1 = c.lane >= 1 && c.lane < 63, own_c = c.lane >= H && c.lane < H + G::FTX && c.fx0 - H + c.lane < gf.n1;
-  cx<T>* xb = xo + off;
+  cx<T>* xb = xo + (int64_t)rhs * gf.N;
 #pragma unroll
   for (int k = 0; k < KR; ++k) {
     const int lj = c.wv + NW * k, q = lj * 64 + c.lane;
@@ -249,38 +242,69 @@ __global__ __launch_bounds__(FusedCfg<T>::NT) void k_down_fused(TileMap tm, Grid
     if (lj >= 2 && lj < HT - 2 && in2) s1[q] = res_at(s2, q, rb[k], rd[k], ih1, ih2);
   }
   __syncthreads();
-  restrict_tile<T, H>(s1, c, gc, bco);
+  TileCtx cr = c;
+  cr.rhs = rhs;
+  restrict_tile<T, H>(s1, cr, gc, bco);
+}
+
+template <typename T, bool STOREX, int R>
+__global__ __launch_bounds__(FusedCfg<T>::NT) void k_down_fused(TileMap tm, Grid gf, Grid gc,
+                                                                const cx<T>* __restrict__ b,
+                                                                const cx<T>* __restrict__ d,
+                                                                const cx<T>* __restrict__ f, cx<T>* __restrict__ xo,
+                                                                cx<T>* __restrict__ bco,
+                                                                const int* __restrict__ skip, T omr, int nrhs) {
+  constexpr int H = 3;
+  typedef Geo<T, H> G;
+  constexpr int NW = G::NW, KR = G::KR;
+  __shared__ cx<T> s1[G::NPIX], s2[G::NPIX];
+  TileCtx c;
+  tile_ctx<H, G::FTX, G::TY>(tm, gf, nullptr, c);  // c.rhs: the block's group of R RHS
+  bool act[R];
+  bool any = false;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int rhs = c.rhs * R + r;
+    act[r] = rhs < nrhs && !(skip && skip[rhs]);
+    any = any || act[r];
+  }
+  if (!any) return;
+  cx<T> rb[R][KR], rd[KR], rf[KR];
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    if (act[r]) load_rows<T, H, NW, KR>(b + (int64_t)(c.rhs * R + r) * gf.N, c, gf, rb[r]);
+  load_rows<T, H, NW, KR>(d, c, gf, rd);
+  load_rows<T, H, NW, KR>(f, c, gf, rf);
+  bool first = true;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    if (!act[r]) continue;
+    if (!first) __syncthreads();  // the previous RHS's full weighting has read s1
+    first = false;
+    down_tile<T, STOREX>(c, gf, gc, c.rhs * R + r, rb[r], rd, rf, xo, bco, omr, s1, s2);
+  }
 }
 
 // ------------------------------------------------------------------ middle (W)
 // x2 = F b + F (b - A F b) is formed again on the halo-2 region of an H = 3
 // tile with the down leg's expressions (so it equals the x2 that leg would
 // have stored, bit for bit), then x' = x2 + P e_c, the residual on the halo-1
-// region, and its full weighting.
+// region, and its full weighting.  R right-hand sides per block, as in the
+// down leg.
 template <typename T>
-__global__ __launch_bounds__(FusedCfg<T>::NT) void k_mid_fused(TileMap tm, Grid gf, Grid gc,
-                                                               const cx<T>* __restrict__ ec,
-                                                               const cx<T>* __restrict__ b,
-                                                               const cx<T>* __restrict__ d,
-                                                               const cx<T>* __restrict__ f, cx<T>* __restrict__ xo,
-                                                               cx<T>* __restrict__ bco,
-                                                               const int* __restrict__ skip, T omr) {
+__device__ __forceinline__ void mid_tile(const TileCtx& c, const Grid& gf, const Grid& gc, int rhs,
+                                         const cx<T> (&rb)[Geo<T, 3>::KR], const cx<T> (&rd)[Geo<T, 3>::KR],
+                                         const cx<T> (&rf)[Geo<T, 3>::KR], const Patch<T>& pv,
+                                         cx<T>* __restrict__ xo, cx<T>* __restrict__ bco, T omr, cx<T>* s1,
+                                         cx<T>* s2) {
   constexpr int H = 3;
   typedef Geo<T, H> G;
   constexpr int NW = G::NW, KR = G::KR, HT = G::HT;
-  __shared__ cx<T> s1[G::NPIX], s2[G::NPIX];
-  TileCtx c;
-  if (!tile_ctx<H, G::FTX, G::TY>(tm, gf, skip, c)) return;
   const T ih1 = (T)gf.ih1, ih2 = (T)gf.ih2;
-  const int64_t off = (int64_t)c.rhs * gf.N;
-  cx<T> rb[KR], rd[KR], rf[KR], vx[KR];
-  load_rows<T, H, NW, KR>(b + off, c, gf, rb);
-  load_rows<T, H, NW, KR>(d, c, gf, rd);
-  load_rows<T, H, NW, KR>(f, c, gf, rf);
-  const Patch<T> pv = load_patch<T, H>(ec + (int64_t)c.rhs * gc.N, gc, c);
+  cx<T> vx[KR];
   // x1 = F b on the whole tile
 #pragma unroll
-  for (int k = 0; k < KR; ++k) s1[(c.wv + NW * k) * 64 + c.lane] = rf[k] * rb[k];
+  for (int k = 0; k < KR; ++k) s1[(c.wv + NW * k) * 64 + c.lane] = cmul_fma(rf[k], rb[k]);
   __syncthreads();
   // x2 on the halo-2 region (the down leg's second sweep)
   const bool in1 = c.lane >= 1 && c.lane < 63;
@@ -299,7 +323,7 @@ __global__ __launch_bounds__(FusedCfg<T>::NT) void k_mid_fused(TileMap tm, Grid 
   __syncthreads();
   // residual on the halo-1 region over x' (s1); owned x' is stored
   const bool in2 = c.lane >= 2 && c.lane < 62, own_c = c.lane >= H && c.lane < H + G::FTX && c.fx0 - H + c.lane < gf.n1;
-  cx<T>* xb = xo + off;
+  cx<T>* xb = xo + (int64_t)rhs * gf.N;
 #pragma unroll
   for (int k = 0; k < KR; ++k) {
     const int lj = c.wv + NW * k, q = lj * 64 + c.lane;
@@ -308,7 +332,52 @@ __global__ __launch_bounds__(FusedCfg<T>::NT) void k_mid_fused(TileMap tm, Grid 
     if (lj >= H && lj < H + G::TY && j < gf.n2 && own_c) xb[(int64_t)j * gf.n1 + (c.fx0 - H + c.lane)] = s1[q];
   }
   __syncthreads();
-  restrict_tile<T, H>(s2, c, gc, bco);
+  TileCtx cr = c;
+  cr.rhs = rhs;
+  restrict_tile<T, H>(s2, cr, gc, bco);
+}
+
+template <typename T, int R>
+__global__ __launch_bounds__(FusedCfg<T>::NT) void k_mid_fused(TileMap tm, Grid gf, Grid gc,
+                                                               const cx<T>* __restrict__ ec,
+                                                               const cx<T>* __restrict__ b,
+                                                               const cx<T>* __restrict__ d,
+                                                               const cx<T>* __restrict__ f, cx<T>* __restrict__ xo,
+                                                               cx<T>* __restrict__ bco,
+                                                               const int* __restrict__ skip, T omr, int nrhs) {
+  constexpr int H = 3;
+  typedef Geo<T, H> G;
+  constexpr int NW = G::NW, KR = G::KR;
+  __shared__ cx<T> s1[G::NPIX], s2[G::NPIX];
+  TileCtx c;
+  tile_ctx<H, G::FTX, G::TY>(tm, gf, nullptr, c);  // c.rhs: the block's group of R RHS
+  bool act[R];
+  bool any = false;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int rhs = c.rhs * R + r;
+    act[r] = rhs < nrhs && !(skip && skip[rhs]);
+    any = any || act[r];
+  }
+  if (!any) return;
+  cx<T> rb[R][KR], rd[KR], rf[KR];
+  Patch<T> pv[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    if (act[r]) {
+      load_rows<T, H, NW, KR>(b + (int64_t)(c.rhs * R + r) * gf.N, c, gf, rb[r]);
+      pv[r] = load_patch<T, H>(ec + (int64_t)(c.rhs * R + r) * gc.N, gc, c);
+    }
+  load_rows<T, H, NW, KR>(d, c, gf, rd);
+  load_rows<T, H, NW, KR>(f, c, gf, rf);
+  bool first = true;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    if (!act[r]) continue;
+    if (!first) __syncthreads();  // the previous RHS's full weighting has read s2
+    first = false;
+    mid_tile<T>(c, gf, gc, c.rhs * R + r, rb[r], rd, rf, pv[r], xo, bco, omr, s1, s2);
+  }
 }
 
 // ------------------------------------------------------------------ up leg
@@ -362,25 +431,37 @@ TileMap tile_map(const Grid& gf, int nrhs) {
   return make_tile_map((gf.n1 + G::FTX - 1) / G::FTX, (gf.n2 + G::TY - 1) / G::TY, nrhs);
 }
 
+// Right-hand sides per block: 2 in the down leg (62 VGPRs, eight wavefronts
+// per SIMD still fit), 1 in the middle leg (two RHS need 104 VGPRs and halve
+// the resident blocks: 0.81 against 0.73 ms on the 1025 x 513 grid).
+template <typename T, int R>
+static void down_r_launch(const Grid& gf, const Grid& gc, int nrhs, const cx<T>* b, const cx<T>* d, const cx<T>* f,
+                          T omr, cx<T>* xo, cx<T>* bco, const int* skip, hipStream_t s) {
+  const TileMap m = tile_map<T, 3>(gf, (nrhs + R - 1) / R);
+  if (xo)
+    hipLaunchKernelGGL((k_down_fused<T, true, R>), dim3(m.T), dim3(FusedCfg<T>::NT), 0, s, m, gf, gc, b, d, f, xo,
+                       bco, skip, omr, nrhs);
+  else
+    hipLaunchKernelGGL((k_down_fused<T, false, R>), dim3(m.T), dim3(FusedCfg<T>::NT), 0, s, m, gf, gc, b, d, f, xo,
+                       bco, skip, omr, nrhs);
+}
+
 template <typename T>
 void launch_down_fused(const Grid& gf, const Grid& gc, int nrhs, const cx<T>* b, const cx<T>* d, const cx<T>* f,
                        double relax, cx<T>* xo, cx<T>* bco, const int* skip, hipStream_t s) {
-  const TileMap m = tile_map<T, 3>(gf, nrhs);
   const T omr = (T)(1.0 - relax);
-  if (xo)
-    hipLaunchKernelGGL((k_down_fused<T, true>), dim3(m.T), dim3(FusedCfg<T>::NT), 0, s, m, gf, gc, b, d, f, xo, bco,
-                       skip, omr);
+  if (nrhs > 1)
+    down_r_launch<T, 2>(gf, gc, nrhs, b, d, f, omr, xo, bco, skip, s);
   else
-    hipLaunchKernelGGL((k_down_fused<T, false>), dim3(m.T), dim3(FusedCfg<T>::NT), 0, s, m, gf, gc, b, d, f, xo, bco,
-                       skip, omr);
+    down_r_launch<T, 1>(gf, gc, nrhs, b, d, f, omr, xo, bco, skip, s);
 }
 
 template <typename T>
 void launch_mid_fused(const Grid& gf, const Grid& gc, int nrhs, const cx<T>* ec, const cx<T>* b, const cx<T>* d,
                       const cx<T>* f, double relax, cx<T>* xo, cx<T>* bco, const int* skip, hipStream_t s) {
   const TileMap m = tile_map<T, 3>(gf, nrhs);
-  hipLaunchKernelGGL((k_mid_fused<T>), dim3(m.T), dim3(FusedCfg<T>::NT), 0, s, m, gf, gc, ec, b, d, f, xo, bco, skip,
-                     (T)(1.0 - relax));
+  hipLaunchKernelGGL((k_mid_fused<T, 1>), dim3(m.T), dim3(FusedCfg<T>::NT), 0, s, m, gf, gc, ec, b, d, f, xo, bco,
+                     skip, (T)(1.0 - relax), nrhs);
 }
 
 template <typename T>

@@ -157,7 +157,7 @@ __global__ __launch_bounds__(256) void k_jacobi0(Grid g, const cx<T>* __restrict
   const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (p >= g.N) return;
   const int64_t o = (int64_t)rhs * g.N + p;
-  x[o] = f[p] * b[o];
+  x[o] = cmul_fma(f[p], b[o]);
 }
 
 // Full weighting with mirror ghosts, fine residual -> coarse right-hand side.
