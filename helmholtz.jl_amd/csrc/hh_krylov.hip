@@ -256,18 +256,29 @@ __device__ __forceinline__ cx<double> hz_eval(const Grid& g, const HzOps<DIM>& o
                 o.d);
 }
 
-// 2-D: the same operands from a sliding window of z in LDS (position q mod R,
-// R a power of two): every operand of node p lies in [p - n1, p + n1]
+// The same operands with the node and its in-plane neighbours from a sliding
+// window of z in LDS (position q mod R, R a power of two): each of them lies
+// in [p - n1, p + n1].  In 3-D the two neighbours a plane away are read from
+// z itself (the window would have to span two planes).
+template <int DIM>
 __device__ __forceinline__ void hz_load_ring(const Grid& g, const cx<float>* zr, int rmask,
-                                             const cx<double>* __restrict__ d, int ci, int cj, int64_t p,
-                                             HzOps<2>& o) {
+                                             const cx<float>* __restrict__ zb, const cx<double>* __restrict__ d,
+                                             int ci, int cj, int ck, int64_t p, HzOps<DIM>& o) {
   const int n1 = g.n1, n2 = g.n2;
-  const int64_t row = (int64_t)cj * n1;
+  const int64_t plane = (int64_t)n1 * n2, pl = DIM == 3 ? (int64_t)ck * plane : 0;
+  const int64_t row = pl + (int64_t)cj * n1;
   o.c = zr[p & rmask];
   o.x1l = zr[(row + lo_nb(ci, g.bc)) & rmask];
   o.x1r = zr[(row + hi_nb(ci, n1, g.bc)) & rmask];
-  o.x2l = zr[((int64_t)lo_nb(cj, g.bc) * n1 + ci) & rmask];
-  o.x2r = zr[((int64_t)hi_nb(cj, n2, g.bc) * n1 + ci) & rmask];
+  o.x2l = zr[(pl + (int64_t)lo_nb(cj, g.bc) * n1 + ci) & rmask];
+  o.x2r = zr[(pl + (int64_t)hi_nb(cj, n2, g.bc) * n1 + ci) & rmask];
+  if (DIM == 3) {
+    // (a thread whose nodes have ended walks past the last plane: clamped)
+    const int kc = min(ck, g.n3 - 1);
+    const int64_t col = (int64_t)cj * n1 + ci;
+    o.x3l = zb[(int64_t)lo_nb(kc, g.bc) * plane + col];
+    o.x3r = zb[(int64_t)hi_nb(kc, g.n3, g.bc) * plane + col];
+  }
   o.d = d[p];
 }
 
@@ -278,7 +289,8 @@ __device__ __forceinline__ void hz_load_ring(const Grid& g, const cx<float>* zr,
 // position and a read one differ by less than R, so they never share a
 // slot: one barrier per step orders the stores before the next step's reads.
 // Each z value crosses HBM once instead of up to three times (the row above
-// has left the caches when the block comes back to it).
+// has left the caches when the block comes back to it); in 3-D the values a
+// plane away are still read from z (about one extra read per value).
 template <int U>
 __device__ __forceinline__ void zring_fill(cx<float>* zr, int rmask, const cx<float>* __restrict__ zb, int64_t N,
                                            int64_t lo, int64_t hi) {
@@ -318,9 +330,10 @@ struct NodeWalk {
 // in turn and the passes were bound by load latency, not HBM bandwidth.  A
 // node past the chunk end (U = 2) loads node p's values and contributes 0.
 // U = 1 keeps the summation order of the stored-w kernels.  With the z
-// window (RING, 2-D) every thread runs every step of its block; a thread
-// whose nodes have ended reads node p0's basis values and diagonal, window
-// slots of no meaning, and contributes nothing.  EARLY (all NV
+// window (RING) every thread runs every step of its block; a thread whose
+// nodes have ended reads node p0's basis values and diagonal, window slots of
+// no meaning (and in 3-D z a plane away at a clamped plane), and contributes
+// nothing.  EARLY (all NV
 // basis values held in registers at once) for NV <= 10; longer restarts
 // read one basis value at a time within their register budget.
 
@@ -331,7 +344,6 @@ __global__ __launch_bounds__(256) void k_dots_hz(KrylovWork kw, Grid g, const cx
                                                  int j, int rmask) {
   constexpr bool EARLY = NV <= 10;
   constexpr int S = 256 * U;
-  static_assert(!RING || DIM == 2, "the z window serves the 2-D stencil");
   const int b = blockIdx.x, c = blockIdx.y;
   if (kw.done[b]) return;
   __shared__ double sh[4];
@@ -377,10 +389,11 @@ __global__ __launch_bounds__(256) void k_dots_hz(KrylovWork kw, Grid g, const cx
       ok[u] = p + 256 * u < p1;
       qq[u] = ok[u] ? p + 256 * u : (p < p1 ? p : p0);
       const int ci = ok[u] ? nw[u].ci : nw[0].ci, cj = ok[u] ? nw[u].cj : nw[0].cj;
+      const int ck = ok[u] ? nw[u].ck : nw[0].ck;
       if constexpr (RING)
-        hz_load_ring(g, zr, rmask, d, p < p1 ? ci : nw[0].ci, p < p1 ? cj : nw[0].cj, qq[u], ho[u]);
+        hz_load_ring<DIM>(g, zr, rmask, zb, d, ci, cj, ck, qq[u], ho[u]);
       else
-        hz_load<DIM>(g, zb, d, ci, cj, ok[u] ? nw[u].ck : nw[0].ck, qq[u], ho[u]);
+        hz_load<DIM>(g, zb, d, ci, cj, ck, qq[u], ho[u]);
     }
     if constexpr (EARLY) {
 #pragma unroll
@@ -431,7 +444,6 @@ __global__ __launch_bounds__(256) void k_update_lazy_hz(KrylovWork kw, Grid g, c
                                                         cx<TV>* __restrict__ wout, int j, int rmask) {
   constexpr bool EARLY = NV <= 10;
   constexpr int S = 256 * U;
-  static_assert(!RING || DIM == 2, "the z window serves the 2-D stencil");
   const int b = blockIdx.x, c = blockIdx.y;
   if (kw.done[b]) return;
   __shared__ double sh[4];
@@ -478,10 +490,11 @@ __global__ __launch_bounds__(256) void k_update_lazy_hz(KrylovWork kw, Grid g, c
       ok[u] = p + 256 * u < p1;
       qq[u] = ok[u] ? p + 256 * u : (p < p1 ? p : p0);
       const int ci = ok[u] ? nw[u].ci : nw[0].ci, cj = ok[u] ? nw[u].cj : nw[0].cj;
+      const int ck = ok[u] ? nw[u].ck : nw[0].ck;
       if constexpr (RING)
-        hz_load_ring(g, zr, rmask, d, p < p1 ? ci : nw[0].ci, p < p1 ? cj : nw[0].cj, qq[u], ho[u]);
+        hz_load_ring<DIM>(g, zr, rmask, zb, d, ci, cj, ck, qq[u], ho[u]);
       else
-        hz_load<DIM>(g, zb, d, ci, cj, ok[u] ? nw[u].ck : nw[0].ck, qq[u], ho[u]);
+        hz_load<DIM>(g, zb, d, ci, cj, ck, qq[u], ho[u]);
     }
     if constexpr (EARLY) {
 #pragma unroll
@@ -826,10 +839,11 @@ static void arnoldi_nv(const KrylovWork& w, const cx<TV>* V, cx<double>* wv, int
   if (t1) kt->begin(s);
   // nodes per thread per iteration: 2 for a complex64 basis up to NV = 10
   constexpr int U = std::is_same<TV, float>::value && NV <= 10 ? 2 : 1;
-  // 2-D: z through the LDS window when 2 n1 + 2 S nodes (S = 256 U) fit in
-  // a power-of-two ring of at most 8192 complex64 values (64 KiB)
+  // z (2-D; in 3-D the in-plane operands) through the LDS window when
+  // 2 n1 + 2 S nodes (S = 256 U) fit in a power-of-two ring of at most 8192
+  // complex64 values (64 KiB)
   int rnodes = 0;
-  if (hz && hz->g.dim == 2) {
+  if (hz) {
     const int64_t need = 2 * (int64_t)hz->g.n1 + 2 * 256 * U;
     int64_t r = 1;
     while (r < need) r <<= 1;
@@ -837,8 +851,11 @@ static void arnoldi_nv(const KrylovWork& w, const cx<TV>* V, cx<double>* wv, int
   }
   const size_t rbytes = sizeof(cx<float>) * (size_t)rnodes;
   if (hz) {
-    if (rnodes)
+    if (rnodes && hz->g.dim == 2)
       hipLaunchKernelGGL((k_dots_hz<NV, TV, U, 2, true>), dim3(w.nrhs, w.nchunk), dim3(256), rbytes, s, w, hz->g,
+                         V, hz->z, hz->d, j, rnodes - 1);
+    else if (rnodes)
+      hipLaunchKernelGGL((k_dots_hz<NV, TV, U, 3, true>), dim3(w.nrhs, w.nchunk), dim3(256), rbytes, s, w, hz->g,
                          V, hz->z, hz->d, j, rnodes - 1);
     else if (hz->g.dim == 2)
       hipLaunchKernelGGL((k_dots_hz<NV, TV, U, 2, false>), dim3(w.nrhs, w.nchunk), dim3(256), 0, s, w, hz->g, V,
@@ -856,8 +873,11 @@ static void arnoldi_nv(const KrylovWork& w, const cx<TV>* V, cx<double>* wv, int
   hipLaunchKernelGGL(k_reduce_dots, dim3(j + (w.lazy ? 2 : 1), w.nrhs), dim3(64), 0, s, w, w.h1, j);
   if (t2) kt->begin(s);
   if (hz) {
-    if (rnodes)
+    if (rnodes && hz->g.dim == 2)
       hipLaunchKernelGGL((k_update_lazy_hz<NV, TV, U, 2, true>), dim3(w.nrhs, w.nchunk), dim3(256), rbytes, s, w,
+                         hz->g, V, hz->z, hz->d, vnext, j, rnodes - 1);
+    else if (rnodes)
+      hipLaunchKernelGGL((k_update_lazy_hz<NV, TV, U, 3, true>), dim3(w.nrhs, w.nchunk), dim3(256), rbytes, s, w,
                          hz->g, V, hz->z, hz->d, vnext, j, rnodes - 1);
     else if (hz->g.dim == 2)
       hipLaunchKernelGGL((k_update_lazy_hz<NV, TV, U, 2, false>), dim3(w.nrhs, w.nchunk), dim3(256), 0, s, w,
