@@ -118,14 +118,34 @@ __global__ __launch_bounds__(256) void k_conv3x3(const bf16* __restrict__ in, in
 // for all 9 taps and all output channels.  Each chunk of 64 pixels is staged
 // once in LDS with 16-byte global loads, transposed so that the MFMA's
 // reduction index (pixels) is contiguous: sA[co][px] = dpre, and the
-// im2col of the input sB[tap][ci][px] (zero outside the image).  Every
+// im2col of the input sB[tap][ci][px] (zero outside the image); a thread
+// loads 4 pixels x 8 channels and stores each channel's 4 pixels as one
+// 8-byte piece (a quarter of the LDS stores of one 2-byte store per value).  Every
 // operand of v_mfma_f32_16x16x16bf16 (lane: 4 consecutive pixels of one
 // channel) is then one aligned 8-byte LDS read; rows are padded to 68
 // entries so the 16 lanes of a group hit distinct banks.  The 9 NCO tiles of
 // 16x16 outputs are spread over the 4 wavefronts; partial sums per block are
 // reduced in a fixed order afterwards (deterministic).
 constexpr int WG_PX = 64;         // pixels per staged chunk
-constexpr int WG_LD = WG_PX + 4;  // padded LDS row
+constexpr int WG_LD = WG_PX + 4;  // padded LDS row (136 B: 8-byte aligned pieces)
+
+// Four pixels' 8 channels (v[i]: pixel i, channels 0..7, two per word) into
+// the transposed tile: channel k's four pixels as one 8-byte store at
+// dst + k WG_LD (dst 8-byte aligned)
+__device__ __forceinline__ void store_px4(bf16* dst, const uint4 (&v)[4]) {
+  const unsigned w[4][4] = {{v[0].x, v[0].y, v[0].z, v[0].w}, {v[1].x, v[1].y, v[1].z, v[1].w},
+                            {v[2].x, v[2].y, v[2].z, v[2].w}, {v[3].x, v[3].y, v[3].z, v[3].w}};
+#pragma unroll
+  for (int h = 0; h < 4; ++h) {  // word h: channels 2h (low half) and 2h + 1 (high half)
+    uint2 lo, hi;
+    lo.x = (w[0][h] & 0xffffu) | (w[1][h] << 16);
+    lo.y = (w[2][h] & 0xffffu) | (w[3][h] << 16);
+    hi.x = (w[0][h] >> 16) | (w[1][h] & 0xffff0000u);
+    hi.y = (w[2][h] >> 16) | (w[3][h] & 0xffff0000u);
+    *(uint2*)(dst + (2 * h) * WG_LD) = lo;
+    *(uint2*)(dst + (2 * h + 1) * WG_LD) = hi;
+  }
+}
 template <int NCO>
 __global__ __launch_bounds__(256) void k_conv_wgrad(const bf16* __restrict__ dpre, const bf16* __restrict__ in,
                                                    int cin, int nx, int ny, int64_t total, int64_t per,
@@ -160,31 +180,36 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const bf16* __restrict__ dpr
         sbase[threadIdx.x] = -1;
       }
     }
-    // dpre: 64 pixels x COUT channels, 8 channels (16 B) per load
-    for (int e = threadIdx.x; e < WG_PX * COUT / 8; e += 256) {
-      const int px = e / (COUT / 8), c8 = (e % (COUT / 8)) * 8;
-      const int64_t q = q0 + px;
-      uint4 v = make_uint4(0u, 0u, 0u, 0u);
-      if (q < qb1) v = *(const uint4*)(dpre + q * COUT + c8);
-      const bf16* hv = (const bf16*)&v;
+    // dpre: 64 pixels x COUT channels; an item is 4 pixels x 8 channels (four
+    // 16-byte loads), stored as one 8-byte row piece per channel
+    for (int e = threadIdx.x; e < (WG_PX / 4) * (COUT / 8); e += 256) {
+      const int px4 = e / (COUT / 8), c8 = (e % (COUT / 8)) * 8;
+      uint4 v[4];
 #pragma unroll
-      for (int k = 0; k < 8; ++k) sA[(c8 + k) * WG_LD + px] = hv[k];
+      for (int i = 0; i < 4; ++i) {
+        const int64_t q = q0 + 4 * px4 + i;
+        v[i] = q < qb1 ? *(const uint4*)(dpre + q * COUT + c8) : make_uint4(0u, 0u, 0u, 0u);
+      }
+      store_px4(sA + c8 * WG_LD + 4 * px4, v);
     }
     __syncthreads();  // pixel coordinates
-    // im2col of the input for this ci block: tap t, pixel px, 8 channels per load
-    for (int e = threadIdx.x; e < 9 * WG_PX * 2; e += 256) {
-      const int t = e / (WG_PX * 2), r = e % (WG_PX * 2);
-      const int px = r >> 1, c8 = (r & 1) * 8;
-      uint4 v = make_uint4(0u, 0u, 0u, 0u);
-      const int img = sbase[px];
-      if (img >= 0) {
-        const int ii = si[px] + t % 3 - 1, jj = sj[px] + t / 3 - 1;
-        if (ii >= 0 && ii < nx && jj >= 0 && jj < ny)
-          v = *(const uint4*)(in + ((int64_t)img * npix + (int64_t)jj * nx + ii) * cin + cib * 16 + c8);
-      }
-      const bf16* hv = (const bf16*)&v;
+    // im2col of the input for this ci block: tap t, 4 pixels x 8 channels per item
+    for (int e = threadIdx.x; e < 9 * (WG_PX / 4) * 2; e += 256) {
+      const int t = e / ((WG_PX / 4) * 2), r = e % ((WG_PX / 4) * 2);
+      const int px4 = r >> 1, c8 = (r & 1) * 8;
+      uint4 v[4];
 #pragma unroll
-      for (int k = 0; k < 8; ++k) sB[(t * 16 + c8 + k) * WG_LD + px] = hv[k];
+      for (int i = 0; i < 4; ++i) {
+        const int px = 4 * px4 + i;
+        v[i] = make_uint4(0u, 0u, 0u, 0u);
+        const int img = sbase[px];
+        if (img >= 0) {
+          const int ii = si[px] + t % 3 - 1, jj = sj[px] + t / 3 - 1;
+          if (ii >= 0 && ii < nx && jj >= 0 && jj < ny)
+            v[i] = *(const uint4*)(in + ((int64_t)img * npix + (int64_t)jj * nx + ii) * cin + cib * 16 + c8);
+        }
+      }
+      store_px4(sB + (t * 16 + c8) * WG_LD + 4 * px4, v);
     }
     __syncthreads();
     // bias gradient: one thread per output channel (ci block 0 only)
