@@ -700,12 +700,18 @@ static int64_t pick_chunks(int64_t total, int64_t* chunk) {
   return (total + c - 1) / c;
 }
 
-// pixel blocks of the weight-gradient GEMM: at most WG_MAXBLK blocks of a
-// multiple of WG_PX pixels, at least 16 chunks each
-constexpr int64_t WG_MAXBLK = 256;
-static int64_t pick_wgrad_blocks(int64_t total, int64_t* per) {
-  int64_t nb = (total + 16 * WG_PX - 1) / (16 * WG_PX);
-  if (nb > WG_MAXBLK) nb = WG_MAXBLK;
+// Pixel blocks of the weight-gradient GEMM: a block walks its chunks of WG_PX
+// pixels one after another (stage, barrier, MFMA), so few long blocks leave
+// the kernel bound by that chain (about 60 us per launch whatever the layer's
+// size when every block took at least 16 chunks).  Blocks of WG_MINCH chunks,
+// as many as the partials' scratch holds for the layer (np weights and
+// biases per block).
+constexpr int64_t WG_MINCH = 2;
+constexpr int64_t WG_SCRATCH = 64LL << 20;  // floats of weight-gradient partials
+static int64_t pick_wgrad_blocks(int64_t total, int64_t np, int64_t* per) {
+  int64_t nb = (total + WG_MINCH * WG_PX - 1) / (WG_MINCH * WG_PX);
+  const int64_t cap = WG_SCRATCH / np;
+  if (nb > cap) nb = cap;
   if (nb < 1) nb = 1;
   int64_t p = (total + nb - 1) / nb;
   p = (p + WG_PX - 1) / WG_PX * WG_PX;
@@ -725,9 +731,9 @@ static void ensure_scratch(UNetWork& ws, int64_t n) {
 void unet_backward(UNet& net, UNetWork& ws, hipStream_t s, int nimg, int nx, int ny, const bf16* x, const float* dy) {
   Dims d = dims_of(nimg, nx, ny);
   bf16* dpre = (bf16*)ws.bufs[2 * A_NUM];
-  // scratch: weight-gradient partials (at most WG_MAXBLK pixel blocks x the
-  // largest layer) and bias partials, or head partials
-  ensure_scratch(ws, (int64_t)WG_MAXBLK * (9 * 128 * 64 + 64) + 64LL * 34);
+  // scratch: weight- and bias-gradient partials (WG_SCRATCH floats), or head
+  // partials
+  ensure_scratch(ws, WG_SCRATCH);
   const int64_t head_off = unet_param_count() - 34;
   // head
   {
@@ -754,8 +760,8 @@ void unet_backward(UNet& net, UNetWork& ws, hipStream_t s, int nimg, int nx, int
                        T * c.co);
     const bf16* in = kIn[l] < 0 ? x : act(ws, kIn[l]);
     int64_t per;
-    const int64_t nb = pick_wgrad_blocks(T, &per);
     const int64_t np = 9LL * c.co * c.ci + c.co;  // the layer's weights and bias
+    const int64_t nb = pick_wgrad_blocks(T, np, &per);
     float* pw = ws.scratch;
     const dim3 g((unsigned)nb, (unsigned)(c.ci / 16));
     switch (c.co / 16) {
