@@ -63,16 +63,19 @@ static T* dmal(size_t n) {
 }
 
 // ------------------------------------------------------------------ kernels
+// A wavefront computes 16 pixels x NCO * 16 output channels (channel block
+// blockIdx.z of a layer with `cout` outputs): NCO is picked per launch so that
+// small grids still give several wavefronts per SIMD.
 template <int NCO>
 __global__ __launch_bounds__(256) void k_conv3x3(const bf16* __restrict__ in, int cin, const bf16* __restrict__ w,
                                                  const float* __restrict__ bias, bf16* __restrict__ out, int nx,
-                                                 int ny, int relu) {
+                                                 int ny, int relu, int cout) {
   const int img = blockIdx.y;
   const int npix = nx * ny;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int tile = blockIdx.x * 64 + wv * 16;
   if (tile >= npix) return;  // uniform per wavefront
-  constexpr int cout = NCO * 16;
+  const int co0 = blockIdx.z * NCO * 16;
   const int m = lane & 15;
   const int kq = (lane >> 4) * 4;
   const int pa = tile + m;
@@ -86,7 +89,7 @@ __global__ __launch_bounds__(256) void k_conv3x3(const bf16* __restrict__ in, in
     const int ii = ia + tap % 3 - 1, jj = ja + tap / 3 - 1;
     const bool ok = pa_ok && ii >= 0 && ii < nx && jj >= 0 && jj < ny;
     const bf16* src = inb + ((int64_t)(ok ? jj : 0) * nx + (ok ? ii : 0)) * cin + kq;
-    const bf16* wtap = w + (int64_t)tap * cout * cin + (int64_t)m * cin + kq;
+    const bf16* wtap = w + (int64_t)tap * cout * cin + (int64_t)(co0 + m) * cin + kq;
     for (int cb = 0; cb < cin; cb += 16) {
       s4 a = ok ? *(const s4*)(src + cb) : s4{0, 0, 0, 0};
 #pragma unroll
@@ -102,7 +105,7 @@ __global__ __launch_bounds__(256) void k_conv3x3(const bf16* __restrict__ in, in
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int p = tile + kq + i;
-      const int co = c * 16 + m;
+      const int co = co0 + c * 16 + m;
       if (p < npix) {
         float v = acc[c][i] + (bias ? bias[co] : 0.f);
         if (relu) v = v > 0.f ? v : 0.f;
@@ -589,16 +592,28 @@ static void ensure_ws(UNetWork& ws, int nimg, int nx, int ny) {
 static bf16* act(UNetWork& ws, int a) { return (bf16*)ws.bufs[a]; }
 static bf16* grd(UNetWork& ws, int a) { return (bf16*)ws.bufs[A_NUM + a]; }
 
+// output channels per wavefront: the most (of the layer's 16-channel blocks)
+// that still leaves 4096 wavefronts (four per SIMD), else one block
+static int conv_nco(int cout, int64_t pixels) {
+  const int nb = cout / 16;
+  const int64_t waves = (pixels + 15) / 16;
+  for (int nco : {8, 6, 4, 3, 2})
+    if (nb % nco == 0 && waves * (nb / nco) >= 4096) return nco;
+  return 1;
+}
+
 static void conv_launch(const bf16* in, int cin, const bf16* w, const float* bias, bf16* out, int cout, int nimg, int nx,
                         int ny, int relu, hipStream_t s) {
-  dim3 grid((unsigned)((nx * ny + 63) / 64), nimg);
-  switch (cout / 16) {
-    case 1: hipLaunchKernelGGL(k_conv3x3<1>, grid, dim3(256), 0, s, in, cin, w, bias, out, nx, ny, relu); break;
-    case 2: hipLaunchKernelGGL(k_conv3x3<2>, grid, dim3(256), 0, s, in, cin, w, bias, out, nx, ny, relu); break;
-    case 3: hipLaunchKernelGGL(k_conv3x3<3>, grid, dim3(256), 0, s, in, cin, w, bias, out, nx, ny, relu); break;
-    case 4: hipLaunchKernelGGL(k_conv3x3<4>, grid, dim3(256), 0, s, in, cin, w, bias, out, nx, ny, relu); break;
-    case 6: hipLaunchKernelGGL(k_conv3x3<6>, grid, dim3(256), 0, s, in, cin, w, bias, out, nx, ny, relu); break;
-    case 8: hipLaunchKernelGGL(k_conv3x3<8>, grid, dim3(256), 0, s, in, cin, w, bias, out, nx, ny, relu); break;
+  if (cout % 16) throw std::runtime_error("unsupported channel count");
+  const int nco = conv_nco(cout, (int64_t)nimg * nx * ny);
+  dim3 grid((unsigned)((nx * ny + 63) / 64), nimg, (unsigned)(cout / 16 / nco));
+  switch (nco) {
+    case 1: hipLaunchKernelGGL(k_conv3x3<1>, grid, dim3(256), 0, s, in, cin, w, bias, out, nx, ny, relu, cout); break;
+    case 2: hipLaunchKernelGGL(k_conv3x3<2>, grid, dim3(256), 0, s, in, cin, w, bias, out, nx, ny, relu, cout); break;
+    case 3: hipLaunchKernelGGL(k_conv3x3<3>, grid, dim3(256), 0, s, in, cin, w, bias, out, nx, ny, relu, cout); break;
+    case 4: hipLaunchKernelGGL(k_conv3x3<4>, grid, dim3(256), 0, s, in, cin, w, bias, out, nx, ny, relu, cout); break;
+    case 6: hipLaunchKernelGGL(k_conv3x3<6>, grid, dim3(256), 0, s, in, cin, w, bias, out, nx, ny, relu, cout); break;
+    case 8: hipLaunchKernelGGL(k_conv3x3<8>, grid, dim3(256), 0, s, in, cin, w, bias, out, nx, ny, relu, cout); break;
     default: throw std::runtime_error("unsupported channel count");
   }
 }
