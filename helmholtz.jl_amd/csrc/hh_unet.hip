@@ -719,12 +719,15 @@ static int64_t pick_chunks(int64_t total, int64_t* chunk) {
 // pixels one after another (stage, barrier, MFMA), so few long blocks leave
 // the kernel bound by that chain (about 60 us per launch whatever the layer's
 // size when every block took at least 16 chunks).  Blocks of WG_MINCH chunks,
-// as many as the partials' scratch holds for the layer (np weights and
-// biases per block).
+// at most WG_MAXBLK of them (past that the partials' reduction costs more
+// than the shorter chains save) and as many as the partials' scratch holds for
+// the layer (np weights and biases per block).
 constexpr int64_t WG_MINCH = 2;
+constexpr int64_t WG_MAXBLK = 1024;
 constexpr int64_t WG_SCRATCH = 64LL << 20;  // floats of weight-gradient partials
 static int64_t pick_wgrad_blocks(int64_t total, int64_t np, int64_t* per) {
   int64_t nb = (total + WG_MINCH * WG_PX - 1) / (WG_MINCH * WG_PX);
+  if (nb > WG_MAXBLK) nb = WG_MAXBLK;
   const int64_t cap = WG_SCRATCH / np;
   if (nb > cap) nb = cap;
   if (nb < 1) nb = 1;
