@@ -41,27 +41,47 @@ def levels_for(n, coarsest=33 * 17):
 
 
 def generate_pairs(n=(513, 513), media_count=8, level=2, sources=32, iters=(0, 16), seed=1000,
-                   capacity=None, log=None):
+                   capacity=None, log=None, workers=4):
+    """Capture solves of ``media_count`` seeded media run ``workers`` at a time
+    (each medium's operator has its own stream, and at 513 x 513 x 32 sources
+    the solve's launches are short enough that one medium leaves the GPU mostly
+    idle); the targets are added to the dataset one medium after another in
+    medium order, so the dataset does not depend on ``workers``."""
+    from concurrent.futures import ThreadPoolExecutor
     nl = [(v - 1) // 2**level + 1 for v in n]
     cap = capacity or media_count * sources * (iters[1] - iters[0])
     data = learned.Dataset(nl[0], nl[1], cap)
     rng = np.random.default_rng(seed)
     L = levels_for(n)
+    # every random draw in medium order first
+    draws = []
     for k in range(media_count):
         ppw = float(rng.uniform(6.0, 12.0))
+        n1 = int(n[0])
+        xs = rng.integers(n1 // 10, n1 - n1 // 10, size=sources)
+        zs = rng.integers(2, max(3, int(n[1]) // 3), size=sources)
+        draws.append((ppw, xs, zs))
+
+    def capture(k):
+        ppw, xs, zs = draws[k]
         M, hp = _medium_problem(n, seed + k, ppw)
         mg = getMGparam(complex, int, L, 1, 30, 1e-6, "Jac", 0.8, 2, 2, "W", "NoMUMPS", 0.5, 0.0)
         mg.prec = "c64"
         s = getShiftedLaplacianMultigridSolver(hp, mg, 0.5, "GMRES", 10)
-        n1 = int(M.nodes[0])
-        xs = rng.integers(n1 // 10, n1 - n1 // 10, size=sources)
-        zs = rng.integers(2, max(3, int(M.nodes[1]) // 3), size=sources)
         B = hz.point_sources(M, [(int(a), int(b)) for a, b in zip(xs, zs)])
         learned.capture_begin(s, level, sources * (iters[1] - iters[0]), iters[0], iters[1])
         solveLinearSystem(None, B, s)
-        data.add_from_solver(s, sweeps=30)
-        if log:
-            log(f"medium {k}: {ppw:.1f} ppw, dataset {data.size()} pairs")
+        return s, ppw
+
+    workers = max(1, workers)
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        for g in range(0, media_count, workers):  # at most `workers` solvers alive
+            done = list(ex.map(capture, range(g, min(g + workers, media_count))))
+            for k, (s, ppw) in enumerate(done, start=g):
+                data.add_from_solver(s, sweeps=30)
+                if log:
+                    log(f"medium {k}: {ppw:.1f} ppw, dataset {data.size()} pairs")
+            del done
     return data
 
 
