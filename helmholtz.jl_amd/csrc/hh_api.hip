@@ -671,6 +671,11 @@ struct Cycle {
     return std::is_same<T, float>::value && S->p.fused && S->p.cycle != 'K' &&
            S->p.relax_type == HH_RELAX_JAC && l + 1 < (int)S->lv.size() && post_at(l) == 2 && fused3_ok(L.g);
   }
+  // fused 3-D down leg (2 pre sweeps from zero + the residual), complex64
+  bool use_down3(const Level& L, int l) const {
+    return std::is_same<T, float>::value && S->p.fused && S->p.cycle != 'K' &&
+           S->p.relax_type == HH_RELAX_JAC && l + 1 < (int)S->lv.size() && pre_at(l) == 2 && fused3_ok(L.g);
+  }
   // damped-Jacobi sweeps before / after the coarse correction on level l
   int pre_at(int l) const { return S->p.per_level ? S->p.pre_l[l] : S->p.pre; }
   int post_at(int l) const { return S->p.per_level ? S->p.post_l[l] : S->p.post; }
@@ -823,7 +828,18 @@ struct Cycle {
       L.cur = o;
       return;
     }
-    if (zero_init) {
+    // the fused 3-D down leg leaves this level's residual in L.r
+    const bool down3 = zero_init && use_down3(L, l);
+    // kernel timing of the fused 3-D legs: algorithmic bytes as for the 2-D legs
+    const double s3 = sizeof(cx<float>), Nf3 = (double)L.g.N, act3 = S->kt_active;
+    if (down3) {
+      L.cur = first ? first : L.xa;
+      const bool td = S->kt.match(KT_DOWN, l);
+      if (td) S->kt.begin(st);
+      launch_down_fused3(L.g, nrhs, (const cx<float>*)b, (const cx<float>*)L.d, (const cx<float>*)L.f, S->p.relax,
+                         (cx<float>*)L.cur, (cx<float>*)L.r, skip, st);
+      if (td) S->kt.end(st, act3 * 3 * Nf3 * s3 + 2 * Nf3 * s3);  // b in, x2 and r out; d and F shared
+    } else if (zero_init) {
       L.cur = first ? first : L.xa;
       if (pre > 0 && S->p.relax_type == HH_RELAX_JAC_GMRES) {
         gm(L, b, pre, true);
@@ -859,8 +875,9 @@ struct Cycle {
     const int ncorr = ncorr_at(l);
     const bool up3 = use_up3(L, l);
     for (int c = 0; c < ncorr; ++c) {
-      launch_stencil<T, T, T, M_RESID>(L.g, nrhs, (const cx<T>*)L.cur, b, (cx<T>*)L.r, (const cx<T>*)L.d, nullptr,
-                                       mk<double>(0, 0), 0.0, skip, st, rows_of(L));
+      if (c > 0 || !down3)
+        launch_stencil<T, T, T, M_RESID>(L.g, nrhs, (const cx<T>*)L.cur, b, (cx<T>*)L.r, (const cx<T>*)L.d, nullptr,
+                                         mk<double>(0, 0), 0.0, skip, st, rows_of(L));
       launch_restrict<T>(L.g, C.g, nrhs, (const cx<T>*)L.r, (cx<T>*)C.b, skip, st);
       C.cur = nullptr;
       run(l + 1, (const cx<T>*)C.b, true, nullptr);
@@ -869,8 +886,11 @@ struct Cycle {
     }
     if (up3) {
       void* o = other(L);
+      const bool tu = S->kt.match(KT_UP, l);
+      if (tu) S->kt.begin(st);
       launch_up_fused3(L.g, C.g, nrhs, (const cx<float>*)L.cur, (const cx<float>*)C.cur, (const cx<float>*)b,
-                       (const cx<float>*)L.d, (const cx<float>*)L.f, (cx<float>*)o, skip, st);
+                       (const cx<float>*)L.f, S->p.relax, (cx<float>*)o, skip, st);
+      if (tu) S->kt.end(st, act3 * (3 * Nf3 * s3 + (double)C.g.N * s3) + Nf3 * s3);  // x, b in, x out; e_c; F
       L.cur = o;
       return;
     }
@@ -888,7 +908,8 @@ struct Cycle {
     }
     // number of buffer writes on level 0: pre (incl. the zero-start sweep) + post,
     // or for the fused legs down + (corrections - 1) + up
-    const int flips = use_fused(L0) ? ncorr_at(0) + 1 : pre_at(0) + (use_up3(L0, 0) ? 1 : post_at(0));
+    const int flips = use_fused(L0) ? ncorr_at(0) + 1
+                                    : (use_down3(L0, 0) ? 1 : pre_at(0)) + (use_up3(L0, 0) ? 1 : post_at(0));
     // choose the ping-pong pair (out, xa) so the last write lands in out;
     // the level's own xb is put back on every exit, exceptions included, so
     // free_work never sees the caller's block in its place

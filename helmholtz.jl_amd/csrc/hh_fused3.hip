@@ -1,75 +1,240 @@
-// Fused up leg for 3-D levels with mirror ghosts (BC = 2), complex64:
+// Fused multigrid legs for 3-D levels with mirror ghosts (BC = 2), complex64.
+//
+// Down leg (pre-smoothing from zero, then the residual):
+//
+//   x1 = F b,  x2 = x1 + F (b - A x1),  r = b - A x2
+//
+// reads b, d, F once and writes x2 and r once (unfused: the zero-start sweep,
+// one sweep and the residual, three passes).  Up leg (last coarse correction,
+// then post-smoothing):
 //
 //   x' = x + P e_c, then two damped-Jacobi sweeps x <- x + F (b - A x)
 //
-// reading x, b, d, F and the coarse correction once and writing the result
-// once (unfused: the prolongation and two sweeps, 3 passes).  A block owns a
-// 60 x 12 column of nodes (a 64 x 16 in-plane tile with a 2-node halo, one tile
-// row per wavefront as in the 2-D legs) and marches it along z.  Step z forms
-// x' on plane z, the first sweep on plane z - 1 and the second on plane z - 2;
-// x' and the first sweep live in LDS rings of three planes, and b, d, F of the
-// three planes in flight in registers; one barrier per step.  Each block covers a run of 16 output
-// planes.  Ghost planes mirror (plane -1 = 1,
-// n3 = n3 - 2) like the in-plane halos, so every formula is the unfused one:
-// the result equals prolongation + two sweeps.
+// reads x, b, F and the coarse correction once and writes the result once
+// (unfused: the prolongation and two sweeps, three passes).  F = relax D^-1
+// with D the operator's diagonal, so a sweep is (1 - relax) x + F (b + ih1 sx
+// + ih2 sy + ih3 sz) with s the neighbour sums along each axis and reads no
+// diagonal; the residual keeps the operator's form b - A x (a split residual
+// cancels terms of size 6 |x| / h^2 down to the residual).
+//
+// A block owns a 28 x 28 column of nodes (a 32 x 32 in-plane tile with a
+// 2-node halo, one position per thread of 1024: a wavefront covers two
+// 32-node tile rows) and marches it along z over a run of output planes.  (A
+// 64-wide tile wastes a quarter of its lanes on 193-wide planes; 512 threads
+// with two positions each need 92-98 VGPRs, 16 resident wavefronts per CU
+// against 32 at 55-60.)  Step z forms the
+// first stage on plane z, the second on plane z - 1 and the third on plane
+// z - 2; the first two stages' values of the last two planes live in LDS
+// rings (in-plane neighbours), a thread's own z-neighbours in registers, and
+// one barrier per step separates every LDS write from its readers.  A run
+// starting at z0 > 0 first forms its first two stages on planes z0 - 2 and
+// z0 - 1 (recomputed, not exchanged), so runs are independent.  Ghost planes
+// and in-plane ghosts mirror (index -1 = 1, n = n - 2): a ghost node's value
+// is its mirror image's at every stage, so every formula is the unfused one.
 #include "hh_kernels.h"
 
 namespace hh {
 
 namespace {
-constexpr int U3_H = 2, U3_RY = 16, U3_NW = 8, U3_KR = U3_RY / U3_NW;
-constexpr int U3_FTX = 64 - 2 * U3_H, U3_TY = U3_RY - 2 * U3_H, U3_NP = 64 * U3_RY;
+constexpr int M3_H = 2, M3_W = 32, M3_RY = 32, M3_NT = 1024;
+constexpr int M3_TX = M3_W - 2 * M3_H, M3_TY = M3_RY - 2 * M3_H, M3_NP = M3_W * M3_RY;
+// NT threads: NT / 32 tile rows per pass, KR rows per thread
+template <int NT>
+struct Geo3 {
+  static constexpr int RPP = NT / M3_W, KR = M3_RY / RPP;
+  static_assert(M3_RY % RPP == 0, "tile rows split evenly over the threads");
+};
+
+// a block's tile, z-run and this thread's in-plane positions
+template <int KR>
+struct March3 {
+  int rhs, z0, z1, za, zb;  // output planes [z0, z1), first-stage planes [za, zb)
+  int fx0, fy0, gi;         // first owned node of the tile; this thread's (mirrored) x index
+  int lj[KR], gj[KR];       // tile rows; their (mirrored) y indices
+  int po[KR];               // in-plane offsets of the (mirrored) positions (a plane has < 2^31 nodes)
+  bool in1[KR];             // inside the halo-1 ring (second stage)
+  bool own[KR];             // owned node (third stage, stored)
+};
+
+template <int NT, int KR>
+__device__ __forceinline__ bool march3_setup(const TileMap& tm, int ytiles, int zrun, const Grid& g,
+                                             const int* __restrict__ skip, March3<KR>& m) {
+  constexpr int RPP = Geo3<NT>::RPP;
+  int xt, yz;
+  tile_of(tm, xt, yz, m.rhs);
+  if (skip && skip[m.rhs]) return false;
+  const int yt = yz % ytiles, zc = yz / ytiles;
+  m.z0 = zc * zrun;
+  m.z1 = min(g.n3, m.z0 + zrun);
+  m.za = max(0, m.z0 - 2);
+  m.zb = min(g.n3, m.z1 + 2);
+  m.fx0 = xt * M3_TX;
+  m.fy0 = yt * M3_TY;
+  const int lane = threadIdx.x & (M3_W - 1), row = threadIdx.x / M3_W;
+  const int gx = m.fx0 - M3_H + lane;
+  m.gi = mirror_idx(gx, g.n1);
+  const bool inx = lane >= 1 && lane < M3_W - 1, ownx = lane >= M3_H && lane < M3_H + M3_TX && gx < g.n1;
+#pragma unroll
+  for (int k = 0; k < KR; ++k) {
+    const int lj = row + RPP * k, gy = m.fy0 - M3_H + lj;
+    m.lj[k] = lj;
+    m.gj[k] = mirror_idx(gy, g.n2);
+    m.po[k] = m.gj[k] * g.n1 + m.gi;
+    m.in1[k] = inx && lj >= 1 && lj < M3_RY - 1;
+    m.own[k] = ownx && lj >= M3_H && lj < M3_H + M3_TY && gy < g.n2;
+  }
+  return true;
+}
+
+// owned node's offset in its plane (only meaningful where own[k])
+template <int KR>
+__device__ __forceinline__ int64_t own_off(const March3<KR>& m, int k, int n1) {
+  return (int64_t)(m.fy0 - M3_H + m.lj[k]) * n1 + (m.fx0 - M3_H + (int)(threadIdx.x & (M3_W - 1)));
+}
+
+// damped-Jacobi sweep at one node without the diagonal:
+// (1 - relax) xc + F (b + ih1 (xl + xr) + ih2 (xm + xp) + ih3 (xd + xu))
+template <typename T>
+__device__ __forceinline__ cx<T> jac7(cx<T> xc, cx<T> xl, cx<T> xr, cx<T> xm, cx<T> xp, cx<T> xd, cx<T> xu,
+                                      cx<T> bq, cx<T> fq, T ih1, T ih2, T ih3, T omr) {
+  const T tr = fma(ih1, xl.re + xr.re, fma(ih2, xm.re + xp.re, fma(ih3, xd.re + xu.re, bq.re)));
+  const T ti = fma(ih1, xl.im + xr.im, fma(ih2, xm.im + xp.im, fma(ih3, xd.im + xu.im, bq.im)));
+  return mk<T>(fma(omr, xc.re, fma(fq.re, tr, -(fq.im * ti))), fma(omr, xc.im, fma(fq.re, ti, fq.im * tr)));
+}
 
 }  // namespace
 
-// Blocks also split z into runs of zrun output planes (zrun even); a run
-// starting at z0 > 0 first forms x' on planes z0 - 2, z0 - 1 and the first
-// sweep on z0 - 1 (recomputed, not exchanged), so runs are independent.
-template <typename T>
-__global__ __launch_bounds__(512) void k_up_fused3(TileMap tm, int ytiles, int zrun, Grid gf, Grid gc,
+// Down leg.  Step z: x1 on plane z (pointwise), x2 on plane z - 1 (stored for
+// the run's own planes), r on plane z - 2 (owned nodes, run's own planes).
+// b and F of plane z are loaded a step ahead and kept two more steps in
+// registers (x2 needs them on plane z - 1, r needs b on plane z - 2); the
+// diagonal of plane z - 2 is loaded at the step's start.
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void k_down_fused3(TileMap tm, int ytiles, int zrun, Grid gf,
+                                                     const cx<T>* __restrict__ b, const cx<T>* __restrict__ d,
+                                                     const cx<T>* __restrict__ f, cx<T>* __restrict__ xo,
+                                                     cx<T>* __restrict__ ro, const int* __restrict__ skip, T omr) {
+  constexpr int KR = Geo3<NT>::KR, W = M3_W;
+  __shared__ cx<T> ra[2][M3_NP], rs[2][M3_NP];
+  March3<KR> m;
+  if (!march3_setup<NT>(tm, ytiles, zrun, gf, skip, m)) return;
+  const int n1 = gf.n1, n3 = gf.n3;
+  const int64_t plane = (int64_t)n1 * gf.n2, off = (int64_t)m.rhs * gf.N;
+  const cx<T>* __restrict__ br = b + off;
+  cx<T>* __restrict__ xr = xo + off;
+  cx<T>* __restrict__ rr = ro + off;
+  const T ih1 = (T)gf.ih1, ih2 = (T)gf.ih2, ih3 = (T)gf.ih3;
+  // b, F of plane z (b0, f0) and z - 1 (b1, f1), b of z - 2 (b2); x1 of
+  // planes z, z - 1, z - 2; x2 of planes z - 1, z - 2, z - 3
+  cx<T> b0[KR], f0[KR], b1[KR], f1[KR], b2[KR], dq[KR];
+  cx<T> xa0[KR], xa1[KR], xa2[KR], sa0[KR], sa1[KR], sa2[KR];
+  cx<T> bn[KR], fn[KR];  // the next step's loads
+  auto load_step = [&](int zz) {  // b, F of plane zz (clamped: always in bounds)
+    const int64_t pz = (int64_t)min(zz, n3 - 1) * plane;
+#pragma unroll
+    for (int k = 0; k < KR; ++k) {
+      bn[k] = br[pz + m.po[k]];
+      fn[k] = f[pz + m.po[k]];
+    }
+  };
+  load_step(m.za);
+  for (int z = m.za; z <= m.z1 + 1; ++z) {
+    {  // the diagonal of plane z - 2 (used last in the step)
+      const int64_t pd = (int64_t)max(z - 2, 0) * plane;
+#pragma unroll
+      for (int k = 0; k < KR; ++k) dq[k] = d[pd + m.po[k]];
+    }
+#pragma unroll
+    for (int k = 0; k < KR; ++k) {
+      b0[k] = bn[k];
+      f0[k] = fn[k];
+    }
+    if (z < m.z1 + 1) load_step(z + 1);
+    const int q = z & 1;
+    if (z < m.zb) {  // x1 = F b on plane z
+#pragma unroll
+      for (int k = 0; k < KR; ++k) {
+        xa0[k] = cmul_fma(f0[k], b0[k]);
+        ra[q][m.lj[k] * W + (threadIdx.x & (W - 1))] = xa0[k];
+      }
+    }
+    const int p = z - 1;
+    if (p >= max(0, m.z0 - 1) && p < min(n3, m.z1 + 1)) {  // x2 on plane p
+      const cx<T>* a = ra[p & 1];
+      const bool lo = p == 0, hi = p == n3 - 1;
+      const bool store = p >= m.z0 && p < m.z1;
+#pragma unroll
+      for (int k = 0; k < KR; ++k) {
+        if (m.in1[k]) {
+          const int qq = m.lj[k] * W + (threadIdx.x & (W - 1));
+          const cx<T> xm = lo ? xa0[k] : xa2[k], xp = hi ? xa2[k] : xa0[k];
+          sa0[k] = jac7(xa1[k], a[qq - 1], a[qq + 1], a[qq - W], a[qq + W], xm, xp, b1[k], f1[k], ih1, ih2, ih3, omr);
+          rs[p & 1][qq] = sa0[k];
+          if (store && m.own[k]) xr[(int64_t)p * plane + own_off(m, k, n1)] = sa0[k];
+        }
+      }
+    }
+    const int r = z - 2;
+    if (r >= m.z0 && r < m.z1) {  // r = b - A x2 on plane r, owned nodes
+      const cx<T>* s = rs[r & 1];
+      const bool lo = r == 0, hi = r == n3 - 1;
+#pragma unroll
+      for (int k = 0; k < KR; ++k) {
+        if (m.own[k]) {
+          const int qq = m.lj[k] * W + (threadIdx.x & (W - 1));
+          const cx<T> sm = lo ? sa0[k] : sa2[k], sp = hi ? sa2[k] : sa0[k];
+          rr[(int64_t)r * plane + own_off(m, k, n1)] =
+              b2[k] - apply7(sa1[k], s[qq - 1], s[qq + 1], s[qq - W], s[qq + W], sm, sp, ih1, ih2, ih3, dq[k]);
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < KR; ++k) {
+      b2[k] = b1[k];
+      b1[k] = b0[k];
+      f1[k] = f0[k];
+      xa2[k] = xa1[k];
+      xa1[k] = xa0[k];
+      sa2[k] = sa1[k];
+      sa1[k] = sa0[k];
+    }
+    __syncthreads();
+  }
+}
+
+// Up leg.  Step z: x' on plane z, the first sweep on plane z - 1, the second
+// on plane z - 2 (owned nodes, run's own planes, stored).  x of plane z and b,
+// F of plane z - 1 are loaded a step ahead; b, F stay one more step in
+// registers for the second sweep.
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void k_up_fused3(TileMap tm, int ytiles, int zrun, Grid gf, Grid gc,
                                                    const cx<T>* __restrict__ x, const cx<T>* __restrict__ ec,
-                                                   const cx<T>* __restrict__ b, const cx<T>* __restrict__ d,
-                                                   const cx<T>* __restrict__ f, cx<T>* __restrict__ xo,
-                                                   const int* __restrict__ skip) {
-  constexpr int H = U3_H, NW = U3_NW, KR = U3_KR, RY = U3_RY;
-  __shared__ cx<T> ra[3][U3_NP], rs[3][U3_NP];
-  int xt, yz, rhs;
-  tile_of(tm, xt, yz, rhs);
-  if (skip && skip[rhs]) return;
-  const int yt = yz % ytiles, zc = yz / ytiles;
-  const int n1 = gf.n1, n2 = gf.n2, n3 = gf.n3;
-  const int z0 = zc * zrun, z1 = min(n3, z0 + zrun);
-  const int za = max(0, z0 - 2), zb = min(n3, z1 + 2);  // planes of x'
-  const int fx0 = xt * U3_FTX, fy0 = yt * U3_TY;
-  const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int gi = mirror_idx(fx0 - H + lane, n1);
-  const int64_t plane = (int64_t)n1 * n2, off = (int64_t)rhs * gf.N;
+                                                   const cx<T>* __restrict__ b, const cx<T>* __restrict__ f,
+                                                   cx<T>* __restrict__ xo, const int* __restrict__ skip, T omr) {
+  constexpr int KR = Geo3<NT>::KR, W = M3_W;
+  __shared__ cx<T> ra[2][M3_NP], rs[2][M3_NP];
+  March3<KR> m;
+  if (!march3_setup<NT>(tm, ytiles, zrun, gf, skip, m)) return;
+  const int n1 = gf.n1, n3 = gf.n3;
+  const int64_t plane = (int64_t)n1 * gf.n2, off = (int64_t)m.rhs * gf.N;
   const int64_t cplane = (int64_t)gc.n1 * gc.n2;
   const cx<T>* __restrict__ xr = x + off;
   const cx<T>* __restrict__ br = b + off;
-  const cx<T>* __restrict__ cr = ec + (int64_t)rhs * gc.N;
+  const cx<T>* __restrict__ cr = ec + (int64_t)m.rhs * gc.N;
   cx<T>* __restrict__ yr = xo + off;
   const T ih1 = (T)gf.ih1, ih2 = (T)gf.ih2, ih3 = (T)gf.ih3;
-  // in-plane position data (rows are wave-uniform)
-  int lj[KR], gj[KR];
-#pragma unroll
-  for (int k = 0; k < KR; ++k) {
-    lj[k] = wv + NW * k;
-    gj[k] = mirror_idx(fy0 - H + lj[k], n2);
-  }
-  const bool in1 = lane >= 1 && lane < 63;
-  const bool own_c = lane >= H && lane < H + U3_FTX && fx0 - H + lane < n1;
   // in-plane bilinear interpolant of coarse plane K at this thread's positions
-  const int ci0 = gi >> 1, ci1 = (gi & 1) ? ci0 + 1 : ci0;
+  const int gi = m.gi, ci0 = gi >> 1, ci1 = (gi & 1) ? ci0 + 1 : ci0;
   const T wi = (gi & 1) ? (T)0.5 : (T)1;
   auto cval = [&](int K, int k) {
+    const int gj = m.gj[k];
     const cx<T>* cp = cr + (int64_t)K * cplane;
-    const int cj0 = gj[k] >> 1, cj1 = (gj[k] & 1) ? cj0 + 1 : cj0;
-    const T w = wi * ((gj[k] & 1) ? (T)0.5 : (T)1);
+    const int cj0 = gj >> 1, cj1 = (gj & 1) ? cj0 + 1 : cj0;
+    const T w = wi * ((gj & 1) ? (T)0.5 : (T)1);
     const int64_t r0 = (int64_t)cj0 * gc.n1, r1 = (int64_t)cj1 * gc.n1;
     cx<T> acc = w * cp[r0 + ci0];
     if (gi & 1) acc = acc + w * cp[r0 + ci1];
-    if (gj[k] & 1) {
+    if (gj & 1) {
       acc = acc + w * cp[r1 + ci0];
       if (gi & 1) acc = acc + w * cp[r1 + ci1];
     }
@@ -77,38 +242,31 @@ __global__ __launch_bounds__(512) void k_up_fused3(TileMap tm, int ytiles, int z
   };
   cx<T> cK[KR];
 #pragma unroll
-  for (int k = 0; k < KR; ++k) cK[k] = cval(za >> 1, k);
-  // b, d, F of planes z (0), z - 1 (1), z - 2 (2); this thread's own x' of
-  // planes z, z - 1, z - 2 and first-sweep values of planes z - 1, z - 2,
-  // z - 3 (the z-neighbours of its positions come from registers, only the
-  // in-plane neighbours from LDS)
-  cx<T> b0[KR], d0[KR], f0[KR], b1[KR], d1[KR], f1[KR], b2[KR], d2[KR], f2[KR];
+  for (int k = 0; k < KR; ++k) cK[k] = cval(m.za >> 1, k);
+  // b, F of plane z - 1 (bp, fp) and z - 2 (b2, f2); x' of planes z, z - 1,
+  // z - 2; first-sweep values of planes z - 1, z - 2, z - 3
+  cx<T> xv[KR], bp[KR], fp[KR], b2[KR], f2[KR];
   cx<T> xa0[KR], xa1[KR], xa2[KR], sa0[KR], sa1[KR], sa2[KR];
-  // x, b, d, F of the next plane, loaded a step before they are used
-  cx<T> xn[KR], bn[KR], dn[KR], fn[KR];
-  auto load_plane = [&](int zz) {
-    const int64_t pz = (int64_t)zz * plane;
+  cx<T> xn[KR], bn[KR], fn[KR];  // the next step's loads
+  auto load_step = [&](int zz) {  // x of plane zz, b and F of plane zz - 1 (clamped: always in bounds)
+    const int64_t pz = (int64_t)min(zz, n3 - 1) * plane, pb = (int64_t)max(zz - 1, 0) * plane;
 #pragma unroll
     for (int k = 0; k < KR; ++k) {
-      const int64_t o = pz + (int64_t)gj[k] * n1 + gi;
-      xn[k] = xr[o];
-      bn[k] = br[o];
-      dn[k] = d[o];
-      fn[k] = f[o];
+      xn[k] = xr[pz + m.po[k]];
+      bn[k] = br[pb + m.po[k]];
+      fn[k] = f[pb + m.po[k]];
     }
   };
-  load_plane(za);
-  for (int z = za; z <= z1 + 1; ++z) {
-    if (z < zb) {  // x' on plane z (its x, b, d, F were loaded one step ahead)
-      cx<T> xv[KR];
+  load_step(m.za);
+  for (int z = m.za; z <= m.z1 + 1; ++z) {
 #pragma unroll
-      for (int k = 0; k < KR; ++k) {
-        xv[k] = xn[k];
-        b0[k] = bn[k];
-        d0[k] = dn[k];
-        f0[k] = fn[k];
-      }
-      if (z + 1 < zb) load_plane(z + 1);
+    for (int k = 0; k < KR; ++k) {
+      xv[k] = xn[k];
+      bp[k] = bn[k];
+      fp[k] = fn[k];
+    }
+    if (z < m.z1 + 1) load_step(z + 1);
+    if (z < m.zb) {  // x' = x + P e_c on plane z
 #pragma unroll
       for (int k = 0; k < KR; ++k) {
         cx<T> e;
@@ -120,59 +278,46 @@ __global__ __launch_bounds__(512) void k_up_fused3(TileMap tm, int ytiles, int z
           e = cK[k];
         }
         xa0[k] = xv[k] + e;
-        ra[z % 3][lj[k] * 64 + lane] = xa0[k];
+        ra[z & 1][m.lj[k] * W + (threadIdx.x & (W - 1))] = xa0[k];
       }
     }
-    if (z - 1 >= max(0, z0 - 1) && z - 1 < min(n3, z1 + 1)) {  // first sweep on plane p = z - 1
-      const int p = z - 1;
-      const cx<T>* a0 = ra[p % 3];
-      cx<T>* s = rs[p % 3];
-      const bool lo = p == 0, hi = p == n3 - 1;  // mirrored z-neighbour
+    const int p = z - 1;
+    if (p >= max(0, m.z0 - 1) && p < min(n3, m.z1 + 1)) {  // first sweep on plane p
+      const cx<T>* a = ra[p & 1];
+      const bool lo = p == 0, hi = p == n3 - 1;
 #pragma unroll
       for (int k = 0; k < KR; ++k) {
-        const int q = lj[k] * 64 + lane;
-        if (lj[k] >= 1 && lj[k] < RY - 1 && in1) {
-          const cx<T> xc = xa1[k];
+        if (m.in1[k]) {
+          const int qq = m.lj[k] * W + (threadIdx.x & (W - 1));
           const cx<T> xm = lo ? xa0[k] : xa2[k], xp = hi ? xa2[k] : xa0[k];
-          const cx<T> Ax = apply7(xc, a0[q - 1], a0[q + 1], a0[q - 64], a0[q + 64], xm, xp, ih1, ih2, ih3, d1[k]);
-          sa0[k] = xc + f1[k] * (b1[k] - Ax);
-          s[q] = sa0[k];
+          sa0[k] = jac7(xa1[k], a[qq - 1], a[qq + 1], a[qq - W], a[qq + W], xm, xp, bp[k], fp[k], ih1, ih2, ih3, omr);
+          rs[p & 1][qq] = sa0[k];
         }
       }
     }
-    if (z - 2 >= z0 && z - 2 < z1) {  // second sweep on plane r = z - 2, owned nodes stored
-      const int r = z - 2;
-      const cx<T>* s0 = rs[r % 3];
-      const int64_t pr = (int64_t)r * plane;
+    const int r = z - 2;
+    if (r >= m.z0 && r < m.z1) {  // second sweep on plane r, owned nodes stored
+      const cx<T>* s = rs[r & 1];
       const bool lo = r == 0, hi = r == n3 - 1;
 #pragma unroll
       for (int k = 0; k < KR; ++k) {
-        const int q = lj[k] * 64 + lane;
-        const int j = fy0 - H + lj[k];
-        if (lj[k] >= H && lj[k] < H + U3_TY && j < n2 && own_c) {
-          const cx<T> xc = sa1[k];
+        if (m.own[k]) {
+          const int qq = m.lj[k] * W + (threadIdx.x & (W - 1));
           const cx<T> sm = lo ? sa0[k] : sa2[k], sp = hi ? sa2[k] : sa0[k];
-          const cx<T> Ax = apply7(xc, s0[q - 1], s0[q + 1], s0[q - 64], s0[q + 64], sm, sp, ih1, ih2, ih3, d2[k]);
-          yr[pr + (int64_t)j * n1 + (fx0 - H + lane)] = xc + f2[k] * (b2[k] - Ax);
+          yr[(int64_t)r * plane + own_off(m, k, n1)] =
+              jac7(sa1[k], s[qq - 1], s[qq + 1], s[qq - W], s[qq + W], sm, sp, b2[k], f2[k], ih1, ih2, ih3, omr);
         }
       }
     }
 #pragma unroll
     for (int k = 0; k < KR; ++k) {
-      b2[k] = b1[k];
-      d2[k] = d1[k];
-      f2[k] = f1[k];
-      b1[k] = b0[k];
-      d1[k] = d0[k];
-      f1[k] = f0[k];
+      b2[k] = bp[k];
+      f2[k] = fp[k];
       xa2[k] = xa1[k];
       xa1[k] = xa0[k];
       sa2[k] = sa1[k];
       sa1[k] = sa0[k];
     }
-    // each stage reads in LDS only planes written in earlier steps (the
-    // z-neighbours it needs from this step are the thread's own registers),
-    // so one barrier per step separates every write from its readers
     __syncthreads();
   }
 }
@@ -181,13 +326,41 @@ bool fused3_ok(const Grid& gf) {
   return gf.dim == 3 && gf.bc == 2 && gf.n1 >= 16 && gf.n2 >= 16 && gf.n3 >= 3;
 }
 
+namespace {
+// z-runs: at most 32 output planes each, and more (down to 8 planes) while a
+// launch has fewer than 2048 blocks; runs recompute two planes on each side
+struct Runs3 {
+  int ytiles, zrun;
+  TileMap m;
+};
+Runs3 runs3(const Grid& g, int nrhs) {
+  Runs3 r;
+  const int xtiles = (g.n1 + M3_TX - 1) / M3_TX;
+  r.ytiles = (g.n2 + M3_TY - 1) / M3_TY;
+  const int64_t per_run = (int64_t)xtiles * r.ytiles * nrhs;
+  int runs = (g.n3 + 31) / 32;
+  const int want = (int)std::min<int64_t>((2048 + per_run - 1) / per_run, std::max(1, g.n3 / 8));
+  runs = std::max(runs, want);
+  r.zrun = (g.n3 + runs - 1) / runs;
+  r.zrun += r.zrun & 1;
+  r.m = make_tile_map(xtiles, r.ytiles * ((g.n3 + r.zrun - 1) / r.zrun), nrhs);
+  return r;
+}
+}  // namespace
+
+void launch_down_fused3(const Grid& gf, int nrhs, const cx<float>* b, const cx<float>* d, const cx<float>* f,
+                        double relax, cx<float>* xo, cx<float>* ro, const int* skip, hipStream_t s) {
+  const Runs3 r = runs3(gf, nrhs);
+  hipLaunchKernelGGL((k_down_fused3<float, M3_NT>), dim3(r.m.T), dim3(M3_NT), 0, s, r.m, r.ytiles, r.zrun, gf, b, d,
+                     f, xo, ro, skip, (float)(1.0 - relax));
+}
+
 void launch_up_fused3(const Grid& gf, const Grid& gc, int nrhs, const cx<float>* x, const cx<float>* ec,
-                      const cx<float>* b, const cx<float>* d, const cx<float>* f, cx<float>* xo, const int* skip,
+                      const cx<float>* b, const cx<float>* f, double relax, cx<float>* xo, const int* skip,
                       hipStream_t s) {
-  const int ytiles = (gf.n2 + U3_TY - 1) / U3_TY, zrun = 16;
-  const TileMap m = make_tile_map((gf.n1 + U3_FTX - 1) / U3_FTX, ytiles * ((gf.n3 + zrun - 1) / zrun), nrhs);
-  hipLaunchKernelGGL((k_up_fused3<float>), dim3(m.T), dim3(512), 0, s, m, ytiles, zrun, gf, gc, x, ec, b, d, f, xo,
-                     skip);
+  const Runs3 r = runs3(gf, nrhs);
+  hipLaunchKernelGGL((k_up_fused3<float, M3_NT>), dim3(r.m.T), dim3(M3_NT), 0, s, r.m, r.ytiles, r.zrun, gf, gc, x,
+                     ec, b, f, xo, skip, (float)(1.0 - relax));
 }
 
 }  // namespace hh

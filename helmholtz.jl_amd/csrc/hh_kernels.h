@@ -67,10 +67,14 @@ void kc_update(int64_t N, int nrhs, int mode, const cx<double>* sc, cx<T>* y, co
 
 // ---- fused 2-D multigrid legs (hh_fused.hip): BC = 2, 2 pre/2 post sweeps ----
 bool fused_ok(const Grid& gf);
-// ---- fused 3-D up leg (hh_fused3.hip): x' = x + P e_c, two Jacobi sweeps, complex64 ----
+// ---- fused 3-D legs (hh_fused3.hip), complex64, f = relax D^-1 ----
 bool fused3_ok(const Grid& gf);
+// down: x2 = two Jacobi sweeps from zero (into xo), r = b - A x2 (into ro)
+void launch_down_fused3(const Grid& gf, int nrhs, const cx<float>* b, const cx<float>* d, const cx<float>* f,
+                        double relax, cx<float>* xo, cx<float>* ro, const int* skip, hipStream_t s);
+// up: x' = x + P e_c, then two Jacobi sweeps (into xo)
 void launch_up_fused3(const Grid& gf, const Grid& gc, int nrhs, const cx<float>* x, const cx<float>* ec,
-                      const cx<float>* b, const cx<float>* d, const cx<float>* f, cx<float>* xo, const int* skip,
+                      const cx<float>* b, const cx<float>* f, double relax, cx<float>* xo, const int* skip,
                       hipStream_t s);
 // f = relax D^-1 (the level's Jacobi factor) and relax itself;
 // xo = null: the smoothed iterate is not stored (the middle leg recomputes it)

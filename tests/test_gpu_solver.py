@@ -419,11 +419,13 @@ def test_jac_gmres_kcycle_solve(prec):
 
 @pytest.mark.parametrize("nodes", [(17, 17, 9), (33, 17, 17), (65, 49, 33), (129, 65, 17)])
 @pytest.mark.parametrize("cycle", ["V", "W"])
-def test_fused_up3_equals_unfused(nodes, cycle):
-    """The fused 3-D up leg (z-marching: x + P e_c and two Jacobi sweeps in
-    one kernel) gives the complex64 cycle of the separate kernels (tiles that
-    straddle the grid edge and the z boundaries included); also checked
-    against the complex128 oracle cycle at complex64 accuracy."""
+def test_fused_legs3_equal_unfused(nodes, cycle):
+    """The fused 3-D legs (z-marching: two Jacobi sweeps from zero and the
+    residual in one kernel; x + P e_c and two Jacobi sweeps in another) give
+    the complex64 cycle of the separate kernels to complex64 rounding (the
+    fused sweeps are evaluated without the diagonal); tiles that straddle the
+    grid edge, several z-runs per grid and the z boundaries included.  Also
+    checked against the complex128 oracle cycle at complex64 accuracy."""
     M, m, g, w = small_case(nodes, seed=77, neumann=True)
     rng = np.random.default_rng(78)
     V = np.asfortranarray(rng.standard_normal((m.size, 3)) + 1j * rng.standard_normal((m.size, 3)))
@@ -433,7 +435,7 @@ def test_fused_up3_equals_unfused(nodes, cycle):
         s.MG.fused = fused
         zs.append(ha.precondition(s, V).astype(np.complex128))
     err = np.linalg.norm(zs[0] - zs[1]) / np.linalg.norm(zs[1])
-    print(f"fused up3 vs unfused {nodes} {cycle}: {err:.2e}")
+    print(f"fused 3-D legs vs unfused {nodes} {cycle}: {err:.2e}")
     assert err <= 2e-6
     p = hr.HelmholtzParam(M, g, m, w, True, True)
     hier = sr.build_hierarchy(p, 3, 0.5)
