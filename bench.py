@@ -33,8 +33,8 @@ launch event-timed inside full solves against its algorithmic bytes;
 ``cpu_baseline`` (the oracle's C/OpenMP restatement of the plain method on
 the host's cores, one source) with ``host`` (CPU model, core counts);
 ``three_d`` (193 x 193 x 97, 32 sources, complex64 vs complex128
-preconditioner, and the plain method one source per call; ``--no-3d`` skips
-it); ``stream`` and ``stream_learned``, an uneven stream of 200 independent
+preconditioner, the fused 3-D legs' event-timed roofline, and the plain
+method one source per call; ``--no-3d`` skips it); ``stream`` and ``stream_learned``, an uneven stream of 200 independent
 problems without and with the trained network (``--no-stream`` skips them).
 
 Usage: python bench.py [--gpus 1] [--steps 2] [--warmup 1] [--plain-rhs 4]
@@ -119,6 +119,12 @@ def c3_lines(log, c64_basis=True):
         out[f"{key}_max_iters"] = int(st["max_iters"])
         out[f"{key}_max_true_res"] = float(max(st["true_res"]))
         out[f"{key}_setup_s"] = round(setup, 3)
+        if key == "c64":
+            # the fused 3-D legs on the finest grid, every launch of a full solve event-timed
+            out["roofline_kernels"] = [kernel_roofline(ha, s, Bd, k, 0, name) for k, name in ROOFLINE_KERNELS_3D]
+            for r in out["roofline_kernels"]:
+                log(f"3-D roofline {r['kernel'].split(' ')[0]}: {r['achieved']:.0f} GB/s ({r['frac']:.3f}), "
+                    f"{r['avg_launch_ms']:.3f} ms/launch")
     # the plain method: complex128 preconditioner, one source per call
     ms, its, res, setup, _ = plain_single_rhs(ha, hp, B, [0, 13], 5, "W", 8, [0.5] * 5, log=log, name="3-D plain")
     out["plain_c128_single_rhs"] = {"ms_per_rhs": round(ms, 2), "iters": its, "max_true_res": max(res),
@@ -218,6 +224,12 @@ ROOFLINE_KERNELS = [
     ("up", 0, "k_up_fused<float> on the 1025x513 grid (x += P e_c, two damped-Jacobi sweeps)"),
     ("mid", 0, "k_mid_fused<float> on the 1025x513 grid (x2 recomputed, x' = x2 + P e_c, r, R r)"),
     ("down", 0, "k_down_fused<float> on the 1025x513 grid (two sweeps from zero, r, R r)"),
+]
+
+
+ROOFLINE_KERNELS_3D = [
+    ("down", 0, "k_down_fused3<float,1024> on the 193x193x97 grid (two sweeps from zero, r = b - A x2; z-marching)"),
+    ("up", 0, "k_up_fused3<float,1024> on the 193x193x97 grid (x += P e_c, two damped-Jacobi sweeps; z-marching)"),
 ]
 
 
