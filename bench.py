@@ -121,7 +121,7 @@ def c3_lines(log, c64_basis=True):
         out[f"{key}_setup_s"] = round(setup, 3)
         if key == "c64":
             # the fused 3-D legs on the finest grid, every launch of a full solve event-timed
-            out["roofline_kernels"] = [kernel_roofline(ha, s, Bd, k, 0, name) for k, name in ROOFLINE_KERNELS_3D]
+            out["roofline_kernels"] = [kernel_roofline(ha, s, Bd, k, lev, name) for k, lev, name in ROOFLINE_KERNELS_3D]
             for r in out["roofline_kernels"]:
                 log(f"3-D roofline {r['kernel'].split(' ')[0]}: {r['achieved']:.0f} GB/s ({r['frac']:.3f}), "
                     f"{r['avg_launch_ms']:.3f} ms/launch")
