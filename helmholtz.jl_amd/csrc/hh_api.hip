@@ -676,6 +676,11 @@ struct Cycle {
     return std::is_same<T, float>::value && S->p.fused && S->p.cycle != 'K' &&
            S->p.relax_type == HH_RELAX_JAC && l + 1 < (int)S->lv.size() && pre_at(l) == 2 && fused3_ok(L.g);
   }
+  // fused 3-D middle step of a W visit (prolongation + the next residual)
+  bool use_mid3(const Level& L, int l) const {
+    return std::is_same<T, float>::value && S->p.fused && S->p.cycle != 'K' &&
+           S->p.relax_type == HH_RELAX_JAC && l + 1 < (int)S->lv.size() && fused3_ok(L.g);
+  }
   // damped-Jacobi sweeps before / after the coarse correction on level l
   int pre_at(int l) const { return S->p.per_level ? S->p.pre_l[l] : S->p.pre; }
   int post_at(int l) const { return S->p.per_level ? S->p.post_l[l] : S->p.post; }
@@ -873,16 +878,30 @@ struct Cycle {
     // current residual (iterating the coarse cycle on the restricted residual
     // instead diverges with rediscretised coarse operators near kh ~ 2).
     const int ncorr = ncorr_at(l);
-    const bool up3 = use_up3(L, l);
+    const bool up3 = use_up3(L, l), mid3 = use_mid3(L, l);
+    bool have_r = down3;  // L.r holds the residual of the current iterate
     for (int c = 0; c < ncorr; ++c) {
-      if (c > 0 || !down3)
+      if (!have_r)
         launch_stencil<T, T, T, M_RESID>(L.g, nrhs, (const cx<T>*)L.cur, b, (cx<T>*)L.r, (const cx<T>*)L.d, nullptr,
                                          mk<double>(0, 0), 0.0, skip, st, rows_of(L));
+      have_r = false;
       launch_restrict<T>(L.g, C.g, nrhs, (const cx<T>*)L.r, (cx<T>*)C.b, skip, st);
       C.cur = nullptr;
       run(l + 1, (const cx<T>*)C.b, true, nullptr);
       if (up3 && c + 1 == ncorr) break;  // the last correction is added by the fused up leg
-      launch_prolong_add<T>(L.g, C.g, nrhs, (const cx<T>*)C.cur, (cx<T>*)L.cur, skip, st);
+      if (mid3 && c + 1 < ncorr) {       // correction and the next residual in one pass
+        void* o = other(L);
+        const bool tm = S->kt.match(KT_MID, l);
+        if (tm) S->kt.begin(st);
+        launch_mid_fused3(L.g, C.g, nrhs, (const cx<float>*)L.cur, (const cx<float>*)C.cur, (const cx<float>*)b,
+                          (const cx<float>*)L.d, (cx<float>*)o, (cx<float>*)L.r, skip, st);
+        // x and b in, x' and r out, the coarse correction; the diagonal shared
+        if (tm) S->kt.end(st, act3 * (4 * Nf3 * s3 + (double)C.g.N * s3) + Nf3 * s3);
+        L.cur = o;
+        have_r = true;
+      } else {
+        launch_prolong_add<T>(L.g, C.g, nrhs, (const cx<T>*)C.cur, (cx<T>*)L.cur, skip, st);
+      }
     }
     if (up3) {
       void* o = other(L);
@@ -906,10 +925,12 @@ struct Cycle {
       coarse_apply<T>(S, L0.g.N, nrhs, b, out, skip, st);
       return;
     }
-    // number of buffer writes on level 0: pre (incl. the zero-start sweep) + post,
-    // or for the fused legs down + (corrections - 1) + up
+    // number of buffer writes on level 0: pre (incl. the zero-start sweep) +
+    // post, or for the fused legs down + (corrections - 1) + up (3-D: each
+    // fused leg one write, the middle steps one each)
     const int flips = use_fused(L0) ? ncorr_at(0) + 1
-                                    : (use_down3(L0, 0) ? 1 : pre_at(0)) + (use_up3(L0, 0) ? 1 : post_at(0));
+                                    : (use_down3(L0, 0) ? 1 : pre_at(0)) + (use_mid3(L0, 0) ? ncorr_at(0) - 1 : 0) +
+                                          (use_up3(L0, 0) ? 1 : post_at(0));
     // choose the ping-pong pair (out, xa) so the last write lands in out;
     // the level's own xb is put back on every exit, exceptions included, so
     // free_work never sees the caller's block in its place

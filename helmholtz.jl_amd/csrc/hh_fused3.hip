@@ -36,8 +36,11 @@
 namespace hh {
 
 namespace {
-constexpr int M3_H = 2, M3_W = 32, M3_RY = 32, M3_NT = 1024;
-constexpr int M3_TX = M3_W - 2 * M3_H, M3_TY = M3_RY - 2 * M3_H, M3_NP = M3_W * M3_RY;
+constexpr int M3_W = 32, M3_RY = 32, M3_NT = 1024, M3_NP = M3_W * M3_RY;
+// owned columns / rows of a tile with an H-node halo (H = 2 for the legs
+// with two stencil stages, 1 for the middle step with one)
+constexpr int m3_tx(int H) { return M3_W - 2 * H; }
+constexpr int m3_ty(int H) { return M3_RY - 2 * H; }
 // NT threads: NT / 32 tile rows per pass, KR rows per thread
 template <int NT>
 struct Geo3 {
@@ -56,9 +59,10 @@ struct March3 {
   bool own[KR];             // owned node (third stage, stored)
 };
 
-template <int NT, int KR>
+template <int NT, int H, int KR>
 __device__ __forceinline__ bool march3_setup(const TileMap& tm, int ytiles, int zrun, const Grid& g,
                                              const int* __restrict__ skip, March3<KR>& m) {
+  constexpr int TX = m3_tx(H), TY = m3_ty(H);
   constexpr int RPP = Geo3<NT>::RPP;
   int xt, yz;
   tile_of(tm, xt, yz, m.rhs);
@@ -66,30 +70,48 @@ __device__ __forceinline__ bool march3_setup(const TileMap& tm, int ytiles, int 
   const int yt = yz % ytiles, zc = yz / ytiles;
   m.z0 = zc * zrun;
   m.z1 = min(g.n3, m.z0 + zrun);
-  m.za = max(0, m.z0 - 2);
-  m.zb = min(g.n3, m.z1 + 2);
-  m.fx0 = xt * M3_TX;
-  m.fy0 = yt * M3_TY;
+  m.za = max(0, m.z0 - H);
+  m.zb = min(g.n3, m.z1 + H);
+  m.fx0 = xt * TX;
+  m.fy0 = yt * TY;
   const int lane = threadIdx.x & (M3_W - 1), row = threadIdx.x / M3_W;
-  const int gx = m.fx0 - M3_H + lane;
+  const int gx = m.fx0 - H + lane;
   m.gi = mirror_idx(gx, g.n1);
-  const bool inx = lane >= 1 && lane < M3_W - 1, ownx = lane >= M3_H && lane < M3_H + M3_TX && gx < g.n1;
+  const bool inx = lane >= 1 && lane < M3_W - 1, ownx = lane >= H && lane < H + TX && gx < g.n1;
 #pragma unroll
   for (int k = 0; k < KR; ++k) {
-    const int lj = row + RPP * k, gy = m.fy0 - M3_H + lj;
+    const int lj = row + RPP * k, gy = m.fy0 - H + lj;
     m.lj[k] = lj;
     m.gj[k] = mirror_idx(gy, g.n2);
     m.po[k] = m.gj[k] * g.n1 + m.gi;
     m.in1[k] = inx && lj >= 1 && lj < M3_RY - 1;
-    m.own[k] = ownx && lj >= M3_H && lj < M3_H + M3_TY && gy < g.n2;
+    m.own[k] = ownx && lj >= H && lj < H + TY && gy < g.n2;
   }
   return true;
 }
 
 // owned node's offset in its plane (only meaningful where own[k])
-template <int KR>
+template <int H, int KR>
 __device__ __forceinline__ int64_t own_off(const March3<KR>& m, int k, int n1) {
-  return (int64_t)(m.fy0 - M3_H + m.lj[k]) * n1 + (m.fx0 - M3_H + (int)(threadIdx.x & (M3_W - 1)));
+  return (int64_t)(m.fy0 - H + m.lj[k]) * n1 + (m.fx0 - H + (int)(threadIdx.x & (M3_W - 1)));
+}
+
+// bilinear interpolant of coarse plane K at the fine in-plane position
+// (gi, gj): the coarse nodes around it, weights 1 or 1/2 per axis
+template <typename T>
+__device__ __forceinline__ cx<T> coarse_at(const cx<T>* __restrict__ cr, const Grid& gc, int K, int gi, int gj) {
+  const cx<T>* cp = cr + (int64_t)K * gc.n1 * gc.n2;
+  const int ci0 = gi >> 1, ci1 = (gi & 1) ? ci0 + 1 : ci0;
+  const int cj0 = gj >> 1, cj1 = (gj & 1) ? cj0 + 1 : cj0;
+  const T w = ((gi & 1) ? (T)0.5 : (T)1) * ((gj & 1) ? (T)0.5 : (T)1);
+  const int64_t r0 = (int64_t)cj0 * gc.n1, r1 = (int64_t)cj1 * gc.n1;
+  cx<T> acc = w * cp[r0 + ci0];
+  if (gi & 1) acc = acc + w * cp[r0 + ci1];
+  if (gj & 1) {
+    acc = acc + w * cp[r1 + ci0];
+    if (gi & 1) acc = acc + w * cp[r1 + ci1];
+  }
+  return acc;
 }
 
 // damped-Jacobi sweep at one node without the diagonal:
@@ -117,7 +139,7 @@ __global__ __launch_bounds__(NT) void k_down_fused3(TileMap tm, int ytiles, int 
   constexpr int KR = Geo3<NT>::KR, W = M3_W;
   __shared__ cx<T> ra[2][M3_NP], rs[2][M3_NP];
   March3<KR> m;
-  if (!march3_setup<NT>(tm, ytiles, zrun, gf, skip, m)) return;
+  if (!march3_setup<NT, 2>(tm, ytiles, zrun, gf, skip, m)) return;
   const int n1 = gf.n1, n3 = gf.n3;
   const int64_t plane = (int64_t)n1 * gf.n2, off = (int64_t)m.rhs * gf.N;
   const cx<T>* __restrict__ br = b + off;
@@ -170,7 +192,7 @@ __global__ __launch_bounds__(NT) void k_down_fused3(TileMap tm, int ytiles, int 
           const cx<T> xm = lo ? xa0[k] : xa2[k], xp = hi ? xa2[k] : xa0[k];
           sa0[k] = jac7(xa1[k], a[qq - 1], a[qq + 1], a[qq - W], a[qq + W], xm, xp, b1[k], f1[k], ih1, ih2, ih3, omr);
           rs[p & 1][qq] = sa0[k];
-          if (store && m.own[k]) xr[(int64_t)p * plane + own_off(m, k, n1)] = sa0[k];
+          if (store && m.own[k]) xr[(int64_t)p * plane + own_off<2>(m, k, n1)] = sa0[k];
         }
       }
     }
@@ -183,7 +205,7 @@ __global__ __launch_bounds__(NT) void k_down_fused3(TileMap tm, int ytiles, int 
         if (m.own[k]) {
           const int qq = m.lj[k] * W + (threadIdx.x & (W - 1));
           const cx<T> sm = lo ? sa0[k] : sa2[k], sp = hi ? sa2[k] : sa0[k];
-          rr[(int64_t)r * plane + own_off(m, k, n1)] =
+          rr[(int64_t)r * plane + own_off<2>(m, k, n1)] =
               b2[k] - apply7(sa1[k], s[qq - 1], s[qq + 1], s[qq - W], s[qq + W], sm, sp, ih1, ih2, ih3, dq[k]);
         }
       }
@@ -214,32 +236,16 @@ __global__ __launch_bounds__(NT) void k_up_fused3(TileMap tm, int ytiles, int zr
   constexpr int KR = Geo3<NT>::KR, W = M3_W;
   __shared__ cx<T> ra[2][M3_NP], rs[2][M3_NP];
   March3<KR> m;
-  if (!march3_setup<NT>(tm, ytiles, zrun, gf, skip, m)) return;
+  if (!march3_setup<NT, 2>(tm, ytiles, zrun, gf, skip, m)) return;
   const int n1 = gf.n1, n3 = gf.n3;
   const int64_t plane = (int64_t)n1 * gf.n2, off = (int64_t)m.rhs * gf.N;
-  const int64_t cplane = (int64_t)gc.n1 * gc.n2;
   const cx<T>* __restrict__ xr = x + off;
   const cx<T>* __restrict__ br = b + off;
   const cx<T>* __restrict__ cr = ec + (int64_t)m.rhs * gc.N;
   cx<T>* __restrict__ yr = xo + off;
   const T ih1 = (T)gf.ih1, ih2 = (T)gf.ih2, ih3 = (T)gf.ih3;
   // in-plane bilinear interpolant of coarse plane K at this thread's positions
-  const int gi = m.gi, ci0 = gi >> 1, ci1 = (gi & 1) ? ci0 + 1 : ci0;
-  const T wi = (gi & 1) ? (T)0.5 : (T)1;
-  auto cval = [&](int K, int k) {
-    const int gj = m.gj[k];
-    const cx<T>* cp = cr + (int64_t)K * cplane;
-    const int cj0 = gj >> 1, cj1 = (gj & 1) ? cj0 + 1 : cj0;
-    const T w = wi * ((gj & 1) ? (T)0.5 : (T)1);
-    const int64_t r0 = (int64_t)cj0 * gc.n1, r1 = (int64_t)cj1 * gc.n1;
-    cx<T> acc = w * cp[r0 + ci0];
-    if (gi & 1) acc = acc + w * cp[r0 + ci1];
-    if (gj & 1) {
-      acc = acc + w * cp[r1 + ci0];
-      if (gi & 1) acc = acc + w * cp[r1 + ci1];
-    }
-    return acc;
-  };
+  auto cval = [&](int K, int k) { return coarse_at(cr, gc, K, m.gi, m.gj[k]); };
   cx<T> cK[KR];
 #pragma unroll
   for (int k = 0; k < KR; ++k) cK[k] = cval(m.za >> 1, k);
@@ -304,7 +310,7 @@ __global__ __launch_bounds__(NT) void k_up_fused3(TileMap tm, int ytiles, int zr
         if (m.own[k]) {
           const int qq = m.lj[k] * W + (threadIdx.x & (W - 1));
           const cx<T> sm = lo ? sa0[k] : sa2[k], sp = hi ? sa2[k] : sa0[k];
-          yr[(int64_t)r * plane + own_off(m, k, n1)] =
+          yr[(int64_t)r * plane + own_off<2>(m, k, n1)] =
               jac7(sa1[k], s[qq - 1], s[qq + 1], s[qq - W], s[qq + W], sm, sp, b2[k], f2[k], ih1, ih2, ih3, omr);
         }
       }
@@ -322,6 +328,94 @@ __global__ __launch_bounds__(NT) void k_up_fused3(TileMap tm, int ytiles, int zr
   }
 }
 
+// Middle step of a W visit (the first coarse correction added, then the
+// residual for the second).  Step z: x' = x + P e_c on plane z (every tile
+// position; owned nodes of the run's planes stored), r = b - A x' on plane
+// z - 1 (owned nodes, run's planes).  One stencil stage: a 1-node halo (30 x
+// 30 owned nodes per tile).  x of plane z and b, d of plane z - 1 are loaded
+// a step ahead.
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void k_mid_fused3(TileMap tm, int ytiles, int zrun, Grid gf, Grid gc,
+                                                    const cx<T>* __restrict__ x, const cx<T>* __restrict__ ec,
+                                                    const cx<T>* __restrict__ b, const cx<T>* __restrict__ d,
+                                                    cx<T>* __restrict__ xo, cx<T>* __restrict__ ro,
+                                                    const int* __restrict__ skip) {
+  constexpr int KR = Geo3<NT>::KR, W = M3_W, H = 1;
+  __shared__ cx<T> ra[2][M3_NP];
+  March3<KR> m;
+  if (!march3_setup<NT, H>(tm, ytiles, zrun, gf, skip, m)) return;
+  const int n1 = gf.n1, n3 = gf.n3;
+  const int64_t plane = (int64_t)n1 * gf.n2, off = (int64_t)m.rhs * gf.N;
+  const cx<T>* __restrict__ xr = x + off;
+  const cx<T>* __restrict__ br = b + off;
+  const cx<T>* __restrict__ cr = ec + (int64_t)m.rhs * gc.N;
+  cx<T>* __restrict__ yr = xo + off;
+  cx<T>* __restrict__ rr = ro + off;
+  const T ih1 = (T)gf.ih1, ih2 = (T)gf.ih2, ih3 = (T)gf.ih3;
+  cx<T> cK[KR];
+#pragma unroll
+  for (int k = 0; k < KR; ++k) cK[k] = coarse_at(cr, gc, m.za >> 1, m.gi, m.gj[k]);
+  // x of plane z (xv), b and d of plane z - 1 (bp, dp); x' of planes z, z - 1, z - 2
+  cx<T> xv[KR], bp[KR], dp[KR], xa0[KR], xa1[KR], xa2[KR];
+  cx<T> xn[KR], bn[KR], dn[KR];  // the next step's loads
+  auto load_step = [&](int zz) {  // x of plane zz, b and d of plane zz - 1 (clamped: always in bounds)
+    const int64_t pz = (int64_t)min(zz, n3 - 1) * plane, pb = (int64_t)max(min(zz, n3) - 1, 0) * plane;
+#pragma unroll
+    for (int k = 0; k < KR; ++k) {
+      xn[k] = xr[pz + m.po[k]];
+      bn[k] = br[pb + m.po[k]];
+      dn[k] = d[pb + m.po[k]];
+    }
+  };
+  load_step(m.za);
+  for (int z = m.za; z <= m.z1; ++z) {
+#pragma unroll
+    for (int k = 0; k < KR; ++k) {
+      xv[k] = xn[k];
+      bp[k] = bn[k];
+      dp[k] = dn[k];
+    }
+    if (z < m.z1) load_step(z + 1);
+    if (z < m.zb) {  // x' = x + P e_c on plane z
+      const bool store = z >= m.z0 && z < m.z1;
+#pragma unroll
+      for (int k = 0; k < KR; ++k) {
+        cx<T> e;
+        if (z & 1) {
+          const cx<T> cN = coarse_at(cr, gc, (z >> 1) + 1, m.gi, m.gj[k]);
+          e = (T)0.5 * (cK[k] + cN);
+          cK[k] = cN;
+        } else {
+          e = cK[k];
+        }
+        xa0[k] = xv[k] + e;
+        ra[z & 1][m.lj[k] * W + (threadIdx.x & (W - 1))] = xa0[k];
+        if (store && m.own[k]) yr[(int64_t)z * plane + own_off<H>(m, k, n1)] = xa0[k];
+      }
+    }
+    const int p = z - 1;
+    if (p >= m.z0 && p < m.z1) {  // r = b - A x' on plane p, owned nodes
+      const cx<T>* a = ra[p & 1];
+      const bool lo = p == 0, hi = p == n3 - 1;
+#pragma unroll
+      for (int k = 0; k < KR; ++k) {
+        if (m.own[k]) {
+          const int qq = m.lj[k] * W + (threadIdx.x & (W - 1));
+          const cx<T> xm = lo ? xa0[k] : xa2[k], xp = hi ? xa2[k] : xa0[k];
+          rr[(int64_t)p * plane + own_off<H>(m, k, n1)] =
+              bp[k] - apply7(xa1[k], a[qq - 1], a[qq + 1], a[qq - W], a[qq + W], xm, xp, ih1, ih2, ih3, dp[k]);
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < KR; ++k) {
+      xa2[k] = xa1[k];
+      xa1[k] = xa0[k];
+    }
+    __syncthreads();
+  }
+}
+
 bool fused3_ok(const Grid& gf) {
   return gf.dim == 3 && gf.bc == 2 && gf.n1 >= 16 && gf.n2 >= 16 && gf.n3 >= 3;
 }
@@ -333,10 +427,10 @@ struct Runs3 {
   int ytiles, zrun;
   TileMap m;
 };
-Runs3 runs3(const Grid& g, int nrhs) {
+Runs3 runs3(const Grid& g, int nrhs, int H) {
   Runs3 r;
-  const int xtiles = (g.n1 + M3_TX - 1) / M3_TX;
-  r.ytiles = (g.n2 + M3_TY - 1) / M3_TY;
+  const int xtiles = (g.n1 + m3_tx(H) - 1) / m3_tx(H);
+  r.ytiles = (g.n2 + m3_ty(H) - 1) / m3_ty(H);
   const int64_t per_run = (int64_t)xtiles * r.ytiles * nrhs;
   int runs = (g.n3 + 31) / 32;
   const int want = (int)std::min<int64_t>((2048 + per_run - 1) / per_run, std::max(1, g.n3 / 8));
@@ -350,15 +444,23 @@ Runs3 runs3(const Grid& g, int nrhs) {
 
 void launch_down_fused3(const Grid& gf, int nrhs, const cx<float>* b, const cx<float>* d, const cx<float>* f,
                         double relax, cx<float>* xo, cx<float>* ro, const int* skip, hipStream_t s) {
-  const Runs3 r = runs3(gf, nrhs);
+  const Runs3 r = runs3(gf, nrhs, 2);
   hipLaunchKernelGGL((k_down_fused3<float, M3_NT>), dim3(r.m.T), dim3(M3_NT), 0, s, r.m, r.ytiles, r.zrun, gf, b, d,
                      f, xo, ro, skip, (float)(1.0 - relax));
+}
+
+void launch_mid_fused3(const Grid& gf, const Grid& gc, int nrhs, const cx<float>* x, const cx<float>* ec,
+                       const cx<float>* b, const cx<float>* d, cx<float>* xo, cx<float>* ro, const int* skip,
+                       hipStream_t s) {
+  const Runs3 r = runs3(gf, nrhs, 1);
+  hipLaunchKernelGGL((k_mid_fused3<float, M3_NT>), dim3(r.m.T), dim3(M3_NT), 0, s, r.m, r.ytiles, r.zrun, gf, gc, x,
+                     ec, b, d, xo, ro, skip);
 }
 
 void launch_up_fused3(const Grid& gf, const Grid& gc, int nrhs, const cx<float>* x, const cx<float>* ec,
                       const cx<float>* b, const cx<float>* f, double relax, cx<float>* xo, const int* skip,
                       hipStream_t s) {
-  const Runs3 r = runs3(gf, nrhs);
+  const Runs3 r = runs3(gf, nrhs, 2);
   hipLaunchKernelGGL((k_up_fused3<float, M3_NT>), dim3(r.m.T), dim3(M3_NT), 0, s, r.m, r.ytiles, r.zrun, gf, gc, x,
                      ec, b, f, xo, skip, (float)(1.0 - relax));
 }

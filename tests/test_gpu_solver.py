@@ -421,7 +421,8 @@ def test_jac_gmres_kcycle_solve(prec):
 @pytest.mark.parametrize("cycle", ["V", "W"])
 def test_fused_legs3_equal_unfused(nodes, cycle):
     """The fused 3-D legs (z-marching: two Jacobi sweeps from zero and the
-    residual in one kernel; x + P e_c and two Jacobi sweeps in another) give
+    residual in one kernel; on W levels x + P e_c and the next residual in a
+    second; x + P e_c and two Jacobi sweeps in a third) give
     the complex64 cycle of the separate kernels to complex64 rounding (the
     fused sweeps are evaluated without the diagonal); tiles that straddle the
     grid edge, several z-runs per grid and the z boundaries included.  Also
