@@ -555,6 +555,16 @@ void ensure_work(hh_solver* S, int nrhs) {
   // blocks in all (64 per CU, so the last wave of blocks is a small tail),
   // chunks of at least 4096 nodes
   int nchunk = (int)std::max<int64_t>(1, std::min<int64_t>((16384 + nrhs - 1) / nrhs, (N + 4095) / 4096));
+  // 3-D: chunk c is in-plane segment c mod nseg3 over run c / nseg3 of the
+  // planes (runs of at least 8 planes) for the H z passes' z march
+  w.nseg3 = 0;
+  if (col3_ok(S->lv[0].g)) {
+    const Grid& g0 = S->lv[0].g;
+    const int nseg = col3_segments(g0);
+    const int runs = std::max(1, std::min(nchunk / nseg, g0.n3 / 8));
+    nchunk = nseg * runs;
+    w.nseg3 = nseg;
+  }
   w.nchunk = nchunk;
   w.part = dalloc<cx<double>>((size_t)nchunk * (K + 1) * nrhs);
   w.partn = dalloc<double>((size_t)nchunk * nrhs);

@@ -150,6 +150,7 @@ struct KrylovWork {
   int64_t N;
   int kmax;              // restart length
   int nchunk;            // node chunks per RHS for the reductions
+  int nseg3;             // 3-D H z passes: in-plane segments (nchunk = nseg3 x runs of planes); 0: linear chunks
   cx<double>* part;      // [nchunk][kmax+1][nrhs] partial dots
   double* partn;         // [nchunk][nrhs] partial squared norms
   cx<double>* h1;        // [kmax+1][nrhs]
@@ -198,6 +199,10 @@ struct HzArgs {
   const cx<float>* z;
   const cx<double>* d;
 };
+// 3-D grids whose H z passes march z over in-plane segments of 512 nodes
+// (n1 <= 768), and the number of such segments per plane
+bool col3_ok(const Grid& g);
+int col3_segments(const Grid& g);
 // One CGS2 Arnoldi step j: orthogonalise wv against V_0..V_j, write V_{j+1},
 // fill column j of H, update Givens rotations and residual estimates.
 // skip_copy: nrhs ints of scratch.  hz (lazy mode only, may be null): take

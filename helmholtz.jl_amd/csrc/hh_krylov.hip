@@ -531,6 +531,237 @@ __global__ __launch_bounds__(256) void k_update_lazy_hz(KrylovWork kw, Grid g, c
   if (threadIdx.x == 0) kw.partn[(int64_t)c * kw.nrhs + b] = t;
 }
 
+// ---- 3-D: the same two passes marching z over in-plane columns ----
+// Chunk c of a RHS is the in-plane segment c % nseg3 (C3_LQ consecutive
+// nodes of a plane, two per thread) over the run c / nseg3 of the planes.
+// LDS holds z on the windows [q0 - n1, q0 + C3_LQ + n1) of four planes:
+// planes k - 1, k, k + 1 give H z on plane k (the in-plane neighbours and the
+// values a plane away), plane k + 2 is loaded during step k and stored after
+// it, one barrier per plane.  In the linear walk the values a plane away
+// were fetched again from HBM (the plane had left L2 by the time a block
+// came back to it): 1.2-1.45x the algorithmic bytes; here each z value is
+// loaded once per block and the window's halo rows are shared in L2 with
+// the neighbouring segments' blocks, which run beside it.
+constexpr int C3_LQ = 512;
+
+struct Col3 {
+  int q0, q1, w0, w1, k0, k1;  // segment [q0, q1), window [w0, w1), planes [k0, k1)
+};
+__device__ __forceinline__ Col3 col3_of(const Grid& g, int c, int nseg) {
+  Col3 r;
+  const int plane = g.n1 * g.n2;
+  const int sg = c % nseg, zr = c / nseg, nzr = max(1, (int)(gridDim.y / nseg));
+  r.q0 = sg * C3_LQ;
+  r.q1 = min(r.q0 + C3_LQ, plane);
+  r.w0 = max(r.q0 - g.n1, 0);
+  r.w1 = min(r.q1 + g.n1, plane);
+  r.k0 = (int)((int64_t)g.n3 * zr / nzr);
+  r.k1 = (int)((int64_t)g.n3 * (zr + 1) / nzr);
+  return r;
+}
+
+// window of plane k (in [0, n3)) into its slot k mod 4 (slots of WS values)
+template <int M>
+__device__ __forceinline__ void col3_fill(cx<float>* zw, const cx<float>* __restrict__ zb, const Col3& cb,
+                                          int64_t plane, int k) {
+  constexpr int WS = 256 * M;
+  cx<float>* dst = zw + (k & 3) * WS;
+  const cx<float>* src = zb + (int64_t)k * plane + cb.w0;
+  for (int e = threadIdx.x; e < cb.w1 - cb.w0; e += 256) dst[e] = src[e];
+}
+
+// H z operands of in-plane node q = cj n1 + ci on plane k from the windows
+template <int M>
+__device__ __forceinline__ void col3_ops(const Grid& g, const cx<float>* zw, const Col3& cb,
+                                         const cx<double>* __restrict__ d, int k, int q, int ci, int cj,
+                                         HzOps<3>& o) {
+  constexpr int WS = 256 * M;
+  const int n1 = g.n1;
+  const cx<float>* wk = zw + (k & 3) * WS;
+  const int row = cj * n1;
+  o.c = wk[q - cb.w0];
+  o.x1l = wk[row + lo_nb(ci, g.bc) - cb.w0];
+  o.x1r = wk[row + hi_nb(ci, n1, g.bc) - cb.w0];
+  o.x2l = wk[lo_nb(cj, g.bc) * n1 + ci - cb.w0];
+  o.x2r = wk[hi_nb(cj, g.n2, g.bc) * n1 + ci - cb.w0];
+  o.x3l = zw[(lo_nb(k, g.bc) & 3) * WS + q - cb.w0];
+  o.x3r = zw[(hi_nb(k, g.n3, g.bc) & 3) * WS + q - cb.w0];
+  o.d = d[(int64_t)k * n1 * g.n2 + q];
+}
+
+// pass 1 (3-D columns): part = V^H (H z), partn = ||H z||^2
+template <int NV, typename TV, int M>
+__global__ __launch_bounds__(256) void k_dots_hz3c(KrylovWork kw, Grid g, const cx<TV>* __restrict__ V,
+                                                   const cx<float>* __restrict__ Z, const cx<double>* __restrict__ d,
+                                                   int j) {
+  constexpr bool EARLY = NV <= 10;
+  constexpr int PT = C3_LQ / 256;  // in-plane nodes per thread (two, taken in turn)
+  const int b = blockIdx.x, c = blockIdx.y;
+  if (kw.done[b]) return;
+  __shared__ double sh[4];
+  extern __shared__ cx<float> zw[];
+  const Col3 cb = col3_of(g, c, kw.nseg3);
+  const int64_t plane = (int64_t)g.n1 * g.n2;
+  const int64_t stride = kw.N * kw.nrhs;
+  const int64_t base = (int64_t)b * kw.N;
+  const cx<float>* __restrict__ zb = Z + base;
+  cx<double> acc[NV];
+#pragma unroll
+  for (int i = 0; i < NV; ++i) acc[i] = mk<double>(0.0, 0.0);
+  double nn = 0.0;
+  const double zs = kw.zunscaled ? kw.vscale[j * kw.nrhs + b] : 1.0;
+  int qv[PT], ci[PT], cj[PT];
+  bool ok[PT];
+#pragma unroll
+  for (int t = 0; t < PT; ++t) {
+    const int q = cb.q0 + threadIdx.x + 256 * t;
+    ok[t] = q < cb.q1;
+    qv[t] = ok[t] ? q : cb.q0;  // a node past the segment reads node q0 and contributes 0
+    ci[t] = qv[t] % g.n1;
+    cj[t] = qv[t] / g.n1;
+  }
+  for (int k = max(cb.k0 - 1, 0); k <= min(cb.k0 + 1, g.n3 - 1); ++k) col3_fill<M>(zw, zb, cb, plane, k);
+  __syncthreads();
+  for (int k = cb.k0; k < cb.k1; ++k) {
+    // the window of plane k + 2, stored after this step's reads
+    const int kp = k + 2;
+    const bool pf = kp < g.n3 && kp <= cb.k1;
+    cx<float> zn[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      const int e = threadIdx.x + 256 * m;
+      zn[m] = pf && e < cb.w1 - cb.w0 ? zb[(int64_t)kp * plane + cb.w0 + e] : mk<float>(0.f, 0.f);
+    }
+    // the thread's nodes one at a time (each with all its loads issued first)
+#pragma unroll 1
+    for (int t = 0; t < PT; ++t) {
+      const int q = t == 0 ? qv[0] : qv[1], qi = t == 0 ? ci[0] : ci[1], qj = t == 0 ? cj[0] : cj[1];
+      const bool qok = t == 0 ? ok[0] : ok[1];
+      const int64_t p = k * plane + q;
+      HzOps<3> ho;
+      cx<TV> vv[EARLY ? NV : 1];
+      col3_ops<M>(g, zw, cb, d, k, q, qi, qj, ho);
+      if constexpr (EARLY) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) vv[i] = V[(int64_t)min(i, j) * stride + base + p];
+      }
+      const cx<double> wv = qok ? zs * hz_eval<3>(g, ho) : mk<double>(0.0, 0.0);
+      nn = nrm_acc(nn, wv);
+#pragma unroll
+      for (int i = 0; i < NV; ++i) {
+        if constexpr (EARLY)
+          acc[i] = cmac_conj(acc[i], cvt<TV, double>(vv[i]), wv);
+        else if (i <= j)
+          acc[i] = cmac_conj(acc[i], cvt<TV, double>(V[(int64_t)i * stride + base + p]), wv);
+      }
+    }
+    if (pf) {
+#pragma unroll
+      for (int m = 0; m < M; ++m) {
+        const int e = threadIdx.x + 256 * m;
+        if (e < cb.w1 - cb.w0) zw[(kp & 3) * (256 * M) + e] = zn[m];
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    if (i <= j) {
+      const double si = kw.vscale[i * kw.nrhs + b];
+      const double re = block_sum(si * acc[i].re, sh);
+      const double im = block_sum(si * acc[i].im, sh);
+      if (threadIdx.x == 0) kw.part[((int64_t)c * (kw.kmax + 1) + i) * kw.nrhs + b] = mk<double>(re, im);
+    }
+  }
+  const double t = block_sum(nn, sh);
+  if (threadIdx.x == 0) kw.partn[(int64_t)c * kw.nrhs + b] = t;
+}
+
+// pass 2 (3-D columns): w' = H z - V h1 (stored in the basis precision),
+// partn = ||w'||^2 of the unrounded w'
+template <int NV, typename TV, int M>
+__global__ __launch_bounds__(256) void k_update_lazy_hz3c(KrylovWork kw, Grid g, const cx<TV>* __restrict__ V,
+                                                          const cx<float>* __restrict__ Z,
+                                                          const cx<double>* __restrict__ d,
+                                                          cx<TV>* __restrict__ wout, int j) {
+  constexpr bool EARLY = NV <= 10;
+  constexpr int PT = C3_LQ / 256;
+  const int b = blockIdx.x, c = blockIdx.y;
+  if (kw.done[b]) return;
+  __shared__ double sh[4];
+  __shared__ cx<double> hs[NV];
+  extern __shared__ cx<float> zw[];
+  if (threadIdx.x < NV) hs[threadIdx.x] = threadIdx.x <= j ? kw.h1s[threadIdx.x * kw.nrhs + b] : mk<double>(0.0, 0.0);
+  const Col3 cb = col3_of(g, c, kw.nseg3);
+  const int64_t plane = (int64_t)g.n1 * g.n2;
+  const int64_t stride = kw.N * kw.nrhs;
+  const int64_t base = (int64_t)b * kw.N;
+  const cx<float>* __restrict__ zb = Z + base;
+  double nn = 0.0;
+  const double zs = kw.zunscaled ? kw.vscale[j * kw.nrhs + b] : 1.0;
+  int qv[PT], ci[PT], cj[PT];
+  bool ok[PT];
+#pragma unroll
+  for (int t = 0; t < PT; ++t) {
+    const int q = cb.q0 + threadIdx.x + 256 * t;
+    ok[t] = q < cb.q1;
+    qv[t] = ok[t] ? q : cb.q0;
+    ci[t] = qv[t] % g.n1;
+    cj[t] = qv[t] / g.n1;
+  }
+  for (int k = max(cb.k0 - 1, 0); k <= min(cb.k0 + 1, g.n3 - 1); ++k) col3_fill<M>(zw, zb, cb, plane, k);
+  __syncthreads();
+  for (int k = cb.k0; k < cb.k1; ++k) {
+    const int kp = k + 2;
+    const bool pf = kp < g.n3 && kp <= cb.k1;
+    cx<float> zn[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      const int e = threadIdx.x + 256 * m;
+      zn[m] = pf && e < cb.w1 - cb.w0 ? zb[(int64_t)kp * plane + cb.w0 + e] : mk<float>(0.f, 0.f);
+    }
+#pragma unroll 1
+    for (int t = 0; t < PT; ++t) {
+      const int q = t == 0 ? qv[0] : qv[1], qi = t == 0 ? ci[0] : ci[1], qj = t == 0 ? cj[0] : cj[1];
+      const bool qok = t == 0 ? ok[0] : ok[1];
+      const int64_t p = k * plane + q;
+      HzOps<3> ho;
+      cx<TV> vv[EARLY ? NV : 1];
+      col3_ops<M>(g, zw, cb, d, k, q, qi, qj, ho);
+      if constexpr (EARLY) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) vv[i] = V[(int64_t)min(i, j) * stride + base + p];
+      }
+      cx<double> wv = zs * hz_eval<3>(g, ho);
+#pragma unroll
+      for (int i = 0; i < NV; ++i) {
+        if constexpr (EARLY)
+          wv = csub_mul(wv, hs[i], cvt<TV, double>(vv[i]));
+        else if (i <= j)
+          wv = csub_mul(wv, hs[i], cvt<TV, double>(V[(int64_t)i * stride + base + p]));
+      }
+      if (qok) {
+        wout[base + p] = cvt<double, TV>(wv);
+        if (kw.stage) kw.stage[base + p] = mk<float>((float)wv.re, (float)wv.im);
+        nn = nrm_acc(nn, wv);
+      }
+    }
+    if (pf) {
+#pragma unroll
+      for (int m = 0; m < M; ++m) {
+        const int e = threadIdx.x + 256 * m;
+        if (e < cb.w1 - cb.w0) zw[(kp & 3) * (256 * M) + e] = zn[m];
+      }
+    }
+    __syncthreads();
+  }
+  const double t = block_sum(nn, sh);
+  if (threadIdx.x == 0) kw.partn[(int64_t)c * kw.nrhs + b] = t;
+}
+
+bool col3_ok(const Grid& g) { return g.dim == 3 && g.n1 <= 768 && (int64_t)g.n1 * g.n2 < (1LL << 30) && g.n3 >= 2; }
+int col3_segments(const Grid& g) { return (int)(((int64_t)g.n1 * g.n2 + C3_LQ - 1) / C3_LQ); }
+
 // lazy mode: fix[b] = 1 where the projection cancelled severely,
 // ||w'||^2 < 1e-8 ||w||^2 (one wavefront per RHS)
 __global__ __launch_bounds__(64) void k_fix_flags(KrylovWork kw) {
@@ -850,7 +1081,18 @@ static void arnoldi_nv(const KrylovWork& w, const cx<TV>* V, cx<double>* wv, int
     if (r <= 8192) rnodes = (int)r;
   }
   const size_t rbytes = sizeof(cx<float>) * (size_t)rnodes;
-  if (hz) {
+  // 3-D: z march over in-plane segments, four plane windows of 1024 or 2048 values
+  const bool c3 = hz && hz->g.dim == 3 && w.nseg3 > 0;
+  const int m3 = c3 && C3_LQ + 2 * hz->g.n1 > 1024 ? 8 : 4;
+  const size_t c3bytes = sizeof(cx<float>) * 4 * 256 * (size_t)m3;
+  if (c3) {
+    if (m3 == 4)
+      hipLaunchKernelGGL((k_dots_hz3c<NV, TV, 4>), dim3(w.nrhs, w.nchunk), dim3(256), c3bytes, s, w, hz->g, V,
+                         hz->z, hz->d, j);
+    else
+      hipLaunchKernelGGL((k_dots_hz3c<NV, TV, 8>), dim3(w.nrhs, w.nchunk), dim3(256), c3bytes, s, w, hz->g, V,
+                         hz->z, hz->d, j);
+  } else if (hz) {
     if (rnodes && hz->g.dim == 2)
       hipLaunchKernelGGL((k_dots_hz<NV, TV, U, 2, true>), dim3(w.nrhs, w.nchunk), dim3(256), rbytes, s, w, hz->g,
                          V, hz->z, hz->d, j, rnodes - 1);
@@ -872,7 +1114,14 @@ static void arnoldi_nv(const KrylovWork& w, const cx<TV>* V, cx<double>* wv, int
   if (t1) kt->end(s, hz ? (sv * (j + 1) + 8.0) * Nf * active + 16.0 * Nf : 16.0 * Nf * (j + 2) * active);
   hipLaunchKernelGGL(k_reduce_dots, dim3(j + (w.lazy ? 2 : 1), w.nrhs), dim3(64), 0, s, w, w.h1, j);
   if (t2) kt->begin(s);
-  if (hz) {
+  if (c3) {
+    if (m3 == 4)
+      hipLaunchKernelGGL((k_update_lazy_hz3c<NV, TV, 4>), dim3(w.nrhs, w.nchunk), dim3(256), c3bytes, s, w, hz->g,
+                         V, hz->z, hz->d, vnext, j);
+    else
+      hipLaunchKernelGGL((k_update_lazy_hz3c<NV, TV, 8>), dim3(w.nrhs, w.nchunk), dim3(256), c3bytes, s, w, hz->g,
+                         V, hz->z, hz->d, vnext, j);
+  } else if (hz) {
     if (rnodes && hz->g.dim == 2)
       hipLaunchKernelGGL((k_update_lazy_hz<NV, TV, U, 2, true>), dim3(w.nrhs, w.nchunk), dim3(256), rbytes, s, w,
                          hz->g, V, hz->z, hz->d, vnext, j, rnodes - 1);
