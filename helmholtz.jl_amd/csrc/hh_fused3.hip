@@ -11,7 +11,12 @@
 //   x' = x + P e_c, then two damped-Jacobi sweeps x <- x + F (b - A x)
 //
 // reads x, b, F and the coarse correction once and writes the result once
-// (unfused: the prolongation and two sweeps, three passes).  F = relax D^-1
+// (unfused: the prolongation and two sweeps, three passes).  On W levels the
+// middle step between the two coarse corrections,
+//
+//   x' = x + P e_c,  r = b - A x'
+//
+// is one pass too (unfused: the prolongation and the residual).  F = relax D^-1
 // with D the operator's diagonal, so a sweep is (1 - relax) x + F (b + ih1 sx
 // + ih2 sy + ih3 sz) with s the neighbour sums along each axis and reads no
 // diagonal; the residual keeps the operator's form b - A x (a split residual
@@ -19,16 +24,17 @@
 //
 // A block owns a 28 x 28 column of nodes (a 32 x 32 in-plane tile with a
 // 2-node halo, one position per thread of 1024: a wavefront covers two
-// 32-node tile rows) and marches it along z over a run of output planes.  (A
+// 32-node tile rows; the middle step, one stencil stage, a 1-node halo and
+// 30 x 30 nodes) and marches it along z over a run of output planes.  (A
 // 64-wide tile wastes a quarter of its lanes on 193-wide planes; 512 threads
 // with two positions each need 92-98 VGPRs, 16 resident wavefronts per CU
-// against 32 at 55-60.)  Step z forms the
-// first stage on plane z, the second on plane z - 1 and the third on plane
-// z - 2; the first two stages' values of the last two planes live in LDS
-// rings (in-plane neighbours), a thread's own z-neighbours in registers, and
-// one barrier per step separates every LDS write from its readers.  A run
-// starting at z0 > 0 first forms its first two stages on planes z0 - 2 and
-// z0 - 1 (recomputed, not exchanged), so runs are independent.  Ghost planes
+// against 32 at 55-60.)  Step z forms the first stage on plane z, the second
+// on plane z - 1 and the third on plane z - 2; the earlier stages' values of
+// the last two planes live in LDS rings (in-plane neighbours), a thread's
+// own z-neighbours in registers, and one barrier per step separates every
+// LDS write from its readers.  A run starting at z0 > 0 first forms its
+// earlier stages on the planes before z0 (recomputed, not exchanged), so
+// runs are independent.  Ghost planes
 // and in-plane ghosts mirror (index -1 = 1, n = n - 2): a ghost node's value
 // is its mirror image's at every stage, so every formula is the unfused one.
 #include "hh_kernels.h"
