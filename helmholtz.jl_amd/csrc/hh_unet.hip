@@ -63,55 +63,77 @@ static T* dmal(size_t n) {
 }
 
 // ------------------------------------------------------------------ kernels
-// A wavefront computes 16 pixels x NCO * 16 output channels (channel block
-// blockIdx.z of a layer with `cout` outputs): NCO is picked per launch so that
-// small grids still give several wavefronts per SIMD.
-template <int NCO>
+// A wavefront computes PM x 16 pixels x NCO * 16 output channels (channel
+// block blockIdx.z of a layer with `cout` outputs).  Every operand load of a
+// (tap, 16-channel) step feeds several MFMAs: the input tile of a pixel group
+// is used for the NCO channel blocks, the weight tile for the PM pixel groups
+// (with one of each, two 512-byte loads per MFMA kept the loads, not the
+// matrix cores, busy).  NCO and PM are picked per launch so that small grids
+// still give several wavefronts per SIMD.
+template <int NCO, int PM>
 __global__ __launch_bounds__(256) void k_conv3x3(const bf16* __restrict__ in, int cin, const bf16* __restrict__ w,
                                                  const float* __restrict__ bias, bf16* __restrict__ out, int nx,
                                                  int ny, int relu, int cout) {
   const int img = blockIdx.y;
   const int npix = nx * ny;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int tile = blockIdx.x * 64 + wv * 16;
+  const int tile = (blockIdx.x * 4 + wv) * 16 * PM;
   if (tile >= npix) return;  // uniform per wavefront
   const int co0 = blockIdx.z * NCO * 16;
   const int m = lane & 15;
   const int kq = (lane >> 4) * 4;
-  const int pa = tile + m;
-  const bool pa_ok = pa < npix;
-  const int ia = pa_ok ? pa % nx : 0, ja = pa_ok ? pa / nx : 0;
-  const bf16* inb = in + (int64_t)img * npix * cin;
-  f4 acc[NCO];
+  int ia[PM], ja[PM];
+  bool pa_ok[PM];
 #pragma unroll
-  for (int c = 0; c < NCO; ++c) acc[c] = f4{0.f, 0.f, 0.f, 0.f};
+  for (int g = 0; g < PM; ++g) {
+    const int pa = tile + 16 * g + m;
+    pa_ok[g] = pa < npix;
+    ia[g] = pa_ok[g] ? pa % nx : 0;
+    ja[g] = pa_ok[g] ? pa / nx : 0;
+  }
+  const bf16* inb = in + (int64_t)img * npix * cin;
+  f4 acc[PM][NCO];
+#pragma unroll
+  for (int g = 0; g < PM; ++g)
+#pragma unroll
+    for (int c = 0; c < NCO; ++c) acc[g][c] = f4{0.f, 0.f, 0.f, 0.f};
   for (int tap = 0; tap < 9; ++tap) {
-    const int ii = ia + tap % 3 - 1, jj = ja + tap / 3 - 1;
-    const bool ok = pa_ok && ii >= 0 && ii < nx && jj >= 0 && jj < ny;
-    const bf16* src = inb + ((int64_t)(ok ? jj : 0) * nx + (ok ? ii : 0)) * cin + kq;
+    const bf16* src[PM];
+    bool ok[PM];
+#pragma unroll
+    for (int g = 0; g < PM; ++g) {
+      const int ii = ia[g] + tap % 3 - 1, jj = ja[g] + tap / 3 - 1;
+      ok[g] = pa_ok[g] && ii >= 0 && ii < nx && jj >= 0 && jj < ny;
+      src[g] = inb + ((int64_t)(ok[g] ? jj : 0) * nx + (ok[g] ? ii : 0)) * cin + kq;
+    }
     const bf16* wtap = w + (int64_t)tap * cout * cin + (int64_t)(co0 + m) * cin + kq;
     for (int cb = 0; cb < cin; cb += 16) {
-      s4 a = ok ? *(const s4*)(src + cb) : s4{0, 0, 0, 0};
+      s4 a[PM];
+#pragma unroll
+      for (int g = 0; g < PM; ++g) a[g] = ok[g] ? *(const s4*)(src[g] + cb) : s4{0, 0, 0, 0};
 #pragma unroll
       for (int c = 0; c < NCO; ++c) {
         const s4 bv = *(const s4*)(wtap + (int64_t)c * 16 * cin + cb);
-        acc[c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bv, acc[c], 0, 0, 0);
+#pragma unroll
+        for (int g = 0; g < PM; ++g) acc[g][c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[g], bv, acc[g][c], 0, 0, 0);
       }
     }
   }
   bf16* ob = out + (int64_t)img * npix * cout;
 #pragma unroll
-  for (int c = 0; c < NCO; ++c)
+  for (int g = 0; g < PM; ++g)
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int p = tile + kq + i;
-      const int co = co0 + c * 16 + m;
-      if (p < npix) {
-        float v = acc[c][i] + (bias ? bias[co] : 0.f);
-        if (relu) v = v > 0.f ? v : 0.f;
-        ob[(int64_t)p * cout + co] = f2bf(v);
+    for (int c = 0; c < NCO; ++c)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int p = tile + 16 * g + kq + i;
+        const int co = co0 + c * 16 + m;
+        if (p < npix) {
+          float v = acc[g][c][i] + (bias ? bias[co] : 0.f);
+          if (relu) v = v > 0.f ? v : 0.f;
+          ob[(int64_t)p * cout + co] = f2bf(v);
+        }
       }
-    }
 }
 
 // Weight and bias gradients of a 3x3 layer as one MFMA GEMM per tap over
@@ -592,28 +614,58 @@ static void ensure_ws(UNetWork& ws, int nimg, int nx, int ny) {
 static bf16* act(UNetWork& ws, int a) { return (bf16*)ws.bufs[a]; }
 static bf16* grd(UNetWork& ws, int a) { return (bf16*)ws.bufs[A_NUM + a]; }
 
-// output channels per wavefront: the most (of the layer's 16-channel blocks)
-// that still leaves 4096 wavefronts (four per SIMD), else one block
-static int conv_nco(int cout, int64_t pixels) {
+// output channels (NCO 16-channel blocks) and pixel groups (PM groups of 16)
+// per wavefront: the fewest operand loads per MFMA (1 / NCO + 1 / PM) with
+// NCO * PM <= 8 (at most 32 accumulator registers: six or more wavefronts per
+// SIMD) that still leaves 2048 wavefronts (two per SIMD)
+static void conv_shape(int cout, int64_t pixels, int* nco_out, int* pm_out) {
   const int nb = cout / 16;
-  const int64_t waves = (pixels + 15) / 16;
-  for (int nco : {8, 6, 4, 3, 2})
-    if (nb % nco == 0 && waves * (nb / nco) >= 4096) return nco;
-  return 1;
+  const int64_t groups = (pixels + 15) / 16;
+  int best_nco = 1, best_pm = 1;
+  double best = 2.0;
+  for (int nco : {8, 6, 4, 3, 2, 1}) {
+    if (nb % nco) continue;
+    for (int pm : {4, 2, 1}) {
+      if (nco * pm > 8) continue;
+      const int64_t waves = (groups + pm - 1) / pm * (nb / nco);
+      const double loads = 1.0 / nco + 1.0 / pm;
+      if (waves >= 2048 && loads < best - 1e-9) {
+        best = loads;
+        best_nco = nco;
+        best_pm = pm;
+      }
+    }
+  }
+  *nco_out = best_nco;
+  *pm_out = best_pm;
+}
+
+template <int NCO>
+static void conv_launch_pm(int pm, dim3 gridbase, int npix, const bf16* in, int cin, const bf16* w,
+                           const float* bias, bf16* out, int cout, int nx, int ny, int relu, hipStream_t s) {
+  dim3 grid((unsigned)((npix + 64 * pm - 1) / (64 * pm)), gridbase.y, gridbase.z);
+  switch (pm) {
+    case 1: hipLaunchKernelGGL((k_conv3x3<NCO, 1>), grid, dim3(256), 0, s, in, cin, w, bias, out, nx, ny, relu, cout); break;
+    case 2: hipLaunchKernelGGL((k_conv3x3<NCO, 2>), grid, dim3(256), 0, s, in, cin, w, bias, out, nx, ny, relu, cout); break;
+    case 4: hipLaunchKernelGGL((k_conv3x3<NCO, 4>), grid, dim3(256), 0, s, in, cin, w, bias, out, nx, ny, relu, cout); break;
+    default: throw std::runtime_error("unsupported pixel grouping");
+  }
 }
 
 static void conv_launch(const bf16* in, int cin, const bf16* w, const float* bias, bf16* out, int cout, int nimg, int nx,
                         int ny, int relu, hipStream_t s) {
-  if (cout % 16) throw std::runtime_error("unsupported channel count");
-  const int nco = conv_nco(cout, (int64_t)nimg * nx * ny);
-  dim3 grid((unsigned)((nx * ny + 63) / 64), nimg, (unsigned)(cout / 16 / nco));
+  if (cout % 16 || cin % 16 || cin <= 0) throw std::runtime_error("unsupported channel count");
+  int nco, pm;
+  conv_shape(cout, (int64_t)nimg * nx * ny, &nco, &pm);
+  const dim3 gb(1, nimg, (unsigned)(cout / 16 / nco));
+  const int npix = nx * ny;
   switch (nco) {
-    case 1: hipLaunchKernelGGL(k_conv3x3<1>, grid, dim3(256), 0, s, in, cin, w, bias, out, nx, ny, relu, cout); break;
-    case 2: hipLaunchKernelGGL(k_conv3x3<2>, grid, dim3(256), 0, s, in, cin, w, bias, out, nx, ny, relu, cout); break;
-    case 3: hipLaunchKernelGGL(k_conv3x3<3>, grid, dim3(256), 0, s, in, cin, w, bias, out, nx, ny, relu, cout); break;
-    case 4: hipLaunchKernelGGL(k_conv3x3<4>, grid, dim3(256), 0, s, in, cin, w, bias, out, nx, ny, relu, cout); break;
-    case 6: hipLaunchKernelGGL(k_conv3x3<6>, grid, dim3(256), 0, s, in, cin, w, bias, out, nx, ny, relu, cout); break;
-    case 8: hipLaunchKernelGGL(k_conv3x3<8>, grid, dim3(256), 0, s, in, cin, w, bias, out, nx, ny, relu, cout); break;
+    case 1: conv_launch_pm<1>(pm, gb, npix, in, cin, w, bias, out, cout, nx, ny, relu, s); break;
+    case 2: conv_launch_pm<2>(pm, gb, npix, in, cin, w, bias, out, cout, nx, ny, relu, s); break;
+    case 3: conv_launch_pm<3>(pm, gb, npix, in, cin, w, bias, out, cout, nx, ny, relu, s); break;
+    case 4: conv_launch_pm<4>(pm, gb, npix, in, cin, w, bias, out, cout, nx, ny, relu, s); break;
+    case 6: conv_launch_pm<6>(pm, gb, npix, in, cin, w, bias, out, cout, nx, ny, relu, s); break;
+    case 8: conv_launch_pm<8>(pm, gb, npix, in, cin, w, bias, out, cout, nx, ny, relu, s); break;
     default: throw std::runtime_error("unsupported channel count");
   }
 }
