@@ -732,10 +732,13 @@ def test_c64_basis_reaches_fp64_residual():
     assert err <= 1e-5
 
 
-@pytest.mark.parametrize("nodes,levels,cycle", [((129, 65), 4, "W"), ((257, 129), 5, "V"), ((33, 17, 17), 3, "W")])
+@pytest.mark.parametrize("nodes,levels,cycle", [((129, 65), 4, "W"), ((257, 129), 5, "V"), ((33, 17, 17), 3, "W"),
+                                                 ((289, 17, 9), 3, "W")])
 def test_c64_basis_batched(nodes, levels, cycle):
     """Several RHS per call, 2-D and 3-D (w = H z formed inside the Arnoldi
-    passes on both), complex64 basis against the complex128 basis."""
+    passes on both; in 3-D marching z over in-plane segments, with windows
+    of 1024 values for n1 <= 256 and of 2048 for 289 x 17 x 9), complex64
+    basis against the complex128 basis."""
     M, m, g, w = small_case(nodes, seed=41, neumann=True)
     H = hr.get_helmholtz_operator7(M, m, w, g, True, True, 2)
     rng = np.random.default_rng(42)
