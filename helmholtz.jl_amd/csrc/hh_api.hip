@@ -890,25 +890,28 @@ struct Cycle {
     const int ncorr = ncorr_at(l);
     const bool up3 = use_up3(L, l), mid3 = use_mid3(L, l);
     bool have_r = down3;  // L.r holds the residual of the current iterate
+    bool have_bc = false;  // C.b holds its restriction
     for (int c = 0; c < ncorr; ++c) {
-      if (!have_r)
-        launch_stencil<T, T, T, M_RESID>(L.g, nrhs, (const cx<T>*)L.cur, b, (cx<T>*)L.r, (const cx<T>*)L.d, nullptr,
-                                         mk<double>(0, 0), 0.0, skip, st, rows_of(L));
-      have_r = false;
-      launch_restrict<T>(L.g, C.g, nrhs, (const cx<T>*)L.r, (cx<T>*)C.b, skip, st);
+      if (!have_bc) {
+        if (!have_r)
+          launch_stencil<T, T, T, M_RESID>(L.g, nrhs, (const cx<T>*)L.cur, b, (cx<T>*)L.r, (const cx<T>*)L.d, nullptr,
+                                           mk<double>(0, 0), 0.0, skip, st, rows_of(L));
+        launch_restrict<T>(L.g, C.g, nrhs, (const cx<T>*)L.r, (cx<T>*)C.b, skip, st);
+      }
+      have_r = have_bc = false;
       C.cur = nullptr;
       run(l + 1, (const cx<T>*)C.b, true, nullptr);
       if (up3 && c + 1 == ncorr) break;  // the last correction is added by the fused up leg
-      if (mid3 && c + 1 < ncorr) {       // correction and the next residual in one pass
+      if (mid3 && c + 1 < ncorr) {       // correction, next residual and its restriction in one pass
         void* o = other(L);
         const bool tm = S->kt.match(KT_MID, l);
         if (tm) S->kt.begin(st);
         launch_mid_fused3(L.g, C.g, nrhs, (const cx<float>*)L.cur, (const cx<float>*)C.cur, (const cx<float>*)b,
-                          (const cx<float>*)L.d, (cx<float>*)o, (cx<float>*)L.r, skip, st);
-        // x and b in, x' and r out, the coarse correction; the diagonal shared
-        if (tm) S->kt.end(st, act3 * (4 * Nf3 * s3 + (double)C.g.N * s3) + Nf3 * s3);
+                          (const cx<float>*)L.d, (cx<float>*)o, (cx<float>*)C.b, skip, st);
+        // x and b in, x' out, the coarse correction in and b_c out; the diagonal shared
+        if (tm) S->kt.end(st, act3 * (3 * Nf3 * s3 + 2 * (double)C.g.N * s3) + Nf3 * s3);
         L.cur = o;
-        have_r = true;
+        have_bc = true;
       } else {
         launch_prolong_add<T>(L.g, C.g, nrhs, (const cx<T>*)C.cur, (cx<T>*)L.cur, skip, st);
       }

@@ -14,9 +14,10 @@
 // (unfused: the prolongation and two sweeps, three passes).  On W levels the
 // middle step between the two coarse corrections,
 //
-//   x' = x + P e_c,  r = b - A x'
+//   x' = x + P e_c,  b_c = R (b - A x')
 //
-// is one pass too (unfused: the prolongation and the residual).  F = relax D^-1
+// is one pass too (unfused: the prolongation, the residual and the full
+// weighting).  F = relax D^-1
 // with D the operator's diagonal, so a sweep is (1 - relax) x + F (b + ih1 sx
 // + ih2 sy + ih3 sz) with s the neighbour sums along each axis and reads no
 // diagonal; the residual keeps the operator's form b - A x (a split residual
@@ -24,8 +25,7 @@
 //
 // A block owns a 28 x 28 column of nodes (a 32 x 32 in-plane tile with a
 // 2-node halo, one position per thread of 1024: a wavefront covers two
-// 32-node tile rows; the middle step, one stencil stage, a 1-node halo and
-// 30 x 30 nodes) and marches it along z over a run of output planes.  (A
+// 32-node tile rows) and marches it along z over a run of output planes.  (A
 // 64-wide tile wastes a quarter of its lanes on 193-wide planes; 512 threads
 // with two positions each need 92-98 VGPRs, 16 resident wavefronts per CU
 // against 32 at 55-60.)  Step z forms the first stage on plane z, the second
@@ -36,15 +36,16 @@
 // earlier stages on the planes before z0 (recomputed, not exchanged), so
 // runs are independent.  Ghost planes
 // and in-plane ghosts mirror (index -1 = 1, n = n - 2): a ghost node's value
-// is its mirror image's at every stage, so every formula is the unfused one.
+// is its mirror image's at every stage (up to the rounding of its neighbour
+// sums, taken in the other order), so every formula is the unfused one.
 #include "hh_kernels.h"
 
 namespace hh {
 
 namespace {
 constexpr int M3_W = 32, M3_RY = 32, M3_NT = 1024, M3_NP = M3_W * M3_RY;
-// owned columns / rows of a tile with an H-node halo (H = 2 for the legs
-// with two stencil stages, 1 for the middle step with one)
+// owned columns / rows of a tile with an H-node halo (every leg uses H = 2:
+// two stencil stages, or one stage and the full weighting)
 constexpr int m3_tx(int H) { return M3_W - 2 * H; }
 constexpr int m3_ty(int H) { return M3_RY - 2 * H; }
 // NT threads: NT / 32 tile rows per pass, KR rows per thread
@@ -335,89 +336,103 @@ __global__ __launch_bounds__(NT) void k_up_fused3(TileMap tm, int ytiles, int zr
 }
 
 // Middle step of a W visit (the first coarse correction added, then the
-// residual for the second).  Step z: x' = x + P e_c on plane z (every tile
-// position; owned nodes of the run's planes stored), r = b - A x' on plane
-// z - 1 (owned nodes, run's planes).  One stencil stage: a 1-node halo (30 x
-// 30 owned nodes per tile).  x of plane z and b, d of plane z - 1 are loaded
-// a step ahead.
+// residual for the second and its full weighting).  Step z: x' = x + P e_c
+// on plane z (every tile position; owned nodes of the run's planes stored),
+// r = b - A x' on plane z - 1 (the tile less its outer ring, kept in LDS),
+// and the in-plane 9-point weighted sum of plane z - 2's r at the tile's
+// coarse nodes, each coarse plane K summed from its fine planes 2K - 1, 2K,
+// 2K + 1 in registers and stored.  The tile's 28 x 28 owned fine nodes start
+// at an even corner, so it owns 14 x 14 coarse nodes whose weights reach one
+// fine node below the owned range (inside the 2-node halo); a run of fine
+// planes [z0, z1) (z0 even) owns the coarse planes K with 2K in [z0, z1).
+// The sums are formed in the order of the separate restriction kernel; at
+// the grid's edges the residual one node beyond the edge is the tile's own
+// value at that mirror position (the mirror node's value up to rounding, the
+// stencil's neighbours being summed in the other order).
 template <typename T, int NT>
 __global__ __launch_bounds__(NT) void k_mid_fused3(TileMap tm, int ytiles, int zrun, Grid gf, Grid gc,
                                                     const cx<T>* __restrict__ x, const cx<T>* __restrict__ ec,
                                                     const cx<T>* __restrict__ b, const cx<T>* __restrict__ d,
-                                                    cx<T>* __restrict__ xo, cx<T>* __restrict__ ro,
+                                                    cx<T>* __restrict__ xo, cx<T>* __restrict__ bco,
                                                     const int* __restrict__ skip) {
-  constexpr int KR = Geo3<NT>::KR, W = M3_W, H = 1;
-  __shared__ cx<T> ra[2][M3_NP];
+  constexpr int KR = Geo3<NT>::KR, W = M3_W, H = 2;
+  static_assert(KR == 1, "one tile position per thread");
+  __shared__ cx<T> ra[2][M3_NP], rr[2][M3_NP];
   March3<KR> m;
   if (!march3_setup<NT, H>(tm, ytiles, zrun, gf, skip, m)) return;
   const int n1 = gf.n1, n3 = gf.n3;
+  const int zb = min(n3, m.z1 + 1);  // x' on [za, zb): r on [z0 - 1, z1) needs one plane each side
   const int64_t plane = (int64_t)n1 * gf.n2, off = (int64_t)m.rhs * gf.N;
   const cx<T>* __restrict__ xr = x + off;
   const cx<T>* __restrict__ br = b + off;
   const cx<T>* __restrict__ cr = ec + (int64_t)m.rhs * gc.N;
   cx<T>* __restrict__ yr = xo + off;
-  cx<T>* __restrict__ rr = ro + off;
+  cx<T>* __restrict__ bc = bco + (int64_t)m.rhs * gc.N;
   const T ih1 = (T)gf.ih1, ih2 = (T)gf.ih2, ih3 = (T)gf.ih3;
-  cx<T> cK[KR];
-#pragma unroll
-  for (int k = 0; k < KR; ++k) cK[k] = coarse_at(cr, gc, m.za >> 1, m.gi, m.gj[k]);
-  // x of plane z (xv), b and d of plane z - 1 (bp, dp); x' of planes z, z - 1, z - 2
-  cx<T> xv[KR], bp[KR], dp[KR], xa0[KR], xa1[KR], xa2[KR];
-  cx<T> xn[KR], bn[KR], dn[KR];  // the next step's loads
-  auto load_step = [&](int zz) {  // x of plane zz, b and d of plane zz - 1 (clamped: always in bounds)
+  const int lane = threadIdx.x & (W - 1), lj = m.lj[0], qq = lj * W + lane;
+  // this thread's coarse node, if any: fine tile position (H + 2 ci, H + 2 cj)
+  const int ci = (lane - H) >> 1, cj = (lj - H) >> 1;
+  const int I = (m.fx0 >> 1) + ci, J = (m.fy0 >> 1) + cj;
+  const bool cown = lane >= H && lane < H + m3_tx(H) && !((lane - H) & 1) && lj >= H && lj < H + m3_ty(H) &&
+                    !((lj - H) & 1) && I < gc.n1 && J < gc.n2;
+  const int64_t cplane = (int64_t)gc.n1 * gc.n2, ccol = (int64_t)J * gc.n1 + I;
+  const T w[3] = {(T)0.25, (T)0.5, (T)0.25};
+  cx<T> cK = coarse_at(cr, gc, m.za >> 1, m.gi, m.gj[0]);
+  // x' of planes z, z - 1, z - 2; in-plane sums of r on planes q - 1, q - 2
+  cx<T> xa0 = mk<T>(0, 0), xa1 = xa0, xa2 = xa0, s1 = xa0, s2 = xa0;
+  cx<T> xn, bn, dn;  // the next step's loads: x of plane zz, b and d of plane zz - 1
+  auto load_step = [&](int zz) {  // clamped plane indices: always in bounds
     const int64_t pz = (int64_t)min(zz, n3 - 1) * plane, pb = (int64_t)max(min(zz, n3) - 1, 0) * plane;
-#pragma unroll
-    for (int k = 0; k < KR; ++k) {
-      xn[k] = xr[pz + m.po[k]];
-      bn[k] = br[pb + m.po[k]];
-      dn[k] = d[pb + m.po[k]];
-    }
+    xn = xr[pz + m.po[0]];
+    bn = br[pb + m.po[0]];
+    dn = d[pb + m.po[0]];
   };
   load_step(m.za);
-  for (int z = m.za; z <= m.z1; ++z) {
-#pragma unroll
-    for (int k = 0; k < KR; ++k) {
-      xv[k] = xn[k];
-      bp[k] = bn[k];
-      dp[k] = dn[k];
-    }
-    if (z < m.z1) load_step(z + 1);
-    if (z < m.zb) {  // x' = x + P e_c on plane z
-      const bool store = z >= m.z0 && z < m.z1;
-#pragma unroll
-      for (int k = 0; k < KR; ++k) {
-        cx<T> e;
-        if (z & 1) {
-          const cx<T> cN = coarse_at(cr, gc, (z >> 1) + 1, m.gi, m.gj[k]);
-          e = (T)0.5 * (cK[k] + cN);
-          cK[k] = cN;
-        } else {
-          e = cK[k];
-        }
-        xa0[k] = xv[k] + e;
-        ra[z & 1][m.lj[k] * W + (threadIdx.x & (W - 1))] = xa0[k];
-        if (store && m.own[k]) yr[(int64_t)z * plane + own_off<H>(m, k, n1)] = xa0[k];
+  for (int z = m.za; z <= m.z1 + 1; ++z) {
+    const cx<T> xv = xn, bp = bn, dp = dn;
+    if (z < m.z1 + 1) load_step(z + 1);
+    if (z < zb) {  // x' = x + P e_c on plane z
+      cx<T> e;
+      if (z & 1) {
+        const cx<T> cN = coarse_at(cr, gc, (z >> 1) + 1, m.gi, m.gj[0]);
+        e = (T)0.5 * (cK + cN);
+        cK = cN;
+      } else {
+        e = cK;
       }
+      xa0 = xv + e;
+      ra[z & 1][qq] = xa0;
+      if (z >= m.z0 && z < m.z1 && m.own[0]) yr[(int64_t)z * plane + own_off<H>(m, 0, n1)] = xa0;
     }
     const int p = z - 1;
-    if (p >= m.z0 && p < m.z1) {  // r = b - A x' on plane p, owned nodes
+    if (p >= max(0, m.z0 - 1) && p < m.z1 && m.in1[0]) {  // r = b - A x' on plane p
       const cx<T>* a = ra[p & 1];
       const bool lo = p == 0, hi = p == n3 - 1;
+      const cx<T> xm = lo ? xa0 : xa2, xp = hi ? xa2 : xa0;
+      rr[p & 1][qq] = bp - apply7(xa1, a[qq - 1], a[qq + 1], a[qq - W], a[qq + W], xm, xp, ih1, ih2, ih3, dp);
+    }
+    const int q = z - 2;
+    if (q >= max(0, m.z0 - 1) && q < m.z1 && cown) {  // full weighting of plane q's r
+      const cx<T>* rq = rr[q & 1];
+      cx<T> sq = mk<T>(0, 0);
 #pragma unroll
-      for (int k = 0; k < KR; ++k) {
-        if (m.own[k]) {
-          const int qq = m.lj[k] * W + (threadIdx.x & (W - 1));
-          const cx<T> xm = lo ? xa0[k] : xa2[k], xp = hi ? xa2[k] : xa0[k];
-          rr[(int64_t)p * plane + own_off<H>(m, k, n1)] =
-              bp[k] - apply7(xa1[k], a[qq - 1], a[qq + 1], a[qq - W], a[qq + W], xm, xp, ih1, ih2, ih3, dp[k]);
+      for (int bb = 0; bb < 3; ++bb)
+#pragma unroll
+        for (int aa = 0; aa < 3; ++aa) sq = sq + (w[bb] * w[aa]) * rq[(lj - 1 + bb) * W + (lane - 1 + aa)];
+      if (q & 1) {  // coarse plane K = (q - 1) / 2 is complete
+        const int K = (q - 1) >> 1;
+        if (2 * K >= m.z0) {
+          const cx<T> sm = K == 0 ? sq : s2;  // plane -1 mirrors plane 1
+          bc[(int64_t)K * cplane + ccol] = w[0] * sm + w[1] * s1 + w[2] * sq;
         }
+      } else if (q == n3 - 1) {  // the last coarse plane: plane n3 mirrors n3 - 2
+        bc[(int64_t)(q >> 1) * cplane + ccol] = w[0] * s1 + w[1] * sq + w[2] * s1;
       }
+      s2 = s1;
+      s1 = sq;
     }
-#pragma unroll
-    for (int k = 0; k < KR; ++k) {
-      xa2[k] = xa1[k];
-      xa1[k] = xa0[k];
-    }
+    xa2 = xa1;
+    xa1 = xa0;
     __syncthreads();
   }
 }
@@ -456,11 +471,11 @@ void launch_down_fused3(const Grid& gf, int nrhs, const cx<float>* b, const cx<f
 }
 
 void launch_mid_fused3(const Grid& gf, const Grid& gc, int nrhs, const cx<float>* x, const cx<float>* ec,
-                       const cx<float>* b, const cx<float>* d, cx<float>* xo, cx<float>* ro, const int* skip,
+                       const cx<float>* b, const cx<float>* d, cx<float>* xo, cx<float>* bco, const int* skip,
                        hipStream_t s) {
-  const Runs3 r = runs3(gf, nrhs, 1);
+  const Runs3 r = runs3(gf, nrhs, 2);
   hipLaunchKernelGGL((k_mid_fused3<float, M3_NT>), dim3(r.m.T), dim3(M3_NT), 0, s, r.m, r.ytiles, r.zrun, gf, gc, x,
-                     ec, b, d, xo, ro, skip);
+                     ec, b, d, xo, bco, skip);
 }
 
 void launch_up_fused3(const Grid& gf, const Grid& gc, int nrhs, const cx<float>* x, const cx<float>* ec,
