@@ -443,7 +443,11 @@ bool fused3_ok(const Grid& gf) {
 
 namespace {
 // z-runs: at most 32 output planes each, and more (down to 8 planes) while a
-// launch has fewer than 2048 blocks; runs recompute two planes on each side
+// launch has fewer than 2048 blocks; runs recompute two planes on each side.
+// A level too small to give even 256 blocks that way (one per CU) takes runs
+// down to 4 planes: its launches are bound by the latency of their steps,
+// not by the recomputed planes (25 x 25 x 13, 32 RHS: 21 -> 12 us per down
+// leg; the same on 49 x 49 x 25, 384 blocks, was slower).
 struct Runs3 {
   int ytiles, zrun;
   TileMap m;
@@ -454,7 +458,8 @@ Runs3 runs3(const Grid& g, int nrhs, int H) {
   r.ytiles = (g.n2 + m3_ty(H) - 1) / m3_ty(H);
   const int64_t per_run = (int64_t)xtiles * r.ytiles * nrhs;
   int runs = (g.n3 + 31) / 32;
-  const int want = (int)std::min<int64_t>((2048 + per_run - 1) / per_run, std::max(1, g.n3 / 8));
+  int want = (int)std::min<int64_t>((2048 + per_run - 1) / per_run, std::max(1, g.n3 / 8));
+  if (per_run * want < 256) want = (int)std::min<int64_t>((256 + per_run - 1) / per_run, (g.n3 + 3) / 4);
   runs = std::max(runs, want);
   r.zrun = (g.n3 + runs - 1) / runs;
   r.zrun += r.zrun & 1;
