@@ -598,12 +598,13 @@ def test_adjoint_solve_3d():
     assert np.linalg.norm(H.conj().T @ x - b) / np.linalg.norm(b) <= 1e-6
 
 
-@pytest.mark.parametrize("nodes", [(17, 9), (33, 17), (65, 33)])
+@pytest.mark.parametrize("nodes", [(17, 9), (33, 17), (65, 33), (13, 13, 7)])
 def test_mfma_coarsest_apply_matches_double(nodes):
     """The complex64 coarsest solve runs on MFMA (fp32 real-ified inverse):
     with a single level the preconditioner is that solve alone, and it must
     equal the complex128 dense inverse to fp32 accuracy, and the exact solve
-    of the shifted operator (SciPy) likewise.  20 RHS: a partial 16-column tile."""
+    of the shifted operator (SciPy) likewise.  20 RHS: a partial 16-column tile.
+    13 x 13 x 7 is the coarsest level of the 193 x 193 x 97 hierarchy."""
     M, m, g, w = small_case(nodes, seed=37, neumann=True)
     rng = np.random.default_rng(13)
     V = np.asfortranarray(rng.standard_normal((m.size, 20)) + 1j * rng.standard_normal((m.size, 20)))
