@@ -444,7 +444,8 @@ bool fused3_ok(const Grid& gf) {
 namespace {
 // z-runs: at most 48 output planes each (193 x 193 x 97 at 32 RHS: three
 // runs of 34, legs 1-2% faster than four of 26), and more (down to 8 planes)
-// while a launch has fewer than 2048 blocks; runs recompute two planes on
+// while a launch has fewer than 1024 blocks (97 x 97 x 49 at 32 RHS: two runs
+// of 26, legs 5-8% faster than four of 14); runs recompute two planes on
 // each side.
 // A level too small to give even 256 blocks that way (one per CU) takes runs
 // down to 4 planes: its launches are bound by the latency of their steps,
@@ -460,7 +461,7 @@ Runs3 runs3(const Grid& g, int nrhs, int H) {
   r.ytiles = (g.n2 + m3_ty(H) - 1) / m3_ty(H);
   const int64_t per_run = (int64_t)xtiles * r.ytiles * nrhs;
   int runs = (g.n3 + 47) / 48;
-  int want = (int)std::min<int64_t>((2048 + per_run - 1) / per_run, std::max(1, g.n3 / 8));
+  int want = (int)std::min<int64_t>((1024 + per_run - 1) / per_run, std::max(1, g.n3 / 8));
   if (per_run * want < 256) want = (int)std::min<int64_t>((256 + per_run - 1) / per_run, (g.n3 + 3) / 4);
   runs = std::max(runs, want);
   r.zrun = (g.n3 + runs - 1) / runs;
