@@ -843,17 +843,27 @@ struct Cycle {
       L.cur = o;
       return;
     }
-    // the fused 3-D down leg leaves this level's residual in L.r
+    // the fused 3-D down leg leaves this level's residual in L.r, or (down3r)
+    // its restriction in the coarse level's right-hand side
     const bool down3 = zero_init && use_down3(L, l);
+    const bool down3r = down3 && down3r_ok(L.g);
     // kernel timing of the fused 3-D legs: algorithmic bytes as for the 2-D legs
     const double s3 = sizeof(cx<float>), Nf3 = (double)L.g.N, act3 = S->kt_active;
     if (down3) {
       L.cur = first ? first : L.xa;
       const bool td = S->kt.match(KT_DOWN, l);
       if (td) S->kt.begin(st);
-      launch_down_fused3(L.g, nrhs, (const cx<float>*)b, (const cx<float>*)L.d, (const cx<float>*)L.f, S->p.relax,
-                         (cx<float>*)L.cur, (cx<float>*)L.r, skip, st);
-      if (td) S->kt.end(st, act3 * 3 * Nf3 * s3 + 2 * Nf3 * s3);  // b in, x2 and r out; d and F shared
+      if (down3r) {
+        Level& C = S->lv[l + 1];
+        launch_down_fused3r(L.g, C.g, nrhs, (const cx<float>*)b, (const cx<float>*)L.d, (const cx<float>*)L.f,
+                            S->p.relax, (cx<float>*)L.cur, (cx<float>*)C.b, skip, st);
+        // b in, x2 and b_c out; d and F shared
+        if (td) S->kt.end(st, act3 * (2 * Nf3 * s3 + (double)C.g.N * s3) + 2 * Nf3 * s3);
+      } else {
+        launch_down_fused3(L.g, nrhs, (const cx<float>*)b, (const cx<float>*)L.d, (const cx<float>*)L.f, S->p.relax,
+                           (cx<float>*)L.cur, (cx<float>*)L.r, skip, st);
+        if (td) S->kt.end(st, act3 * 3 * Nf3 * s3 + 2 * Nf3 * s3);  // b in, x2 and r out; d and F shared
+      }
     } else if (zero_init) {
       L.cur = first ? first : L.xa;
       if (pre > 0 && S->p.relax_type == HH_RELAX_JAC_GMRES) {
@@ -889,8 +899,8 @@ struct Cycle {
     // instead diverges with rediscretised coarse operators near kh ~ 2).
     const int ncorr = ncorr_at(l);
     const bool up3 = use_up3(L, l), mid3 = use_mid3(L, l);
-    bool have_r = down3;  // L.r holds the residual of the current iterate
-    bool have_bc = false;  // C.b holds its restriction
+    bool have_r = down3 && !down3r;  // L.r holds the residual of the current iterate
+    bool have_bc = down3r;            // C.b holds its restriction
     for (int c = 0; c < ncorr; ++c) {
       if (!have_bc) {
         if (!have_r)

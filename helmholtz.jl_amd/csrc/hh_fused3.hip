@@ -17,7 +17,9 @@
 //   x' = x + P e_c,  b_c = R (b - A x')
 //
 // is one pass too (unfused: the prolongation, the residual and the full
-// weighting).  F = relax D^-1
+// weighting).  On levels where it costs no extra tiles the down leg forms
+// the full weighting of its residual as well (b_c = R r, r never stored).
+// F = relax D^-1
 // with D the operator's diagonal, so a sweep is (1 - relax) x + F (b + ih1 sx
 // + ih2 sy + ih3 sz) with s the neighbour sums along each axis and reads no
 // diagonal; the residual keeps the operator's form b - A x (a split residual
@@ -227,6 +229,112 @@ __global__ __launch_bounds__(NT) void k_down_fused3(TileMap tm, int ytiles, int 
       sa2[k] = sa1[k];
       sa1[k] = sa0[k];
     }
+    __syncthreads();
+  }
+}
+
+// Down leg with the full weighting of its residual (b_c = R r written, r
+// itself never stored).  Step z: x1 on plane z, x2 on plane z - 1 (stored for
+// the run's own planes), r on plane z - 2 over the tile less its two outer
+// rings (kept in LDS), and the in-plane 9-point sums of plane z - 3's r at
+// the tile's coarse nodes, each coarse plane K summed from its fine planes
+// 2K - 1, 2K, 2K + 1 in registers and stored, as in the middle step.  The
+// weights reach one fine node past the owned range, where r needs a 3-node
+// halo: the tile owns 26 x 26 fine nodes (13 x 13 coarse nodes, starting at
+// an even corner) and a run recomputes three planes on each side.  Used on
+// levels where 26-wide tiles are no more numerous than 28-wide ones.
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void k_down_fused3r(TileMap tm, int ytiles, int zrun, Grid gf, Grid gc,
+                                                      const cx<T>* __restrict__ b, const cx<T>* __restrict__ d,
+                                                      const cx<T>* __restrict__ f, cx<T>* __restrict__ xo,
+                                                      cx<T>* __restrict__ bco, const int* __restrict__ skip, T omr) {
+  constexpr int KR = Geo3<NT>::KR, W = M3_W, H = 3;
+  static_assert(KR == 1, "one tile position per thread");
+  __shared__ cx<T> ra[2][M3_NP], rs[2][M3_NP], rr[2][M3_NP];
+  March3<KR> m;
+  if (!march3_setup<NT, H>(tm, ytiles, zrun, gf, skip, m)) return;
+  const int n1 = gf.n1, n3 = gf.n3;
+  const int64_t plane = (int64_t)n1 * gf.n2, off = (int64_t)m.rhs * gf.N;
+  const cx<T>* __restrict__ br = b + off;
+  cx<T>* __restrict__ xr = xo + off;
+  cx<T>* __restrict__ bc = bco + (int64_t)m.rhs * gc.N;
+  const T ih1 = (T)gf.ih1, ih2 = (T)gf.ih2, ih3 = (T)gf.ih3;
+  const int lane = threadIdx.x & (W - 1), lj = m.lj[0], qq = lj * W + lane;
+  const bool in2 = lane >= 2 && lane < W - 2 && lj >= 2 && lj < M3_RY - 2;  // r's positions
+  // this thread's coarse node, if any: fine tile position (H + 2 ci, H + 2 cj)
+  const int ci = (lane - H) >> 1, cj = (lj - H) >> 1;
+  const int I = (m.fx0 >> 1) + ci, J = (m.fy0 >> 1) + cj;
+  const bool cown = lane >= H && lane < H + m3_tx(H) && !((lane - H) & 1) && lj >= H && lj < H + m3_ty(H) &&
+                    !((lj - H) & 1) && I < gc.n1 && J < gc.n2;
+  const int64_t cplane = (int64_t)gc.n1 * gc.n2, ccol = (int64_t)J * gc.n1 + I;
+  const T w[3] = {(T)0.25, (T)0.5, (T)0.25};
+  // planes of each stage: x2 on [p0, p1), r on [r0, r1) (the coarse planes of
+  // the run need r one plane beyond it on each side)
+  const int p0 = max(0, m.z0 - 2), p1 = min(n3, m.z1 + 2), r0 = max(0, m.z0 - 1), r1 = min(n3, m.z1 + 1);
+  // b, F of plane z (b0, f0) and z - 1 (b1, f1), b of z - 2 (b2); x1 of
+  // planes z, z - 1, z - 2; x2 of planes z - 1, z - 2, z - 3; in-plane sums
+  // of r on planes q - 1, q - 2
+  cx<T> b0, f0, b1 = mk<T>(0, 0), f1 = b1, b2 = b1, dq;
+  cx<T> xa0 = b1, xa1 = b1, xa2 = b1, sa0 = b1, sa1 = b1, sa2 = b1, s1 = b1, s2 = b1;
+  cx<T> bn, fn;  // the next step's loads
+  auto load_step = [&](int zz) {  // b, F of plane zz (clamped: always in bounds)
+    const int64_t pz = (int64_t)min(zz, n3 - 1) * plane;
+    bn = br[pz + m.po[0]];
+    fn = f[pz + m.po[0]];
+  };
+  load_step(m.za);
+  for (int z = m.za; z <= r1 + 2; ++z) {
+    dq = d[(int64_t)min(max(z - 2, 0), n3 - 1) * plane + m.po[0]];  // the diagonal of plane z - 2
+    b0 = bn;
+    f0 = fn;
+    if (z < r1 + 2) load_step(z + 1);
+    if (z < m.zb) {  // x1 = F b on plane z
+      xa0 = cmul_fma(f0, b0);
+      ra[z & 1][qq] = xa0;
+    }
+    const int p = z - 1;
+    if (p >= p0 && p < p1 && m.in1[0]) {  // x2 on plane p
+      const cx<T>* a = ra[p & 1];
+      const bool lo = p == 0, hi = p == n3 - 1;
+      const cx<T> xm = lo ? xa0 : xa2, xp = hi ? xa2 : xa0;
+      sa0 = jac7(xa1, a[qq - 1], a[qq + 1], a[qq - W], a[qq + W], xm, xp, b1, f1, ih1, ih2, ih3, omr);
+      rs[p & 1][qq] = sa0;
+      if (p >= m.z0 && p < m.z1 && m.own[0]) xr[(int64_t)p * plane + own_off<H>(m, 0, n1)] = sa0;
+    }
+    const int r = z - 2;
+    if (r >= r0 && r < r1 && in2) {  // r = b - A x2 on plane r
+      const cx<T>* sx = rs[r & 1];
+      const bool lo = r == 0, hi = r == n3 - 1;
+      const cx<T> sm = lo ? sa0 : sa2, sp = hi ? sa2 : sa0;
+      rr[r & 1][qq] = b2 - apply7(sa1, sx[qq - 1], sx[qq + 1], sx[qq - W], sx[qq + W], sm, sp, ih1, ih2, ih3, dq);
+    }
+    const int q = z - 3;
+    if (q >= r0 && q < r1 && cown) {  // full weighting of plane q's r
+      const cx<T>* rq = rr[q & 1];
+      cx<T> sq = mk<T>(0, 0);
+#pragma unroll
+      for (int bb = 0; bb < 3; ++bb)
+#pragma unroll
+        for (int aa = 0; aa < 3; ++aa) sq = sq + (w[bb] * w[aa]) * rq[(lj - 1 + bb) * W + (lane - 1 + aa)];
+      if (q & 1) {  // coarse plane K = (q - 1) / 2 is complete
+        const int K = (q - 1) >> 1;
+        if (2 * K >= m.z0) {
+          const cx<T> sm = K == 0 ? sq : s2;  // plane -1 mirrors plane 1
+          bc[(int64_t)K * cplane + ccol] = w[0] * sm + w[1] * s1 + w[2] * sq;
+        }
+      } else if (q == n3 - 1) {  // the last coarse plane: plane n3 mirrors n3 - 2
+        bc[(int64_t)(q >> 1) * cplane + ccol] = w[0] * s1 + w[1] * sq + w[2] * s1;
+      }
+      s2 = s1;
+      s1 = sq;
+    }
+    b2 = b1;
+    b1 = b0;
+    f1 = f0;
+    xa2 = xa1;
+    xa1 = xa0;
+    sa2 = sa1;
+    sa1 = sa0;
     __syncthreads();
   }
 }
@@ -476,6 +584,21 @@ void launch_down_fused3(const Grid& gf, int nrhs, const cx<float>* b, const cx<f
   const Runs3 r = runs3(gf, nrhs, 2);
   hipLaunchKernelGGL((k_down_fused3<float, M3_NT>), dim3(r.m.T), dim3(M3_NT), 0, s, r.m, r.ytiles, r.zrun, gf, b, d,
                      f, xo, ro, skip, (float)(1.0 - relax));
+}
+
+// the down leg with its restriction: levels whose 26-wide tiles (3-node
+// halo) are as many as 28-wide ones along both in-plane axes
+bool down3r_ok(const Grid& gf) {
+  return fused3_ok(gf) && (gf.n1 + m3_tx(3) - 1) / m3_tx(3) == (gf.n1 + m3_tx(2) - 1) / m3_tx(2) &&
+         (gf.n2 + m3_ty(3) - 1) / m3_ty(3) == (gf.n2 + m3_ty(2) - 1) / m3_ty(2);
+}
+
+void launch_down_fused3r(const Grid& gf, const Grid& gc, int nrhs, const cx<float>* b, const cx<float>* d,
+                         const cx<float>* f, double relax, cx<float>* xo, cx<float>* bco, const int* skip,
+                         hipStream_t s) {
+  const Runs3 r = runs3(gf, nrhs, 3);
+  hipLaunchKernelGGL((k_down_fused3r<float, M3_NT>), dim3(r.m.T), dim3(M3_NT), 0, s, r.m, r.ytiles, r.zrun, gf, gc,
+                     b, d, f, xo, bco, skip, (float)(1.0 - relax));
 }
 
 void launch_mid_fused3(const Grid& gf, const Grid& gc, int nrhs, const cx<float>* x, const cx<float>* ec,
