@@ -228,7 +228,7 @@ ROOFLINE_KERNELS = [
 
 
 ROOFLINE_KERNELS_3D = [
-    ("down", 0, "k_down_fused3<float,1024> on the 193x193x97 grid (two sweeps from zero, r = b - A x2; z-marching)"),
+    ("down", 0, "k_down_fused3<float,1024> on the 193x193x97 grid (two sweeps from zero, b_c = R (b - A x2); z-marching)"),
     ("mid", 0, "k_mid_fused3<float,1024> on the 193x193x97 grid (x += P e_c, b_c = R (b - A x); z-marching)"),
     ("up", 0, "k_up_fused3<float,1024> on the 193x193x97 grid (x += P e_c, two damped-Jacobi sweeps; z-marching)"),
 ]

@@ -843,27 +843,20 @@ struct Cycle {
       L.cur = o;
       return;
     }
-    // the fused 3-D down leg leaves this level's residual in L.r, or (down3r)
-    // its restriction in the coarse level's right-hand side
+    // the fused 3-D down leg leaves the restriction of this level's residual
+    // in the coarse level's right-hand side
     const bool down3 = zero_init && use_down3(L, l);
-    const bool down3r = down3 && down3r_ok(L.g);
     // kernel timing of the fused 3-D legs: algorithmic bytes as for the 2-D legs
     const double s3 = sizeof(cx<float>), Nf3 = (double)L.g.N, act3 = S->kt_active;
     if (down3) {
       L.cur = first ? first : L.xa;
+      Level& C = S->lv[l + 1];
       const bool td = S->kt.match(KT_DOWN, l);
       if (td) S->kt.begin(st);
-      if (down3r) {
-        Level& C = S->lv[l + 1];
-        launch_down_fused3r(L.g, C.g, nrhs, (const cx<float>*)b, (const cx<float>*)L.d, (const cx<float>*)L.f,
-                            S->p.relax, (cx<float>*)L.cur, (cx<float>*)C.b, skip, st);
-        // b in, x2 and b_c out; d and F shared
-        if (td) S->kt.end(st, act3 * (2 * Nf3 * s3 + (double)C.g.N * s3) + 2 * Nf3 * s3);
-      } else {
-        launch_down_fused3(L.g, nrhs, (const cx<float>*)b, (const cx<float>*)L.d, (const cx<float>*)L.f, S->p.relax,
-                           (cx<float>*)L.cur, (cx<float>*)L.r, skip, st);
-        if (td) S->kt.end(st, act3 * 3 * Nf3 * s3 + 2 * Nf3 * s3);  // b in, x2 and r out; d and F shared
-      }
+      launch_down_fused3(L.g, C.g, nrhs, (const cx<float>*)b, (const cx<float>*)L.d, (const cx<float>*)L.f,
+                         S->p.relax, (cx<float>*)L.cur, (cx<float>*)C.b, skip, st);
+      // b in, x2 and b_c out; d and F shared
+      if (td) S->kt.end(st, act3 * (2 * Nf3 * s3 + (double)C.g.N * s3) + 2 * Nf3 * s3);
     } else if (zero_init) {
       L.cur = first ? first : L.xa;
       if (pre > 0 && S->p.relax_type == HH_RELAX_JAC_GMRES) {
@@ -899,16 +892,14 @@ struct Cycle {
     // instead diverges with rediscretised coarse operators near kh ~ 2).
     const int ncorr = ncorr_at(l);
     const bool up3 = use_up3(L, l), mid3 = use_mid3(L, l);
-    bool have_r = down3 && !down3r;  // L.r holds the residual of the current iterate
-    bool have_bc = down3r;            // C.b holds its restriction
+    bool have_bc = down3;  // C.b holds the restriction of the current residual
     for (int c = 0; c < ncorr; ++c) {
       if (!have_bc) {
-        if (!have_r)
-          launch_stencil<T, T, T, M_RESID>(L.g, nrhs, (const cx<T>*)L.cur, b, (cx<T>*)L.r, (const cx<T>*)L.d, nullptr,
-                                           mk<double>(0, 0), 0.0, skip, st, rows_of(L));
+        launch_stencil<T, T, T, M_RESID>(L.g, nrhs, (const cx<T>*)L.cur, b, (cx<T>*)L.r, (const cx<T>*)L.d, nullptr,
+                                         mk<double>(0, 0), 0.0, skip, st, rows_of(L));
         launch_restrict<T>(L.g, C.g, nrhs, (const cx<T>*)L.r, (cx<T>*)C.b, skip, st);
       }
-      have_r = have_bc = false;
+      have_bc = false;
       C.cur = nullptr;
       run(l + 1, (const cx<T>*)C.b, true, nullptr);
       if (up3 && c + 1 == ncorr) break;  // the last correction is added by the fused up leg

@@ -1,12 +1,12 @@
 // Fused multigrid legs for 3-D levels with mirror ghosts (BC = 2), complex64.
 //
-// Down leg (pre-smoothing from zero, then the residual):
+// Down leg (pre-smoothing from zero, then the restricted residual):
 //
-//   x1 = F b,  x2 = x1 + F (b - A x1),  r = b - A x2
+//   x1 = F b,  x2 = x1 + F (b - A x1),  b_c = R (b - A x2)
 //
-// reads b, d, F once and writes x2 and r once (unfused: the zero-start sweep,
-// one sweep and the residual, three passes).  Up leg (last coarse correction,
-// then post-smoothing):
+// reads b, d, F once and writes x2 and b_c once (unfused: the zero-start
+// sweep, one sweep, the residual and the full weighting, four passes).  Up
+// leg (last coarse correction, then post-smoothing):
 //
 //   x' = x + P e_c, then two damped-Jacobi sweeps x <- x + F (b - A x)
 //
@@ -17,9 +17,7 @@
 //   x' = x + P e_c,  b_c = R (b - A x')
 //
 // is one pass too (unfused: the prolongation, the residual and the full
-// weighting).  On levels where it costs no extra tiles the down leg forms
-// the full weighting of its residual as well (b_c = R r, r never stored).
-// F = relax D^-1
+// weighting).  F = relax D^-1
 // with D the operator's diagonal, so a sweep is (1 - relax) x + F (b + ih1 sx
 // + ih2 sy + ih3 sz) with s the neighbour sums along each axis and reads no
 // diagonal; the residual keeps the operator's form b - A x (a split residual
@@ -46,8 +44,9 @@ namespace hh {
 
 namespace {
 constexpr int M3_W = 32, M3_RY = 32, M3_NT = 1024, M3_NP = M3_W * M3_RY;
-// owned columns / rows of a tile with an H-node halo (every leg uses H = 2:
-// two stencil stages, or one stage and the full weighting)
+// owned columns / rows of a tile with an H-node halo (H = 2: two stencil
+// stages, or one stage and the full weighting; the down leg's two stages and
+// full weighting need H = 3)
 constexpr int m3_tx(int H) { return M3_W - 2 * H; }
 constexpr int m3_ty(int H) { return M3_RY - 2 * H; }
 // NT threads: NT / 32 tile rows per pass, KR rows per thread
@@ -135,105 +134,7 @@ __device__ __forceinline__ cx<T> jac7(cx<T> xc, cx<T> xl, cx<T> xr, cx<T> xm, cx
 
 }  // namespace
 
-// Down leg.  Step z: x1 on plane z (pointwise), x2 on plane z - 1 (stored for
-// the run's own planes), r on plane z - 2 (owned nodes, run's own planes).
-// b and F of plane z are loaded a step ahead and kept two more steps in
-// registers (x2 needs them on plane z - 1, r needs b on plane z - 2); the
-// diagonal of plane z - 2 is loaded at the step's start.
-template <typename T, int NT>
-__global__ __launch_bounds__(NT) void k_down_fused3(TileMap tm, int ytiles, int zrun, Grid gf,
-                                                     const cx<T>* __restrict__ b, const cx<T>* __restrict__ d,
-                                                     const cx<T>* __restrict__ f, cx<T>* __restrict__ xo,
-                                                     cx<T>* __restrict__ ro, const int* __restrict__ skip, T omr) {
-  constexpr int KR = Geo3<NT>::KR, W = M3_W;
-  __shared__ cx<T> ra[2][M3_NP], rs[2][M3_NP];
-  March3<KR> m;
-  if (!march3_setup<NT, 2>(tm, ytiles, zrun, gf, skip, m)) return;
-  const int n1 = gf.n1, n3 = gf.n3;
-  const int64_t plane = (int64_t)n1 * gf.n2, off = (int64_t)m.rhs * gf.N;
-  const cx<T>* __restrict__ br = b + off;
-  cx<T>* __restrict__ xr = xo + off;
-  cx<T>* __restrict__ rr = ro + off;
-  const T ih1 = (T)gf.ih1, ih2 = (T)gf.ih2, ih3 = (T)gf.ih3;
-  // b, F of plane z (b0, f0) and z - 1 (b1, f1), b of z - 2 (b2); x1 of
-  // planes z, z - 1, z - 2; x2 of planes z - 1, z - 2, z - 3
-  cx<T> b0[KR], f0[KR], b1[KR], f1[KR], b2[KR], dq[KR];
-  cx<T> xa0[KR], xa1[KR], xa2[KR], sa0[KR], sa1[KR], sa2[KR];
-  cx<T> bn[KR], fn[KR];  // the next step's loads
-  auto load_step = [&](int zz) {  // b, F of plane zz (clamped: always in bounds)
-    const int64_t pz = (int64_t)min(zz, n3 - 1) * plane;
-#pragma unroll
-    for (int k = 0; k < KR; ++k) {
-      bn[k] = br[pz + m.po[k]];
-      fn[k] = f[pz + m.po[k]];
-    }
-  };
-  load_step(m.za);
-  for (int z = m.za; z <= m.z1 + 1; ++z) {
-    {  // the diagonal of plane z - 2 (used last in the step)
-      const int64_t pd = (int64_t)max(z - 2, 0) * plane;
-#pragma unroll
-      for (int k = 0; k < KR; ++k) dq[k] = d[pd + m.po[k]];
-    }
-#pragma unroll
-    for (int k = 0; k < KR; ++k) {
-      b0[k] = bn[k];
-      f0[k] = fn[k];
-    }
-    if (z < m.z1 + 1) load_step(z + 1);
-    const int q = z & 1;
-    if (z < m.zb) {  // x1 = F b on plane z
-#pragma unroll
-      for (int k = 0; k < KR; ++k) {
-        xa0[k] = cmul_fma(f0[k], b0[k]);
-        ra[q][m.lj[k] * W + (threadIdx.x & (W - 1))] = xa0[k];
-      }
-    }
-    const int p = z - 1;
-    if (p >= max(0, m.z0 - 1) && p < min(n3, m.z1 + 1)) {  // x2 on plane p
-      const cx<T>* a = ra[p & 1];
-      const bool lo = p == 0, hi = p == n3 - 1;
-      const bool store = p >= m.z0 && p < m.z1;
-#pragma unroll
-      for (int k = 0; k < KR; ++k) {
-        if (m.in1[k]) {
-          const int qq = m.lj[k] * W + (threadIdx.x & (W - 1));
-          const cx<T> xm = lo ? xa0[k] : xa2[k], xp = hi ? xa2[k] : xa0[k];
-          sa0[k] = jac7(xa1[k], a[qq - 1], a[qq + 1], a[qq - W], a[qq + W], xm, xp, b1[k], f1[k], ih1, ih2, ih3, omr);
-          rs[p & 1][qq] = sa0[k];
-          if (store && m.own[k]) xr[(int64_t)p * plane + own_off<2>(m, k, n1)] = sa0[k];
-        }
-      }
-    }
-    const int r = z - 2;
-    if (r >= m.z0 && r < m.z1) {  // r = b - A x2 on plane r, owned nodes
-      const cx<T>* s = rs[r & 1];
-      const bool lo = r == 0, hi = r == n3 - 1;
-#pragma unroll
-      for (int k = 0; k < KR; ++k) {
-        if (m.own[k]) {
-          const int qq = m.lj[k] * W + (threadIdx.x & (W - 1));
-          const cx<T> sm = lo ? sa0[k] : sa2[k], sp = hi ? sa2[k] : sa0[k];
-          rr[(int64_t)r * plane + own_off<2>(m, k, n1)] =
-              b2[k] - apply7(sa1[k], s[qq - 1], s[qq + 1], s[qq - W], s[qq + W], sm, sp, ih1, ih2, ih3, dq[k]);
-        }
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < KR; ++k) {
-      b2[k] = b1[k];
-      b1[k] = b0[k];
-      f1[k] = f0[k];
-      xa2[k] = xa1[k];
-      xa1[k] = xa0[k];
-      sa2[k] = sa1[k];
-      sa1[k] = sa0[k];
-    }
-    __syncthreads();
-  }
-}
-
-// Down leg with the full weighting of its residual (b_c = R r written, r
+// Down leg, with the full weighting of its residual (b_c = R r written, r
 // itself never stored).  Step z: x1 on plane z, x2 on plane z - 1 (stored for
 // the run's own planes), r on plane z - 2 over the tile less its two outer
 // rings (kept in LDS), and the in-plane 9-point sums of plane z - 3's r at
@@ -241,10 +142,12 @@ __global__ __launch_bounds__(NT) void k_down_fused3(TileMap tm, int ytiles, int 
 // 2K - 1, 2K, 2K + 1 in registers and stored, as in the middle step.  The
 // weights reach one fine node past the owned range, where r needs a 3-node
 // halo: the tile owns 26 x 26 fine nodes (13 x 13 coarse nodes, starting at
-// an even corner) and a run recomputes three planes on each side.  Used on
-// levels where 26-wide tiles are no more numerous than 28-wide ones.
+// an even corner) and a run recomputes three planes on each side.  (On
+// 193 x 193 x 97 that is 31% more tiles than 28 x 28 owned nodes give, and
+// still faster than storing r for a separate restriction: 0.96 against
+// 1.02 + 0.26 ms.)
 template <typename T, int NT>
-__global__ __launch_bounds__(NT) void k_down_fused3r(TileMap tm, int ytiles, int zrun, Grid gf, Grid gc,
+__global__ __launch_bounds__(NT) void k_down_fused3(TileMap tm, int ytiles, int zrun, Grid gf, Grid gc,
                                                       const cx<T>* __restrict__ b, const cx<T>* __restrict__ d,
                                                       const cx<T>* __restrict__ f, cx<T>* __restrict__ xo,
                                                       cx<T>* __restrict__ bco, const int* __restrict__ skip, T omr) {
@@ -579,25 +482,11 @@ Runs3 runs3(const Grid& g, int nrhs, int H) {
 }
 }  // namespace
 
-void launch_down_fused3(const Grid& gf, int nrhs, const cx<float>* b, const cx<float>* d, const cx<float>* f,
-                        double relax, cx<float>* xo, cx<float>* ro, const int* skip, hipStream_t s) {
-  const Runs3 r = runs3(gf, nrhs, 2);
-  hipLaunchKernelGGL((k_down_fused3<float, M3_NT>), dim3(r.m.T), dim3(M3_NT), 0, s, r.m, r.ytiles, r.zrun, gf, b, d,
-                     f, xo, ro, skip, (float)(1.0 - relax));
-}
-
-// the down leg with its restriction: levels whose 26-wide tiles (3-node
-// halo) are as many as 28-wide ones along both in-plane axes
-bool down3r_ok(const Grid& gf) {
-  return fused3_ok(gf) && (gf.n1 + m3_tx(3) - 1) / m3_tx(3) == (gf.n1 + m3_tx(2) - 1) / m3_tx(2) &&
-         (gf.n2 + m3_ty(3) - 1) / m3_ty(3) == (gf.n2 + m3_ty(2) - 1) / m3_ty(2);
-}
-
-void launch_down_fused3r(const Grid& gf, const Grid& gc, int nrhs, const cx<float>* b, const cx<float>* d,
+void launch_down_fused3(const Grid& gf, const Grid& gc, int nrhs, const cx<float>* b, const cx<float>* d,
                          const cx<float>* f, double relax, cx<float>* xo, cx<float>* bco, const int* skip,
                          hipStream_t s) {
   const Runs3 r = runs3(gf, nrhs, 3);
-  hipLaunchKernelGGL((k_down_fused3r<float, M3_NT>), dim3(r.m.T), dim3(M3_NT), 0, s, r.m, r.ytiles, r.zrun, gf, gc,
+  hipLaunchKernelGGL((k_down_fused3<float, M3_NT>), dim3(r.m.T), dim3(M3_NT), 0, s, r.m, r.ytiles, r.zrun, gf, gc,
                      b, d, f, xo, bco, skip, (float)(1.0 - relax));
 }
 

@@ -69,15 +69,10 @@ void kc_update(int64_t N, int nrhs, int mode, const cx<double>* sc, cx<T>* y, co
 bool fused_ok(const Grid& gf);
 // ---- fused 3-D legs (hh_fused3.hip), complex64, f = relax D^-1 ----
 bool fused3_ok(const Grid& gf);
-// down: x2 = two Jacobi sweeps from zero (into xo), r = b - A x2 (into ro)
-void launch_down_fused3(const Grid& gf, int nrhs, const cx<float>* b, const cx<float>* d, const cx<float>* f,
-                        double relax, cx<float>* xo, cx<float>* ro, const int* skip, hipStream_t s);
-// down with the restriction: x2 as above (into xo), b_c = R (b - A x2) (into bco);
-// levels where down3r_ok (26-wide tiles no more numerous than 28-wide ones)
-bool down3r_ok(const Grid& gf);
-void launch_down_fused3r(const Grid& gf, const Grid& gc, int nrhs, const cx<float>* b, const cx<float>* d,
-                         const cx<float>* f, double relax, cx<float>* xo, cx<float>* bco, const int* skip,
-                         hipStream_t s);
+// down: x2 = two Jacobi sweeps from zero (into xo), b_c = R (b - A x2) (into bco)
+void launch_down_fused3(const Grid& gf, const Grid& gc, int nrhs, const cx<float>* b, const cx<float>* d,
+                        const cx<float>* f, double relax, cx<float>* xo, cx<float>* bco, const int* skip,
+                        hipStream_t s);
 // middle step of a W visit: x' = x + P e_c (into xo), b_c = R (b - A x') (into bco)
 void launch_mid_fused3(const Grid& gf, const Grid& gc, int nrhs, const cx<float>* x, const cx<float>* ec,
                        const cx<float>* b, const cx<float>* d, cx<float>* xo, cx<float>* bco, const int* skip,
