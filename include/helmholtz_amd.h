@@ -69,10 +69,10 @@ typedef struct hh_mg_params {
                          one captured hipGraph, one host sync per cycle      */
   int32_t autotune;   /* 1: time stencil launch shapes per level at first solve  */
   int32_t fused;      /* 1: LDS-tiled fused smoothing/transfer legs on 2-D levels,
-                         z-marching down (pre-smoothing + residual), middle (W:
-                         correction + residual) and up legs on 3-D complex64
-                         levels (mirror ghosts, 2 pre / 2 post sweeps); 0: one
-                         kernel per step */
+                         z-marching down (pre-smoothing + restricted residual),
+                         middle (W: correction + restricted residual) and up
+                         legs on 3-D complex64 levels (mirror ghosts, 2 pre /
+                         2 post sweeps); 0: one kernel per step */
   int32_t wlevels;    /* with cycle 'W': only levels 0..wlevels-1 apply the double
                          correction, coarser levels a single one (0 = every level) */
   int32_t per_level;  /* 1: sweep counts per level from pre_l/post_l (MG.relaxPre
