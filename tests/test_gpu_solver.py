@@ -420,9 +420,10 @@ def test_jac_gmres_kcycle_solve(prec):
 @pytest.mark.parametrize("nodes", [(17, 17, 9), (33, 17, 17), (65, 49, 33), (129, 65, 17)])
 @pytest.mark.parametrize("cycle", ["V", "W"])
 def test_fused_legs3_equal_unfused(nodes, cycle):
-    """The fused 3-D legs (z-marching: two Jacobi sweeps from zero and the
-    residual in one kernel; on W levels x + P e_c and the next residual in a
-    second; x + P e_c and two Jacobi sweeps in a third) give
+    """The fused 3-D legs (z-marching: two Jacobi sweeps from zero, the
+    residual and its full weighting in one kernel; on W levels x + P e_c, the
+    next residual and its full weighting in a second; x + P e_c and two
+    Jacobi sweeps in a third) give
     the complex64 cycle of the separate kernels to complex64 rounding (the
     fused sweeps are evaluated without the diagonal); tiles that straddle the
     grid edge, several z-runs per grid and the z boundaries included.  Also
