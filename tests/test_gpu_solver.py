@@ -417,7 +417,7 @@ def test_jac_gmres_kcycle_solve(prec):
         assert abs(it - ito) <= (1 if prec == "c128" else 2)
 
 
-@pytest.mark.parametrize("nodes", [(17, 17, 9), (33, 17, 17), (65, 49, 33), (129, 65, 17)])
+@pytest.mark.parametrize("nodes", [(17, 17, 9), (33, 17, 17), (65, 49, 33), (129, 65, 17), (97, 49, 33)])
 @pytest.mark.parametrize("cycle", ["V", "W"])
 def test_fused_legs3_equal_unfused(nodes, cycle):
     """The fused 3-D legs (z-marching: two Jacobi sweeps from zero, the
